@@ -1,0 +1,268 @@
+/*
+ * slothdb_gpu.h -- C ABI of the MI355X GROUP BY hash-aggregate operator.
+ *
+ * This is the drop-in boundary for SlothDB's GROUP BY hot path: the
+ * PhysicalHashAggregate operator (src/execution/physical_planner.cpp:6009-12627)
+ * together with the fused WHERE filter that feeds it (SimplePredicate,
+ * physical_planner.cpp:110-135, 180-327, 396-496, 505-720) and the TopN pushdown
+ * that consumes it (PhysicalOperator::SetTopNHint / SetRowLimit,
+ * include/slothdb/execution/physical_operator.hpp:44-58).
+ *
+ * All signatures use plain pointers and sizes. Host-side callers (a C++
+ * PhysicalOperator subclass, a ctypes/cgo/JNI binding) own their input
+ * buffers; the library owns its device buffers and the host result buffers
+ * it hands back, which stay valid until the next gbagg_finalize/gbagg_reset
+ * or gbagg_destroy on the same handle.
+ *
+ * Threading: a handle is single-owner (one HIP stream per handle), matching
+ * the reference's single-threaded GetData contract. Different handles may be
+ * used from different threads.
+ *
+ * Errors: every call returns a gbagg_status; on failure the message is kept
+ * in thread-local storage and returned by gbagg_last_error(). This mirrors
+ * the C API's slothdb_status {OK, ERROR, INVALID} + result->error convention
+ * (include/slothdb/api/slothdb.h:24-28; src/api/slothdb_api.cpp:88-94).
+ */
+#ifndef SLOTHDB_GPU_H
+#define SLOTHDB_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GBAGG_ABI_VERSION 1
+
+/* Limits of one plan. */
+#define GBAGG_MAX_COLUMNS 16
+#define GBAGG_MAX_GROUP_COLS 2
+#define GBAGG_MAX_AGGS 16
+#define GBAGG_MAX_PREDICATES 8
+#define GBAGG_MAX_IN_SET 64
+#define GBAGG_MAX_TOPN 1024
+
+/* Status codes: same values as slothdb_status (include/slothdb/api/slothdb.h:24-28). */
+typedef enum {
+    GBAGG_OK = 0,
+    GBAGG_ERROR = 1,
+    GBAGG_INVALID = 2,
+} gbagg_status;
+
+/* Column types: same values as LogicalTypeId / slothdb_type
+ * (include/slothdb/api/slothdb.h:31-39). VARCHAR columns arrive dictionary
+ * encoded, as ParquetColumnData::str_dict_indices + str_dict_values
+ * (include/slothdb/storage/parquet.hpp:166-168). */
+typedef enum {
+    GBAGG_TYPE_INTEGER = 5,
+    GBAGG_TYPE_BIGINT = 6,
+    GBAGG_TYPE_DOUBLE = 12,
+    GBAGG_TYPE_VARCHAR = 15,
+} gbagg_type;
+
+/* One input column of one batch of rows. Mirrors the typed-array contract of
+ * ParquetColumnData (include/slothdb/storage/parquet.hpp:147-207):
+ *   data      INTEGER int32[n], BIGINT int64[n], DOUBLE double[n],
+ *             VARCHAR uint32 dictionary codes[n]
+ *   validity  one byte per row, 0 = NULL (ParquetColumnData::validity);
+ *             a null pointer means all rows are valid (all_valid = true)
+ * For gbagg_consume these are host pointers; for gbagg_consume_device,
+ * data and validity are device pointers. The dictionary (VARCHAR only) is
+ * always host memory: dict_offsets has dict_size + 1 entries into dict_bytes.
+ * Codes must be < dict_size. */
+typedef struct {
+    gbagg_type type;
+    const void *data;
+    const uint8_t *validity;
+    const uint32_t *dict_offsets;
+    const char *dict_bytes;
+    uint32_t dict_size;
+    uint32_t flags; /* GBAGG_DICT_* */
+} gbagg_column_view;
+
+/* The dictionary's entries are pairwise distinct, so equal codes mean equal
+ * strings and codes can be grouped without canonicalisation. When every batch
+ * passes the same dictionary (pointer and size) with this flag, the operator
+ * skips the per-batch host pass over the dictionary. */
+#define GBAGG_DICT_UNIQUE 1u
+
+/* Aggregate functions on the path (agg_emit_helpers.cpp:8-98; return types
+ * per src/binder/binder.cpp:1345-1392):
+ *   COUNT(*)  BIGINT, never NULL
+ *   COUNT(c)  BIGINT, non-NULL inputs
+ *   SUM(c)    BIGINT for INTEGER/BIGINT c, DOUBLE for DOUBLE c; NULL if no input
+ *   AVG(c)    DOUBLE; NULL if no input
+ *   MIN/MAX   type of c; NULL if no input
+ * Integer SUM is exact two's-complement int64 (the reference accumulates in
+ * double and casts; the two agree while partial sums stay below 2^53). */
+typedef enum {
+    GBAGG_AGG_COUNT_STAR = 0,
+    GBAGG_AGG_COUNT = 1,
+    GBAGG_AGG_SUM = 2,
+    GBAGG_AGG_AVG = 3,
+    GBAGG_AGG_MIN = 4,
+    GBAGG_AGG_MAX = 5,
+} gbagg_agg_kind;
+
+typedef struct {
+    gbagg_agg_kind kind;
+    int32_t column; /* input column index; -1 for COUNT(*) */
+} gbagg_agg;
+
+/* Compiled simple predicate, one conjunct of the WHERE clause. Same fields
+ * and meaning as SimplePredicate (physical_planner.cpp:110-135), filled the
+ * way TryCompileSimplePredicateImpl does (:180-312):
+ *   INTEGER/BIGINT column, LT..NE   compare the int64 value with ival
+ *   DOUBLE column, LT..NE           compare the double value with dval
+ *   INTEGER/BIGINT column, IN       value is in in_set[0..in_count)
+ *   VARCHAR column, EQ/NE           string (not) equal to sval[0..slen)
+ *   VARCHAR column, LIKE/NOT_LIKE   LIKE '%sval%' (case-sensitive contains)
+ * A NULL in a predicate column drops the row (EvalSimplePredicates, :396-403). */
+typedef enum {
+    GBAGG_CMP_LT = 0,
+    GBAGG_CMP_LE = 1,
+    GBAGG_CMP_GT = 2,
+    GBAGG_CMP_GE = 3,
+    GBAGG_CMP_EQ = 4,
+    GBAGG_CMP_NE = 5,
+    GBAGG_CMP_IN = 6,
+    GBAGG_CMP_LIKE = 7,
+    GBAGG_CMP_NOT_LIKE = 8,
+} gbagg_cmp_op;
+
+typedef struct {
+    int32_t column;
+    gbagg_cmp_op op;
+    int64_t ival;
+    double dval;
+    const int64_t *in_set;
+    uint32_t in_count;
+    const char *sval;
+    uint32_t slen;
+} gbagg_predicate;
+
+/* Execution strategies. AUTO lets the planner (HyperLogLog estimate, or the
+ * learned selector when one is loaded) pick. */
+typedef enum {
+    GBAGG_STRATEGY_AUTO = 0,
+    GBAGG_STRATEGY_DIRECT = 1,      /* dense key range -> direct-mapped LDS slots */
+    GBAGG_STRATEGY_LDS_HASH = 2,    /* per-workgroup LDS hash table + global merge */
+    GBAGG_STRATEGY_GLOBAL_HASH = 3, /* global hash table only */
+    GBAGG_STRATEGY_RADIX = 4,       /* radix partition + per-partition LDS tables */
+} gbagg_strategy;
+
+/* The planned aggregate: what PhysicalPlanner::PlanAggregate
+ * (physical_planner.cpp:14356-14362) hands PhysicalHashAggregate, after the
+ * planner's Filter->Scan peek-through has compiled the WHERE clause
+ * (:7138-7161) and the OrderBy/Limit above it has forwarded its hints
+ * (:1958-1975, :4026-4031).
+ * Output columns are the group columns in order, then the aggregates in order
+ * (the TopN col_idx space of SetTopNHint, physical_planner.cpp:6017-6027). */
+typedef struct {
+    uint32_t num_columns;
+    const gbagg_type *column_types;
+    /* Per input column: 0 = the caller guarantees no NULLs in any batch
+     * (validity must then be a null pointer), 1 = may contain NULLs.
+     * A null pointer here means every column may contain NULLs. */
+    const uint8_t *column_nullable;
+    uint32_t num_group_cols;
+    const uint32_t *group_cols;
+    uint32_t num_aggs;
+    const gbagg_agg *aggs;
+    uint32_t num_predicates;
+    const gbagg_predicate *predicates;
+    /* SetTopNHint(col_idx, ascending, limit): topn_col < 0 disables. */
+    int32_t topn_col;
+    int32_t topn_ascending;
+    uint64_t topn_limit;
+    /* SetRowLimit(n): bare LIMIT without ORDER BY; 0 disables. */
+    uint64_t row_limit;
+    /* Expected input rows (ReserveExpectedRows analogue); 0 = unknown. */
+    uint64_t expected_rows;
+    /* 0: rows with a NULL group key form one NULL group (SQL; the generic
+     *    path, physical_planner.cpp:6187);
+     * 1: rows with a NULL group key are dropped (the fused radix/inline
+     *    paths, physical_planner.cpp:7556-7557, 10480-10481). */
+    int32_t drop_null_keys;
+    gbagg_strategy strategy;
+} gbagg_query_desc;
+
+/* One result column, host memory owned by the handle. */
+typedef struct {
+    gbagg_type type;
+    const void *data;        /* INTEGER int32, BIGINT int64, DOUBLE double; VARCHAR: unused */
+    const uint8_t *validity; /* one byte per row, 0 = NULL; never a null pointer */
+    const uint32_t *str_offsets; /* VARCHAR: num_rows + 1 offsets into str_bytes */
+    const char *str_bytes;
+} gbagg_result_column;
+
+typedef struct {
+    uint64_t num_rows;
+    uint32_t num_columns;
+    const gbagg_result_column *columns;
+} gbagg_result_view;
+
+/* What the planner chose and what the last run measured. */
+typedef struct {
+    gbagg_strategy strategy;
+    uint64_t estimated_groups;  /* HLL or selector estimate that sized the tables */
+    uint64_t groups;            /* exact group count of the last finalize */
+    uint64_t table_capacity;    /* global table slots */
+    uint32_t lds_slots;         /* per-workgroup LDS table slots */
+    uint32_t radix_bits;        /* partition fanout = 2^radix_bits (RADIX only) */
+    uint32_t workgroups;        /* aggregate-kernel grid */
+    uint32_t retries;           /* re-runs after a table overflow */
+    uint64_t rows_consumed;
+    double ms_aggregate;        /* HIP-event time of the aggregate kernel(s), last run */
+    double ms_total;            /* HIP-event time from first consume to finalize done */
+} gbagg_plan_info;
+
+typedef struct gbagg_handle gbagg_handle;
+
+/* Library / device information. */
+int32_t gbagg_abi_version(void);
+const char *gbagg_last_error(void);
+gbagg_status gbagg_device_name(char *buf, uint32_t buflen);
+
+/* PhysicalHashAggregate lifecycle (physical_planner.cpp:6009-6060):
+ *   plan_create  ~ constructor + SetTopNHint/SetRowLimit/Init
+ *   consume      ~ the RGConsumerFn callback of the fused Parquet paths
+ *                  (physical_planner.cpp:1060-1062), one call per batch
+ *   finalize     ~ ComputeAggregates' merge + emit (result_rows_)
+ *   result       ~ GetData's chunks, as whole columns */
+gbagg_status gbagg_plan_create(const gbagg_query_desc *query, gbagg_handle **out);
+gbagg_status gbagg_consume(gbagg_handle *h, const gbagg_column_view *columns, uint64_t num_rows);
+gbagg_status gbagg_consume_device(gbagg_handle *h, const gbagg_column_view *columns,
+                                  uint64_t num_rows);
+gbagg_status gbagg_finalize(gbagg_handle *h);
+gbagg_status gbagg_result(gbagg_handle *h, gbagg_result_view *out);
+gbagg_status gbagg_reset(gbagg_handle *h);
+gbagg_status gbagg_plan_info_get(gbagg_handle *h, gbagg_plan_info *out);
+void gbagg_destroy(gbagg_handle *h);
+
+/* GPUEngine::HashAggregate (include/slothdb/execution/gpu/gpu_engine.hpp:79-85):
+ * GROUP BY int32 key, SUM and COUNT of int32 values. Host buffers in and out;
+ * out_* hold `capacity` entries; *out_groups receives the group count (an
+ * error is returned if it exceeds capacity). */
+gbagg_status gbagg_hash_aggregate_i32(const int32_t *keys, const int32_t *values, uint64_t count,
+                                      int32_t *out_keys, int64_t *out_sums, int64_t *out_counts,
+                                      uint64_t capacity, uint64_t *out_groups);
+
+/* Device memory helpers for callers without their own allocator. */
+gbagg_status gbagg_device_alloc(uint64_t bytes, void **out);
+gbagg_status gbagg_device_free(void *ptr);
+gbagg_status gbagg_memcpy_h2d(void *dst, const void *src, uint64_t bytes);
+gbagg_status gbagg_memcpy_d2h(void *dst, const void *src, uint64_t bytes);
+gbagg_status gbagg_device_synchronize(void);
+
+/* Hash of the reference's hash maps for a 64-bit key: ankerl::unordered_dense
+ * hash<uint64_t> = wyhash::mix(x, 0x9E3779B97F4A7C15)
+ * (include/third_party/unordered_dense.h:158-191, 266-267). Host-side, for
+ * callers that need the operator's partition bits. */
+uint64_t gbagg_hash_u64(uint64_t x);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SLOTHDB_GPU_H */

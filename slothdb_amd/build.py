@@ -1,0 +1,70 @@
+"""Builds the native libraries in-tree.
+
+  slothdb_amd/lib/libslothdb_gpu.so      the GROUP BY operator (HIP, gfx950) + C ABI
+  slothdb_amd/lib/libslothdb_datagen.so  seeded synthetic inputs (host C++)
+
+Run as `python -m slothdb_amd.build` or through __graft_entry__.build().
+"""
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+LIB = os.path.join(HERE, "lib")
+CSRC = os.path.join(HERE, "csrc")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+# Packed FP32 instructions are disabled for every HIP translation unit.
+HIP_FLAGS = [
+    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-command-line-argument",
+]
+
+
+def _run(cmd, cwd=None):
+    r = subprocess.run(cmd, cwd=cwd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    out = "\n".join(l for l in r.stdout.splitlines() if "packed-fp32-ops" not in l)
+    if r.returncode != 0:
+        raise RuntimeError(f"build failed ({r.returncode}): {' '.join(cmd)}\n{out}")
+    return out
+
+
+def build_gpu():
+    os.makedirs(LIB, exist_ok=True)
+    out = os.path.join(LIB, "libslothdb_gpu.so")
+    tmp = out + ".tmp"
+    _run([HIPCC, *HIP_FLAGS, "-o", tmp, os.path.join(CSRC, "gbagg.hip")])
+    os.replace(tmp, out)
+    return out
+
+
+def build_datagen():
+    os.makedirs(LIB, exist_ok=True)
+    out = os.path.join(LIB, "libslothdb_datagen.so")
+    tmp = out + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-pthread", "-o", tmp,
+          os.path.join(CSRC, "datagen.cpp")])
+    os.replace(tmp, out)
+    return out
+
+
+def build_oracle():
+    """The CPU oracle (test infrastructure) and, with the reference tree
+    present, oracle/_ref built from the reference's own aggregator sources."""
+    make = shutil.which("make") or "make"
+    return _run([make, "-s", "-B", "-C", os.path.join(ROOT, "oracle")])
+
+
+def build_all():
+    build_datagen()
+    build_oracle()
+    build_gpu()
+
+
+if __name__ == "__main__":
+    build_all()
+    print("built", file=sys.stderr)

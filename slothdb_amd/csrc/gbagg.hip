@@ -1,0 +1,1059 @@
+// gbagg.hip -- host side of the MI355X GROUP BY operator and its C ABI
+// (include/slothdb_gpu.h).
+//
+// A handle holds one planned aggregate (PhysicalHashAggregate's groups,
+// aggregates, compiled predicates and TopN hint), its HIP stream, and its
+// device buffers. Batches handed to consume/consume_device are queued; the
+// whole pipeline runs on the handle's stream in finalize:
+//
+//   k_hll (per batch) -> k_hll_finish       table sized on the device
+//   k_table_init
+//   k_aggregate (per batch)                 filter + hash + LDS table + global merge
+//   k_occ_count -> k_scan_blocks -> k_compact_write    (full result), or
+//   k_topn_local -> k_topn_final -> k_gather_write     (TopN hint, K <= 32)
+//
+// One device->host copy of the small plan block ends the run; a probe
+// overflow (estimate far too low) reruns with a 4x table.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/slothdb_gpu.h"
+#include "gbagg_kernels.hpp"
+
+using namespace gbagg;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+struct ApiError {
+    gbagg_status status;
+    std::string msg;
+};
+
+[[noreturn]] void fail(gbagg_status st, const std::string &msg) { throw ApiError{st, msg}; }
+
+#define HIPCHK(expr)                                                                       \
+    do {                                                                                   \
+        hipError_t _e = (expr);                                                            \
+        if (_e != hipSuccess)                                                              \
+            fail(GBAGG_ERROR, std::string(#expr) + ": " + hipGetErrorString(_e));          \
+    } while (0)
+
+template <typename Fn>
+gbagg_status guarded(Fn &&fn) {
+    try {
+        fn();
+        return GBAGG_OK;
+    } catch (const ApiError &e) {
+        g_last_error = e.msg;
+        return e.status;
+    } catch (const std::exception &e) {
+        g_last_error = e.what();
+        return GBAGG_ERROR;
+    }
+}
+
+uint64_t next_pow2(uint64_t x) {
+    uint64_t p = 1;
+    while (p < x) p <<= 1;
+    return p;
+}
+
+struct DeviceInfo {
+    int cus = 256;
+    size_t lds_per_cu = 160 * 1024;
+    size_t lds_per_block = 64 * 1024;
+    std::string name;
+};
+
+DeviceInfo &device_info() {
+    static DeviceInfo info;
+    static bool init = false;
+    if (!init) {
+        int dev = 0;
+        HIPCHK(hipGetDevice(&dev));
+        hipDeviceProp_t p;
+        HIPCHK(hipGetDeviceProperties(&p, dev));
+        info.cus = p.multiProcessorCount;
+        info.lds_per_cu = p.maxSharedMemoryPerMultiProcessor ? p.maxSharedMemoryPerMultiProcessor
+                                                             : 65536;
+        info.lds_per_block = p.sharedMemPerBlock;
+        info.name = p.gcnArchName;
+        init = true;
+    }
+    return info;
+}
+
+// Device buffer that only grows.
+struct DevBuf {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    DevBuf(DevBuf &&o) noexcept : ptr(o.ptr), bytes(o.bytes) {
+        o.ptr = nullptr;
+        o.bytes = 0;
+    }
+    DevBuf &operator=(DevBuf &&o) noexcept {
+        std::swap(ptr, o.ptr);
+        std::swap(bytes, o.bytes);
+        return *this;
+    }
+    void ensure(size_t n) {
+        if (n <= bytes) return;
+        if (ptr) HIPCHK(hipFree(ptr));
+        ptr = nullptr;
+        bytes = 0;
+        HIPCHK(hipMalloc(&ptr, n));
+        bytes = n;
+    }
+    template <typename T>
+    T *as() const { return static_cast<T *>(ptr); }
+    ~DevBuf() {
+        if (ptr) (void)hipFree(ptr);
+    }
+};
+
+int col_kind(gbagg_type t) {
+    switch (t) {
+    case GBAGG_TYPE_INTEGER: return CK_I32;
+    case GBAGG_TYPE_BIGINT: return CK_I64;
+    case GBAGG_TYPE_DOUBLE: return CK_F64;
+    default: return CK_U32;
+    }
+}
+
+size_t type_width(gbagg_type t) {
+    return (t == GBAGG_TYPE_INTEGER || t == GBAGG_TYPE_VARCHAR) ? 4 : 8;
+}
+
+bool valid_type(int t) {
+    return t == GBAGG_TYPE_INTEGER || t == GBAGG_TYPE_BIGINT || t == GBAGG_TYPE_DOUBLE ||
+           t == GBAGG_TYPE_VARCHAR;
+}
+
+// Host string dictionary for one VARCHAR group column: codes of every batch
+// map to one id space so groups match across batches.
+struct KeyDict {
+    int mode = 0;  // 0 unset, 1 identity (unique dictionary shared by all batches), 2 map
+    const uint32_t *offsets = nullptr;
+    const char *bytes = nullptr;
+    uint32_t size = 0;
+    std::vector<uint32_t> own_offsets;  // identity mode over a host-consumed batch
+    std::vector<char> own_bytes;
+    std::unordered_map<std::string, uint32_t> map;
+    std::vector<std::string> strings;
+
+    std::string lookup(uint32_t id) const {
+        if (mode == 1) {
+            const uint32_t *o = own_offsets.empty() ? offsets : own_offsets.data();
+            const char *b = own_bytes.empty() ? bytes : own_bytes.data();
+            return std::string(b + o[id], o[id + 1] - o[id]);
+        }
+        return strings[id];
+    }
+};
+
+struct OutCol {
+    int kind;
+    gbagg_type type;
+    int val_acc;
+    int cnt_acc;
+    int key_type;
+};
+
+struct Batch {
+    DevBatch db{};
+    uint64_t nrows = 0;
+    std::vector<std::unique_ptr<DevBuf>> owned;  // device copies / remaps / match tables
+};
+
+}  // namespace
+
+struct gbagg_handle {
+    // plan
+    std::vector<gbagg_type> col_types;
+    std::vector<uint8_t> nullable;
+    std::vector<uint32_t> group_cols;
+    std::vector<gbagg_agg> aggs;
+    std::vector<gbagg_predicate> preds;
+    std::vector<std::vector<int64_t>> in_sets;
+    std::vector<std::string> pred_str;
+    int32_t topn_col = -1, topn_asc = 0;
+    uint64_t topn_limit = 0, row_limit = 0, expected_rows = 0;
+    int32_t drop_null_keys = 0;
+    gbagg_strategy strategy = GBAGG_STRATEGY_AUTO;
+
+    int nu = 0;
+    int ucol[MAXU];
+    uint32_t umask_all = 0, umask_hll = 0;
+    DevQuery dq{};
+    std::vector<OutCol> outs;
+    KeyDict kdict[2];
+    bool key_varchar[2] = {false, false};
+    // TopN on the device
+    bool topn_device = false;
+    DevTopn tspec{};
+    uint32_t topn_k = 0;
+
+    // runtime
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[4] = {};
+    std::vector<Batch> batches;
+    std::vector<DevBuf> in_set_bufs;
+    DevBuf dplan, table, hll, blkcnt, cand_slot, cand_ord, final_slot;
+    std::vector<DevBuf> out_data, out_valid;
+    DevPlan *hplan = nullptr;  // pinned
+    uint64_t cap_max = 0;
+    uint64_t rows_total = 0;
+    bool finalized = false;
+    uint64_t result_rows = 0;
+    uint32_t lds_slots = 0, agg_grid = 0, retries = 0;
+    double ms_agg = 0, ms_total = 0;
+
+    // host result
+    std::vector<std::vector<uint8_t>> res_data, res_valid;
+    std::vector<std::vector<uint32_t>> res_str_off;
+    std::vector<std::vector<char>> res_str_bytes;
+    std::vector<gbagg_result_column> res_cols;
+    bool result_ready = false;
+
+    ~gbagg_handle() {
+        if (hplan) (void)hipHostFree(hplan);
+        for (auto &e : ev)
+            if (e) (void)hipEventDestroy(e);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+int add_u(gbagg_handle *h, uint32_t col) {
+    for (int u = 0; u < h->nu; u++)
+        if ((uint32_t)h->ucol[u] == col) return u;
+    if (h->nu >= MAXU) fail(GBAGG_INVALID, "a plan may reference at most 8 distinct input columns");
+    h->ucol[h->nu] = (int)col;
+    h->dq.ukind[h->nu] = col_kind(h->col_types[col]);
+    return h->nu++;
+}
+
+int add_acc(gbagg_handle *h, int op, int u) {
+    for (int a = 0; a < h->dq.nacc; a++)
+        if (h->dq.acc_op[a] == op && h->dq.acc_u[a] == u) return a;
+    if (h->dq.nacc >= MAX_ACC) fail(GBAGG_INVALID, "too many accumulators in one plan");
+    h->dq.acc_op[h->dq.nacc] = op;
+    h->dq.acc_u[h->dq.nacc] = u;
+    return h->dq.nacc++;
+}
+
+void build_plan(gbagg_handle *h, const gbagg_query_desc *q) {
+    if (!q) fail(GBAGG_INVALID, "null query");
+    if (q->num_columns == 0 || q->num_columns > GBAGG_MAX_COLUMNS || !q->column_types)
+        fail(GBAGG_INVALID, "num_columns must be 1..16 with column_types");
+    for (uint32_t c = 0; c < q->num_columns; c++) {
+        if (!valid_type(q->column_types[c])) fail(GBAGG_INVALID, "unsupported column type");
+        h->col_types.push_back(q->column_types[c]);
+        h->nullable.push_back(q->column_nullable ? q->column_nullable[c] : 1);
+    }
+    if (q->num_group_cols > GBAGG_MAX_GROUP_COLS) fail(GBAGG_INVALID, "at most 2 group columns");
+    if (q->num_aggs > GBAGG_MAX_AGGS) fail(GBAGG_INVALID, "at most 16 aggregates");
+    if (q->num_predicates > GBAGG_MAX_PREDICATES) fail(GBAGG_INVALID, "at most 8 predicates");
+    if (q->num_aggs == 0 && q->num_group_cols == 0) fail(GBAGG_INVALID, "nothing to aggregate");
+    h->drop_null_keys = q->drop_null_keys;
+    h->strategy = q->strategy;
+    h->topn_col = q->topn_col;
+    h->topn_asc = q->topn_ascending;
+    h->topn_limit = q->topn_limit;
+    h->row_limit = q->row_limit;
+    h->expected_rows = q->expected_rows;
+    h->dq.drop_null_keys = q->drop_null_keys;
+
+    // group keys
+    for (uint32_t g = 0; g < q->num_group_cols; g++) {
+        uint32_t c = q->group_cols[g];
+        if (c >= q->num_columns) fail(GBAGG_INVALID, "group column out of range");
+        gbagg_type t = h->col_types[c];
+        if (t == GBAGG_TYPE_DOUBLE) fail(GBAGG_INVALID, "DOUBLE group keys are not supported");
+        h->group_cols.push_back(c);
+        h->key_varchar[g] = (t == GBAGG_TYPE_VARCHAR);
+    }
+    if (q->num_group_cols == 0) {
+        h->dq.key_mode = KM_NONE;
+    } else if (q->num_group_cols == 1) {
+        h->dq.key_mode = KM_SINGLE;
+        h->dq.key_u[0] = add_u(h, h->group_cols[0]);
+        h->dq.key_signed32[0] = h->col_types[h->group_cols[0]] != GBAGG_TYPE_VARCHAR;
+    } else {
+        for (int g = 0; g < 2; g++)
+            if (h->col_types[h->group_cols[g]] == GBAGG_TYPE_BIGINT)
+                fail(GBAGG_INVALID, "two-column keys with a BIGINT component are not supported");
+        bool any_null = h->nullable[h->group_cols[0]] || h->nullable[h->group_cols[1]];
+        if (any_null && !q->drop_null_keys)
+            fail(GBAGG_INVALID, "two-column keys over nullable columns require drop_null_keys");
+        h->dq.key_mode = KM_PACK32;
+        for (int g = 0; g < 2; g++) {
+            h->dq.key_u[g] = add_u(h, h->group_cols[g]);
+            h->dq.key_signed32[g] = h->col_types[h->group_cols[g]] == GBAGG_TYPE_INTEGER;
+        }
+    }
+
+    // predicates
+    for (uint32_t i = 0; i < q->num_predicates; i++) {
+        gbagg_predicate p = q->predicates[i];
+        if (p.column < 0 || (uint32_t)p.column >= q->num_columns) fail(GBAGG_INVALID, "predicate column out of range");
+        gbagg_type t = h->col_types[p.column];
+        DevPred &dp = h->dq.pred[i];
+        dp.u = add_u(h, (uint32_t)p.column);
+        dp.op = p.op;
+        dp.ival = p.ival;
+        dp.dval = p.dval;
+        h->in_sets.emplace_back();
+        h->pred_str.emplace_back();
+        if (t == GBAGG_TYPE_VARCHAR) {
+            if (p.op != GBAGG_CMP_EQ && p.op != GBAGG_CMP_NE && p.op != GBAGG_CMP_LIKE &&
+                p.op != GBAGG_CMP_NOT_LIKE)
+                fail(GBAGG_INVALID, "VARCHAR predicates support =, <>, LIKE '%x%', NOT LIKE");
+            dp.kind = PK_DICT;
+            h->pred_str.back().assign(p.sval ? p.sval : "", p.sval ? p.slen : 0);
+        } else if (p.op == GBAGG_CMP_IN) {
+            if (t == GBAGG_TYPE_DOUBLE) fail(GBAGG_INVALID, "IN is supported on INTEGER/BIGINT");
+            if (p.in_count > GBAGG_MAX_IN_SET || (p.in_count && !p.in_set))
+                fail(GBAGG_INVALID, "IN set too large or missing");
+            dp.kind = PK_IN;
+            h->in_sets.back().assign(p.in_set, p.in_set + p.in_count);
+        } else if (p.op <= GBAGG_CMP_NE) {
+            dp.kind = (t == GBAGG_TYPE_DOUBLE) ? PK_F64 : PK_INT;
+        } else {
+            fail(GBAGG_INVALID, "LIKE is supported on VARCHAR columns only");
+        }
+        h->preds.push_back(p);
+    }
+    h->dq.npred = (int32_t)q->num_predicates;
+    h->umask_hll = (1u << h->nu) - 1;  // keys + predicate columns
+
+    // aggregates -> accumulators (acc 0 is always COUNT(*))
+    add_acc(h, ACC_CNT_STAR, -1);
+    for (uint32_t a = 0; a < q->num_aggs; a++) {
+        gbagg_agg ag = q->aggs[a];
+        OutCol oc{};
+        oc.cnt_acc = -1;
+        if (ag.kind == GBAGG_AGG_COUNT_STAR) {
+            oc.kind = OUT_COUNT;
+            oc.type = GBAGG_TYPE_BIGINT;
+            oc.val_acc = 0;
+        } else {
+            if (ag.column < 0 || (uint32_t)ag.column >= q->num_columns)
+                fail(GBAGG_INVALID, "aggregate column out of range");
+            gbagg_type t = h->col_types[ag.column];
+            int u = add_u(h, (uint32_t)ag.column);
+            int cnt = h->nullable[ag.column] ? add_acc(h, ACC_CNT_VALID, u) : 0;
+            switch (ag.kind) {
+            case GBAGG_AGG_COUNT:
+                oc.kind = OUT_COUNT;
+                oc.type = GBAGG_TYPE_BIGINT;
+                oc.val_acc = cnt;
+                break;
+            case GBAGG_AGG_SUM:
+            case GBAGG_AGG_AVG:
+                if (t == GBAGG_TYPE_VARCHAR) fail(GBAGG_INVALID, "SUM/AVG require a numeric argument");
+                oc.cnt_acc = cnt;
+                if (t == GBAGG_TYPE_DOUBLE) {
+                    oc.val_acc = add_acc(h, ACC_SUM_F, u);
+                    oc.kind = ag.kind == GBAGG_AGG_SUM ? OUT_SUM_F : OUT_AVG_F;
+                    oc.type = GBAGG_TYPE_DOUBLE;
+                } else {
+                    oc.val_acc = add_acc(h, ACC_SUM_I, u);
+                    oc.kind = ag.kind == GBAGG_AGG_SUM ? OUT_SUM_I : OUT_AVG_I;
+                    oc.type = ag.kind == GBAGG_AGG_SUM ? GBAGG_TYPE_BIGINT : GBAGG_TYPE_DOUBLE;
+                }
+                break;
+            case GBAGG_AGG_MIN:
+            case GBAGG_AGG_MAX: {
+                if (t == GBAGG_TYPE_VARCHAR) fail(GBAGG_INVALID, "MIN/MAX over VARCHAR is not supported");
+                bool mn = ag.kind == GBAGG_AGG_MIN;
+                oc.cnt_acc = cnt;
+                oc.type = t;
+                if (t == GBAGG_TYPE_DOUBLE) {
+                    oc.val_acc = add_acc(h, mn ? ACC_MIN_F : ACC_MAX_F, u);
+                    oc.kind = OUT_MM_F;
+                } else {
+                    oc.val_acc = add_acc(h, mn ? ACC_MIN_I : ACC_MAX_I, u);
+                    oc.kind = t == GBAGG_TYPE_INTEGER ? OUT_MM_I32 : OUT_MM_I64;
+                }
+                break;
+            }
+            default: fail(GBAGG_INVALID, "unknown aggregate kind");
+            }
+            // count of non-NULL inputs decides NULL results; COUNT(*) when non-nullable
+            if (oc.kind != OUT_COUNT && !h->nullable[ag.column]) oc.cnt_acc = 0;
+        }
+        h->aggs.push_back(ag);
+        h->outs.push_back(oc);
+    }
+    // key output columns go first
+    std::vector<OutCol> keys;
+    for (uint32_t g = 0; g < h->group_cols.size(); g++) {
+        OutCol oc{};
+        oc.kind = OUT_KEY;
+        oc.type = h->col_types[h->group_cols[g]];
+        oc.val_acc = (int)g;
+        oc.cnt_acc = -1;
+        oc.key_type = col_kind(oc.type);
+        keys.push_back(oc);
+    }
+    h->outs.insert(h->outs.begin(), keys.begin(), keys.end());
+    h->dq.nu = h->nu;
+    h->umask_all = (1u << h->nu) - 1;
+
+    // TopN on the device: order column must never be NULL, K <= 32, and a
+    // group-column order only for single numeric keys.
+    if (h->topn_col >= 0 && h->topn_limit > 0) {
+        if ((uint32_t)h->topn_col >= h->outs.size()) fail(GBAGG_INVALID, "topn_col out of range");
+        const OutCol &oc = h->outs[h->topn_col];
+        DevTopn &t = h->tspec;
+        t.ascending = h->topn_asc;
+        t.key_mode = h->dq.key_mode;
+        t.key_signed[0] = h->dq.key_signed32[0];
+        t.key_signed[1] = h->dq.key_signed32[1];
+        t.acc = oc.val_acc;
+        t.cnt_acc = oc.cnt_acc < 0 ? 0 : oc.cnt_acc;
+        bool ok = h->topn_limit <= 32;
+        bool never_null = oc.cnt_acc <= 0;
+        switch (oc.kind) {
+        case OUT_KEY:
+            t.kind = TK_KEY;
+            ok = ok && h->dq.key_mode == KM_SINGLE && !h->key_varchar[0] &&
+                 (h->drop_null_keys || !h->nullable[h->group_cols[0]]);
+            break;
+        case OUT_COUNT: t.kind = TK_ACC_I; break;
+        case OUT_SUM_I:
+        case OUT_MM_I32:
+        case OUT_MM_I64: t.kind = TK_ACC_I; ok = ok && never_null; break;
+        case OUT_MM_F: t.kind = TK_ACC_O; ok = ok && never_null; break;
+        case OUT_SUM_F: t.kind = TK_SUM_F; ok = ok && never_null; break;
+        case OUT_AVG_I: t.kind = TK_AVG_I; ok = ok && never_null; break;
+        default: t.kind = TK_AVG_F; ok = ok && never_null; break;
+        }
+        h->topn_device = ok;
+        h->topn_k = (uint32_t)std::min<uint64_t>(h->topn_limit, 32);
+        t.k = h->topn_k;
+    }
+}
+
+void ensure_runtime(gbagg_handle *h) {
+    if (h->stream) return;
+    device_info();
+    HIPCHK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+    for (auto &e : h->ev) HIPCHK(hipEventCreate(&e));
+    HIPCHK(hipHostMalloc((void **)&h->hplan, sizeof(DevPlan), hipHostMallocDefault));
+    std::memset(h->hplan, 0, sizeof(DevPlan));
+    h->dplan.ensure(sizeof(DevPlan));
+    h->in_set_bufs.resize(h->preds.size());
+    for (size_t i = 0; i < h->preds.size(); i++) {
+        if (h->in_sets[i].empty()) continue;
+        h->in_set_bufs[i].ensure(h->in_sets[i].size() * 8);
+        HIPCHK(hipMemcpy(h->in_set_bufs[i].ptr, h->in_sets[i].data(), h->in_sets[i].size() * 8,
+                         hipMemcpyHostToDevice));
+    }
+}
+
+bool like_contains(const char *s, uint32_t l, const std::string &nd) {
+    if (nd.empty()) return true;
+    if (l < nd.size()) return false;
+    return std::search(s, s + l, nd.begin(), nd.end()) != s + l;
+}
+
+// Registers one batch; `device` tells whether data/validity are device pointers.
+void add_batch(gbagg_handle *h, const gbagg_column_view *cols, uint64_t n, bool device) {
+    if (h->finalized) fail(GBAGG_INVALID, "consume after finalize: call gbagg_reset first");
+    if (!cols && n) fail(GBAGG_INVALID, "null columns");
+    ensure_runtime(h);
+    Batch b;
+    b.nrows = n;
+    b.db.nrows = n;
+    for (int u = 0; u < h->nu; u++) {
+        const gbagg_column_view &c = cols[h->ucol[u]];
+        if (c.type != h->col_types[h->ucol[u]]) fail(GBAGG_INVALID, "column type differs from the plan");
+        if (n && !c.data) fail(GBAGG_INVALID, "null column data");
+        if (c.validity && !h->nullable[h->ucol[u]])
+            fail(GBAGG_INVALID, "validity given for a column declared non-nullable");
+        size_t w = type_width(c.type);
+        if (device || n == 0) {
+            b.db.data[u] = c.data;
+            b.db.valid[u] = c.validity;
+        } else {
+            auto buf = std::make_unique<DevBuf>();
+            buf->ensure(n * w);
+            HIPCHK(hipMemcpyAsync(buf->ptr, c.data, n * w, hipMemcpyHostToDevice, h->stream));
+            b.db.data[u] = buf->ptr;
+            b.owned.push_back(std::move(buf));
+            if (c.validity) {
+                auto vb = std::make_unique<DevBuf>();
+                vb->ensure(n);
+                HIPCHK(hipMemcpyAsync(vb->ptr, c.validity, n, hipMemcpyHostToDevice, h->stream));
+                b.db.valid[u] = vb->as<uint8_t>();
+                b.owned.push_back(std::move(vb));
+            }
+        }
+    }
+    // VARCHAR group keys: canonical ids across batches
+    for (size_t g = 0; g < h->group_cols.size(); g++) {
+        if (!h->key_varchar[g]) continue;
+        const gbagg_column_view &c = cols[h->group_cols[g]];
+        KeyDict &kd = h->kdict[g];
+        if (c.dict_size && (!c.dict_offsets || !c.dict_bytes)) fail(GBAGG_INVALID, "missing dictionary");
+        b.db.key_dict_size[g] = c.dict_size;
+        bool unique = c.flags & GBAGG_DICT_UNIQUE;
+        if (kd.mode == 0) kd.mode = unique ? 1 : 2;
+        if (kd.mode == 1) {
+            if (!unique) fail(GBAGG_INVALID, "all batches must keep GBAGG_DICT_UNIQUE once set");
+            if (kd.offsets == nullptr && kd.own_offsets.empty()) {
+                kd.size = c.dict_size;
+                if (device) {
+                    kd.offsets = c.dict_offsets;
+                    kd.bytes = c.dict_bytes;
+                } else {
+                    kd.own_offsets.assign(c.dict_offsets, c.dict_offsets + c.dict_size + 1);
+                    kd.own_bytes.assign(c.dict_bytes, c.dict_bytes + c.dict_offsets[c.dict_size]);
+                    kd.offsets = c.dict_offsets;
+                    kd.bytes = c.dict_bytes;
+                }
+            } else if (kd.offsets != c.dict_offsets || kd.size != c.dict_size) {
+                fail(GBAGG_INVALID, "GBAGG_DICT_UNIQUE batches must share one dictionary");
+            }
+            b.db.remap[g] = nullptr;
+        } else {
+            std::vector<uint32_t> remap(c.dict_size);
+            for (uint32_t d = 0; d < c.dict_size; d++) {
+                std::string s(c.dict_bytes + c.dict_offsets[d], c.dict_offsets[d + 1] - c.dict_offsets[d]);
+                auto it = kd.map.find(s);
+                if (it == kd.map.end()) {
+                    uint32_t id = (uint32_t)kd.strings.size();
+                    kd.map.emplace(s, id);
+                    kd.strings.push_back(std::move(s));
+                    remap[d] = id;
+                } else {
+                    remap[d] = it->second;
+                }
+            }
+            auto buf = std::make_unique<DevBuf>();
+            buf->ensure(std::max<size_t>(4, remap.size() * 4));
+            if (!remap.empty())
+                HIPCHK(hipMemcpyAsync(buf->ptr, remap.data(), remap.size() * 4, hipMemcpyHostToDevice, h->stream));
+            b.db.remap[g] = buf->as<uint32_t>();
+            b.owned.push_back(std::move(buf));
+            HIPCHK(hipStreamSynchronize(h->stream));  // remap vector is on the host stack
+        }
+    }
+    // predicates: IN sets and per-dictionary-entry VARCHAR matches
+    for (size_t i = 0; i < h->preds.size(); i++) {
+        const gbagg_predicate &p = h->preds[i];
+        if (h->dq.pred[i].kind == PK_IN) {
+            b.db.pred_ptr[i] = h->in_set_bufs[i].ptr;
+            b.db.pred_count[i] = (uint32_t)h->in_sets[i].size();
+        } else if (h->dq.pred[i].kind == PK_DICT) {
+            const gbagg_column_view &c = cols[p.column];
+            if (c.dict_size && (!c.dict_offsets || !c.dict_bytes)) fail(GBAGG_INVALID, "missing dictionary");
+            std::vector<uint8_t> m(std::max<uint32_t>(1, c.dict_size));
+            const std::string &nd = h->pred_str[i];
+            for (uint32_t d = 0; d < c.dict_size; d++) {
+                const char *s = c.dict_bytes + c.dict_offsets[d];
+                uint32_t l = c.dict_offsets[d + 1] - c.dict_offsets[d];
+                bool r;
+                if (p.op == GBAGG_CMP_LIKE || p.op == GBAGG_CMP_NOT_LIKE) {
+                    r = like_contains(s, l, nd);
+                    if (p.op == GBAGG_CMP_NOT_LIKE) r = !r;
+                } else {
+                    bool eq = l == nd.size() && (l == 0 || std::memcmp(s, nd.data(), l) == 0);
+                    r = p.op == GBAGG_CMP_EQ ? eq : !eq;
+                }
+                m[d] = r;
+            }
+            auto buf = std::make_unique<DevBuf>();
+            buf->ensure(m.size());
+            HIPCHK(hipMemcpyAsync(buf->ptr, m.data(), m.size(), hipMemcpyHostToDevice, h->stream));
+            HIPCHK(hipStreamSynchronize(h->stream));
+            b.db.pred_ptr[i] = buf->ptr;
+            b.db.pred_count[i] = c.dict_size;
+            b.owned.push_back(std::move(buf));
+        }
+    }
+    if (!device) HIPCHK(hipStreamSynchronize(h->stream));  // host buffers valid only during the call
+    h->rows_total += n;
+    h->batches.push_back(std::move(b));
+}
+
+uint32_t agg_lds_slots(const gbagg_handle *h, size_t *bytes_out) {
+    const DeviceInfo &di = device_info();
+    size_t budget = std::min<size_t>(di.lds_per_cu / 2, di.lds_per_block);
+    uint32_t S = 4096;
+    size_t per = 8 * (1 + (size_t)h->dq.nacc);
+    while (S > 64 && S * per + 16 > budget) S >>= 1;
+    *bytes_out = S * per + 16;
+    return S;
+}
+
+void run_pipeline(gbagg_handle *h, bool use_hll) {
+    const DeviceInfo &di = device_info();
+    DevTable T;
+    T.rec = h->table.as<uint64_t>();
+    T.stride = 1 + h->dq.nacc;
+    T.max_probe = 4096;
+    T.plan = h->dplan.as<DevPlan>();
+    hipStream_t s = h->stream;
+
+    HIPCHK(hipMemcpyAsync(h->dplan.ptr, h->hplan, sizeof(DevPlan), hipMemcpyHostToDevice, s));
+    HIPCHK(hipEventRecord(h->ev[0], s));
+    if (use_hll) {
+        uint32_t part = 0;
+        for (auto &b : h->batches) {
+            if (b.nrows == 0) continue;
+            uint32_t grid = (uint32_t)std::min<uint64_t>((b.nrows + 255) / 256, (uint64_t)di.cus * 2);
+            hipLaunchKernelGGL(k_hll, dim3(grid), dim3(256), 0, s, h->dq, b.db, h->umask_hll,
+                               h->hll.as<uint8_t>() + (uint64_t)part * HLL_M);
+            part += grid;
+        }
+        if (part == 0) {  // no rows: an all-zero sketch estimates 0 groups
+            HIPCHK(hipMemsetAsync(h->hll.ptr, 0, HLL_M, s));
+            part = 1;
+        }
+        hipLaunchKernelGGL(k_hll_finish, dim3(1), dim3(1024), 0, s, h->hll.as<const uint8_t>(), part,
+                           T.plan, 2.0, (uint64_t)1024);
+    } else {
+        hipLaunchKernelGGL(k_plan_fixed, dim3(1), dim3(64), 0, s, T.plan);
+    }
+    {
+        uint32_t grid = (uint32_t)std::min<uint64_t>((h->cap_max + 2 + 255) / 256, (uint64_t)di.cus * 8);
+        hipLaunchKernelGGL(k_table_init, dim3(grid), dim3(256), 0, s, T, h->dq);
+    }
+    HIPCHK(hipEventRecord(h->ev[1], s));
+    const bool use_lds = h->dq.key_mode != KM_NONE && h->strategy != GBAGG_STRATEGY_GLOBAL_HASH;
+    size_t lds_bytes = 0;
+    uint32_t S = use_lds ? agg_lds_slots(h, &lds_bytes) : 0;
+    h->lds_slots = S;
+    for (auto &b : h->batches) {
+        if (b.nrows == 0) continue;
+        uint64_t tiles = (b.nrows + (uint64_t)AGG_BLOCK * AGG_ROWS - 1) / ((uint64_t)AGG_BLOCK * AGG_ROWS);
+        if (use_lds) {
+            uint32_t per_cu = (uint32_t)std::max<size_t>(1, std::min<size_t>(4, di.lds_per_cu / lds_bytes));
+            uint32_t grid = (uint32_t)std::min<uint64_t>(tiles, (uint64_t)di.cus * per_cu);
+            h->agg_grid = grid;
+            hipLaunchKernelGGL(k_aggregate, dim3(grid), dim3(AGG_BLOCK), lds_bytes, s, h->dq, b.db, T, S,
+                               S * 3 / 4, h->umask_all);
+        } else {
+            uint32_t grid = (uint32_t)std::min<uint64_t>(tiles, (uint64_t)di.cus * 8);
+            h->agg_grid = grid;
+            hipLaunchKernelGGL(k_aggregate_global, dim3(grid), dim3(AGG_BLOCK), 0, s, h->dq, b.db, T,
+                               h->umask_all);
+        }
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(h->ev[2], s));
+
+    // outputs
+    DevOut o{};
+    o.ncols = (int32_t)h->outs.size();
+    o.key_mode = h->dq.key_mode;
+    for (size_t c = 0; c < h->outs.size(); c++) {
+        o.data[c] = h->out_data[c].ptr;
+        o.valid[c] = h->out_valid[c].as<uint8_t>();
+        o.kind[c] = h->outs[c].kind;
+        o.val_acc[c] = h->outs[c].val_acc;
+        o.cnt_acc[c] = h->outs[c].cnt_acc;
+        o.key_type[c] = h->outs[c].key_type;
+    }
+    if (h->topn_device) {
+        DevTopn t = h->tspec;
+        uint32_t grid = (uint32_t)std::min<uint64_t>((h->cap_max + 2 + 4095) / 4096, (uint64_t)di.cus * 4);
+        t.cand_slot = h->cand_slot.as<uint32_t>();
+        t.cand_ord = h->cand_ord.as<uint64_t>();
+        t.final_slot = h->final_slot.as<uint32_t>();
+        if (t.k <= 16) {
+            hipLaunchKernelGGL(k_topn_local<16>, dim3(grid), dim3(256), 0, s, T, t);
+            hipLaunchKernelGGL(k_topn_final<16>, dim3(1), dim3(1024), 0, s, T, t, grid);
+        } else {
+            hipLaunchKernelGGL(k_topn_local<32>, dim3(grid), dim3(256), 0, s, T, t);
+            hipLaunchKernelGGL(k_topn_final<32>, dim3(1), dim3(1024), 0, s, T, t, grid);
+        }
+        hipLaunchKernelGGL(k_gather_write, dim3(1), dim3(64), 0, s, T, t, o);
+    } else {
+        uint64_t nblk = (h->cap_max + 2 + COMPACT_CHUNK - 1) / COMPACT_CHUNK;
+        uint32_t grid = (uint32_t)std::min<uint64_t>(nblk, (uint64_t)di.cus * 4);
+        hipLaunchKernelGGL(k_occ_count, dim3(grid), dim3(256), 0, s, T, h->blkcnt.as<uint32_t>());
+        hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, s, T, h->blkcnt.as<uint32_t>());
+        hipLaunchKernelGGL(k_compact_write, dim3(grid), dim3(256), 0, s, T, h->blkcnt.as<const uint32_t>(), o);
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(h->ev[3], s));
+    HIPCHK(hipMemcpyAsync(h->hplan, h->dplan.ptr, sizeof(DevPlan), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    float a = 0, t = 0;
+    HIPCHK(hipEventElapsedTime(&a, h->ev[1], h->ev[2]));
+    HIPCHK(hipEventElapsedTime(&t, h->ev[0], h->ev[3]));
+    h->ms_agg = a;
+    h->ms_total = t;
+}
+
+void finalize(gbagg_handle *h) {
+    if (h->finalized) fail(GBAGG_INVALID, "already finalized: call gbagg_reset first");
+    ensure_runtime(h);
+    const DeviceInfo &di = device_info();
+    uint64_t rows = std::max<uint64_t>(h->rows_total, h->expected_rows);
+    if (rows > (1ULL << 30)) fail(GBAGG_INVALID, "more than 2^30 rows in one aggregate");
+    uint64_t cap_max = next_pow2(std::max<uint64_t>(1024, rows + rows / 4 + 2));
+    if (h->dq.key_mode == KM_NONE) cap_max = 1024;
+    if (cap_max > h->cap_max) {
+        h->cap_max = cap_max;
+        h->table.ensure((cap_max + 2) * (1 + (size_t)h->dq.nacc) * 8);
+        uint64_t nblk = (cap_max + 2 + COMPACT_CHUNK - 1) / COMPACT_CHUNK;
+        h->blkcnt.ensure(nblk * 4 + 16);
+    }
+    // HLL partial sketches
+    uint64_t parts = 0;
+    for (auto &b : h->batches)
+        if (b.nrows) parts += std::min<uint64_t>((b.nrows + 255) / 256, (uint64_t)di.cus * 2);
+    h->hll.ensure(std::max<uint64_t>(1, parts) * HLL_M);
+    // outputs: full result needs cap_max + 2 rows, TopN only K rows
+    uint64_t out_rows = h->topn_device ? h->topn_k : h->cap_max + 2;
+    h->out_data.resize(h->outs.size());
+    h->out_valid.resize(h->outs.size());
+    for (size_t c = 0; c < h->outs.size(); c++) {
+        h->out_data[c].ensure(out_rows * 8);
+        h->out_valid[c].ensure(out_rows);
+    }
+    if (h->topn_device) {
+        uint32_t grid = (uint32_t)std::min<uint64_t>((h->cap_max + 2 + 4095) / 4096, (uint64_t)di.cus * 4);
+        h->cand_slot.ensure((size_t)grid * h->topn_k * 4);
+        h->cand_ord.ensure((size_t)grid * h->topn_k * 16);
+        h->final_slot.ensure((size_t)h->topn_k * 4);
+    }
+    std::memset(h->hplan, 0, sizeof(DevPlan));
+    h->hplan->cap_max = h->cap_max;
+    h->retries = 0;
+    bool use_hll = true;
+    for (;;) {
+        run_pipeline(h, use_hll);
+        if (!h->hplan->overflow) break;
+        if (h->hplan->cap >= h->cap_max) fail(GBAGG_ERROR, "global hash table overflow at maximum size");
+        uint64_t next = std::min<uint64_t>(h->cap_max, h->hplan->cap * 4);
+        std::memset(h->hplan, 0, sizeof(DevPlan));
+        h->hplan->cap_max = h->cap_max;
+        h->hplan->forced_cap = next;
+        use_hll = false;
+        h->retries++;
+    }
+    h->result_rows = h->hplan->groups;
+    h->finalized = true;
+    h->result_ready = false;
+}
+
+void download_result(gbagg_handle *h) {
+    if (!h->finalized) fail(GBAGG_INVALID, "gbagg_result before gbagg_finalize");
+    if (h->result_ready) return;
+    uint64_t n = h->result_rows;
+    const size_t nc = h->outs.size();
+    bool scalar_empty = h->dq.key_mode == KM_NONE && n == 0;
+    uint64_t rows = scalar_empty ? 1 : n;
+    h->res_data.assign(nc, {});
+    h->res_valid.assign(nc, {});
+    h->res_str_off.assign(nc, {});
+    h->res_str_bytes.assign(nc, {});
+    for (size_t c = 0; c < nc; c++) {
+        size_t w = (h->outs[c].type == GBAGG_TYPE_INTEGER || h->outs[c].type == GBAGG_TYPE_VARCHAR) ? 4 : 8;
+        h->res_data[c].assign(rows * w, 0);
+        h->res_valid[c].assign(rows, 0);
+        if (n) {
+            HIPCHK(hipMemcpyAsync(h->res_data[c].data(), h->out_data[c].ptr, n * w, hipMemcpyDeviceToHost, h->stream));
+            HIPCHK(hipMemcpyAsync(h->res_valid[c].data(), h->out_valid[c].ptr, n, hipMemcpyDeviceToHost, h->stream));
+        }
+    }
+    HIPCHK(hipStreamSynchronize(h->stream));
+    if (scalar_empty) {  // SQL: an aggregate without GROUP BY over no rows is one row
+        for (size_t c = 0; c < nc; c++) h->res_valid[c][0] = h->outs[c].kind == OUT_COUNT;
+    }
+    // VARCHAR keys: ids -> strings
+    for (size_t g = 0; g < h->group_cols.size(); g++) {
+        if (!h->key_varchar[g]) continue;
+        const uint32_t *ids = (const uint32_t *)h->res_data[g].data();
+        auto &off = h->res_str_off[g];
+        auto &bytes = h->res_str_bytes[g];
+        off.assign(rows + 1, 0);
+        for (uint64_t r = 0; r < rows; r++) {
+            off[r] = (uint32_t)bytes.size();
+            if (h->res_valid[g][r]) {
+                std::string s = h->kdict[g].lookup(ids[r]);
+                bytes.insert(bytes.end(), s.begin(), s.end());
+            }
+        }
+        off[rows] = (uint32_t)bytes.size();
+    }
+    // TopN that could not run on the device: sort the full result here.
+    if (h->topn_col >= 0 && h->topn_limit > 0 && !h->topn_device && rows > 0) {
+        std::vector<uint64_t> idx(rows);
+        for (uint64_t i = 0; i < rows; i++) idx[i] = i;
+        auto cell_cmp = [&](size_t c, uint64_t a, uint64_t b) -> int {
+            bool va = h->res_valid[c][a], vb = h->res_valid[c][b];
+            if (va != vb) return va ? -1 : 1;  // NULLs last
+            if (!va) return 0;
+            const OutCol &oc = h->outs[c];
+            if (oc.type == GBAGG_TYPE_VARCHAR) {
+                const auto &off = h->res_str_off[c];
+                const char *B = h->res_str_bytes[c].data();
+                std::string_view x(B + off[a], off[a + 1] - off[a]), y(B + off[b], off[b + 1] - off[b]);
+                return x < y ? -1 : (x > y ? 1 : 0);
+            }
+            if (oc.type == GBAGG_TYPE_DOUBLE) {
+                double x, y;
+                std::memcpy(&x, &h->res_data[c][a * 8], 8);
+                std::memcpy(&y, &h->res_data[c][b * 8], 8);
+                return x < y ? -1 : (x > y ? 1 : 0);
+            }
+            int64_t x, y;
+            if (oc.type == GBAGG_TYPE_INTEGER) {
+                int32_t x32, y32;
+                std::memcpy(&x32, &h->res_data[c][a * 4], 4);
+                std::memcpy(&y32, &h->res_data[c][b * 4], 4);
+                x = x32;
+                y = y32;
+            } else {
+                std::memcpy(&x, &h->res_data[c][a * 8], 8);
+                std::memcpy(&y, &h->res_data[c][b * 8], 8);
+            }
+            return x < y ? -1 : (x > y ? 1 : 0);
+        };
+        size_t oc = (size_t)h->topn_col;
+        bool asc = h->topn_asc;
+        auto less = [&](uint64_t a, uint64_t b) {
+            int c = cell_cmp(oc, a, b);
+            if (c) return asc ? c < 0 : c > 0;
+            for (size_t g = 0; g < h->group_cols.size(); g++) {
+                c = cell_cmp(g, a, b);
+                if (c) return c < 0;
+            }
+            return false;
+        };
+        uint64_t k = std::min<uint64_t>(rows, h->topn_limit);
+        std::partial_sort(idx.begin(), idx.begin() + k, idx.end(), less);
+        idx.resize(k);
+        auto permute = [&](std::vector<uint8_t> &v, size_t w) {
+            std::vector<uint8_t> out(k * w);
+            for (uint64_t i = 0; i < k; i++) std::memcpy(&out[i * w], &v[idx[i] * w], w);
+            v.swap(out);
+        };
+        for (size_t c = 0; c < nc; c++) {
+            size_t w = (h->outs[c].type == GBAGG_TYPE_INTEGER || h->outs[c].type == GBAGG_TYPE_VARCHAR) ? 4 : 8;
+            if (h->outs[c].type == GBAGG_TYPE_VARCHAR) {
+                std::vector<uint32_t> no(k + 1, 0);
+                std::vector<char> nb;
+                for (uint64_t i = 0; i < k; i++) {
+                    no[i] = (uint32_t)nb.size();
+                    const auto &off = h->res_str_off[c];
+                    nb.insert(nb.end(), h->res_str_bytes[c].begin() + off[idx[i]],
+                              h->res_str_bytes[c].begin() + off[idx[i] + 1]);
+                }
+                no[k] = (uint32_t)nb.size();
+                h->res_str_off[c].swap(no);
+                h->res_str_bytes[c].swap(nb);
+            }
+            permute(h->res_data[c], w);
+            permute(h->res_valid[c], 1);
+        }
+        rows = k;
+    } else if (h->row_limit > 0 && rows > h->row_limit && !(h->topn_col >= 0 && h->topn_limit > 0)) {
+        rows = h->row_limit;  // bare LIMIT: any `row_limit` groups
+    }
+    h->res_cols.assign(nc, {});
+    for (size_t c = 0; c < nc; c++) {
+        gbagg_result_column &rc = h->res_cols[c];
+        rc.type = h->outs[c].type;
+        rc.data = h->res_data[c].data();
+        rc.validity = h->res_valid[c].data();
+        if (rc.type == GBAGG_TYPE_VARCHAR) {
+            rc.str_offsets = h->res_str_off[c].data();
+            rc.str_bytes = h->res_str_bytes[c].data();
+        }
+    }
+    h->result_rows = rows;
+    h->result_ready = true;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t gbagg_abi_version(void) { return GBAGG_ABI_VERSION; }
+
+const char *gbagg_last_error(void) { return g_last_error.c_str(); }
+
+gbagg_status gbagg_device_name(char *buf, uint32_t buflen) {
+    return guarded([&] {
+        if (!buf || !buflen) fail(GBAGG_INVALID, "null buffer");
+        std::string n = device_info().name;
+        std::snprintf(buf, buflen, "%s", n.c_str());
+    });
+}
+
+gbagg_status gbagg_plan_create(const gbagg_query_desc *query, gbagg_handle **out) {
+    return guarded([&] {
+        if (!out) fail(GBAGG_INVALID, "null out");
+        *out = nullptr;
+        auto h = std::make_unique<gbagg_handle>();
+        build_plan(h.get(), query);
+        *out = h.release();
+    });
+}
+
+gbagg_status gbagg_consume(gbagg_handle *h, const gbagg_column_view *columns, uint64_t num_rows) {
+    return guarded([&] {
+        if (!h) fail(GBAGG_INVALID, "null handle");
+        add_batch(h, columns, num_rows, false);
+    });
+}
+
+gbagg_status gbagg_consume_device(gbagg_handle *h, const gbagg_column_view *columns,
+                                  uint64_t num_rows) {
+    return guarded([&] {
+        if (!h) fail(GBAGG_INVALID, "null handle");
+        add_batch(h, columns, num_rows, true);
+    });
+}
+
+gbagg_status gbagg_finalize(gbagg_handle *h) {
+    return guarded([&] {
+        if (!h) fail(GBAGG_INVALID, "null handle");
+        finalize(h);
+    });
+}
+
+gbagg_status gbagg_result(gbagg_handle *h, gbagg_result_view *out) {
+    return guarded([&] {
+        if (!h || !out) fail(GBAGG_INVALID, "null argument");
+        download_result(h);
+        out->num_rows = h->result_rows;
+        out->num_columns = (uint32_t)h->res_cols.size();
+        out->columns = h->res_cols.data();
+    });
+}
+
+gbagg_status gbagg_reset(gbagg_handle *h) {
+    return guarded([&] {
+        if (!h) fail(GBAGG_INVALID, "null handle");
+        if (h->stream) HIPCHK(hipStreamSynchronize(h->stream));
+        h->batches.clear();
+        h->rows_total = 0;
+        h->finalized = false;
+        h->result_ready = false;
+        h->result_rows = 0;
+        for (auto &kd : h->kdict) kd = KeyDict();
+    });
+}
+
+gbagg_status gbagg_plan_info_get(gbagg_handle *h, gbagg_plan_info *out) {
+    return guarded([&] {
+        if (!h || !out) fail(GBAGG_INVALID, "null argument");
+        std::memset(out, 0, sizeof(*out));
+        out->strategy = (h->dq.key_mode == KM_NONE || h->strategy == GBAGG_STRATEGY_GLOBAL_HASH)
+                            ? GBAGG_STRATEGY_GLOBAL_HASH
+                            : GBAGG_STRATEGY_LDS_HASH;
+        if (h->hplan) {
+            out->estimated_groups = h->hplan->est_groups;
+            out->table_capacity = h->hplan->cap;
+            out->groups = h->hplan->groups;
+        }
+        out->lds_slots = h->lds_slots;
+        out->workgroups = h->agg_grid;
+        out->retries = h->retries;
+        out->rows_consumed = h->rows_total;
+        out->ms_aggregate = h->ms_agg;
+        out->ms_total = h->ms_total;
+    });
+}
+
+void gbagg_destroy(gbagg_handle *h) {
+    if (!h) return;
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    delete h;
+}
+
+gbagg_status gbagg_hash_aggregate_i32(const int32_t *keys, const int32_t *values, uint64_t count,
+                                      int32_t *out_keys, int64_t *out_sums, int64_t *out_counts,
+                                      uint64_t capacity, uint64_t *out_groups) {
+    gbagg_handle *h = nullptr;
+    gbagg_status st = guarded([&] {
+        if ((!keys || !values) && count) fail(GBAGG_INVALID, "null input");
+        if (!out_groups) fail(GBAGG_INVALID, "null out_groups");
+        gbagg_type types[2] = {GBAGG_TYPE_INTEGER, GBAGG_TYPE_INTEGER};
+        uint8_t nullable[2] = {0, 0};
+        uint32_t gcol = 0;
+        gbagg_agg aggs[2] = {{GBAGG_AGG_SUM, 1}, {GBAGG_AGG_COUNT_STAR, -1}};
+        gbagg_query_desc q{};
+        q.num_columns = 2;
+        q.column_types = types;
+        q.column_nullable = nullable;
+        q.num_group_cols = 1;
+        q.group_cols = &gcol;
+        q.num_aggs = 2;
+        q.aggs = aggs;
+        q.topn_col = -1;
+        auto hp = std::make_unique<gbagg_handle>();
+        build_plan(hp.get(), &q);
+        h = hp.release();
+        gbagg_column_view cols[2] = {};
+        cols[0].type = GBAGG_TYPE_INTEGER;
+        cols[0].data = keys;
+        cols[1].type = GBAGG_TYPE_INTEGER;
+        cols[1].data = values;
+        add_batch(h, cols, count, false);
+        finalize(h);
+        download_result(h);
+        uint64_t n = h->result_rows;
+        *out_groups = n;
+        if (n > capacity) fail(GBAGG_INVALID, "output capacity too small");
+        std::memcpy(out_keys, h->res_data[0].data(), n * 4);
+        std::memcpy(out_sums, h->res_data[1].data(), n * 8);
+        std::memcpy(out_counts, h->res_data[2].data(), n * 8);
+    });
+    gbagg_destroy(h);
+    return st;
+}
+
+gbagg_status gbagg_device_alloc(uint64_t bytes, void **out) {
+    return guarded([&] {
+        if (!out) fail(GBAGG_INVALID, "null out");
+        HIPCHK(hipMalloc(out, std::max<uint64_t>(bytes, 1)));
+    });
+}
+
+gbagg_status gbagg_device_free(void *ptr) {
+    return guarded([&] { HIPCHK(hipFree(ptr)); });
+}
+
+gbagg_status gbagg_memcpy_h2d(void *dst, const void *src, uint64_t bytes) {
+    return guarded([&] { HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice)); });
+}
+
+gbagg_status gbagg_memcpy_d2h(void *dst, const void *src, uint64_t bytes) {
+    return guarded([&] { HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost)); });
+}
+
+gbagg_status gbagg_device_synchronize(void) {
+    return guarded([&] { HIPCHK(hipDeviceSynchronize()); });
+}
+
+uint64_t gbagg_hash_u64(uint64_t x) {
+    unsigned __int128 r = (unsigned __int128)x * HASH_C;
+    return (uint64_t)r ^ (uint64_t)(r >> 64);
+}
+
+}  // extern "C"

@@ -1,0 +1,292 @@
+// gbagg_device.hpp -- device-side building blocks of the GROUP BY operator:
+// the parameter blocks the host passes to kernels, the key hash, row loading,
+// predicate evaluation, group-key construction and the accumulator updates
+// shared by every aggregation strategy.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace gbagg {
+
+constexpr int MAXU = 8;          // distinct input columns one plan may reference
+constexpr int MAX_ACC = 16;      // 64-bit accumulators per group
+constexpr int MAX_PRED = 8;
+constexpr uint64_t SENTINEL = 0x8000000000000000ULL;  // empty key slot
+constexpr uint64_t HASH_C = 0x9E3779B97F4A7C15ULL;
+constexpr int HLL_P = 12;
+constexpr int HLL_M = 1 << HLL_P;
+
+// Accumulator kinds. Every accumulator is one 64-bit word per group.
+enum AccOp : int32_t {
+    ACC_CNT_STAR = 0,   // += 1 per row
+    ACC_CNT_VALID = 1,  // += 1 per non-NULL input
+    ACC_SUM_I = 2,      // int64 two's-complement sum
+    ACC_SUM_F = 3,      // IEEE double sum
+    ACC_MIN_I = 4,      // int64 min
+    ACC_MAX_I = 5,      // int64 max
+    ACC_MIN_F = 6,      // min of the order-preserving uint64 image of a double
+    ACC_MAX_F = 7,      // max of the same
+};
+
+enum ColKind : int32_t { CK_I32 = 0, CK_I64 = 1, CK_F64 = 2, CK_U32 = 3 };
+enum PredKind : int32_t { PK_INT = 0, PK_F64 = 1, PK_IN = 2, PK_DICT = 3 };
+enum KeyMode : int32_t {
+    KM_NONE = 0,    // no GROUP BY: one group, key word 0
+    KM_SINGLE = 1,  // one INTEGER / BIGINT / VARCHAR-code column, key = int64 image
+    KM_PACK32 = 2,  // two 32-bit columns: key = (u32)c0 << 32 | (u32)c1
+};
+
+struct DevPred {
+    int32_t u;     // loaded-column slot
+    int32_t kind;  // PredKind
+    int32_t op;    // gbagg_cmp_op for PK_INT / PK_F64
+    int32_t pad;
+    int64_t ival;
+    double dval;
+};
+
+// Per-query constants.
+struct DevQuery {
+    int32_t nu;
+    int32_t ukind[MAXU];
+    int32_t key_mode;
+    int32_t key_u[2];
+    int32_t key_signed32[2];  // KM_PACK32: component is a signed INTEGER (order only)
+    int32_t drop_null_keys;
+    int32_t npred;
+    DevPred pred[MAX_PRED];
+    int32_t nacc;
+    int32_t acc_op[MAX_ACC];
+    int32_t acc_u[MAX_ACC];  // -1 for COUNT(*)
+};
+
+// Per-batch device pointers.
+struct DevBatch {
+    const void *data[MAXU];
+    const uint8_t *valid[MAXU];
+    const uint32_t *remap[2];     // VARCHAR group keys: code -> canonical id (null = identity)
+    uint32_t key_dict_size[2];    // VARCHAR group keys: codes >= size drop the row
+    const void *pred_ptr[MAX_PRED];   // PK_IN: int64 set; PK_DICT: uint8 match per code
+    uint32_t pred_count[MAX_PRED];    // PK_IN: set size; PK_DICT: dictionary size
+    uint64_t nrows;
+};
+
+// Device-resident plan values computed on the GPU (no host round trip).
+struct DevPlan {
+    uint64_t est_groups;   // HyperLogLog estimate
+    uint64_t cap;          // global table slots in use (power of two)
+    uint64_t groups;       // occupied slots after the run (compaction)
+    uint32_t overflow;     // a probe sequence exceeded its bound: rerun larger
+    uint32_t topn_count;
+    uint64_t forced_cap;   // host override for a rerun (0 = from the estimate)
+    uint64_t cap_max;      // allocation limit
+};
+
+// The global table: (cap + 2) records of (1 + nacc) words: key, accumulators.
+// Record cap is the group whose key equals SENTINEL, record cap + 1 the NULL
+// key group; both are occupied iff their COUNT(*) accumulator is non-zero.
+struct DevTable {
+    uint64_t *rec;
+    uint32_t stride;
+    uint32_t max_probe;
+    DevPlan *plan;
+};
+
+__device__ __forceinline__ uint64_t hash64(uint64_t x) {
+    // wyhash mum/mix with the 64-bit golden ratio: lo ^ hi of x * C
+    // (include/third_party/unordered_dense.h:158-191, 266-267).
+    return (x * HASH_C) ^ __umul64hi(x, HASH_C);
+}
+
+__host__ __device__ __forceinline__ uint64_t acc_identity(int op) {
+    switch (op) {
+    case ACC_MIN_I: return 0x7FFFFFFFFFFFFFFFULL;
+    case ACC_MAX_I: return 0x8000000000000000ULL;
+    case ACC_MIN_F: return ~0ULL;
+    default: return 0ULL;
+    }
+}
+
+// Order-preserving image of a double: x < y  <=>  ord(x) < ord(y) for
+// non-NaN values; -0.0 maps to the image of +0.0.
+__host__ __device__ __forceinline__ uint64_t f64_ord(uint64_t bits) {
+    if (bits == 0x8000000000000000ULL) bits = 0;
+    return (bits >> 63) ? ~bits : (bits | 0x8000000000000000ULL);
+}
+__host__ __device__ __forceinline__ uint64_t f64_unord(uint64_t o) {
+    return (o >> 63) ? (o & 0x7FFFFFFFFFFFFFFFULL) : ~o;
+}
+
+struct Row {
+    uint64_t v[MAXU];
+    uint32_t vm;  // bit u set: column slot u is valid in this row
+};
+
+__device__ __forceinline__ uint64_t sel(const Row &row, int u) {
+    uint64_t x = row.v[0];
+#pragma unroll
+    for (int i = 1; i < MAXU; i++)
+        if (u == i) x = row.v[i];
+    return x;
+}
+
+__device__ __forceinline__ void load_row(const DevQuery &q, const DevBatch &b, uint64_t r,
+                                         uint32_t umask, Row &row) {
+    row.vm = 0;
+#pragma unroll
+    for (int u = 0; u < MAXU; u++) {
+        row.v[u] = 0;
+        if (u < q.nu && ((umask >> u) & 1)) {
+            uint64_t x;
+            switch (q.ukind[u]) {
+            case CK_I32: x = (uint64_t)(int64_t)((const int32_t *)b.data[u])[r]; break;
+            case CK_U32: x = (uint64_t)((const uint32_t *)b.data[u])[r]; break;
+            default: x = ((const uint64_t *)b.data[u])[r]; break;
+            }
+            row.v[u] = x;
+            uint32_t ok = b.valid[u] ? (b.valid[u][r] != 0) : 1u;
+            row.vm |= ok << u;
+        }
+    }
+}
+
+__device__ __forceinline__ bool cmp_i64(int64_t v, int64_t lit, int op) {
+    switch (op) {
+    case 0: return v < lit;
+    case 1: return v <= lit;
+    case 2: return v > lit;
+    case 3: return v >= lit;
+    case 4: return v == lit;
+    case 5: return v != lit;
+    default: return false;
+    }
+}
+
+__device__ __forceinline__ bool cmp_f64(double v, double lit, int op) {
+    switch (op) {
+    case 0: return v < lit;
+    case 1: return v <= lit;
+    case 2: return v > lit;
+    case 3: return v >= lit;
+    case 4: return v == lit;
+    case 5: return v != lit;
+    default: return false;
+    }
+}
+
+// EvalSimplePredicates semantics (physical_planner.cpp:396-496): a NULL in a
+// predicate column drops the row; integer columns compare int64-exact.
+__device__ __forceinline__ bool eval_preds(const DevQuery &q, const DevBatch &b, const Row &row) {
+    for (int p = 0; p < q.npred; p++) {
+        const DevPred &P = q.pred[p];
+        if (!((row.vm >> P.u) & 1)) return false;
+        uint64_t x = sel(row, P.u);
+        bool ok;
+        switch (P.kind) {
+        case PK_INT: ok = cmp_i64((int64_t)x, P.ival, P.op); break;
+        case PK_F64: ok = cmp_f64(__longlong_as_double((long long)x), P.dval, P.op); break;
+        case PK_IN: {
+            const int64_t *s = (const int64_t *)b.pred_ptr[p];
+            ok = false;
+            for (uint32_t i = 0; i < b.pred_count[p]; i++) ok |= (s[i] == (int64_t)x);
+            break;
+        }
+        default: {
+            uint32_t c = (uint32_t)x;
+            ok = c < b.pred_count[p] && ((const uint8_t *)b.pred_ptr[p])[c] != 0;
+            break;
+        }
+        }
+        if (!ok) return false;
+    }
+    return true;
+}
+
+enum KeyResult : int { KEY_DROP = 0, KEY_OK = 1, KEY_NULL = 2 };
+
+__device__ __forceinline__ int make_key(const DevQuery &q, const DevBatch &b, const Row &row,
+                                        uint64_t &w) {
+    w = 0;
+    if (q.key_mode == KM_NONE) return KEY_OK;
+    uint64_t c0 = sel(row, q.key_u[0]);
+    bool v0 = (row.vm >> q.key_u[0]) & 1;
+    if (!v0) return q.drop_null_keys ? KEY_DROP : KEY_NULL;
+    if (b.remap[0] || b.key_dict_size[0]) {
+        uint32_t c = (uint32_t)c0;
+        if (c >= b.key_dict_size[0]) return KEY_DROP;
+        if (b.remap[0]) c0 = b.remap[0][c];
+    }
+    if (q.key_mode == KM_SINGLE) {
+        w = c0;
+        return KEY_OK;
+    }
+    uint64_t c1 = sel(row, q.key_u[1]);
+    bool v1 = (row.vm >> q.key_u[1]) & 1;
+    if (!v1) return KEY_DROP;  // packed keys are planned only when NULLs drop
+    if (b.remap[1] || b.key_dict_size[1]) {
+        uint32_t c = (uint32_t)c1;
+        if (c >= b.key_dict_size[1]) return KEY_DROP;
+        if (b.remap[1]) c1 = b.remap[1][c];
+    }
+    w = (c0 << 32) | (c1 & 0xFFFFFFFFULL);
+    return KEY_OK;
+}
+
+// ---- accumulator updates ------------------------------------------------
+
+template <int SCOPE>
+__device__ __forceinline__ void acc_apply(uint64_t *p, int op, uint64_t x) {
+    switch (op) {
+    case ACC_CNT_STAR:
+    case ACC_CNT_VALID:
+    case ACC_SUM_I:
+        __hip_atomic_fetch_add(p, x, __ATOMIC_RELAXED, SCOPE);
+        break;
+    case ACC_SUM_F:
+        if (SCOPE == __HIP_MEMORY_SCOPE_WORKGROUP)
+            __hip_atomic_fetch_add((double *)p, __longlong_as_double((long long)x), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+        else
+            unsafeAtomicAdd((double *)p, __longlong_as_double((long long)x));
+        break;
+    case ACC_MIN_I:
+        __hip_atomic_fetch_min((int64_t *)p, (int64_t)x, __ATOMIC_RELAXED, SCOPE);
+        break;
+    case ACC_MAX_I:
+        __hip_atomic_fetch_max((int64_t *)p, (int64_t)x, __ATOMIC_RELAXED, SCOPE);
+        break;
+    case ACC_MIN_F:
+        __hip_atomic_fetch_min(p, x, __ATOMIC_RELAXED, SCOPE);
+        break;
+    default:
+        __hip_atomic_fetch_max(p, x, __ATOMIC_RELAXED, SCOPE);
+        break;
+    }
+}
+
+// Value a row contributes to accumulator a, or false if it contributes nothing.
+__device__ __forceinline__ bool acc_input(const DevQuery &q, int a, const Row &row, uint64_t &x) {
+    int op = q.acc_op[a], u = q.acc_u[a];
+    if (op == ACC_CNT_STAR) { x = 1; return true; }
+    if (!((row.vm >> u) & 1)) return false;
+    uint64_t v = sel(row, u);
+    switch (op) {
+    case ACC_CNT_VALID: x = 1; break;
+    case ACC_MIN_F:
+    case ACC_MAX_F: x = f64_ord(v); break;
+    case ACC_SUM_F:
+        // double inputs carry their bits; integer inputs are not planned here
+        x = v;
+        break;
+    default: x = v; break;
+    }
+    return true;
+}
+
+// Whether a partial accumulator value changes the merged result.
+__device__ __forceinline__ bool acc_nontrivial(int op, uint64_t x) {
+    return x != acc_identity(op);
+}
+
+}  // namespace gbagg

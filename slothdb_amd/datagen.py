@@ -1,0 +1,166 @@
+"""Seeded synthetic inputs for the GROUP BY workloads (wraps
+libslothdb_datagen.so; see slothdb_amd/csrc/datagen.cpp for the generators).
+
+Workloads, with the sizes they are benchmarked at:
+  plain      SELECT k, COUNT(*), SUM(v), MIN(v), MAX(v) GROUP BY k
+             1M rows, k int32 uniform [0, 1000), v int64 uniform [-1e6, 1e6]
+  highcard   GROUP BY k with COUNT, SUM(v1), MIN/MAX(v2), AVG(v3)
+             100M rows, k int64 Zipf(1.1) over 10M sparse ids, v1 int64
+             uniform [-1e9, 1e9], v2 int32 uniform, v3 double Normal(100, 25)
+  composite  WHERE c < 30 GROUP BY a, b COUNT(*), SUM(c) ORDER BY COUNT DESC LIMIT 10
+             50M rows, a int32 uniform [0, 65536), b VARCHAR 4-32 bytes Zipf(0.9)
+             from a 1M-string dictionary, c int32 uniform [0, 100)
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import gbagg as G
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libslothdb_datagen.so")
+_lib = None
+
+SEED = 42
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m slothdb_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        u64, i64, i32, u32, dbl, vp = C.c_uint64, C.c_int64, C.c_int, C.c_uint32, C.c_double, C.c_void_p
+        L.sdg_rand.restype = u64
+        L.sdg_rand.argtypes = [u64, u32, u64]
+        L.sdg_key_id.restype = i64
+        L.sdg_key_id.argtypes = [u64, u64]
+        L.sdg_zipf_ranks.argtypes = [u64, u32, u32, dbl, u64, u64, u64, vp, i32]
+        L.sdg_zipf_expected_distinct.restype = dbl
+        L.sdg_zipf_expected_distinct.argtypes = [dbl, u64, dbl]
+        L.sdg_gen_plain.argtypes = [u64, u64, u64, i64, i64, i64, vp, vp, i32]
+        L.sdg_gen_highcard.argtypes = [u64, u64, u64, u64, dbl, vp, vp, vp, vp, i32]
+        L.sdg_gen_dict.restype = i64
+        L.sdg_gen_dict.argtypes = [u64, u32, vp, vp]
+        L.sdg_gen_composite.argtypes = [u64, u64, u64, i64, u32, dbl, i64, vp, vp, vp, i32]
+        L.sdg_gen_query_i32.argtypes = [u64, u64, u64, dbl, i64, i64, vp, vp, i32]
+        _lib = L
+    return _lib
+
+
+def _threads():
+    return min(16, os.cpu_count() or 1)
+
+
+def plain(n=1_000_000, nkeys=1000, vlo=-1_000_000, vhi=1_000_000, seed=SEED, row0=0):
+    k = np.empty(n, np.int32)
+    v = np.empty(n, np.int64)
+    lib().sdg_gen_plain(seed, row0, n, nkeys, vlo, vhi, k.ctypes.data, v.ctypes.data, _threads())
+    return k, v
+
+
+def highcard(n=100_000_000, nkeys=10_000_000, s=1.1, seed=SEED, row0=0, columns=("k", "v1", "v2", "v3")):
+    out = {}
+    k = np.empty(n, np.int64) if "k" in columns else None
+    v1 = np.empty(n, np.int64) if "v1" in columns else None
+    v2 = np.empty(n, np.int32) if "v2" in columns else None
+    v3 = np.empty(n, np.float64) if "v3" in columns else None
+    p = lambda a: None if a is None else a.ctypes.data
+    rc = lib().sdg_gen_highcard(seed, row0, n, nkeys, s, p(k), p(v1), p(v2), p(v3), _threads())
+    if rc:
+        raise ValueError("bad highcard parameters")
+    for name, a in (("k", k), ("v1", v1), ("v2", v2), ("v3", v3)):
+        if a is not None:
+            out[name] = a
+    return out
+
+
+def dictionary(size=1_000_000, seed=SEED):
+    off = np.empty(size + 1, np.uint32)
+    data = np.empty(size * 32 + 1, np.uint8)
+    used = lib().sdg_gen_dict(seed, size, off.ctypes.data, data.ctypes.data)
+    if used < 0:
+        raise RuntimeError("dictionary generation failed")
+    return G.Dictionary(offsets=off, data=data[:max(int(used), 1)].copy(), unique=True)
+
+
+def composite(n=50_000_000, a_card=65536, dict_size=1_000_000, s=0.9, c_card=100, seed=SEED, row0=0):
+    a = np.empty(n, np.int32)
+    b = np.empty(n, np.uint32)
+    c = np.empty(n, np.int32)
+    lib().sdg_gen_composite(seed, row0, n, a_card, dict_size, s, c_card, a.ctypes.data, b.ctypes.data,
+                            c.ctypes.data, _threads())
+    return a, b, c
+
+
+def query_i32(n, distinct, s, seed, vlo=-1_000_000_000, vhi=1_000_000_000, with_values=True):
+    k = np.empty(n, np.int32)
+    v = np.empty(n, np.int64) if with_values else None
+    rc = lib().sdg_gen_query_i32(seed, n, distinct, s, vlo, vhi, k.ctypes.data,
+                                 None if v is None else v.ctypes.data, _threads())
+    if rc:
+        raise ValueError("bad query parameters")
+    return k, v
+
+
+def zipf_ranks(n, nkeys, s, seed=SEED, streams=(1, 2), row0=0):
+    out = np.empty(n, np.uint64)
+    rc = lib().sdg_zipf_ranks(seed, streams[0], streams[1], s, nkeys, row0, n, out.ctypes.data, _threads())
+    if rc:
+        raise ValueError("bad zipf parameters")
+    return out
+
+
+def zipf_expected_distinct(s, nkeys, rows):
+    return lib().sdg_zipf_expected_distinct(s, nkeys, float(rows))
+
+
+# ---- the workloads as (Query, columns) ---------------------------------
+
+def plain_query():
+    return G.Query([G.INTEGER, G.BIGINT], [0],
+                   [(G.COUNT_STAR, None), (G.SUM, 1), (G.MIN, 1), (G.MAX, 1)],
+                   nullable=[False, False])
+
+
+def highcard_query(n=0):
+    return G.Query([G.BIGINT, G.BIGINT, G.INTEGER, G.DOUBLE], [0],
+                   [(G.COUNT_STAR, None), (G.SUM, 1), (G.MIN, 2), (G.MAX, 2), (G.AVG, 3)],
+                   nullable=[False] * 4, expected_rows=n)
+
+
+def composite_query(topn=10):
+    return G.Query([G.INTEGER, G.VARCHAR, G.INTEGER], [0, 1],
+                   [(G.COUNT_STAR, None), (G.SUM, 2)],
+                   predicates=[(2, G.LT, 30)], nullable=[False] * 3,
+                   topn=(2, False, topn) if topn else None)
+
+
+def plain_workload(n=1_000_000, seed=SEED):
+    k, v = plain(n, seed=seed)
+    q = G.Query([G.INTEGER, G.BIGINT], [0],
+                [(G.COUNT_STAR, None), (G.SUM, 1), (G.MIN, 1), (G.MAX, 1)],
+                nullable=[False, False])
+    return q, [G.Column(G.INTEGER, k), G.Column(G.BIGINT, v)]
+
+
+def highcard_workload(n=100_000_000, nkeys=10_000_000, seed=SEED):
+    d = highcard(n, nkeys, seed=seed)
+    q = G.Query([G.BIGINT, G.BIGINT, G.INTEGER, G.DOUBLE], [0],
+                [(G.COUNT_STAR, None), (G.SUM, 1), (G.MIN, 2), (G.MAX, 2), (G.AVG, 3)],
+                nullable=[False] * 4, expected_rows=n)
+    cols = [G.Column(G.BIGINT, d["k"]), G.Column(G.BIGINT, d["v1"]),
+            G.Column(G.INTEGER, d["v2"]), G.Column(G.DOUBLE, d["v3"])]
+    return q, cols
+
+
+def composite_workload(n=50_000_000, dict_size=1_000_000, seed=SEED, topn=10):
+    dic = dictionary(dict_size, seed=seed)
+    a, b, c = composite(n, dict_size=dict_size, seed=seed)
+    q = G.Query([G.INTEGER, G.VARCHAR, G.INTEGER], [0, 1],
+                [(G.COUNT_STAR, None), (G.SUM, 2)],
+                predicates=[(2, G.LT, 30)], nullable=[False] * 3,
+                topn=(2, False, topn) if topn else None)
+    cols = [G.Column(G.INTEGER, a), G.Column(G.VARCHAR, b, dictionary=dic), G.Column(G.INTEGER, c)]
+    return q, cols

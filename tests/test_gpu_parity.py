@@ -1,0 +1,197 @@
+"""GPU parity: the HIP operator (through the C ABI) against the CPU oracle on
+the same seeded inputs. Integer results bit-exact; DOUBLE SUM/AVG within
+1e-12 relative of the Kahan-Neumaier oracle."""
+import numpy as np
+import pytest
+
+from slothdb_amd import datagen, gbagg as G
+from oracle import oracle as O
+from tests._util import assert_same_groups, assert_topn_tie_aware, oracle_rows
+
+pytestmark = pytest.mark.gpu
+
+
+def _exact(q, cols, dicts=None):
+    c, v, t, ov = O.groupby_exact(q, cols)
+    assert not ov
+    return oracle_rows(c, v, t, dicts), t
+
+
+def _check(q, cols, device=True, dicts=None):
+    print(f"[parity] groups={q.group_cols} aggs={q.aggs} rows={len(cols[0])}", flush=True)
+    res, info = G.run(q, cols, device=device)
+    exp, types = _exact(q, cols, dicts)
+    worst = assert_same_groups(res.rows(), exp, len(q.group_cols), types)
+    return res, info, worst
+
+
+# ---- the reference's own known-answer tests ------------------------------
+
+def _emp():
+    # test_aggregation.cpp:7-14: (dept VARCHAR, salary INTEGER)
+    depts = G.Dictionary(["Engineering", "Sales", "Marketing"])
+    code = np.array([0, 0, 1, 1, 2], np.uint32)
+    salary = np.array([100, 120, 80, 90, 95], np.int32)
+    return depts, [G.Column(G.VARCHAR, code, dictionary=depts), G.Column(G.INTEGER, salary)]
+
+
+def test_emp_scalar_aggregates():
+    _, cols = _emp()
+    q = G.Query([G.VARCHAR, G.INTEGER], [],
+                [(G.COUNT_STAR, None), (G.SUM, 1), (G.AVG, 1), (G.MIN, 1), (G.MAX, 1)],
+                nullable=[False, False])
+    res, _ = G.run(q, cols)
+    assert res.rows() == [(5, 485, 97.0, 80, 120)]  # test_aggregation.cpp:16-56
+
+
+def test_emp_group_by_dept():
+    _, cols = _emp()
+    q = G.Query([G.VARCHAR, G.INTEGER], [0],
+                [(G.COUNT_STAR, None), (G.SUM, 1), (G.AVG, 1), (G.MIN, 1), (G.MAX, 1)],
+                nullable=[False, False])
+    res, _ = G.run(q, cols)
+    got = {r[0]: r[1:] for r in res.rows()}
+    # test_aggregation.cpp:58-120
+    assert got == {"Engineering": (2, 220, 110.0, 100, 120), "Sales": (2, 170, 85.0, 80, 90),
+                   "Marketing": (1, 95, 95.0, 95, 95)}
+
+
+def test_emp_order_by_count_desc_tiebreak():
+    _, cols = _emp()
+    q = G.Query([G.VARCHAR, G.INTEGER], [0], [(G.COUNT_STAR, None)], nullable=[False, False],
+                topn=(1, False, 3))
+    res, _ = G.run(q, cols)
+    # test_aggregation.cpp:196-212: cnt DESC, dept ASC
+    assert res.rows() == [("Engineering", 2), ("Sales", 2), ("Marketing", 1)]
+
+
+def test_empty_table_count():
+    q = G.Query([G.INTEGER], [], [(G.COUNT_STAR, None), (G.SUM, 0)], nullable=[False])
+    res, _ = G.run(q, [G.Column(G.INTEGER, np.zeros(0, np.int32))])
+    assert res.rows() == [(0, None)]  # test_aggregation.cpp:122-130 + SUM of empty = NULL
+
+
+def test_parallel_sum_1_to_10000():
+    x = np.arange(1, 10001, dtype=np.int32)
+    q = G.Query([G.INTEGER], [], [(G.SUM, 0)], nullable=[False])
+    res, _ = G.run(q, [G.Column(G.INTEGER, x)])
+    assert res.rows() == [(50005000,)]  # test_parallel.cpp:105-123
+
+
+def test_parallel_50k_five_depts():
+    i = np.arange(50000)
+    depts = G.Dictionary(["Eng", "Sales", "Marketing", "HR", "Finance"])
+    q = G.Query([G.INTEGER, G.VARCHAR, G.INTEGER], [1], [(G.COUNT_STAR, None)], nullable=[False] * 3)
+    cols = [G.Column(G.INTEGER, i), G.Column(G.VARCHAR, (i % 5).astype(np.uint32), dictionary=depts),
+            G.Column(G.INTEGER, 50000 + (i % 100) * 1000)]
+    res, _ = G.run(q, cols, device=False)
+    assert sorted(res.rows()) == sorted((d, 10000) for d in ["Eng", "Sales", "Marketing", "HR", "Finance"])
+
+
+def test_gpu_engine_hash_aggregate():
+    k, s, c = G.hash_aggregate_i32([1, 2, 1, 2, 1], [10, 20, 30, 40, 50])
+    got = {int(a): (int(b), int(d)) for a, b, d in zip(k, s, c)}
+    assert got == {1: (90, 3), 2: (60, 2)}  # test_performance.cpp:153-171
+
+
+# ---- workloads at oracle-friendly sizes ----------------------------------
+
+def test_plain_1m():
+    q, cols = datagen.plain_workload(n=1_000_000)
+    res, info, _ = _check(q, cols)
+    assert res.num_rows == 1000
+
+
+def test_plain_host_batches_rowgroups():
+    q, cols = datagen.plain_workload(n=500_000)
+    op = G.GpuHashAggregate(q)
+    op.Init()
+    RG = 122_880  # ROW_GROUP_SIZE
+    for s in range(0, 500_000, RG):
+        op.consume([G.Column(c.type, c.values[s:s + RG]) for c in cols])
+    op.finalize()
+    exp, types = _exact(q, cols)
+    assert_same_groups(op.result().rows(), exp, 1, types)
+    op.close()
+
+
+@pytest.mark.parametrize("n,nkeys", [(2_000_000, 1_000_000), (4_000_000, 10_000_000)])
+def test_highcard(n, nkeys):
+    q, cols = datagen.highcard_workload(n=n, nkeys=nkeys)
+    res, info, worst = _check(q, cols)
+    print(f"highcard n={n} groups={res.num_rows} worst_rel={worst:.2e} info={info}")
+
+
+@pytest.mark.parametrize("strategy", [G.GLOBAL_HASH, G.LDS_HASH])
+def test_strategies_agree(strategy):
+    q, cols = datagen.highcard_workload(n=1_000_000, nkeys=100_000)
+    q.strategy = strategy
+    _check(q, cols)
+
+
+def test_composite_filter_topn():
+    q, cols = datagen.composite_workload(n=2_000_000, dict_size=100_000, topn=10)
+    dic = cols[1].dictionary
+    res, info = G.run(q, cols, device=True)
+    exp, types = _exact(datagen.composite_query(topn=None), cols, {1: dic})
+    got = res.rows()
+    assert len(got) == 10
+    assert_topn_tie_aware(got, exp, order_col=2, k=10, types=types)
+
+
+def test_composite_full_result():
+    q, cols = datagen.composite_workload(n=1_000_000, dict_size=50_000, topn=None)
+    _check(q, cols, dicts={1: cols[1].dictionary})
+
+
+def test_nullable_values_and_keys():
+    rng = np.random.default_rng(7)
+    n = 300_000
+    k = rng.integers(-50, 50, n).astype(np.int64)
+    kv = (rng.random(n) > 0.05).astype(np.uint8)
+    v = rng.integers(-10**12, 10**12, n).astype(np.int64)
+    vv = (rng.random(n) > 0.3).astype(np.uint8)
+    d = rng.normal(5, 3, n)
+    dv = (rng.random(n) > 0.5).astype(np.uint8)
+    cols = [G.Column(G.BIGINT, k, kv), G.Column(G.BIGINT, v, vv), G.Column(G.DOUBLE, d, dv)]
+    q = G.Query([G.BIGINT, G.BIGINT, G.DOUBLE], [0],
+                [(G.COUNT_STAR, None), (G.COUNT, 1), (G.SUM, 1), (G.AVG, 1), (G.MIN, 1), (G.MAX, 2),
+                 (G.SUM, 2), (G.AVG, 2), (G.MIN, 2)])
+    res, info, _ = _check(q, cols)
+    assert any(r[0] is None for r in res.rows())  # the NULL-key group
+    q.drop_null_keys = True
+    _check(q, cols)
+
+
+def test_predicates_int_double_in_like():
+    rng = np.random.default_rng(11)
+    n = 200_000
+    a = rng.integers(0, 1000, n).astype(np.int32)
+    b = rng.integers(-5, 5, n).astype(np.int64)
+    x = rng.normal(0, 1, n)
+    words = G.Dictionary(["apple", "banana", "cherry", "grape", "pineapple", ""])
+    s = rng.integers(0, 6, n).astype(np.uint32)
+    cols = [G.Column(G.INTEGER, a), G.Column(G.BIGINT, b), G.Column(G.DOUBLE, x),
+            G.Column(G.VARCHAR, s, dictionary=words)]
+    for preds in ([(0, G.GE, 100), (0, G.LT, 900)], [(1, G.IN, [-3, 0, 4])], [(2, G.GT, 0.25)],
+                  [(3, G.LIKE, "apple")], [(3, G.NE, "")], [(3, G.NOT_LIKE, "an"), (1, G.NE, 0)]):
+        q = G.Query([G.INTEGER, G.BIGINT, G.DOUBLE, G.VARCHAR], [1],
+                    [(G.COUNT_STAR, None), (G.SUM, 0), (G.MAX, 2)], predicates=preds,
+                    nullable=[False] * 4)
+        _check(q, cols)
+
+
+def test_sentinel_and_extreme_keys():
+    keys = np.array([-2**63, 2**63 - 1, 0, -1, -2**63, 5, 2**63 - 1] * 1000, np.int64)
+    vals = np.arange(len(keys), dtype=np.int64)
+    q = G.Query([G.BIGINT, G.BIGINT], [0], [(G.COUNT_STAR, None), (G.SUM, 1), (G.MIN, 1)],
+                nullable=[False, False])
+    _check(q, [G.Column(G.BIGINT, keys), G.Column(G.BIGINT, vals)])
+
+
+def test_topn_count_desc_matches_exact():
+    q, cols = datagen.highcard_workload(n=1_000_000, nkeys=200_000)
+    q.topn = (1, False, 10)
+    res, info = G.run(q, cols, device=True)
+    exp, types = _exact(datagen.highcard_query(), cols)
+    assert_topn_tie_aware(res.rows(), exp, order_col=1, k=10, types=types)
