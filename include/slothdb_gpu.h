@@ -202,19 +202,29 @@ typedef struct {
     const gbagg_result_column *columns;
 } gbagg_result_view;
 
-/* What the planner chose and what the last run measured. */
+/* What the planner chose and what the last run measured (HIP events on the
+ * handle's stream). */
 typedef struct {
-    gbagg_strategy strategy;
-    uint64_t estimated_groups;  /* HLL or selector estimate that sized the tables */
-    uint64_t groups;            /* exact group count of the last finalize */
-    uint64_t table_capacity;    /* global table slots */
+    gbagg_strategy strategy;    /* strategy the last finalize ran */
+    uint64_t estimated_groups;  /* HyperLogLog estimate (radix strategy) */
+    uint64_t groups;            /* result rows of the last finalize */
+    uint64_t table_capacity;    /* global (fallback) table slots */
     uint32_t lds_slots;         /* per-workgroup LDS table slots */
-    uint32_t radix_bits;        /* partition fanout = 2^radix_bits (RADIX only) */
-    uint32_t workgroups;        /* aggregate-kernel grid */
+    uint32_t radix_bits;        /* partition fanout = 2^radix_bits (radix strategy) */
+    uint32_t workgroups;        /* aggregate / scatter grid */
     uint32_t retries;           /* re-runs after a table overflow */
     uint64_t rows_consumed;
-    double ms_aggregate;        /* HIP-event time of the aggregate kernel(s), last run */
-    double ms_total;            /* HIP-event time from first consume to finalize done */
+    uint64_t heavy_keys;        /* heavy-hitter keys aggregated in LDS (radix strategy) */
+    uint64_t cold_rows;         /* rows written to partitions (radix strategy) */
+    uint64_t sample_rows;       /* sampled rows that passed the filter */
+    uint64_t sample_distinct;   /* distinct keys among them */
+    uint64_t sample_f1;         /* keys seen exactly once among them */
+    double ms_aggregate;        /* the input pass: k_aggregate or k_rp_scatter */
+    double ms_total;            /* first kernel to result ready */
+    double ms_sample;           /* heavy-hitter sample and selection */
+    double ms_count;            /* radix count pass + HyperLogLog */
+    double ms_partition;        /* per-partition aggregation + heavy fold */
+    double ms_emit;             /* result compaction / TopN */
 } gbagg_plan_info;
 
 typedef struct gbagg_handle gbagg_handle;
@@ -239,6 +249,18 @@ gbagg_status gbagg_result(gbagg_handle *h, gbagg_result_view *out);
 gbagg_status gbagg_reset(gbagg_handle *h);
 gbagg_status gbagg_plan_info_get(gbagg_handle *h, gbagg_plan_info *out);
 void gbagg_destroy(gbagg_handle *h);
+
+/* Tuning knobs of one handle (no counterpart in the reference, whose
+ * strategy constants are fixed: N_RADIX = 16, radix_count_agg.hpp:304-305).
+ * Set by the autotuner and the learned strategy selector; results never
+ * depend on them.
+ *   "strategy"      gbagg_strategy to run (AUTO decides per query)
+ *   "heavy_max"     heavy-hitter keys aggregated in LDS by the radix strategy (0-4096)
+ *   "heavy_min"     minimum sample count of a heavy key (1-64)
+ *   "part_bits"     radix fanout override, -1 = from the HyperLogLog estimate
+ *   "lds_kb"        LDS budget of a per-partition table (8-160)
+ *   "fallback_cap"  initial slots of the radix strategy's global fallback table */
+gbagg_status gbagg_set_param(gbagg_handle *h, const char *name, int64_t value);
 
 /* GPUEngine::HashAggregate (include/slothdb/execution/gpu/gpu_engine.hpp:79-85):
  * GROUP BY int32 key, SUM and COUNT of int32 values. Host buffers in and out;

@@ -27,7 +27,7 @@
 #include <vector>
 
 #include "../../include/slothdb_gpu.h"
-#include "gbagg_kernels.hpp"
+#include "gbagg_radix.hpp"
 
 using namespace gbagg;
 
@@ -212,15 +212,26 @@ struct gbagg_handle {
     hipEvent_t ev[4] = {};
     std::vector<Batch> batches;
     std::vector<DevBuf> in_set_bufs;
-    DevBuf dplan, table, hll, blkcnt, cand_slot, cand_ord, final_slot;
+    DevBuf dplan, table, hll, blkcnt, cand_rec, cand_null;
+    DevBuf skeys, scnt, hkeys, hidx, hdense, hfinal, hist, hpart, base, ptotal, pstart, recs;
     std::vector<DevBuf> out_data, out_valid;
     DevPlan *hplan = nullptr;  // pinned
     uint64_t cap_max = 0;
+    uint64_t out_capacity = 0;
+    uint32_t cand_cap = 0;
     uint64_t rows_total = 0;
     bool finalized = false;
     uint64_t result_rows = 0;
-    uint32_t lds_slots = 0, agg_grid = 0, retries = 0;
-    double ms_agg = 0, ms_total = 0;
+    // tuning (autotuner / selector may override)
+    uint32_t heavy_max = 1024;
+    uint32_t heavy_min = 4;
+    uint64_t radix_fallback_cap = 65536;
+    int32_t force_part_bits = -1;   // radix fanout override (-1 = from the estimate)
+    uint32_t k4_lds_kb = 100;       // per-partition LDS table budget
+    // last run
+    gbagg_strategy used_strategy = GBAGG_STRATEGY_AUTO;
+    uint32_t lds_slots = 0, agg_grid = 0, retries = 0, radix_bits = 0;
+    double ms_agg = 0, ms_total = 0, ms_sample = 0, ms_count = 0, ms_partition = 0, ms_emit = 0;
 
     // host result
     std::vector<std::vector<uint8_t>> res_data, res_valid;
@@ -594,77 +605,60 @@ void add_batch(gbagg_handle *h, const gbagg_column_view *cols, uint64_t n, bool 
     h->batches.push_back(std::move(b));
 }
 
-uint32_t agg_lds_slots(const gbagg_handle *h, size_t *bytes_out) {
-    const DeviceInfo &di = device_info();
-    size_t budget = std::min<size_t>(di.lds_per_cu / 2, di.lds_per_block);
-    uint32_t S = 4096;
-    size_t per = 8 * (1 + (size_t)h->dq.nacc);
-    while (S > 64 && S * per + 16 > budget) S >>= 1;
-    *bytes_out = S * per + 16;
+size_t lds_table_bytes(uint32_t S, int nacc) { return (size_t)S * 8 * (1 + nacc) + 16; }
+
+// Largest power-of-two LDS table within `budget` bytes.
+uint32_t lds_slots_for(size_t budget, int nacc, uint32_t smax) {
+    uint32_t S = smax;
+    while (S > 64 && lds_table_bytes(S, nacc) > budget) S >>= 1;
     return S;
 }
 
-void run_pipeline(gbagg_handle *h, bool use_hll) {
-    const DeviceInfo &di = device_info();
+template <typename K>
+void allow_lds(K kernel, size_t bytes) {
+    if (bytes > 64 * 1024)
+        HIPCHK(hipFuncSetAttribute((const void *)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+}
+
+struct Timer {
+    gbagg_handle *h;
+    std::vector<hipEvent_t> evs;
+    void mark() {
+        hipEvent_t e;
+        HIPCHK(hipEventCreate(&e));
+        HIPCHK(hipEventRecord(e, h->stream));
+        evs.push_back(e);
+    }
+    void truncate(size_t n) {
+        while (evs.size() > n) {
+            (void)hipEventDestroy(evs.back());
+            evs.pop_back();
+        }
+    }
+    float between(size_t a, size_t b) {
+        float ms = 0;
+        if (a < evs.size() && b < evs.size()) HIPCHK(hipEventElapsedTime(&ms, evs[a], evs[b]));
+        return ms;
+    }
+    ~Timer() {
+        for (auto e : evs) (void)hipEventDestroy(e);
+    }
+};
+
+DevTable make_table(gbagg_handle *h) {
     DevTable T;
     T.rec = h->table.as<uint64_t>();
     T.stride = 1 + h->dq.nacc;
     T.max_probe = 4096;
     T.plan = h->dplan.as<DevPlan>();
-    hipStream_t s = h->stream;
+    return T;
+}
 
-    HIPCHK(hipMemcpyAsync(h->dplan.ptr, h->hplan, sizeof(DevPlan), hipMemcpyHostToDevice, s));
-    HIPCHK(hipEventRecord(h->ev[0], s));
-    if (use_hll) {
-        uint32_t part = 0;
-        for (auto &b : h->batches) {
-            if (b.nrows == 0) continue;
-            uint32_t grid = (uint32_t)std::min<uint64_t>((b.nrows + 255) / 256, (uint64_t)di.cus * 2);
-            hipLaunchKernelGGL(k_hll, dim3(grid), dim3(256), 0, s, h->dq, b.db, h->umask_hll,
-                               h->hll.as<uint8_t>() + (uint64_t)part * HLL_M);
-            part += grid;
-        }
-        if (part == 0) {  // no rows: an all-zero sketch estimates 0 groups
-            HIPCHK(hipMemsetAsync(h->hll.ptr, 0, HLL_M, s));
-            part = 1;
-        }
-        hipLaunchKernelGGL(k_hll_finish, dim3(1), dim3(1024), 0, s, h->hll.as<const uint8_t>(), part,
-                           T.plan, 2.0, (uint64_t)1024);
-    } else {
-        hipLaunchKernelGGL(k_plan_fixed, dim3(1), dim3(64), 0, s, T.plan);
-    }
-    {
-        uint32_t grid = (uint32_t)std::min<uint64_t>((h->cap_max + 2 + 255) / 256, (uint64_t)di.cus * 8);
-        hipLaunchKernelGGL(k_table_init, dim3(grid), dim3(256), 0, s, T, h->dq);
-    }
-    HIPCHK(hipEventRecord(h->ev[1], s));
-    const bool use_lds = h->dq.key_mode != KM_NONE && h->strategy != GBAGG_STRATEGY_GLOBAL_HASH;
-    size_t lds_bytes = 0;
-    uint32_t S = use_lds ? agg_lds_slots(h, &lds_bytes) : 0;
-    h->lds_slots = S;
-    for (auto &b : h->batches) {
-        if (b.nrows == 0) continue;
-        uint64_t tiles = (b.nrows + (uint64_t)AGG_BLOCK * AGG_ROWS - 1) / ((uint64_t)AGG_BLOCK * AGG_ROWS);
-        if (use_lds) {
-            uint32_t per_cu = (uint32_t)std::max<size_t>(1, std::min<size_t>(4, di.lds_per_cu / lds_bytes));
-            uint32_t grid = (uint32_t)std::min<uint64_t>(tiles, (uint64_t)di.cus * per_cu);
-            h->agg_grid = grid;
-            hipLaunchKernelGGL(k_aggregate, dim3(grid), dim3(AGG_BLOCK), lds_bytes, s, h->dq, b.db, T, S,
-                               S * 3 / 4, h->umask_all);
-        } else {
-            uint32_t grid = (uint32_t)std::min<uint64_t>(tiles, (uint64_t)di.cus * 8);
-            h->agg_grid = grid;
-            hipLaunchKernelGGL(k_aggregate_global, dim3(grid), dim3(AGG_BLOCK), 0, s, h->dq, b.db, T,
-                               h->umask_all);
-        }
-    }
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipEventRecord(h->ev[2], s));
-
-    // outputs
+DevOut make_out(gbagg_handle *h) {
     DevOut o{};
     o.ncols = (int32_t)h->outs.size();
     o.key_mode = h->dq.key_mode;
+    o.capacity = h->out_capacity;
     for (size_t c = 0; c < h->outs.size(); c++) {
         o.data[c] = h->out_data[c].ptr;
         o.valid[c] = h->out_valid[c].as<uint8_t>();
@@ -673,87 +667,374 @@ void run_pipeline(gbagg_handle *h, bool use_hll) {
         o.cnt_acc[c] = h->outs[c].cnt_acc;
         o.key_type[c] = h->outs[c].key_type;
     }
+    return o;
+}
+
+DevTopn make_topn(gbagg_handle *h) {
+    DevTopn t = h->tspec;
+    t.enabled = h->topn_device ? 1 : 0;
+    t.rec_words = 1 + h->dq.nacc;
+    t.cand_rec = h->cand_rec.as<uint64_t>();
+    t.cand_null = h->cand_null.as<uint8_t>();
+    t.cand_cap = h->cand_cap;
+    return t;
+}
+
+// Resets the run counters of the device plan block; the sample / heavy-hitter
+// statistics of this run (already read back into hplan) are kept.
+void upload_plan(gbagg_handle *h, uint64_t cap) {
+    DevPlan keep = *h->hplan;
+    std::memset(h->hplan, 0, sizeof(DevPlan));
+    h->hplan->sample_rows = keep.sample_rows;
+    h->hplan->sample_distinct = keep.sample_distinct;
+    h->hplan->sample_f1 = keep.sample_f1;
+    h->hplan->heavy_count = keep.heavy_count;
+    h->hplan->heavy_sample_rows = keep.heavy_sample_rows;
+    h->hplan->heavy_threshold = keep.heavy_threshold;
+    h->hplan->cap = cap;
+    h->hplan->cap_max = h->cap_max;
+    HIPCHK(hipMemcpyAsync(h->dplan.ptr, h->hplan, sizeof(DevPlan), hipMemcpyHostToDevice, h->stream));
+}
+
+void read_plan(gbagg_handle *h) {
+    HIPCHK(hipMemcpyAsync(h->hplan, h->dplan.ptr, sizeof(DevPlan), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+}
+
+// Global-table groups -> result (append) or TopN candidates.
+void emit_table(gbagg_handle *h, const DevTable &T, uint64_t cap) {
+    const DeviceInfo &di = device_info();
+    hipStream_t s = h->stream;
     if (h->topn_device) {
-        DevTopn t = h->tspec;
-        uint32_t grid = (uint32_t)std::min<uint64_t>((h->cap_max + 2 + 4095) / 4096, (uint64_t)di.cus * 4);
-        t.cand_slot = h->cand_slot.as<uint32_t>();
-        t.cand_ord = h->cand_ord.as<uint64_t>();
-        t.final_slot = h->final_slot.as<uint32_t>();
-        if (t.k <= 16) {
-            hipLaunchKernelGGL(k_topn_local<16>, dim3(grid), dim3(256), 0, s, T, t);
-            hipLaunchKernelGGL(k_topn_final<16>, dim3(1), dim3(1024), 0, s, T, t, grid);
-        } else {
-            hipLaunchKernelGGL(k_topn_local<32>, dim3(grid), dim3(256), 0, s, T, t);
-            hipLaunchKernelGGL(k_topn_final<32>, dim3(1), dim3(1024), 0, s, T, t, grid);
-        }
-        hipLaunchKernelGGL(k_gather_write, dim3(1), dim3(64), 0, s, T, t, o);
+        DevTopn t = make_topn(h);
+        uint32_t grid = (uint32_t)std::min<uint64_t>((cap + 2 + 4095) / 4096, (uint64_t)di.cus * 4);
+        if (t.k <= 16) hipLaunchKernelGGL(k_topn_table<16>, dim3(grid), dim3(256), 0, s, T, t);
+        else hipLaunchKernelGGL(k_topn_table<32>, dim3(grid), dim3(256), 0, s, T, t);
     } else {
-        uint64_t nblk = (h->cap_max + 2 + COMPACT_CHUNK - 1) / COMPACT_CHUNK;
+        uint64_t nblk = (cap + 2 + COMPACT_CHUNK - 1) / COMPACT_CHUNK;
         uint32_t grid = (uint32_t)std::min<uint64_t>(nblk, (uint64_t)di.cus * 4);
+        uint64_t *bases = (uint64_t *)(h->blkcnt.as<uint8_t>() + ((nblk * 4 + 15) / 16) * 16);
         hipLaunchKernelGGL(k_occ_count, dim3(grid), dim3(256), 0, s, T, h->blkcnt.as<uint32_t>());
-        hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, s, T, h->blkcnt.as<uint32_t>());
-        hipLaunchKernelGGL(k_compact_write, dim3(grid), dim3(256), 0, s, T, h->blkcnt.as<const uint32_t>(), o);
+        hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, s, T, h->blkcnt.as<uint32_t>(), bases);
+        hipLaunchKernelGGL(k_compact_write, dim3(grid), dim3(256), 0, s, T, (const uint64_t *)bases,
+                           make_out(h));
     }
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipEventRecord(h->ev[3], s));
-    HIPCHK(hipMemcpyAsync(h->hplan, h->dplan.ptr, sizeof(DevPlan), hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
-    float a = 0, t = 0;
-    HIPCHK(hipEventElapsedTime(&a, h->ev[1], h->ev[2]));
-    HIPCHK(hipEventElapsedTime(&t, h->ev[0], h->ev[3]));
-    h->ms_agg = a;
-    h->ms_total = t;
+}
+
+void topn_final(gbagg_handle *h) {
+    if (!h->topn_device) return;
+    DevTopn t = make_topn(h);
+    if (t.k <= 16) hipLaunchKernelGGL(k_topn_final<16>, dim3(1), dim3(1024), 0, h->stream, t, h->dplan.as<DevPlan>(), make_out(h));
+    else hipLaunchKernelGGL(k_topn_final<32>, dim3(1), dim3(1024), 0, h->stream, t, h->dplan.as<DevPlan>(), make_out(h));
+}
+
+void ensure_table(gbagg_handle *h, uint64_t cap) {
+    h->table.ensure((cap + 2) * (1 + (size_t)h->dq.nacc) * 8);
+    uint64_t nblk = (cap + 2 + COMPACT_CHUNK - 1) / COMPACT_CHUNK;
+    h->blkcnt.ensure(((nblk * 4 + 15) / 16) * 16 + nblk * 8 + 64);
+}
+
+// Heavy-hitter sample over all batches (strided), then the heavy table.
+void run_sample(gbagg_handle *h, RadixParams &rp) {
+    const DeviceInfo &di = device_info();
+    hipStream_t s = h->stream;
+    const uint32_t S_TOTAL = 131072;
+    rp.scap = 1u << 18;
+    rp.hmax = h->heavy_max;
+    rp.hslots = (uint32_t)next_pow2(2ull * std::max<uint32_t>(rp.hmax, 1));
+    rp.heavy_min = h->heavy_min;
+    // the heavy table must fit the scatter kernel's LDS next to its cursors
+    // (8192 partitions at most) and the count kernel's LDS
+    const size_t lds_max = device_info().lds_per_block;
+    auto fits = [&] {
+        size_t k3 = (size_t)rp.hslots * 12 + (size_t)h->dq.nacc * rp.hmax * 8 + 8192 * 4;
+        size_t k1 = (size_t)rp.hslots * 8 + 8192 * 4 + HLL_M * 4;
+        return k3 <= lds_max && k1 <= lds_max;
+    };
+    while (rp.hmax > 0 && !fits()) {
+        rp.hmax /= 2;
+        rp.hslots = (uint32_t)next_pow2(2ull * std::max<uint32_t>(rp.hmax, 1));
+    }
+    h->skeys.ensure((size_t)rp.scap * 8);
+    h->scnt.ensure((size_t)rp.scap * 4);
+    h->hkeys.ensure((size_t)rp.hslots * 8);
+    h->hidx.ensure((size_t)rp.hslots * 4);
+    h->hdense.ensure((size_t)std::max<uint32_t>(rp.hmax, 1) * 8);
+    h->hfinal.ensure((size_t)std::max<uint32_t>(rp.hmax, 1) * (1 + h->dq.nacc) * 8);
+    rp.skeys = h->skeys.as<uint64_t>();
+    rp.scnt = h->scnt.as<uint32_t>();
+    rp.hkeys = h->hkeys.as<uint64_t>();
+    rp.hidx = h->hidx.as<uint32_t>();
+    rp.hdense = h->hdense.as<uint64_t>();
+    rp.hfinal = h->hfinal.as<uint64_t>();
+    hipLaunchKernelGGL(k_sample_init, dim3(256), dim3(256), 0, s, rp);
+    for (auto &b : h->batches) {
+        if (!b.nrows) continue;
+        uint64_t want = std::max<uint64_t>(1, (uint64_t)((double)S_TOTAL * b.nrows / std::max<uint64_t>(1, h->rows_total)));
+        want = std::min<uint64_t>(want, b.nrows);
+        rp.sample_n = (uint32_t)want;
+        rp.sample_stride = std::max<uint64_t>(1, b.nrows / want);
+        uint32_t grid = (uint32_t)std::min<uint64_t>((want + 255) / 256, (uint64_t)di.cus * 2);
+        hipLaunchKernelGGL(k_sample, dim3(grid), dim3(256), 0, s, h->dq, b.db, rp, h->umask_hll,
+                           h->dplan.as<DevPlan>());
+    }
+    hipLaunchKernelGGL(k_heavy_select, dim3(1), dim3(1024), 0, s, rp, h->dplan.as<DevPlan>());
+}
+
+void run_lds_hash(gbagg_handle *h, Timer &tm, bool global_only) {
+    const DeviceInfo &di = device_info();
+    hipStream_t s = h->stream;
+    // table size from the sample: distinct keys seen, plus a Good-Turing style
+    // allowance for unseen keys (f1 / sample rows of the remaining rows)
+    const DevPlan &hp = *h->hplan;
+    double seen = (double)hp.sample_distinct;
+    double rest = hp.sample_rows ? (double)hp.sample_f1 / hp.sample_rows * (double)h->rows_total : 0;
+    uint64_t want = (uint64_t)(2.0 * (seen + rest)) + 1024;
+    if (h->dq.key_mode == KM_NONE) want = 1024;
+    uint64_t cap = std::min<uint64_t>(h->cap_max, next_pow2(want));
+    for (;;) {
+        ensure_table(h, cap);
+        DevTable T = make_table(h);
+        upload_plan(h, cap);
+        uint32_t grid = (uint32_t)std::min<uint64_t>((cap + 2 + 255) / 256, (uint64_t)di.cus * 8);
+        hipLaunchKernelGGL(k_table_init, dim3(grid), dim3(256), 0, s, T, h->dq);
+        tm.mark();  // aggregate start
+        size_t lds_bytes = 0;
+        uint32_t S = 0;
+        if (!global_only) {
+            size_t budget = std::min<size_t>(di.lds_per_cu / 2, di.lds_per_block);
+            S = lds_slots_for(budget, h->dq.nacc, 4096);
+            lds_bytes = lds_table_bytes(S, h->dq.nacc);
+            allow_lds(k_aggregate, lds_bytes);
+        }
+        h->lds_slots = S;
+        for (auto &b : h->batches) {
+            if (!b.nrows) continue;
+            uint64_t tiles = (b.nrows + (uint64_t)AGG_BLOCK * AGG_ROWS - 1) / ((uint64_t)AGG_BLOCK * AGG_ROWS);
+            if (!global_only) {
+                // few workgroups for small inputs: each flushes its LDS table
+                uint32_t per_cu = (uint32_t)std::max<size_t>(1, std::min<size_t>(4, di.lds_per_cu / lds_bytes));
+                uint64_t by_rows = std::max<uint64_t>(1, b.nrows / 16384);
+                uint32_t g = (uint32_t)std::min<uint64_t>({tiles, (uint64_t)di.cus * per_cu, by_rows});
+                h->agg_grid = g;
+                hipLaunchKernelGGL(k_aggregate, dim3(g), dim3(AGG_BLOCK), lds_bytes, s, h->dq, b.db, T, S,
+                                   S * 3 / 4, h->umask_all);
+            } else {
+                uint32_t g = (uint32_t)std::min<uint64_t>(tiles, (uint64_t)di.cus * 8);
+                h->agg_grid = g;
+                hipLaunchKernelGGL(k_aggregate_global, dim3(g), dim3(AGG_BLOCK), 0, s, h->dq, b.db, T,
+                                   h->umask_all);
+            }
+        }
+        HIPCHK(hipGetLastError());
+        tm.mark();  // aggregate end
+        emit_table(h, T, cap);
+        topn_final(h);
+        tm.mark();  // emit end
+        read_plan(h);
+        if (!(h->hplan->overflow & 1)) break;
+        if (cap >= h->cap_max) fail(GBAGG_ERROR, "global hash table overflow at maximum size");
+        cap = std::min<uint64_t>(h->cap_max, cap * 4);
+        h->retries++;
+        tm.truncate(2);
+    }
+}
+
+void run_radix(gbagg_handle *h, Timer &tm, RadixParams rp) {
+    const DeviceInfo &di = device_info();
+    hipStream_t s = h->stream;
+    const int nacc = h->dq.nacc;
+    rp.bucket_bits = 13;
+    const uint32_t nb = 1u << rp.bucket_bits;
+    // record layout: key, then each loaded slot an accumulator reads, then validity
+    uint32_t words = 1;
+    bool nullable_inputs = false;
+    std::memset(rp.rec_pos, 0, sizeof(rp.rec_pos));
+    for (int a = 0; a < nacc; a++) {
+        int u = h->dq.acc_u[a];
+        if (u < 0 || rp.rec_pos[u]) continue;
+        rp.rec_pos[u] = (int32_t)words++;
+        if (h->nullable[h->ucol[u]]) nullable_inputs = true;
+    }
+    rp.vmask_word = nullable_inputs ? words++ : 0;
+    rp.rec_words = words;
+    // workgroup slots for count + scatter
+    const size_t k3_fixed = (size_t)rp.hslots * 12 + (size_t)nacc * rp.hmax * 8;
+    const size_t k1_lds = (size_t)rp.hslots * 8 + (size_t)nb * 4 + HLL_M * 4;
+    const size_t k3_lds_max = k3_fixed + (size_t)nb * 4;
+    const uint32_t per_cu = (uint32_t)std::max<size_t>(1, std::min<size_t>(4, di.lds_per_cu / std::max(k1_lds, k3_lds_max)));
+    std::vector<uint32_t> wg_off, wg_n;
+    std::vector<uint64_t> rows_per;
+    uint32_t W = 0;
+    for (auto &b : h->batches) {
+        const uint64_t tile = (uint64_t)RP_BLOCK * AGG_ROWS;
+        uint64_t tiles = std::max<uint64_t>(1, (b.nrows + tile - 1) / tile);
+        uint32_t g = (uint32_t)std::min<uint64_t>(tiles, (uint64_t)di.cus * per_cu);
+        uint64_t per = ((b.nrows + g - 1) / g + tile - 1) / tile * tile;
+        g = (uint32_t)std::max<uint64_t>(1, (b.nrows + per - 1) / std::max<uint64_t>(per, 1));
+        wg_off.push_back(W);
+        wg_n.push_back(b.nrows ? g : 0);
+        rows_per.push_back(per);
+        W += b.nrows ? g : 0;
+    }
+    W = std::max<uint32_t>(W, 1);
+    rp.W = W;
+    h->hist.ensure((size_t)W * nb * 4);
+    h->hll.ensure((size_t)W * HLL_M);
+    h->hpart.ensure((size_t)W * nacc * std::max<uint32_t>(rp.hmax, 1) * 8);
+    rp.hist = h->hist.as<uint32_t>();
+    rp.hll = h->hll.as<uint8_t>();
+    rp.hpart = h->hpart.as<uint64_t>();
+    HIPCHK(hipMemsetAsync(rp.hll, 0, (size_t)W * HLL_M, s));
+
+    // fallback global table (small; NULL / SENTINEL keys and LDS overflow)
+    uint64_t cap = h->radix_fallback_cap;
+    for (;;) {
+        ensure_table(h, cap);
+        DevTable T = make_table(h);
+        upload_plan(h, cap);
+        uint32_t grid = (uint32_t)std::min<uint64_t>((cap + 2 + 255) / 256, (uint64_t)di.cus * 8);
+        hipLaunchKernelGGL(k_table_init, dim3(grid), dim3(256), 0, s, T, h->dq);
+        tm.mark();  // count start
+        allow_lds(k_rp_count, k1_lds);
+        for (size_t i = 0; i < h->batches.size(); i++) {
+            auto &b = h->batches[i];
+            if (!b.nrows) continue;
+            RadixParams r = rp;
+            r.wg_offset = wg_off[i];
+            r.rows_per_wg = rows_per[i];
+            hipLaunchKernelGGL(k_rp_count, dim3(wg_n[i]), dim3(RP_BLOCK), k1_lds, s, h->dq, b.db, r, h->umask_hll);
+        }
+        hipLaunchKernelGGL(k_hll_finish, dim3(1), dim3(1024), 0, s, (const uint8_t *)rp.hll, W, T.plan);
+        tm.mark();  // count end
+        read_plan(h);  // HLL estimate and heavy count pick the fanout
+        const uint64_t est = h->hplan->est_groups;
+        const uint64_t heavy = h->hplan->heavy_count;
+        const uint64_t cold = est > heavy ? est - heavy : 1;
+        const size_t k4_budget = std::min<size_t>(di.lds_per_block, (size_t)h->k4_lds_kb * 1024);
+        const uint32_t S = lds_slots_for(k4_budget, nacc, 8192);
+        uint32_t pbits = 0;
+        while (pbits < rp.bucket_bits && (double)cold / (double)(1u << pbits) > 0.5 * S) pbits++;
+        if (h->force_part_bits >= 0) pbits = std::min<uint32_t>((uint32_t)h->force_part_bits, rp.bucket_bits);
+        rp.part_bits = pbits;
+        const uint32_t P = 1u << pbits;
+        h->radix_bits = pbits;
+        h->lds_slots = S;
+        h->base.ensure((size_t)W * P * 4);
+        h->ptotal.ensure((size_t)P * 4);
+        h->pstart.ensure((size_t)(P + 1) * 8);
+        h->recs.ensure(std::max<uint64_t>(1, h->rows_total) * rp.rec_words * 8);
+        rp.base = h->base.as<uint32_t>();
+        rp.ptotal = h->ptotal.as<uint32_t>();
+        rp.pstart = h->pstart.as<uint64_t>();
+        rp.recs = h->recs.as<uint64_t>();
+        hipLaunchKernelGGL(k_rp_scan_w, dim3((P + 255) / 256), dim3(256), 0, s, rp);
+        hipLaunchKernelGGL(k_rp_scan_p, dim3(1), dim3(1024), 0, s, rp, T.plan);
+        const size_t k3_lds = k3_fixed + (size_t)P * 4;
+        allow_lds(k_rp_scatter, k3_lds);
+        tm.mark();  // scatter start
+        for (size_t i = 0; i < h->batches.size(); i++) {
+            auto &b = h->batches[i];
+            if (!b.nrows) continue;
+            RadixParams r = rp;
+            r.wg_offset = wg_off[i];
+            r.rows_per_wg = rows_per[i];
+            hipLaunchKernelGGL(k_rp_scatter, dim3(wg_n[i]), dim3(RP_BLOCK), k3_lds, s, h->dq, b.db, r, T,
+                               h->umask_all);
+        }
+        h->agg_grid = W;
+        tm.mark();  // scatter end
+        const size_t k4_lds = lds_table_bytes(S, nacc);
+        DevOut o = make_out(h);
+        DevTopn t = make_topn(h);
+        if (t.k <= 16) {
+            allow_lds(k_rp_agg<16>, k4_lds);
+            hipLaunchKernelGGL(k_rp_agg<16>, dim3(P), dim3(512), k4_lds, s, h->dq, rp, T, o, t, S, S * 7 / 8);
+            hipLaunchKernelGGL(k_heavy_reduce<16>, dim3(t.enabled ? 1 : std::max<uint32_t>(1, (rp.hmax + 255) / 256)), dim3(256), 0, s,
+                               h->dq, rp, T, o, t);
+        } else {
+            allow_lds(k_rp_agg<32>, k4_lds);
+            hipLaunchKernelGGL(k_rp_agg<32>, dim3(P), dim3(512), k4_lds, s, h->dq, rp, T, o, t, S, S * 7 / 8);
+            hipLaunchKernelGGL(k_heavy_reduce<32>, dim3(t.enabled ? 1 : std::max<uint32_t>(1, (rp.hmax + 255) / 256)), dim3(256), 0, s,
+                               h->dq, rp, T, o, t);
+        }
+        HIPCHK(hipGetLastError());
+        tm.mark();  // partition aggregate end
+        emit_table(h, T, cap);
+        topn_final(h);
+        tm.mark();  // emit end
+        read_plan(h);
+        if (!(h->hplan->overflow & 1)) break;
+        if (cap >= h->cap_max) fail(GBAGG_ERROR, "fallback hash table overflow at maximum size");
+        cap = std::min<uint64_t>(h->cap_max, cap * 16);
+        h->retries++;
+        tm.truncate(2);
+    }
 }
 
 void finalize(gbagg_handle *h) {
     if (h->finalized) fail(GBAGG_INVALID, "already finalized: call gbagg_reset first");
     ensure_runtime(h);
-    const DeviceInfo &di = device_info();
     uint64_t rows = std::max<uint64_t>(h->rows_total, h->expected_rows);
     if (rows > (1ULL << 30)) fail(GBAGG_INVALID, "more than 2^30 rows in one aggregate");
-    uint64_t cap_max = next_pow2(std::max<uint64_t>(1024, rows + rows / 4 + 2));
-    if (h->dq.key_mode == KM_NONE) cap_max = 1024;
-    if (cap_max > h->cap_max) {
-        h->cap_max = cap_max;
-        h->table.ensure((cap_max + 2) * (1 + (size_t)h->dq.nacc) * 8);
-        uint64_t nblk = (cap_max + 2 + COMPACT_CHUNK - 1) / COMPACT_CHUNK;
-        h->blkcnt.ensure(nblk * 4 + 16);
-    }
-    // HLL partial sketches
-    uint64_t parts = 0;
-    for (auto &b : h->batches)
-        if (b.nrows) parts += std::min<uint64_t>((b.nrows + 255) / 256, (uint64_t)di.cus * 2);
-    h->hll.ensure(std::max<uint64_t>(1, parts) * HLL_M);
-    // outputs: full result needs cap_max + 2 rows, TopN only K rows
-    uint64_t out_rows = h->topn_device ? h->topn_k : h->cap_max + 2;
+    h->cap_max = std::max<uint64_t>(h->cap_max, next_pow2(std::max<uint64_t>(1024, rows + rows / 4 + 2)));
+    // result buffers: every row may be its own group; TopN keeps K rows
+    h->out_capacity = h->topn_device ? h->topn_k : h->rows_total + 2;
     h->out_data.resize(h->outs.size());
     h->out_valid.resize(h->outs.size());
     for (size_t c = 0; c < h->outs.size(); c++) {
-        h->out_data[c].ensure(out_rows * 8);
-        h->out_valid[c].ensure(out_rows);
+        h->out_data[c].ensure(std::max<uint64_t>(1, h->out_capacity) * 8);
+        h->out_valid[c].ensure(std::max<uint64_t>(1, h->out_capacity));
     }
+    h->retries = 0;
+    h->radix_bits = 0;
+    Timer tm{h, {}};
+    tm.mark();  // start
+    RadixParams rp{};
+    gbagg_strategy st = h->strategy;
+    if (h->dq.key_mode == KM_NONE) st = GBAGG_STRATEGY_GLOBAL_HASH;
     if (h->topn_device) {
-        uint32_t grid = (uint32_t)std::min<uint64_t>((h->cap_max + 2 + 4095) / 4096, (uint64_t)di.cus * 4);
-        h->cand_slot.ensure((size_t)grid * h->topn_k * 4);
-        h->cand_ord.ensure((size_t)grid * h->topn_k * 16);
-        h->final_slot.ensure((size_t)h->topn_k * 4);
+        uint32_t bound = 4096 + 64 * h->topn_k * 8 + (uint32_t)std::min<uint64_t>(h->heavy_max, 65536);
+        h->cand_cap = std::max<uint32_t>(bound, 16384) * 4;
+        h->cand_rec.ensure((size_t)h->cand_cap * (1 + h->dq.nacc) * 8);
+        h->cand_null.ensure(h->cand_cap);
     }
     std::memset(h->hplan, 0, sizeof(DevPlan));
-    h->hplan->cap_max = h->cap_max;
-    h->retries = 0;
-    bool use_hll = true;
-    for (;;) {
-        run_pipeline(h, use_hll);
-        if (!h->hplan->overflow) break;
-        if (h->hplan->cap >= h->cap_max) fail(GBAGG_ERROR, "global hash table overflow at maximum size");
-        uint64_t next = std::min<uint64_t>(h->cap_max, h->hplan->cap * 4);
-        std::memset(h->hplan, 0, sizeof(DevPlan));
-        h->hplan->cap_max = h->cap_max;
-        h->hplan->forced_cap = next;
-        use_hll = false;
-        h->retries++;
+    upload_plan(h, 1024);
+    if (st == GBAGG_STRATEGY_GLOBAL_HASH && h->dq.key_mode == KM_NONE) {
+        tm.mark();
+        run_lds_hash(h, tm, true);
+    } else {
+        run_sample(h, rp);
+        tm.mark();  // sample end
+        read_plan(h);
+        if (st == GBAGG_STRATEGY_AUTO || st == GBAGG_STRATEGY_DIRECT) {
+            const DevPlan &p = *h->hplan;
+            bool low_card = p.sample_distinct <= 4096 && p.sample_f1 * 8 <= p.sample_distinct + 8;
+            st = (low_card || h->rows_total <= 65536) ? GBAGG_STRATEGY_LDS_HASH : GBAGG_STRATEGY_RADIX;
+        }
+        if (st == GBAGG_STRATEGY_RADIX) run_radix(h, tm, rp);
+        else run_lds_hash(h, tm, st == GBAGG_STRATEGY_GLOBAL_HASH);
+    }
+    h->used_strategy = st;
+    // phase times from the recorded events
+    const size_t ne = tm.evs.size();
+    h->ms_total = tm.between(0, ne - 1);
+    h->ms_sample = tm.between(0, 1);
+    if (st == GBAGG_STRATEGY_RADIX && ne >= 8) {
+        // start, sample, count start, count end, scatter start, scatter end, partition end, emit end
+        h->ms_count = tm.between(2, 3);
+        h->ms_agg = tm.between(4, 5);
+        h->ms_partition = tm.between(5, 6);
+        h->ms_emit = tm.between(6, 7);
+    } else if (ne >= 5) {
+        h->ms_count = 0;
+        h->ms_agg = tm.between(ne - 3, ne - 2);
+        h->ms_partition = 0;
+        h->ms_emit = tm.between(ne - 2, ne - 1);
     }
     h->result_rows = h->hplan->groups;
+    if (h->result_rows > h->out_capacity) fail(GBAGG_ERROR, "result larger than its buffer");
     h->finalized = true;
     h->result_ready = false;
 }
@@ -963,20 +1244,45 @@ gbagg_status gbagg_plan_info_get(gbagg_handle *h, gbagg_plan_info *out) {
     return guarded([&] {
         if (!h || !out) fail(GBAGG_INVALID, "null argument");
         std::memset(out, 0, sizeof(*out));
-        out->strategy = (h->dq.key_mode == KM_NONE || h->strategy == GBAGG_STRATEGY_GLOBAL_HASH)
-                            ? GBAGG_STRATEGY_GLOBAL_HASH
-                            : GBAGG_STRATEGY_LDS_HASH;
+        out->strategy = h->used_strategy;
         if (h->hplan) {
             out->estimated_groups = h->hplan->est_groups;
             out->table_capacity = h->hplan->cap;
             out->groups = h->hplan->groups;
+            out->heavy_keys = h->hplan->heavy_count;
+            out->cold_rows = h->hplan->cold_rows;
+            out->sample_rows = h->hplan->sample_rows;
+            out->sample_distinct = h->hplan->sample_distinct;
+            out->sample_f1 = h->hplan->sample_f1;
         }
         out->lds_slots = h->lds_slots;
+        out->radix_bits = h->radix_bits;
         out->workgroups = h->agg_grid;
         out->retries = h->retries;
         out->rows_consumed = h->rows_total;
         out->ms_aggregate = h->ms_agg;
         out->ms_total = h->ms_total;
+        out->ms_sample = h->ms_sample;
+        out->ms_count = h->ms_count;
+        out->ms_partition = h->ms_partition;
+        out->ms_emit = h->ms_emit;
+    });
+}
+
+gbagg_status gbagg_set_param(gbagg_handle *h, const char *name, int64_t value) {
+    return guarded([&] {
+        if (!h || !name) fail(GBAGG_INVALID, "null argument");
+        std::string n(name);
+        auto in = [&](int64_t lo, int64_t hi) {
+            if (value < lo || value > hi) fail(GBAGG_INVALID, n + " out of range");
+        };
+        if (n == "heavy_max") { in(0, 4096); h->heavy_max = (uint32_t)value; }
+        else if (n == "heavy_min") { in(1, 64); h->heavy_min = (uint32_t)value; }
+        else if (n == "fallback_cap") { in(1024, 1LL << 30); h->radix_fallback_cap = next_pow2((uint64_t)value); }
+        else if (n == "part_bits") { in(-1, 13); h->force_part_bits = (int32_t)value; }
+        else if (n == "lds_kb") { in(8, 160); h->k4_lds_kb = (uint32_t)value; }
+        else if (n == "strategy") { in(0, 4); h->strategy = (gbagg_strategy)value; }
+        else fail(GBAGG_INVALID, "unknown parameter " + n);
     });
 }
 

@@ -72,15 +72,22 @@ struct DevBatch {
     uint64_t nrows;
 };
 
-// Device-resident plan values computed on the GPU (no host round trip).
+// Device-resident plan values and run statistics.
 struct DevPlan {
     uint64_t est_groups;   // HyperLogLog estimate
     uint64_t cap;          // global table slots in use (power of two)
-    uint64_t groups;       // occupied slots after the run (compaction)
-    uint32_t overflow;     // a probe sequence exceeded its bound: rerun larger
-    uint32_t topn_count;
-    uint64_t forced_cap;   // host override for a rerun (0 = from the estimate)
-    uint64_t cap_max;      // allocation limit
+    uint64_t groups;       // result rows written so far
+    uint32_t overflow;     // bit 0: table probe bound hit; bit 1: candidate buffer full
+    uint32_t cand_count;   // TopN candidates pushed
+    uint64_t cap_max;      // allocation limit of the global table
+    uint64_t sample_rows;      // rows sampled for heavy hitters (after the filter)
+    uint64_t sample_distinct;  // distinct keys in the sample
+    uint64_t sample_f1;        // keys seen exactly once in the sample
+    uint64_t heavy_count;      // keys in the heavy-hitter table
+    uint64_t heavy_sample_rows;// sample rows whose key is heavy
+    uint64_t cold_rows;        // rows scattered to partitions
+    uint32_t heavy_threshold;  // minimum sample count of a heavy key
+    uint32_t pad;
 };
 
 // The global table: (cap + 2) records of (1 + nacc) words: key, accumulators.

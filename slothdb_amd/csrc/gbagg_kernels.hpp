@@ -1,17 +1,21 @@
-// gbagg_kernels.hpp -- the HIP kernels of the GROUP BY operator.
+// gbagg_kernels.hpp -- kernels shared by every aggregation strategy, and the
+// LDS-hash strategy for low and moderate group counts.
 //
-//   k_hll            HyperLogLog sketch of the (filtered) group keys, per block
-//   k_hll_finish     merges the block sketches, estimates the group count and
-//                    sizes the global table on the device (no host round trip)
-//   k_table_init     global table: empty keys, identity accumulators
-//   k_aggregate      fused filter + key hash + per-workgroup LDS hash table
-//                    with atomic COUNT/SUM/MIN/MAX; rows whose key finds no LDS
-//                    slot, and the LDS table at the end, merge into the global
-//                    table with device-scope atomics
+//   k_hll_finish      merges HyperLogLog partial sketches into an estimate
+//   k_table_init      global hash table: empty keys, identity accumulators
+//   k_aggregate       LDS-hash strategy: fused filter + key hash + per-workgroup
+//                     LDS table with atomic COUNT/SUM/MIN/MAX; rows whose key
+//                     finds no LDS slot, and the LDS table at the end, merge
+//                     into the global table with device-scope atomics
+//   k_aggregate_global  global table only (scalar aggregates, tests)
 //   k_occ_count / k_scan_blocks / k_compact_write
-//                    dense, columnar result in slot order
-//   k_topn_local / k_topn_final / k_gather_write
-//                    block-wise TopN straight off the table (SetTopNHint)
+//                     append the global table's groups to the dense result
+//   k_topn_table      TopN candidates of the global table
+//   k_topn_final      block-wise TopN over all producers' candidates
+//
+// Every group leaves the operator through write_group(): dense result
+// columns in full mode, or one of K candidate records per producer block in
+// TopN mode (SetTopNHint).
 #pragma once
 
 #include "gbagg_device.hpp"
@@ -27,35 +31,16 @@ constexpr uint32_t INVALID_SLOT = 0xFFFFFFFFu;
 // HyperLogLog (Flajolet et al. 2007), p = 12: register = top 12 hash bits,
 // rank = leading zeros of the remaining bits + 1.
 // ------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_hll(DevQuery q, DevBatch b, uint32_t umask,
-                                             uint8_t *partial) {
-    __shared__ uint32_t reg[HLL_M];
-    for (int i = threadIdx.x; i < HLL_M; i += blockDim.x) reg[i] = 0;
-    __syncthreads();
-    const uint64_t n = b.nrows;
-    for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n;
-         r += (uint64_t)gridDim.x * blockDim.x) {
-        Row row;
-        load_row(q, b, r, umask, row);
-        if (q.npred && !eval_preds(q, b, row)) continue;
-        uint64_t w;
-        int kr = make_key(q, b, row, w);
-        if (kr == KEY_DROP) continue;
-        uint64_t h = kr == KEY_NULL ? 0x5bd1e9955bd1e995ULL : hash64(w);
-        uint32_t idx = (uint32_t)(h >> (64 - HLL_P));
-        uint64_t rest = h << HLL_P;
-        uint32_t rank = rest ? (uint32_t)__clzll((long long)rest) + 1 : (64 - HLL_P + 1);
-        __hip_atomic_fetch_max(&reg[idx], rank, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    }
-    __syncthreads();
-    uint8_t *out = partial + (uint64_t)blockIdx.x * HLL_M;
-    for (int i = threadIdx.x; i < HLL_M; i += blockDim.x) out[i] = (uint8_t)reg[i];
+__device__ __forceinline__ void hll_add(uint32_t *reg, uint64_t h) {
+    uint32_t idx = (uint32_t)(h >> (64 - HLL_P));
+    uint64_t rest = h << HLL_P;
+    uint32_t rank = rest ? (uint32_t)__clzll((long long)rest) + 1 : (64 - HLL_P + 1);
+    __hip_atomic_fetch_max(&reg[idx], rank, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
-// One block of 1024 threads. growth: table slots per estimated group.
+// One block of 1024 threads: max-merge `nparts` sketches, write the estimate.
 __global__ __launch_bounds__(1024) void k_hll_finish(const uint8_t *partial, uint32_t nparts,
-                                                     DevPlan *plan, double growth,
-                                                     uint64_t min_cap) {
+                                                     DevPlan *plan) {
     __shared__ double s_sum[1024];
     __shared__ uint32_t s_zero[1024];
     double sum = 0.0;
@@ -81,28 +66,7 @@ __global__ __launch_bounds__(1024) void k_hll_finish(const uint8_t *partial, uin
         const double alpha = 0.7213 / (1.0 + 1.079 / m);
         double e = alpha * m * m / s_sum[0];
         if (e <= 2.5 * m && s_zero[0] > 0) e = m * log(m / (double)s_zero[0]);
-        uint64_t est = (uint64_t)(e + 0.5);
-        plan->est_groups = est;
-        uint64_t want = plan->forced_cap ? plan->forced_cap : (uint64_t)((double)est * growth) + 1;
-        uint64_t cap = min_cap;
-        while (cap < want && cap < plan->cap_max) cap <<= 1;
-        if (cap > plan->cap_max) cap = plan->cap_max;
-        plan->cap = cap;
-        plan->overflow = 0;
-        plan->groups = 0;
-        plan->topn_count = 0;
-    }
-}
-
-// Host-sized variant (estimate skipped): cap comes from forced_cap.
-__global__ void k_plan_fixed(DevPlan *plan) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        uint64_t cap = plan->forced_cap;
-        if (cap > plan->cap_max) cap = plan->cap_max;
-        plan->cap = cap;
-        plan->overflow = 0;
-        plan->groups = 0;
-        plan->topn_count = 0;
+        plan->est_groups = (uint64_t)(e + 0.5);
     }
 }
 
@@ -155,7 +119,28 @@ __device__ __forceinline__ void g_update(const DevQuery &q, const DevTable &T, u
     }
 }
 
-// Per-workgroup LDS table: keys[S] then acc[nacc][S] then a fill counter.
+// Merge one partial state (acc a at acc[a * stride]) into global record g.
+__device__ __forceinline__ void g_merge(const DevQuery &q, const DevTable &T, uint64_t g,
+                                        const uint64_t *acc, uint32_t stride) {
+    uint64_t *dst = T.rec + g * T.stride + 1;
+    for (int a = 0; a < q.nacc; a++) {
+        uint64_t x = acc[(uint64_t)a * stride];
+        if (acc_nontrivial(q.acc_op[a], x)) acc_apply<__HIP_MEMORY_SCOPE_AGENT>(dst + a, q.acc_op[a], x);
+    }
+}
+
+// Rows with a NULL group key or the SENTINEL key go to the table's two side
+// records (cap + 1, cap); returns false for rows the key logic dropped.
+__device__ __forceinline__ bool side_row(const DevQuery &q, const DevTable &T, uint64_t cap, int kr,
+                                         uint64_t w, const Row &row) {
+    if (kr == KEY_NULL) { g_update(q, T, cap + 1, row); return true; }
+    if (w == SENTINEL) { g_update(q, T, cap, row); return true; }
+    return false;
+}
+
+// ------------------------------------------------------------------------
+// Per-workgroup LDS hash table: keys[S], then acc[nacc][S], then a fill count.
+// ------------------------------------------------------------------------
 struct LdsTable {
     uint64_t *key;
     uint64_t *acc;
@@ -163,6 +148,23 @@ struct LdsTable {
     uint32_t S;
     uint32_t limit;
 };
+
+__device__ __forceinline__ LdsTable lds_table_setup(uint64_t *smem, const DevQuery &q, uint32_t S,
+                                                    uint32_t limit) {
+    LdsTable t;
+    t.S = S;
+    t.limit = limit;
+    t.key = smem;
+    t.acc = smem + S;
+    t.fill = (uint32_t *)(t.acc + (uint64_t)S * q.nacc);
+    for (uint32_t i = threadIdx.x; i < S; i += blockDim.x) t.key[i] = SENTINEL;
+    for (int a = 0; a < q.nacc; a++) {
+        uint64_t id = acc_identity(q.acc_op[a]);
+        for (uint32_t i = threadIdx.x; i < S; i += blockDim.x) t.acc[(uint64_t)a * S + i] = id;
+    }
+    if (threadIdx.x == 0) *t.fill = 0;
+    return t;
+}
 
 __device__ __forceinline__ int lds_find(const LdsTable &t, uint64_t h, uint64_t k) {
     const uint32_t mask = t.S - 1;
@@ -187,12 +189,19 @@ __device__ __forceinline__ int lds_find(const LdsTable &t, uint64_t h, uint64_t 
     return -1;
 }
 
+__device__ __forceinline__ void lds_update(const DevQuery &q, const LdsTable &t, int slot,
+                                           const Row &row) {
+    for (int a = 0; a < q.nacc; a++) {
+        uint64_t x;
+        if (acc_input(q, a, row, x))
+            acc_apply<__HIP_MEMORY_SCOPE_WORKGROUP>(t.acc + (uint64_t)a * t.S + slot, q.acc_op[a], x);
+    }
+}
+
 // All 64 lanes carry the same key (scalar aggregates, clustered keys): reduce
-// across the wave and let one lane apply. Returns true if handled.
-__device__ __forceinline__ bool wave_uniform_apply(const DevQuery &q, const Row &row, uint64_t w,
-                                                   uint64_t *dst, uint32_t dst_stride, bool lds) {
-    uint64_t w0 = __shfl(w, 0);
-    if (__ballot(w == w0) != ~0ULL) return false;
+// across the wave and let lane 0 apply. dst holds acc a at dst[a * stride].
+__device__ __forceinline__ void wave_reduce_apply(const DevQuery &q, const Row &row, uint64_t *dst,
+                                                  uint32_t stride, bool lds) {
     const int lane = threadIdx.x & 63;
     for (int a = 0; a < q.nacc; a++) {
         int op = q.acc_op[a];
@@ -202,8 +211,9 @@ __device__ __forceinline__ bool wave_uniform_apply(const DevQuery &q, const Row 
         for (int off = 32; off > 0; off >>= 1) {
             uint64_t y = __shfl_xor(x, off);
             switch (op) {
-            case ACC_SUM_F: x = (uint64_t)__double_as_longlong(
-                                __longlong_as_double((long long)x) + __longlong_as_double((long long)y));
+            case ACC_SUM_F:
+                x = (uint64_t)__double_as_longlong(__longlong_as_double((long long)x) +
+                                                   __longlong_as_double((long long)y));
                 break;
             case ACC_MIN_I: x = (int64_t)x < (int64_t)y ? x : y; break;
             case ACC_MAX_I: x = (int64_t)x > (int64_t)y ? x : y; break;
@@ -213,31 +223,18 @@ __device__ __forceinline__ bool wave_uniform_apply(const DevQuery &q, const Row 
             }
         }
         if (lane == 0 && acc_nontrivial(op, x)) {
-            if (lds) acc_apply<__HIP_MEMORY_SCOPE_WORKGROUP>(dst + (uint64_t)a * dst_stride, op, x);
-            else acc_apply<__HIP_MEMORY_SCOPE_AGENT>(dst + a, op, x);
+            if (lds) acc_apply<__HIP_MEMORY_SCOPE_WORKGROUP>(dst + (uint64_t)a * stride, op, x);
+            else acc_apply<__HIP_MEMORY_SCOPE_AGENT>(dst + (uint64_t)a * stride, op, x);
         }
     }
-    return true;
 }
 
 __global__ __launch_bounds__(AGG_BLOCK) void k_aggregate(DevQuery q, DevBatch b, DevTable T,
                                                          uint32_t S, uint32_t fill_limit,
                                                          uint32_t umask) {
     extern __shared__ uint64_t smem[];
-    LdsTable t;
-    t.S = S;
-    t.limit = fill_limit;
-    t.key = smem;
-    t.acc = smem + S;
-    t.fill = (uint32_t *)(t.acc + (uint64_t)S * q.nacc);
-    for (uint32_t i = threadIdx.x; i < S; i += blockDim.x) t.key[i] = SENTINEL;
-    for (int a = 0; a < q.nacc; a++) {
-        uint64_t id = acc_identity(q.acc_op[a]);
-        for (uint32_t i = threadIdx.x; i < S; i += blockDim.x) t.acc[(uint64_t)a * S + i] = id;
-    }
-    if (threadIdx.x == 0) *t.fill = 0;
+    LdsTable t = lds_table_setup(smem, q, S, fill_limit);
     __syncthreads();
-
     const uint64_t cap = T.plan->cap;
     const uint64_t mask = cap - 1;
     const uint64_t n = b.nrows;
@@ -258,23 +255,11 @@ __global__ __launch_bounds__(AGG_BLOCK) void k_aggregate(DevQuery q, DevBatch b,
             int kr = KEY_DROP;
             if (live) kr = make_key(q, b, rows[j], w);
             if (kr == KEY_DROP) continue;
-            if (kr == KEY_NULL) {
-                g_update(q, T, cap + 1, rows[j]);
-                continue;
-            }
-            if (w == SENTINEL) {
-                g_update(q, T, cap, rows[j]);
-                continue;
-            }
+            if (side_row(q, T, cap, kr, w, rows[j])) continue;
             const uint64_t h = hash64(w);
             int slot = lds_find(t, h, w);
             if (slot >= 0) {
-                for (int a = 0; a < q.nacc; a++) {
-                    uint64_t x;
-                    if (acc_input(q, a, rows[j], x))
-                        acc_apply<__HIP_MEMORY_SCOPE_WORKGROUP>(t.acc + (uint64_t)a * S + slot,
-                                                                q.acc_op[a], x);
-                }
+                lds_update(q, t, slot, rows[j]);
             } else {
                 int64_t g = g_find(T, mask, h, w);
                 if (g >= 0) g_update(q, T, (uint64_t)g, rows[j]);
@@ -286,16 +271,11 @@ __global__ __launch_bounds__(AGG_BLOCK) void k_aggregate(DevQuery q, DevBatch b,
         uint64_t k = t.key[i];
         if (k == SENTINEL) continue;
         int64_t g = g_find(T, mask, hash64(k), k);
-        if (g < 0) continue;
-        uint64_t *acc = T.rec + (uint64_t)g * T.stride + 1;
-        for (int a = 0; a < q.nacc; a++) {
-            uint64_t x = t.acc[(uint64_t)a * S + i];
-            if (acc_nontrivial(q.acc_op[a], x)) acc_apply<__HIP_MEMORY_SCOPE_AGENT>(acc + a, q.acc_op[a], x);
-        }
+        if (g >= 0) g_merge(q, T, (uint64_t)g, t.acc + i, S);
     }
 }
 
-// Scalar aggregate / global-table-only variant (no LDS table).
+// Global-table-only variant (no LDS table).
 __global__ __launch_bounds__(AGG_BLOCK) void k_aggregate_global(DevQuery q, DevBatch b, DevTable T,
                                                                 uint32_t umask) {
     const uint64_t cap = T.plan->cap;
@@ -324,14 +304,13 @@ __global__ __launch_bounds__(AGG_BLOCK) void k_aggregate_global(DevQuery q, DevB
                     int64_t g = -1;
                     if ((threadIdx.x & 63) == 0) g = g_find(T, mask, hash64(w0), w0);
                     g = __shfl(g, 0);
-                    if (g >= 0) wave_uniform_apply(q, rows[j], w, T.rec + (uint64_t)g * T.stride + 1, 1, false);
+                    if (g >= 0) wave_reduce_apply(q, rows[j], T.rec + (uint64_t)g * T.stride + 1, 1, false);
                     continue;
                 }
             }
             if (kr == KEY_DROP) continue;
-            int64_t g;
-            if (kr == KEY_NULL) g = (int64_t)(cap + 1);
-            else g = g_find(T, mask, hash64(w), w);
+            if (side_row(q, T, cap, kr, w, rows[j])) continue;
+            int64_t g = g_find(T, mask, hash64(w), w);
             if (g >= 0) g_update(q, T, (uint64_t)g, rows[j]);
         }
     }
@@ -363,6 +342,7 @@ struct DevOut {
     int32_t key_type[MAX_OUT]; // OUT_KEY: CK_I32 / CK_I64 / CK_U32
     int32_t ncols;
     int32_t key_mode;
+    uint64_t capacity;         // rows the columns hold
 };
 
 __device__ __forceinline__ uint64_t key_component(int key_mode, uint64_t key, int comp) {
@@ -370,132 +350,85 @@ __device__ __forceinline__ uint64_t key_component(int key_mode, uint64_t key, in
     return key;
 }
 
-__device__ __forceinline__ void write_row(const DevOut &o, const uint64_t *rec, bool null_key,
-                                          uint64_t row) {
-    const uint64_t key = rec[0];
-    const uint64_t *acc = rec + 1;
+// One result row from a group state: acc a at acc[a * stride].
+__device__ __forceinline__ void write_group(const DevOut &o, uint64_t key, const uint64_t *acc,
+                                            uint32_t stride, bool null_key, uint64_t row) {
+    if (row >= o.capacity) return;
     for (int c = 0; c < o.ncols; c++) {
-        uint64_t cnt = o.cnt_acc[c] >= 0 ? acc[o.cnt_acc[c]] : 1;
+        const int va = o.val_acc[c];
+        uint64_t cnt = o.cnt_acc[c] >= 0 ? acc[(uint64_t)o.cnt_acc[c] * stride] : 1;
         uint8_t ok = cnt != 0;
+        uint64_t v = o.kind[c] == OUT_KEY ? 0 : acc[(uint64_t)va * stride];
         switch (o.kind[c]) {
         case OUT_KEY: {
-            uint64_t v = key_component(o.key_mode, key, o.val_acc[c]);
+            uint64_t kv = key_component(o.key_mode, key, va);
             ok = !null_key;
-            if (o.key_type[c] == CK_I64) ((int64_t *)o.data[c])[row] = ok ? (int64_t)v : 0;
-            else ((int32_t *)o.data[c])[row] = ok ? (int32_t)(uint32_t)v : 0;
+            if (o.key_type[c] == CK_I64) ((int64_t *)o.data[c])[row] = ok ? (int64_t)kv : 0;
+            else ((int32_t *)o.data[c])[row] = ok ? (int32_t)(uint32_t)kv : 0;
             break;
         }
         case OUT_COUNT:
-            ((int64_t *)o.data[c])[row] = (int64_t)acc[o.val_acc[c]];
+            ((int64_t *)o.data[c])[row] = (int64_t)v;
             ok = 1;
             break;
         case OUT_SUM_I:
         case OUT_MM_I64:
-            ((int64_t *)o.data[c])[row] = ok ? (int64_t)acc[o.val_acc[c]] : 0;
+            ((int64_t *)o.data[c])[row] = ok ? (int64_t)v : 0;
             break;
         case OUT_MM_I32:
-            ((int32_t *)o.data[c])[row] = ok ? (int32_t)(int64_t)acc[o.val_acc[c]] : 0;
+            ((int32_t *)o.data[c])[row] = ok ? (int32_t)(int64_t)v : 0;
             break;
         case OUT_SUM_F:
-            ((double *)o.data[c])[row] = ok ? __longlong_as_double((long long)acc[o.val_acc[c]]) : 0.0;
+            ((double *)o.data[c])[row] = ok ? __longlong_as_double((long long)v) : 0.0;
             break;
         case OUT_AVG_I:
-            ((double *)o.data[c])[row] = ok ? (double)(int64_t)acc[o.val_acc[c]] / (double)cnt : 0.0;
+            ((double *)o.data[c])[row] = ok ? (double)(int64_t)v / (double)cnt : 0.0;
             break;
         case OUT_AVG_F:
-            ((double *)o.data[c])[row] =
-                ok ? __longlong_as_double((long long)acc[o.val_acc[c]]) / (double)cnt : 0.0;
+            ((double *)o.data[c])[row] = ok ? __longlong_as_double((long long)v) / (double)cnt : 0.0;
             break;
         default:  // OUT_MM_F
-            ((double *)o.data[c])[row] =
-                ok ? __longlong_as_double((long long)f64_unord(acc[o.val_acc[c]])) : 0.0;
+            ((double *)o.data[c])[row] = ok ? __longlong_as_double((long long)f64_unord(v)) : 0.0;
             break;
         }
         o.valid[c][row] = ok;
     }
 }
 
-__device__ __forceinline__ bool slot_occupied(const uint64_t *rec, uint64_t s, uint64_t cap) {
-    return s < cap ? rec[0] != SENTINEL : rec[1] != 0;  // side slots: COUNT(*) != 0
-}
-
-__global__ __launch_bounds__(256) void k_occ_count(DevTable T, uint32_t *blk_counts) {
-    const uint64_t cap = T.plan->cap;
-    const uint64_t nrec = cap + 2;
-    const uint64_t nblk = (nrec + COMPACT_CHUNK - 1) / COMPACT_CHUNK;
-    __shared__ uint32_t s_cnt;
-    for (uint64_t blk = blockIdx.x; blk < nblk; blk += gridDim.x) {
-        if (threadIdx.x == 0) s_cnt = 0;
-        __syncthreads();
-        uint32_t c = 0;
-        uint64_t end = min(nrec, (blk + 1) * COMPACT_CHUNK);
-        for (uint64_t s = blk * COMPACT_CHUNK + threadIdx.x; s < end; s += blockDim.x)
-            c += slot_occupied(T.rec + s * T.stride, s, cap);
-        atomicAdd(&s_cnt, c);
-        __syncthreads();
-        if (threadIdx.x == 0) blk_counts[blk] = s_cnt;
-        __syncthreads();
+// Block-wide exclusive prefix of `mine` plus one atomicAdd on *counter for the
+// block total: returns this thread's first output row. All threads must call.
+__device__ __forceinline__ uint64_t block_reserve(uint32_t mine, uint64_t *counter) {
+    __shared__ uint32_t s_wave[16];
+    __shared__ uint64_t s_base;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+    uint32_t incl = mine;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        uint32_t y = __shfl_up(incl, off);
+        if (lane >= off) incl += y;
     }
-}
-
-// Exclusive scan of the block counts in place; total -> plan->groups.
-__global__ __launch_bounds__(1024) void k_scan_blocks(DevTable T, uint32_t *blk_counts) {
-    const uint64_t nrec = T.plan->cap + 2;
-    const uint32_t nblk = (uint32_t)((nrec + COMPACT_CHUNK - 1) / COMPACT_CHUNK);
-    __shared__ uint64_t s_part[1024];
-    const uint32_t per = (nblk + blockDim.x - 1) / blockDim.x;
-    const uint32_t b0 = threadIdx.x * per, b1 = min(nblk, b0 + per);
-    uint64_t local = 0;
-    for (uint32_t i = b0; i < b1; i++) local += blk_counts[i];
-    s_part[threadIdx.x] = local;
+    if (lane == 63) s_wave[wave] = incl;
     __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {  // Hillis-Steele inclusive scan
-        uint64_t v = threadIdx.x >= (uint32_t)off ? s_part[threadIdx.x - off] : 0;
-        __syncthreads();
-        s_part[threadIdx.x] += v;
-        __syncthreads();
-    }
-    uint64_t run = threadIdx.x ? s_part[threadIdx.x - 1] : 0;
-    for (uint32_t i = b0; i < b1; i++) {
-        uint32_t c = blk_counts[i];
-        blk_counts[i] = (uint32_t)run;
-        run += c;
-    }
-    if (threadIdx.x == blockDim.x - 1) T.plan->groups = s_part[blockDim.x - 1];
-}
-
-__global__ __launch_bounds__(256) void k_compact_write(DevTable T, const uint32_t *blk_offsets,
-                                                       DevOut o) {
-    const uint64_t cap = T.plan->cap;
-    const uint64_t nrec = cap + 2;
-    const uint64_t nblk = (nrec + COMPACT_CHUNK - 1) / COMPACT_CHUNK;
-    __shared__ uint32_t s_wave[4];
-    __shared__ uint32_t s_base;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (uint64_t blk = blockIdx.x; blk < nblk; blk += gridDim.x) {
-        if (threadIdx.x == 0) s_base = blk_offsets[blk];
-        __syncthreads();
-        for (uint64_t s0 = blk * COMPACT_CHUNK; s0 < min(nrec, (blk + 1) * COMPACT_CHUNK); s0 += blockDim.x) {
-            uint64_t s = s0 + threadIdx.x;
-            const uint64_t *rec = T.rec + s * T.stride;
-            bool occ = s < nrec && slot_occupied(rec, s, cap);
-            uint64_t bal = __ballot(occ);
-            if (lane == 0) s_wave[wave] = (uint32_t)__popcll(bal);
-            __syncthreads();
-            uint32_t before = 0;
-            for (int w = 0; w < wave; w++) before += s_wave[w];
-            uint32_t rank = (uint32_t)__popcll(bal & ((1ULL << lane) - 1ULL));
-            if (occ) write_row(o, rec, s == cap + 1, s_base + before + rank);
-            __syncthreads();
-            if (threadIdx.x == 0) s_base += s_wave[0] + s_wave[1] + s_wave[2] + s_wave[3];
-            __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t run = 0;
+        for (int w = 0; w < nw; w++) {
+            uint32_t c = s_wave[w];
+            s_wave[w] = run;
+            run += c;
         }
+        s_base = run ? __hip_atomic_fetch_add(counter, (uint64_t)run, __ATOMIC_RELAXED,
+                                              __HIP_MEMORY_SCOPE_AGENT)
+                     : 0;
     }
+    __syncthreads();
+    const uint64_t first = s_base + s_wave[wave] + incl - mine;
+    __syncthreads();  // the shared words are reused by the next call
+    return first;
 }
 
 // ------------------------------------------------------------------------
-// TopN (SetTopNHint): order value of a slot as an unsigned "larger is
-// better" word, ties broken by the smaller group key.
+// TopN (SetTopNHint): order value of a group as an unsigned "larger is
+// better" word; ties go to the smaller group key.
 // ------------------------------------------------------------------------
 enum TopnKind : int32_t {
     TK_KEY = 0,    // the group key word (single-column keys)
@@ -507,6 +440,7 @@ enum TopnKind : int32_t {
 };
 
 struct DevTopn {
+    int32_t enabled;
     int32_t kind;
     int32_t acc;
     int32_t cnt_acc;
@@ -514,9 +448,10 @@ struct DevTopn {
     int32_t key_mode;
     int32_t key_signed[2];
     uint32_t k;
-    uint32_t *cand_slot;  // [blocks][k]
-    uint64_t *cand_ord;   // [blocks][k][2]
-    uint32_t *final_slot; // [k]
+    uint32_t rec_words;   // 1 + nacc
+    uint64_t *cand_rec;   // candidate group states: key, accs
+    uint8_t *cand_null;
+    uint32_t cand_cap;
 };
 
 __device__ __forceinline__ uint64_t key_order(const DevTopn &t, uint64_t key) {
@@ -528,29 +463,29 @@ __device__ __forceinline__ uint64_t key_order(const DevTopn &t, uint64_t key) {
     return t.key_signed[0] ? key ^ 0x8000000000000000ULL : key;
 }
 
-__device__ __forceinline__ void slot_order(const DevTopn &t, const uint64_t *rec, bool null_key,
-                                           uint64_t &ord, uint64_t &tie) {
-    const uint64_t *acc = rec + 1;
+__device__ __forceinline__ void group_order(const DevTopn &t, uint64_t key, const uint64_t *acc,
+                                            uint32_t stride, bool null_key, uint64_t &ord,
+                                            uint64_t &tie) {
     uint64_t o;
+    const uint64_t v = acc[(uint64_t)t.acc * stride];
     switch (t.kind) {
-    case TK_KEY: o = null_key ? 0 : key_order(t, rec[0]) ; break;
-    case TK_ACC_I: o = acc[t.acc] ^ 0x8000000000000000ULL; break;
-    case TK_ACC_O: o = acc[t.acc]; break;
-    case TK_SUM_F: o = f64_ord(acc[t.acc]); break;
+    case TK_KEY: o = null_key ? 0 : key_order(t, key); break;
+    case TK_ACC_I: o = v ^ 0x8000000000000000ULL; break;
+    case TK_ACC_O: o = v; break;
+    case TK_SUM_F: o = f64_ord(v); break;
     case TK_AVG_I: {
-        double v = (double)(int64_t)acc[t.acc] / (double)acc[t.cnt_acc];
-        o = f64_ord((uint64_t)__double_as_longlong(v));
+        double d = (double)(int64_t)v / (double)acc[(uint64_t)t.cnt_acc * stride];
+        o = f64_ord((uint64_t)__double_as_longlong(d));
         break;
     }
     default: {
-        double v = __longlong_as_double((long long)acc[t.acc]) / (double)acc[t.cnt_acc];
-        o = f64_ord((uint64_t)__double_as_longlong(v));
+        double d = __longlong_as_double((long long)v) / (double)acc[(uint64_t)t.cnt_acc * stride];
+        o = f64_ord((uint64_t)__double_as_longlong(d));
         break;
     }
     }
     ord = t.ascending ? ~o : o;
-    // smaller key wins ties; the NULL key sorts first
-    tie = null_key ? ~0ULL : ~key_order(t, rec[0]);
+    tie = null_key ? ~0ULL : ~key_order(t, key);  // the NULL key sorts first
 }
 
 __device__ __forceinline__ bool topn_better(uint64_t oa, uint64_t ta, uint32_t sa, uint64_t ob,
@@ -568,7 +503,7 @@ struct TopList {
 #pragma unroll
         for (int i = 0; i < KMAX; i++) { o[i] = 0; t[i] = 0; s[i] = INVALID_SLOT; }
     }
-    // Replace the worst entry if the candidate beats it (K <= KMAX entries used).
+    // Replace the worst of the first K entries if the candidate beats it.
     __device__ __forceinline__ void offer(uint64_t oc, uint64_t tc, uint32_t sc, uint32_t K) {
         int w = 0;
         uint64_t wo = o[0], wt = t[0];
@@ -586,17 +521,16 @@ struct TopList {
     }
 };
 
-// Block-wide selection of the best K of blockDim.x * KMAX candidates held in
-// per-thread lists; writes them best-first to out_*.
+// Block-wide selection of the best K entries over the per-thread lists;
+// winners' ids land in s_win[0..K) best-first (INVALID_SLOT when fewer).
 template <int KMAX>
-__device__ void block_select(TopList<KMAX> &L, uint32_t K, uint32_t *out_slot, uint64_t *out_ord) {
-    __shared__ uint64_t s_o[1024 / 64], s_t[1024 / 64];
-    __shared__ uint32_t s_s[1024 / 64], s_owner[1024 / 64];
+__device__ void block_select(TopList<KMAX> &L, uint32_t K, uint32_t *s_win) {
+    __shared__ uint64_t s_o[16], s_t[16];
+    __shared__ uint32_t s_s[16], s_owner[16];
     __shared__ uint32_t s_win_owner;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = (blockDim.x + 63) >> 6;
     uint32_t used = 0;  // bit i: list entry i already emitted
     for (uint32_t r = 0; r < K; r++) {
-        // this thread's best unused entry
         uint64_t bo = 0, bt = 0;
         uint32_t bs = INVALID_SLOT;
         int bi = -1;
@@ -619,9 +553,10 @@ __device__ void block_select(TopList<KMAX> &L, uint32_t K, uint32_t *out_slot, u
             uint64_t wo = s_o[0], wt = s_t[0];
             uint32_t ws = s_s[0], wn = s_owner[0];
             for (int w = 1; w < nwaves; w++)
-                if (topn_better(s_o[w], s_t[w], s_s[w], wo, wt, ws)) { wo = s_o[w]; wt = s_t[w]; ws = s_s[w]; wn = s_owner[w]; }
-            out_slot[r] = ws;
-            if (out_ord) { out_ord[2 * r] = wo; out_ord[2 * r + 1] = wt; }
+                if (topn_better(s_o[w], s_t[w], s_s[w], wo, wt, ws)) {
+                    wo = s_o[w]; wt = s_t[w]; ws = s_s[w]; wn = s_owner[w];
+                }
+            s_win[r] = ws;
             s_win_owner = ws == INVALID_SLOT ? 0xFFFFFFFFu : wn;
         }
         __syncthreads();
@@ -630,8 +565,123 @@ __device__ void block_select(TopList<KMAX> &L, uint32_t K, uint32_t *out_slot, u
     }
 }
 
+// Appends this block's winners (producer-local ids in s_win) to the candidate
+// buffer; fetch(id, key, acc_ptr, stride, null) describes a producer group.
+template <typename Fetch>
+__device__ void topn_push(const DevTopn &t, DevPlan *plan, const uint32_t *s_win, Fetch fetch) {
+    __shared__ uint32_t s_nwin, s_base;
+    if (threadIdx.x == 0) {
+        uint32_t n = 0;
+        for (uint32_t r = 0; r < t.k; r++) n += s_win[r] != INVALID_SLOT;
+        s_nwin = n;
+        s_base = n ? __hip_atomic_fetch_add(&plan->cand_count, n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
+    }
+    __syncthreads();
+    for (uint32_t r = threadIdx.x; r < s_nwin; r += blockDim.x) {
+        uint32_t dst = s_base + r;
+        if (dst >= t.cand_cap) {
+            __hip_atomic_fetch_or(&plan->overflow, 2u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            continue;
+        }
+        uint64_t key;
+        const uint64_t *acc;
+        uint32_t stride;
+        bool null_key;
+        fetch(s_win[r], key, acc, stride, null_key);
+        uint64_t *out = t.cand_rec + (uint64_t)dst * t.rec_words;
+        out[0] = key;
+        for (uint32_t a = 0; a + 1 < t.rec_words; a++) out[1 + a] = acc[(uint64_t)a * stride];
+        t.cand_null[dst] = null_key;
+    }
+    __syncthreads();
+}
+
+// ------------------------------------------------------------------------
+// Global table -> result (append) or TopN candidates
+// ------------------------------------------------------------------------
+__device__ __forceinline__ bool slot_occupied(const uint64_t *rec, uint64_t s, uint64_t cap) {
+    return s < cap ? rec[0] != SENTINEL : rec[1] != 0;  // side records: COUNT(*) != 0
+}
+
+__global__ __launch_bounds__(256) void k_occ_count(DevTable T, uint32_t *blk_counts) {
+    const uint64_t cap = T.plan->cap;
+    const uint64_t nrec = cap + 2;
+    const uint64_t nblk = (nrec + COMPACT_CHUNK - 1) / COMPACT_CHUNK;
+    __shared__ uint32_t s_cnt;
+    for (uint64_t blk = blockIdx.x; blk < nblk; blk += gridDim.x) {
+        if (threadIdx.x == 0) s_cnt = 0;
+        __syncthreads();
+        uint32_t c = 0;
+        uint64_t end = min(nrec, (blk + 1) * COMPACT_CHUNK);
+        for (uint64_t s = blk * COMPACT_CHUNK + threadIdx.x; s < end; s += blockDim.x)
+            c += slot_occupied(T.rec + s * T.stride, s, cap);
+        atomicAdd(&s_cnt, c);
+        __syncthreads();
+        if (threadIdx.x == 0) blk_counts[blk] = s_cnt;
+        __syncthreads();
+    }
+}
+
+// Exclusive scan of the block counts in place, starting at the rows already
+// in the result (plan->groups); plan->groups grows by the table's groups.
+__global__ __launch_bounds__(1024) void k_scan_blocks(DevTable T, uint32_t *blk_counts,
+                                                      uint64_t *blk_base) {
+    const uint64_t nrec = T.plan->cap + 2;
+    const uint32_t nblk = (uint32_t)((nrec + COMPACT_CHUNK - 1) / COMPACT_CHUNK);
+    __shared__ uint64_t s_part[1024];
+    const uint32_t per = (nblk + blockDim.x - 1) / blockDim.x;
+    const uint32_t b0 = threadIdx.x * per, b1 = min(nblk, b0 + per);
+    uint64_t local = 0;
+    for (uint32_t i = b0; i < b1; i++) local += blk_counts[i];
+    s_part[threadIdx.x] = local;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {  // Hillis-Steele inclusive scan
+        uint64_t v = threadIdx.x >= (uint32_t)off ? s_part[threadIdx.x - off] : 0;
+        __syncthreads();
+        s_part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    const uint64_t start = T.plan->groups;
+    uint64_t run = start + (threadIdx.x ? s_part[threadIdx.x - 1] : 0);
+    for (uint32_t i = b0; i < b1; i++) {
+        blk_base[i] = run;
+        run += blk_counts[i];
+    }
+    __syncthreads();
+    if (threadIdx.x == blockDim.x - 1) T.plan->groups = start + s_part[blockDim.x - 1];
+}
+
+__global__ __launch_bounds__(256) void k_compact_write(DevTable T, const uint64_t *blk_base, DevOut o) {
+    const uint64_t cap = T.plan->cap;
+    const uint64_t nrec = cap + 2;
+    const uint64_t nblk = (nrec + COMPACT_CHUNK - 1) / COMPACT_CHUNK;
+    __shared__ uint32_t s_wave[4];
+    __shared__ uint64_t s_base;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (uint64_t blk = blockIdx.x; blk < nblk; blk += gridDim.x) {
+        if (threadIdx.x == 0) s_base = blk_base[blk];
+        __syncthreads();
+        for (uint64_t s0 = blk * COMPACT_CHUNK; s0 < min(nrec, (blk + 1) * COMPACT_CHUNK); s0 += blockDim.x) {
+            uint64_t s = s0 + threadIdx.x;
+            const uint64_t *rec = T.rec + s * T.stride;
+            bool occ = s < nrec && slot_occupied(rec, s, cap);
+            uint64_t bal = __ballot(occ);
+            if (lane == 0) s_wave[wave] = (uint32_t)__popcll(bal);
+            __syncthreads();
+            uint32_t before = 0;
+            for (int w = 0; w < wave; w++) before += s_wave[w];
+            uint32_t rank = (uint32_t)__popcll(bal & ((1ULL << lane) - 1ULL));
+            if (occ) write_group(o, rec[0], rec + 1, 1, s == cap + 1, s_base + before + rank);
+            __syncthreads();
+            if (threadIdx.x == 0) s_base += s_wave[0] + s_wave[1] + s_wave[2] + s_wave[3];
+            __syncthreads();
+        }
+    }
+}
+
 template <int KMAX>
-__global__ __launch_bounds__(256) void k_topn_local(DevTable T, DevTopn t) {
+__global__ __launch_bounds__(256) void k_topn_table(DevTable T, DevTopn t) {
+    __shared__ uint32_t s_win[KMAX];
     const uint64_t cap = T.plan->cap, nrec = cap + 2;
     TopList<KMAX> L;
     L.init();
@@ -641,39 +691,44 @@ __global__ __launch_bounds__(256) void k_topn_local(DevTable T, DevTopn t) {
         const uint64_t *rec = T.rec + s * T.stride;
         if (!slot_occupied(rec, s, cap)) continue;
         uint64_t o, ti;
-        slot_order(t, rec, s == cap + 1, o, ti);
+        group_order(t, rec[0], rec + 1, 1, s == cap + 1, o, ti);
         L.offer(o, ti, (uint32_t)s, t.k);
     }
-    block_select<KMAX>(L, t.k, t.cand_slot + (uint64_t)blockIdx.x * t.k,
-                       t.cand_ord + (uint64_t)blockIdx.x * t.k * 2);
+    block_select<KMAX>(L, t.k, s_win);
+    topn_push(t, T.plan, s_win, [&](uint32_t s, uint64_t &key, const uint64_t *&acc, uint32_t &stride, bool &nk) {
+        const uint64_t *rec = T.rec + (uint64_t)s * T.stride;
+        key = rec[0];
+        acc = rec + 1;
+        stride = 1;
+        nk = s == cap + 1;
+    });
 }
 
+// The final TopN over every producer's candidates: one block of 1024.
 template <int KMAX>
-__global__ __launch_bounds__(1024) void k_topn_final(DevTable T, DevTopn t, uint32_t nblocks) {
+__global__ __launch_bounds__(1024) void k_topn_final(DevTopn t, DevPlan *plan, DevOut o) {
+    __shared__ uint32_t s_win[KMAX];
     TopList<KMAX> L;
     L.init();
-    const uint64_t total = (uint64_t)nblocks * t.k;
-    for (uint64_t i = threadIdx.x; i < total; i += blockDim.x) {
-        uint32_t s = t.cand_slot[i];
-        if (s == INVALID_SLOT) continue;
-        L.offer(t.cand_ord[2 * i], t.cand_ord[2 * i + 1], s, t.k);
+    const uint32_t total = min(plan->cand_count, t.cand_cap);
+    for (uint32_t i = threadIdx.x; i < total; i += blockDim.x) {
+        const uint64_t *rec = t.cand_rec + (uint64_t)i * t.rec_words;
+        uint64_t ord, tie;
+        group_order(t, rec[0], rec + 1, 1, t.cand_null[i] != 0, ord, tie);
+        L.offer(ord, tie, i, t.k);
     }
-    block_select<KMAX>(L, t.k, t.final_slot, nullptr);
+    block_select<KMAX>(L, t.k, s_win);
+    for (uint32_t r = threadIdx.x; r < t.k; r += blockDim.x) {
+        uint32_t i = s_win[r];
+        if (i == INVALID_SLOT) continue;
+        const uint64_t *rec = t.cand_rec + (uint64_t)i * t.rec_words;
+        write_group(o, rec[0], rec + 1, 1, t.cand_null[i] != 0, r);
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
         uint32_t n = 0;
-        for (uint32_t r = 0; r < t.k; r++) n += t.final_slot[r] != INVALID_SLOT;
-        T.plan->topn_count = n;
-        T.plan->groups = n;
-    }
-}
-
-__global__ void k_gather_write(DevTable T, DevTopn t, DevOut o) {
-    const uint64_t cap = T.plan->cap;
-    for (uint32_t r = threadIdx.x; r < t.k; r += blockDim.x) {
-        uint32_t s = t.final_slot[r];
-        if (s == INVALID_SLOT) continue;
-        write_row(o, T.rec + (uint64_t)s * T.stride, s == cap + 1, r);
+        for (uint32_t r = 0; r < t.k; r++) n += s_win[r] != INVALID_SLOT;
+        plan->groups = n;
     }
 }
 

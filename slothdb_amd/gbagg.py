@@ -74,7 +74,10 @@ class PlanInfo(C.Structure):
     _fields_ = [("strategy", C.c_int), ("estimated_groups", C.c_uint64), ("groups", C.c_uint64),
                 ("table_capacity", C.c_uint64), ("lds_slots", C.c_uint32),
                 ("radix_bits", C.c_uint32), ("workgroups", C.c_uint32), ("retries", C.c_uint32),
-                ("rows_consumed", C.c_uint64), ("ms_aggregate", C.c_double), ("ms_total", C.c_double)]
+                ("rows_consumed", C.c_uint64), ("heavy_keys", C.c_uint64), ("cold_rows", C.c_uint64),
+                ("sample_rows", C.c_uint64), ("sample_distinct", C.c_uint64), ("sample_f1", C.c_uint64),
+                ("ms_aggregate", C.c_double), ("ms_total", C.c_double), ("ms_sample", C.c_double),
+                ("ms_count", C.c_double), ("ms_partition", C.c_double), ("ms_emit", C.c_double)]
 
 
 # Every symbol include/slothdb_gpu.h declares, with its ctypes signature.
@@ -90,6 +93,7 @@ _SIGS = {
     "gbagg_reset": (C.c_int, [C.c_void_p]),
     "gbagg_plan_info_get": (C.c_int, [C.c_void_p, C.POINTER(PlanInfo)]),
     "gbagg_destroy": (None, [C.c_void_p]),
+    "gbagg_set_param": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int64]),
     "gbagg_hash_aggregate_i32": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p,
                                            C.c_void_p, C.c_void_p, C.c_uint64,
                                            C.POINTER(C.c_uint64)]),
@@ -331,8 +335,9 @@ class Result:
 class GpuHashAggregate:
     """The operator: plan once, consume batches, finalize, read the result."""
 
-    def __init__(self, query):
+    def __init__(self, query, params=None):
         self.query = query
+        self.params = dict(params or {})
         self._h = None
 
     # PhysicalOperator surface
@@ -350,6 +355,8 @@ class GpuHashAggregate:
             h = C.c_void_p()
             _check(lib().gbagg_plan_create(C.byref(q), C.byref(h)))
             self._h = h
+            for k, v in self.params.items():
+                _check(lib().gbagg_set_param(self._h, k.encode(), int(v)))
         else:
             _check(lib().gbagg_reset(self._h))
         self._emit_pos = 0
@@ -429,9 +436,9 @@ class GpuHashAggregate:
             pass
 
 
-def run(query, columns, device=False):
+def run(query, columns, device=False, params=None):
     """Plan, consume one batch, finalize and return the Result."""
-    op = GpuHashAggregate(query)
+    op = GpuHashAggregate(query, params)
     op.Init()
     if device:
         dcols = [DeviceColumn(c) for c in columns]

@@ -1,0 +1,87 @@
+"""The C-ABI library loads without a GPU, exports every symbol the public
+header declares, and validates plans on the host (no device calls)."""
+import ctypes as C
+import json
+import os
+import re
+import subprocess
+
+import pytest
+
+from slothdb_amd import gbagg as G
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "slothdb_gpu.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(gbagg_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_declarations_are_bound():
+    names = declared_functions()
+    assert len(names) >= 18
+    assert set(names) == set(G._SIGS), set(names) ^ set(G._SIGS)
+
+
+def test_library_exports_every_declared_symbol():
+    out = subprocess.run(["nm", "-D", "--defined-only", G.LIB_PATH], capture_output=True, text=True,
+                         check=True).stdout
+    exported = set(re.findall(r"\sT\s(\w+)$", out, flags=re.M))
+    missing = [n for n in declared_functions() if n not in exported]
+    assert not missing, missing
+    lib = G.lib()  # loads and binds every signature
+    for n in declared_functions():
+        assert hasattr(lib, n)
+
+
+def test_abi_version_and_host_hash():
+    assert G.lib().gbagg_abi_version() == 1
+    with open(os.path.join(ROOT, "tests", "golden", "hash_u64.json")) as f:
+        gold = json.load(f)
+    for x, h in gold["u64"]:
+        assert G.lib().gbagg_hash_u64(x) == h
+
+
+def _plan(q):
+    d, keep = q.desc()
+    h = C.c_void_p()
+    st = G.lib().gbagg_plan_create(C.byref(d), C.byref(h))
+    msg = G.lib().gbagg_last_error().decode()
+    if st == G.OK:
+        G.lib().gbagg_destroy(h)
+    return st, msg
+
+
+def test_valid_plans_create_without_device():
+    assert _plan(G.Query([G.INTEGER, G.BIGINT], [0], [(G.COUNT_STAR, None), (G.SUM, 1)]))[0] == G.OK
+    assert _plan(G.Query([G.INTEGER, G.VARCHAR, G.INTEGER], [0, 1], [(G.SUM, 2)],
+                         predicates=[(2, G.LT, 30)], nullable=[0, 0, 0], topn=(2, False, 10)))[0] == G.OK
+    assert _plan(G.Query([G.DOUBLE], [], [(G.AVG, 0), (G.MIN, 0)]))[0] == G.OK
+
+
+@pytest.mark.parametrize("q,needle", [
+    (G.Query([G.DOUBLE], [0], [(G.COUNT_STAR, None)]), "DOUBLE group keys"),
+    (G.Query([G.BIGINT, G.BIGINT], [0, 1], [(G.COUNT_STAR, None)], nullable=[0, 0]), "BIGINT"),
+    (G.Query([G.INTEGER, G.INTEGER], [0, 1], [(G.COUNT_STAR, None)]), "drop_null_keys"),
+    (G.Query([G.INTEGER, G.VARCHAR], [0], [(G.SUM, 1)]), "numeric"),
+    (G.Query([G.INTEGER, G.VARCHAR], [0], [(G.MIN, 1)]), "VARCHAR"),
+    (G.Query([G.INTEGER], [0], [(G.COUNT_STAR, None)], predicates=[(0, G.LIKE, "x")]), "LIKE"),
+    (G.Query([G.INTEGER], [0], [(G.SUM, 3)]), "out of range"),
+    (G.Query([G.INTEGER] * 10, [0], [(G.SUM, c) for c in range(1, 10)]), "8 distinct"),
+    (G.Query([G.INTEGER], [], []), "nothing"),
+    (G.Query([G.INTEGER], [0], [(G.COUNT_STAR, None)], topn=(5, False, 3)), "topn_col"),
+])
+def test_invalid_plans_are_rejected(q, needle):
+    st, msg = _plan(q)
+    assert st == G.INVALID
+    assert needle in msg
+
+
+def test_null_arguments():
+    assert G.lib().gbagg_plan_create(None, None) == G.INVALID
+    assert G.lib().gbagg_finalize(None) == G.INVALID
+    assert G.lib().gbagg_reset(None) == G.INVALID
+    G.lib().gbagg_destroy(None)

@@ -195,3 +195,80 @@ def test_topn_count_desc_matches_exact():
     res, info = G.run(q, cols, device=True)
     exp, types = _exact(datagen.highcard_query(), cols)
     assert_topn_tie_aware(res.rows(), exp, order_col=1, k=10, types=types)
+
+
+# ---- radix-partitioned strategy ------------------------------------------
+
+@pytest.mark.parametrize("strategy", [G.AUTO, G.RADIX])
+def test_radix_highcard(strategy):
+    q, cols = datagen.highcard_workload(n=3_000_000, nkeys=2_000_000)
+    q.strategy = strategy
+    res, info, worst = _check(q, cols)
+    assert info["strategy"] == G.RADIX
+    assert info["heavy_keys"] > 0 and info["cold_rows"] < 3_000_000
+    print(f"radix info {info} worst_rel={worst:.2e}")
+
+
+@pytest.mark.parametrize("params", [
+    {"heavy_max": 0},                                  # no heavy hitters: the hot key is cold
+    {"part_bits": 0, "lds_kb": 8, "fallback_cap": 1024},  # tiny LDS: most rows take the fallback table, which grows
+    {"part_bits": 13},                                 # maximal fanout: many empty partitions
+])
+def test_radix_params(params):
+    q, cols = datagen.highcard_workload(n=500_000, nkeys=200_000)
+    q.strategy = G.RADIX
+    print(f"[parity] radix params {params}", flush=True)
+    res, info = G.run(q, cols, device=True, params=params)
+    exp, types = _exact(q, cols)
+    assert_same_groups(res.rows(), exp, 1, types)
+    print(info)
+
+
+def test_radix_nullable_keys_and_inputs():
+    rng = np.random.default_rng(21)
+    n = 400_000
+    k = rng.integers(-2**40, 2**40, n // 4).astype(np.int64)[rng.integers(0, n // 4, n)]
+    cols = [G.Column(G.BIGINT, k, (rng.random(n) > 0.02).astype(np.uint8)),
+            G.Column(G.INTEGER, rng.integers(-1000, 1000, n), (rng.random(n) > 0.3).astype(np.uint8)),
+            G.Column(G.DOUBLE, rng.normal(0, 1e6, n), (rng.random(n) > 0.6).astype(np.uint8))]
+    for drop in (False, True):
+        q = G.Query([G.BIGINT, G.INTEGER, G.DOUBLE], [0],
+                    [(G.COUNT_STAR, None), (G.COUNT, 1), (G.SUM, 1), (G.MIN, 1), (G.MAX, 2), (G.AVG, 2)],
+                    strategy=G.RADIX, drop_null_keys=drop)
+        _check(q, cols)
+
+
+def test_radix_host_batches():
+    q, cols = datagen.highcard_workload(n=600_000, nkeys=300_000)
+    q.strategy = G.RADIX
+    op = G.GpuHashAggregate(q)
+    op.Init()
+    RG = 122_880
+    for s in range(0, 600_000, RG):
+        op.consume([G.Column(c.type, c.values[s:s + RG]) for c in cols])
+    op.finalize()
+    exp, types = _exact(q, cols)
+    assert_same_groups(op.result().rows(), exp, 1, types)
+    assert op.plan_info()["strategy"] == G.RADIX
+    op.close()
+
+
+def test_radix_composite_topn_and_full():
+    q, cols = datagen.composite_workload(n=3_000_000, dict_size=200_000, topn=10)
+    dic = cols[1].dictionary
+    exp, types = _exact(datagen.composite_query(topn=None), cols, {1: dic})
+    res, info = G.run(q, cols, device=True)
+    assert info["strategy"] == G.RADIX and res.num_rows == 10
+    assert_topn_tie_aware(res.rows(), exp, order_col=2, k=10, types=types)
+    q2 = datagen.composite_query(topn=None)
+    res2, info2 = G.run(q2, cols, device=True)
+    assert_same_groups(res2.rows(), exp, 2, types)
+
+
+def test_radix_topn_sum_ascending():
+    q, cols = datagen.highcard_workload(n=800_000, nkeys=400_000)
+    q.strategy = G.RADIX
+    q.topn = (2, True, 25)  # ORDER BY SUM(v1) ASC LIMIT 25
+    res, info = G.run(q, cols, device=True)
+    exp, types = _exact(datagen.highcard_query(), cols)
+    assert_topn_tie_aware(res.rows(), exp, order_col=2, k=25, types=types, descending=False)

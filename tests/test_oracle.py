@@ -1,0 +1,304 @@
+"""The CPU oracle pinned against the reference's own tests, the reference's
+compiled aggregators (golden fixtures), independent implementations (pandas,
+brute force) and closed forms. Runs without a GPU."""
+import json
+import os
+from collections import defaultdict
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from slothdb_amd import datagen, gbagg as G
+from tests._util import assert_same_groups, oracle_rows
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def load(name):
+    with open(os.path.join(GOLD, name)) as f:
+        return json.load(f)
+
+
+def exact_rows(q, cols, dicts=None):
+    c, v, t, ov = O.groupby_exact(q, cols)
+    return oracle_rows(c, v, t, dicts), t, ov
+
+
+# ---- hash -----------------------------------------------------------------
+
+def closed_form_hash(x):
+    p = (x & (2**64 - 1)) * 0x9E3779B97F4A7C15
+    return (p & (2**64 - 1)) ^ (p >> 64)
+
+
+def test_hash_golden_vectors():
+    g = load("hash_u64.json")
+    for x, h in g["i64"]:
+        assert O.hash_u64(x) == h
+        assert closed_form_hash(x) == h
+    for x, h in g["u64"]:
+        assert O.hash_u64(x) == h
+    # int32 keys are sign-extended to 64 bits before hashing
+    for x, h in g["i32"]:
+        assert O.hash_u64(x) == h
+
+
+def test_hash_matches_reference_build_when_present():
+    R = O.ref()
+    if R is None:
+        pytest.skip("oracle/_ref not built (no reference tree)")
+    rng = np.random.default_rng(5)
+    for x in rng.integers(-2**63, 2**63 - 1, 2000, dtype=np.int64).tolist():
+        assert R.ref_hash_i64(x) == O.hash_u64(x)
+
+
+# ---- the reference's known-answer tests ------------------------------------
+
+def emp():
+    depts = G.Dictionary(["Engineering", "Sales", "Marketing"])
+    return depts, [G.Column(G.VARCHAR, np.array([0, 0, 1, 1, 2], np.uint32), dictionary=depts),
+                   G.Column(G.INTEGER, np.array([100, 120, 80, 90, 95], np.int32))]
+
+
+def test_known_answers_emp():
+    depts, cols = emp()
+    aggs = [(G.COUNT_STAR, None), (G.SUM, 1), (G.AVG, 1), (G.MIN, 1), (G.MAX, 1)]
+    rows, _, _ = exact_rows(G.Query([G.VARCHAR, G.INTEGER], [], aggs, nullable=[0, 0]), cols)
+    assert rows == [(5, 485, 97.0, 80, 120)]  # test_aggregation.cpp:16-56
+    rows, _, _ = exact_rows(G.Query([G.VARCHAR, G.INTEGER], [0], aggs, nullable=[0, 0]), cols, {0: depts})
+    assert sorted(rows) == [("Engineering", 2, 220, 110.0, 100, 120), ("Marketing", 1, 95, 95.0, 95, 95),
+                            ("Sales", 2, 170, 85.0, 80, 90)]  # test_aggregation.cpp:58-120
+
+
+def test_known_answer_empty_and_series():
+    q = G.Query([G.INTEGER], [], [(G.COUNT_STAR, None), (G.SUM, 0)], nullable=[0])
+    rows, _, _ = exact_rows(q, [G.Column(G.INTEGER, np.zeros(0, np.int32))])
+    assert rows == [(0, None)]  # test_aggregation.cpp:122-130
+    q = G.Query([G.INTEGER], [], [(G.SUM, 0)], nullable=[0])
+    rows, _, _ = exact_rows(q, [G.Column(G.INTEGER, np.arange(1, 10001, dtype=np.int32))])
+    assert rows == [(50005000,)]  # test_parallel.cpp:105-123
+
+
+def test_known_answer_gpu_engine_hash_aggregate():
+    q = G.Query([G.INTEGER, G.INTEGER], [0], [(G.SUM, 1), (G.COUNT_STAR, None)], nullable=[0, 0])
+    cols = [G.Column(G.INTEGER, [1, 2, 1, 2, 1]), G.Column(G.INTEGER, [10, 20, 30, 40, 50])]
+    rows, _, _ = exact_rows(q, cols)
+    assert sorted(rows) == [(1, 90, 3), (2, 60, 2)]  # test_performance.cpp:153-171
+
+
+def test_closed_form_arithmetic_series_per_group():
+    # key = i % m, value = i: SUM per key is an arithmetic series
+    n, m = 1_000_003, 97
+    i = np.arange(n, dtype=np.int64)
+    q = G.Query([G.BIGINT, G.BIGINT], [0], [(G.COUNT_STAR, None), (G.SUM, 1), (G.MIN, 1), (G.MAX, 1)],
+                nullable=[0, 0])
+    rows, _, _ = exact_rows(q, [G.Column(G.BIGINT, i % m), G.Column(G.BIGINT, i)])
+    for k, cnt, s, mn, mx in rows:
+        c = (n - 1 - k) // m + 1
+        assert cnt == c and mn == k and mx == k + (c - 1) * m
+        assert s == c * k + m * c * (c - 1) // 2
+
+
+# ---- brute force on tiny inputs --------------------------------------------
+
+def brute(q, cols):
+    """Direct restatement of SQL GROUP BY semantics in pure Python."""
+    n = len(cols[0])
+    groups = defaultdict(list)
+
+    def val(c, r):
+        col = cols[c]
+        if col.validity is not None and not col.validity[r]:
+            return None
+        x = col.values[r].item()
+        if col.type == G.VARCHAR:
+            return col.dictionary.get(x)
+        return x
+
+    for r in range(n):
+        ok = True
+        for (c, op, lit) in q.predicates:
+            x = val(c, r)
+            if x is None:
+                ok = False
+                break
+            if op == G.IN:
+                ok = x in lit
+            elif op in (G.LIKE, G.NOT_LIKE):
+                ok = (lit in x) == (op == G.LIKE)
+            else:
+                if cols[c].type in (G.INTEGER, G.BIGINT) and isinstance(lit, float):
+                    lit = int(lit)  # ival = (int64)dval
+                ok = {G.LT: x < lit, G.LE: x <= lit, G.GT: x > lit, G.GE: x >= lit,
+                      G.EQ: x == lit, G.NE: x != lit}[op]
+            if not ok:
+                break
+        if not ok:
+            continue
+        key = tuple(val(c, r) for c in q.group_cols)
+        if q.drop_null_keys and any(k is None for k in key):
+            continue
+        groups[key].append(r)
+    out = []
+    for key, rs in groups.items():
+        row = list(key)
+        for kind, c in q.aggs:
+            if kind == G.COUNT_STAR:
+                row.append(len(rs))
+                continue
+            xs = [val(c, r) for r in rs if val(c, r) is not None]
+            if kind == G.COUNT:
+                row.append(len(xs))
+            elif not xs:
+                row.append(None)
+            elif kind == G.SUM:
+                row.append(sum(xs) if cols[c].type != G.DOUBLE else float(np.sum(np.array(xs, np.longdouble))))
+            elif kind == G.AVG:
+                row.append(float(sum(xs)) / len(xs) if cols[c].type != G.DOUBLE
+                           else float(np.sum(np.array(xs, np.longdouble)) / len(xs)))
+            elif kind == G.MIN:
+                row.append(min(xs))
+            else:
+                row.append(max(xs))
+        out.append(tuple(row))
+    return out
+
+
+def tiny_table(seed, n=400):
+    rng = np.random.default_rng(seed)
+    words = G.Dictionary(["a", "bb", "ab", "ba", "", "abc", "bb"])  # duplicate "bb"
+    cols = [G.Column(G.INTEGER, rng.integers(-3, 4, n), (rng.random(n) > 0.1).astype(np.uint8)),
+            G.Column(G.BIGINT, rng.integers(-2**50, 2**50, n), (rng.random(n) > 0.2).astype(np.uint8)),
+            G.Column(G.DOUBLE, rng.normal(0, 10, n), (rng.random(n) > 0.2).astype(np.uint8)),
+            G.Column(G.VARCHAR, rng.integers(0, 7, n).astype(np.uint32), (rng.random(n) > 0.1).astype(np.uint8),
+                     dictionary=words),
+            G.Column(G.INTEGER, rng.integers(0, 5, n))]
+    return cols
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_exact_vs_brute_force(seed):
+    cols = tiny_table(seed)
+    types = [c.type for c in cols]
+    aggs = [(G.COUNT_STAR, None), (G.COUNT, 1), (G.SUM, 1), (G.MIN, 2), (G.MAX, 0), (G.AVG, 2), (G.SUM, 2),
+            (G.AVG, 0)]
+    cases = [
+        G.Query(types, [0], aggs),
+        G.Query(types, [3], aggs),
+        G.Query(types, [4, 3], aggs, drop_null_keys=True),
+        G.Query(types, [0], aggs, predicates=[(4, G.LT, 3), (2, G.GT, -5.5)]),
+        G.Query(types, [4], aggs, predicates=[(3, G.LIKE, "b")]),
+        G.Query(types, [4], aggs, predicates=[(3, G.NE, "bb"), (0, G.IN, [-1, 2, 3])]),
+        G.Query(types, [0], aggs, predicates=[(4, G.LE, 2.7)]),  # int column, double literal
+        G.Query(types, [], aggs),
+    ]
+    for q in cases:
+        dicts = {i: cols[c].dictionary for i, c in enumerate(q.group_cols) if cols[c].type == G.VARCHAR}
+        got, t, _ = exact_rows(q, cols, dicts)
+        exp = brute(q, cols)
+        if not q.group_cols and not exp:
+            exp = [tuple([0 if k in (G.COUNT_STAR, G.COUNT) else None for k, _ in q.aggs])]
+        assert_same_groups(got, exp, len(q.group_cols), t)
+
+
+def test_planted_mistake_is_caught():
+    """The comparison helper flags an oracle result with one SUM off by one."""
+    cols = tiny_table(3)
+    q = G.Query([c.type for c in cols], [0], [(G.SUM, 1)])
+    c, v, t, _ = O.groupby_exact(q, cols)
+    good = oracle_rows(c, v, t)
+    i = next(r for r in range(len(good)) if good[r][1] is not None)
+    c[1][i] += 1
+    bad = oracle_rows(c, v, t)
+    with pytest.raises(AssertionError):
+        assert_same_groups(bad, good, 1, t)
+
+
+def test_neumaier_sum():
+    x = np.array([1e16, 1.0, -1e16, 1.0] * 1000)
+    q = G.Query([G.INTEGER, G.DOUBLE], [0], [(G.SUM, 1), (G.AVG, 1)], nullable=[0, 0])
+    rows, _, _ = exact_rows(q, [G.Column(G.INTEGER, np.zeros(len(x), np.int32)), G.Column(G.DOUBLE, x)])
+    assert rows == [(0, 2000.0, 0.5)]  # naive left-to-right summation gives 0
+
+
+def test_int64_sum_beyond_2_53_is_exact():
+    v = np.array([2**53 + 1, 2**53 + 1, -3], np.int64)
+    q = G.Query([G.INTEGER, G.BIGINT], [0], [(G.SUM, 1)], nullable=[0, 0])
+    rows, _, ov = exact_rows(q, [G.Column(G.INTEGER, np.zeros(3, np.int32)), G.Column(G.BIGINT, v)])
+    assert rows == [(0, 2**54 - 1)] and not ov
+
+
+# ---- the reference's compiled aggregators (golden fixtures) ----------------
+
+def test_inline_row_agg_golden():
+    from tests.golden.make_golden import inline_columns
+    g = load("inline_row_agg.json")
+    a, b, v0, v1, m0, m1 = inline_columns(**g["params"])
+    cols = [G.Column(G.INTEGER, a), G.Column(G.INTEGER, b), G.Column(G.BIGINT, v0, m0), G.Column(G.BIGINT, v1, m1)]
+    q = G.Query([G.INTEGER, G.INTEGER, G.BIGINT, G.BIGINT], [0, 1],
+                [(G.COUNT_STAR, None), (G.SUM, 2), (G.SUM, 3), (G.COUNT, 2), (G.COUNT, 3)],
+                nullable=[0, 0, 1, 1])
+    rows, _, _ = exact_rows(q, cols)
+    # InlineRowAgg reports SUM 0 where the oracle reports NULL (no non-NULL input)
+    got = sorted((r[0], r[1], r[2], r[3] or 0, r[4] or 0, r[5], r[6]) for r in rows)
+    assert got == [tuple(r) for r in g["rows"]]
+
+
+def test_radix_count_golden():
+    g = load("radix_count.json")
+    p = g["params"]
+    k = datagen.highcard(p["n"], p["nkeys"], p["s"], seed=p["seed"], columns=("k",))["k"]
+    q = G.Query([G.BIGINT], [0], [(G.COUNT_STAR, None)], nullable=[0])
+    c, v, t, _ = O.groupby_exact(q, [G.Column(G.BIGINT, k)])
+    keys, counts = c[0], c[1]
+    assert len(keys) == g["groups"] and int(counts.sum()) == g["rows"]
+    assert int(np.bitwise_xor.reduce((keys * np.int64(1000003)) ^ counts)) == g["xor_checksum"]
+    assert sorted(counts.tolist(), reverse=True)[:50] == g["top50_counts"]
+    order = np.lexsort((keys, counts))
+    assert [[int(keys[i]), int(counts[i])] for i in order[-200:]] == g["all_pairs_sorted_head"]
+    # the restated RadixCountAgg gives the same groups and top-k counts
+    pk, pc = O.port_radix_count(k, 8, 0)
+    assert sorted(zip(pk.tolist(), pc.tolist())) == sorted(zip(keys.tolist(), counts.tolist()))
+    _, tc = O.port_radix_count(k, 8, 50)
+    assert tc.tolist() == g["top50_counts"]
+
+
+def test_plain_1m_golden_pandas():
+    g = load("plain_1m.json")
+    q, cols = datagen.plain_workload(1_000_000)
+    rows, t, _ = exact_rows(q, cols)
+    assert sorted(rows) == [tuple(r) for r in g["rows"]]
+
+
+def test_port_equals_exact_where_double_is_exact():
+    for q, cols in (datagen.plain_workload(300_000), datagen.highcard_workload(300_000, 50_000)):
+        exp, t, _ = exact_rows(q, cols)
+        c, v, t2, _ = O.port_groupby(q, cols)
+        got = oracle_rows(c, v, t2)
+        assert_same_groups(got, exp, 1, t, rel=1e-9)
+
+
+def test_port_topn_count_desc():
+    q, cols = datagen.highcard_workload(200_000, 20_000)
+    q.topn = (1, False, 10)
+    c, v, t, _ = O.port_groupby(q, cols)
+    got = sorted(c[1].tolist(), reverse=True)
+    q.topn = None
+    allc, _, _, _ = O.groupby_exact(q, cols)
+    assert got == sorted(allc[1].tolist(), reverse=True)[:10]
+
+
+def test_oracle_topn_tie_break():
+    q = G.Query([G.INTEGER], [0], [(G.COUNT_STAR, None)], nullable=[0])
+    k = np.array([5, 5, 3, 3, 9, 1, 1, 7], np.int32)
+    res = O.OrcResult()
+    import ctypes as C
+    qd, keep = q.desc()
+    views, keep2 = G._views([G.Column(G.INTEGER, k)], False)
+    O.lib().orc_groupby_exact(C.byref(qd), views, len(k), 1, C.byref(res))
+    idx = np.zeros(4, np.uint64)
+    m = O.lib().orc_topn(C.byref(res), 1, 1, 0, 4, idx.ctypes.data)
+    c, v, t = O._to_table(res)
+    O.lib().orc_result_free(C.byref(res))
+    assert [(int(c[0][i]), int(c[1][i])) for i in idx[:m]] == [(1, 2), (3, 2), (5, 2), (7, 1)]
