@@ -225,6 +225,7 @@ typedef struct {
     double ms_count;            /* radix count pass + HyperLogLog */
     double ms_partition;        /* per-partition aggregation + heavy fold */
     double ms_emit;             /* result compaction / TopN */
+    char kernel_shape[32];      /* compiled query shape the row kernels ran, or "dynamic" */
 } gbagg_plan_info;
 
 typedef struct gbagg_handle gbagg_handle;
@@ -259,7 +260,9 @@ void gbagg_destroy(gbagg_handle *h);
  *   "heavy_min"     minimum sample count of a heavy key (1-64)
  *   "part_bits"     radix fanout override, -1 = from the HyperLogLog estimate
  *   "lds_kb"        LDS budget of a per-partition table (8-160)
- *   "fallback_cap"  initial slots of the radix strategy's global fallback table */
+ *   "fallback_cap"  initial slots of the radix strategy's global fallback table
+ *   "static_shapes" 1 = use compile-time specialised row kernels when the query
+ *                   shape matches one (default), 0 = always the generic kernels */
 gbagg_status gbagg_set_param(gbagg_handle *h, const char *name, int64_t value);
 
 /* GPUEngine::HashAggregate (include/slothdb/execution/gpu/gpu_engine.hpp:79-85):

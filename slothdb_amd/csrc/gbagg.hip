@@ -33,6 +33,67 @@ using namespace gbagg;
 
 namespace {
 
+// ---- compiled query shapes -------------------------------------------------
+// Queries whose shape matches one of these run straight-line specialisations
+// of the row kernels; every other query runs the same kernels over DynShape.
+using KAgg = void (*)(const DevQuery *, const DevBatch *, DevTable, uint32_t, uint32_t);
+using KCount = void (*)(const DevCtx *, const DevBatch *, uint32_t, uint64_t);
+using KScatter = void (*)(const DevCtx *, const DevBatch *, DevTable, uint32_t, uint64_t);
+using KPart = void (*)(const DevCtx *, DevTable, uint32_t, uint32_t);
+
+struct ShapeKernels {
+    const char *name;
+    bool dynamic;
+    ShapeSig sig;
+    KAgg agg;
+    KCount count;
+    KScatter scatter;
+    KPart part16, part32;
+};
+
+template <class S>
+ShapeKernels shape_kernels(const char *name) {
+    ShapeKernels k{};
+    k.name = name;
+    if constexpr (S::kStatic) k.sig = S::sig();
+    k.dynamic = !S::kStatic;
+    k.agg = &k_aggregate<S>;
+    k.count = &k_rp_count<S>;
+    k.scatter = &k_rp_scatter<S>;
+    k.part16 = &k_rp_agg<S, 16>;
+    k.part32 = &k_rp_agg<S, 32>;
+    return k;
+}
+
+// slot kinds: 2 bits per slot; accumulator ops / input slots: 4 bits each
+// (input slot 15 = COUNT(*)); predicate: slot | kind << 3 | op << 5.
+// GROUP BY int32 k: COUNT(*), SUM/MIN/MAX(int64 v)                     (plain aggregate)
+using ShPlain = StaticShape<2, 1, 0x4, 0, KM_SINGLE, 0, 0, 4, 0x5420, 0x111F, 0, 0>;
+// GROUP BY int32 k: COUNT(*), SUM(int64 v)                              (mixed-query batches)
+using ShCountSum = StaticShape<2, 1, 0x4, 0, KM_SINGLE, 0, 0, 2, 0x20, 0x1F, 0, 0>;
+// GROUP BY int64 k: COUNT(*), SUM(int64), MIN/MAX(int32), SUM/AVG(double) (high cardinality)
+using ShHighcard = StaticShape<4, 1, 0x85, 0, KM_SINGLE, 0, 0, 5, 0x35420, 0x3221F, 0, 0>;
+// WHERE c < lit GROUP BY int32 a, VARCHAR b: COUNT(*), SUM(int32 c)   (composite key + filter)
+using ShComposite = StaticShape<3, 3, 0xC, 0, KM_PACK32, 0, 1, 2, 0x20, 0x2F, 1, 2>;
+// GROUP BY int64 k: COUNT(*)                                          (RadixCountAgg shape)
+using ShCount64 = StaticShape<1, 1, 0x1, 0, KM_SINGLE, 0, 0, 1, 0x0, 0xF, 0, 0>;
+
+const std::vector<ShapeKernels> &shape_table() {
+    static const std::vector<ShapeKernels> t = {
+        shape_kernels<ShPlain>("plain_i32_i64"),
+        shape_kernels<ShCountSum>("count_sum_i32_i64"),
+        shape_kernels<ShHighcard>("highcard_i64"),
+        shape_kernels<ShComposite>("composite_i32_dict_filter"),
+        shape_kernels<ShCount64>("count_i64"),
+    };
+    return t;
+}
+
+const ShapeKernels &dynamic_kernels() {
+    static const ShapeKernels k = shape_kernels<DynShape>("dynamic");
+    return k;
+}
+
 thread_local std::string g_last_error;
 
 struct ApiError {
@@ -199,6 +260,8 @@ struct gbagg_handle {
     int ucol[MAXU];
     uint32_t umask_all = 0, umask_hll = 0;
     DevQuery dq{};
+    const ShapeKernels *kern = nullptr;
+    bool allow_static = true;
     std::vector<OutCol> outs;
     KeyDict kdict[2];
     bool key_varchar[2] = {false, false};
@@ -213,9 +276,13 @@ struct gbagg_handle {
     std::vector<Batch> batches;
     std::vector<DevBuf> in_set_bufs;
     DevBuf dplan, table, hll, blkcnt, cand_rec, cand_null;
-    DevBuf skeys, scnt, hkeys, hidx, hdense, hfinal, hist, hpart, base, ptotal, pstart, recs;
+    DevBuf ghist, hllm, skeys, scnt, hkeys, hidx, hdense, hfinal, hist, hpart, base, ptotal, pstart, recs;
     std::vector<DevBuf> out_data, out_valid;
     DevPlan *hplan = nullptr;  // pinned
+    DevCtx *hctx = nullptr;    // pinned parameter blocks
+    DevBatch *hbatches = nullptr;
+    size_t hbatch_cap = 0;
+    DevBuf ctx, dbatches;
     uint64_t cap_max = 0;
     uint64_t out_capacity = 0;
     uint32_t cand_cap = 0;
@@ -242,6 +309,8 @@ struct gbagg_handle {
 
     ~gbagg_handle() {
         if (hplan) (void)hipHostFree(hplan);
+        if (hctx) (void)hipHostFree(hctx);
+        if (hbatches) (void)hipHostFree(hbatches);
         for (auto &e : ev)
             if (e) (void)hipEventDestroy(e);
         if (stream) (void)hipStreamDestroy(stream);
@@ -351,6 +420,7 @@ void build_plan(gbagg_handle *h, const gbagg_query_desc *q) {
         h->preds.push_back(p);
     }
     h->dq.npred = (int32_t)q->num_predicates;
+    h->dq.nkp = h->nu;
     h->umask_hll = (1u << h->nu) - 1;  // keys + predicate columns
 
     // aggregates -> accumulators (acc 0 is always COUNT(*))
@@ -462,6 +532,37 @@ void build_plan(gbagg_handle *h, const gbagg_query_desc *q) {
     }
 }
 
+ShapeSig plan_sig(const gbagg_handle *h) {
+    ShapeSig g{};
+    const DevQuery &q = h->dq;
+    g.nu = q.nu;
+    g.nkp = q.nkp;
+    for (int u = 0; u < q.nu; u++) {
+        g.kinds |= (uint32_t)q.ukind[u] << (2 * u);
+        if (h->nullable[h->ucol[u]]) g.nulls |= 1u << u;
+    }
+    g.km = q.key_mode;
+    g.ku0 = q.key_mode != KM_NONE ? q.key_u[0] : 0;
+    g.ku1 = q.key_mode == KM_PACK32 ? q.key_u[1] : 0;
+    g.na = q.nacc;
+    for (int a = 0; a < q.nacc; a++) {
+        g.ops |= (uint64_t)q.acc_op[a] << (4 * a);
+        g.aus |= (uint64_t)(q.acc_u[a] < 0 ? 15 : q.acc_u[a]) << (4 * a);
+    }
+    g.np = q.npred;
+    for (int p = 0; p < q.npred; p++)
+        g.pred |= (uint32_t)(q.pred[p].u | (q.pred[p].kind << 3) | (q.pred[p].op << 5)) << (9 * p);
+    return g;
+}
+
+void pick_kernels(gbagg_handle *h) {
+    h->kern = &dynamic_kernels();
+    if (!h->allow_static || h->dq.npred > 2) return;
+    const ShapeSig g = plan_sig(h);
+    for (const auto &k : shape_table())
+        if (k.sig == g) { h->kern = &k; return; }
+}
+
 void ensure_runtime(gbagg_handle *h) {
     if (h->stream) return;
     device_info();
@@ -470,6 +571,9 @@ void ensure_runtime(gbagg_handle *h) {
     HIPCHK(hipHostMalloc((void **)&h->hplan, sizeof(DevPlan), hipHostMallocDefault));
     std::memset(h->hplan, 0, sizeof(DevPlan));
     h->dplan.ensure(sizeof(DevPlan));
+    HIPCHK(hipHostMalloc((void **)&h->hctx, sizeof(DevCtx), hipHostMallocDefault));
+    std::memset(h->hctx, 0, sizeof(DevCtx));
+    h->ctx.ensure(sizeof(DevCtx));
     h->in_set_bufs.resize(h->preds.size());
     for (size_t i = 0; i < h->preds.size(); i++) {
         if (h->in_sets[i].empty()) continue;
@@ -654,8 +758,18 @@ DevTable make_table(gbagg_handle *h) {
     return T;
 }
 
-DevOut make_out(gbagg_handle *h) {
-    DevOut o{};
+// Device addresses of the parameter blocks (pointer arithmetic only).
+DevCtx *dctx(gbagg_handle *h) { return h->ctx.as<DevCtx>(); }
+const DevQuery *d_q(gbagg_handle *h) { return &dctx(h)->q; }
+const DevOut *d_o(gbagg_handle *h) { return &dctx(h)->o; }
+const DevTopn *d_t(gbagg_handle *h) { return &dctx(h)->t; }
+const RadixParams *d_rp(gbagg_handle *h) { return &dctx(h)->rp; }
+const DevBatch *d_b(gbagg_handle *h, size_t i) { return h->dbatches.as<DevBatch>() + i; }
+
+// Fills the output / TopN blocks of the host context.
+void fill_out_topn(gbagg_handle *h) {
+    DevOut &o = h->hctx->o;
+    o = DevOut{};
     o.ncols = (int32_t)h->outs.size();
     o.key_mode = h->dq.key_mode;
     o.capacity = h->out_capacity;
@@ -667,17 +781,33 @@ DevOut make_out(gbagg_handle *h) {
         o.cnt_acc[c] = h->outs[c].cnt_acc;
         o.key_type[c] = h->outs[c].key_type;
     }
-    return o;
-}
-
-DevTopn make_topn(gbagg_handle *h) {
-    DevTopn t = h->tspec;
+    DevTopn &t = h->hctx->t;
+    t = h->tspec;
     t.enabled = h->topn_device ? 1 : 0;
     t.rec_words = 1 + h->dq.nacc;
     t.cand_rec = h->cand_rec.as<uint64_t>();
     t.cand_null = h->cand_null.as<uint8_t>();
     t.cand_cap = h->cand_cap;
-    return t;
+}
+
+// The host context is pinned; callers modify it only after the stream has
+// consumed the previous upload (every caller syncs through read_plan first).
+void upload_ctx(gbagg_handle *h) {
+    HIPCHK(hipMemcpyAsync(h->ctx.ptr, h->hctx, sizeof(DevCtx), hipMemcpyHostToDevice, h->stream));
+}
+
+void upload_batches(gbagg_handle *h) {
+    size_t nb = std::max<size_t>(1, h->batches.size());
+    if (h->hbatch_cap < nb) {
+        if (h->hbatches) HIPCHK(hipHostFree(h->hbatches));
+        h->hbatches = nullptr;
+        HIPCHK(hipHostMalloc((void **)&h->hbatches, nb * sizeof(DevBatch), hipHostMallocDefault));
+        h->hbatch_cap = nb;
+    }
+    for (size_t i = 0; i < h->batches.size(); i++) h->hbatches[i] = h->batches[i].db;
+    h->dbatches.ensure(nb * sizeof(DevBatch));
+    HIPCHK(hipMemcpyAsync(h->dbatches.ptr, h->hbatches, h->batches.size() * sizeof(DevBatch),
+                          hipMemcpyHostToDevice, h->stream));
 }
 
 // Resets the run counters of the device plan block; the sample / heavy-hitter
@@ -706,26 +836,25 @@ void emit_table(gbagg_handle *h, const DevTable &T, uint64_t cap) {
     const DeviceInfo &di = device_info();
     hipStream_t s = h->stream;
     if (h->topn_device) {
-        DevTopn t = make_topn(h);
         uint32_t grid = (uint32_t)std::min<uint64_t>((cap + 2 + 4095) / 4096, (uint64_t)di.cus * 4);
-        if (t.k <= 16) hipLaunchKernelGGL(k_topn_table<16>, dim3(grid), dim3(256), 0, s, T, t);
-        else hipLaunchKernelGGL(k_topn_table<32>, dim3(grid), dim3(256), 0, s, T, t);
+        if (h->topn_k <= 16) hipLaunchKernelGGL(k_topn_table<16>, dim3(grid), dim3(256), 0, s, T, d_t(h));
+        else hipLaunchKernelGGL(k_topn_table<32>, dim3(grid), dim3(256), 0, s, T, d_t(h));
     } else {
         uint64_t nblk = (cap + 2 + COMPACT_CHUNK - 1) / COMPACT_CHUNK;
         uint32_t grid = (uint32_t)std::min<uint64_t>(nblk, (uint64_t)di.cus * 4);
         uint64_t *bases = (uint64_t *)(h->blkcnt.as<uint8_t>() + ((nblk * 4 + 15) / 16) * 16);
         hipLaunchKernelGGL(k_occ_count, dim3(grid), dim3(256), 0, s, T, h->blkcnt.as<uint32_t>());
         hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, s, T, h->blkcnt.as<uint32_t>(), bases);
-        hipLaunchKernelGGL(k_compact_write, dim3(grid), dim3(256), 0, s, T, (const uint64_t *)bases,
-                           make_out(h));
+        hipLaunchKernelGGL(k_compact_write, dim3(grid), dim3(256), 0, s, T, (const uint64_t *)bases, d_o(h));
     }
 }
 
 void topn_final(gbagg_handle *h) {
     if (!h->topn_device) return;
-    DevTopn t = make_topn(h);
-    if (t.k <= 16) hipLaunchKernelGGL(k_topn_final<16>, dim3(1), dim3(1024), 0, h->stream, t, h->dplan.as<DevPlan>(), make_out(h));
-    else hipLaunchKernelGGL(k_topn_final<32>, dim3(1), dim3(1024), 0, h->stream, t, h->dplan.as<DevPlan>(), make_out(h));
+    if (h->topn_k <= 16)
+        hipLaunchKernelGGL(k_topn_final<16>, dim3(1), dim3(1024), 0, h->stream, d_t(h), h->dplan.as<DevPlan>(), d_o(h));
+    else
+        hipLaunchKernelGGL(k_topn_final<32>, dim3(1), dim3(1024), 0, h->stream, d_t(h), h->dplan.as<DevPlan>(), d_o(h));
 }
 
 void ensure_table(gbagg_handle *h, uint64_t cap) {
@@ -735,9 +864,10 @@ void ensure_table(gbagg_handle *h, uint64_t cap) {
 }
 
 // Heavy-hitter sample over all batches (strided), then the heavy table.
-void run_sample(gbagg_handle *h, RadixParams &rp) {
+void run_sample(gbagg_handle *h) {
     const DeviceInfo &di = device_info();
     hipStream_t s = h->stream;
+    RadixParams &rp = h->hctx->rp;
     const uint32_t S_TOTAL = 131072;
     rp.scap = 1u << 18;
     rp.hmax = h->heavy_max;
@@ -745,7 +875,7 @@ void run_sample(gbagg_handle *h, RadixParams &rp) {
     rp.heavy_min = h->heavy_min;
     // the heavy table must fit the scatter kernel's LDS next to its cursors
     // (8192 partitions at most) and the count kernel's LDS
-    const size_t lds_max = device_info().lds_per_block;
+    const size_t lds_max = di.lds_per_block;
     auto fits = [&] {
         size_t k3 = (size_t)rp.hslots * 12 + (size_t)h->dq.nacc * rp.hmax * 8 + 8192 * 4;
         size_t k1 = (size_t)rp.hslots * 8 + 8192 * 4 + HLL_M * 4;
@@ -767,18 +897,25 @@ void run_sample(gbagg_handle *h, RadixParams &rp) {
     rp.hidx = h->hidx.as<uint32_t>();
     rp.hdense = h->hdense.as<uint64_t>();
     rp.hfinal = h->hfinal.as<uint64_t>();
-    hipLaunchKernelGGL(k_sample_init, dim3(256), dim3(256), 0, s, rp);
-    for (auto &b : h->batches) {
+    upload_ctx(h);
+    hipLaunchKernelGGL(k_sample_init, dim3(256), dim3(256), 0, s, d_rp(h));
+    for (size_t i = 0; i < h->batches.size(); i++) {
+        auto &b = h->batches[i];
         if (!b.nrows) continue;
         uint64_t want = std::max<uint64_t>(1, (uint64_t)((double)S_TOTAL * b.nrows / std::max<uint64_t>(1, h->rows_total)));
         want = std::min<uint64_t>(want, b.nrows);
-        rp.sample_n = (uint32_t)want;
-        rp.sample_stride = std::max<uint64_t>(1, b.nrows / want);
+        uint64_t stride = std::max<uint64_t>(1, b.nrows / want);
         uint32_t grid = (uint32_t)std::min<uint64_t>((want + 255) / 256, (uint64_t)di.cus * 2);
-        hipLaunchKernelGGL(k_sample, dim3(grid), dim3(256), 0, s, h->dq, b.db, rp, h->umask_hll,
-                           h->dplan.as<DevPlan>());
+        hipLaunchKernelGGL(k_sample, dim3(grid), dim3(256), 0, s, (const DevCtx *)dctx(h), d_b(h, i),
+                           (uint32_t)want, stride, h->dplan.as<DevPlan>());
     }
-    hipLaunchKernelGGL(k_heavy_select, dim3(1), dim3(1024), 0, s, rp, h->dplan.as<DevPlan>());
+    h->ghist.ensure(65 * 4);
+    HIPCHK(hipMemsetAsync(h->ghist.ptr, 0, 65 * 4, s));
+    const uint32_t sgrid = (uint32_t)std::min<uint64_t>((rp.scap + 255) / 256, (uint64_t)di.cus * 2);
+    hipLaunchKernelGGL(k_heavy_hist, dim3(sgrid), dim3(256), 0, s, d_rp(h), h->dplan.as<DevPlan>(), h->ghist.as<uint32_t>());
+    hipLaunchKernelGGL(k_heavy_threshold, dim3(1), dim3(64), 0, s, d_rp(h), h->dplan.as<DevPlan>(),
+                       (const uint32_t *)h->ghist.as<uint32_t>());
+    hipLaunchKernelGGL(k_heavy_insert, dim3(sgrid), dim3(256), 0, s, d_rp(h), h->dplan.as<DevPlan>());
 }
 
 void run_lds_hash(gbagg_handle *h, Timer &tm, bool global_only) {
@@ -797,7 +934,7 @@ void run_lds_hash(gbagg_handle *h, Timer &tm, bool global_only) {
         DevTable T = make_table(h);
         upload_plan(h, cap);
         uint32_t grid = (uint32_t)std::min<uint64_t>((cap + 2 + 255) / 256, (uint64_t)di.cus * 8);
-        hipLaunchKernelGGL(k_table_init, dim3(grid), dim3(256), 0, s, T, h->dq);
+        hipLaunchKernelGGL(k_table_init, dim3(grid), dim3(256), 0, s, T, d_q(h));
         tm.mark();  // aggregate start
         size_t lds_bytes = 0;
         uint32_t S = 0;
@@ -805,10 +942,11 @@ void run_lds_hash(gbagg_handle *h, Timer &tm, bool global_only) {
             size_t budget = std::min<size_t>(di.lds_per_cu / 2, di.lds_per_block);
             S = lds_slots_for(budget, h->dq.nacc, 4096);
             lds_bytes = lds_table_bytes(S, h->dq.nacc);
-            allow_lds(k_aggregate, lds_bytes);
+            allow_lds(h->kern->agg, lds_bytes);
         }
         h->lds_slots = S;
-        for (auto &b : h->batches) {
+        for (size_t i = 0; i < h->batches.size(); i++) {
+            auto &b = h->batches[i];
             if (!b.nrows) continue;
             uint64_t tiles = (b.nrows + (uint64_t)AGG_BLOCK * AGG_ROWS - 1) / ((uint64_t)AGG_BLOCK * AGG_ROWS);
             if (!global_only) {
@@ -817,13 +955,12 @@ void run_lds_hash(gbagg_handle *h, Timer &tm, bool global_only) {
                 uint64_t by_rows = std::max<uint64_t>(1, b.nrows / 16384);
                 uint32_t g = (uint32_t)std::min<uint64_t>({tiles, (uint64_t)di.cus * per_cu, by_rows});
                 h->agg_grid = g;
-                hipLaunchKernelGGL(k_aggregate, dim3(g), dim3(AGG_BLOCK), lds_bytes, s, h->dq, b.db, T, S,
-                                   S * 3 / 4, h->umask_all);
+                hipLaunchKernelGGL(h->kern->agg, dim3(g), dim3(AGG_BLOCK), lds_bytes, s, d_q(h), d_b(h, i), T, S,
+                                   S * 3 / 4);
             } else {
                 uint32_t g = (uint32_t)std::min<uint64_t>(tiles, (uint64_t)di.cus * 8);
                 h->agg_grid = g;
-                hipLaunchKernelGGL(k_aggregate_global, dim3(g), dim3(AGG_BLOCK), 0, s, h->dq, b.db, T,
-                                   h->umask_all);
+                hipLaunchKernelGGL(k_aggregate_global, dim3(g), dim3(AGG_BLOCK), 0, s, d_q(h), d_b(h, i), T);
             }
         }
         HIPCHK(hipGetLastError());
@@ -840,10 +977,11 @@ void run_lds_hash(gbagg_handle *h, Timer &tm, bool global_only) {
     }
 }
 
-void run_radix(gbagg_handle *h, Timer &tm, RadixParams rp) {
+void run_radix(gbagg_handle *h, Timer &tm) {
     const DeviceInfo &di = device_info();
     hipStream_t s = h->stream;
     const int nacc = h->dq.nacc;
+    RadixParams &rp = h->hctx->rp;
     rp.bucket_bits = 13;
     const uint32_t nb = 1u << rp.bucket_bits;
     // record layout: key, then each loaded slot an accumulator reads, then validity
@@ -893,19 +1031,21 @@ void run_radix(gbagg_handle *h, Timer &tm, RadixParams rp) {
         ensure_table(h, cap);
         DevTable T = make_table(h);
         upload_plan(h, cap);
+        upload_ctx(h);
         uint32_t grid = (uint32_t)std::min<uint64_t>((cap + 2 + 255) / 256, (uint64_t)di.cus * 8);
-        hipLaunchKernelGGL(k_table_init, dim3(grid), dim3(256), 0, s, T, h->dq);
+        hipLaunchKernelGGL(k_table_init, dim3(grid), dim3(256), 0, s, T, d_q(h));
         tm.mark();  // count start
-        allow_lds(k_rp_count, k1_lds);
+        allow_lds(h->kern->count, k1_lds);
         for (size_t i = 0; i < h->batches.size(); i++) {
-            auto &b = h->batches[i];
-            if (!b.nrows) continue;
-            RadixParams r = rp;
-            r.wg_offset = wg_off[i];
-            r.rows_per_wg = rows_per[i];
-            hipLaunchKernelGGL(k_rp_count, dim3(wg_n[i]), dim3(RP_BLOCK), k1_lds, s, h->dq, b.db, r, h->umask_hll);
+            if (!h->batches[i].nrows) continue;
+            hipLaunchKernelGGL(h->kern->count, dim3(wg_n[i]), dim3(RP_BLOCK), k1_lds, s, (const DevCtx *)dctx(h),
+                               d_b(h, i), wg_off[i], rows_per[i]);
         }
-        hipLaunchKernelGGL(k_hll_finish, dim3(1), dim3(1024), 0, s, (const uint8_t *)rp.hll, W, T.plan);
+        h->hllm.ensure(HLL_M);
+        hipLaunchKernelGGL(k_hll_merge, dim3(HLL_M / 256), dim3(256), 0, s, (const uint8_t *)rp.hll, W,
+                           h->hllm.as<uint8_t>());
+        hipLaunchKernelGGL(k_hll_finish, dim3(1), dim3(1024), 0, s, (const uint8_t *)h->hllm.as<uint8_t>(), 1u,
+                           T.plan);
         tm.mark();  // count end
         read_plan(h);  // HLL estimate and heavy count pick the fanout
         const uint64_t est = h->hplan->est_groups;
@@ -928,35 +1068,31 @@ void run_radix(gbagg_handle *h, Timer &tm, RadixParams rp) {
         rp.ptotal = h->ptotal.as<uint32_t>();
         rp.pstart = h->pstart.as<uint64_t>();
         rp.recs = h->recs.as<uint64_t>();
-        hipLaunchKernelGGL(k_rp_scan_w, dim3((P + 255) / 256), dim3(256), 0, s, rp);
-        hipLaunchKernelGGL(k_rp_scan_p, dim3(1), dim3(1024), 0, s, rp, T.plan);
+        upload_ctx(h);
+        hipLaunchKernelGGL(k_rp_scan_w, dim3((P + 255) / 256), dim3(256), 0, s, d_rp(h));
+        hipLaunchKernelGGL(k_rp_scan_p, dim3(1), dim3(1024), 0, s, d_rp(h), T.plan);
         const size_t k3_lds = k3_fixed + (size_t)P * 4;
-        allow_lds(k_rp_scatter, k3_lds);
+        allow_lds(h->kern->scatter, k3_lds);
         tm.mark();  // scatter start
         for (size_t i = 0; i < h->batches.size(); i++) {
-            auto &b = h->batches[i];
-            if (!b.nrows) continue;
-            RadixParams r = rp;
-            r.wg_offset = wg_off[i];
-            r.rows_per_wg = rows_per[i];
-            hipLaunchKernelGGL(k_rp_scatter, dim3(wg_n[i]), dim3(RP_BLOCK), k3_lds, s, h->dq, b.db, r, T,
-                               h->umask_all);
+            if (!h->batches[i].nrows) continue;
+            hipLaunchKernelGGL(h->kern->scatter, dim3(wg_n[i]), dim3(RP_BLOCK), k3_lds, s, (const DevCtx *)dctx(h),
+                               d_b(h, i), T, wg_off[i], rows_per[i]);
         }
         h->agg_grid = W;
         tm.mark();  // scatter end
         const size_t k4_lds = lds_table_bytes(S, nacc);
-        DevOut o = make_out(h);
-        DevTopn t = make_topn(h);
-        if (t.k <= 16) {
-            allow_lds(k_rp_agg<16>, k4_lds);
-            hipLaunchKernelGGL(k_rp_agg<16>, dim3(P), dim3(512), k4_lds, s, h->dq, rp, T, o, t, S, S * 7 / 8);
-            hipLaunchKernelGGL(k_heavy_reduce<16>, dim3(t.enabled ? 1 : std::max<uint32_t>(1, (rp.hmax + 255) / 256)), dim3(256), 0, s,
-                               h->dq, rp, T, o, t);
+        const uint32_t hgrid = h->topn_device ? 1 : std::max<uint32_t>(1, (rp.hmax + 255) / 256);
+        const dim3 fgrid(std::max<uint32_t>(1, (rp.hmax + 63) / 64), (uint32_t)nacc);
+        if (rp.hmax) hipLaunchKernelGGL(k_heavy_fold, fgrid, dim3(256), 0, s, (const DevCtx *)dctx(h), T);
+        if (h->topn_k <= 16) {
+            allow_lds(h->kern->part16, k4_lds);
+            hipLaunchKernelGGL(h->kern->part16, dim3(P), dim3(RA_BLOCK), k4_lds, s, (const DevCtx *)dctx(h), T, S, S * 7 / 8);
+            if (rp.hmax) hipLaunchKernelGGL(k_heavy_emit<16>, dim3(hgrid), dim3(256), 0, s, (const DevCtx *)dctx(h), T);
         } else {
-            allow_lds(k_rp_agg<32>, k4_lds);
-            hipLaunchKernelGGL(k_rp_agg<32>, dim3(P), dim3(512), k4_lds, s, h->dq, rp, T, o, t, S, S * 7 / 8);
-            hipLaunchKernelGGL(k_heavy_reduce<32>, dim3(t.enabled ? 1 : std::max<uint32_t>(1, (rp.hmax + 255) / 256)), dim3(256), 0, s,
-                               h->dq, rp, T, o, t);
+            allow_lds(h->kern->part32, k4_lds);
+            hipLaunchKernelGGL(h->kern->part32, dim3(P), dim3(RA_BLOCK), k4_lds, s, (const DevCtx *)dctx(h), T, S, S * 7 / 8);
+            if (rp.hmax) hipLaunchKernelGGL(k_heavy_emit<32>, dim3(hgrid), dim3(256), 0, s, (const DevCtx *)dctx(h), T);
         }
         HIPCHK(hipGetLastError());
         tm.mark();  // partition aggregate end
@@ -990,7 +1126,6 @@ void finalize(gbagg_handle *h) {
     h->radix_bits = 0;
     Timer tm{h, {}};
     tm.mark();  // start
-    RadixParams rp{};
     gbagg_strategy st = h->strategy;
     if (h->dq.key_mode == KM_NONE) st = GBAGG_STRATEGY_GLOBAL_HASH;
     if (h->topn_device) {
@@ -1000,12 +1135,17 @@ void finalize(gbagg_handle *h) {
         h->cand_null.ensure(h->cand_cap);
     }
     std::memset(h->hplan, 0, sizeof(DevPlan));
+    std::memset(h->hctx, 0, sizeof(DevCtx));
+    h->hctx->q = h->dq;
+    fill_out_topn(h);
+    upload_batches(h);
+    upload_ctx(h);
     upload_plan(h, 1024);
     if (st == GBAGG_STRATEGY_GLOBAL_HASH && h->dq.key_mode == KM_NONE) {
         tm.mark();
         run_lds_hash(h, tm, true);
     } else {
-        run_sample(h, rp);
+        run_sample(h);
         tm.mark();  // sample end
         read_plan(h);
         if (st == GBAGG_STRATEGY_AUTO || st == GBAGG_STRATEGY_DIRECT) {
@@ -1013,7 +1153,7 @@ void finalize(gbagg_handle *h) {
             bool low_card = p.sample_distinct <= 4096 && p.sample_f1 * 8 <= p.sample_distinct + 8;
             st = (low_card || h->rows_total <= 65536) ? GBAGG_STRATEGY_LDS_HASH : GBAGG_STRATEGY_RADIX;
         }
-        if (st == GBAGG_STRATEGY_RADIX) run_radix(h, tm, rp);
+        if (st == GBAGG_STRATEGY_RADIX) run_radix(h, tm);
         else run_lds_hash(h, tm, st == GBAGG_STRATEGY_GLOBAL_HASH);
     }
     h->used_strategy = st;
@@ -1191,6 +1331,7 @@ gbagg_status gbagg_plan_create(const gbagg_query_desc *query, gbagg_handle **out
         *out = nullptr;
         auto h = std::make_unique<gbagg_handle>();
         build_plan(h.get(), query);
+        pick_kernels(h.get());
         *out = h.release();
     });
 }
@@ -1266,6 +1407,7 @@ gbagg_status gbagg_plan_info_get(gbagg_handle *h, gbagg_plan_info *out) {
         out->ms_count = h->ms_count;
         out->ms_partition = h->ms_partition;
         out->ms_emit = h->ms_emit;
+        std::snprintf(out->kernel_shape, sizeof(out->kernel_shape), "%s", h->kern ? h->kern->name : "");
     });
 }
 
@@ -1282,6 +1424,7 @@ gbagg_status gbagg_set_param(gbagg_handle *h, const char *name, int64_t value) {
         else if (n == "part_bits") { in(-1, 13); h->force_part_bits = (int32_t)value; }
         else if (n == "lds_kb") { in(8, 160); h->k4_lds_kb = (uint32_t)value; }
         else if (n == "strategy") { in(0, 4); h->strategy = (gbagg_strategy)value; }
+        else if (n == "static_shapes") { in(0, 1); h->allow_static = value != 0; pick_kernels(h); }
         else fail(GBAGG_INVALID, "unknown parameter " + n);
     });
 }
@@ -1314,6 +1457,7 @@ gbagg_status gbagg_hash_aggregate_i32(const int32_t *keys, const int32_t *values
         q.topn_col = -1;
         auto hp = std::make_unique<gbagg_handle>();
         build_plan(hp.get(), &q);
+        pick_kernels(hp.get());
         h = hp.release();
         gbagg_column_view cols[2] = {};
         cols[0].type = GBAGG_TYPE_INTEGER;

@@ -49,6 +49,7 @@ struct DevPred {
 // Per-query constants.
 struct DevQuery {
     int32_t nu;
+    int32_t nkp;               // leading slots holding keys and predicate columns
     int32_t ukind[MAXU];
     int32_t key_mode;
     int32_t key_u[2];
@@ -130,11 +131,17 @@ struct Row {
     uint32_t vm;  // bit u set: column slot u is valid in this row
 };
 
+// row.v[u] for a runtime (wave-uniform) u. The mask passes through an empty
+// register constraint so the compiler cannot rebuild the select chain into a
+// load through a computed address, which would move the whole Row to scratch.
 __device__ __forceinline__ uint64_t sel(const Row &row, int u) {
-    uint64_t x = row.v[0];
+    uint64_t x = 0;
 #pragma unroll
-    for (int i = 1; i < MAXU; i++)
-        if (u == i) x = row.v[i];
+    for (int i = 0; i < MAXU; i++) {
+        uint64_t m = 0ULL - (uint64_t)(u == i);
+        asm volatile("" : "+v"(m));
+        x |= row.v[i] & m;
+    }
     return x;
 }
 
@@ -294,6 +301,212 @@ __device__ __forceinline__ bool acc_input(const DevQuery &q, int a, const Row &r
 // Whether a partial accumulator value changes the merged result.
 __device__ __forceinline__ bool acc_nontrivial(int op, uint64_t x) {
     return x != acc_identity(op);
+}
+
+// ------------------------------------------------------------------------
+// Query shapes. The per-row code is written once against a shape: DynShape
+// reads the plan at run time (any query); StaticShape fixes the column kinds,
+// key layout, predicates and accumulator ops at compile time, so the row loop
+// compiles to straight-line code for the query shapes the host registers.
+// ------------------------------------------------------------------------
+struct ShapeSig {
+    int nu;          // loaded slots
+    int nkp;         // leading slots holding keys and predicate columns
+    uint32_t kinds;  // ColKind of slot u at bits 2u..2u+1
+    uint32_t nulls;  // bit u: slot u may carry a validity array
+    int km, ku0, ku1;
+    int na;
+    uint64_t ops;    // AccOp of accumulator a at bits 4a..4a+3
+    uint64_t aus;    // input slot of accumulator a at bits 4a..4a+3 (15 = none)
+    int np;
+    uint32_t pred;   // predicate p: slot (3 bits), kind (2 bits), op (4 bits) at bit 9p
+    bool operator==(const ShapeSig &o) const {
+        return nu == o.nu && nkp == o.nkp && kinds == o.kinds && nulls == o.nulls && km == o.km &&
+               ku0 == o.ku0 && ku1 == o.ku1 && na == o.na && ops == o.ops && aus == o.aus &&
+               np == o.np && pred == o.pred;
+    }
+};
+
+struct DynShape {
+    static constexpr bool kStatic = false;
+    const DevQuery &q;
+    __device__ explicit DynShape(const DevQuery &q_) : q(q_) {}
+    __device__ int nu() const { return q.nu; }
+    __device__ int nkp() const { return q.nkp; }
+    __device__ int kind(int u) const { return q.ukind[u]; }
+    __device__ bool nullable(int) const { return true; }
+    __device__ int key_mode() const { return q.key_mode; }
+    __device__ int key_u(int i) const { return q.key_u[i]; }
+    __device__ int npred() const { return q.npred; }
+    __device__ int pred_u(int p) const { return q.pred[p].u; }
+    __device__ int pred_kind(int p) const { return q.pred[p].kind; }
+    __device__ int pred_op(int p) const { return q.pred[p].op; }
+    __device__ int nacc() const { return q.nacc; }
+    __device__ int op(int a) const { return q.acc_op[a]; }
+    __device__ int au(int a) const { return q.acc_u[a]; }
+};
+
+template <int NU, int NKP, uint32_t KINDS, uint32_t NULLS, int KM, int KU0, int KU1, int NA, uint64_t OPS,
+          uint64_t AUS, int NP, uint32_t PRED>
+struct StaticShape {
+    static constexpr bool kStatic = true;
+    const DevQuery &q;
+    __device__ explicit StaticShape(const DevQuery &q_) : q(q_) {}
+    __host__ __device__ static constexpr int nu() { return NU; }
+    __host__ __device__ static constexpr int nkp() { return NKP; }
+    __host__ __device__ static constexpr int kind(int u) { return (int)((KINDS >> (2 * u)) & 3u); }
+    __host__ __device__ static constexpr bool nullable(int u) { return (NULLS >> u) & 1u; }
+    __host__ __device__ static constexpr int key_mode() { return KM; }
+    __host__ __device__ static constexpr int key_u(int i) { return i == 0 ? KU0 : KU1; }
+    __host__ __device__ static constexpr int npred() { return NP; }
+    __host__ __device__ static constexpr int pred_u(int p) { return (int)((PRED >> (9 * p)) & 7u); }
+    __host__ __device__ static constexpr int pred_kind(int p) { return (int)((PRED >> (9 * p + 3)) & 3u); }
+    __host__ __device__ static constexpr int pred_op(int p) { return (int)((PRED >> (9 * p + 5)) & 15u); }
+    __host__ __device__ static constexpr int nacc() { return NA; }
+    __host__ __device__ static constexpr int op(int a) { return (int)((OPS >> (4 * a)) & 15u); }
+    __host__ __device__ static constexpr int au(int a) {
+        return ((AUS >> (4 * a)) & 15u) == 15u ? -1 : (int)((AUS >> (4 * a)) & 15u);
+    }
+    // radix record word of slot u (key at word 0; inputs in first-use order)
+    __host__ __device__ static constexpr int rec_pos(int u) {
+        int w = 1;
+        for (int a = 0; a < NA; a++) {
+            int x = au(a);
+            if (x < 0) continue;
+            bool seen = false;
+            for (int b = 0; b < a; b++) seen = seen || au(b) == x;
+            if (seen) continue;
+            if (x == u) return w;
+            w++;
+        }
+        return 0;
+    }
+    __host__ __device__ static constexpr int rec_inputs() {
+        int w = 0;
+        for (int u = 0; u < 8; u++) w += rec_pos(u) ? 1 : 0;
+        return w;
+    }
+    __host__ __device__ static constexpr bool any_nullable_input() {
+        bool n = false;
+        for (int a = 0; a < NA; a++) n = n || (au(a) >= 0 && nullable(au(a)));
+        return n;
+    }
+    static constexpr ShapeSig sig() {
+        return ShapeSig{NU, NKP, KINDS, NULLS, KM, KU0, KU1, NA, OPS, AUS, NP, PRED};
+    }
+};
+
+// row.v[u]: a plain register for static shapes (u is a compile-time constant).
+template <class S>
+__device__ __forceinline__ uint64_t sel_s(const Row &row, int u) {
+    if constexpr (S::kStatic) return row.v[u];
+    else return sel(row, u);
+}
+
+// Loads the first `nload` slots of row r.
+template <class S>
+__device__ __forceinline__ void load_row_s(const S &sh, const DevBatch &b, uint64_t r, int nload, Row &row) {
+    row.vm = 0;
+#pragma unroll
+    for (int u = 0; u < MAXU; u++) {
+        row.v[u] = 0;
+        if (u < nload) {
+            uint64_t x;
+            switch (sh.kind(u)) {
+            case CK_I32: x = (uint64_t)(int64_t)((const int32_t *)b.data[u])[r]; break;
+            case CK_U32: x = (uint64_t)((const uint32_t *)b.data[u])[r]; break;
+            default: x = ((const uint64_t *)b.data[u])[r]; break;
+            }
+            row.v[u] = x;
+            uint32_t ok = 1u;
+            if (sh.nullable(u) && b.valid[u]) ok = b.valid[u][r] != 0;
+            row.vm |= ok << u;
+        }
+    }
+}
+
+template <class S>
+__device__ __forceinline__ bool eval_preds_s(const S &sh, const DevQuery &q, const DevBatch &b, const Row &row) {
+#pragma unroll
+    for (int p = 0; p < MAX_PRED; p++) {
+        if (p >= sh.npred()) break;
+        const int u = sh.pred_u(p);
+        if (sh.nullable(u) && !((row.vm >> u) & 1)) return false;
+        const uint64_t x = sel_s<S>(row, u);
+        bool ok;
+        switch (sh.pred_kind(p)) {
+        case PK_INT: ok = cmp_i64((int64_t)x, q.pred[p].ival, sh.pred_op(p)); break;
+        case PK_F64: ok = cmp_f64(__longlong_as_double((long long)x), q.pred[p].dval, sh.pred_op(p)); break;
+        case PK_IN: {
+            const int64_t *st = (const int64_t *)b.pred_ptr[p];
+            ok = false;
+            for (uint32_t i = 0; i < b.pred_count[p]; i++) ok |= (st[i] == (int64_t)x);
+            break;
+        }
+        default: {
+            uint32_t c = (uint32_t)x;
+            ok = c < b.pred_count[p] && ((const uint8_t *)b.pred_ptr[p])[c] != 0;
+            break;
+        }
+        }
+        if (!ok) return false;
+    }
+    return true;
+}
+
+template <class S>
+__device__ __forceinline__ int make_key_s(const S &sh, const DevQuery &q, const DevBatch &b, const Row &row,
+                                          uint64_t &w) {
+    w = 0;
+    if (sh.key_mode() == KM_NONE) return KEY_OK;
+    const int u0 = sh.key_u(0);
+    uint64_t c0 = sel_s<S>(row, u0);
+    if (sh.nullable(u0) && !((row.vm >> u0) & 1)) return q.drop_null_keys ? KEY_DROP : KEY_NULL;
+    if (sh.kind(u0) == CK_U32 && (b.remap[0] || b.key_dict_size[0])) {
+        uint32_t c = (uint32_t)c0;
+        if (c >= b.key_dict_size[0]) return KEY_DROP;
+        if (b.remap[0]) c0 = b.remap[0][c];
+    }
+    if (sh.key_mode() == KM_SINGLE) {
+        w = c0;
+        return KEY_OK;
+    }
+    const int u1 = sh.key_u(1);
+    uint64_t c1 = sel_s<S>(row, u1);
+    if (sh.nullable(u1) && !((row.vm >> u1) & 1)) return KEY_DROP;
+    if (sh.kind(u1) == CK_U32 && (b.remap[1] || b.key_dict_size[1])) {
+        uint32_t c = (uint32_t)c1;
+        if (c >= b.key_dict_size[1]) return KEY_DROP;
+        if (b.remap[1]) c1 = b.remap[1][c];
+    }
+    w = (c0 << 32) | (c1 & 0xFFFFFFFFULL);
+    return KEY_OK;
+}
+
+template <class S>
+__device__ __forceinline__ bool acc_input_s(const S &sh, int a, const Row &row, uint64_t &x) {
+    const int op = sh.op(a), u = sh.au(a);
+    if (op == ACC_CNT_STAR) { x = 1; return true; }
+    if (sh.nullable(u) && !((row.vm >> u) & 1)) return false;
+    const uint64_t v = sel_s<S>(row, u);
+    switch (op) {
+    case ACC_CNT_VALID: x = 1; break;
+    case ACC_MIN_F:
+    case ACC_MAX_F: x = f64_ord(v); break;
+    default: x = v; break;
+    }
+    return true;
+}
+
+// acc a of one row into dst[a * stride] with atomics of the given scope.
+template <int SCOPE, class S>
+__device__ __forceinline__ void row_update_s(const S &sh, const Row &row, uint64_t *dst, uint64_t stride) {
+#pragma unroll
+    for (int a = 0; a < MAX_ACC; a++) {
+        if (a >= sh.nacc()) break;
+        uint64_t x;
+        if (acc_input_s(sh, a, row, x)) acc_apply<SCOPE>(dst + (uint64_t)a * stride, sh.op(a), x);
+    }
 }
 
 }  // namespace gbagg

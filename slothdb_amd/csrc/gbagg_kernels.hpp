@@ -38,6 +38,22 @@ __device__ __forceinline__ void hll_add(uint32_t *reg, uint64_t h) {
     __hip_atomic_fetch_max(&reg[idx], rank, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
+// Register-wise max over `nparts` sketches (thread per register).
+__global__ __launch_bounds__(256) void k_hll_merge(const uint8_t *partial, uint32_t nparts, uint8_t *merged) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= HLL_M) return;
+    uint32_t m0 = 0, m1 = 0, m2 = 0, m3 = 0;
+    uint32_t p = 0;
+    for (; p + 4 <= nparts; p += 4) {
+        m0 = max(m0, (uint32_t)partial[(uint64_t)p * HLL_M + i]);
+        m1 = max(m1, (uint32_t)partial[(uint64_t)(p + 1) * HLL_M + i]);
+        m2 = max(m2, (uint32_t)partial[(uint64_t)(p + 2) * HLL_M + i]);
+        m3 = max(m3, (uint32_t)partial[(uint64_t)(p + 3) * HLL_M + i]);
+    }
+    for (; p < nparts; p++) m0 = max(m0, (uint32_t)partial[(uint64_t)p * HLL_M + i]);
+    merged[i] = (uint8_t)max(max(m0, m1), max(m2, m3));
+}
+
 // One block of 1024 threads: max-merge `nparts` sketches, write the estimate.
 __global__ __launch_bounds__(1024) void k_hll_finish(const uint8_t *partial, uint32_t nparts,
                                                      DevPlan *plan) {
@@ -70,7 +86,8 @@ __global__ __launch_bounds__(1024) void k_hll_finish(const uint8_t *partial, uin
     }
 }
 
-__global__ __launch_bounds__(256) void k_table_init(DevTable T, DevQuery q) {
+__global__ __launch_bounds__(256) void k_table_init(DevTable T, const DevQuery *__restrict__ qp) {
+    const DevQuery &q = *qp;
     const uint64_t cap = T.plan->cap;
     const uint64_t nrec = cap + 2;
     uint64_t id[MAX_ACC];
@@ -229,11 +246,15 @@ __device__ __forceinline__ void wave_reduce_apply(const DevQuery &q, const Row &
     }
 }
 
-__global__ __launch_bounds__(AGG_BLOCK) void k_aggregate(DevQuery q, DevBatch b, DevTable T,
-                                                         uint32_t S, uint32_t fill_limit,
-                                                         uint32_t umask) {
+template <class S>
+__global__ __launch_bounds__(AGG_BLOCK) void k_aggregate(const DevQuery *__restrict__ qp,
+                                                         const DevBatch *__restrict__ bp, DevTable T,
+                                                         uint32_t lds_slots, uint32_t fill_limit) {
+    const DevQuery &q = *qp;
+    const DevBatch &b = *bp;
+    const S sh(q);
     extern __shared__ uint64_t smem[];
-    LdsTable t = lds_table_setup(smem, q, S, fill_limit);
+    LdsTable t = lds_table_setup(smem, q, lds_slots, fill_limit);
     __syncthreads();
     const uint64_t cap = T.plan->cap;
     const uint64_t mask = cap - 1;
@@ -244,40 +265,43 @@ __global__ __launch_bounds__(AGG_BLOCK) void k_aggregate(DevQuery q, DevBatch b,
 #pragma unroll
         for (int j = 0; j < AGG_ROWS; j++) {
             uint64_t r = base + (uint64_t)j * AGG_BLOCK + threadIdx.x;
-            if (r < n) load_row(q, b, r, umask, rows[j]);
+            if (r < n) load_row_s(sh, b, r, sh.nu(), rows[j]);
         }
 #pragma unroll
         for (int j = 0; j < AGG_ROWS; j++) {
             uint64_t r = base + (uint64_t)j * AGG_BLOCK + threadIdx.x;
             bool live = r < n;
-            if (live && q.npred) live = eval_preds(q, b, rows[j]);
+            if (live && sh.npred()) live = eval_preds_s(sh, q, b, rows[j]);
             uint64_t w = 0;
             int kr = KEY_DROP;
-            if (live) kr = make_key(q, b, rows[j], w);
+            if (live) kr = make_key_s(sh, q, b, rows[j], w);
             if (kr == KEY_DROP) continue;
             if (side_row(q, T, cap, kr, w, rows[j])) continue;
             const uint64_t h = hash64(w);
             int slot = lds_find(t, h, w);
             if (slot >= 0) {
-                lds_update(q, t, slot, rows[j]);
+                row_update_s<__HIP_MEMORY_SCOPE_WORKGROUP>(sh, rows[j], t.acc + slot, t.S);
             } else {
                 int64_t g = g_find(T, mask, h, w);
-                if (g >= 0) g_update(q, T, (uint64_t)g, rows[j]);
+                if (g >= 0) row_update_s<__HIP_MEMORY_SCOPE_AGENT>(sh, rows[j], T.rec + (uint64_t)g * T.stride + 1, 1);
             }
         }
     }
     __syncthreads();
-    for (uint32_t i = threadIdx.x; i < S; i += blockDim.x) {
+    for (uint32_t i = threadIdx.x; i < t.S; i += blockDim.x) {
         uint64_t k = t.key[i];
         if (k == SENTINEL) continue;
         int64_t g = g_find(T, mask, hash64(k), k);
-        if (g >= 0) g_merge(q, T, (uint64_t)g, t.acc + i, S);
+        if (g >= 0) g_merge(q, T, (uint64_t)g, t.acc + i, t.S);
     }
 }
 
 // Global-table-only variant (no LDS table).
-__global__ __launch_bounds__(AGG_BLOCK) void k_aggregate_global(DevQuery q, DevBatch b, DevTable T,
-                                                                uint32_t umask) {
+__global__ __launch_bounds__(AGG_BLOCK) void k_aggregate_global(const DevQuery *__restrict__ qp,
+                                                                const DevBatch *__restrict__ bp,
+                                                                DevTable T) {
+    const DevQuery &q = *qp;
+    const DevBatch &b = *bp;
     const uint64_t cap = T.plan->cap;
     const uint64_t mask = cap - 1;
     const uint64_t n = b.nrows;
@@ -287,7 +311,7 @@ __global__ __launch_bounds__(AGG_BLOCK) void k_aggregate_global(DevQuery q, DevB
 #pragma unroll
         for (int j = 0; j < AGG_ROWS; j++) {
             uint64_t r = base + (uint64_t)j * AGG_BLOCK + threadIdx.x;
-            if (r < n) load_row(q, b, r, umask, rows[j]);
+            if (r < n) load_row(q, b, r, (1u << q.nu) - 1, rows[j]);
         }
 #pragma unroll
         for (int j = 0; j < AGG_ROWS; j++) {
@@ -651,7 +675,9 @@ __global__ __launch_bounds__(1024) void k_scan_blocks(DevTable T, uint32_t *blk_
     if (threadIdx.x == blockDim.x - 1) T.plan->groups = start + s_part[blockDim.x - 1];
 }
 
-__global__ __launch_bounds__(256) void k_compact_write(DevTable T, const uint64_t *blk_base, DevOut o) {
+__global__ __launch_bounds__(256) void k_compact_write(DevTable T, const uint64_t *blk_base,
+                                                       const DevOut *__restrict__ op) {
+    const DevOut &o = *op;
     const uint64_t cap = T.plan->cap;
     const uint64_t nrec = cap + 2;
     const uint64_t nblk = (nrec + COMPACT_CHUNK - 1) / COMPACT_CHUNK;
@@ -680,7 +706,8 @@ __global__ __launch_bounds__(256) void k_compact_write(DevTable T, const uint64_
 }
 
 template <int KMAX>
-__global__ __launch_bounds__(256) void k_topn_table(DevTable T, DevTopn t) {
+__global__ __launch_bounds__(256) void k_topn_table(DevTable T, const DevTopn *__restrict__ tp) {
+    const DevTopn &t = *tp;
     __shared__ uint32_t s_win[KMAX];
     const uint64_t cap = T.plan->cap, nrec = cap + 2;
     TopList<KMAX> L;
@@ -706,7 +733,10 @@ __global__ __launch_bounds__(256) void k_topn_table(DevTable T, DevTopn t) {
 
 // The final TopN over every producer's candidates: one block of 1024.
 template <int KMAX>
-__global__ __launch_bounds__(1024) void k_topn_final(DevTopn t, DevPlan *plan, DevOut o) {
+__global__ __launch_bounds__(1024) void k_topn_final(const DevTopn *__restrict__ tp, DevPlan *plan,
+                                                     const DevOut *__restrict__ op) {
+    const DevTopn &t = *tp;
+    const DevOut &o = *op;
     __shared__ uint32_t s_win[KMAX];
     TopList<KMAX> L;
     L.init();

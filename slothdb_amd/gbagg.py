@@ -77,7 +77,8 @@ class PlanInfo(C.Structure):
                 ("rows_consumed", C.c_uint64), ("heavy_keys", C.c_uint64), ("cold_rows", C.c_uint64),
                 ("sample_rows", C.c_uint64), ("sample_distinct", C.c_uint64), ("sample_f1", C.c_uint64),
                 ("ms_aggregate", C.c_double), ("ms_total", C.c_double), ("ms_sample", C.c_double),
-                ("ms_count", C.c_double), ("ms_partition", C.c_double), ("ms_emit", C.c_double)]
+                ("ms_count", C.c_double), ("ms_partition", C.c_double), ("ms_emit", C.c_double),
+                ("kernel_shape", C.c_char * 32)]
 
 
 # Every symbol include/slothdb_gpu.h declares, with its ctypes signature.
@@ -422,7 +423,9 @@ class GpuHashAggregate:
     def plan_info(self):
         pi = PlanInfo()
         _check(lib().gbagg_plan_info_get(self._h, C.byref(pi)))
-        return {f: getattr(pi, f) for f, _ in PlanInfo._fields_}
+        out = {f: getattr(pi, f) for f, _ in PlanInfo._fields_}
+        out["kernel_shape"] = out["kernel_shape"].decode()
+        return out
 
     def close(self):
         if self._h is not None:

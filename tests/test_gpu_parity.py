@@ -272,3 +272,20 @@ def test_radix_topn_sum_ascending():
     res, info = G.run(q, cols, device=True)
     exp, types = _exact(datagen.highcard_query(), cols)
     assert_topn_tie_aware(res.rows(), exp, order_col=2, k=25, types=types, descending=False)
+
+
+@pytest.mark.parametrize("static", [1, 0])
+def test_static_and_dynamic_kernels_agree(static):
+    """Compiled query shapes and the generic row kernels give the same result."""
+    for (q, cols) in (datagen.plain_workload(300_000), datagen.highcard_workload(600_000, 200_000),
+                      datagen.composite_workload(800_000, dict_size=30_000, topn=None)):
+        for strategy in (G.LDS_HASH, G.RADIX):
+            q.strategy = strategy
+            res, info = G.run(q, cols, device=True, params={"static_shapes": static})
+            if static:
+                assert info["kernel_shape"] != "dynamic"
+            else:
+                assert info["kernel_shape"] == "dynamic"
+            dicts = {i: cols[c].dictionary for i, c in enumerate(q.group_cols) if cols[c].type == G.VARCHAR}
+            exp, types = _exact(q, cols, dicts)
+            assert_same_groups(res.rows(), exp, len(q.group_cols), types)
