@@ -38,7 +38,7 @@ namespace {
 // of the row kernels; every other query runs the same kernels over DynShape.
 using KAgg = void (*)(const DevQuery *, const DevBatch *, DevTable, uint32_t, uint32_t);
 using KCount = void (*)(const DevCtx *, const DevBatch *, uint32_t, uint64_t);
-using KScatter = void (*)(const DevCtx *, const DevBatch *, DevTable, uint32_t, uint64_t);
+using KScatter = void (*)(const DevCtx *, const DevBatch *, DevTable, uint32_t, uint64_t, uint32_t);
 using KPart = void (*)(const DevCtx *, DevTable, uint32_t, uint32_t);
 
 struct ShapeKernels {
@@ -48,7 +48,7 @@ struct ShapeKernels {
     KAgg agg;
     KCount count;
     KScatter scatter;
-    KPart part16, part32;
+    KPart part_full, part16, part32;
 };
 
 template <class S>
@@ -60,8 +60,9 @@ ShapeKernels shape_kernels(const char *name) {
     k.agg = &k_aggregate<S>;
     k.count = &k_rp_count<S>;
     k.scatter = &k_rp_scatter<S>;
-    k.part16 = &k_rp_agg<S, 16>;
-    k.part32 = &k_rp_agg<S, 32>;
+    k.part_full = &k_rp_agg<S>;
+    k.part16 = &k_rp_agg_topn<S, 16>;
+    k.part32 = &k_rp_agg_topn<S, 32>;
     return k;
 }
 
@@ -294,7 +295,7 @@ struct gbagg_handle {
     uint32_t heavy_min = 4;
     uint64_t radix_fallback_cap = 65536;
     int32_t force_part_bits = -1;   // radix fanout override (-1 = from the estimate)
-    uint32_t k4_lds_kb = 100;       // per-partition LDS table budget
+    uint32_t k4_lds_kb = 78;        // per-partition LDS table budget (two workgroups per CU)
     // last run
     gbagg_strategy used_strategy = GBAGG_STRATEGY_AUTO;
     uint32_t lds_slots = 0, agg_grid = 0, retries = 0, radix_bits = 0;
@@ -718,6 +719,14 @@ uint32_t lds_slots_for(size_t budget, int nacc, uint32_t smax) {
     return S;
 }
 
+// Largest LDS table of any size (a multiple of 64 slots) within `budget` bytes.
+uint32_t lds_slots_exact(size_t budget, int nacc, uint32_t smax) {
+    const size_t per = (size_t)8 * (1 + nacc);
+    size_t S = budget > 16 ? (budget - 16) / per : 0;
+    S = std::min<size_t>(smax, S / 64 * 64);
+    return (uint32_t)std::max<size_t>(64, S);
+}
+
 template <typename K>
 void allow_lds(K kernel, size_t bytes) {
     if (bytes > 64 * 1024)
@@ -863,6 +872,23 @@ void ensure_table(gbagg_handle *h, uint64_t cap) {
     h->blkcnt.ensure(((nblk * 4 + 15) / 16) * 16 + nblk * 8 + 64);
 }
 
+// Partition record layout: key, then each loaded slot an accumulator reads,
+// then the validity word when an input is nullable.
+void record_layout(gbagg_handle *h) {
+    RadixParams &rp = h->hctx->rp;
+    uint32_t words = 1;
+    bool nullable_inputs = false;
+    std::memset(rp.rec_pos, 0, sizeof(rp.rec_pos));
+    for (int a = 0; a < h->dq.nacc; a++) {
+        int u = h->dq.acc_u[a];
+        if (u < 0 || rp.rec_pos[u]) continue;
+        rp.rec_pos[u] = (int32_t)words++;
+        if (h->nullable[h->ucol[u]]) nullable_inputs = true;
+    }
+    rp.vmask_word = nullable_inputs ? words++ : 0;
+    rp.rec_words = words;
+}
+
 // Heavy-hitter sample over all batches (strided), then the heavy table.
 void run_sample(gbagg_handle *h) {
     const DeviceInfo &di = device_info();
@@ -871,19 +897,23 @@ void run_sample(gbagg_handle *h) {
     const uint32_t S_TOTAL = 131072;
     rp.scap = 1u << 18;
     rp.hmax = h->heavy_max;
-    rp.hslots = (uint32_t)next_pow2(2ull * std::max<uint32_t>(rp.hmax, 1));
+    // load factor <= 1/4: the longest probe among a wave's 64 lanes sets the
+    // wave's lookup time
+    rp.hslots = (uint32_t)next_pow2(4ull * std::max<uint32_t>(rp.hmax, 1));
     rp.heavy_min = h->heavy_min;
     // the heavy table must fit the scatter kernel's LDS next to its cursors
-    // (8192 partitions at most) and the count kernel's LDS
+    // (8192 partitions at most) and its smallest record ring, and the count
+    // kernel's LDS
     const size_t lds_max = di.lds_per_block;
+    record_layout(h);
     auto fits = [&] {
-        size_t k3 = (size_t)rp.hslots * 12 + (size_t)h->dq.nacc * rp.hmax * 8 + 8192 * 4;
+        size_t k3 = scatter_lds_bytes(rp.hslots, h->dq.nacc, rp.hmax, 8192, 256, rp.rec_words);
         size_t k1 = (size_t)rp.hslots * 8 + 8192 * 4 + HLL_M * 4;
         return k3 <= lds_max && k1 <= lds_max;
     };
     while (rp.hmax > 0 && !fits()) {
         rp.hmax /= 2;
-        rp.hslots = (uint32_t)next_pow2(2ull * std::max<uint32_t>(rp.hmax, 1));
+        rp.hslots = (uint32_t)next_pow2(4ull * std::max<uint32_t>(rp.hmax, 1));
     }
     h->skeys.ensure((size_t)rp.scap * 8);
     h->scnt.ensure((size_t)rp.scap * 4);
@@ -984,42 +1014,45 @@ void run_radix(gbagg_handle *h, Timer &tm) {
     RadixParams &rp = h->hctx->rp;
     rp.bucket_bits = 13;
     const uint32_t nb = 1u << rp.bucket_bits;
-    // record layout: key, then each loaded slot an accumulator reads, then validity
-    uint32_t words = 1;
-    bool nullable_inputs = false;
-    std::memset(rp.rec_pos, 0, sizeof(rp.rec_pos));
-    for (int a = 0; a < nacc; a++) {
-        int u = h->dq.acc_u[a];
-        if (u < 0 || rp.rec_pos[u]) continue;
-        rp.rec_pos[u] = (int32_t)words++;
-        if (h->nullable[h->ucol[u]]) nullable_inputs = true;
-    }
-    rp.vmask_word = nullable_inputs ? words++ : 0;
-    rp.rec_words = words;
+    record_layout(h);
     // workgroup slots for count + scatter
-    const size_t k3_fixed = (size_t)rp.hslots * 12 + (size_t)nacc * rp.hmax * 8;
+    const size_t k3_fixed = scatter_lds_bytes(rp.hslots, nacc, rp.hmax, 0, 256, rp.rec_words);
     const size_t k1_lds = (size_t)rp.hslots * 8 + (size_t)nb * 4 + HLL_M * 4;
     const size_t k3_lds_max = k3_fixed + (size_t)nb * 4;
-    const uint32_t per_cu = (uint32_t)std::max<size_t>(1, std::min<size_t>(4, di.lds_per_cu / std::max(k1_lds, k3_lds_max)));
-    std::vector<uint32_t> wg_off, wg_n;
+    // The scatter pass runs as many workgroups per CU as its LDS allows; the
+    // lighter count pass runs g1 times as many, each over 1/g1 of a scatter
+    // workgroup's rows, so its histogram rows stay contiguous per scatter
+    // workgroup.
+    const uint32_t per_cu3 = (uint32_t)std::max<size_t>(1, std::min<size_t>(4, di.lds_per_cu / k3_lds_max));
+    const uint32_t per_cu1 = (uint32_t)std::max<size_t>(1, std::min<size_t>(4, di.lds_per_cu / k1_lds));
+    const uint32_t g1 = std::max<uint32_t>(1, per_cu1 / per_cu3);
+    std::vector<uint32_t> wg_off, wg_n, wg3_off, wg3_n;
     std::vector<uint64_t> rows_per;
-    uint32_t W = 0;
+    uint32_t W = 0, W3 = 0;
     for (auto &b : h->batches) {
-        const uint64_t tile = (uint64_t)RP_BLOCK * AGG_ROWS;
+        const uint64_t tile = (uint64_t)RP_BLOCK * AGG_ROWS * 2 * g1;
         uint64_t tiles = std::max<uint64_t>(1, (b.nrows + tile - 1) / tile);
-        uint32_t g = (uint32_t)std::min<uint64_t>(tiles, (uint64_t)di.cus * per_cu);
-        uint64_t per = ((b.nrows + g - 1) / g + tile - 1) / tile * tile;
-        g = (uint32_t)std::max<uint64_t>(1, (b.nrows + per - 1) / std::max<uint64_t>(per, 1));
+        uint32_t g = (uint32_t)std::min<uint64_t>(tiles, (uint64_t)di.cus * per_cu3);
+        uint64_t per3 = ((b.nrows + g - 1) / g + tile - 1) / tile * tile;
+        const uint64_t per1 = per3 / g1;
+        const uint32_t n1 = b.nrows ? (uint32_t)((b.nrows + per1 - 1) / per1) : 0;
+        const uint32_t n3 = (n1 + g1 - 1) / g1;
         wg_off.push_back(W);
-        wg_n.push_back(b.nrows ? g : 0);
-        rows_per.push_back(per);
-        W += b.nrows ? g : 0;
+        wg_n.push_back(n1);
+        wg3_off.push_back(W3);
+        wg3_n.push_back(n3);
+        rows_per.push_back(per1);
+        W += n1;
+        W3 += n3;
     }
     W = std::max<uint32_t>(W, 1);
+    W3 = std::max<uint32_t>(W3, 1);
     rp.W = W;
+    rp.W3 = W3;
+    rp.g1 = g1;
     h->hist.ensure((size_t)W * nb * 4);
     h->hll.ensure((size_t)W * HLL_M);
-    h->hpart.ensure((size_t)W * nacc * std::max<uint32_t>(rp.hmax, 1) * 8);
+    h->hpart.ensure((size_t)W3 * nacc * std::max<uint32_t>(rp.hmax, 1) * 8);
     rp.hist = h->hist.as<uint32_t>();
     rp.hll = h->hll.as<uint8_t>();
     rp.hpart = h->hpart.as<uint64_t>();
@@ -1041,10 +1074,11 @@ void run_radix(gbagg_handle *h, Timer &tm) {
             hipLaunchKernelGGL(h->kern->count, dim3(wg_n[i]), dim3(RP_BLOCK), k1_lds, s, (const DevCtx *)dctx(h),
                                d_b(h, i), wg_off[i], rows_per[i]);
         }
-        h->hllm.ensure(HLL_M);
-        hipLaunchKernelGGL(k_hll_merge, dim3(HLL_M / 256), dim3(256), 0, s, (const uint8_t *)rp.hll, W,
+        const uint32_t hslices = std::min<uint32_t>(32, W);
+        h->hllm.ensure((size_t)HLL_M * hslices);
+        hipLaunchKernelGGL(k_hll_merge, dim3(HLL_M / 256, hslices), dim3(256), 0, s, (const uint8_t *)rp.hll, W,
                            h->hllm.as<uint8_t>());
-        hipLaunchKernelGGL(k_hll_finish, dim3(1), dim3(1024), 0, s, (const uint8_t *)h->hllm.as<uint8_t>(), 1u,
+        hipLaunchKernelGGL(k_hll_finish, dim3(1), dim3(1024), 0, s, (const uint8_t *)h->hllm.as<uint8_t>(), hslices,
                            T.plan);
         tm.mark();  // count end
         read_plan(h);  // HLL estimate and heavy count pick the fanout
@@ -1052,7 +1086,7 @@ void run_radix(gbagg_handle *h, Timer &tm) {
         const uint64_t heavy = h->hplan->heavy_count;
         const uint64_t cold = est > heavy ? est - heavy : 1;
         const size_t k4_budget = std::min<size_t>(di.lds_per_block, (size_t)h->k4_lds_kb * 1024);
-        const uint32_t S = lds_slots_for(k4_budget, nacc, 8192);
+        const uint32_t S = lds_slots_exact(k4_budget, nacc, 8192);
         uint32_t pbits = 0;
         while (pbits < rp.bucket_bits && (double)cold / (double)(1u << pbits) > 0.5 * S) pbits++;
         if (h->force_part_bits >= 0) pbits = std::min<uint32_t>((uint32_t)h->force_part_bits, rp.bucket_bits);
@@ -1069,23 +1103,35 @@ void run_radix(gbagg_handle *h, Timer &tm) {
         rp.pstart = h->pstart.as<uint64_t>();
         rp.recs = h->recs.as<uint64_t>();
         upload_ctx(h);
-        hipLaunchKernelGGL(k_rp_scan_w, dim3((P + 255) / 256), dim3(256), 0, s, d_rp(h));
+        hipLaunchKernelGGL(k_rp_scan_w, dim3((P + SCAN_PARTS - 1) / SCAN_PARTS), dim3(SCAN_PARTS * SCAN_CHUNKS), 0, s,
+                           d_rp(h));
         hipLaunchKernelGGL(k_rp_scan_p, dim3(1), dim3(1024), 0, s, d_rp(h), T.plan);
-        const size_t k3_lds = k3_fixed + (size_t)P * 4;
+        // record ring: as deep as the LDS left next to the cursors allows
+        uint32_t ring = 2048;
+        while (ring > 256 && scatter_lds_bytes(rp.hslots, nacc, rp.hmax, P, ring, rp.rec_words) > di.lds_per_block)
+            ring >>= 1;
+        rp.ring = ring;
+        const size_t k3_lds = scatter_lds_bytes(rp.hslots, nacc, rp.hmax, P, ring, rp.rec_words);
+        if (k3_lds > di.lds_per_block) fail(GBAGG_ERROR, "scatter kernel LDS exceeds the device limit");
+        upload_ctx(h);
         allow_lds(h->kern->scatter, k3_lds);
         tm.mark();  // scatter start
         for (size_t i = 0; i < h->batches.size(); i++) {
             if (!h->batches[i].nrows) continue;
-            hipLaunchKernelGGL(h->kern->scatter, dim3(wg_n[i]), dim3(RP_BLOCK), k3_lds, s, (const DevCtx *)dctx(h),
-                               d_b(h, i), T, wg_off[i], rows_per[i]);
+            hipLaunchKernelGGL(h->kern->scatter, dim3(wg3_n[i]), dim3(RP_BLOCK), k3_lds, s, (const DevCtx *)dctx(h),
+                               d_b(h, i), T, wg3_off[i], rows_per[i] * g1, wg_off[i]);
         }
-        h->agg_grid = W;
+        h->agg_grid = W3;
         tm.mark();  // scatter end
         const size_t k4_lds = lds_table_bytes(S, nacc);
         const uint32_t hgrid = h->topn_device ? 1 : std::max<uint32_t>(1, (rp.hmax + 255) / 256);
         const dim3 fgrid(std::max<uint32_t>(1, (rp.hmax + 63) / 64), (uint32_t)nacc);
         if (rp.hmax) hipLaunchKernelGGL(k_heavy_fold, fgrid, dim3(256), 0, s, (const DevCtx *)dctx(h), T);
-        if (h->topn_k <= 16) {
+        if (!h->topn_device) {
+            allow_lds(h->kern->part_full, k4_lds);
+            hipLaunchKernelGGL(h->kern->part_full, dim3(P), dim3(RA_BLOCK), k4_lds, s, (const DevCtx *)dctx(h), T, S, S * 7 / 8);
+            if (rp.hmax) hipLaunchKernelGGL(k_heavy_emit<16>, dim3(hgrid), dim3(256), 0, s, (const DevCtx *)dctx(h), T);
+        } else if (h->topn_k <= 16) {
             allow_lds(h->kern->part16, k4_lds);
             hipLaunchKernelGGL(h->kern->part16, dim3(P), dim3(RA_BLOCK), k4_lds, s, (const DevCtx *)dctx(h), T, S, S * 7 / 8);
             if (rp.hmax) hipLaunchKernelGGL(k_heavy_emit<16>, dim3(hgrid), dim3(256), 0, s, (const DevCtx *)dctx(h), T);
@@ -1100,6 +1146,7 @@ void run_radix(gbagg_handle *h, Timer &tm) {
         topn_final(h);
         tm.mark();  // emit end
         read_plan(h);
+        if (h->hplan->overflow & OVF_RING) fail(GBAGG_ERROR, "scatter record ring wait bound exceeded");
         if (!(h->hplan->overflow & 1)) break;
         if (cap >= h->cap_max) fail(GBAGG_ERROR, "fallback hash table overflow at maximum size");
         cap = std::min<uint64_t>(h->cap_max, cap * 16);

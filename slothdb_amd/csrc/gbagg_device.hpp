@@ -101,6 +101,22 @@ struct DevTable {
     DevPlan *plan;
 };
 
+// Global-memory accesses through address space 1. Pointers read out of
+// parameter blocks are generic, and a generic (flat) access is counted by
+// both the vector-memory and the LDS wait counters, so every LDS operation
+// would also wait for outstanding row loads; global_load/global_store are
+// counted by the vector-memory counter alone.
+#define GBAGG_GLOBAL __attribute__((address_space(1)))
+template <class T>
+__device__ __forceinline__ T gld(const T *p) {
+    return *(const GBAGG_GLOBAL T *)p;
+}
+template <class T>
+__device__ __forceinline__ void gst(T *p, T v) {
+    *(GBAGG_GLOBAL T *)p = v;
+}
+typedef uint64_t u64x2 __attribute__((ext_vector_type(2)));  // 16-byte global access
+
 __device__ __forceinline__ uint64_t hash64(uint64_t x) {
     // wyhash mum/mix with the 64-bit golden ratio: lo ^ hi of x * C
     // (include/third_party/unordered_dense.h:158-191, 266-267).
@@ -154,12 +170,12 @@ __device__ __forceinline__ void load_row(const DevQuery &q, const DevBatch &b, u
         if (u < q.nu && ((umask >> u) & 1)) {
             uint64_t x;
             switch (q.ukind[u]) {
-            case CK_I32: x = (uint64_t)(int64_t)((const int32_t *)b.data[u])[r]; break;
-            case CK_U32: x = (uint64_t)((const uint32_t *)b.data[u])[r]; break;
-            default: x = ((const uint64_t *)b.data[u])[r]; break;
+            case CK_I32: x = (uint64_t)(int64_t)gld((const int32_t *)b.data[u] + r); break;
+            case CK_U32: x = (uint64_t)gld((const uint32_t *)b.data[u] + r); break;
+            default: x = gld((const uint64_t *)b.data[u] + r); break;
             }
             row.v[u] = x;
-            uint32_t ok = b.valid[u] ? (b.valid[u][r] != 0) : 1u;
+            uint32_t ok = b.valid[u] ? (gld(b.valid[u] + r) != 0) : 1u;
             row.vm |= ok << u;
         }
     }
@@ -192,7 +208,9 @@ __device__ __forceinline__ bool cmp_f64(double v, double lit, int op) {
 // EvalSimplePredicates semantics (physical_planner.cpp:396-496): a NULL in a
 // predicate column drops the row; integer columns compare int64-exact.
 __device__ __forceinline__ bool eval_preds(const DevQuery &q, const DevBatch &b, const Row &row) {
-    for (int p = 0; p < q.npred; p++) {
+#pragma unroll
+    for (int p = 0; p < MAX_PRED; p++) {  // unrolled: b.pred_ptr[p] stays a register
+        if (p >= q.npred) break;
         const DevPred &P = q.pred[p];
         if (!((row.vm >> P.u) & 1)) return false;
         uint64_t x = sel(row, P.u);
@@ -329,6 +347,8 @@ struct ShapeSig {
 
 struct DynShape {
     static constexpr bool kStatic = false;
+    // record width is a run-time value; 0 keeps the vector record path off
+    __host__ __device__ static constexpr int rec_words() { return 1; }
     const DevQuery &q;
     __device__ explicit DynShape(const DevQuery &q_) : q(q_) {}
     __device__ int nu() const { return q.nu; }
@@ -386,6 +406,9 @@ struct StaticShape {
         for (int u = 0; u < 8; u++) w += rec_pos(u) ? 1 : 0;
         return w;
     }
+    __host__ __device__ static constexpr int rec_words() {
+        return 1 + rec_inputs() + (any_nullable_input() ? 1 : 0);
+    }
     __host__ __device__ static constexpr bool any_nullable_input() {
         bool n = false;
         for (int a = 0; a < NA; a++) n = n || (au(a) >= 0 && nullable(au(a)));
@@ -413,13 +436,13 @@ __device__ __forceinline__ void load_row_s(const S &sh, const DevBatch &b, uint6
         if (u < nload) {
             uint64_t x;
             switch (sh.kind(u)) {
-            case CK_I32: x = (uint64_t)(int64_t)((const int32_t *)b.data[u])[r]; break;
-            case CK_U32: x = (uint64_t)((const uint32_t *)b.data[u])[r]; break;
-            default: x = ((const uint64_t *)b.data[u])[r]; break;
+            case CK_I32: x = (uint64_t)(int64_t)gld((const int32_t *)b.data[u] + r); break;
+            case CK_U32: x = (uint64_t)gld((const uint32_t *)b.data[u] + r); break;
+            default: x = gld((const uint64_t *)b.data[u] + r); break;
             }
             row.v[u] = x;
             uint32_t ok = 1u;
-            if (sh.nullable(u) && b.valid[u]) ok = b.valid[u][r] != 0;
+            if (sh.nullable(u) && b.valid[u]) ok = gld(b.valid[u] + r) != 0;
             row.vm |= ok << u;
         }
     }

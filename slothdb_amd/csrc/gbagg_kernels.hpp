@@ -38,20 +38,24 @@ __device__ __forceinline__ void hll_add(uint32_t *reg, uint64_t h) {
     __hip_atomic_fetch_max(&reg[idx], rank, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
-// Register-wise max over `nparts` sketches (thread per register).
+// Register-wise max over `nparts` sketches: thread per register, blockIdx.y
+// picks one of gridDim.y slices of the sketches; slice y's merge goes to
+// merged[y * HLL_M ...] (k_hll_finish folds the slices).
 __global__ __launch_bounds__(256) void k_hll_merge(const uint8_t *partial, uint32_t nparts, uint8_t *merged) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= HLL_M) return;
+    const uint32_t per = (nparts + gridDim.y - 1) / gridDim.y;
+    const uint32_t p0 = min(nparts, blockIdx.y * per), p1 = min(nparts, p0 + per);
     uint32_t m0 = 0, m1 = 0, m2 = 0, m3 = 0;
-    uint32_t p = 0;
-    for (; p + 4 <= nparts; p += 4) {
+    uint32_t p = p0;
+    for (; p + 4 <= p1; p += 4) {
         m0 = max(m0, (uint32_t)partial[(uint64_t)p * HLL_M + i]);
         m1 = max(m1, (uint32_t)partial[(uint64_t)(p + 1) * HLL_M + i]);
         m2 = max(m2, (uint32_t)partial[(uint64_t)(p + 2) * HLL_M + i]);
         m3 = max(m3, (uint32_t)partial[(uint64_t)(p + 3) * HLL_M + i]);
     }
-    for (; p < nparts; p++) m0 = max(m0, (uint32_t)partial[(uint64_t)p * HLL_M + i]);
-    merged[i] = (uint8_t)max(max(m0, m1), max(m2, m3));
+    for (; p < p1; p++) m0 = max(m0, (uint32_t)partial[(uint64_t)p * HLL_M + i]);
+    merged[(uint64_t)blockIdx.y * HLL_M + i] = (uint8_t)max(max(m0, m1), max(m2, m3));
 }
 
 // One block of 1024 threads: max-merge `nparts` sketches, write the estimate.
@@ -111,6 +115,7 @@ __device__ __forceinline__ int64_t g_find(const DevTable &T, uint64_t mask, uint
                                           uint64_t k) {
     if (k == SENTINEL) return (int64_t)(mask + 1);
     uint64_t pos = h & mask;
+#pragma nounroll
     for (uint32_t i = 0; i < T.max_probe; i++) {
         uint64_t *kp = T.rec + pos * T.stride;
         uint64_t cur = __hip_atomic_load(kp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -183,27 +188,33 @@ __device__ __forceinline__ LdsTable lds_table_setup(uint64_t *smem, const DevQue
     return t;
 }
 
+// Any table size: the home slot is the low 32 hash bits scaled to [0, S)
+// (Lemire's multiply-shift), so S need not be a power of two and the LDS can
+// be filled exactly. Radix partitions use the top hash bits, which stay
+// independent of these.
 __device__ __forceinline__ int lds_find(const LdsTable &t, uint64_t h, uint64_t k) {
-    const uint32_t mask = t.S - 1;
-    uint32_t pos = (uint32_t)h & mask;
-    for (int i = 0; i < 32; i++) {
-        uint64_t cur = __hip_atomic_load(&t.key[pos], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        if (cur == k) return (int)pos;
-        if (cur == SENTINEL) {
-            if (__hip_atomic_load(t.fill, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) >= t.limit)
-                return -1;
-            uint64_t expected = SENTINEL;
-            __hip_atomic_compare_exchange_strong(&t.key[pos], &expected, k, __ATOMIC_RELAXED,
-                                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-            if (expected == SENTINEL) {
-                __hip_atomic_fetch_add(t.fill, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                return (int)pos;
-            }
-            if (expected == k) return (int)pos;
+    uint32_t pos = (uint32_t)(((uint64_t)(uint32_t)h * t.S) >> 32);
+    uint32_t probes = 1;
+    uint64_t cur = __hip_atomic_load(&t.key[pos], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    for (;;) {
+        // scan to k or to an empty slot (single exit: cheap divergent loop)
+        while (cur != k && cur != SENTINEL && probes < 32) {
+            pos = pos + 1 == t.S ? 0 : pos + 1;
+            cur = __hip_atomic_load(&t.key[pos], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            probes++;
         }
-        pos = (pos + 1) & mask;
+        if (cur == k) return (int)pos;
+        if (cur != SENTINEL) return -1;  // probe bound
+        if (__hip_atomic_load(t.fill, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) >= t.limit) return -1;
+        uint64_t expected = SENTINEL;
+        __hip_atomic_compare_exchange_strong(&t.key[pos], &expected, k, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (expected == SENTINEL) {
+            __hip_atomic_fetch_add(t.fill, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            return (int)pos;
+        }
+        cur = expected;  // taken meanwhile: by k itself (found) or by another key (keep probing)
     }
-    return -1;
 }
 
 __device__ __forceinline__ void lds_update(const DevQuery &q, const LdsTable &t, int slot,
@@ -251,7 +262,7 @@ __global__ __launch_bounds__(AGG_BLOCK) void k_aggregate(const DevQuery *__restr
                                                          const DevBatch *__restrict__ bp, DevTable T,
                                                          uint32_t lds_slots, uint32_t fill_limit) {
     const DevQuery &q = *qp;
-    const DevBatch &b = *bp;
+    const DevBatch b = *bp;  // column pointers stay in registers
     const S sh(q);
     extern __shared__ uint64_t smem[];
     LdsTable t = lds_table_setup(smem, q, lds_slots, fill_limit);
@@ -301,7 +312,7 @@ __global__ __launch_bounds__(AGG_BLOCK) void k_aggregate_global(const DevQuery *
                                                                 const DevBatch *__restrict__ bp,
                                                                 DevTable T) {
     const DevQuery &q = *qp;
-    const DevBatch &b = *bp;
+    const DevBatch b = *bp;  // column pointers stay in registers
     const uint64_t cap = T.plan->cap;
     const uint64_t mask = cap - 1;
     const uint64_t n = b.nrows;
@@ -387,35 +398,35 @@ __device__ __forceinline__ void write_group(const DevOut &o, uint64_t key, const
         case OUT_KEY: {
             uint64_t kv = key_component(o.key_mode, key, va);
             ok = !null_key;
-            if (o.key_type[c] == CK_I64) ((int64_t *)o.data[c])[row] = ok ? (int64_t)kv : 0;
-            else ((int32_t *)o.data[c])[row] = ok ? (int32_t)(uint32_t)kv : 0;
+            if (o.key_type[c] == CK_I64) gst((int64_t *)o.data[c] + row, ok ? (int64_t)kv : (int64_t)0);
+            else gst((int32_t *)o.data[c] + row, ok ? (int32_t)(uint32_t)kv : (int32_t)0);
             break;
         }
         case OUT_COUNT:
-            ((int64_t *)o.data[c])[row] = (int64_t)v;
+            gst((int64_t *)o.data[c] + row, (int64_t)v);
             ok = 1;
             break;
         case OUT_SUM_I:
         case OUT_MM_I64:
-            ((int64_t *)o.data[c])[row] = ok ? (int64_t)v : 0;
+            gst((int64_t *)o.data[c] + row, ok ? (int64_t)v : (int64_t)0);
             break;
         case OUT_MM_I32:
-            ((int32_t *)o.data[c])[row] = ok ? (int32_t)(int64_t)v : 0;
+            gst((int32_t *)o.data[c] + row, ok ? (int32_t)(int64_t)v : (int32_t)0);
             break;
         case OUT_SUM_F:
-            ((double *)o.data[c])[row] = ok ? __longlong_as_double((long long)v) : 0.0;
+            gst((double *)o.data[c] + row, ok ? __longlong_as_double((long long)v) : 0.0);
             break;
         case OUT_AVG_I:
-            ((double *)o.data[c])[row] = ok ? (double)(int64_t)v / (double)cnt : 0.0;
+            gst((double *)o.data[c] + row, ok ? (double)(int64_t)v / (double)cnt : 0.0);
             break;
         case OUT_AVG_F:
-            ((double *)o.data[c])[row] = ok ? __longlong_as_double((long long)v) / (double)cnt : 0.0;
+            gst((double *)o.data[c] + row, ok ? __longlong_as_double((long long)v) / (double)cnt : 0.0);
             break;
         default:  // OUT_MM_F
-            ((double *)o.data[c])[row] = ok ? __longlong_as_double((long long)f64_unord(v)) : 0.0;
+            gst((double *)o.data[c] + row, ok ? __longlong_as_double((long long)f64_unord(v)) : 0.0);
             break;
         }
-        o.valid[c][row] = ok;
+        gst(o.valid[c] + row, ok);
     }
 }
 

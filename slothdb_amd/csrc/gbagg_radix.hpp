@@ -27,9 +27,20 @@ namespace gbagg {
 constexpr int RP_BLOCK = 1024;   // count / scatter workgroup: 16 waves share one LDS table
 constexpr int RA_BLOCK = 1024;   // per-partition aggregation workgroup
 constexpr int RA_ROWS = 4;       // records per thread in flight
+// Scatter workgroup roles: the first SC_PROD threads load and classify rows;
+// the other SC_CONS threads only write cold records to HBM, taking them from
+// an LDS ring. The vector-memory counter drains in issue order for loads and
+// stores alike, so stores issued by the loading waves would hold up their
+// later loads until the stores are acknowledged.
+constexpr int SC_PROD = 768;
+constexpr int SC_CONS = RP_BLOCK - SC_PROD;
+constexpr uint32_t SC_SPIN_MAX = 1u << 24;  // bounded waits: an error, never a hang
+constexpr uint32_t OVF_RING = 4u;           // DevPlan::overflow bit: ring wait bound hit
 
 struct RadixParams {
-    uint32_t W;             // workgroup slots over all batches
+    uint32_t W;             // count-pass workgroup slots over all batches (histogram rows)
+    uint32_t W3;            // scatter-pass workgroup slots (heavy partial states)
+    uint32_t g1;            // count workgroups per scatter workgroup
     uint32_t bucket_bits;   // count histogram: 1 << bucket_bits hash buckets
     uint32_t part_bits;     // partitions P = 1 << part_bits (<= bucket_bits)
     uint32_t *hist;         // [W][1 << bucket_bits]
@@ -46,7 +57,7 @@ struct RadixParams {
     uint64_t *hdense;       // [hmax] key by dense index
     uint32_t hslots;
     uint32_t hmax;
-    uint64_t *hpart;        // [W][nacc][hmax] per-workgroup partial states
+    uint64_t *hpart;        // [W3][nacc][hmax] per-scatter-workgroup partial states
     uint64_t *hfinal;       // [hmax][1 + nacc] folded states
     uint8_t *hll;           // [W][HLL_M]
     // sample
@@ -54,6 +65,8 @@ struct RadixParams {
     uint32_t *scnt;         // [scap]
     uint32_t scap;
     uint32_t heavy_min;     // minimum sample count of a heavy key
+    uint32_t ring;          // scatter LDS ring entries (power of two >= 256)
+    uint32_t pad;
 };
 
 // Everything a launch reads from device memory (kernel arguments would be
@@ -75,12 +88,38 @@ __global__ void k_sample_init(const RadixParams *__restrict__ rpp) {
         rp.hkeys[i] = SENTINEL;
 }
 
+// Sample keys are first counted in a small per-block LDS table, so a hot key
+// costs one global atomic per block rather than one per sampled row.
+__device__ __forceinline__ void sample_global_add(const RadixParams &rp, uint64_t w, uint32_t c) {
+    const uint32_t mask = rp.scap - 1;
+    uint32_t pos = (uint32_t)hash64(w) & mask;
+    for (uint32_t probe = 0; probe < rp.scap; probe++) {
+        uint64_t cur = __hip_atomic_load(&rp.skeys[pos], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (cur == SENTINEL) {
+            uint64_t e = SENTINEL;
+            __hip_atomic_compare_exchange_strong(&rp.skeys[pos], &e, w, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT);
+            cur = e == SENTINEL ? w : e;
+        }
+        if (cur == w) {
+            atomicAdd(&rp.scnt[pos], c);
+            return;
+        }
+        pos = (pos + 1) & mask;
+    }
+}
+
+constexpr uint32_t SAMPLE_LDS = 1024;
+
 __global__ __launch_bounds__(256) void k_sample(const DevCtx *__restrict__ cx, const DevBatch *__restrict__ bp,
                                                 uint32_t sample_n, uint64_t sample_stride, DevPlan *plan) {
     const DevQuery &q = cx->q;
     const RadixParams &rp = cx->rp;
-    const DevBatch &b = *bp;
-    const uint32_t mask = rp.scap - 1;
+    const DevBatch b = *bp;  // column pointers stay in registers
+    __shared__ uint64_t lk[SAMPLE_LDS];
+    __shared__ uint32_t lc[SAMPLE_LDS];
+    for (uint32_t i = threadIdx.x; i < SAMPLE_LDS; i += blockDim.x) { lk[i] = SENTINEL; lc[i] = 0; }
+    __syncthreads();
     uint32_t kept = 0;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < sample_n; i += gridDim.x * blockDim.x) {
         uint64_t r = (uint64_t)i * sample_stride;
@@ -91,22 +130,27 @@ __global__ __launch_bounds__(256) void k_sample(const DevCtx *__restrict__ cx, c
         uint64_t w;
         if (make_key(q, b, row, w) != KEY_OK || w == SENTINEL) continue;
         kept++;
-        uint32_t pos = (uint32_t)hash64(w) & mask;
-        for (uint32_t probe = 0; probe < rp.scap; probe++) {
-            uint64_t cur = __hip_atomic_load(&rp.skeys[pos], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        uint32_t pos = (uint32_t)hash64(w) & (SAMPLE_LDS - 1);
+        bool done = false;
+        for (int probe = 0; probe < 16 && !done; probe++) {
+            uint64_t cur = __hip_atomic_load(&lk[pos], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             if (cur == SENTINEL) {
                 uint64_t e = SENTINEL;
-                __hip_atomic_compare_exchange_strong(&rp.skeys[pos], &e, w, __ATOMIC_RELAXED,
-                                                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_compare_exchange_strong(&lk[pos], &e, w, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_WORKGROUP);
                 cur = e == SENTINEL ? w : e;
             }
             if (cur == w) {
-                atomicAdd(&rp.scnt[pos], 1u);
-                break;
+                atomicAdd(&lc[pos], 1u);
+                done = true;
             }
-            pos = (pos + 1) & mask;
+            pos = (pos + 1) & (SAMPLE_LDS - 1);
         }
+        if (!done) sample_global_add(rp, w, 1);
     }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < SAMPLE_LDS; i += blockDim.x)
+        if (lc[i]) sample_global_add(rp, lk[i], lc[i]);
     if (kept) atomicAdd((unsigned long long *)&plan->sample_rows, (unsigned long long)kept);
 }
 
@@ -176,35 +220,75 @@ __global__ __launch_bounds__(256) void k_heavy_insert(const RadixParams *__restr
 }
 
 // Static heavy table lookup in LDS: membership only.
-__device__ __forceinline__ bool heavy_has(const uint64_t *hk, uint32_t hmask, uint64_t h, uint64_t k) {
-    uint32_t pos = (uint32_t)h & hmask;
-    for (;;) {
-        uint64_t cur = hk[pos];
-        if (cur == k) return true;
-        if (cur == SENTINEL) return false;
-        pos = (pos + 1) & hmask;
+// Runs f(row) over rows [r0, r1) of a batch, R rows per thread per tile of
+// BLOCK * R rows. Tile k+1's loads are issued before tile k's work, into the
+// other of two register buffers (no copy between them), so a row's loads are
+// consumed where the row is processed and never wait behind stores issued for
+// earlier rows: the vector-memory counter is shared by loads and stores and
+// drains in issue order. Row indexes are clamped to the last row, so every
+// load is unconditional and in bounds.
+template <int R, int BLOCK, class S, class F>
+__device__ __forceinline__ void for_rows_pipelined(const S &sh, const DevBatch &b, uint64_t r0, uint64_t r1,
+                                                   int nload, uint32_t tid, F &&f) {
+    if (r0 >= r1) return;
+    const uint64_t last = r1 - 1, step = (uint64_t)BLOCK * R;
+    Row A[R], B[R];
+    auto load = [&](uint64_t base, Row(&dst)[R]) {
+#pragma unroll
+        for (int j = 0; j < R; j++)
+            load_row_s(sh, b, min(base + (uint64_t)j * BLOCK + tid, last), nload, dst[j]);
+    };
+    auto run = [&](uint64_t base, const Row(&src)[R]) {
+#pragma unroll
+        for (int j = 0; j < R; j++)
+            if (base + (uint64_t)j * BLOCK + tid < r1) f(src[j]);
+    };
+    load(r0, A);
+    for (uint64_t base = r0; base < r1; base += 2 * step) {
+        load(base + step, B);
+        run(base, A);
+        load(base + 2 * step, A);
+        run(base + step, B);
     }
+}
+
+// Linear probe of the heavy table in LDS with a single loop exit: the slot
+// of k, or of the first empty slot. (Separate "found" and "empty" exits make
+// the compiler keep two exit masks per iteration, several scalar
+// instructions each.)
+__device__ __forceinline__ uint32_t heavy_probe(const uint64_t *hk, uint32_t hmask, uint64_t h, uint64_t k,
+                                                bool &found) {
+    uint32_t pos = (uint32_t)h & hmask;
+    uint64_t cur = hk[pos];
+    while (cur != k && cur != SENTINEL) {
+        pos = (pos + 1) & hmask;
+        cur = hk[pos];
+    }
+    found = cur == k;
+    return pos;
+}
+
+__device__ __forceinline__ bool heavy_has(const uint64_t *hk, uint32_t hmask, uint64_t h, uint64_t k) {
+    bool found;
+    heavy_probe(hk, hmask, h, k, found);
+    return found;
 }
 
 // Static heavy table lookup in LDS: dense index or -1.
 __device__ __forceinline__ int heavy_find(const uint64_t *hk, const uint32_t *hi, uint32_t hmask,
                                           uint64_t h, uint64_t k) {
-    uint32_t pos = (uint32_t)h & hmask;
-    for (;;) {
-        uint64_t cur = hk[pos];
-        if (cur == k) return (int)hi[pos];
-        if (cur == SENTINEL) return -1;
-        pos = (pos + 1) & hmask;
-    }
+    bool found;
+    const uint32_t pos = heavy_probe(hk, hmask, h, k, found);
+    return found ? (int)hi[pos] : -1;
 }
 
 template <class S>
-__global__ __launch_bounds__(RP_BLOCK) void k_rp_count(const DevCtx *__restrict__ cx,
+__global__ __launch_bounds__(RP_BLOCK) __attribute__((amdgpu_waves_per_eu(8))) void k_rp_count(const DevCtx *__restrict__ cx,
                                                        const DevBatch *__restrict__ bp, uint32_t wg_offset,
                                                        uint64_t rows_per_wg) {
     const DevQuery &q = cx->q;
     const RadixParams &rp = cx->rp;
-    const DevBatch &b = *bp;
+    const DevBatch b = *bp;  // column pointers stay in registers
     const S sh(q);
     extern __shared__ uint64_t smem[];
     uint64_t *hk = smem;                                  // [hslots]
@@ -219,26 +303,16 @@ __global__ __launch_bounds__(RP_BLOCK) void k_rp_count(const DevCtx *__restrict_
     const uint64_t r0 = (uint64_t)blockIdx.x * rows_per_wg;
     const uint64_t r1 = min(b.nrows, r0 + rows_per_wg);
     const uint32_t shift = 64 - rp.bucket_bits;
-    for (uint64_t base = r0; base < r1; base += (uint64_t)RP_BLOCK * AGG_ROWS) {
-        Row rows[AGG_ROWS];
-#pragma unroll
-        for (int j = 0; j < AGG_ROWS; j++) {
-            uint64_t r = base + (uint64_t)j * RP_BLOCK + threadIdx.x;
-            if (r < r1) load_row_s(sh, b, r, sh.nkp(), rows[j]);
-        }
-#pragma unroll
-        for (int j = 0; j < AGG_ROWS; j++) {
-            uint64_t r = base + (uint64_t)j * RP_BLOCK + threadIdx.x;
-            if (r >= r1) continue;
-            if (sh.npred() && !eval_preds_s(sh, q, b, rows[j])) continue;
-            uint64_t w;
-            if (make_key_s(sh, q, b, rows[j], w) != KEY_OK || w == SENTINEL) continue;
-            const uint64_t h = hash64(w);
-            hll_add(reg, h);
-            if (heavy_has(hk, hmask, h, w)) continue;
-            __hip_atomic_fetch_add(&cnt[h >> shift], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        }
-    }
+    auto count_row = [&](const Row &row) {
+        if (sh.npred() && !eval_preds_s(sh, q, b, row)) return;
+        uint64_t w;
+        if (make_key_s(sh, q, b, row, w) != KEY_OK || w == SENTINEL) return;
+        const uint64_t h = hash64(w);
+        hll_add(reg, h);
+        if (heavy_has(hk, hmask, h, w)) return;
+        __hip_atomic_fetch_add(&cnt[h >> shift], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    };
+    for_rows_pipelined<S::kStatic ? AGG_ROWS : 1, RP_BLOCK>(sh, b, r0, r1, sh.nkp(), threadIdx.x, count_row);
     __syncthreads();
     const uint32_t wg = wg_offset + blockIdx.x;
     uint32_t *out = rp.hist + (uint64_t)wg * nb;
@@ -247,23 +321,40 @@ __global__ __launch_bounds__(RP_BLOCK) void k_rp_count(const DevCtx *__restrict_
     for (uint32_t i = threadIdx.x; i < HLL_M; i += blockDim.x) hl[i] = (uint8_t)reg[i];
 }
 
-// Thread per partition: exclusive prefix over workgroups; partition totals.
-__global__ void k_rp_scan_w(const RadixParams *__restrict__ rpp) {
+// Exclusive prefix over count workgroups per partition, and partition
+// totals. Block = 64 partitions x 16 chunks of the workgroup range: chunk
+// sums, then a prefix over the chunks in LDS, then each chunk writes its
+// offsets.
+constexpr int SCAN_PARTS = 64, SCAN_CHUNKS = 16;
+__global__ __launch_bounds__(SCAN_PARTS * SCAN_CHUNKS) void k_rp_scan_w(const RadixParams *__restrict__ rpp) {
     const RadixParams &rp = *rpp;
+    __shared__ uint32_t s_sum[SCAN_CHUNKS][SCAN_PARTS];
     const uint32_t P = 1u << rp.part_bits;
     const uint32_t shift = rp.bucket_bits - rp.part_bits;
     const uint32_t nb = 1u << rp.bucket_bits;
-    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < P; p += gridDim.x * blockDim.x) {
-        uint32_t run = 0;
-        for (uint32_t w = 0; w < rp.W; w++) {
-            const uint32_t *h = rp.hist + (uint64_t)w * nb + ((uint64_t)p << shift);
-            uint32_t c = 0;
-            for (uint32_t j = 0; j < (1u << shift); j++) c += h[j];
-            rp.base[(uint64_t)w * P + p] = run;
-            run += c;
-        }
-        rp.ptotal[p] = run;
+    const uint32_t pl = threadIdx.x % SCAN_PARTS, c = threadIdx.x / SCAN_PARTS;
+    const uint32_t p = blockIdx.x * SCAN_PARTS + pl;
+    const uint32_t per = (rp.W + SCAN_CHUNKS - 1) / SCAN_CHUNKS;
+    const uint32_t w0 = min(rp.W, c * per), w1 = min(rp.W, w0 + per);
+    auto part_count = [&](uint32_t w) {
+        const uint32_t *h = rp.hist + (uint64_t)w * nb + ((uint64_t)p << shift);
+        uint32_t x = 0;
+        for (uint32_t j = 0; j < (1u << shift); j++) x += h[j];
+        return x;
+    };
+    uint32_t sum = 0;
+    if (p < P)
+        for (uint32_t w = w0; w < w1; w++) sum += part_count(w);
+    s_sum[c][pl] = sum;
+    __syncthreads();
+    if (p >= P) return;
+    uint32_t run = 0;
+    for (uint32_t k = 0; k < c; k++) run += s_sum[k][pl];
+    for (uint32_t w = w0; w < w1; w++) {
+        rp.base[(uint64_t)w * P + p] = run;
+        run += part_count(w);
     }
+    if (c == SCAN_CHUNKS - 1) rp.ptotal[p] = run;
 }
 
 // One block of 1024: exclusive scan of the partition totals.
@@ -307,20 +398,34 @@ __device__ __forceinline__ int vmask_word_s(const RadixParams &rp) {
     else return (int)rp.vmask_word;
 }
 
+// LDS bytes of the scatter kernel.
+__host__ __device__ inline size_t scatter_lds_bytes(uint32_t hslots, int nacc, uint32_t hmax, uint32_t P,
+                                                    uint32_t ring, uint32_t rw) {
+    return (size_t)hslots * 12 + ((size_t)nacc * hmax + 1) / 2 * 16 + (size_t)ring * (rw * 8 + 12) + (size_t)P * 4 + 8;
+}
+
 template <class S>
 __global__ __launch_bounds__(RP_BLOCK) void k_rp_scatter(const DevCtx *__restrict__ cx,
                                                          const DevBatch *__restrict__ bp, DevTable T,
-                                                         uint32_t wg_offset, uint64_t rows_per_wg) {
+                                                         uint32_t wg_offset, uint64_t rows_per_wg,
+                                                         uint32_t count_wg_offset) {
     const DevQuery &q = cx->q;
     const RadixParams &rp = cx->rp;
-    const DevBatch &b = *bp;
+    const DevBatch b = *bp;  // column pointers stay in registers
     const S sh(q);
     extern __shared__ uint64_t smem[];
     const uint32_t P = 1u << rp.part_bits;
+    const uint32_t NR = rp.ring;
+    const uint32_t rw = S::kStatic ? (uint32_t)S::rec_words() : rp.rec_words;
     uint64_t *hk = smem;                                      // [hslots]
     uint64_t *hacc = hk + rp.hslots;                          // [nacc][hmax]
-    uint32_t *hi = (uint32_t *)(hacc + (uint64_t)q.nacc * rp.hmax);  // [hslots]
+    uint64_t *ring = hacc + (((uint64_t)q.nacc * rp.hmax + 1) & ~1ULL);  // [NR][rw] records in transit (16-B aligned)
+    uint32_t *hi = (uint32_t *)(ring + (uint64_t)NR * rw);    // [hslots]
     uint32_t *cur = hi + rp.hslots;                           // [P]
+    uint32_t *rpos = cur + P;                                 // [NR] destination record of the entry
+    uint32_t *rpub = rpos + NR;                               // [NR] n + 1 once entry n is readable
+    uint32_t *rfree = rpub + NR;                              // [NR] next entry number the slot may take
+    uint32_t *rctl = rfree + NR;                              // [0] entries reserved, [1] producers done
     const uint32_t wg = wg_offset + blockIdx.x;
     for (uint32_t i = threadIdx.x; i < rp.hslots; i += blockDim.x) {
         hk[i] = rp.hkeys[i];
@@ -330,48 +435,114 @@ __global__ __launch_bounds__(RP_BLOCK) void k_rp_scatter(const DevCtx *__restric
         uint64_t id = acc_identity(q.acc_op[a]);
         for (uint32_t i = threadIdx.x; i < rp.hmax; i += blockDim.x) hacc[(uint64_t)a * rp.hmax + i] = id;
     }
+    for (uint32_t i = threadIdx.x; i < NR; i += blockDim.x) {
+        rpub[i] = 0;
+        rfree[i] = i;
+    }
+    if (threadIdx.x < 2) rctl[threadIdx.x] = 0;
+    // this workgroup's rows are those of count workgroups g1 * b .. g1 * b + g1 - 1,
+    // which are contiguous in every partition
+    const uint64_t cw = (uint64_t)count_wg_offset + (uint64_t)blockIdx.x * rp.g1;
     for (uint32_t p = threadIdx.x; p < P; p += blockDim.x)
-        cur[p] = (uint32_t)rp.pstart[p] + rp.base[(uint64_t)wg * P + p];
+        cur[p] = (uint32_t)rp.pstart[p] + rp.base[cw * P + p];
     __syncthreads();
     const uint32_t hmask = rp.hslots - 1;
     const uint64_t cap = T.plan->cap;
     const uint64_t r0 = (uint64_t)blockIdx.x * rows_per_wg;
     const uint64_t r1 = min(b.nrows, r0 + rows_per_wg);
     const uint32_t shift = 64 - rp.part_bits;
-    const uint32_t rw = rp.rec_words;
     const int vw = vmask_word_s<S>(rp);
-    for (uint64_t base = r0; base < r1; base += (uint64_t)RP_BLOCK * AGG_ROWS) {
-        Row rows[AGG_ROWS];
-#pragma unroll
-        for (int j = 0; j < AGG_ROWS; j++) {
-            uint64_t r = base + (uint64_t)j * RP_BLOCK + threadIdx.x;
-            if (r < r1) load_row_s(sh, b, r, sh.nu(), rows[j]);
-        }
-#pragma unroll
-        for (int j = 0; j < AGG_ROWS; j++) {
-            uint64_t r = base + (uint64_t)j * RP_BLOCK + threadIdx.x;
-            if (r >= r1) continue;
-            if (sh.npred() && !eval_preds_s(sh, q, b, rows[j])) continue;
+    const uint32_t lane = threadIdx.x & 63;
+
+    if (threadIdx.x < SC_PROD) {
+        // ---- producers: filter, key, hash; heavy rows into LDS state, cold rows into the ring
+        bool failed = false;
+        auto scatter_row = [&](const Row &row) {
+            if (sh.npred() && !eval_preds_s(sh, q, b, row)) return;
             uint64_t w;
-            int kr = make_key_s(sh, q, b, rows[j], w);
-            if (kr == KEY_DROP) continue;
-            if (side_row(q, T, cap, kr, w, rows[j])) continue;
+            int kr = make_key_s(sh, q, b, row, w);
+            if (kr == KEY_DROP) return;
+            if (side_row(q, T, cap, kr, w, row)) return;
             const uint64_t h = hash64(w);
             int d = heavy_find(hk, hi, hmask, h, w);
             if (d >= 0) {
-                row_update_s<__HIP_MEMORY_SCOPE_WORKGROUP>(sh, rows[j], hacc + d, rp.hmax);
-                continue;
+                row_update_s<__HIP_MEMORY_SCOPE_WORKGROUP>(sh, row, hacc + d, rp.hmax);
+                return;
             }
             const uint32_t p = P > 1 ? (uint32_t)(h >> shift) : 0;
             const uint32_t pos = __hip_atomic_fetch_add(&cur[p], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-            uint64_t *rec = rp.recs + (uint64_t)pos * rw;
-            rec[0] = w;
-#pragma unroll
-            for (int u = 0; u < MAXU; u++) {
-                const int rpu = rec_pos_s<S>(rp, u);
-                if (rpu) rec[rpu] = rows[j].v[u];
+            // one reservation per wave: the first active lane takes the wave's entries
+            const uint64_t act = __ballot(1);
+            const uint32_t before = __popcll(act & ((1ULL << lane) - 1));
+            uint32_t n0 = 0;
+            if (before == 0)
+                n0 = __hip_atomic_fetch_add(&rctl[0], (uint32_t)__popcll(act), __ATOMIC_RELAXED,
+                                            __HIP_MEMORY_SCOPE_WORKGROUP);
+            const uint32_t n = __builtin_amdgcn_readfirstlane(n0) + before;
+            const uint32_t i = n & (NR - 1);
+            uint32_t spins = 0;
+            while (__hip_atomic_load(&rfree[i], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) != n) {
+                if (++spins > SC_SPIN_MAX || failed) {
+                    failed = true;
+                    return;
+                }
+                __builtin_amdgcn_s_sleep(1);
             }
-            if (vw) rec[vw] = rows[j].vm;
+            uint64_t *e = ring + (uint64_t)i * rw;
+            if constexpr (S::kStatic && S::rec_words() % 2 == 0) {
+                uint64_t tmp[S::rec_words()];
+                tmp[0] = w;
+#pragma unroll
+                for (int u = 0; u < MAXU; u++)
+                    if (S::rec_pos(u)) tmp[S::rec_pos(u)] = row.v[u];
+                if (vw) tmp[vw] = row.vm;
+#pragma unroll
+                for (int k = 0; k < S::rec_words(); k += 2) *(u64x2 *)(e + k) = u64x2{tmp[k], tmp[k + 1]};
+            } else {
+                e[0] = w;
+#pragma unroll
+                for (int u = 0; u < MAXU; u++) {
+                    const int rpu = rec_pos_s<S>(rp, u);
+                    if (rpu) e[rpu] = row.v[u];
+                }
+                if (vw) e[vw] = row.vm;
+            }
+            rpos[i] = pos;
+            __hip_atomic_store(&rpub[i], n + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+        };
+        for_rows_pipelined<S::kStatic ? AGG_ROWS : 1, SC_PROD>(sh, b, r0, r1, sh.nu(), threadIdx.x, scatter_row);
+        if (failed) __hip_atomic_fetch_or(&T.plan->overflow, OVF_RING, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (lane == 0) __hip_atomic_fetch_add(&rctl[1], 64u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+    } else {
+        // ---- consumers: entries c, c + SC_CONS, ... in order, to their partition offsets
+        for (uint32_t n = threadIdx.x - SC_PROD;; n += SC_CONS) {
+            const uint32_t i = n & (NR - 1);
+            bool have = true;
+            uint32_t spins = 0;
+            while (__hip_atomic_load(&rpub[i], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) != n + 1) {
+                // producers done: every reserved entry is published, so n is past the end
+                if (__hip_atomic_load(&rctl[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) == (uint32_t)SC_PROD &&
+                    n >= __hip_atomic_load(&rctl[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) {
+                    have = false;
+                    break;
+                }
+                if (++spins > SC_SPIN_MAX) {
+                    __hip_atomic_fetch_or(&T.plan->overflow, OVF_RING, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    have = false;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+            if (!have) break;
+            const uint64_t *e = ring + (uint64_t)i * rw;
+            uint64_t *rec = rp.recs + (uint64_t)rpos[i] * rw;
+            if constexpr (S::kStatic && S::rec_words() % 2 == 0) {
+#pragma unroll
+                for (int k = 0; k < S::rec_words(); k += 2) gst((u64x2 *)(rec + k), *(const u64x2 *)(e + k));
+            } else {
+                for (uint32_t k = 0; k < rw; k++) gst(rec + k, e[k]);
+            }
+            __hip_atomic_store(&rfree[i], n + NR, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
         }
     }
     __syncthreads();
@@ -380,9 +551,9 @@ __global__ __launch_bounds__(RP_BLOCK) void k_rp_scatter(const DevCtx *__restric
 }
 
 // One workgroup per partition: all of the partition's groups in LDS.
-template <class SH, int KMAX>
-__global__ __launch_bounds__(RA_BLOCK) void k_rp_agg(const DevCtx *__restrict__ cx, DevTable T,
-                                                     uint32_t S, uint32_t limit) {
+template <class SH, int KMAX, bool TOPN>
+__device__ __forceinline__ void rp_agg_body(const DevCtx *__restrict__ cx, DevTable T, uint32_t S,
+                                            uint32_t limit) {
     const DevQuery &q = cx->q;
     const RadixParams &rp = cx->rp;
     const DevOut &o = cx->o;
@@ -399,17 +570,31 @@ __global__ __launch_bounds__(RA_BLOCK) void k_rp_agg(const DevCtx *__restrict__ 
     for (uint64_t base = r0; base < r1; base += (uint64_t)RA_BLOCK * RA_ROWS) {
         Row rows[RA_ROWS];
         uint64_t keys[RA_ROWS];
+        // clamped to the partition's last record: every load is unconditional
 #pragma unroll
         for (int j = 0; j < RA_ROWS; j++) {
-            const uint64_t r = base + (uint64_t)j * RA_BLOCK + threadIdx.x;
-            if (r >= r1) continue;
+            const uint64_t r = min(base + (uint64_t)j * RA_BLOCK + threadIdx.x, r1 - 1);
             const uint64_t *rec = rp.recs + r * rw;
-            keys[j] = rec[0];
-            rows[j].vm = vw ? (uint32_t)rec[vw] : 0xFFFFFFFFu;
+            if constexpr (SH::kStatic && SH::rec_words() % 2 == 0) {
+                uint64_t tmp[SH::rec_words()];
 #pragma unroll
-            for (int u = 0; u < MAXU; u++) {
-                const int rpu = rec_pos_s<SH>(rp, u);
-                rows[j].v[u] = rpu ? rec[rpu] : 0;
+                for (int k = 0; k < SH::rec_words(); k += 2) {
+                    const u64x2 v2 = gld((const u64x2 *)(rec + k));
+                    tmp[k] = v2.x;
+                    tmp[k + 1] = v2.y;
+                }
+                keys[j] = tmp[0];
+                rows[j].vm = vw ? (uint32_t)tmp[vw] : 0xFFFFFFFFu;
+#pragma unroll
+                for (int u = 0; u < MAXU; u++) rows[j].v[u] = SH::rec_pos(u) ? tmp[SH::rec_pos(u)] : 0;
+            } else {
+                keys[j] = gld(rec);
+                rows[j].vm = vw ? (uint32_t)gld(rec + vw) : 0xFFFFFFFFu;
+#pragma unroll
+                for (int u = 0; u < MAXU; u++) {
+                    const int rpu = rec_pos_s<SH>(rp, u);
+                    rows[j].v[u] = rpu ? gld(rec + rpu) : 0;
+                }
             }
         }
 #pragma unroll
@@ -428,30 +613,45 @@ __global__ __launch_bounds__(RA_BLOCK) void k_rp_agg(const DevCtx *__restrict__ 
         }
     }
     __syncthreads();
-    if (!t.enabled) {
+    if constexpr (!TOPN) {
         uint32_t mine = 0;
         for (uint32_t i = threadIdx.x; i < S; i += blockDim.x) mine += lt.key[i] != SENTINEL;
         uint64_t row = block_reserve(mine, &T.plan->groups);
         for (uint32_t i = threadIdx.x; i < S; i += blockDim.x)
             if (lt.key[i] != SENTINEL) write_group(o, lt.key[i], lt.acc + i, S, false, row++);
-        return;
+    } else {
+        __shared__ uint32_t s_win[KMAX];
+        TopList<KMAX> L;
+        L.init();
+        for (uint32_t i = threadIdx.x; i < S; i += blockDim.x) {
+            if (lt.key[i] == SENTINEL) continue;
+            uint64_t ord, tie;
+            group_order(t, lt.key[i], lt.acc + i, S, false, ord, tie);
+            L.offer(ord, tie, i, t.k);
+        }
+        block_select<KMAX>(L, t.k, s_win);
+        topn_push(t, T.plan, s_win,
+                  [&](uint32_t i, uint64_t &key, const uint64_t *&acc, uint32_t &stride, bool &nk) {
+                      key = lt.key[i];
+                      acc = lt.acc + i;
+                      stride = S;
+                      nk = false;
+                  });
     }
-    __shared__ uint32_t s_win[KMAX];
-    TopList<KMAX> L;
-    L.init();
-    for (uint32_t i = threadIdx.x; i < S; i += blockDim.x) {
-        if (lt.key[i] == SENTINEL) continue;
-        uint64_t ord, tie;
-        group_order(t, lt.key[i], lt.acc + i, S, false, ord, tie);
-        L.offer(ord, tie, i, t.k);
-    }
-    block_select<KMAX>(L, t.k, s_win);
-    topn_push(t, T.plan, s_win, [&](uint32_t i, uint64_t &key, const uint64_t *&acc, uint32_t &stride, bool &nk) {
-        key = lt.key[i];
-        acc = lt.acc + i;
-        stride = S;
-        nk = false;
-    });
+}
+
+// Full result: two workgroups per CU (LDS tables sized for it).
+template <class SH>
+__global__ __launch_bounds__(RA_BLOCK) __attribute__((amdgpu_waves_per_eu(8))) void k_rp_agg(
+    const DevCtx *__restrict__ cx, DevTable T, uint32_t S, uint32_t limit) {
+    rp_agg_body<SH, 1, false>(cx, T, S, limit);
+}
+
+// TopN: K candidates per partition.
+template <class SH, int KMAX>
+__global__ __launch_bounds__(RA_BLOCK) void k_rp_agg_topn(const DevCtx *__restrict__ cx, DevTable T, uint32_t S,
+                                                          uint32_t limit) {
+    rp_agg_body<SH, KMAX, true>(cx, T, S, limit);
 }
 
 // Folds the heavy partial states over workgroups: block = 64 heavy keys x 4
@@ -484,7 +684,7 @@ __global__ __launch_bounds__(256) void k_heavy_fold(const DevCtx *__restrict__ c
         const uint64_t stride = (uint64_t)q.nacc * rp.hmax;
         const uint64_t *src = rp.hpart + (uint64_t)a * rp.hmax + d;
         uint32_t w = ph;
-        for (; w + 12 < rp.W; w += 16) {
+        for (; w + 12 < rp.W3; w += 16) {
             uint64_t y0 = src[(uint64_t)w * stride], y1 = src[(uint64_t)(w + 4) * stride];
             uint64_t y2 = src[(uint64_t)(w + 8) * stride], y3 = src[(uint64_t)(w + 12) * stride];
             x = fold_acc(op, x, y0);
@@ -492,7 +692,7 @@ __global__ __launch_bounds__(256) void k_heavy_fold(const DevCtx *__restrict__ c
             x = fold_acc(op, x, y2);
             x = fold_acc(op, x, y3);
         }
-        for (; w < rp.W; w += 4) x = fold_acc(op, x, src[(uint64_t)w * stride]);
+        for (; w < rp.W3; w += 4) x = fold_acc(op, x, src[(uint64_t)w * stride]);
     }
     s_part[ph][dl] = x;
     __syncthreads();
