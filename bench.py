@@ -11,12 +11,18 @@ A step is one full pass of the operator over the resident input: Init
 compaction). Inputs sit in HBM before the timed region; results stay in HBM
 (the host copy is not part of the step).
 
-Prints one JSON line. `roofline` is for the dominant kernel (k_aggregate):
-algorithmic bytes = 28 B/row of input read once, over its average
-HIP-event duration on the operator's stream. `cpu_baseline` times the
-restated reference CPU path (oracle/oracle.c orc_port_groupby, the
-per-thread-map path of physical_planner.cpp:7613-7931) with the reference's
-own 8-thread cap on a bounded prefix of the same data.
+Prints one JSON line. `roofline` is for the dominant kernel of the radix
+strategy, k_rp_scatter (the one pass that reads every input row): algorithmic
+bytes = 28 B/row of input read once, over the kernel's average duration
+measured with HIP events on the operator's stream (plan_info ms_aggregate).
+`traffic` is the HBM bytes one launch moves, from rocprofv3 FETCH_SIZE and
+WRITE_SIZE, read from profiles/k_rp_scatter_highcard_100m_traffic.json.
+`cpu_baseline` times the restated reference CPU path (oracle/oracle.c
+orc_port_groupby, the per-thread-map path of physical_planner.cpp:7613-7931)
+with the reference's own 8-thread cap on the same 100M rows.
+
+`--copy-calibration` instead copies 1 GiB device to device three times with
+hipMemcpy: the reference point that scales the rocprofv3 byte counters.
 """
 import argparse
 import json
@@ -41,10 +47,14 @@ def main():
     ap.add_argument("--rows", type=int, default=100_000_000)
     ap.add_argument("--keys", type=int, default=10_000_000)
     ap.add_argument("--strategy", type=int, default=0)
-    ap.add_argument("--cpu-sample-rows", type=int, default=20_000_000)
+    ap.add_argument("--cpu-sample-rows", type=int, default=100_000_000)
     ap.add_argument("--cpu-threads", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (use --gpus 1)", file=sys.stderr)
         sys.exit(2)
@@ -90,10 +100,10 @@ def main():
           f"groups {groups}, plan {info}", file=sys.stderr, flush=True)
 
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", "traffic_highcard_100m.json")
-    if os.path.exists(tpath):
+    tpath = os.path.join(ROOT, "profiles", "k_rp_scatter_highcard_100m_traffic.json")
+    if os.path.exists(tpath) and info["strategy"] == 4 and args.rows == 100_000_000:
         with open(tpath) as f:
-            traffic = json.load(f).get("k_aggregate_bytes")
+            traffic = json.load(f).get("hbm_bytes_per_launch")
 
     cpu = None
     if not args.no_cpu:
@@ -120,13 +130,31 @@ def main():
                    "strategy": info["strategy"], "parallelism": "1 GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_aggregate", "bytes_per_launch": agg_bytes, "avg_ms": agg_avg},
+                     "kernel": "k_rp_scatter" if info["strategy"] == 4 else "k_aggregate",
+                     "bytes_per_launch": agg_bytes, "avg_ms": agg_avg},
         "pipeline": {"algorithmic_bytes": pipe_bytes, "device_ms": tot_avg, "GB/s": pipe_gbs,
                      "frac": pipe_gbs / HBM_PEAK_GBS},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
     op.close()
+
+
+def copy_calibration():
+    """1 GiB device-to-device copies through the HIP runtime (known bytes)."""
+    import ctypes as C
+    from slothdb_amd import gbagg as G
+    n = 1 << 30
+    a, b = C.c_void_p(), C.c_void_p()
+    G._check(G.lib().gbagg_device_alloc(n, C.byref(a)))
+    G._check(G.lib().gbagg_device_alloc(n, C.byref(b)))
+    hip = C.CDLL("libamdhip64.so")
+    for _ in range(3):
+        assert hip.hipMemcpy(b, a, C.c_size_t(n), 3) == 0
+    assert hip.hipDeviceSynchronize() == 0
+    G._check(G.lib().gbagg_device_free(a))
+    G._check(G.lib().gbagg_device_free(b))
+    print(json.dumps({"copy_calibration_bytes": n, "copies": 3}), flush=True)
 
 
 if __name__ == "__main__":

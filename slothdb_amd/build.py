@@ -2,6 +2,8 @@
 
   slothdb_amd/lib/libslothdb_gpu.so      the GROUP BY operator (HIP, gfx950) + C ABI
   slothdb_amd/lib/libslothdb_datagen.so  seeded synthetic inputs (host C++)
+  slothdb_amd/lib/libslothdb_gpu_op.so   C++20 PhysicalGpuHashAggregate over the C ABI
+  slothdb_amd/lib/test_operator          its C++ test program (tests/cpp/test_operator.cpp)
 
 Run as `python -m slothdb_amd.build` or through __graft_entry__.build().
 """
@@ -52,6 +54,23 @@ def build_datagen():
     return out
 
 
+def build_host():
+    """The C++20 operator wrapper and its test program, linked to the HIP library."""
+    os.makedirs(LIB, exist_ok=True)
+    host = os.path.join(HERE, "host")
+    so = os.path.join(LIB, "libslothdb_gpu_op.so")
+    _run(["g++", "-std=c++20", "-O2", "-Wall", "-fPIC", "-shared", "-o", so + ".tmp",
+          os.path.join(host, "gpu_hash_aggregate.cpp"), "-L" + LIB, "-lslothdb_gpu",
+          "-Wl,-rpath,$ORIGIN"])
+    os.replace(so + ".tmp", so)
+    exe = os.path.join(LIB, "test_operator")
+    _run(["g++", "-std=c++20", "-O2", "-Wall", "-o", exe + ".tmp",
+          os.path.join(ROOT, "tests", "cpp", "test_operator.cpp"), "-L" + LIB, "-lslothdb_gpu_op",
+          "-lslothdb_gpu", "-Wl,-rpath,$ORIGIN"])
+    os.replace(exe + ".tmp", exe)
+    return so
+
+
 def build_oracle():
     """The CPU oracle (test infrastructure) and, with the reference tree
     present, oracle/_ref built from the reference's own aggregator sources."""
@@ -63,6 +82,7 @@ def build_all():
     build_datagen()
     build_oracle()
     build_gpu()
+    build_host()
 
 
 if __name__ == "__main__":
