@@ -276,7 +276,7 @@ struct gbagg_handle {
     hipEvent_t ev[4] = {};
     std::vector<Batch> batches;
     std::vector<DevBuf> in_set_bufs;
-    DevBuf dplan, table, hll, blkcnt, cand_rec, cand_null;
+    DevBuf dplan, table, hll, blkcnt, cand_rec, cand_null, topn_lvl;
     DevBuf ghist, hllm, skeys, scnt, hkeys, hidx, hdense, hfinal, hist, hpart, base, ptotal, pstart, recs;
     std::vector<DevBuf> out_data, out_valid;
     DevPlan *hplan = nullptr;  // pinned
@@ -860,10 +860,22 @@ void emit_table(gbagg_handle *h, const DevTable &T, uint64_t cap) {
 
 void topn_final(gbagg_handle *h) {
     if (!h->topn_device) return;
-    if (h->topn_k <= 16)
-        hipLaunchKernelGGL(k_topn_final<16>, dim3(1), dim3(1024), 0, h->stream, d_t(h), h->dplan.as<DevPlan>(), d_o(h));
-    else
-        hipLaunchKernelGGL(k_topn_final<32>, dim3(1), dim3(1024), 0, h->stream, d_t(h), h->dplan.as<DevPlan>(), d_o(h));
+    // level-1 blocks cover the whole candidate buffer; the level-2 block
+    // holds their lists (nlvl * KMAX <= 1024 * TOPN_NI2)
+    const uint32_t nlvl = (h->cand_cap + TOPN_L1 - 1) / TOPN_L1;
+    const uint32_t kmax = h->topn_k <= 16 ? 16 : 32;
+    if ((uint64_t)nlvl * kmax > 1024ull * TOPN_NI2) fail(GBAGG_ERROR, "TopN candidate buffer too large");
+    h->topn_lvl.ensure((size_t)nlvl * kmax * 4);
+    uint32_t *lvl = h->topn_lvl.as<uint32_t>();
+    if (kmax == 16) {
+        hipLaunchKernelGGL(k_topn_level1<16>, dim3(nlvl), dim3(1024), 0, h->stream, d_t(h), h->dplan.as<DevPlan>(), lvl);
+        hipLaunchKernelGGL(k_topn_final<16>, dim3(1), dim3(1024), 0, h->stream, d_t(h), h->dplan.as<DevPlan>(), d_o(h),
+                           (const uint32_t *)lvl, nlvl);
+    } else {
+        hipLaunchKernelGGL(k_topn_level1<32>, dim3(nlvl), dim3(1024), 0, h->stream, d_t(h), h->dplan.as<DevPlan>(), lvl);
+        hipLaunchKernelGGL(k_topn_final<32>, dim3(1), dim3(1024), 0, h->stream, d_t(h), h->dplan.as<DevPlan>(), d_o(h),
+                           (const uint32_t *)lvl, nlvl);
+    }
 }
 
 void ensure_table(gbagg_handle *h, uint64_t cap) {
@@ -999,6 +1011,7 @@ void run_lds_hash(gbagg_handle *h, Timer &tm, bool global_only) {
         topn_final(h);
         tm.mark();  // emit end
         read_plan(h);
+        if (h->hplan->overflow & 2) fail(GBAGG_ERROR, "TopN candidate buffer overflow");
         if (!(h->hplan->overflow & 1)) break;
         if (cap >= h->cap_max) fail(GBAGG_ERROR, "global hash table overflow at maximum size");
         cap = std::min<uint64_t>(h->cap_max, cap * 4);
@@ -1086,7 +1099,7 @@ void run_radix(gbagg_handle *h, Timer &tm) {
         const uint64_t heavy = h->hplan->heavy_count;
         const uint64_t cold = est > heavy ? est - heavy : 1;
         const size_t k4_budget = std::min<size_t>(di.lds_per_block, (size_t)h->k4_lds_kb * 1024);
-        const uint32_t S = lds_slots_exact(k4_budget, nacc, 8192);
+        const uint32_t S = lds_slots_exact(k4_budget, nacc, RA_SMAX);
         uint32_t pbits = 0;
         while (pbits < rp.bucket_bits && (double)cold / (double)(1u << pbits) > 0.5 * S) pbits++;
         if (h->force_part_bits >= 0) pbits = std::min<uint32_t>((uint32_t)h->force_part_bits, rp.bucket_bits);
@@ -1127,6 +1140,9 @@ void run_radix(gbagg_handle *h, Timer &tm) {
         const uint32_t hgrid = h->topn_device ? 1 : std::max<uint32_t>(1, (rp.hmax + 255) / 256);
         const dim3 fgrid(std::max<uint32_t>(1, (rp.hmax + 63) / 64), (uint32_t)nacc);
         if (rp.hmax) hipLaunchKernelGGL(k_heavy_fold, fgrid, dim3(256), 0, s, (const DevCtx *)dctx(h), T);
+        // partitions own the first P * K candidates; other producers append after them
+        if (h->topn_device)
+            hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, s, &T.plan->cand_count, P * h->topn_k);
         if (!h->topn_device) {
             allow_lds(h->kern->part_full, k4_lds);
             hipLaunchKernelGGL(h->kern->part_full, dim3(P), dim3(RA_BLOCK), k4_lds, s, (const DevCtx *)dctx(h), T, S, S * 7 / 8);
@@ -1147,6 +1163,7 @@ void run_radix(gbagg_handle *h, Timer &tm) {
         tm.mark();  // emit end
         read_plan(h);
         if (h->hplan->overflow & OVF_RING) fail(GBAGG_ERROR, "scatter record ring wait bound exceeded");
+        if (h->hplan->overflow & 2) fail(GBAGG_ERROR, "TopN candidate buffer overflow");
         if (!(h->hplan->overflow & 1)) break;
         if (cap >= h->cap_max) fail(GBAGG_ERROR, "fallback hash table overflow at maximum size");
         cap = std::min<uint64_t>(h->cap_max, cap * 16);
@@ -1176,8 +1193,10 @@ void finalize(gbagg_handle *h) {
     gbagg_strategy st = h->strategy;
     if (h->dq.key_mode == KM_NONE) st = GBAGG_STRATEGY_GLOBAL_HASH;
     if (h->topn_device) {
-        uint32_t bound = 4096 + 64 * h->topn_k * 8 + (uint32_t)std::min<uint64_t>(h->heavy_max, 65536);
-        h->cand_cap = std::max<uint32_t>(bound, 16384) * 4;
+        // producers of K candidates each: every radix partition (<= 8192), the
+        // heavy-key block, and the global-table blocks (<= 4 per CU)
+        const uint64_t producers = 8192 + 1 + (uint64_t)device_info().cus * 4;
+        h->cand_cap = (uint32_t)(producers * h->topn_k + 4096);
         h->cand_rec.ensure((size_t)h->cand_cap * (1 + h->dq.nacc) * 8);
         h->cand_null.ensure(h->cand_cap);
     }

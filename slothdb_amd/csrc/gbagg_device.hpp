@@ -89,6 +89,7 @@ struct DevPlan {
     uint64_t cold_rows;        // rows scattered to partitions
     uint32_t heavy_threshold;  // minimum sample count of a heavy key
     uint32_t pad;
+    uint64_t topn_lb;          // TopN: no group with a smaller order value can be in the top K
 };
 
 // The global table: (cap + 2) records of (1 + nacc) words: key, accumulators.

@@ -90,6 +90,9 @@ __global__ __launch_bounds__(1024) void k_hll_finish(const uint8_t *partial, uin
     }
 }
 
+// *p = v, in stream order.
+__global__ void k_set_u32(uint32_t *p, uint32_t v) { *p = v; }
+
 __global__ __launch_bounds__(256) void k_table_init(DevTable T, const DevQuery *__restrict__ qp) {
     const DevQuery &q = *qp;
     const uint64_t cap = T.plan->cap;
@@ -129,6 +132,19 @@ __device__ __forceinline__ int64_t g_find(const DevTable &T, uint64_t mask, uint
         pos = (pos + 1) & mask;
     }
     __hip_atomic_fetch_or(&T.plan->overflow, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return -1;
+}
+
+// Lookup without insert: the record of k, or -1 when k is absent.
+__device__ __forceinline__ int64_t g_lookup(const DevTable &T, uint64_t mask, uint64_t h, uint64_t k) {
+    uint64_t pos = h & mask;
+#pragma nounroll
+    for (uint32_t i = 0; i < T.max_probe; i++) {
+        const uint64_t cur = __hip_atomic_load(T.rec + pos * T.stride, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (cur == k) return (int64_t)pos;
+        if (cur == SENTINEL) return -1;
+        pos = (pos + 1) & mask;
+    }
     return -1;
 }
 
@@ -523,11 +539,15 @@ __device__ __forceinline__ void group_order(const DevTopn &t, uint64_t key, cons
     tie = null_key ? ~0ULL : ~key_order(t, key);  // the NULL key sorts first
 }
 
+// (oa, ta) larger first, then the smaller producer slot. Empty entries always
+// carry (0, 0, INVALID_SLOT), so the plain lexicographic order puts them after
+// every real entry. Evaluated without branches: a short-circuit chain here
+// compiles to divergent control flow whose mask bookkeeping dominates the
+// block-wide selections.
 __device__ __forceinline__ bool topn_better(uint64_t oa, uint64_t ta, uint32_t sa, uint64_t ob,
                                             uint64_t tb, uint32_t sb) {
-    if (sb == INVALID_SLOT) return sa != INVALID_SLOT;
-    if (sa == INVALID_SLOT) return false;
-    return oa > ob || (oa == ob && (ta > tb || (ta == tb && sa < sb)));
+    const bool o_gt = oa > ob, o_eq = oa == ob, t_gt = ta > tb, t_eq = ta == tb, s_lt = sa < sb;
+    return o_gt | (o_eq & (t_gt | (t_eq & s_lt)));
 }
 
 template <int KMAX>
@@ -600,18 +620,102 @@ __device__ void block_select(TopList<KMAX> &L, uint32_t K, uint32_t *s_win) {
     }
 }
 
+// One round of wave-wide argmax under topn_better: every lane ends with the
+// wave's best (o, t, s) and the lane that holds it.
+__device__ __forceinline__ void wave_best(uint64_t &bo, uint64_t &bt, uint32_t &bs, uint32_t &owner) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const uint64_t oo = __shfl_xor(bo, off), ot = __shfl_xor(bt, off);
+        const uint32_t os = __shfl_xor(bs, off), ow = __shfl_xor(owner, off);
+        if (topn_better(oo, ot, os, bo, bt, bs) || (!topn_better(bo, bt, bs, oo, ot, os) && ow < owner)) {
+            bo = oo; bt = ot; bs = os; owner = ow;
+        }
+    }
+}
+
+// Exact top-K (K <= KMAX) of NI candidate entries per thread, best first,
+// into s_win: each wave takes its K best by K rounds of wave argmax (no
+// block barrier), then wave 0 picks the block's K best from the waves' lists.
+// Entry j of a thread is (o[j], t[j], s[j]); s[j] = INVALID_SLOT is empty.
+template <int KMAX, int NI>
+__device__ void block_topk(const uint64_t (&o)[NI], const uint64_t (&t)[NI], const uint32_t (&s)[NI], uint32_t K,
+                           uint32_t *s_win) {
+    __shared__ uint64_t w_o[16 * KMAX], w_t[16 * KMAX];
+    __shared__ uint32_t w_s[16 * KMAX];
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = (blockDim.x + 63) >> 6;
+    uint32_t taken = 0;
+    for (uint32_t r = 0; r < K; r++) {
+        uint64_t bo = 0, bt = 0;
+        uint32_t bs = INVALID_SLOT;
+        int bj = -1;
+#pragma unroll
+        for (int j = 0; j < NI; j++)
+            if (!((taken >> j) & 1) && topn_better(o[j], t[j], s[j], bo, bt, bs)) {
+                bo = o[j]; bt = t[j]; bs = s[j]; bj = j;
+            }
+        uint32_t owner = lane;
+        wave_best(bo, bt, bs, owner);
+        if (owner == lane && bj >= 0 && bs != INVALID_SLOT) taken |= 1u << bj;
+        if (lane == 0) {
+            w_o[wave * KMAX + r] = bo;
+            w_t[wave * KMAX + r] = bt;
+            w_s[wave * KMAX + r] = bs;
+        }
+    }
+    __syncthreads();
+    if (wave == 0) {
+        constexpr int NE = (16 * KMAX + 63) / 64;
+        uint64_t eo[NE], et[NE];
+        uint32_t es[NE];
+#pragma unroll
+        for (int j = 0; j < NE; j++) {
+            const uint32_t e = lane + 64 * j;
+            const bool ok = e / KMAX < nwaves && e % KMAX < K;
+            eo[j] = ok ? w_o[e] : 0;
+            et[j] = ok ? w_t[e] : 0;
+            es[j] = ok ? w_s[e] : INVALID_SLOT;
+        }
+        uint32_t tk = 0;
+        for (uint32_t r = 0; r < K; r++) {
+            uint64_t bo = 0, bt = 0;
+            uint32_t bs = INVALID_SLOT;
+            int bj = -1;
+#pragma unroll
+            for (int j = 0; j < NE; j++)
+                if (!((tk >> j) & 1) && topn_better(eo[j], et[j], es[j], bo, bt, bs)) {
+                    bo = eo[j]; bt = et[j]; bs = es[j]; bj = j;
+                }
+            uint32_t owner = lane;
+            wave_best(bo, bt, bs, owner);
+            if (owner == lane && bj >= 0 && bs != INVALID_SLOT) tk |= 1u << bj;
+            if (lane == 0) s_win[r] = bs;
+        }
+    }
+    __syncthreads();
+}
+
 // Appends this block's winners (producer-local ids in s_win) to the candidate
 // buffer; fetch(id, key, acc_ptr, stride, null) describes a producer group.
+// With a fixed slot (fixed_slot >= 0) the block owns candidates
+// [fixed_slot * K, fixed_slot * K + K) and marks the ones it leaves unused as
+// empty (cand_null = CAND_EMPTY); thousands of producers then need no shared
+// counter, whose same-address atomics would serialise them.
+constexpr uint8_t CAND_EMPTY = 2;
 template <typename Fetch>
-__device__ void topn_push(const DevTopn &t, DevPlan *plan, const uint32_t *s_win, Fetch fetch) {
+__device__ void topn_push(const DevTopn &t, DevPlan *plan, const uint32_t *s_win, Fetch fetch,
+                          int64_t fixed_slot = -1) {
     __shared__ uint32_t s_nwin, s_base;
     if (threadIdx.x == 0) {
         uint32_t n = 0;
         for (uint32_t r = 0; r < t.k; r++) n += s_win[r] != INVALID_SLOT;
         s_nwin = n;
-        s_base = n ? __hip_atomic_fetch_add(&plan->cand_count, n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
+        if (fixed_slot >= 0) s_base = (uint32_t)fixed_slot * t.k;
+        else s_base = n ? __hip_atomic_fetch_add(&plan->cand_count, n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
     }
     __syncthreads();
+    if (fixed_slot >= 0)
+        for (uint32_t r = s_nwin + threadIdx.x; r < t.k; r += blockDim.x)
+            if (s_base + r < t.cand_cap) t.cand_null[s_base + r] = CAND_EMPTY;
     for (uint32_t r = threadIdx.x; r < s_nwin; r += blockDim.x) {
         uint32_t dst = s_base + r;
         if (dst >= t.cand_cap) {
@@ -742,25 +846,64 @@ __global__ __launch_bounds__(256) void k_topn_table(DevTable T, const DevTopn *_
     });
 }
 
-// The final TopN over every producer's candidates: one block of 1024.
+// The final TopN over every producer's candidates, in two levels.
+// Level 1: block b takes candidates [b * TOPN_L1, (b + 1) * TOPN_L1) and writes
+// the indexes of its K best to lvl[b * KMAX ...] (INVALID_SLOT when fewer).
+constexpr int TOPN_NI1 = 4;
+constexpr uint32_t TOPN_L1 = 1024 * TOPN_NI1;
+template <int KMAX>
+__global__ __launch_bounds__(1024) void k_topn_level1(const DevTopn *__restrict__ tp, const DevPlan *__restrict__ plan,
+                                                      uint32_t *lvl) {
+    const DevTopn &t = *tp;
+    __shared__ uint32_t s_win[KMAX];
+    const uint32_t total = min(plan->cand_count, t.cand_cap);
+    uint64_t o[TOPN_NI1], ti[TOPN_NI1];
+    uint32_t sl[TOPN_NI1];
+#pragma unroll
+    for (int j = 0; j < TOPN_NI1; j++) {
+        const uint32_t i = blockIdx.x * TOPN_L1 + j * 1024 + threadIdx.x;
+        o[j] = 0;
+        ti[j] = 0;
+        sl[j] = INVALID_SLOT;
+        if (i < total && t.cand_null[i] != CAND_EMPTY) {
+            const uint64_t *rec = t.cand_rec + (uint64_t)i * t.rec_words;
+            group_order(t, rec[0], rec + 1, 1, t.cand_null[i] != 0, o[j], ti[j]);
+            sl[j] = i;
+        }
+    }
+    block_topk<KMAX, TOPN_NI1>(o, ti, sl, t.k, s_win);
+    for (uint32_t r = threadIdx.x; r < KMAX; r += blockDim.x)
+        lvl[blockIdx.x * KMAX + r] = r < t.k ? s_win[r] : INVALID_SLOT;
+}
+
+// Level 2: one block over the level-1 winners (nlvl blocks' lists); writes the
+// result rows best first.
+constexpr int TOPN_NI2 = 4;
 template <int KMAX>
 __global__ __launch_bounds__(1024) void k_topn_final(const DevTopn *__restrict__ tp, DevPlan *plan,
-                                                     const DevOut *__restrict__ op) {
+                                                     const DevOut *__restrict__ op, const uint32_t *lvl,
+                                                     uint32_t nlvl) {
     const DevTopn &t = *tp;
     const DevOut &o = *op;
     __shared__ uint32_t s_win[KMAX];
-    TopList<KMAX> L;
-    L.init();
-    const uint32_t total = min(plan->cand_count, t.cand_cap);
-    for (uint32_t i = threadIdx.x; i < total; i += blockDim.x) {
-        const uint64_t *rec = t.cand_rec + (uint64_t)i * t.rec_words;
-        uint64_t ord, tie;
-        group_order(t, rec[0], rec + 1, 1, t.cand_null[i] != 0, ord, tie);
-        L.offer(ord, tie, i, t.k);
+    uint64_t ord[TOPN_NI2], tie[TOPN_NI2];
+    uint32_t sl[TOPN_NI2];
+#pragma unroll
+    for (int j = 0; j < TOPN_NI2; j++) {
+        const uint32_t e = j * 1024 + threadIdx.x;
+        ord[j] = 0;
+        tie[j] = 0;
+        sl[j] = INVALID_SLOT;
+        if (e < nlvl * KMAX && lvl[e] != INVALID_SLOT) {
+            const uint32_t i = lvl[e];
+            const uint64_t *rec = t.cand_rec + (uint64_t)i * t.rec_words;
+            group_order(t, rec[0], rec + 1, 1, t.cand_null[i] != 0, ord[j], tie[j]);
+            sl[j] = i;
+        }
     }
-    block_select<KMAX>(L, t.k, s_win);
+    block_topk<KMAX, TOPN_NI2>(ord, tie, sl, t.k, s_win);
     for (uint32_t r = threadIdx.x; r < t.k; r += blockDim.x) {
-        uint32_t i = s_win[r];
+        const uint32_t i = s_win[r];
         if (i == INVALID_SLOT) continue;
         const uint64_t *rec = t.cand_rec + (uint64_t)i * t.rec_words;
         write_group(o, rec[0], rec + 1, 1, t.cand_null[i] != 0, r);

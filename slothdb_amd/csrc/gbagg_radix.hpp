@@ -27,6 +27,8 @@ namespace gbagg {
 constexpr int RP_BLOCK = 1024;   // count / scatter workgroup: 16 waves share one LDS table
 constexpr int RA_BLOCK = 1024;   // per-partition aggregation workgroup
 constexpr int RA_ROWS = 4;       // records per thread in flight
+constexpr uint32_t RA_SMAX = 8192;  // largest per-partition LDS table
+constexpr int RA_TOPN_NI = (RA_SMAX + RA_BLOCK - 1) / RA_BLOCK;
 // Scatter workgroup roles: the first SC_PROD threads load and classify rows;
 // the other SC_CONS threads only write cold records to HBM, taking them from
 // an LDS ring. The vector-memory counter drains in issue order for loads and
@@ -563,6 +565,8 @@ __device__ __forceinline__ void rp_agg_body(const DevCtx *__restrict__ cx, DevTa
     const int vw = vmask_word_s<SH>(rp);
     extern __shared__ uint64_t smem[];
     LdsTable lt = lds_table_setup(smem, q, S, limit);
+    __shared__ uint32_t s_spill;  // some row of this partition went to the fallback table
+    if (threadIdx.x == 0) s_spill = 0;
     __syncthreads();
     const uint32_t p = blockIdx.x;
     const uint64_t r0 = rp.pstart[p], r1 = rp.pstart[p + 1];
@@ -607,12 +611,28 @@ __device__ __forceinline__ void rp_agg_body(const DevCtx *__restrict__ cx, DevTa
             if (slot >= 0) {
                 row_update_s<__HIP_MEMORY_SCOPE_WORKGROUP>(sh, rows[j], lt.acc + slot, lt.S);
             } else {
+                s_spill = 1;
                 int64_t g = g_find(T, mask, h, w);
                 if (g >= 0) row_update_s<__HIP_MEMORY_SCOPE_AGENT>(sh, rows[j], T.rec + (uint64_t)g * T.stride + 1, 1);
             }
         }
     }
     __syncthreads();
+    // A key can reach the fallback table from one row while another row of
+    // the same key, racing with the table filling up, claims an LDS slot. Such
+    // a key must leave the operator once: fold its LDS state into its
+    // fallback record, which the table emit produces, and drop the LDS copy.
+    if (s_spill) {
+        for (uint32_t i = threadIdx.x; i < S; i += blockDim.x) {
+            const uint64_t k = lt.key[i];
+            if (k == SENTINEL) continue;
+            const int64_t g = g_lookup(T, mask, hash64(k), k);
+            if (g < 0) continue;
+            g_merge(q, T, (uint64_t)g, lt.acc + i, S);
+            lt.key[i] = SENTINEL;
+        }
+        __syncthreads();
+    }
     if constexpr (!TOPN) {
         uint32_t mine = 0;
         for (uint32_t i = threadIdx.x; i < S; i += blockDim.x) mine += lt.key[i] != SENTINEL;
@@ -620,23 +640,72 @@ __device__ __forceinline__ void rp_agg_body(const DevCtx *__restrict__ cx, DevTa
         for (uint32_t i = threadIdx.x; i < S; i += blockDim.x)
             if (lt.key[i] != SENTINEL) write_group(o, lt.key[i], lt.acc + i, S, false, row++);
     } else {
+        // The partition's candidates are its groups whose order value reaches
+        // topn_lb, a lower bound of the query's K-th best value that other
+        // partitions raise as they finish (any partition's K-th best is one).
+        // Groups below it cannot be in the top K. When no more than K groups
+        // pass, all of them are candidates and no selection runs.
         __shared__ uint32_t s_win[KMAX];
-        TopList<KMAX> L;
-        L.init();
-        for (uint32_t i = threadIdx.x; i < S; i += blockDim.x) {
-            if (lt.key[i] == SENTINEL) continue;
-            uint64_t ord, tie;
-            group_order(t, lt.key[i], lt.acc + i, S, false, ord, tie);
-            L.offer(ord, tie, i, t.k);
+        __shared__ uint32_t s_nsurv, s_pos;
+        if (threadIdx.x == 0) {
+            s_nsurv = 0;
+            s_pos = 0;
         }
-        block_select<KMAX>(L, t.k, s_win);
-        topn_push(t, T.plan, s_win,
-                  [&](uint32_t i, uint64_t &key, const uint64_t *&acc, uint32_t &stride, bool &nk) {
-                      key = lt.key[i];
-                      acc = lt.acc + i;
-                      stride = S;
-                      nk = false;
-                  });
+        __syncthreads();
+        const uint64_t lb = __hip_atomic_load(&T.plan->topn_lb, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        // at most RA_TOPN_NI slots per thread (S <= 8192, RA_BLOCK threads)
+        uint64_t o[RA_TOPN_NI], ti[RA_TOPN_NI];
+        uint32_t sl[RA_TOPN_NI];
+        uint32_t mine = 0;
+#pragma unroll
+        for (int j = 0; j < RA_TOPN_NI; j++) {
+            const uint32_t i = threadIdx.x + j * RA_BLOCK;
+            uint64_t oj = 0, tj = 0;
+            sl[j] = INVALID_SLOT;
+            if (i < S && lt.key[i] != SENTINEL) {
+                group_order(t, lt.key[i], lt.acc + i, S, false, oj, tj);
+                if (oj >= lb) {
+                    sl[j] = i;
+                    mine++;
+                } else {
+                    oj = 0;
+                    tj = 0;
+                }
+            }
+            o[j] = oj;
+            ti[j] = tj;
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) mine += __shfl_xor(mine, off);
+        if ((threadIdx.x & 63) == 0 && mine)
+            __hip_atomic_fetch_add(&s_nsurv, mine, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        __syncthreads();
+        const uint32_t nsurv = s_nsurv;
+        if (nsurv <= t.k) {
+#pragma unroll
+            for (int j = 0; j < RA_TOPN_NI; j++)
+                if (sl[j] != INVALID_SLOT)
+                    s_win[__hip_atomic_fetch_add(&s_pos, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)] = sl[j];
+            for (uint32_t r = nsurv + threadIdx.x; r < KMAX; r += blockDim.x) s_win[r] = INVALID_SLOT;
+            __syncthreads();
+        } else {
+            block_topk<KMAX, RA_TOPN_NI>(o, ti, sl, t.k, s_win);
+            if (threadIdx.x == 0) {
+                const uint32_t kth = s_win[t.k - 1];
+                uint64_t ok, tk;
+                group_order(t, lt.key[kth], lt.acc + kth, S, false, ok, tk);
+                __hip_atomic_fetch_max(&T.plan->topn_lb, ok, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+        topn_push(
+            t, T.plan, s_win,
+            [&](uint32_t i, uint64_t &key, const uint64_t *&acc, uint32_t &stride, bool &nk) {
+                key = lt.key[i];
+                acc = lt.acc + i;
+                stride = S;
+                nk = false;
+            },
+            (int64_t)blockIdx.x);  // partition p owns candidates [p * K, p * K + K)
     }
 }
 

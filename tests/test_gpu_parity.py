@@ -265,6 +265,18 @@ def test_radix_composite_topn_and_full():
     assert_same_groups(res2.rows(), exp, 2, types)
 
 
+@pytest.mark.parametrize("k", [10, 32])
+def test_radix_topn_all_partitions_produce(k):
+    # 8192 partitions each push K candidates: the candidate buffer must hold
+    # every producer's list (a group dropped here would change the top K)
+    q, cols = datagen.composite_workload(n=2_000_000, dict_size=100_000, topn=k)
+    q.strategy = G.RADIX
+    res, info = G.run(q, cols, device=True, params={"part_bits": 13})
+    assert info["radix_bits"] == 13 and res.num_rows == k
+    exp, types = _exact(datagen.composite_query(topn=None), cols, {1: cols[1].dictionary})
+    assert_topn_tie_aware(res.rows(), exp, order_col=2, k=k, types=types)
+
+
 def test_radix_topn_sum_ascending():
     q, cols = datagen.highcard_workload(n=800_000, nkeys=400_000)
     q.strategy = G.RADIX
