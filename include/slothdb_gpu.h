@@ -273,6 +273,60 @@ gbagg_status gbagg_hash_aggregate_i32(const int32_t *keys, const int32_t *values
                                       int32_t *out_keys, int64_t *out_sums, int64_t *out_counts,
                                       uint64_t capacity, uint64_t *out_groups);
 
+/* ---- Batches: many GROUP BY queries in one submission --------------------
+ * For callers with many small queries, which would otherwise run the
+ * aggregate operator (physical_planner.cpp:6009-12627) once per query.
+ * Queries of the shape
+ *   one INTEGER/BIGINT group column, 1-4 aggregates among COUNT(*),
+ *   COUNT/SUM/MIN/MAX over INTEGER/BIGINT columns, columns declared
+ *   not nullable, no predicates, no TopN, no row limit
+ * run together: a per-query sketch sizes each hash table, then one persistent
+ * work-queue kernel aggregates every query's row tiles, all captured once in a
+ * HIP graph and replayed while the batch keeps the same queries and inputs.
+ * Any other query runs on the single-query operator inside the same call.
+ * Columns are device pointers (gbagg_consume_device rules) and must stay valid
+ * until the batch is destroyed or run on other inputs. */
+typedef struct gbagg_batch gbagg_batch;
+
+typedef struct {
+    const gbagg_query_desc *query;
+    const gbagg_column_view *columns; /* query->num_columns device columns */
+    uint64_t num_rows;
+} gbagg_batch_item;
+
+/* Launch shape of the batched kernels; gbagg_batch_autotune picks one. */
+typedef struct {
+    uint32_t tile_rows;  /* rows per work item */
+    uint32_t lds_slots;  /* LDS hash-table slots per workgroup */
+    uint32_t block;      /* threads per aggregation workgroup */
+    uint32_t wgs_per_cu; /* resident aggregation workgroups per CU */
+} gbagg_batch_tune;
+
+typedef struct {
+    uint32_t queries;        /* items of the last run */
+    uint32_t batched;        /* of which ran in the batched kernels */
+    uint32_t single;         /* of which ran on the single-query operator */
+    uint32_t graph_captures; /* graphs built so far */
+    uint32_t graph_replays;  /* graph launches so far */
+    double ms_device;        /* last run: batched graph, launch to completion (HIP events) */
+    double ms_single;        /* last run: single-query operator runs (host clock) */
+    gbagg_batch_tune tune;
+} gbagg_batch_info;
+
+gbagg_status gbagg_batch_create(gbagg_batch **out);
+gbagg_status gbagg_batch_set_tune(gbagg_batch *b, const gbagg_batch_tune *tune);
+/* "cap_pct": hash-table slots per estimated distinct key, in percent
+ * (default 150; a query whose table overflows re-runs on the single-query
+ * operator). */
+gbagg_status gbagg_batch_set_param(gbagg_batch *b, const char *name, int64_t value);
+/* Times each candidate launch shape on these items and keeps the fastest. */
+gbagg_status gbagg_batch_autotune(gbagg_batch *b, const gbagg_batch_item *items, uint32_t count);
+gbagg_status gbagg_batch_run(gbagg_batch *b, const gbagg_batch_item *items, uint32_t count);
+/* Result of item `index` of the last run; valid until the next run or destroy. */
+gbagg_status gbagg_batch_result(gbagg_batch *b, uint32_t index, gbagg_result_view *out);
+gbagg_status gbagg_batch_info_get(gbagg_batch *b, gbagg_batch_info *out);
+void gbagg_batch_destroy(gbagg_batch *b);
+
 /* Device memory helpers for callers without their own allocator. */
 gbagg_status gbagg_device_alloc(uint64_t bytes, void **out);
 gbagg_status gbagg_device_free(void *ptr);

@@ -18,6 +18,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -1571,5 +1572,409 @@ uint64_t gbagg_hash_u64(uint64_t x) {
     unsigned __int128 r = (unsigned __int128)x * HASH_C;
     return (uint64_t)r ^ (uint64_t)(r >> 64);
 }
+
+}  // extern "C"
+
+// =============================================================================
+// Batches (gbagg_batch_*): many queries, one captured graph of six kernels
+// =============================================================================
+#include "gbagg_batch.hpp"
+
+struct gbagg_batch {
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    BatchTune tune{65536, 2048, 512, 2};
+    uint32_t cap_pct = 150;  // table slots per estimated key (percent)
+    // the batched queries of the last run
+    std::vector<uint32_t> item_of;        // batched query j -> item index
+    std::vector<BatchQuery> hq;           // uploaded description
+    std::vector<BatchQuery> dq_back;      // device state read back after the run
+    std::vector<uint2> items;
+    std::vector<std::vector<gbagg_type>> out_types;  // per batched query: key, aggs
+    uint32_t nacc_max = 1;
+    DevBuf dq, ditems, dctl, keys, acc, out_key, out_acc, hll;
+    uint64_t pool_slots = 0;
+    // graph of the current shape
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    uint64_t graph_sig = 0;
+    uint32_t captures = 0, replays = 0;
+    // items on the single-query operator
+    std::vector<gbagg_handle *> single;   // per item: handle or null
+    // last run
+    uint32_t n_items = 0;
+    double ms_device = 0, ms_single = 0;
+    // host results of batched queries, per item (filled on demand)
+    std::vector<std::vector<std::vector<uint8_t>>> res_data;
+    std::vector<std::vector<std::vector<uint8_t>>> res_valid;
+    std::vector<std::vector<gbagg_result_column>> res_cols;
+    std::vector<int> res_ready;
+    std::vector<int32_t> batched_index;   // item -> batched query j, or -1
+
+    void drop_graph() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+        graph_sig = 0;
+    }
+    void drop_single() {
+        for (auto *h : single)
+            if (h) gbagg_destroy(h);
+        single.clear();
+    }
+    ~gbagg_batch() {
+        drop_graph();
+        drop_single();
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+// The query shape the batched kernels run; anything else goes to the
+// single-query operator.
+bool batchable(const gbagg_batch_item &it) {
+    const gbagg_query_desc *q = it.query;
+    if (!q || !it.columns || q->num_group_cols != 1 || q->num_predicates != 0 || q->topn_col >= 0 ||
+        q->row_limit != 0 || q->num_aggs < 1 || q->num_aggs > (uint32_t)BQ_MAXACC || !q->column_nullable ||
+        it.num_rows > 0x7FFFFFFFull)
+        return false;
+    auto intcol = [&](int32_t c) {
+        if (c < 0 || (uint32_t)c >= q->num_columns) return false;
+        const gbagg_type t = q->column_types[c];
+        return (t == GBAGG_TYPE_INTEGER || t == GBAGG_TYPE_BIGINT) && q->column_nullable[c] == 0 &&
+               it.columns[c].validity == nullptr && it.columns[c].type == t;
+    };
+    if (!intcol((int32_t)q->group_cols[0])) return false;
+    for (uint32_t a = 0; a < q->num_aggs; a++) {
+        const gbagg_agg &g = q->aggs[a];
+        if (g.kind == GBAGG_AGG_COUNT_STAR) continue;
+        if (g.kind == GBAGG_AGG_AVG || !intcol(g.column)) return false;
+    }
+    return true;
+}
+
+uint64_t batch_pool_bound(uint64_t rows, uint32_t cap_pct) {
+    // the estimate kernel's capacity formula with the estimate at its cap (all rows distinct)
+    uint64_t cap = (rows * cap_pct + 99) / 100 + 64;
+    return (cap + BQ_CHUNK - 1) / BQ_CHUNK * BQ_CHUNK;
+}
+
+uint64_t mix_sig(uint64_t h, uint64_t v) { return (h ^ v) * 0x100000001B3ull + (h >> 29); }
+
+// Builds the batched query list, buffers and (when the shape changed) the graph.
+void batch_prepare(gbagg_batch *b, const gbagg_batch_item *items, uint32_t count) {
+    const DeviceInfo &di = device_info();
+    if (!b->stream) {
+        HIPCHK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+        HIPCHK(hipEventCreate(&b->ev0));
+        HIPCHK(hipEventCreate(&b->ev1));
+    }
+    b->hq.clear();
+    b->item_of.clear();
+    b->out_types.clear();
+    b->items.clear();
+    b->batched_index.assign(count, -1);
+    b->nacc_max = 1;
+    uint64_t pool = 0, sig = 0xcbf29ce484222325ull;
+    for (uint32_t i = 0; i < count; i++) {
+        if (!batchable(items[i])) continue;
+        const gbagg_query_desc *q = items[i].query;
+        BatchQuery bq{};
+        const uint32_t kc = q->group_cols[0];
+        bq.key = items[i].columns[kc].data;
+        bq.key64 = q->column_types[kc] == GBAGG_TYPE_BIGINT;
+        bq.rows = items[i].num_rows;
+        bq.nacc = q->num_aggs;
+        std::vector<gbagg_type> types{q->column_types[kc]};
+        for (uint32_t a = 0; a < q->num_aggs; a++) {
+            const gbagg_agg &g = q->aggs[a];
+            int op = ACC_CNT_STAR;
+            gbagg_type t = GBAGG_TYPE_BIGINT;
+            if (g.kind == GBAGG_AGG_SUM) op = ACC_SUM_I;
+            if (g.kind == GBAGG_AGG_MIN) op = ACC_MIN_I;
+            if (g.kind == GBAGG_AGG_MAX) op = ACC_MAX_I;
+            if (op != ACC_CNT_STAR) {
+                bq.val[a] = items[i].columns[g.column].data;
+                if (q->column_types[g.column] == GBAGG_TYPE_BIGINT) bq.val64 |= 1u << a;
+                if (g.kind != GBAGG_AGG_SUM) t = q->column_types[g.column];
+            }
+            bq.ops |= (uint32_t)op << (4 * a);
+            types.push_back(t);
+        }
+        b->nacc_max = std::max<uint32_t>(b->nacc_max, bq.nacc);
+        const uint32_t j = (uint32_t)b->hq.size();
+        b->batched_index[i] = (int32_t)j;
+        b->item_of.push_back(i);
+        b->out_types.push_back(types);
+        bq.item0 = (uint32_t)b->items.size();
+        for (uint64_t t = 0; t * b->tune.tile_rows < bq.rows; t++) b->items.push_back(make_uint2(j, (uint32_t)t));
+        bq.nitems = (uint32_t)b->items.size() - bq.item0;
+        pool += batch_pool_bound(bq.rows, b->cap_pct);
+        sig = mix_sig(sig, (uint64_t)(uintptr_t)bq.key);
+        sig = mix_sig(sig, bq.rows);
+        sig = mix_sig(sig, bq.ops | (uint64_t)bq.val64 << 32);
+        for (int a = 0; a < BQ_MAXACC; a++) sig = mix_sig(sig, (uint64_t)(uintptr_t)bq.val[a]);
+        b->hq.push_back(bq);
+    }
+    const BatchTune &tn = b->tune;
+    sig = mix_sig(sig, tn.tile_rows);
+    sig = mix_sig(sig, tn.lds_slots);
+    sig = mix_sig(sig, tn.block);
+    sig = mix_sig(sig, tn.wgs_per_cu);
+    sig = mix_sig(sig, b->cap_pct);
+    const uint32_t nq = (uint32_t)b->hq.size();
+    if (nq == 0) {
+        b->drop_graph();
+        return;
+    }
+    if (sig == b->graph_sig && b->exec) return;  // same queries, inputs and shape: replay
+    b->drop_graph();
+    pool = std::max<uint64_t>(pool, BQ_CHUNK);
+    b->pool_slots = pool;
+    b->dq.ensure(nq * sizeof(BatchQuery));
+    b->ditems.ensure(std::max<size_t>(1, b->items.size()) * sizeof(uint2));
+    b->dctl.ensure(sizeof(BatchCtl));
+    b->keys.ensure(pool * 8);
+    b->acc.ensure(pool * 8 * BQ_MAXACC);
+    b->out_key.ensure(pool * 8);
+    b->out_acc.ensure(pool * 8 * BQ_MAXACC);
+    b->hll.ensure(std::max<size_t>(1, b->items.size()) * BQ_HLL_M);
+    BatchCtl ctl{};
+    ctl.pool_slots = pool;
+    ctl.n_items = (uint32_t)b->items.size();
+    ctl.lds_slots = tn.lds_slots;
+    ctl.cap_pct = b->cap_pct;
+    HIPCHK(hipMemcpyAsync(b->dq.ptr, b->hq.data(), nq * sizeof(BatchQuery), hipMemcpyHostToDevice, b->stream));
+    if (!b->items.empty())
+        HIPCHK(hipMemcpyAsync(b->ditems.ptr, b->items.data(), b->items.size() * sizeof(uint2), hipMemcpyHostToDevice,
+                              b->stream));
+    HIPCHK(hipMemcpyAsync(b->dctl.ptr, &ctl, sizeof(ctl), hipMemcpyHostToDevice, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+
+    BatchPool bp{b->keys.as<uint64_t>(), b->acc.as<uint64_t>(), b->out_key.as<int64_t>(), b->out_acc.as<int64_t>()};
+    const size_t agg_lds = (size_t)tn.lds_slots * 8 * (1 + b->nacc_max);
+    if (agg_lds > di.lds_per_block) fail(GBAGG_INVALID, "batch tune: LDS table larger than the device allows");
+    allow_lds(k_b_agg, agg_lds);
+    const uint32_t chunks = (uint32_t)std::min<uint64_t>((pool + BQ_CHUNK - 1) / BQ_CHUNK, (uint64_t)di.cus * 16);
+    BatchQuery *dqp = b->dq.as<BatchQuery>();
+    BatchCtl *dctl = b->dctl.as<BatchCtl>();
+    // capture: every node is a kernel (no memset nodes)
+    HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    const uint32_t hgrid = (uint32_t)std::min<size_t>(std::max<size_t>(1, b->items.size()), (size_t)di.cus * 8);
+    hipLaunchKernelGGL(k_b_hll, dim3(hgrid), dim3(256), 0, b->stream, (const BatchQuery *)dqp, (const BatchCtl *)dctl,
+                       (const uint2 *)b->ditems.ptr, b->hll.as<uint8_t>(), tn);
+    hipLaunchKernelGGL(k_b_estimate, dim3(nq), dim3(256), 0, b->stream, dqp, (const BatchCtl *)dctl,
+                       (const uint8_t *)b->hll.as<uint8_t>());
+    hipLaunchKernelGGL(k_b_layout, dim3(1), dim3(1024), 0, b->stream, dqp, nq, dctl);
+    hipLaunchKernelGGL(k_b_init, dim3(chunks), dim3(BQ_CHUNK), 0, b->stream, (const BatchQuery *)dqp, nq,
+                       (const BatchCtl *)dctl, bp);
+    hipLaunchKernelGGL(k_b_agg, dim3((uint32_t)di.cus * tn.wgs_per_cu), dim3(tn.block), agg_lds, b->stream, dqp, dctl,
+                       (const uint2 *)b->ditems.ptr, bp, tn);
+    hipLaunchKernelGGL(k_b_compact, dim3(chunks), dim3(BQ_CHUNK), 0, b->stream, dqp, nq, (const BatchCtl *)dctl, bp);
+    HIPCHK(hipStreamEndCapture(b->stream, &b->graph));
+    HIPCHK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    b->graph_sig = sig;
+    b->captures++;
+}
+
+void batch_run(gbagg_batch *b, const gbagg_batch_item *items, uint32_t count) {
+    if (count && !items) fail(GBAGG_INVALID, "null items");
+    for (uint32_t i = 0; i < count; i++)
+        if (!items[i].query || (items[i].query->num_columns && !items[i].columns))
+            fail(GBAGG_INVALID, "batch item without query or columns");
+    b->drop_single();
+    b->res_ready.assign(count, 0);
+    b->res_data.assign(count, {});
+    b->res_valid.assign(count, {});
+    b->res_cols.assign(count, {});
+    b->n_items = count;
+    batch_prepare(b, items, count);
+    b->single.assign(count, nullptr);
+    b->ms_device = 0;
+    const uint32_t nq = (uint32_t)b->hq.size();
+    if (nq) {
+        HIPCHK(hipEventRecord(b->ev0, b->stream));
+        HIPCHK(hipGraphLaunch(b->exec, b->stream));
+        HIPCHK(hipEventRecord(b->ev1, b->stream));
+        b->replays++;
+        b->dq_back.resize(nq);
+        HIPCHK(hipMemcpyAsync(b->dq_back.data(), b->dq.ptr, nq * sizeof(BatchQuery), hipMemcpyDeviceToHost, b->stream));
+        HIPCHK(hipStreamSynchronize(b->stream));
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+        b->ms_device = ms;
+    }
+    // single-query operator: the queries outside the batched shape, and
+    // batched queries whose table overflowed
+    const auto t0 = std::chrono::steady_clock::now();
+    for (uint32_t i = 0; i < count; i++) {
+        const int32_t j = b->batched_index[i];
+        if (j >= 0 && !b->dq_back[j].overflow) continue;
+        gbagg_handle *h = nullptr;
+        gbagg_status st = gbagg_plan_create(items[i].query, &h);
+        if (st == GBAGG_OK) st = gbagg_consume_device(h, items[i].columns, items[i].num_rows);
+        if (st == GBAGG_OK) st = gbagg_finalize(h);
+        if (st != GBAGG_OK) {
+            std::string msg = std::string("batch item ") + std::to_string(i) + ": " + gbagg_last_error();
+            if (h) gbagg_destroy(h);
+            fail(st, msg);
+        }
+        b->single[i] = h;
+        if (j >= 0) b->batched_index[i] = -1;
+    }
+    b->ms_single = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+void batch_result(gbagg_batch *b, uint32_t i, gbagg_result_view *out) {
+    if (i >= b->n_items) fail(GBAGG_INVALID, "batch result index out of range");
+    if (b->single[i]) {
+        HIPCHK(hipSuccess);
+        gbagg_status st = gbagg_result(b->single[i], out);
+        if (st != GBAGG_OK) fail(st, gbagg_last_error());
+        return;
+    }
+    const int32_t j = b->batched_index[i];
+    if (j < 0) fail(GBAGG_ERROR, "batch item has no result");
+    if (!b->res_ready[i]) {
+        const BatchQuery &q = b->dq_back[j];
+        const uint64_t n = q.groups;
+        const auto &types = b->out_types[j];
+        auto &data = b->res_data[i];
+        auto &valid = b->res_valid[i];
+        data.assign(types.size(), {});
+        valid.assign(types.size(), std::vector<uint8_t>(n, 1));
+        std::vector<int64_t> tmp(n);
+        for (size_t c = 0; c < types.size(); c++) {
+            const int64_t *src = c == 0 ? b->out_key.as<int64_t>() + q.tbl_off
+                                        : b->out_acc.as<int64_t>() + (uint64_t)(c - 1) * b->pool_slots + q.tbl_off;
+            if (n) HIPCHK(hipMemcpyAsync(tmp.data(), src, n * 8, hipMemcpyDeviceToHost, b->stream));
+            HIPCHK(hipStreamSynchronize(b->stream));
+            if (types[c] == GBAGG_TYPE_INTEGER) {
+                data[c].resize(n * 4);
+                int32_t *d = (int32_t *)data[c].data();
+                for (uint64_t r = 0; r < n; r++) d[r] = (int32_t)tmp[r];
+            } else {
+                data[c].resize(n * 8);
+                if (n) std::memcpy(data[c].data(), tmp.data(), n * 8);
+            }
+        }
+        auto &cols = b->res_cols[i];
+        cols.assign(types.size(), gbagg_result_column{});
+        for (size_t c = 0; c < types.size(); c++) {
+            cols[c].type = types[c];
+            cols[c].data = data[c].data();
+            cols[c].validity = valid[c].data();
+        }
+        b->res_ready[i] = 1;
+    }
+    out->num_rows = b->dq_back[j].groups;
+    out->num_columns = (uint32_t)b->res_cols[i].size();
+    out->columns = b->res_cols[i].data();
+}
+
+}  // namespace
+
+extern "C" {
+
+gbagg_status gbagg_batch_create(gbagg_batch **out) {
+    return guarded([&] {
+        if (!out) fail(GBAGG_INVALID, "null out");
+        *out = new gbagg_batch();
+    });
+}
+
+gbagg_status gbagg_batch_set_tune(gbagg_batch *b, const gbagg_batch_tune *t) {
+    return guarded([&] {
+        if (!b || !t) fail(GBAGG_INVALID, "null argument");
+        const bool pow2 = t->lds_slots && !(t->lds_slots & (t->lds_slots - 1));
+        if (t->tile_rows < 1024 || t->tile_rows > (1u << 24) || !pow2 || t->lds_slots < 256 || t->lds_slots > 8192 ||
+            (t->block != 256 && t->block != 512 && t->block != 1024) || t->wgs_per_cu < 1 || t->wgs_per_cu > 8)
+            fail(GBAGG_INVALID, "batch tune out of range");
+        b->tune = BatchTune{t->tile_rows, t->lds_slots, t->block, t->wgs_per_cu};
+        b->drop_graph();
+    });
+}
+
+gbagg_status gbagg_batch_set_param(gbagg_batch *b, const char *name, int64_t value) {
+    return guarded([&] {
+        if (!b || !name) fail(GBAGG_INVALID, "null argument");
+        const std::string n(name);
+        if (n == "cap_pct") {
+            if (value < 1 || value > 1000) fail(GBAGG_INVALID, "cap_pct out of range [1, 1000]");
+            b->cap_pct = (uint32_t)value;
+        } else {
+            fail(GBAGG_INVALID, "unknown batch parameter: " + n);
+        }
+        b->drop_graph();
+    });
+}
+
+gbagg_status gbagg_batch_autotune(gbagg_batch *b, const gbagg_batch_item *items, uint32_t count) {
+    return guarded([&] {
+        if (!b) fail(GBAGG_INVALID, "null batch");
+        const DeviceInfo &di = device_info();
+        const BatchTune keep = b->tune;
+        BatchTune best = keep;
+        double best_ms = 1e30;
+        for (uint32_t tile : {32768u, 65536u, 131072u})
+            for (uint32_t slots : {1024u, 2048u, 4096u})
+                for (uint32_t block : {256u, 512u, 1024u})
+                    for (uint32_t wpc : {1u, 2u, 4u}) {
+                        if ((size_t)slots * 8 * (1 + BQ_MAXACC) > di.lds_per_block) continue;
+                        if ((size_t)block * wpc > 2048) continue;  // 32 waves per CU
+                        b->tune = BatchTune{tile, slots, block, wpc};
+                        b->drop_graph();
+                        batch_run(b, items, count);  // capture
+                        double ms = 1e30;
+                        for (int rep = 0; rep < 2; rep++) {
+                            batch_run(b, items, count);  // replays
+                            ms = std::min(ms, b->ms_device);
+                        }
+                        if (ms < best_ms) {
+                            best_ms = ms;
+                            best = b->tune;
+                        }
+                    }
+        b->tune = best;
+        b->drop_graph();
+    });
+}
+
+gbagg_status gbagg_batch_run(gbagg_batch *b, const gbagg_batch_item *items, uint32_t count) {
+    return guarded([&] {
+        if (!b) fail(GBAGG_INVALID, "null batch");
+        batch_run(b, items, count);
+    });
+}
+
+gbagg_status gbagg_batch_result(gbagg_batch *b, uint32_t index, gbagg_result_view *out) {
+    return guarded([&] {
+        if (!b || !out) fail(GBAGG_INVALID, "null argument");
+        batch_result(b, index, out);
+    });
+}
+
+gbagg_status gbagg_batch_info_get(gbagg_batch *b, gbagg_batch_info *out) {
+    return guarded([&] {
+        if (!b || !out) fail(GBAGG_INVALID, "null argument");
+        std::memset(out, 0, sizeof(*out));
+        out->queries = b->n_items;
+        for (uint32_t i = 0; i < b->n_items && i < b->single.size(); i++) {
+            if (b->single[i]) out->single++;
+            else if (b->batched_index[i] >= 0) out->batched++;
+        }
+        out->graph_captures = b->captures;
+        out->graph_replays = b->replays;
+        out->ms_device = b->ms_device;
+        out->ms_single = b->ms_single;
+        out->tune = gbagg_batch_tune{b->tune.tile_rows, b->tune.lds_slots, b->tune.block, b->tune.wgs_per_cu};
+    });
+}
+
+void gbagg_batch_destroy(gbagg_batch *b) { delete b; }
 
 }  // extern "C"

@@ -12,6 +12,7 @@ Workloads, with the sizes they are benchmarked at:
              from a 1M-string dictionary, c int32 uniform [0, 100)
 """
 import ctypes as C
+import math
 import os
 
 import numpy as np
@@ -164,3 +165,39 @@ def composite_workload(n=50_000_000, dict_size=1_000_000, seed=SEED, topn=10):
                 topn=(2, False, topn) if topn else None)
     cols = [G.Column(G.INTEGER, a), G.Column(G.VARCHAR, b, dictionary=dic), G.Column(G.INTEGER, c)]
     return q, cols
+
+
+# ---- the batched workload: many uneven COUNT/SUM queries -----------------
+
+def batch_specs(nq=512, seed=SEED, rows_lo=1_000, rows_hi=4_000_000, distinct_hi=1_000_000, s_hi=1.2):
+    """Per query (rows, distinct, zipf s, seed): rows log-uniform in
+    [rows_lo, rows_hi], distinct log-uniform in [1, distinct_hi] capped at
+    rows, Zipf s uniform in [0, s_hi]; counter-based, so any subset of the
+    queries can be regenerated alone."""
+    specs = []
+    for q in range(nq):
+        u = [rand(seed, 40 + j, q) / 2.0 ** 64 for j in range(3)]
+        rows = int(round(math.exp(math.log(rows_lo) + u[0] * (math.log(rows_hi) - math.log(rows_lo)))))
+        distinct = int(round(math.exp(u[1] * math.log(distinct_hi))))
+        distinct = max(1, min(distinct, rows))
+        s = u[2] * s_hi
+        specs.append((rows, distinct, s, (seed * 1_000_003 + q) & 0xFFFFFFFFFFFF))
+    return specs
+
+
+def batch_query():
+    return G.Query([G.INTEGER, G.BIGINT], [0], [(G.COUNT_STAR, None), (G.SUM, 1)], nullable=[False, False])
+
+
+def batch_workload(nq=512, seed=SEED, **kw):
+    """[(Query, [key int32 column, value int64 column])] for batch_specs."""
+    out = []
+    for rows, distinct, s, qseed in batch_specs(nq, seed, **kw):
+        k, v = query_i32(rows, distinct, s, qseed)
+        out.append((batch_query(), [G.Column(G.INTEGER, k), G.Column(G.BIGINT, v)]))
+    return out
+
+
+def rand(seed, stream, i):
+    """The generators' counter-based 64-bit value for (seed, stream, i)."""
+    return int(lib().sdg_rand(seed, stream, i))

@@ -98,6 +98,14 @@ _SIGS = {
     "gbagg_hash_aggregate_i32": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p,
                                            C.c_void_p, C.c_void_p, C.c_uint64,
                                            C.POINTER(C.c_uint64)]),
+    "gbagg_batch_create": (C.c_int, [C.POINTER(C.c_void_p)]),
+    "gbagg_batch_set_tune": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "gbagg_batch_set_param": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int64]),
+    "gbagg_batch_autotune": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32]),
+    "gbagg_batch_run": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32]),
+    "gbagg_batch_result": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p]),
+    "gbagg_batch_info_get": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "gbagg_batch_destroy": (None, [C.c_void_p]),
     "gbagg_device_alloc": (C.c_int, [C.c_uint64, C.POINTER(C.c_void_p)]),
     "gbagg_device_free": (C.c_int, [C.c_void_p]),
     "gbagg_memcpy_h2d": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64]),
@@ -333,6 +341,104 @@ class Result:
         return out
 
 
+def _result_from_view(rv):
+    """Copies a gbagg_result_view into a Result."""
+    n = rv.num_rows
+    cols, valid, types = [], [], []
+    for c in range(rv.num_columns):
+        rc = rv.columns[c]
+        types.append(rc.type)
+        v = np.ctypeslib.as_array(C.cast(rc.validity, C.POINTER(C.c_uint8)), (n,)).astype(bool) \
+            if n else np.zeros(0, bool)
+        if rc.type == VARCHAR:
+            off = np.ctypeslib.as_array(C.cast(rc.str_offsets, C.POINTER(C.c_uint32)), (n + 1,)) \
+                if n else np.zeros(1, np.uint32)
+            total = int(off[-1]) if n else 0
+            raw = C.string_at(rc.str_bytes, total) if total else b""
+            col = [raw[off[i]:off[i + 1]].decode() for i in range(n)]
+        else:
+            dt = _NP[rc.type]
+            col = np.ctypeslib.as_array(C.cast(rc.data, C.POINTER(np.ctypeslib.as_ctypes_type(dt))),
+                                        (n,)).copy() if n else np.zeros(0, dt)
+        cols.append(col)
+        valid.append(v)
+    return Result(cols, valid, types)
+
+
+class BatchItem(C.Structure):
+    _fields_ = [("query", C.c_void_p), ("columns", C.c_void_p), ("num_rows", C.c_uint64)]
+
+
+class BatchTune(C.Structure):
+    _fields_ = [("tile_rows", C.c_uint32), ("lds_slots", C.c_uint32), ("block", C.c_uint32),
+                ("wgs_per_cu", C.c_uint32)]
+
+
+class BatchInfo(C.Structure):
+    _fields_ = [("queries", C.c_uint32), ("batched", C.c_uint32), ("single", C.c_uint32),
+                ("graph_captures", C.c_uint32), ("graph_replays", C.c_uint32),
+                ("ms_device", C.c_double), ("ms_single", C.c_double), ("tune", BatchTune)]
+
+
+class GpuBatch:
+    """Many GROUP BY queries in one submission (gbagg_batch_*): each entry is
+    (Query, [DeviceColumn]); the columns must outlive the batch's runs."""
+
+    def __init__(self):
+        h = C.c_void_p()
+        _check(lib().gbagg_batch_create(C.byref(h)))
+        self._h = h
+        self._keep = []
+        self._items = None
+
+    def set_queries(self, entries):
+        keep, items = [], (BatchItem * max(1, len(entries)))()
+        for i, (q, dcols) in enumerate(entries):
+            desc, k = q.desc()
+            views, kv = _views(dcols, device=True)
+            keep += [desc, k, views, kv, dcols]
+            items[i].query = C.addressof(desc)
+            items[i].columns = C.addressof(views)
+            items[i].num_rows = dcols[0].n if dcols else 0
+        self._keep, self._items, self._n = keep, items, len(entries)
+
+    def set_tune(self, tile_rows, lds_slots, block, wgs_per_cu):
+        t = BatchTune(tile_rows, lds_slots, block, wgs_per_cu)
+        _check(lib().gbagg_batch_set_tune(self._h, C.byref(t)))
+
+    def set_param(self, name, value):
+        _check(lib().gbagg_batch_set_param(self._h, name.encode(), int(value)))
+
+    def autotune(self):
+        _check(lib().gbagg_batch_autotune(self._h, self._items, self._n))
+
+    def run(self):
+        _check(lib().gbagg_batch_run(self._h, self._items, self._n))
+
+    def result(self, i):
+        rv = ResultView()
+        _check(lib().gbagg_batch_result(self._h, i, C.byref(rv)))
+        return _result_from_view(rv)
+
+    def info(self):
+        bi = BatchInfo()
+        _check(lib().gbagg_batch_info_get(self._h, C.byref(bi)))
+        d = {k: getattr(bi, k) for k, _ in BatchInfo._fields_ if k != "tune"}
+        d["tune"] = {k: getattr(bi.tune, k) for k, _ in BatchTune._fields_}
+        return d
+
+    def close(self):
+        if self._h:
+            lib().gbagg_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class GpuHashAggregate:
     """The operator: plan once, consume batches, finalize, read the result."""
 
@@ -388,26 +494,7 @@ class GpuHashAggregate:
             return self._result
         rv = ResultView()
         _check(lib().gbagg_result(self._h, C.byref(rv)))
-        n = rv.num_rows
-        cols, valid, types = [], [], []
-        for c in range(rv.num_columns):
-            rc = rv.columns[c]
-            types.append(rc.type)
-            v = np.ctypeslib.as_array(C.cast(rc.validity, C.POINTER(C.c_uint8)), (n,)).astype(bool) \
-                if n else np.zeros(0, bool)
-            if rc.type == VARCHAR:
-                off = np.ctypeslib.as_array(C.cast(rc.str_offsets, C.POINTER(C.c_uint32)), (n + 1,)) \
-                    if n else np.zeros(1, np.uint32)
-                total = int(off[-1]) if n else 0
-                raw = C.string_at(rc.str_bytes, total) if total else b""
-                col = [raw[off[i]:off[i + 1]].decode() for i in range(n)]
-            else:
-                dt = _NP[rc.type]
-                col = np.ctypeslib.as_array(C.cast(rc.data, C.POINTER(np.ctypeslib.as_ctypes_type(dt))),
-                                            (n,)).copy() if n else np.zeros(0, dt)
-            cols.append(col)
-            valid.append(v)
-        self._result = Result(cols, valid, types)
+        self._result = _result_from_view(rv)
         return self._result
 
     def GetData(self):

@@ -1,0 +1,137 @@
+"""Batches of GROUP BY queries (gbagg_batch_*): every query's result equals
+the oracle bit for bit, graph replays equal the single-query operator's
+results, queries outside the batched shape and queries whose hash table
+overflows run on the single-query operator, and the autotuner keeps results
+exact."""
+import numpy as np
+import pytest
+
+from slothdb_amd import datagen, gbagg as G
+from oracle import oracle as O
+from tests._util import assert_same_groups, oracle_rows
+
+pytestmark = pytest.mark.gpu
+
+
+def _exact(q, cols):
+    c, v, t, ov = O.groupby_exact(q, cols)
+    assert not ov
+    return oracle_rows(c, v, t), t
+
+
+def _entries(specs_kw, nq, extra=()):
+    host = datagen.batch_workload(nq=nq, seed=7, **specs_kw) + list(extra)
+    dev = [(q, [G.DeviceColumn(c) for c in cols]) for q, cols in host]
+    return host, dev
+
+
+def _special_queries():
+    rng = np.random.default_rng(3)
+    out = []
+    # one row; one distinct key; every row its own key (BIGINT key, MIN/MAX)
+    out.append((datagen.batch_query(), [G.Column(G.INTEGER, np.array([-5], np.int32)),
+                                        G.Column(G.BIGINT, np.array([1 << 40], np.int64))]))
+    n = 50_000
+    out.append((datagen.batch_query(), [G.Column(G.INTEGER, np.full(n, 123, np.int32)),
+                                        G.Column(G.BIGINT, rng.integers(-1 << 50, 1 << 50, n))]))
+    k = rng.permutation(np.arange(-n, n, 2, dtype=np.int64) * 1_000_003)
+    q = G.Query([G.BIGINT, G.INTEGER], [0], [(G.COUNT_STAR, None), (G.MIN, 1), (G.MAX, 1), (G.SUM, 1)],
+                nullable=[False, False])
+    out.append((q, [G.Column(G.BIGINT, k), G.Column(G.INTEGER, rng.integers(-2**31, 2**31, len(k)).astype(np.int32))]))
+    # outside the batched shape: AVG, and a predicate
+    q_avg = G.Query([G.INTEGER, G.BIGINT], [0], [(G.AVG, 1), (G.COUNT_STAR, None)], nullable=[False, False])
+    kk, vv = datagen.query_i32(30_000, 500, 0.7, 99)
+    out.append((q_avg, [G.Column(G.INTEGER, kk), G.Column(G.BIGINT, vv)]))
+    q_pred = G.Query([G.INTEGER, G.BIGINT], [0], [(G.COUNT_STAR, None), (G.SUM, 1)],
+                     predicates=[(1, G.GT, 0)], nullable=[False, False])
+    out.append((q_pred, [G.Column(G.INTEGER, kk), G.Column(G.BIGINT, vv)]))
+    return out
+
+
+def test_batch_matches_oracle():
+    host, dev = _entries(dict(rows_hi=300_000, distinct_hi=200_000), 40, _special_queries())
+    b = G.GpuBatch()
+    b.set_queries(dev)
+    b.run()
+    info = b.info()
+    print(f"[batch] {info}", flush=True)
+    assert info["queries"] == len(host) and info["single"] == 2 and info["batched"] == len(host) - 2
+    for i, (q, cols) in enumerate(host):
+        exp, types = _exact(q, cols)
+        assert_same_groups(b.result(i).rows(), exp, 1, types)
+    b.close()
+
+
+def test_batch_graph_replays_match_single_operator():
+    host, dev = _entries(dict(rows_hi=200_000, distinct_hi=100_000), 24)
+    b = G.GpuBatch()
+    b.set_queries(dev)
+    eager = []
+    for q, dcols in dev:  # the single-query operator, eagerly
+        op = G.GpuHashAggregate(q)
+        op.Init()
+        op.consume_device(dcols)
+        op.finalize()
+        eager.append(sorted(op.result().rows()))
+        op.close()
+    for rep in range(3):
+        b.run()
+        G.lib().gbagg_device_synchronize()
+        for i in range(len(dev)):
+            assert sorted(b.result(i).rows()) == eager[i], f"replay {rep} query {i}"
+        # eager work between replays
+        op = G.GpuHashAggregate(dev[rep][0])
+        op.Init()
+        op.consume_device(dev[rep][1])
+        op.finalize()
+        assert sorted(op.result().rows()) == eager[rep]
+        op.close()
+    info = b.info()
+    assert info["graph_captures"] == 1 and info["graph_replays"] == 3
+    b.close()
+
+
+def test_batch_table_overflow_falls_back():
+    # tables at 1% of the estimate overflow: those queries re-run on the
+    # single-query operator, the small ones still fit their 64 spare slots
+    host, dev = _entries(dict(rows_hi=100_000, distinct_hi=50_000), 10)
+    b = G.GpuBatch()
+    b.set_param("cap_pct", 1)
+    b.set_queries(dev)
+    b.run()
+    info = b.info()
+    assert info["single"] >= 1
+    for i, (q, cols) in enumerate(host):
+        exp, types = _exact(q, cols)
+        assert_same_groups(b.result(i).rows(), exp, 1, types)
+    b.close()
+
+
+def test_batch_one_query_all_distinct():
+    n = 409_600
+    k = np.arange(n, dtype=np.int32) * 7 - 1_000_000
+    v = np.arange(n, dtype=np.int64)
+    q = datagen.batch_query()
+    cols = [G.Column(G.INTEGER, k), G.Column(G.BIGINT, v)]
+    b = G.GpuBatch()
+    b.set_queries([(q, [G.DeviceColumn(c) for c in cols])])
+    b.run()
+    assert b.info()["batched"] == 1
+    exp, types = _exact(q, cols)
+    assert_same_groups(b.result(0).rows(), exp, 1, types)
+    b.close()
+
+
+def test_batch_autotune_keeps_results_exact():
+    host, dev = _entries(dict(rows_hi=100_000, distinct_hi=50_000), 12)
+    b = G.GpuBatch()
+    b.set_queries(dev)
+    b.autotune()
+    b.run()
+    tune = b.info()["tune"]
+    print(f"[batch] tuned {tune}", flush=True)
+    assert tune["block"] in (256, 512, 1024) and tune["lds_slots"] in (1024, 2048, 4096)
+    for i, (q, cols) in enumerate(host):
+        exp, types = _exact(q, cols)
+        assert_same_groups(b.result(i).rows(), exp, 1, types)
+    b.close()
