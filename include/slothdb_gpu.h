@@ -226,6 +226,9 @@ typedef struct {
     double ms_partition;        /* per-partition aggregation + heavy fold */
     double ms_emit;             /* result compaction / TopN */
     char kernel_shape[32];      /* compiled query shape the row kernels ran, or "dynamic" */
+    double selector_distinct;   /* learned selector's distinct-group prediction (0 = not run) */
+    int32_t selector_used;      /* 1: the plan followed the selector; 0: default plan (or no selector) */
+    int32_t selector_path;      /* the selector's path: 0 LDS_HASH, 1 GLOBAL_HASH, 2 RADIX; -1 = not run */
 } gbagg_plan_info;
 
 typedef struct gbagg_handle gbagg_handle;
@@ -326,6 +329,38 @@ gbagg_status gbagg_batch_run(gbagg_batch *b, const gbagg_batch_item *items, uint
 gbagg_status gbagg_batch_result(gbagg_batch *b, uint32_t index, gbagg_result_view *out);
 gbagg_status gbagg_batch_info_get(gbagg_batch *b, gbagg_batch_info *out);
 void gbagg_batch_destroy(gbagg_batch *b);
+
+/* ---- Learned strategy selector ----------------------------------------------
+ * An MLP (32 -> 64 -> 64 -> 64 -> 4, FP32, ReLU) over features of the AUTO
+ * plan's row sample predicts the distinct-group count, as the correction
+ * log2((distinct + 1) / (default + 1)) / 4 to the sample's own estimate
+ * default = seen + f1 / sampled * rows, and the fastest path (LDS_HASH,
+ * GLOBAL_HASH, RADIX). With a selector set, an AUTO plan takes
+ * the predicted path and sizes its hash table from the predicted count, unless
+ * the prediction and the sample's own estimate differ by more than 4x; then
+ * the default plan runs. Results never depend on it.
+ * Parameter vector: GBAGG_SELECTOR_PARAMS floats = weights and biases of the
+ * four layers in order (W row-major [out][in], then b), then the input
+ * standardisation mean[32] and scale[32]. */
+#define GBAGG_SELECTOR_FEATURES 32
+#define GBAGG_SELECTOR_OUTPUTS 4
+#define GBAGG_SELECTOR_PARAMS 10756
+
+/* Per handle; params = NULL removes the selector. */
+gbagg_status gbagg_selector_set(gbagg_handle *h, const float *params, uint32_t num_params);
+/* Features (32 floats) and, with a selector, its raw outputs (4 floats) of
+ * the last finalize; features are kept when a selector is set or after
+ * gbagg_set_param(h, "features", 1). Either pointer may be NULL. */
+gbagg_status gbagg_plan_features(gbagg_handle *h, float *features, float *outputs);
+/* Trains on the GPU: minibatch (64) Adam over `epochs` passes in row order.
+ * x: n x 32 features; y: n x 2 (target correction as above, path class
+ * 0..2). params: initial parameters in (including the standardisation), the
+ * trained ones out. epoch_loss: `epochs` floats of mean loss, or NULL. */
+gbagg_status gbagg_selector_train(const float *x, const float *y, uint32_t n, uint32_t epochs, float lr,
+                                  float lambda, float *params, uint32_t num_params, float *epoch_loss);
+/* Forward pass on the GPU: out = n x 4 outputs for n x 32 features. */
+gbagg_status gbagg_selector_infer(const float *params, uint32_t num_params, const float *x, uint32_t n,
+                                  float *out);
 
 /* Device memory helpers for callers without their own allocator. */
 gbagg_status gbagg_device_alloc(uint64_t bytes, void **out);

@@ -29,8 +29,12 @@
 
 #include "../../include/slothdb_gpu.h"
 #include "gbagg_radix.hpp"
+#include "gbagg_selector.hpp"
 
 using namespace gbagg;
+
+static_assert(SEL_NPARAM == GBAGG_SELECTOR_PARAMS, "selector parameter layout");
+static_assert(SEL_IN == GBAGG_SELECTOR_FEATURES && SEL_OUT == GBAGG_SELECTOR_OUTPUTS, "selector shape");
 
 namespace {
 
@@ -297,6 +301,14 @@ struct gbagg_handle {
     uint64_t radix_fallback_cap = 65536;
     int32_t force_part_bits = -1;   // radix fanout override (-1 = from the estimate)
     uint32_t k4_lds_kb = 78;        // per-partition LDS table budget (two workgroups per CU)
+    // learned selector
+    std::vector<float> sel_params;  // empty: no selector
+    bool collect_features = false;
+    DevBuf dsel, dfeat, dpred;
+    float feat[SEL_IN] = {}, pred[SEL_OUT] = {};
+    bool feat_valid = false;
+    double sel_distinct = 0;
+    int32_t sel_used = 0, sel_path = -1;
     // last run
     gbagg_strategy used_strategy = GBAGG_STRATEGY_AUTO;
     uint32_t lds_slots = 0, agg_grid = 0, retries = 0, radix_bits = 0;
@@ -961,6 +973,27 @@ void run_sample(gbagg_handle *h) {
     hipLaunchKernelGGL(k_heavy_insert, dim3(sgrid), dim3(256), 0, s, d_rp(h), h->dplan.as<DevPlan>());
 }
 
+// Selector features of the sample just taken, and with a selector its
+// outputs; one small copy back.
+void run_features(gbagg_handle *h, bool infer) {
+    hipStream_t s = h->stream;
+    uint32_t key_kind = 0;
+    if (h->dq.key_mode == KM_PACK32) key_kind = 2;
+    else if (h->dq.key_mode == KM_SINGLE && h->dq.ukind[h->dq.key_u[0]] == CK_I64) key_kind = 1;
+    h->dfeat.ensure(SEL_IN * 4);
+    hipLaunchKernelGGL(k_sel_features, dim3(1), dim3(1024), 0, s, d_rp(h), (const DevPlan *)h->dplan.as<DevPlan>(),
+                       h->rows_total, key_kind, h->dfeat.as<float>());
+    if (infer) {
+        h->dpred.ensure(SEL_OUT * 4);
+        hipLaunchKernelGGL(k_sel_infer, dim3(1), dim3(SEL_H), 0, s, (const float *)h->dsel.as<float>(),
+                           (const float *)h->dfeat.as<float>(), h->dpred.as<float>());
+        HIPCHK(hipMemcpyAsync(h->pred, h->dpred.ptr, SEL_OUT * 4, hipMemcpyDeviceToHost, s));
+    }
+    HIPCHK(hipMemcpyAsync(h->feat, h->dfeat.ptr, SEL_IN * 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    h->feat_valid = true;
+}
+
 void run_lds_hash(gbagg_handle *h, Timer &tm, bool global_only) {
     const DeviceInfo &di = device_info();
     hipStream_t s = h->stream;
@@ -970,6 +1003,7 @@ void run_lds_hash(gbagg_handle *h, Timer &tm, bool global_only) {
     double seen = (double)hp.sample_distinct;
     double rest = hp.sample_rows ? (double)hp.sample_f1 / hp.sample_rows * (double)h->rows_total : 0;
     uint64_t want = (uint64_t)(2.0 * (seen + rest)) + 1024;
+    if (h->sel_used) want = (uint64_t)(2.0 * h->sel_distinct) + 1024;  // the learned count sizes the table
     if (h->dq.key_mode == KM_NONE) want = 1024;
     uint64_t cap = std::min<uint64_t>(h->cap_max, next_pow2(want));
     for (;;) {
@@ -1188,6 +1222,10 @@ void finalize(gbagg_handle *h) {
         h->out_valid[c].ensure(std::max<uint64_t>(1, h->out_capacity));
     }
     h->retries = 0;
+    h->sel_used = 0;
+    h->sel_path = -1;
+    h->sel_distinct = 0;
+    h->feat_valid = false;
     h->radix_bits = 0;
     Timer tm{h, {}};
     tm.mark();  // start
@@ -1213,12 +1251,34 @@ void finalize(gbagg_handle *h) {
         run_lds_hash(h, tm, true);
     } else {
         run_sample(h);
+        const bool sel = !h->sel_params.empty();
+        if (sel || h->collect_features) run_features(h, sel);
         tm.mark();  // sample end
         read_plan(h);
         if (st == GBAGG_STRATEGY_AUTO || st == GBAGG_STRATEGY_DIRECT) {
             const DevPlan &p = *h->hplan;
             bool low_card = p.sample_distinct <= 4096 && p.sample_f1 * 8 <= p.sample_distinct + 8;
             st = (low_card || h->rows_total <= 65536) ? GBAGG_STRATEGY_LDS_HASH : GBAGG_STRATEGY_RADIX;
+            if (sel) {
+                // the selector's path and count, unless they disagree with the
+                // sample's own estimate by more than 4x (then the default plan)
+                // output 0 is log2((distinct + 1) / (default + 1)) / 4, where
+                // default is the sample's own estimate (seen + unseen share)
+                const double seen = (double)p.sample_distinct;
+                const double rest = p.sample_rows ? (double)p.sample_f1 / p.sample_rows * (double)h->rows_total : 0;
+                const double dflt = std::max(1.0, std::min((double)h->rows_total, seen + rest));
+                const double pred = std::max(1.0, (dflt + 1.0) * std::exp2(4.0 * (double)h->pred[0]) - 1.0);
+                int path = 0;
+                for (int k = 1; k < 3; k++)
+                    if (h->pred[1 + k] > h->pred[1 + path]) path = k;
+                h->sel_distinct = pred;
+                h->sel_path = path;
+                if (pred <= 4.0 * dflt && pred * 4.0 >= dflt) {
+                    h->sel_used = 1;
+                    st = path == 0 ? GBAGG_STRATEGY_LDS_HASH : path == 1 ? GBAGG_STRATEGY_GLOBAL_HASH
+                                                                       : GBAGG_STRATEGY_RADIX;
+                }
+            }
         }
         if (st == GBAGG_STRATEGY_RADIX) run_radix(h, tm);
         else run_lds_hash(h, tm, st == GBAGG_STRATEGY_GLOBAL_HASH);
@@ -1475,6 +1535,9 @@ gbagg_status gbagg_plan_info_get(gbagg_handle *h, gbagg_plan_info *out) {
         out->ms_partition = h->ms_partition;
         out->ms_emit = h->ms_emit;
         std::snprintf(out->kernel_shape, sizeof(out->kernel_shape), "%s", h->kern ? h->kern->name : "");
+        out->selector_distinct = h->sel_distinct;
+        out->selector_used = h->sel_used;
+        out->selector_path = h->sel_path;
     });
 }
 
@@ -1490,6 +1553,7 @@ gbagg_status gbagg_set_param(gbagg_handle *h, const char *name, int64_t value) {
         else if (n == "fallback_cap") { in(1024, 1LL << 30); h->radix_fallback_cap = next_pow2((uint64_t)value); }
         else if (n == "part_bits") { in(-1, 13); h->force_part_bits = (int32_t)value; }
         else if (n == "lds_kb") { in(8, 160); h->k4_lds_kb = (uint32_t)value; }
+        else if (n == "features") { in(0, 1); h->collect_features = value != 0; }
         else if (n == "strategy") { in(0, 4); h->strategy = (gbagg_strategy)value; }
         else if (n == "static_shapes") { in(0, 1); h->allow_static = value != 0; pick_kernels(h); }
         else fail(GBAGG_INVALID, "unknown parameter " + n);
@@ -1543,6 +1607,82 @@ gbagg_status gbagg_hash_aggregate_i32(const int32_t *keys, const int32_t *values
     });
     gbagg_destroy(h);
     return st;
+}
+
+gbagg_status gbagg_selector_set(gbagg_handle *h, const float *params, uint32_t num_params) {
+    return guarded([&] {
+        if (!h) fail(GBAGG_INVALID, "null handle");
+        if (!params) {
+            h->sel_params.clear();
+            return;
+        }
+        if (num_params != (uint32_t)SEL_NPARAM) fail(GBAGG_INVALID, "selector parameter count differs from GBAGG_SELECTOR_PARAMS");
+        for (uint32_t i = 0; i < num_params; i++)
+            if (!std::isfinite(params[i])) fail(GBAGG_INVALID, "selector parameters are not finite");
+        for (int i = 0; i < SEL_IN; i++)
+            if (!(params[SEL_OFF_SIG + i] > 0)) fail(GBAGG_INVALID, "selector feature scale must be positive");
+        ensure_runtime(h);
+        h->sel_params.assign(params, params + num_params);
+        h->dsel.ensure((size_t)num_params * 4);
+        HIPCHK(hipMemcpy(h->dsel.ptr, params, (size_t)num_params * 4, hipMemcpyHostToDevice));
+    });
+}
+
+gbagg_status gbagg_plan_features(gbagg_handle *h, float *features, float *outputs) {
+    return guarded([&] {
+        if (!h) fail(GBAGG_INVALID, "null handle");
+        if (!h->feat_valid) fail(GBAGG_INVALID, "no features: finalize an AUTO plan with a selector or \"features\" set");
+        if (features) std::memcpy(features, h->feat, sizeof(h->feat));
+        if (outputs) {
+            if (h->sel_params.empty()) fail(GBAGG_INVALID, "no selector outputs without a selector");
+            std::memcpy(outputs, h->pred, sizeof(h->pred));
+        }
+    });
+}
+
+gbagg_status gbagg_selector_train(const float *x, const float *y, uint32_t n, uint32_t epochs, float lr, float lambda,
+                                  float *params, uint32_t num_params, float *epoch_loss) {
+    return guarded([&] {
+        if (!x || !y || !params || n == 0 || epochs == 0) fail(GBAGG_INVALID, "bad training arguments");
+        if (num_params != (uint32_t)SEL_NPARAM) fail(GBAGG_INVALID, "selector parameter count differs");
+        DevBuf dx, dy, dp, dm, dv, dl;
+        dx.ensure((size_t)n * SEL_IN * 4);
+        dy.ensure((size_t)n * 2 * 4);
+        dp.ensure((size_t)SEL_NPARAM * 4);
+        dm.ensure((size_t)SEL_NW * 4);
+        dv.ensure((size_t)SEL_NW * 4);
+        dl.ensure((size_t)epochs * 4);
+        HIPCHK(hipMemcpy(dx.ptr, x, (size_t)n * SEL_IN * 4, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(dy.ptr, y, (size_t)n * 2 * 4, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(dp.ptr, params, (size_t)SEL_NPARAM * 4, hipMemcpyHostToDevice));
+        std::vector<float> zero(SEL_NW, 0.f);
+        HIPCHK(hipMemcpy(dm.ptr, zero.data(), (size_t)SEL_NW * 4, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(dv.ptr, zero.data(), (size_t)SEL_NW * 4, hipMemcpyHostToDevice));
+        hipLaunchKernelGGL(k_sel_train, dim3(1), dim3(SEL_TT), 0, 0, dp.as<float>(), dm.as<float>(), dv.as<float>(),
+                           (const float *)dx.as<float>(), (const float *)dy.as<float>(), n, epochs, lr, lambda,
+                           dl.as<float>());
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpy(params, dp.ptr, (size_t)SEL_NPARAM * 4, hipMemcpyDeviceToHost));
+        if (epoch_loss) HIPCHK(hipMemcpy(epoch_loss, dl.ptr, (size_t)epochs * 4, hipMemcpyDeviceToHost));
+    });
+}
+
+gbagg_status gbagg_selector_infer(const float *params, uint32_t num_params, const float *x, uint32_t n, float *out) {
+    return guarded([&] {
+        if (!params || !x || !out) fail(GBAGG_INVALID, "null argument");
+        if (num_params != (uint32_t)SEL_NPARAM) fail(GBAGG_INVALID, "selector parameter count differs");
+        if (n == 0) return;
+        DevBuf dp, dx, dy;
+        dp.ensure((size_t)SEL_NPARAM * 4);
+        dx.ensure((size_t)n * SEL_IN * 4);
+        dy.ensure((size_t)n * SEL_OUT * 4);
+        HIPCHK(hipMemcpy(dp.ptr, params, (size_t)SEL_NPARAM * 4, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(dx.ptr, x, (size_t)n * SEL_IN * 4, hipMemcpyHostToDevice));
+        hipLaunchKernelGGL(k_sel_infer, dim3(n), dim3(SEL_H), 0, 0, (const float *)dp.as<float>(),
+                           (const float *)dx.as<float>(), dy.as<float>());
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpy(out, dy.ptr, (size_t)n * SEL_OUT * 4, hipMemcpyDeviceToHost));
+    });
 }
 
 gbagg_status gbagg_device_alloc(uint64_t bytes, void **out) {

@@ -1,0 +1,261 @@
+// gbagg_selector.hpp -- the learned strategy selector.
+//
+// A 32 -> 64 -> 64 -> 64 -> 4 ReLU MLP (FP32) reads features of the
+// heavy-hitter sample (the strided sample every AUTO plan takes first) and
+// predicts the distinct-group count, as log2((distinct + 1) / (default + 1)) / 4
+// over the sample's own estimate, and three path logits
+// (LDS_HASH, GLOBAL_HASH, RADIX). The operator uses it to pick the path and
+// to size the hash table; results never depend on it (every path is exact and
+// retries on table overflow), and a prediction that disagrees with the
+// sample's own estimate by more than 4x is ignored in favour of the default
+// plan.
+//
+//   k_sel_features  one block: frequency statistics of the sample table
+//   k_sel_infer     one block of 64 per input row: forward pass
+//   k_sel_train     one block: minibatch Adam over the whole training set,
+//                   forward and backward passes with activations in LDS
+#pragma once
+
+#include "gbagg_radix.hpp"
+
+namespace gbagg {
+
+constexpr int SEL_IN = 32, SEL_H = 64, SEL_OUT = 4;
+// parameter layout (floats): W1[H][IN] b1[H] W2[H][H] b2[H] W3[H][H] b3[H]
+// W4[OUT][H] b4[OUT], then the input standardisation mu[IN] sigma[IN]
+constexpr int SEL_OFF_W1 = 0;
+constexpr int SEL_OFF_B1 = SEL_OFF_W1 + SEL_H * SEL_IN;
+constexpr int SEL_OFF_W2 = SEL_OFF_B1 + SEL_H;
+constexpr int SEL_OFF_B2 = SEL_OFF_W2 + SEL_H * SEL_H;
+constexpr int SEL_OFF_W3 = SEL_OFF_B2 + SEL_H;
+constexpr int SEL_OFF_B3 = SEL_OFF_W3 + SEL_H * SEL_H;
+constexpr int SEL_OFF_W4 = SEL_OFF_B3 + SEL_H;
+constexpr int SEL_OFF_B4 = SEL_OFF_W4 + SEL_OUT * SEL_H;
+constexpr int SEL_NW = SEL_OFF_B4 + SEL_OUT;  // trained parameters
+constexpr int SEL_OFF_MU = SEL_NW;
+constexpr int SEL_OFF_SIG = SEL_OFF_MU + SEL_IN;
+constexpr int SEL_NPARAM = SEL_OFF_SIG + SEL_IN;
+
+// ---- features -----------------------------------------------------------
+// feat[0] log2(rows+1)/32        feat[1] log2(n+1)/32 (sampled rows)
+// feat[2] log2(d+1)/32 (distinct in the sample)   feat[3] d/n
+// feat[4..6] f1/d f2/d f3/d (keys seen once, twice, three times)
+// feat[7] log2(max+1)/32         feat[8] max/n
+// feat[9] sample entropy / log2(n)
+// feat[10..25] share of sampled keys with count in [2^j, 2^(j+1)), j = 0..15
+// feat[26] log2(GEE+1)/32, GEE = d - f1 + f1 sqrt(rows/n)
+// feat[27] log2(Chao1+1)/32, Chao1 = d + f1^2 / (2 max(f2, 1))
+// feat[28..30] key: one 4-byte column / one 8-byte column / two 4-byte columns
+// feat[31] log2(rows/n + 1)/32
+__global__ __launch_bounds__(1024) void k_sel_features(const RadixParams *__restrict__ rpp,
+                                                       const DevPlan *__restrict__ plan, uint64_t rows,
+                                                       uint32_t key_kind, float *feat) {
+    const RadixParams &rp = *rpp;
+    __shared__ unsigned long long s_d, s_f1, s_f2, s_f3, s_max;
+    __shared__ double s_ent;
+    __shared__ unsigned long long s_hist[16];
+    if (threadIdx.x == 0) {
+        s_d = s_f1 = s_f2 = s_f3 = s_max = 0;
+        s_ent = 0;
+    }
+    if (threadIdx.x < 16) s_hist[threadIdx.x] = 0;
+    __syncthreads();
+    unsigned long long d = 0, f1 = 0, f2 = 0, f3 = 0, mx = 0;
+    double ent = 0;
+    for (uint32_t i = threadIdx.x; i < rp.scap; i += blockDim.x) {
+        const uint32_t c = rp.scnt[i];
+        if (!c) continue;
+        d++;
+        f1 += c == 1;
+        f2 += c == 2;
+        f3 += c == 3;
+        mx = mx > c ? mx : c;
+        ent += (double)c * log2((double)c);
+        const uint32_t bin = min(15u, 31u - (uint32_t)__clz(c));
+        atomicAdd(&s_hist[bin], 1ull);
+    }
+    atomicAdd(&s_d, d);
+    atomicAdd(&s_f1, f1);
+    atomicAdd(&s_f2, f2);
+    atomicAdd(&s_f3, f3);
+    atomicMax(&s_max, mx);
+    atomicAdd(&s_ent, ent);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const double n = (double)max<uint64_t>(plan->sample_rows, 1), D = (double)max<unsigned long long>(s_d, 1);
+        const double R = (double)rows;
+        const double gee = (double)(s_d - s_f1) + (double)s_f1 * sqrt(R / n);
+        const double chao = (double)s_d + (double)s_f1 * (double)s_f1 / (2.0 * (double)max<unsigned long long>(s_f2, 1));
+        feat[0] = (float)(log2(R + 1) / 32);
+        feat[1] = (float)(log2(n + 1) / 32);
+        feat[2] = (float)(log2((double)s_d + 1) / 32);
+        feat[3] = (float)((double)s_d / n);
+        feat[4] = (float)(s_f1 / D);
+        feat[5] = (float)(s_f2 / D);
+        feat[6] = (float)(s_f3 / D);
+        feat[7] = (float)(log2((double)s_max + 1) / 32);
+        feat[8] = (float)((double)s_max / n);
+        feat[9] = (float)((log2(n) - s_ent / n) / max(1.0, log2(n)));
+        for (int j = 0; j < 16; j++) feat[10 + j] = (float)(s_hist[j] / D);
+        feat[26] = (float)(log2(gee + 1) / 32);
+        feat[27] = (float)(log2(chao + 1) / 32);
+        feat[28] = key_kind == 0;
+        feat[29] = key_kind == 1;
+        feat[30] = key_kind == 2;
+        feat[31] = (float)(log2(R / n + 1) / 32);
+    }
+}
+
+// ---- inference: one block of SEL_H threads per input row ---------------------
+__device__ __forceinline__ void sel_layer(const float *W, const float *b, const float *in, int nin, float *out,
+                                          bool relu) {
+    const int j = threadIdx.x;
+    float z = b[j];
+    for (int i = 0; i < nin; i++) z += W[j * nin + i] * in[i];
+    out[j] = relu ? fmaxf(z, 0.f) : z;
+}
+
+__global__ __launch_bounds__(SEL_H) void k_sel_infer(const float *__restrict__ P, const float *__restrict__ X,
+                                                     float *Y) {
+    __shared__ float x[SEL_IN], h1[SEL_H], h2[SEL_H], h3[SEL_H];
+    const float *xr = X + (uint64_t)blockIdx.x * SEL_IN;
+    if (threadIdx.x < SEL_IN)
+        x[threadIdx.x] = (xr[threadIdx.x] - P[SEL_OFF_MU + threadIdx.x]) / P[SEL_OFF_SIG + threadIdx.x];
+    __syncthreads();
+    sel_layer(P + SEL_OFF_W1, P + SEL_OFF_B1, x, SEL_IN, h1, true);
+    __syncthreads();
+    sel_layer(P + SEL_OFF_W2, P + SEL_OFF_B2, h1, SEL_H, h2, true);
+    __syncthreads();
+    sel_layer(P + SEL_OFF_W3, P + SEL_OFF_B3, h2, SEL_H, h3, true);
+    __syncthreads();
+    if (threadIdx.x < SEL_OUT) {
+        float z = P[SEL_OFF_B4 + threadIdx.x];
+        for (int i = 0; i < SEL_H; i++) z += P[SEL_OFF_W4 + threadIdx.x * SEL_H + i] * h3[i];
+        Y[(uint64_t)blockIdx.x * SEL_OUT + threadIdx.x] = z;
+    }
+}
+
+// ---- training -------------------------------------------------------------
+// Loss per row: (out0 - y_log)^2 + lambda * cross-entropy(out1..3, y_class),
+// averaged over a minibatch of SEL_B rows (the last one may be short). Rows
+// are visited in the given order every epoch. Adam (beta1 0.9, beta2 0.999,
+// eps 1e-8). M and V hold the moments (SEL_NW floats each, zeroed by the
+// caller). loss[e] receives the mean loss of epoch e.
+constexpr int SEL_B = 64;
+constexpr int SEL_TT = 256;
+
+__device__ __forceinline__ void sel_adam(float *P, float *M, float *V, int i, float g, float lr, float c1, float c2) {
+    const float m = 0.9f * M[i] + 0.1f * g;
+    const float v = 0.999f * V[i] + 0.001f * g * g;
+    M[i] = m;
+    V[i] = v;
+    P[i] -= lr * (m / c1) / (sqrtf(v / c2) + 1e-8f);
+}
+
+// dst[b][j] = act(sum_i W[j][i] src[b][i] + bias[j]) for nb rows
+__device__ __forceinline__ void sel_fwd(const float *W, const float *bias, const float *src, int nin, float *dst,
+                                        int nout, int nb, bool relu) {
+    for (int e = threadIdx.x; e < nb * nout; e += blockDim.x) {
+        const int b = e / nout, j = e % nout;
+        float z = bias[j];
+        for (int i = 0; i < nin; i++) z += W[j * nin + i] * src[b * nin + i];
+        dst[b * nout + j] = relu ? fmaxf(z, 0.f) : z;
+    }
+}
+
+// dsrc[b][i] = (sum_j W[j][i] dz[b][j]) * (src[b][i] > 0), the gradient at the
+// previous layer's pre-activation
+__device__ __forceinline__ void sel_bwd_act(const float *W, const float *dz, int nout, const float *src, int nin,
+                                            float *dsrc, int nb) {
+    for (int e = threadIdx.x; e < nb * nin; e += blockDim.x) {
+        const int b = e / nin, i = e % nin;
+        float g = 0;
+        for (int j = 0; j < nout; j++) g += W[j * nin + i] * dz[b * nout + j];
+        dsrc[b * nin + i] = src[b * nin + i] > 0.f ? g : 0.f;
+    }
+}
+
+// gradient of W (nout x nin) and bias, then the Adam step on them
+__device__ __forceinline__ void sel_bwd_w(float *P, float *M, float *V, int offW, int offB, const float *dz,
+                                          const float *src, int nin, int nout, int nb, float lr, float c1, float c2) {
+    for (int e = threadIdx.x; e < nout * nin + nout; e += blockDim.x) {
+        float g = 0;
+        if (e < nout * nin) {
+            const int j = e / nin, i = e % nin;
+            for (int b = 0; b < nb; b++) g += dz[b * nout + j] * src[b * nin + i];
+            sel_adam(P, M, V, offW + e, g, lr, c1, c2);
+        } else {
+            const int j = e - nout * nin;
+            for (int b = 0; b < nb; b++) g += dz[b * nout + j];
+            sel_adam(P, M, V, offB + j, g, lr, c1, c2);
+        }
+    }
+}
+
+__global__ __launch_bounds__(SEL_TT) void k_sel_train(float *P, float *M, float *V, const float *__restrict__ X,
+                                                      const float *__restrict__ Y, uint32_t n, uint32_t epochs,
+                                                      float lr, float lambda, float *loss) {
+    __shared__ float a0[SEL_B * SEL_IN], a1[SEL_B * SEL_H], a2[SEL_B * SEL_H], a3[SEL_B * SEL_H];
+    __shared__ float o[SEL_B * SEL_OUT], d3[SEL_B * SEL_H], d2[SEL_B * SEL_H];
+    __shared__ float s_loss;
+    uint32_t step = 0;
+    for (uint32_t ep = 0; ep < epochs; ep++) {
+        if (threadIdx.x == 0) s_loss = 0;
+        for (uint32_t r0 = 0; r0 < n; r0 += SEL_B) {
+            const int nb = (int)min<uint32_t>(SEL_B, n - r0);
+            step++;
+            const float c1 = 1.f - powf(0.9f, (float)step), c2 = 1.f - powf(0.999f, (float)step);
+            for (int e = threadIdx.x; e < nb * SEL_IN; e += blockDim.x) {
+                const int b = e / SEL_IN, i = e % SEL_IN;
+                a0[e] = (X[(uint64_t)(r0 + b) * SEL_IN + i] - P[SEL_OFF_MU + i]) / P[SEL_OFF_SIG + i];
+            }
+            __syncthreads();
+            sel_fwd(P + SEL_OFF_W1, P + SEL_OFF_B1, a0, SEL_IN, a1, SEL_H, nb, true);
+            __syncthreads();
+            sel_fwd(P + SEL_OFF_W2, P + SEL_OFF_B2, a1, SEL_H, a2, SEL_H, nb, true);
+            __syncthreads();
+            sel_fwd(P + SEL_OFF_W3, P + SEL_OFF_B3, a2, SEL_H, a3, SEL_H, nb, true);
+            __syncthreads();
+            sel_fwd(P + SEL_OFF_W4, P + SEL_OFF_B4, a3, SEL_H, o, SEL_OUT, nb, false);
+            __syncthreads();
+            // output gradient (in place over o) and the minibatch loss
+            if (threadIdx.x < (uint32_t)nb) {
+                const int b = threadIdx.x;
+                const float *y = Y + (uint64_t)(r0 + b) * 2;
+                const float diff = o[b * SEL_OUT] - y[0];
+                const int cls = (int)y[1];
+                const float l1 = o[b * SEL_OUT + 1], l2 = o[b * SEL_OUT + 2], l3 = o[b * SEL_OUT + 3];
+                const float mxl = fmaxf(l1, fmaxf(l2, l3));
+                const float e1 = expf(l1 - mxl), e2 = expf(l2 - mxl), e3 = expf(l3 - mxl), se = e1 + e2 + e3;
+                const float lc = (cls == 0 ? l1 : cls == 1 ? l2 : l3) - mxl;
+                atomicAdd(&s_loss, diff * diff + lambda * (logf(se) - lc));
+                const float inv = 1.f / (float)nb;
+                o[b * SEL_OUT + 0] = 2.f * diff * inv;
+                o[b * SEL_OUT + 1] = lambda * (e1 / se - (cls == 0)) * inv;
+                o[b * SEL_OUT + 2] = lambda * (e2 / se - (cls == 1)) * inv;
+                o[b * SEL_OUT + 3] = lambda * (e3 / se - (cls == 2)) * inv;
+            }
+            __syncthreads();
+            // layer 4, then 3, 2, 1: the activation gradient below each layer
+            // uses the layer's weights before its update
+            sel_bwd_act(P + SEL_OFF_W4, o, SEL_OUT, a3, SEL_H, d3, nb);
+            __syncthreads();
+            sel_bwd_w(P, M, V, SEL_OFF_W4, SEL_OFF_B4, o, a3, SEL_H, SEL_OUT, nb, lr, c1, c2);
+            __syncthreads();
+            sel_bwd_act(P + SEL_OFF_W3, d3, SEL_H, a2, SEL_H, d2, nb);
+            __syncthreads();
+            sel_bwd_w(P, M, V, SEL_OFF_W3, SEL_OFF_B3, d3, a2, SEL_H, SEL_H, nb, lr, c1, c2);
+            __syncthreads();
+            sel_bwd_act(P + SEL_OFF_W2, d2, SEL_H, a1, SEL_H, d3, nb);  // d3 now holds layer-1 gradients
+            __syncthreads();
+            sel_bwd_w(P, M, V, SEL_OFF_W2, SEL_OFF_B2, d2, a1, SEL_H, SEL_H, nb, lr, c1, c2);
+            __syncthreads();
+            sel_bwd_w(P, M, V, SEL_OFF_W1, SEL_OFF_B1, d3, a0, SEL_IN, SEL_H, nb, lr, c1, c2);
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) loss[ep] = s_loss / (float)n;
+        __syncthreads();
+    }
+}
+
+}  // namespace gbagg

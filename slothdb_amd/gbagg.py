@@ -78,7 +78,8 @@ class PlanInfo(C.Structure):
                 ("sample_rows", C.c_uint64), ("sample_distinct", C.c_uint64), ("sample_f1", C.c_uint64),
                 ("ms_aggregate", C.c_double), ("ms_total", C.c_double), ("ms_sample", C.c_double),
                 ("ms_count", C.c_double), ("ms_partition", C.c_double), ("ms_emit", C.c_double),
-                ("kernel_shape", C.c_char * 32)]
+                ("kernel_shape", C.c_char * 32), ("selector_distinct", C.c_double),
+                ("selector_used", C.c_int32), ("selector_path", C.c_int32)]
 
 
 # Every symbol include/slothdb_gpu.h declares, with its ctypes signature.
@@ -106,6 +107,11 @@ _SIGS = {
     "gbagg_batch_result": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p]),
     "gbagg_batch_info_get": (C.c_int, [C.c_void_p, C.c_void_p]),
     "gbagg_batch_destroy": (None, [C.c_void_p]),
+    "gbagg_selector_set": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32]),
+    "gbagg_plan_features": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "gbagg_selector_train": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_float, C.c_float,
+                                       C.c_void_p, C.c_uint32, C.c_void_p]),
+    "gbagg_selector_infer": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p]),
     "gbagg_device_alloc": (C.c_int, [C.c_uint64, C.POINTER(C.c_void_p)]),
     "gbagg_device_free": (C.c_int, [C.c_void_p]),
     "gbagg_memcpy_h2d": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64]),
@@ -442,9 +448,12 @@ class GpuBatch:
 class GpuHashAggregate:
     """The operator: plan once, consume batches, finalize, read the result."""
 
-    def __init__(self, query, params=None):
+    def __init__(self, query, params=None, selector=None):
+        """selector: None, "default" (slothdb_amd/selector_weights.json) or a
+        parameter vector of the learned strategy selector."""
         self.query = query
         self.params = dict(params or {})
+        self.selector = selector
         self._h = None
 
     # PhysicalOperator surface
@@ -464,6 +473,10 @@ class GpuHashAggregate:
             self._h = h
             for k, v in self.params.items():
                 _check(lib().gbagg_set_param(self._h, k.encode(), int(v)))
+            if self.selector is not None:
+                from . import selector as S
+                p = S.load()[0] if isinstance(self.selector, str) and self.selector == "default" else self.selector
+                S.attach(self, p)
         else:
             _check(lib().gbagg_reset(self._h))
         self._emit_pos = 0

@@ -1,0 +1,134 @@
+"""The learned selector's HIP kernels against a plain PyTorch FP32 reference
+(forward pass; minibatch Adam training), and operators with a selector
+attached still return exact results."""
+import numpy as np
+import pytest
+import torch
+
+from slothdb_amd import datagen, gbagg as G, selector as S
+from oracle import oracle as O
+from tests._util import assert_same_groups, oracle_rows
+
+pytestmark = pytest.mark.gpu
+
+
+def _torch_params(p):
+    layers, (mu, sig) = S.unpack(p)
+    return [(torch.tensor(W.copy(), requires_grad=True), torch.tensor(b.copy(), requires_grad=True))
+            for W, b in layers], torch.tensor(mu.copy()), torch.tensor(sig.copy())
+
+
+def _torch_forward(layers, mu, sig, x):
+    h = (x - mu) / sig
+    for i, (W, b) in enumerate(layers):
+        h = h @ W.T + b
+        if i < len(layers) - 1:
+            h = torch.relu(h)
+    return h
+
+
+def _random_problem(n, seed):
+    rng = np.random.default_rng(seed)
+    x = rng.normal(0, 1, (n, S.N_IN)).astype(np.float32)
+    y = np.stack([rng.uniform(0, 1, n), rng.integers(0, 3, n)], axis=1).astype(np.float32)
+    mu = x.mean(axis=0)
+    sig = x.std(axis=0) + 1e-3
+    return x, y, S.init_params(mu, sig, seed=seed)
+
+
+def test_infer_matches_torch():
+    x, _, p = _random_problem(37, 1)
+    got = S.infer(p, x)
+    layers, mu, sig = _torch_params(p)
+    with torch.no_grad():
+        exp = _torch_forward(layers, mu, sig, torch.tensor(x)).numpy()
+    np.testing.assert_allclose(got, exp, rtol=1e-4, atol=1e-5)
+
+
+def test_training_matches_torch_adam():
+    # 150 rows = minibatches of 64, 64 and 22; two epochs in row order
+    x, y, p0 = _random_problem(150, 2)
+    lr, lam, epochs = 1e-3, 0.5, 2
+    got, loss = S.train(x, y, p0, epochs=epochs, lr=lr, lam=lam)
+    layers, mu, sig = _torch_params(p0)
+    params = [t for wb in layers for t in wb]
+    opt = torch.optim.Adam(params, lr=lr, betas=(0.9, 0.999), eps=1e-8)
+    X, Yt = torch.tensor(x), torch.tensor(y)
+    losses = []
+    for _ in range(epochs):
+        total = 0.0
+        for r0 in range(0, len(x), 64):
+            xb, yb = X[r0:r0 + 64], Yt[r0:r0 + 64]
+            out = _torch_forward(layers, mu, sig, xb)
+            reg = (out[:, 0] - yb[:, 0]) ** 2
+            ce = torch.nn.functional.cross_entropy(out[:, 1:], yb[:, 1].long(), reduction="none")
+            per_row = reg + lam * ce
+            total += float(per_row.sum())
+            opt.zero_grad()
+            per_row.mean().backward()
+            opt.step()
+        losses.append(total / len(x))
+    exp = np.concatenate([t.detach().numpy().ravel() for t in params])
+    np.testing.assert_allclose(got[:S.N_W], exp, rtol=2e-3, atol=2e-5)
+    np.testing.assert_allclose(loss, losses, rtol=1e-4)
+    assert np.array_equal(got[S.N_W:], p0[S.N_W:])  # the standardisation is not trained
+
+
+def _exact(q, cols, dicts=None):
+    c, v, t, ov = O.groupby_exact(q, cols)
+    assert not ov
+    return oracle_rows(c, v, t, dicts), t
+
+
+@pytest.mark.parametrize("seed", [3, 4])
+def test_selector_plans_stay_exact(seed):
+    # an untrained selector picks arbitrary paths and counts: results must not change
+    _, _, p = _random_problem(8, seed)
+    p[S.N_W:S.N_W + S.N_IN] = 0.0
+    p[S.N_W + S.N_IN:] = 1.0
+    workloads = [datagen.plain_workload(n=300_000),
+                 datagen.highcard_workload(n=600_000, nkeys=300_000),
+                 datagen.composite_workload(n=500_000, dict_size=50_000, topn=None)]
+    for q, cols in workloads:
+        op = G.GpuHashAggregate(q)
+        op.Init()
+        S.attach(op, p)
+        op.consume([G.Column(c.type, c.values, dictionary=c.dictionary) for c in cols])
+        op.finalize()
+        info = op.plan_info()
+        assert info["selector_path"] in (0, 1, 2) and info["selector_distinct"] >= 1
+        f, o = S.features(op, with_outputs=True)
+        assert np.all(np.isfinite(f)) and np.all(np.isfinite(o))
+        dicts = {i: c.dictionary for i, c in enumerate(cols) if c.dictionary is not None}
+        exp, types = _exact(q, cols, dicts)
+        assert_same_groups(op.result().rows(), exp, len(q.group_cols), types)
+        op.close()
+
+
+def test_features_of_a_known_sample():
+    # 2M rows over 1000 uniform INTEGER keys: every key is sampled many times
+    q, cols = datagen.plain_workload(n=2_000_000)
+    op = G.GpuHashAggregate(q, {"features": 1})
+    op.Init()
+    op.consume([G.Column(c.type, c.values) for c in cols])
+    op.finalize()
+    f = S.features(op)
+    op.close()
+    assert f[28] == 1 and f[29] == 0 and f[30] == 0
+    assert abs((2 ** (32 * f[2]) - 1) - 1000) < 0.5  # all 1000 keys seen
+    assert f[4] == 0  # no singletons
+    assert abs((2 ** (32 * f[0]) - 1) - 2_000_000) < 2_000_000 * 1e-6
+
+
+def test_default_selector_keeps_results_exact():
+    q, cols = datagen.highcard_workload(n=2_000_000, nkeys=1_000_000)
+    op = G.GpuHashAggregate(q, selector="default")
+    op.Init()
+    op.consume([G.Column(c.type, c.values) for c in cols])
+    op.finalize()
+    info = op.plan_info()
+    print(f"[selector] {info}", flush=True)
+    assert info["selector_path"] in (0, 1, 2)
+    exp, types = _exact(q, cols)
+    assert_same_groups(op.result().rows(), exp, 1, types)
+    op.close()

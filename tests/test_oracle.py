@@ -302,3 +302,4 @@ def test_oracle_topn_tie_break():
     c, v, t = O._to_table(res)
     O.lib().orc_result_free(C.byref(res))
     assert [(int(c[0][i]), int(c[1][i])) for i in idx[:m]] == [(1, 2), (3, 2), (5, 2), (7, 1)]
+
