@@ -145,7 +145,10 @@ typedef struct {
  * learned selector when one is loaded) pick. */
 typedef enum {
     GBAGG_STRATEGY_AUTO = 0,
-    GBAGG_STRATEGY_DIRECT = 1,      /* dense key range -> direct-mapped LDS slots */
+    GBAGG_STRATEGY_DIRECT = 1,      /* one-column key over a narrow exact range: one record
+                                       per key value, no hashing (IntGroupAgg's dense
+                                       slot_of[key], int_group_agg.cpp:69-277); a range
+                                       wider than 2^22 runs the AUTO plan instead */
     GBAGG_STRATEGY_LDS_HASH = 2,    /* per-workgroup LDS hash table + global merge */
     GBAGG_STRATEGY_GLOBAL_HASH = 3, /* global hash table only */
     GBAGG_STRATEGY_RADIX = 4,       /* radix partition + per-partition LDS tables */
@@ -165,6 +168,10 @@ typedef struct {
      * (validity must then be a null pointer), 1 = may contain NULLs.
      * A null pointer here means every column may contain NULLs. */
     const uint8_t *column_nullable;
+    /* 0-2 group columns. Two columns with a BIGINT among them form a 16-byte
+     * key (InlineRowAggBigKey / RadixMultiAggBigKey, inline_row_agg.cpp:269-517,
+     * radix_multi_agg.cpp:218-225); two 32-bit columns pack into 8 bytes. Two
+     * columns over nullable data require drop_null_keys. */
     uint32_t num_group_cols;
     const uint32_t *group_cols;
     uint32_t num_aggs;
@@ -264,6 +271,8 @@ void gbagg_destroy(gbagg_handle *h);
  *   "part_bits"     radix fanout override, -1 = from the HyperLogLog estimate
  *   "lds_kb"        LDS budget of a per-partition table (8-160)
  *   "fallback_cap"  initial slots of the radix strategy's global fallback table
+ *   "wide_seed"     first seed of the 16-byte key mixing (reruns take the next
+ *                   seeds while two groups share a key word; results never change)
  *   "static_shapes" 1 = use compile-time specialised row kernels when the query
  *                   shape matches one (default), 0 = always the generic kernels */
 gbagg_status gbagg_set_param(gbagg_handle *h, const char *name, int64_t value);
@@ -331,34 +340,35 @@ gbagg_status gbagg_batch_info_get(gbagg_batch *b, gbagg_batch_info *out);
 void gbagg_batch_destroy(gbagg_batch *b);
 
 /* ---- Learned strategy selector ----------------------------------------------
- * An MLP (32 -> 64 -> 64 -> 64 -> 4, FP32, ReLU) over features of the AUTO
+ * An MLP (32 -> 64 -> 64 -> 64 -> 5, FP32, ReLU) over features of the AUTO
  * plan's row sample predicts the distinct-group count, as the correction
  * log2((distinct + 1) / (default + 1)) / 4 to the sample's own estimate
- * default = seen + f1 / sampled * rows, and the fastest path (LDS_HASH,
- * GLOBAL_HASH, RADIX). With a selector set, an AUTO plan takes
+ * default = seen + f1 / sampled * rows, and the fastest path (DIRECT,
+ * LDS_HASH, GLOBAL_HASH, RADIX). With a selector set, an AUTO plan takes
  * the predicted path and sizes its hash table from the predicted count, unless
  * the prediction and the sample's own estimate differ by more than 4x; then
- * the default plan runs. Results never depend on it.
+ * the default plan runs. DIRECT falls back to LDS_HASH when the exact key
+ * range is too wide. Results never depend on it.
  * Parameter vector: GBAGG_SELECTOR_PARAMS floats = weights and biases of the
  * four layers in order (W row-major [out][in], then b), then the input
  * standardisation mean[32] and scale[32]. */
 #define GBAGG_SELECTOR_FEATURES 32
-#define GBAGG_SELECTOR_OUTPUTS 4
-#define GBAGG_SELECTOR_PARAMS 10756
+#define GBAGG_SELECTOR_OUTPUTS 5
+#define GBAGG_SELECTOR_PARAMS 10821
 
 /* Per handle; params = NULL removes the selector. */
 gbagg_status gbagg_selector_set(gbagg_handle *h, const float *params, uint32_t num_params);
-/* Features (32 floats) and, with a selector, its raw outputs (4 floats) of
+/* Features (32 floats) and, with a selector, its raw outputs (5 floats) of
  * the last finalize; features are kept when a selector is set or after
  * gbagg_set_param(h, "features", 1). Either pointer may be NULL. */
 gbagg_status gbagg_plan_features(gbagg_handle *h, float *features, float *outputs);
 /* Trains on the GPU: minibatch (64) Adam over `epochs` passes in row order.
  * x: n x 32 features; y: n x 2 (target correction as above, path class
- * 0..2). params: initial parameters in (including the standardisation), the
+ * 0..3). params: initial parameters in (including the standardisation), the
  * trained ones out. epoch_loss: `epochs` floats of mean loss, or NULL. */
 gbagg_status gbagg_selector_train(const float *x, const float *y, uint32_t n, uint32_t epochs, float lr,
                                   float lambda, float *params, uint32_t num_params, float *epoch_loss);
-/* Forward pass on the GPU: out = n x 4 outputs for n x 32 features. */
+/* Forward pass on the GPU: out = n x 5 outputs for n x 32 features. */
 gbagg_status gbagg_selector_infer(const float *params, uint32_t num_params, const float *x, uint32_t n,
                                   float *out);
 

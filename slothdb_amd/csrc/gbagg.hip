@@ -45,6 +45,8 @@ using KAgg = void (*)(const DevQuery *, const DevBatch *, DevTable, uint32_t, ui
 using KCount = void (*)(const DevCtx *, const DevBatch *, uint32_t, uint64_t);
 using KScatter = void (*)(const DevCtx *, const DevBatch *, DevTable, uint32_t, uint64_t, uint32_t);
 using KPart = void (*)(const DevCtx *, DevTable, uint32_t, uint32_t);
+using KRange = void (*)(const DevQuery *, const DevBatch *, DevPlan *);
+using KDirect = void (*)(const DevQuery *, const DevBatch *, DevTable, int64_t, uint32_t, uint32_t);
 
 struct ShapeKernels {
     const char *name;
@@ -54,6 +56,8 @@ struct ShapeKernels {
     KCount count;
     KScatter scatter;
     KPart part_full, part16, part32;
+    KRange range;
+    KDirect direct_lds, direct_glb;
 };
 
 template <class S>
@@ -68,6 +72,9 @@ ShapeKernels shape_kernels(const char *name) {
     k.part_full = &k_rp_agg<S>;
     k.part16 = &k_rp_agg_topn<S, 16>;
     k.part32 = &k_rp_agg_topn<S, 32>;
+    k.range = &k_key_range<S>;
+    k.direct_lds = &k_direct<S, true>;
+    k.direct_glb = &k_direct<S, false>;
     return k;
 }
 
@@ -271,6 +278,8 @@ struct gbagg_handle {
     std::vector<OutCol> outs;
     KeyDict kdict[2];
     bool key_varchar[2] = {false, false};
+    int32_t wide_acc[2] = {0, 0};  // KM_WIDE: MIN / MAX of the kept key component
+    uint64_t wide_seed0 = 0;       // KM_WIDE: first seed of wide_mix
     // TopN on the device
     bool topn_device = false;
     DevTopn tspec{};
@@ -389,17 +398,21 @@ void build_plan(gbagg_handle *h, const gbagg_query_desc *q) {
         h->dq.key_u[0] = add_u(h, h->group_cols[0]);
         h->dq.key_signed32[0] = h->col_types[h->group_cols[0]] != GBAGG_TYPE_VARCHAR;
     } else {
-        for (int g = 0; g < 2; g++)
-            if (h->col_types[h->group_cols[g]] == GBAGG_TYPE_BIGINT)
-                fail(GBAGG_INVALID, "two-column keys with a BIGINT component are not supported");
         bool any_null = h->nullable[h->group_cols[0]] || h->nullable[h->group_cols[1]];
         if (any_null && !q->drop_null_keys)
             fail(GBAGG_INVALID, "two-column keys over nullable columns require drop_null_keys");
-        h->dq.key_mode = KM_PACK32;
+        // two 32-bit components pack into the key word; a BIGINT component
+        // makes a 16-byte key (KM_WIDE), keeping a non-VARCHAR component in
+        // hidden accumulators
+        const bool big0 = h->col_types[h->group_cols[0]] == GBAGG_TYPE_BIGINT;
+        const bool big1 = h->col_types[h->group_cols[1]] == GBAGG_TYPE_BIGINT;
+        h->dq.key_mode = big0 || big1 ? KM_WIDE : KM_PACK32;
         for (int g = 0; g < 2; g++) {
             h->dq.key_u[g] = add_u(h, h->group_cols[g]);
             h->dq.key_signed32[g] = h->col_types[h->group_cols[g]] == GBAGG_TYPE_INTEGER;
         }
+        h->dq.wide_comp = h->key_varchar[1] ? 0 : 1;
+        h->dq.wide_seed = 0;
     }
 
     // predicates
@@ -439,6 +452,11 @@ void build_plan(gbagg_handle *h, const gbagg_query_desc *q) {
 
     // aggregates -> accumulators (acc 0 is always COUNT(*))
     add_acc(h, ACC_CNT_STAR, -1);
+    if (h->dq.key_mode == KM_WIDE) {
+        const int u = h->dq.key_u[h->dq.wide_comp];
+        h->wide_acc[0] = add_acc(h, ACC_MIN_I, u);
+        h->wide_acc[1] = add_acc(h, ACC_MAX_I, u);
+    }
     for (uint32_t a = 0; a < q->num_aggs; a++) {
         gbagg_agg ag = q->aggs[a];
         OutCol oc{};
@@ -557,7 +575,7 @@ ShapeSig plan_sig(const gbagg_handle *h) {
     }
     g.km = q.key_mode;
     g.ku0 = q.key_mode != KM_NONE ? q.key_u[0] : 0;
-    g.ku1 = q.key_mode == KM_PACK32 ? q.key_u[1] : 0;
+    g.ku1 = q.key_mode == KM_PACK32 || q.key_mode == KM_WIDE ? q.key_u[1] : 0;
     g.na = q.nacc;
     for (int a = 0; a < q.nacc; a++) {
         g.ops |= (uint64_t)q.acc_op[a] << (4 * a);
@@ -795,6 +813,11 @@ void fill_out_topn(gbagg_handle *h) {
     o.ncols = (int32_t)h->outs.size();
     o.key_mode = h->dq.key_mode;
     o.capacity = h->out_capacity;
+    o.wide_acc[0] = h->wide_acc[0];
+    o.wide_acc[1] = h->wide_acc[1];
+    o.wide_comp = h->dq.wide_comp;
+    o.wide_seed = h->dq.wide_seed;
+    o.flags = &h->dplan.as<DevPlan>()->overflow;
     for (size_t c = 0; c < h->outs.size(); c++) {
         o.data[c] = h->out_data[c].ptr;
         o.valid[c] = h->out_valid[c].as<uint8_t>();
@@ -810,6 +833,9 @@ void fill_out_topn(gbagg_handle *h) {
     t.cand_rec = h->cand_rec.as<uint64_t>();
     t.cand_null = h->cand_null.as<uint8_t>();
     t.cand_cap = h->cand_cap;
+    t.wide_acc[0] = h->wide_acc[0];
+    t.wide_acc[1] = h->wide_acc[1];
+    t.flags = o.flags;
 }
 
 // The host context is pinned; callers modify it only after the stream has
@@ -967,6 +993,7 @@ void run_sample(gbagg_handle *h) {
     h->ghist.ensure(65 * 4);
     HIPCHK(hipMemsetAsync(h->ghist.ptr, 0, 65 * 4, s));
     const uint32_t sgrid = (uint32_t)std::min<uint64_t>((rp.scap + 255) / 256, (uint64_t)di.cus * 2);
+    hipLaunchKernelGGL(k_key_range_init, dim3(1), dim3(1), 0, s, h->dplan.as<DevPlan>());  // sampled key range
     hipLaunchKernelGGL(k_heavy_hist, dim3(sgrid), dim3(256), 0, s, d_rp(h), h->dplan.as<DevPlan>(), h->ghist.as<uint32_t>());
     hipLaunchKernelGGL(k_heavy_threshold, dim3(1), dim3(64), 0, s, d_rp(h), h->dplan.as<DevPlan>(),
                        (const uint32_t *)h->ghist.as<uint32_t>());
@@ -978,7 +1005,8 @@ void run_sample(gbagg_handle *h) {
 void run_features(gbagg_handle *h, bool infer) {
     hipStream_t s = h->stream;
     uint32_t key_kind = 0;
-    if (h->dq.key_mode == KM_PACK32) key_kind = 2;
+    if (h->dq.key_mode == KM_WIDE) key_kind = 3;
+    else if (h->dq.key_mode == KM_PACK32) key_kind = 2;
     else if (h->dq.key_mode == KM_SINGLE && h->dq.ukind[h->dq.key_u[0]] == CK_I64) key_kind = 1;
     h->dfeat.ensure(SEL_IN * 4);
     hipLaunchKernelGGL(k_sel_features, dim3(1), dim3(1024), 0, s, d_rp(h), (const DevPlan *)h->dplan.as<DevPlan>(),
@@ -994,7 +1022,7 @@ void run_features(gbagg_handle *h, bool infer) {
     h->feat_valid = true;
 }
 
-void run_lds_hash(gbagg_handle *h, Timer &tm, bool global_only) {
+void run_lds_hash(gbagg_handle *h, Timer &tm, bool global_only, uint64_t est = 0) {
     const DeviceInfo &di = device_info();
     hipStream_t s = h->stream;
     // table size from the sample: distinct keys seen, plus a Good-Turing style
@@ -1004,6 +1032,7 @@ void run_lds_hash(gbagg_handle *h, Timer &tm, bool global_only) {
     double rest = hp.sample_rows ? (double)hp.sample_f1 / hp.sample_rows * (double)h->rows_total : 0;
     uint64_t want = (uint64_t)(2.0 * (seen + rest)) + 1024;
     if (h->sel_used) want = (uint64_t)(2.0 * h->sel_distinct) + 1024;  // the learned count sizes the table
+    if (est) want = 2 * est + 1024;  // a HyperLogLog count over all rows
     if (h->dq.key_mode == KM_NONE) want = 1024;
     uint64_t cap = std::min<uint64_t>(h->cap_max, next_pow2(want));
     for (;;) {
@@ -1055,7 +1084,10 @@ void run_lds_hash(gbagg_handle *h, Timer &tm, bool global_only) {
     }
 }
 
-void run_radix(gbagg_handle *h, Timer &tm) {
+// Returns false, with nothing aggregated, when the HyperLogLog count of cold
+// groups exceeds what the largest fanout's LDS tables hold (est_out: the
+// count); the caller then runs the global table.
+bool run_radix(gbagg_handle *h, Timer &tm, uint64_t &est_out) {
     const DeviceInfo &di = device_info();
     hipStream_t s = h->stream;
     const int nacc = h->dq.nacc;
@@ -1135,8 +1167,16 @@ void run_radix(gbagg_handle *h, Timer &tm) {
         const uint64_t cold = est > heavy ? est - heavy : 1;
         const size_t k4_budget = std::min<size_t>(di.lds_per_block, (size_t)h->k4_lds_kb * 1024);
         const uint32_t S = lds_slots_exact(k4_budget, nacc, RA_SMAX);
+        if (h->force_part_bits < 0 && (double)cold > 0.75 * (double)S * (double)(1u << rp.bucket_bits)) {
+            est_out = est;
+            return false;
+        }
+        // fanout: half-full LDS tables, and at most 64K rows per partition
+        // workgroup (few groups over many rows still spread over the CUs)
         uint32_t pbits = 0;
-        while (pbits < rp.bucket_bits && (double)cold / (double)(1u << pbits) > 0.5 * S) pbits++;
+        while (pbits < rp.bucket_bits && ((double)cold / (double)(1u << pbits) > 0.5 * S ||
+                                          (double)h->rows_total / (double)(1u << pbits) > 65536.0))
+            pbits++;
         if (h->force_part_bits >= 0) pbits = std::min<uint32_t>((uint32_t)h->force_part_bits, rp.bucket_bits);
         rp.part_bits = pbits;
         const uint32_t P = 1u << pbits;
@@ -1205,7 +1245,101 @@ void run_radix(gbagg_handle *h, Timer &tm) {
         h->retries++;
         tm.truncate(2);
     }
+    return true;
 }
+
+// ---- direct-mapped strategy ---------------------------------------------------
+constexpr uint64_t DIRECT_MAX_SPAN = 1ull << 22;
+
+size_t direct_lds_budget() {
+    const DeviceInfo &di = device_info();
+    return std::min<size_t>(di.lds_per_cu / 2, di.lds_per_block);
+}
+
+// Widest key range whose dense table [nacc][span] fits one workgroup's LDS.
+uint64_t direct_lds_span(const gbagg_handle *h) { return direct_lds_budget() / (8 * (size_t)h->dq.nacc); }
+
+// Exact key range of the filtered rows (one pass over the key and filter
+// columns): false for multi-column keys or a range wider than DIRECT_MAX_SPAN.
+bool direct_range(gbagg_handle *h, uint64_t &span, int64_t &kmin) {
+    if (h->dq.key_mode != KM_SINGLE) return false;
+    const DeviceInfo &di = device_info();
+    hipStream_t s = h->stream;
+    hipLaunchKernelGGL(k_key_range_init, dim3(1), dim3(1), 0, s, h->dplan.as<DevPlan>());
+    for (size_t i = 0; i < h->batches.size(); i++) {
+        const auto &b = h->batches[i];
+        if (!b.nrows) continue;
+        const uint64_t tiles = (b.nrows + (uint64_t)AGG_BLOCK * AGG_ROWS - 1) / ((uint64_t)AGG_BLOCK * AGG_ROWS);
+        const uint32_t g = (uint32_t)std::min<uint64_t>(tiles, (uint64_t)di.cus * 4);
+        hipLaunchKernelGGL(h->kern->range, dim3(g), dim3(AGG_BLOCK), 0, s, d_q(h), d_b(h, i), h->dplan.as<DevPlan>());
+    }
+    HIPCHK(hipGetLastError());
+    read_plan(h);
+    const DevPlan &p = *h->hplan;
+    if (p.key_min > p.key_max) {  // no key: every row NULL-keyed or filtered out
+        kmin = 0;
+        span = 1;
+        return true;
+    }
+    const uint64_t d = (uint64_t)p.key_max - (uint64_t)p.key_min;
+    if (d >= DIRECT_MAX_SPAN) return false;
+    kmin = p.key_min;
+    span = d + 1;
+    return true;
+}
+
+// Key word w owns table record w - kmin; NULL and SENTINEL keys keep their
+// side records. A workgroup-private LDS table when [nacc][span] fits.
+void run_direct(gbagg_handle *h, Timer &tm, uint64_t span, int64_t kmin) {
+    const DeviceInfo &di = device_info();
+    hipStream_t s = h->stream;
+    const uint64_t cap = std::max<uint64_t>(1024, next_pow2(span));
+    ensure_table(h, cap);
+    DevTable T = make_table(h);
+    upload_plan(h, cap);
+    const uint32_t igrid = (uint32_t)std::min<uint64_t>((cap + 2 + 255) / 256, (uint64_t)di.cus * 8);
+    hipLaunchKernelGGL(k_table_init, dim3(igrid), dim3(256), 0, s, T, d_q(h));
+    tm.mark();  // aggregate start
+    const size_t lds_bytes = (size_t)span * h->dq.nacc * 8;
+    const bool lds = span < direct_lds_span(h);
+    const uint32_t cache = lds ? 0 : lds_slots_for(direct_lds_budget(), h->dq.nacc, 4096);
+    const size_t cache_bytes = lds ? 0 : lds_table_bytes(cache, h->dq.nacc);
+    if (lds) allow_lds(h->kern->direct_lds, lds_bytes);
+    else allow_lds(h->kern->direct_glb, cache_bytes);
+    h->lds_slots = lds ? (uint32_t)span : 0;
+    for (size_t i = 0; i < h->batches.size(); i++) {
+        const auto &b = h->batches[i];
+        if (!b.nrows) continue;
+        const uint64_t tiles = (b.nrows + (uint64_t)AGG_BLOCK * AGG_ROWS - 1) / ((uint64_t)AGG_BLOCK * AGG_ROWS);
+        uint32_t g;
+        if (lds) {
+            // each workgroup folds its whole table at the end: few of them for small inputs
+            const uint32_t per_cu = (uint32_t)std::max<size_t>(1, std::min<size_t>(4, di.lds_per_cu / std::max<size_t>(lds_bytes, 1)));
+            const uint64_t by_rows = std::max<uint64_t>(1, b.nrows / std::max<uint64_t>(16384, span * 4));
+            g = (uint32_t)std::min<uint64_t>({tiles, (uint64_t)di.cus * per_cu, by_rows});
+            hipLaunchKernelGGL(h->kern->direct_lds, dim3(g), dim3(AGG_BLOCK), lds_bytes, s, d_q(h), d_b(h, i), T, kmin,
+                               (uint32_t)span, 0u);
+        } else {
+            // an LDS hash cache for the frequent keys in front of the dense records
+            const uint32_t per_cu = (uint32_t)std::max<size_t>(1, std::min<size_t>(4, di.lds_per_cu / cache_bytes));
+            const uint64_t by_rows = std::max<uint64_t>(1, b.nrows / 16384);
+            g = (uint32_t)std::min<uint64_t>({tiles, (uint64_t)di.cus * per_cu, by_rows});
+            hipLaunchKernelGGL(h->kern->direct_glb, dim3(g), dim3(AGG_BLOCK), cache_bytes, s, d_q(h), d_b(h, i), T,
+                               kmin, (uint32_t)span, cache);
+        }
+        h->agg_grid = g;
+    }
+    HIPCHK(hipGetLastError());
+    tm.mark();  // aggregate end
+    emit_table(h, T, cap);
+    topn_final(h);
+    tm.mark();  // emit end
+    read_plan(h);
+    if (h->hplan->overflow & OVF_DIRECT) fail(GBAGG_ERROR, "direct-mapped strategy: a key outside its measured range");
+    if (h->hplan->overflow & 2) fail(GBAGG_ERROR, "TopN candidate buffer overflow");
+}
+
+void finalize_once(gbagg_handle *h);
 
 void finalize(gbagg_handle *h) {
     if (h->finalized) fail(GBAGG_INVALID, "already finalized: call gbagg_reset first");
@@ -1227,6 +1361,23 @@ void finalize(gbagg_handle *h) {
     h->sel_distinct = 0;
     h->feat_valid = false;
     h->radix_bits = 0;
+    // 16-byte keys rerun with the next seed while two groups share a key word
+    for (uint32_t attempt = 0;; attempt++) {
+        h->dq.wide_seed = h->wide_seed0 + attempt;
+        finalize_once(h);
+        if (h->dq.key_mode != KM_WIDE) break;
+        read_plan(h);
+        if (!(h->hplan->overflow & OVF_WIDE)) break;
+        if (attempt == 7) fail(GBAGG_ERROR, "16-byte group keys: key words still shared after 8 seeds");
+        h->retries++;
+    }
+    h->result_rows = h->hplan->groups;
+    if (h->result_rows > h->out_capacity) fail(GBAGG_ERROR, "result larger than its buffer");
+    h->finalized = true;
+    h->result_ready = false;
+}
+
+void finalize_once(gbagg_handle *h) {
     Timer tm{h, {}};
     tm.mark();  // start
     gbagg_strategy st = h->strategy;
@@ -1246,7 +1397,18 @@ void finalize(gbagg_handle *h) {
     upload_batches(h);
     upload_ctx(h);
     upload_plan(h, 1024);
-    if (st == GBAGG_STRATEGY_GLOBAL_HASH && h->dq.key_mode == KM_NONE) {
+    uint64_t span = 0;
+    int64_t kmin = 0;
+    if (st == GBAGG_STRATEGY_DIRECT) {
+        tm.mark();  // no sample
+        if (!direct_range(h, span, kmin)) {
+            st = GBAGG_STRATEGY_AUTO;
+            tm.truncate(1);
+        }
+    }
+    if (st == GBAGG_STRATEGY_DIRECT) {
+        run_direct(h, tm, span, kmin);
+    } else if (st == GBAGG_STRATEGY_GLOBAL_HASH && h->dq.key_mode == KM_NONE) {
         tm.mark();
         run_lds_hash(h, tm, true);
     } else {
@@ -1255,10 +1417,15 @@ void finalize(gbagg_handle *h) {
         if (sel || h->collect_features) run_features(h, sel);
         tm.mark();  // sample end
         read_plan(h);
-        if (st == GBAGG_STRATEGY_AUTO || st == GBAGG_STRATEGY_DIRECT) {
+        if (st == GBAGG_STRATEGY_AUTO) {
             const DevPlan &p = *h->hplan;
             bool low_card = p.sample_distinct <= 4096 && p.sample_f1 * 8 <= p.sample_distinct + 8;
             st = (low_card || h->rows_total <= 65536) ? GBAGG_STRATEGY_LDS_HASH : GBAGG_STRATEGY_RADIX;
+            // a low-cardinality one-column key whose sampled range fits a
+            // workgroup's LDS as a dense table goes direct-mapped
+            if (st == GBAGG_STRATEGY_LDS_HASH && h->dq.key_mode == KM_SINGLE && p.key_min <= p.key_max &&
+                (uint64_t)p.key_max - (uint64_t)p.key_min < direct_lds_span(h))
+                st = GBAGG_STRATEGY_DIRECT;
             if (sel) {
                 // the selector's path and count, unless they disagree with the
                 // sample's own estimate by more than 4x (then the default plan)
@@ -1269,19 +1436,31 @@ void finalize(gbagg_handle *h) {
                 const double dflt = std::max(1.0, std::min((double)h->rows_total, seen + rest));
                 const double pred = std::max(1.0, (dflt + 1.0) * std::exp2(4.0 * (double)h->pred[0]) - 1.0);
                 int path = 0;
-                for (int k = 1; k < 3; k++)
+                for (int k = 1; k < SEL_CLS; k++)
                     if (h->pred[1 + k] > h->pred[1 + path]) path = k;
                 h->sel_distinct = pred;
                 h->sel_path = path;
                 if (pred <= 4.0 * dflt && pred * 4.0 >= dflt) {
                     h->sel_used = 1;
-                    st = path == 0 ? GBAGG_STRATEGY_LDS_HASH : path == 1 ? GBAGG_STRATEGY_GLOBAL_HASH
-                                                                       : GBAGG_STRATEGY_RADIX;
+                    static const gbagg_strategy paths[SEL_CLS] = {GBAGG_STRATEGY_DIRECT, GBAGG_STRATEGY_LDS_HASH,
+                                                                  GBAGG_STRATEGY_GLOBAL_HASH, GBAGG_STRATEGY_RADIX};
+                    st = paths[path];
                 }
             }
+            // direct-mapped needs the exact range: too wide, and the
+            // low-cardinality hash path runs instead
+            if (st == GBAGG_STRATEGY_DIRECT && !direct_range(h, span, kmin)) st = GBAGG_STRATEGY_LDS_HASH;
         }
-        if (st == GBAGG_STRATEGY_RADIX) run_radix(h, tm);
-        else run_lds_hash(h, tm, st == GBAGG_STRATEGY_GLOBAL_HASH);
+        uint64_t est = 0;
+        if (st == GBAGG_STRATEGY_DIRECT) {
+            run_direct(h, tm, span, kmin);
+        } else if (st == GBAGG_STRATEGY_RADIX && !run_radix(h, tm, est)) {
+            // more groups than the partitions' LDS tables hold: one global table
+            st = GBAGG_STRATEGY_GLOBAL_HASH;
+            run_lds_hash(h, tm, true, est);
+        } else if (st != GBAGG_STRATEGY_RADIX) {
+            run_lds_hash(h, tm, st == GBAGG_STRATEGY_GLOBAL_HASH);
+        }
     }
     h->used_strategy = st;
     // phase times from the recorded events
@@ -1300,10 +1479,6 @@ void finalize(gbagg_handle *h) {
         h->ms_partition = 0;
         h->ms_emit = tm.between(ne - 2, ne - 1);
     }
-    h->result_rows = h->hplan->groups;
-    if (h->result_rows > h->out_capacity) fail(GBAGG_ERROR, "result larger than its buffer");
-    h->finalized = true;
-    h->result_ready = false;
 }
 
 void download_result(gbagg_handle *h) {
@@ -1554,6 +1729,7 @@ gbagg_status gbagg_set_param(gbagg_handle *h, const char *name, int64_t value) {
         else if (n == "part_bits") { in(-1, 13); h->force_part_bits = (int32_t)value; }
         else if (n == "lds_kb") { in(8, 160); h->k4_lds_kb = (uint32_t)value; }
         else if (n == "features") { in(0, 1); h->collect_features = value != 0; }
+        else if (n == "wide_seed") { in(0, 1LL << 40); h->wide_seed0 = (uint64_t)value; }
         else if (n == "strategy") { in(0, 4); h->strategy = (gbagg_strategy)value; }
         else if (n == "static_shapes") { in(0, 1); h->allow_static = value != 0; pick_kernels(h); }
         else fail(GBAGG_INVALID, "unknown parameter " + n);

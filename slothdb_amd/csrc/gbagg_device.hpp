@@ -35,6 +35,7 @@ enum KeyMode : int32_t {
     KM_NONE = 0,    // no GROUP BY: one group, key word 0
     KM_SINGLE = 1,  // one INTEGER / BIGINT / VARCHAR-code column, key = int64 image
     KM_PACK32 = 2,  // two 32-bit columns: key = (u32)c0 << 32 | (u32)c1
+    KM_WIDE = 3,    // two columns, one of them BIGINT: key = c0 ^ wide_mix(c1, seed)
 };
 
 struct DevPred {
@@ -60,6 +61,9 @@ struct DevQuery {
     int32_t nacc;
     int32_t acc_op[MAX_ACC];
     int32_t acc_u[MAX_ACC];  // -1 for COUNT(*)
+    int32_t wide_comp;       // KM_WIDE: the component kept in hidden accumulators (never VARCHAR)
+    int32_t pad;
+    uint64_t wide_seed;      // KM_WIDE: seed of wide_mix
 };
 
 // Per-batch device pointers.
@@ -90,6 +94,8 @@ struct DevPlan {
     uint32_t heavy_threshold;  // minimum sample count of a heavy key
     uint32_t pad;
     uint64_t topn_lb;          // TopN: no group with a smaller order value can be in the top K
+    int64_t key_min;           // direct-mapped strategy: exact key word range
+    int64_t key_max;
 };
 
 // The global table: (cap + 2) records of (1 + nacc) words: key, accumulators.
@@ -122,6 +128,21 @@ __device__ __forceinline__ uint64_t hash64(uint64_t x) {
     // wyhash mum/mix with the 64-bit golden ratio: lo ^ hi of x * C
     // (include/third_party/unordered_dense.h:158-191, 266-267).
     return (x * HASH_C) ^ __umul64hi(x, HASH_C);
+}
+
+// KM_WIDE keys (two columns, at least one of them 64-bit; 16 bytes in all).
+// With c the component named by wide_comp and d the other one, the table key
+// word is w = d ^ wide_mix(c, seed): for a fixed c it is one-to-one in d, so
+// a group is fixed by (w, c), and d = w ^ wide_mix(c, seed). c travels in two
+// hidden accumulators, MIN(c) and MAX(c): they differ only if two groups
+// share a key word, which the emit detects (the operator then reruns with
+// another seed). wide_mix is the SplitMix64 output function (Steele, Lea,
+// Flood 2014) of c + seed * golden ratio.
+__host__ __device__ __forceinline__ uint64_t wide_mix(uint64_t c1, uint64_t seed) {
+    uint64_t z = c1 + seed * HASH_C;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
 }
 
 __host__ __device__ __forceinline__ uint64_t acc_identity(int op) {
@@ -262,7 +283,10 @@ __device__ __forceinline__ int make_key(const DevQuery &q, const DevBatch &b, co
         if (c >= b.key_dict_size[1]) return KEY_DROP;
         if (b.remap[1]) c1 = b.remap[1][c];
     }
-    w = (c0 << 32) | (c1 & 0xFFFFFFFFULL);
+    if (q.key_mode == KM_WIDE)
+        w = q.wide_comp ? c0 ^ wide_mix(c1, q.wide_seed) : c1 ^ wide_mix(c0, q.wide_seed);
+    else
+        w = (c0 << 32) | (c1 & 0xFFFFFFFFULL);
     return KEY_OK;
 }
 
@@ -503,7 +527,10 @@ __device__ __forceinline__ int make_key_s(const S &sh, const DevQuery &q, const 
         if (c >= b.key_dict_size[1]) return KEY_DROP;
         if (b.remap[1]) c1 = b.remap[1][c];
     }
-    w = (c0 << 32) | (c1 & 0xFFFFFFFFULL);
+    if (sh.key_mode() == KM_WIDE)
+        w = q.wide_comp ? c0 ^ wide_mix(c1, q.wide_seed) : c1 ^ wide_mix(c0, q.wide_seed);
+    else
+        w = (c0 << 32) | (c1 & 0xFFFFFFFFULL);
     return KEY_OK;
 }
 

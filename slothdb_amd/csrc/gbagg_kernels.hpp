@@ -323,6 +323,142 @@ __global__ __launch_bounds__(AGG_BLOCK) void k_aggregate(const DevQuery *__restr
     }
 }
 
+// ------------------------------------------------------------------------
+// Direct-mapped strategy (IntGroupAgg's dense slot_of[key],
+// src/execution/int_group_agg.cpp:69-277): with the exact key range
+// [kmin, kmin + span) known, key word w owns global record w - kmin, so a row
+// needs no hash and no probe.
+// ------------------------------------------------------------------------
+constexpr uint32_t OVF_DIRECT = 16u;  // DevPlan::overflow bit: a key outside the measured range
+
+// Exact range of the key words of the rows that pass the filter (NULL, dropped
+// and SENTINEL keys excluded): one atomic min / max per wave.
+template <class S>
+__global__ __launch_bounds__(AGG_BLOCK) void k_key_range(const DevQuery *__restrict__ qp,
+                                                         const DevBatch *__restrict__ bp, DevPlan *plan) {
+    const DevQuery &q = *qp;
+    const DevBatch b = *bp;
+    const S sh(q);
+    const uint64_t n = b.nrows;
+    const uint64_t tile = (uint64_t)AGG_BLOCK * AGG_ROWS;
+    int64_t lo = INT64_MAX, hi = INT64_MIN;
+    for (uint64_t base = (uint64_t)blockIdx.x * tile; base < n; base += (uint64_t)gridDim.x * tile) {
+        Row rows[AGG_ROWS];
+#pragma unroll
+        for (int j = 0; j < AGG_ROWS; j++) {
+            const uint64_t r = base + (uint64_t)j * AGG_BLOCK + threadIdx.x;
+            if (r < n) load_row_s(sh, b, r, sh.nkp(), rows[j]);
+        }
+#pragma unroll
+        for (int j = 0; j < AGG_ROWS; j++) {
+            const uint64_t r = base + (uint64_t)j * AGG_BLOCK + threadIdx.x;
+            bool live = r < n;
+            if (live && sh.npred()) live = eval_preds_s(sh, q, b, rows[j]);
+            uint64_t w = 0;
+            if (live && make_key_s(sh, q, b, rows[j], w) == KEY_OK && w != SENTINEL) {
+                lo = min(lo, (int64_t)w);
+                hi = max(hi, (int64_t)w);
+            }
+        }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        lo = min(lo, (int64_t)__shfl_xor(lo, off));
+        hi = max(hi, (int64_t)__shfl_xor(hi, off));
+    }
+    if ((threadIdx.x & 63) == 0 && lo <= hi) {
+        __hip_atomic_fetch_min(&plan->key_min, lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_fetch_max(&plan->key_max, hi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+__global__ void k_key_range_init(DevPlan *plan) {
+    plan->key_min = INT64_MAX;
+    plan->key_max = INT64_MIN;
+}
+
+// LDS: one workgroup-private dense table [nacc][span], folded into the global
+// records at the end (span * nacc words must fit the workgroup's LDS). Else a
+// workgroup-private LDS hash table of `cache` slots absorbs the frequent keys
+// (as in k_aggregate) and the other rows update their global record directly
+// with device-scope atomics.
+template <class S, bool LDS>
+__global__ __launch_bounds__(AGG_BLOCK) void k_direct(const DevQuery *__restrict__ qp,
+                                                      const DevBatch *__restrict__ bp, DevTable T, int64_t kmin,
+                                                      uint32_t span, uint32_t cache) {
+    const DevQuery &q = *qp;
+    const DevBatch b = *bp;
+    const S sh(q);
+    extern __shared__ uint64_t smem[];
+    LdsTable t{};
+    if constexpr (LDS) {
+        for (int a = 0; a < sh.nacc(); a++) {
+            const uint64_t id = acc_identity(sh.op(a));
+            for (uint32_t i = threadIdx.x; i < span; i += blockDim.x) smem[(uint64_t)a * span + i] = id;
+        }
+    } else {
+        t = lds_table_setup(smem, q, cache, cache * 3 / 4);
+    }
+    __syncthreads();
+    const uint64_t cap = T.plan->cap;
+    const uint64_t n = b.nrows;
+    const uint64_t tile = (uint64_t)AGG_BLOCK * AGG_ROWS;
+    for (uint64_t base = (uint64_t)blockIdx.x * tile; base < n; base += (uint64_t)gridDim.x * tile) {
+        Row rows[AGG_ROWS];
+#pragma unroll
+        for (int j = 0; j < AGG_ROWS; j++) {
+            const uint64_t r = base + (uint64_t)j * AGG_BLOCK + threadIdx.x;
+            if (r < n) load_row_s(sh, b, r, sh.nu(), rows[j]);
+        }
+#pragma unroll
+        for (int j = 0; j < AGG_ROWS; j++) {
+            const uint64_t r = base + (uint64_t)j * AGG_BLOCK + threadIdx.x;
+            bool live = r < n;
+            if (live && sh.npred()) live = eval_preds_s(sh, q, b, rows[j]);
+            uint64_t w = 0;
+            int kr = KEY_DROP;
+            if (live) kr = make_key_s(sh, q, b, rows[j], w);
+            if (kr == KEY_DROP) continue;
+            if (side_row(q, T, cap, kr, w, rows[j])) continue;
+            const uint64_t slot = w - (uint64_t)kmin;
+            if (slot >= span) {  // cannot happen with the exact range; never written out of bounds
+                __hip_atomic_fetch_or(&T.plan->overflow, OVF_DIRECT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                continue;
+            }
+            if constexpr (LDS) {
+                row_update_s<__HIP_MEMORY_SCOPE_WORKGROUP>(sh, rows[j], smem + slot, span);
+            } else {
+                const int ls = lds_find(t, hash64(w), w);
+                if (ls >= 0) {
+                    row_update_s<__HIP_MEMORY_SCOPE_WORKGROUP>(sh, rows[j], t.acc + ls, t.S);
+                } else {
+                    uint64_t *rec = T.rec + slot * T.stride;
+                    if (__hip_atomic_load(rec, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == SENTINEL)
+                        __hip_atomic_store(rec, w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    row_update_s<__HIP_MEMORY_SCOPE_AGENT>(sh, rows[j], rec + 1, 1);
+                }
+            }
+        }
+    }
+    __syncthreads();
+    if constexpr (LDS) {
+        for (uint32_t i = threadIdx.x; i < span; i += blockDim.x) {
+            if (smem[i] == 0) continue;  // COUNT(*) of the slot (accumulator 0)
+            __hip_atomic_store(T.rec + (uint64_t)i * T.stride, (uint64_t)kmin + i, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+            g_merge(q, T, i, smem + i, span);
+        }
+    } else {
+        for (uint32_t i = threadIdx.x; i < t.S; i += blockDim.x) {
+            const uint64_t k = t.key[i];
+            if (k == SENTINEL) continue;
+            const uint64_t slot = k - (uint64_t)kmin;  // within [0, span): only in-range keys enter the cache
+            __hip_atomic_store(T.rec + slot * T.stride, k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            g_merge(q, T, slot, t.acc + i, t.S);
+        }
+    }
+}
+
 // Global-table-only variant (no LDS table).
 __global__ __launch_bounds__(AGG_BLOCK) void k_aggregate_global(const DevQuery *__restrict__ qp,
                                                                 const DevBatch *__restrict__ bp,
@@ -394,10 +530,30 @@ struct DevOut {
     int32_t ncols;
     int32_t key_mode;
     uint64_t capacity;         // rows the columns hold
+    int32_t wide_acc[2];       // KM_WIDE: the MIN and MAX accumulators of component wide_comp
+    int32_t wide_comp;
+    int32_t pad;
+    uint64_t wide_seed;
+    uint32_t *flags;           // DevPlan::overflow
 };
 
-__device__ __forceinline__ uint64_t key_component(int key_mode, uint64_t key, int comp) {
-    if (key_mode == KM_PACK32) return comp == 0 ? (key >> 32) : (key & 0xFFFFFFFFULL);
+constexpr uint32_t OVF_WIDE = 8u;  // DevPlan::overflow bit: two KM_WIDE keys share a key word
+
+// KM_WIDE: the group's component kept in accumulators (flags a shared key word).
+__device__ __forceinline__ uint64_t wide_kept(const int32_t (&wa)[2], uint32_t *flags, const uint64_t *acc,
+                                            uint32_t stride) {
+    const uint64_t lo = acc[(uint64_t)wa[0] * stride], hi = acc[(uint64_t)wa[1] * stride];
+    if (lo != hi) __hip_atomic_fetch_or(flags, OVF_WIDE, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return lo;
+}
+
+__device__ __forceinline__ uint64_t key_component(const DevOut &o, uint64_t key, int comp, const uint64_t *acc,
+                                                  uint32_t stride) {
+    if (o.key_mode == KM_PACK32) return comp == 0 ? (key >> 32) : (key & 0xFFFFFFFFULL);
+    if (o.key_mode == KM_WIDE) {
+        const uint64_t c = wide_kept(o.wide_acc, o.flags, acc, stride);
+        return comp == o.wide_comp ? c : key ^ wide_mix(c, o.wide_seed);
+    }
     return key;
 }
 
@@ -412,7 +568,7 @@ __device__ __forceinline__ void write_group(const DevOut &o, uint64_t key, const
         uint64_t v = o.kind[c] == OUT_KEY ? 0 : acc[(uint64_t)va * stride];
         switch (o.kind[c]) {
         case OUT_KEY: {
-            uint64_t kv = key_component(o.key_mode, key, va);
+            uint64_t kv = key_component(o, key, va, acc, stride);
             ok = !null_key;
             if (o.key_type[c] == CK_I64) gst((int64_t *)o.data[c] + row, ok ? (int64_t)kv : (int64_t)0);
             else gst((int32_t *)o.data[c] + row, ok ? (int32_t)(uint32_t)kv : (int32_t)0);
@@ -503,6 +659,8 @@ struct DevTopn {
     uint64_t *cand_rec;   // candidate group states: key, accs
     uint8_t *cand_null;
     uint32_t cand_cap;
+    int32_t wide_acc[2];  // KM_WIDE: the MIN and MAX accumulators of the kept component
+    uint32_t *flags;      // DevPlan::overflow
 };
 
 __device__ __forceinline__ uint64_t key_order(const DevTopn &t, uint64_t key) {
@@ -519,6 +677,8 @@ __device__ __forceinline__ void group_order(const DevTopn &t, uint64_t key, cons
                                             uint64_t &tie) {
     uint64_t o;
     const uint64_t v = acc[(uint64_t)t.acc * stride];
+    // every group of a TopN query passes here once: the shared-key-word check
+    if (t.key_mode == KM_WIDE) (void)wide_kept(t.wide_acc, t.flags, acc, stride);
     switch (t.kind) {
     case TK_KEY: o = null_key ? 0 : key_order(t, key); break;
     case TK_ACC_I: o = v ^ 0x8000000000000000ULL; break;

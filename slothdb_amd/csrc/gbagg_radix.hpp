@@ -168,12 +168,24 @@ __global__ __launch_bounds__(256) void k_heavy_hist(const RadixParams *__restric
     if (threadIdx.x < 65) hist[threadIdx.x] = 0;
     if (threadIdx.x == 0) { s_distinct = 0; s_f1 = 0; }
     __syncthreads();
+    int64_t lo = INT64_MAX, hi = INT64_MIN;  // sampled key word range (plan->key_min/max preset)
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < rp.scap; i += gridDim.x * blockDim.x) {
         uint32_t c = rp.scnt[i];
         if (!c) continue;
         atomicAdd(&hist[min(c, 64u)], 1u);
         atomicAdd(&s_distinct, 1ULL);
         if (c == 1) atomicAdd(&s_f1, 1ULL);
+        lo = min(lo, (int64_t)rp.skeys[i]);
+        hi = max(hi, (int64_t)rp.skeys[i]);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        lo = min(lo, (int64_t)__shfl_xor(lo, off));
+        hi = max(hi, (int64_t)__shfl_xor(hi, off));
+    }
+    if ((threadIdx.x & 63) == 0 && lo <= hi) {
+        __hip_atomic_fetch_min(&plan->key_min, lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_fetch_max(&plan->key_max, hi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     __syncthreads();
     if (threadIdx.x < 65 && hist[threadIdx.x]) atomicAdd(&ghist[threadIdx.x], hist[threadIdx.x]);

@@ -1,10 +1,10 @@
 // gbagg_selector.hpp -- the learned strategy selector.
 //
-// A 32 -> 64 -> 64 -> 64 -> 4 ReLU MLP (FP32) reads features of the
+// A 32 -> 64 -> 64 -> 64 -> 5 ReLU MLP (FP32) reads features of the
 // heavy-hitter sample (the strided sample every AUTO plan takes first) and
 // predicts the distinct-group count, as log2((distinct + 1) / (default + 1)) / 4
-// over the sample's own estimate, and three path logits
-// (LDS_HASH, GLOBAL_HASH, RADIX). The operator uses it to pick the path and
+// over the sample's own estimate, and four path logits
+// (DIRECT, LDS_HASH, GLOBAL_HASH, RADIX). The operator uses it to pick the path and
 // to size the hash table; results never depend on it (every path is exact and
 // retries on table overflow), and a prediction that disagrees with the
 // sample's own estimate by more than 4x is ignored in favour of the default
@@ -20,7 +20,8 @@
 
 namespace gbagg {
 
-constexpr int SEL_IN = 32, SEL_H = 64, SEL_OUT = 4;
+constexpr int SEL_IN = 32, SEL_H = 64, SEL_OUT = 5;
+constexpr int SEL_CLS = SEL_OUT - 1;  // path logits
 // parameter layout (floats): W1[H][IN] b1[H] W2[H][H] b2[H] W3[H][H] b3[H]
 // W4[OUT][H] b4[OUT], then the input standardisation mu[IN] sigma[IN]
 constexpr int SEL_OFF_W1 = 0;
@@ -45,26 +46,35 @@ constexpr int SEL_NPARAM = SEL_OFF_SIG + SEL_IN;
 // feat[10..25] share of sampled keys with count in [2^j, 2^(j+1)), j = 0..15
 // feat[26] log2(GEE+1)/32, GEE = d - f1 + f1 sqrt(rows/n)
 // feat[27] log2(Chao1+1)/32, Chao1 = d + f1^2 / (2 max(f2, 1))
-// feat[28..30] key: one 4-byte column / one 8-byte column / two 4-byte columns
+// feat[28] key bytes / 16 (4, 8 or 16)    feat[29] two key columns
+// feat[30] log2(span + 1)/64, span = largest - smallest sampled key word
+//          (single-column keys; 1 otherwise)
 // feat[31] log2(rows/n + 1)/32
 __global__ __launch_bounds__(1024) void k_sel_features(const RadixParams *__restrict__ rpp,
                                                        const DevPlan *__restrict__ plan, uint64_t rows,
                                                        uint32_t key_kind, float *feat) {
     const RadixParams &rp = *rpp;
     __shared__ unsigned long long s_d, s_f1, s_f2, s_f3, s_max;
+    __shared__ long long s_lo, s_hi;
     __shared__ double s_ent;
     __shared__ unsigned long long s_hist[16];
     if (threadIdx.x == 0) {
         s_d = s_f1 = s_f2 = s_f3 = s_max = 0;
         s_ent = 0;
+        s_lo = INT64_MAX;
+        s_hi = INT64_MIN;
     }
     if (threadIdx.x < 16) s_hist[threadIdx.x] = 0;
     __syncthreads();
     unsigned long long d = 0, f1 = 0, f2 = 0, f3 = 0, mx = 0;
+    long long lo = INT64_MAX, hi = INT64_MIN;
     double ent = 0;
     for (uint32_t i = threadIdx.x; i < rp.scap; i += blockDim.x) {
         const uint32_t c = rp.scnt[i];
         if (!c) continue;
+        const long long k = (long long)rp.skeys[i];
+        lo = min(lo, k);
+        hi = max(hi, k);
         d++;
         f1 += c == 1;
         f2 += c == 2;
@@ -79,6 +89,8 @@ __global__ __launch_bounds__(1024) void k_sel_features(const RadixParams *__rest
     atomicAdd(&s_f2, f2);
     atomicAdd(&s_f3, f3);
     atomicMax(&s_max, mx);
+    atomicMin(&s_lo, lo);
+    atomicMax(&s_hi, hi);
     atomicAdd(&s_ent, ent);
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -99,9 +111,12 @@ __global__ __launch_bounds__(1024) void k_sel_features(const RadixParams *__rest
         for (int j = 0; j < 16; j++) feat[10 + j] = (float)(s_hist[j] / D);
         feat[26] = (float)(log2(gee + 1) / 32);
         feat[27] = (float)(log2(chao + 1) / 32);
-        feat[28] = key_kind == 0;
-        feat[29] = key_kind == 1;
-        feat[30] = key_kind == 2;
+        // key_kind: 0 one 4-byte column, 1 one 8-byte column, 2 two 4-byte
+        // columns, 3 a 16-byte key
+        feat[28] = key_kind == 0 ? 0.25f : key_kind == 3 ? 1.f : 0.5f;
+        feat[29] = key_kind >= 2;
+        const double span = s_lo <= s_hi ? (double)((unsigned long long)s_hi - (unsigned long long)s_lo) : 0.0;
+        feat[30] = key_kind <= 1 ? (float)(log2(span + 1) / 64) : 1.f;
         feat[31] = (float)(log2(R / n + 1) / 32);
     }
 }
@@ -136,7 +151,7 @@ __global__ __launch_bounds__(SEL_H) void k_sel_infer(const float *__restrict__ P
 }
 
 // ---- training -------------------------------------------------------------
-// Loss per row: (out0 - y_log)^2 + lambda * cross-entropy(out1..3, y_class),
+// Loss per row: (out0 - y_log)^2 + lambda * cross-entropy(out1..SEL_CLS, y_class),
 // averaged over a minibatch of SEL_B rows (the last one may be short). Rows
 // are visited in the given order every epoch. Adam (beta1 0.9, beta2 0.999,
 // eps 1e-8). M and V hold the moments (SEL_NW floats each, zeroed by the
@@ -222,18 +237,21 @@ __global__ __launch_bounds__(SEL_TT) void k_sel_train(float *P, float *M, float 
             if (threadIdx.x < (uint32_t)nb) {
                 const int b = threadIdx.x;
                 const float *y = Y + (uint64_t)(r0 + b) * 2;
-                const float diff = o[b * SEL_OUT] - y[0];
+                float *ob = o + b * SEL_OUT;
+                const float diff = ob[0] - y[0];
                 const int cls = (int)y[1];
-                const float l1 = o[b * SEL_OUT + 1], l2 = o[b * SEL_OUT + 2], l3 = o[b * SEL_OUT + 3];
-                const float mxl = fmaxf(l1, fmaxf(l2, l3));
-                const float e1 = expf(l1 - mxl), e2 = expf(l2 - mxl), e3 = expf(l3 - mxl), se = e1 + e2 + e3;
-                const float lc = (cls == 0 ? l1 : cls == 1 ? l2 : l3) - mxl;
+                float mxl = ob[1];
+                for (int c = 1; c < SEL_CLS; c++) mxl = fmaxf(mxl, ob[1 + c]);
+                float e[SEL_CLS], se = 0.f;
+                for (int c = 0; c < SEL_CLS; c++) {
+                    e[c] = expf(ob[1 + c] - mxl);
+                    se += e[c];
+                }
+                const float lc = ob[1 + cls] - mxl;
                 atomicAdd(&s_loss, diff * diff + lambda * (logf(se) - lc));
                 const float inv = 1.f / (float)nb;
-                o[b * SEL_OUT + 0] = 2.f * diff * inv;
-                o[b * SEL_OUT + 1] = lambda * (e1 / se - (cls == 0)) * inv;
-                o[b * SEL_OUT + 2] = lambda * (e2 / se - (cls == 1)) * inv;
-                o[b * SEL_OUT + 3] = lambda * (e3 / se - (cls == 2)) * inv;
+                ob[0] = 2.f * diff * inv;
+                for (int c = 0; c < SEL_CLS; c++) ob[1 + c] = lambda * (e[c] / se - (cls == c)) * inv;
             }
             __syncthreads();
             // layer 4, then 3, 2, 1: the activation gradient below each layer

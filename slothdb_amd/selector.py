@@ -1,10 +1,10 @@
 """The learned strategy selector: parameters, GPU training and inference
 through the C ABI (gbagg_selector_*), and attachment to an operator.
 
-The model is a 32 -> 64 -> 64 -> 64 -> 4 ReLU MLP in FP32. Outputs: the
+The model is a 32 -> 64 -> 64 -> 64 -> 5 ReLU MLP in FP32. Outputs: the
 distinct-group count as log2((distinct + 1) / (default + 1)) / 4, a correction
 to the sample's own estimate (default_estimate), and logits of the paths
-LDS_HASH, GLOBAL_HASH, RADIX. The parameter vector follows include/slothdb_gpu.h
+DIRECT, LDS_HASH, GLOBAL_HASH, RADIX. The parameter vector follows include/slothdb_gpu.h
 (GBAGG_SELECTOR_PARAMS floats)."""
 import ctypes as C
 import json
@@ -15,11 +15,11 @@ import numpy as np
 
 from . import gbagg as G
 
-N_IN, N_H, N_OUT = 32, 64, 4
+N_IN, N_H, N_OUT = 32, 64, 5
 LAYERS = [(N_H, N_IN), (N_H, N_H), (N_H, N_H), (N_OUT, N_H)]
 N_W = sum(o * i + o for o, i in LAYERS)
 N_PARAMS = N_W + 2 * N_IN
-PATHS = [G.LDS_HASH, G.GLOBAL_HASH, G.RADIX]
+PATHS = [G.DIRECT, G.LDS_HASH, G.GLOBAL_HASH, G.RADIX]
 WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "selector_weights.json")
 
 
@@ -61,7 +61,7 @@ def train(x, y, params, epochs=300, lr=1e-3, lam=0.5):
 
 
 def infer(params, x):
-    """Forward pass on the GPU (gbagg_selector_infer): n x 4 outputs."""
+    """Forward pass on the GPU (gbagg_selector_infer): n x N_OUT outputs."""
     x, p = _f32(x), _f32(params)
     out = np.zeros((len(x), N_OUT), np.float32)
     if len(x):

@@ -3,12 +3,15 @@
     python -m slothdb_amd.selector_train [--queries N] [--rows-hi R] [--out PATH]
 
 For each synthetic query (rows log-uniform in [1e4, rows_hi], distinct keys
-log-uniform in [1, rows], Zipf s uniform in [0, 1.5], key = one INTEGER, one
-BIGINT or two INTEGER columns; SELECT key, COUNT(*), SUM(v) GROUP BY key):
+log-uniform in [1, rows], Zipf s uniform in [0, 1.5], key = one INTEGER
+(sparse ids, or dense ranks from an offset), one BIGINT, two INTEGER or two
+BIGINT columns, i.e. 4, 8 or 16 key bytes;
+SELECT key, COUNT(*), SUM(v) GROUP BY key):
 
   * the AUTO plan's sample features (gbagg_plan_features),
   * the exact group count (every path returns it; they must agree),
-  * the device time of each path LDS_HASH, GLOBAL_HASH, RADIX (best of 2).
+  * the device time of each path DIRECT, LDS_HASH, GLOBAL_HASH, RADIX (best
+    of 2; DIRECT only where the key range allows it, else never the label).
 
 Labels: the group count as selector.target (a correction to the sample's own
 estimate) and the fastest path. The model trains on 80% of
@@ -26,7 +29,7 @@ import numpy as np
 
 from . import datagen, gbagg as G, selector as S
 
-KEYS = ("i32", "i64", "2xi32")
+KEYS = ("i32", "i32dense", "i64", "2xi32", "2xi64")
 
 
 def _mix64(x):
@@ -45,6 +48,16 @@ def make_query(rows, distinct, s, key, seed):
     ranks = datagen.zipf_ranks(rows, distinct, s, seed=seed)
     ids = _mix64(ranks ^ np.uint64(seed & 0xFFFFFFFF))
     v = (ranks.astype(np.int64) % 1000) - 500
+    if key == "i32dense":
+        k = (ranks.astype(np.int64) - (seed % 100_000)).astype(np.int32)
+        q = G.Query([G.INTEGER, G.BIGINT], [0], [(G.COUNT_STAR, None), (G.SUM, 1)], nullable=[False, False])
+        return q, [G.Column(G.INTEGER, k), G.Column(G.BIGINT, v)]
+    if key == "2xi64":
+        ids2 = _mix64(ranks ^ np.uint64(0xA5A5A5A5 ^ (seed & 0xFFFF)))
+        q = G.Query([G.BIGINT, G.BIGINT, G.BIGINT], [0, 1], [(G.COUNT_STAR, None), (G.SUM, 2)],
+                    nullable=[False] * 3)
+        return q, [G.Column(G.BIGINT, ids.view(np.int64)), G.Column(G.BIGINT, ids2.view(np.int64)),
+                   G.Column(G.BIGINT, v)]
     if key == "i64":
         q = G.Query([G.BIGINT, G.BIGINT], [0], [(G.COUNT_STAR, None), (G.SUM, 1)], nullable=[False, False])
         return q, [G.Column(G.BIGINT, ids.view(np.int64)), G.Column(G.BIGINT, v)]
@@ -61,7 +74,8 @@ def specs(n, rows_hi, seed):
         u = [datagen.rand(seed, 60 + j, i) / 2.0 ** 64 for j in range(4)]
         rows = int(round(math.exp(math.log(1e4) + u[0] * (math.log(rows_hi) - math.log(1e4)))))
         distinct = max(1, min(rows, int(round(math.exp(u[1] * math.log(rows))))))
-        out.append((rows, distinct, 1.5 * u[2], KEYS[int(u[3] * 3) % 3], (seed * 7919 + i) & 0xFFFFFFFFFFFF))
+        out.append((rows, distinct, 1.5 * u[2], KEYS[int(u[3] * len(KEYS)) % len(KEYS)],
+                    (seed * 7919 + i) & 0xFFFFFFFFFFFF))
     return out
 
 
@@ -74,7 +88,11 @@ def run_plan(q, dcols, strategy, reps=2, want_features=False):
         op.consume_device(dcols)
         op.finalize()
         info = op.plan_info()
-        best = min(best, info["ms_total"])
+        # a forced DIRECT over a too-wide key range runs another plan: not a DIRECT time
+        if strategy == G.DIRECT and info["strategy"] != G.DIRECT:
+            best = math.inf
+        else:
+            best = min(best, info["ms_total"])
         groups = info["groups"]
     if want_features:
         feat = S.features(op)
@@ -84,10 +102,11 @@ def run_plan(q, dcols, strategy, reps=2, want_features=False):
     return best, groups, feat, used
 
 
-def collect(n, rows_hi, seed, log=print):
+def collect(n, rows_hi, seed, log=print, first=0, count=None, dump=None):
     X, Y, T, AUTO_T, DEF, G_ = [], [], [], [], [], []
     t0 = time.time()
-    for i, (rows, distinct, s, key, qseed) in enumerate(specs(n, rows_hi, seed)):
+    todo = specs(n, rows_hi, seed)[first:None if count is None else first + count]
+    for i, (rows, distinct, s, key, qseed) in enumerate(todo, start=first):
         q, cols = make_query(rows, distinct, s, key, qseed)
         dcols = [G.DeviceColumn(c) for c in cols]
         t_auto, g_auto, feat, used = run_plan(q, dcols, G.AUTO, want_features=True)
@@ -108,8 +127,23 @@ def collect(n, rows_hi, seed, log=print):
         DEF.append(float(S.default_estimate(feat)))
         if (i + 1) % 50 == 0:
             log(f"[selector] collected {i + 1}/{n} queries in {time.time() - t0:.0f}s", flush=True)
-    return (np.array(X, np.float32), np.array(Y, np.float32), np.array(T), np.array(AUTO_T), np.array(DEF),
-            np.array(G_, np.float64))
+            if dump:  # progress survives an interrupted collection
+                _save(dump, X, Y, T, AUTO_T, DEF, G_)
+    out = (np.array(X, np.float32), np.array(Y, np.float32), np.array(T), np.array(AUTO_T), np.array(DEF),
+           np.array(G_, np.float64))
+    if dump:
+        _save(dump, *out)
+    return out
+
+
+def _save(path, X, Y, T, AUTO_T, DEF, GT):
+    np.savez(path, X=np.array(X, np.float32), Y=np.array(Y, np.float32), T=np.array(T), AUTO_T=np.array(AUTO_T),
+             DEF=np.array(DEF), GT=np.array(GT, np.float64))
+
+
+def _load(paths):
+    parts = [np.load(p) for p in paths.split(",")]
+    return tuple(np.concatenate([z[k] for z in parts]) for k in ("X", "Y", "T", "AUTO_T", "DEF", "GT"))
 
 
 def evaluate(params, X, Y, T, AUTO_T, DEF, true):
@@ -120,7 +154,10 @@ def evaluate(params, X, Y, T, AUTO_T, DEF, true):
     path = np.argmax(out[:, 1:], axis=1)
     # the operator's rule: the selector's plan unless it is 4x off the default estimate
     used = (pred <= 4 * DEF) & (pred * 4 >= DEF)
-    t_sel = np.where(used, T[np.arange(len(T)), path], AUTO_T)
+    t_path = T[np.arange(len(T)), path]
+    # DIRECT over a too-wide key range falls back to LDS_HASH in the operator
+    t_path = np.where(np.isfinite(t_path), t_path, T[:, S.PATHS.index(G.LDS_HASH)])
+    t_sel = np.where(used, t_path, AUTO_T)
     best = T.min(axis=1)
     return {
         "queries": int(len(X)),
@@ -130,7 +167,8 @@ def evaluate(params, X, Y, T, AUTO_T, DEF, true):
         "selector_used": float(np.mean(used)),
         "time_selector_over_best": float(t_sel.sum() / best.sum()),
         "time_default_over_best": float(AUTO_T.sum() / best.sum()),
-        "time_radix_only_over_best": float(T[:, 2].sum() / best.sum()),
+        "time_radix_only_over_best": float(T[:, S.PATHS.index(G.RADIX)].sum() / best.sum()),
+        "path_label_counts": [int(np.sum(Y[:, 1] == c)) for c in range(len(S.PATHS))],
     }
 
 
@@ -142,16 +180,19 @@ def main():
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--out", default=S.WEIGHTS)
-    ap.add_argument("--dump", help="also save the collected data (.npz)")
-    ap.add_argument("--data", help="train on data saved by --dump instead of collecting")
+    ap.add_argument("--dump", help="also save the collected data (.npz), every 50 queries")
+    ap.add_argument("--first", type=int, default=0, help="collect queries first .. first + count - 1 only")
+    ap.add_argument("--count", type=int, default=None)
+    ap.add_argument("--collect-only", action="store_true")
+    ap.add_argument("--data", help="train on data saved by --dump (comma-separated files) instead of collecting")
     args = ap.parse_args()
     if args.data:
-        z = np.load(args.data)
-        X, Y, T, AUTO_T, DEF, GT = (z[k] for k in ("X", "Y", "T", "AUTO_T", "DEF", "GT"))
+        X, Y, T, AUTO_T, DEF, GT = _load(args.data)
     else:
-        X, Y, T, AUTO_T, DEF, GT = collect(args.queries, args.rows_hi, args.seed)
-        if args.dump:
-            np.savez(args.dump, X=X, Y=Y, T=T, AUTO_T=AUTO_T, DEF=DEF, GT=GT)
+        X, Y, T, AUTO_T, DEF, GT = collect(args.queries, args.rows_hi, args.seed, first=args.first,
+                                           count=args.count, dump=args.dump)
+        if args.collect_only:
+            return 0
     n_train = int(0.8 * len(X))
     mu = X[:n_train].mean(axis=0)
     sigma = X[:n_train].std(axis=0) + 1e-3

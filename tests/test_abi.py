@@ -60,11 +60,16 @@ def test_valid_plans_create_without_device():
     assert _plan(G.Query([G.INTEGER, G.VARCHAR, G.INTEGER], [0, 1], [(G.SUM, 2)],
                          predicates=[(2, G.LT, 30)], nullable=[0, 0, 0], topn=(2, False, 10)))[0] == G.OK
     assert _plan(G.Query([G.DOUBLE], [], [(G.AVG, 0), (G.MIN, 0)]))[0] == G.OK
+    # 16-byte keys: two BIGINT columns, BIGINT with a VARCHAR or INTEGER column
+    for types in ([G.BIGINT, G.BIGINT], [G.VARCHAR, G.BIGINT], [G.BIGINT, G.INTEGER]):
+        assert _plan(G.Query(types + [G.BIGINT], [0, 1], [(G.COUNT_STAR, None), (G.SUM, 2)],
+                             nullable=[0, 0, 0]))[0] == G.OK
 
 
 @pytest.mark.parametrize("q,needle", [
     (G.Query([G.DOUBLE], [0], [(G.COUNT_STAR, None)]), "DOUBLE group keys"),
-    (G.Query([G.BIGINT, G.BIGINT], [0, 1], [(G.COUNT_STAR, None)], nullable=[0, 0]), "BIGINT"),
+    (G.Query([G.INTEGER] * 3, [0, 1, 2], [(G.COUNT_STAR, None)], nullable=[0, 0, 0]), "at most 2 group"),
+    (G.Query([G.BIGINT, G.BIGINT], [0, 1], [(G.COUNT_STAR, None)]), "drop_null_keys"),
     (G.Query([G.INTEGER, G.INTEGER], [0, 1], [(G.COUNT_STAR, None)]), "drop_null_keys"),
     (G.Query([G.INTEGER, G.VARCHAR], [0], [(G.SUM, 1)]), "numeric"),
     (G.Query([G.INTEGER, G.VARCHAR], [0], [(G.MIN, 1)]), "VARCHAR"),
