@@ -93,6 +93,14 @@ typedef struct {
  *   SUM(c)    BIGINT for INTEGER/BIGINT c, DOUBLE for DOUBLE c; NULL if no input
  *   AVG(c)    DOUBLE; NULL if no input
  *   MIN/MAX   type of c; NULL if no input
+ *   COUNT(DISTINCT c)  BIGINT, distinct non-NULL values of an INTEGER, BIGINT
+ *             or VARCHAR c (the generic path's distinct_set,
+ *             physical_planner.cpp:12325-12330). Runs in two phases: GROUP BY
+ *             (group column, c) with partial states of the other aggregates,
+ *             then GROUP BY the group column over those rows. All DISTINCT
+ *             aggregates of a query share one c; at most one group column,
+ *             which must be non-nullable or have drop_null_keys; a nullable c
+ *             allows only DISTINCT aggregates next to it.
  * Integer SUM is exact two's-complement int64 (the reference accumulates in
  * double and casts; the two agree while partial sums stay below 2^53). */
 typedef enum {
@@ -102,6 +110,7 @@ typedef enum {
     GBAGG_AGG_AVG = 3,
     GBAGG_AGG_MIN = 4,
     GBAGG_AGG_MAX = 5,
+    GBAGG_AGG_COUNT_DISTINCT = 6,
 } gbagg_agg_kind;
 
 typedef struct {

@@ -230,13 +230,14 @@ typedef struct {
     uint8_t *fhas;
 } accs;
 
-typedef struct {
+typedef struct gtable {
     uint32_t na;
     uint64_t ngroups, cap_groups;
     gkey *keys;
     accs a;
     uint64_t tcap;
     int64_t *table; /* group index or -1 */
+    struct gtable *dset; /* COUNT(DISTINCT): one entry per (group * 16 + agg, value) seen */
 } gtable;
 
 static inline uint64_t gkey_hash(const gkey *k) {
@@ -307,6 +308,7 @@ static uint64_t gtable_find(gtable *t, const gkey *k, uint64_t h) {
 }
 
 static void gtable_free(gtable *t) {
+    if (t->dset) { gtable_free(t->dset); free(t->dset); }
     free(t->keys); free(t->table);
     free(t->a.cnt); free(t->a.isum); free(t->a.fsum); free(t->a.fcomp);
     free(t->a.imin); free(t->a.imax); free(t->a.fmin); free(t->a.fmax); free(t->a.fhas);
@@ -324,6 +326,7 @@ typedef struct {
     const gbagg_query_desc *q;
     const gbagg_column_view *cols;
     uint32_t **canon; /* per group col: canonical dictionary codes or NULL */
+    uint32_t **acanon; /* per aggregate: canonical codes of a VARCHAR COUNT(DISTINCT) argument */
     pred_rt *preds;
 } exact_ctx;
 
@@ -354,6 +357,23 @@ static inline void row_update(const exact_ctx *x, gtable *t, uint64_t g, uint64_
         if (ag->kind == GBAGG_AGG_COUNT_STAR) { t->a.cnt[i]++; continue; }
         const gbagg_column_view *c = &x->cols[ag->column];
         if (!col_valid(c, r)) continue;
+        if (ag->kind == GBAGG_AGG_COUNT_DISTINCT) {
+            /* distinct non-NULL values per group (physical_planner.cpp:12325-12330) */
+            gkey dk;
+            int64_t v = col_i64(c, r);
+            if (x->acanon[a]) v = x->acanon[a][(uint32_t)v];
+            dk.k[0] = (int64_t)(g * 16 + a);
+            dk.k[1] = v;
+            dk.nullmask = 0;
+            if (!t->dset) {
+                t->dset = xcalloc(1, sizeof(gtable));
+                gtable_init(t->dset, 0);
+            }
+            uint64_t before = t->dset->ngroups;
+            gtable_find(t->dset, &dk, gkey_hash(&dk));
+            if (t->dset->ngroups > before) t->a.cnt[i]++;
+            continue;
+        }
         t->a.cnt[i]++;
         if (ag->kind == GBAGG_AGG_COUNT) continue;
         if (c->type == GBAGG_TYPE_DOUBLE) {
@@ -439,7 +459,8 @@ static int out_type_of(const gbagg_query_desc *q, const gbagg_column_view *cols,
     const gbagg_agg *ag = &q->aggs[a];
     switch (ag->kind) {
     case GBAGG_AGG_COUNT_STAR:
-    case GBAGG_AGG_COUNT: return GBAGG_TYPE_BIGINT;
+    case GBAGG_AGG_COUNT:
+    case GBAGG_AGG_COUNT_DISTINCT: return GBAGG_TYPE_BIGINT;
     case GBAGG_AGG_AVG: return GBAGG_TYPE_DOUBLE;
     case GBAGG_AGG_SUM:
         return cols[ag->column].type == GBAGG_TYPE_DOUBLE ? GBAGG_TYPE_DOUBLE : GBAGG_TYPE_BIGINT;
@@ -477,6 +498,7 @@ static void emit_exact(const gbagg_query_desc *q, const gbagg_column_view *cols,
         switch (ag->kind) {
         case GBAGG_AGG_COUNT_STAR:
         case GBAGG_AGG_COUNT:
+        case GBAGG_AGG_COUNT_DISTINCT:
             out->cells[c][row] = cnt;
             out->valid[c][row] = 1;
             break;
@@ -521,12 +543,17 @@ int orc_groupby_exact(const gbagg_query_desc *q, const gbagg_column_view *cols, 
     if (threads < 1) threads = 1;
     exact_ctx x;
     uint32_t *canon[GBAGG_MAX_GROUP_COLS] = {0};
+    uint32_t *acanon[GBAGG_MAX_AGGS] = {0};
     pred_rt rt[GBAGG_MAX_PREDICATES];
     x.q = q;
     x.cols = cols;
     for (uint32_t g = 0; g < q->num_group_cols; g++)
         if (cols[q->group_cols[g]].type == GBAGG_TYPE_VARCHAR) canon[g] = dict_canonical(&cols[q->group_cols[g]]);
     x.canon = canon;
+    for (uint32_t a = 0; a < q->num_aggs; a++)
+        if (q->aggs[a].kind == GBAGG_AGG_COUNT_DISTINCT && cols[q->aggs[a].column].type == GBAGG_TYPE_VARCHAR)
+            acanon[a] = dict_canonical(&cols[q->aggs[a].column]);
+    x.acanon = acanon;
     preds_prepare(q, cols, rt);
     x.preds = rt;
 
@@ -569,7 +596,7 @@ int orc_groupby_exact(const gbagg_query_desc *q, const gbagg_column_view *cols, 
     if (scalar_agg) {
         for (uint32_t a = 0; a < q->num_aggs; a++) {
             int k = q->aggs[a].kind;
-            out->valid[a][0] = (k == GBAGG_AGG_COUNT_STAR || k == GBAGG_AGG_COUNT);
+            out->valid[a][0] = (k == GBAGG_AGG_COUNT_STAR || k == GBAGG_AGG_COUNT || k == GBAGG_AGG_COUNT_DISTINCT);
         }
     }
     for (int j = 0; j < nj; j++)
@@ -578,6 +605,7 @@ int orc_groupby_exact(const gbagg_query_desc *q, const gbagg_column_view *cols, 
     free(th);
     preds_release(q, rt);
     for (uint32_t g = 0; g < q->num_group_cols; g++) free(canon[g]);
+    for (uint32_t a = 0; a < q->num_aggs; a++) free(acanon[a]);
     return 0;
 }
 

@@ -279,6 +279,11 @@ struct gbagg_handle {
     KeyDict kdict[2];
     bool key_varchar[2] = {false, false};
     int32_t wide_acc[2] = {0, 0};  // KM_WIDE: MIN / MAX of the kept key component
+    // COUNT(DISTINCT): phase 1 groups by (group column, argument) with partial
+    // states of the other aggregates; this handle is phase 2 over its result
+    std::unique_ptr<gbagg_handle> phase1;
+    std::vector<uint32_t> p1_dict_off;  // phase 1's VARCHAR group key dictionary, as phase 2 input
+    std::vector<char> p1_dict_bytes;
     uint64_t wide_seed0 = 0;       // KM_WIDE: first seed of wide_mix
     // TopN on the device
     bool topn_device = false;
@@ -360,8 +365,14 @@ int add_acc(gbagg_handle *h, int op, int u) {
     return h->dq.nacc++;
 }
 
+void build_distinct(gbagg_handle *h, const gbagg_query_desc *q);
+void pick_kernels(gbagg_handle *h);
+
 void build_plan(gbagg_handle *h, const gbagg_query_desc *q) {
     if (!q) fail(GBAGG_INVALID, "null query");
+    if (q->aggs && q->num_aggs <= GBAGG_MAX_AGGS)
+        for (uint32_t a = 0; a < q->num_aggs; a++)
+            if (q->aggs[a].kind == GBAGG_AGG_COUNT_DISTINCT) return build_distinct(h, q);
     if (q->num_columns == 0 || q->num_columns > GBAGG_MAX_COLUMNS || !q->column_types)
         fail(GBAGG_INVALID, "num_columns must be 1..16 with column_types");
     for (uint32_t c = 0; c < q->num_columns; c++) {
@@ -562,6 +573,151 @@ void build_plan(gbagg_handle *h, const gbagg_query_desc *q) {
         h->topn_k = (uint32_t)std::min<uint64_t>(h->topn_limit, 32);
         t.k = h->topn_k;
     }
+}
+
+// COUNT(DISTINCT x) in two phases (the generic path's per-group distinct_set,
+// physical_planner.cpp:12325-12330, as a decomposed aggregate):
+//   phase 1  GROUP BY (k, x) with partial states of the other aggregates
+//            (COUNT, SUM, MIN, MAX; AVG as SUM and COUNT), NULL keys dropped
+//   phase 2  GROUP BY k over phase 1's device-resident result: COUNT(*) is the
+//            distinct count, the partial states combine (SUM of COUNT and SUM,
+//            MIN of MIN, MAX of MAX, AVG = SUM of sums / SUM of counts)
+// Rows with a NULL x must still reach the other aggregates, so a nullable x
+// allows only DISTINCT aggregates; a nullable k needs drop_null_keys.
+void build_distinct(gbagg_handle *h, const gbagg_query_desc *q) {
+    if (q->num_columns == 0 || q->num_columns > GBAGG_MAX_COLUMNS || !q->column_types)
+        fail(GBAGG_INVALID, "num_columns must be 1..16 with column_types");
+    if (q->num_group_cols > 1) fail(GBAGG_INVALID, "COUNT(DISTINCT) supports at most one group column");
+    if (q->num_group_cols && (!q->group_cols || q->group_cols[0] >= q->num_columns))
+        fail(GBAGG_INVALID, "group column out of range");
+    auto nullable = [&](uint32_t c) { return q->column_nullable ? q->column_nullable[c] != 0 : true; };
+    int32_t x = -1;
+    bool only_distinct = true;
+    for (uint32_t a = 0; a < q->num_aggs; a++) {
+        const gbagg_agg &g = q->aggs[a];
+        if (g.kind != GBAGG_AGG_COUNT_STAR && (g.column < 0 || (uint32_t)g.column >= q->num_columns))
+            fail(GBAGG_INVALID, "aggregate column out of range");
+        if (g.kind != GBAGG_AGG_COUNT_DISTINCT) {
+            only_distinct = false;
+            continue;
+        }
+        if (x >= 0 && g.column != x) fail(GBAGG_INVALID, "all COUNT(DISTINCT) aggregates must share one column");
+        x = g.column;
+    }
+    const gbagg_type xt = q->column_types[x];
+    if (xt != GBAGG_TYPE_INTEGER && xt != GBAGG_TYPE_BIGINT && xt != GBAGG_TYPE_VARCHAR)
+        fail(GBAGG_INVALID, "COUNT(DISTINCT) supports INTEGER, BIGINT and VARCHAR arguments");
+    const uint32_t ng = q->num_group_cols;
+    if (ng && nullable(q->group_cols[0]) && !q->drop_null_keys)
+        fail(GBAGG_INVALID, "COUNT(DISTINCT) over a nullable group column requires drop_null_keys");
+    if (nullable((uint32_t)x) && !only_distinct)
+        fail(GBAGG_INVALID, "a nullable COUNT(DISTINCT) argument allows only DISTINCT aggregates");
+
+    // phase 1: partial states, deduplicated
+    std::vector<gbagg_agg> a1;
+    auto p1_agg = [&](gbagg_agg_kind k, int32_t c) {
+        for (size_t i = 0; i < a1.size(); i++)
+            if (a1[i].kind == k && a1[i].column == c) return (uint32_t)i;
+        a1.push_back(gbagg_agg{k, c});
+        return (uint32_t)(a1.size() - 1);
+    };
+    std::vector<uint32_t> src(q->num_aggs, 0), src_cnt(q->num_aggs, 0);  // phase-1 agg feeding user agg a
+    for (uint32_t a = 0; a < q->num_aggs; a++) {
+        const gbagg_agg &g = q->aggs[a];
+        switch (g.kind) {
+        case GBAGG_AGG_COUNT_DISTINCT: break;
+        case GBAGG_AGG_COUNT_STAR: src[a] = p1_agg(GBAGG_AGG_COUNT_STAR, -1); break;
+        case GBAGG_AGG_COUNT: src[a] = p1_agg(GBAGG_AGG_COUNT, g.column); break;
+        case GBAGG_AGG_SUM: src[a] = p1_agg(GBAGG_AGG_SUM, g.column); break;
+        case GBAGG_AGG_AVG:
+            src[a] = p1_agg(GBAGG_AGG_SUM, g.column);
+            src_cnt[a] = p1_agg(GBAGG_AGG_COUNT, g.column);
+            break;
+        case GBAGG_AGG_MIN:
+        case GBAGG_AGG_MAX: src[a] = p1_agg(g.kind, g.column); break;
+        default: fail(GBAGG_INVALID, "unknown aggregate kind");
+        }
+    }
+    uint32_t gc1[2];
+    if (ng) gc1[0] = q->group_cols[0];
+    gc1[ng] = (uint32_t)x;
+    gbagg_query_desc d1 = *q;
+    d1.num_group_cols = ng + 1;
+    d1.group_cols = gc1;
+    d1.num_aggs = (uint32_t)a1.size();
+    d1.aggs = a1.empty() ? nullptr : a1.data();
+    d1.topn_col = -1;
+    d1.topn_limit = 0;
+    d1.row_limit = 0;
+    d1.drop_null_keys = 1;
+    h->phase1 = std::make_unique<gbagg_handle>();
+    build_plan(h->phase1.get(), &d1);
+    pick_kernels(h->phase1.get());
+
+    // phase 2 over phase 1's output columns: [k,] x, partial states
+    const gbagg_handle *p1 = h->phase1.get();
+    std::vector<gbagg_type> t2;
+    std::vector<uint8_t> n2;
+    for (size_t c = 0; c < p1->outs.size(); c++) {
+        t2.push_back(p1->outs[c].type);
+        const bool key = c <= ng;
+        const bool cnt = !key && p1->outs[c].kind == OUT_COUNT;
+        const int32_t arg = key ? -1 : a1[c - ng - 1].column;
+        n2.push_back(key || cnt ? 0 : (arg >= 0 && nullable((uint32_t)arg)) ? 1 : 0);
+    }
+    const uint32_t base = ng + 1;  // first partial-state column
+    std::vector<gbagg_agg> a2;
+    std::vector<int> avg_hidden(q->num_aggs, -1);
+    for (uint32_t a = 0; a < q->num_aggs; a++) {
+        const gbagg_agg &g = q->aggs[a];
+        const int32_t col = (int32_t)(base + src[a]);
+        switch (g.kind) {
+        case GBAGG_AGG_COUNT_DISTINCT: a2.push_back({GBAGG_AGG_COUNT_STAR, -1}); break;
+        case GBAGG_AGG_MIN:
+        case GBAGG_AGG_MAX: a2.push_back({g.kind, col}); break;
+        default: a2.push_back({GBAGG_AGG_SUM, col}); break;  // COUNT(*), COUNT, SUM; AVG's sum
+        }
+    }
+    for (uint32_t a = 0; a < q->num_aggs; a++)
+        if (q->aggs[a].kind == GBAGG_AGG_AVG) {  // AVG's count: a hidden output, removed below
+            avg_hidden[a] = (int)a2.size();
+            a2.push_back({GBAGG_AGG_SUM, (int32_t)(base + src_cnt[a])});
+        }
+    uint32_t gc2 = 0;
+    gbagg_query_desc d2{};
+    d2.num_columns = (uint32_t)t2.size();
+    d2.column_types = t2.data();
+    d2.column_nullable = n2.data();
+    d2.num_group_cols = ng;
+    d2.group_cols = &gc2;
+    d2.num_aggs = (uint32_t)a2.size();
+    d2.aggs = a2.data();
+    d2.topn_col = q->topn_col;
+    d2.topn_ascending = q->topn_ascending;
+    d2.topn_limit = q->topn_limit;
+    d2.row_limit = q->row_limit;
+    d2.drop_null_keys = q->drop_null_keys;
+    d2.strategy = GBAGG_STRATEGY_AUTO;
+    build_plan(h, &d2);
+    // user-visible semantics of the phase-2 outputs
+    bool topn_avg = false;
+    for (uint32_t a = 0; a < q->num_aggs; a++) {
+        OutCol &oc = h->outs[ng + a];
+        const gbagg_agg_kind k = q->aggs[a].kind;
+        if (k == GBAGG_AGG_COUNT_STAR || k == GBAGG_AGG_COUNT) {
+            oc.kind = OUT_COUNT;  // sums of counts: never NULL
+            oc.cnt_acc = -1;
+        } else if (k == GBAGG_AGG_AVG) {
+            const OutCol &hid = h->outs[ng + avg_hidden[a]];
+            oc.kind = oc.kind == OUT_SUM_F ? OUT_AVG_F : OUT_AVG_I;
+            oc.type = GBAGG_TYPE_DOUBLE;
+            oc.cnt_acc = hid.val_acc;
+            topn_avg = topn_avg || (int32_t)(ng + a) == q->topn_col;
+        }
+    }
+    h->outs.resize(ng + q->num_aggs);
+    if (topn_avg) h->topn_device = false;  // AVG order: the full result is sorted on the host
+    h->strategy = q->strategy == GBAGG_STRATEGY_DIRECT ? GBAGG_STRATEGY_AUTO : q->strategy;
 }
 
 ShapeSig plan_sig(const gbagg_handle *h) {
@@ -1171,10 +1327,13 @@ bool run_radix(gbagg_handle *h, Timer &tm, uint64_t &est_out) {
             est_out = est;
             return false;
         }
-        // fanout: half-full LDS tables, and at most 64K rows per partition
-        // workgroup (few groups over many rows still spread over the CUs)
+        // fanout: LDS tables at most 4/5 full on average (fewer partitions mean
+        // fewer partially written record lines in the scatter; the per-partition
+        // fill limit is 7/8, several standard deviations of a partition's group
+        // count above the mean at these sizes), and at most 64K rows per
+        // partition workgroup (few groups over many rows still spread over the CUs)
         uint32_t pbits = 0;
-        while (pbits < rp.bucket_bits && ((double)cold / (double)(1u << pbits) > 0.5 * S ||
+        while (pbits < rp.bucket_bits && ((double)cold / (double)(1u << pbits) > 0.8 * S ||
                                           (double)h->rows_total / (double)(1u << pbits) > 65536.0))
             pbits++;
         if (h->force_part_bits >= 0) pbits = std::min<uint32_t>((uint32_t)h->force_part_bits, rp.bucket_bits);
@@ -1341,8 +1500,11 @@ void run_direct(gbagg_handle *h, Timer &tm, uint64_t span, int64_t kmin) {
 
 void finalize_once(gbagg_handle *h);
 
+void run_phase1(gbagg_handle *h);
+
 void finalize(gbagg_handle *h) {
     if (h->finalized) fail(GBAGG_INVALID, "already finalized: call gbagg_reset first");
+    if (h->phase1) run_phase1(h);
     ensure_runtime(h);
     uint64_t rows = std::max<uint64_t>(h->rows_total, h->expected_rows);
     if (rows > (1ULL << 30)) fail(GBAGG_INVALID, "more than 2^30 rows in one aggregate");
@@ -1375,6 +1537,42 @@ void finalize(gbagg_handle *h) {
     if (h->result_rows > h->out_capacity) fail(GBAGG_ERROR, "result larger than its buffer");
     h->finalized = true;
     h->result_ready = false;
+}
+
+// COUNT(DISTINCT): phase 1 aggregates the consumed rows; its device-resident
+// result columns become this handle's only input batch.
+void run_phase1(gbagg_handle *h) {
+    gbagg_handle *p1 = h->phase1.get();
+    finalize(p1);
+    const size_t nc = p1->outs.size();
+    std::vector<gbagg_column_view> v(nc);
+    for (size_t c = 0; c < nc; c++) {
+        v[c] = gbagg_column_view{};
+        v[c].type = p1->outs[c].type;
+        v[c].data = p1->out_data[c].ptr;
+        v[c].validity = h->nullable[c] ? p1->out_valid[c].as<uint8_t>() : nullptr;
+        if (p1->outs[c].type == GBAGG_TYPE_VARCHAR && c < h->group_cols.size()) {
+            // phase 1's canonical ids index its key dictionary: pass it on as a unique one
+            const KeyDict &kd = p1->kdict[c];
+            const uint32_t n = kd.mode == 1 ? kd.size : (uint32_t)kd.strings.size();
+            h->p1_dict_off.assign(1, 0);
+            h->p1_dict_bytes.clear();
+            for (uint32_t i = 0; i < n; i++) {
+                const std::string str = kd.lookup(i);
+                h->p1_dict_bytes.insert(h->p1_dict_bytes.end(), str.begin(), str.end());
+                h->p1_dict_off.push_back((uint32_t)h->p1_dict_bytes.size());
+            }
+            if (h->p1_dict_bytes.empty()) h->p1_dict_bytes.push_back(0);
+            v[c].dict_offsets = h->p1_dict_off.data();
+            v[c].dict_bytes = h->p1_dict_bytes.data();
+            v[c].dict_size = n;
+            v[c].flags = GBAGG_DICT_UNIQUE;
+        }
+    }
+    h->batches.clear();
+    h->rows_total = 0;
+    for (auto &kd : h->kdict) kd = KeyDict();
+    if (p1->result_rows) add_batch(h, v.data(), p1->result_rows, true);
 }
 
 void finalize_once(gbagg_handle *h) {
@@ -1641,7 +1839,7 @@ gbagg_status gbagg_plan_create(const gbagg_query_desc *query, gbagg_handle **out
 gbagg_status gbagg_consume(gbagg_handle *h, const gbagg_column_view *columns, uint64_t num_rows) {
     return guarded([&] {
         if (!h) fail(GBAGG_INVALID, "null handle");
-        add_batch(h, columns, num_rows, false);
+        add_batch(h->phase1 ? h->phase1.get() : h, columns, num_rows, false);
     });
 }
 
@@ -1649,7 +1847,7 @@ gbagg_status gbagg_consume_device(gbagg_handle *h, const gbagg_column_view *colu
                                   uint64_t num_rows) {
     return guarded([&] {
         if (!h) fail(GBAGG_INVALID, "null handle");
-        add_batch(h, columns, num_rows, true);
+        add_batch(h->phase1 ? h->phase1.get() : h, columns, num_rows, true);
     });
 }
 
@@ -1670,16 +1868,21 @@ gbagg_status gbagg_result(gbagg_handle *h, gbagg_result_view *out) {
     });
 }
 
+void reset_handle(gbagg_handle *h) {
+    if (h->stream) HIPCHK(hipStreamSynchronize(h->stream));
+    h->batches.clear();
+    h->rows_total = 0;
+    h->finalized = false;
+    h->result_ready = false;
+    h->result_rows = 0;
+    for (auto &kd : h->kdict) kd = KeyDict();
+    if (h->phase1) reset_handle(h->phase1.get());
+}
+
 gbagg_status gbagg_reset(gbagg_handle *h) {
     return guarded([&] {
         if (!h) fail(GBAGG_INVALID, "null handle");
-        if (h->stream) HIPCHK(hipStreamSynchronize(h->stream));
-        h->batches.clear();
-        h->rows_total = 0;
-        h->finalized = false;
-        h->result_ready = false;
-        h->result_rows = 0;
-        for (auto &kd : h->kdict) kd = KeyDict();
+        reset_handle(h);
     });
 }
 
@@ -1968,7 +2171,7 @@ bool batchable(const gbagg_batch_item &it) {
     for (uint32_t a = 0; a < q->num_aggs; a++) {
         const gbagg_agg &g = q->aggs[a];
         if (g.kind == GBAGG_AGG_COUNT_STAR) continue;
-        if (g.kind == GBAGG_AGG_AVG || !intcol(g.column)) return false;
+        if (g.kind == GBAGG_AGG_AVG || g.kind == GBAGG_AGG_COUNT_DISTINCT || !intcol(g.column)) return false;
     }
     return true;
 }

@@ -67,6 +67,7 @@ __global__ __launch_bounds__(1024) void k_hll_finish(const uint8_t *partial, uin
     uint32_t zeros = 0;
     for (int i = threadIdx.x; i < HLL_M; i += blockDim.x) {
         uint32_t m = 0;
+#pragma unroll 8
         for (uint32_t p = 0; p < nparts; p++) m = max(m, (uint32_t)partial[(uint64_t)p * HLL_M + i]);
         sum += ldexp(1.0, -(int)m);
         zeros += (m == 0);

@@ -150,10 +150,17 @@ __global__ __launch_bounds__(256) void k_sample(const DevCtx *__restrict__ cx, c
         }
         if (!done) sample_global_add(rp, w, 1);
     }
+    // sampled rows: wave sums, then one device-scope add per block
+    __shared__ uint32_t s_kept;
+    if (threadIdx.x == 0) s_kept = 0;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) kept += __shfl_xor(kept, off);
     __syncthreads();
+    if ((threadIdx.x & 63) == 0 && kept) atomicAdd(&s_kept, kept);
     for (uint32_t i = threadIdx.x; i < SAMPLE_LDS; i += blockDim.x)
         if (lc[i]) sample_global_add(rp, lk[i], lc[i]);
-    if (kept) atomicAdd((unsigned long long *)&plan->sample_rows, (unsigned long long)kept);
+    __syncthreads();
+    if (threadIdx.x == 0 && s_kept) atomicAdd((unsigned long long *)&plan->sample_rows, (unsigned long long)s_kept);
 }
 
 // Heavy-hitter selection over the sample table, in three small kernels:
@@ -165,8 +172,9 @@ __global__ __launch_bounds__(256) void k_heavy_hist(const RadixParams *__restric
     const RadixParams &rp = *rpp;
     __shared__ uint32_t hist[65];
     __shared__ unsigned long long s_distinct, s_f1;
+    __shared__ int64_t s_lo, s_hi;
     if (threadIdx.x < 65) hist[threadIdx.x] = 0;
-    if (threadIdx.x == 0) { s_distinct = 0; s_f1 = 0; }
+    if (threadIdx.x == 0) { s_distinct = 0; s_f1 = 0; s_lo = INT64_MAX; s_hi = INT64_MIN; }
     __syncthreads();
     int64_t lo = INT64_MAX, hi = INT64_MIN;  // sampled key word range (plan->key_min/max preset)
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < rp.scap; i += gridDim.x * blockDim.x) {
@@ -184,14 +192,18 @@ __global__ __launch_bounds__(256) void k_heavy_hist(const RadixParams *__restric
         hi = max(hi, (int64_t)__shfl_xor(hi, off));
     }
     if ((threadIdx.x & 63) == 0 && lo <= hi) {
-        __hip_atomic_fetch_min(&plan->key_min, lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_fetch_max(&plan->key_max, hi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_fetch_min(&s_lo, lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        __hip_atomic_fetch_max(&s_hi, hi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     }
     __syncthreads();
     if (threadIdx.x < 65 && hist[threadIdx.x]) atomicAdd(&ghist[threadIdx.x], hist[threadIdx.x]);
     if (threadIdx.x == 0) {
         if (s_distinct) atomicAdd((unsigned long long *)&plan->sample_distinct, s_distinct);
         if (s_f1) atomicAdd((unsigned long long *)&plan->sample_f1, s_f1);
+        if (s_lo <= s_hi) {  // one device-scope update per block
+            __hip_atomic_fetch_min(&plan->key_min, s_lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_max(&plan->key_max, s_hi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
     }
 }
 

@@ -21,7 +21,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libslothdb_gpu.so")
 # gbagg_type (LogicalTypeId values)
 INTEGER, BIGINT, DOUBLE, VARCHAR = 5, 6, 12, 15
 # gbagg_agg_kind
-COUNT_STAR, COUNT, SUM, AVG, MIN, MAX = 0, 1, 2, 3, 4, 5
+COUNT_STAR, COUNT, SUM, AVG, MIN, MAX, COUNT_DISTINCT = 0, 1, 2, 3, 4, 5, 6
 # gbagg_cmp_op
 LT, LE, GT, GE, EQ, NE, IN, LIKE, NOT_LIKE = range(9)
 # gbagg_strategy

@@ -30,7 +30,8 @@ std::string lower(std::string s) {
 gbagg_agg_kind agg_kind(const AggregateSpec &a) {
     const std::string f = lower(a.function_name);
     if (f == "count_star" || (f == "count" && a.arg < 0)) return GBAGG_AGG_COUNT_STAR;
-    if (f == "count") return GBAGG_AGG_COUNT;
+    if (f == "count") return a.is_distinct ? GBAGG_AGG_COUNT_DISTINCT : GBAGG_AGG_COUNT;
+    if (a.is_distinct) throw GpuAggregateException(GBAGG_INVALID, "DISTINCT is supported for COUNT only");
     if (f == "sum") return GBAGG_AGG_SUM;
     if (f == "avg" || f == "mean") return GBAGG_AGG_AVG;
     if (f == "min") return GBAGG_AGG_MIN;
@@ -97,7 +98,8 @@ std::vector<LogicalTypeId> PhysicalGpuHashAggregate::GetTypes() const {
     for (auto &a : aggregates_) {
         switch (agg_kind(a)) {
         case GBAGG_AGG_COUNT_STAR:
-        case GBAGG_AGG_COUNT: out.push_back(LogicalTypeId::BIGINT); break;
+        case GBAGG_AGG_COUNT:
+        case GBAGG_AGG_COUNT_DISTINCT: out.push_back(LogicalTypeId::BIGINT); break;
         case GBAGG_AGG_SUM:
             out.push_back(input_types_.at(a.arg) == LogicalTypeId::DOUBLE ? LogicalTypeId::DOUBLE
                                                                            : LogicalTypeId::BIGINT);

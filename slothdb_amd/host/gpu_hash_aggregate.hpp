@@ -114,11 +114,13 @@ struct ColumnBatch {
     idx_t num_rows = 0;
 };
 
-// Bound aggregate (BoundFunction{function_name, arguments}): "count_star",
-// "count", "sum", "avg", "min", "max"; arg = input column, -1 for COUNT(*).
+// Bound aggregate (BoundFunction{function_name, arguments, is_distinct}):
+// "count_star", "count", "sum", "avg", "min", "max"; arg = input column, -1
+// for COUNT(*); is_distinct with "count" is COUNT(DISTINCT arg).
 struct AggregateSpec {
     std::string function_name;
     int32_t arg = -1;
+    bool is_distinct = false;
 };
 
 // One compiled WHERE conjunct (SimplePredicate, physical_planner.cpp:110-135).

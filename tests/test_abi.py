@@ -60,6 +60,10 @@ def test_valid_plans_create_without_device():
     assert _plan(G.Query([G.INTEGER, G.VARCHAR, G.INTEGER], [0, 1], [(G.SUM, 2)],
                          predicates=[(2, G.LT, 30)], nullable=[0, 0, 0], topn=(2, False, 10)))[0] == G.OK
     assert _plan(G.Query([G.DOUBLE], [], [(G.AVG, 0), (G.MIN, 0)]))[0] == G.OK
+    # COUNT(DISTINCT) with other aggregates, grouped and scalar
+    assert _plan(G.Query([G.INTEGER, G.VARCHAR, G.DOUBLE], [0],
+                         [(G.COUNT_DISTINCT, 1), (G.AVG, 2), (G.COUNT_STAR, None)], nullable=[0, 0, 1]))[0] == G.OK
+    assert _plan(G.Query([G.BIGINT], [], [(G.COUNT_DISTINCT, 0)]))[0] == G.OK
     # 16-byte keys: two BIGINT columns, BIGINT with a VARCHAR or INTEGER column
     for types in ([G.BIGINT, G.BIGINT], [G.VARCHAR, G.BIGINT], [G.BIGINT, G.INTEGER]):
         assert _plan(G.Query(types + [G.BIGINT], [0, 1], [(G.COUNT_STAR, None), (G.SUM, 2)],
@@ -77,6 +81,11 @@ def test_valid_plans_create_without_device():
     (G.Query([G.INTEGER], [0], [(G.SUM, 3)]), "out of range"),
     (G.Query([G.INTEGER] * 10, [0], [(G.SUM, c) for c in range(1, 10)]), "8 distinct"),
     (G.Query([G.INTEGER], [], []), "nothing"),
+    (G.Query([G.DOUBLE], [], [(G.COUNT_DISTINCT, 0)]), "COUNT(DISTINCT) supports"),
+    (G.Query([G.INTEGER] * 3, [0], [(G.COUNT_DISTINCT, 1), (G.COUNT_DISTINCT, 2)], nullable=[0, 0, 0]), "share one"),
+    (G.Query([G.INTEGER] * 3, [0], [(G.COUNT_DISTINCT, 1), (G.SUM, 2)], nullable=[0, 1, 0]), "nullable COUNT(DISTINCT)"),
+    (G.Query([G.INTEGER] * 2, [0], [(G.COUNT_DISTINCT, 1)], nullable=[1, 0]), "drop_null_keys"),
+    (G.Query([G.INTEGER] * 3, [0, 1], [(G.COUNT_DISTINCT, 2)], nullable=[0, 0, 0]), "one group column"),
     (G.Query([G.INTEGER], [0], [(G.COUNT_STAR, None)], topn=(5, False, 3)), "topn_col"),
 ])
 def test_invalid_plans_are_rejected(q, needle):

@@ -150,6 +150,8 @@ def brute(q, cols):
             xs = [val(c, r) for r in rs if val(c, r) is not None]
             if kind == G.COUNT:
                 row.append(len(xs))
+            elif kind == G.COUNT_DISTINCT:
+                row.append(len(set(xs)))
             elif not xs:
                 row.append(None)
             elif kind == G.SUM:
@@ -200,6 +202,34 @@ def test_exact_vs_brute_force(seed):
         if not q.group_cols and not exp:
             exp = [tuple([0 if k in (G.COUNT_STAR, G.COUNT) else None for k, _ in q.aggs])]
         assert_same_groups(got, exp, len(q.group_cols), t)
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_count_distinct_vs_brute_force(seed):
+    # distinct non-NULL values per group (physical_planner.cpp:12325-12330);
+    # VARCHAR values by content (the dictionary repeats "bb")
+    cols = tiny_table(seed)
+    types = [c.type for c in cols]
+    for x in (0, 1, 3, 4):
+        for q in (G.Query(types, [4], [(G.COUNT_DISTINCT, x), (G.COUNT_STAR, None), (G.SUM, 1)]),
+                  G.Query(types, [3], [(G.COUNT_DISTINCT, x)], drop_null_keys=True),
+                  G.Query(types, [], [(G.COUNT_DISTINCT, x), (G.MAX, 2)])):
+            dicts = {i: cols[c].dictionary for i, c in enumerate(q.group_cols) if cols[c].type == G.VARCHAR}
+            got, t, _ = exact_rows(q, cols, dicts)
+            assert_same_groups(got, brute(q, cols), len(q.group_cols), t)
+
+
+def test_count_distinct_vs_pandas_nunique():
+    import pandas as pd
+    rng = np.random.default_rng(11)
+    n = 200_000
+    k = rng.integers(0, 300, n).astype(np.int32)
+    x = rng.integers(-5000, 5000, n)
+    xv = (rng.random(n) > 0.1).astype(np.uint8)
+    q = G.Query([G.INTEGER, G.BIGINT], [0], [(G.COUNT_DISTINCT, 1)], nullable=[False, True])
+    got, _, _ = exact_rows(q, [G.Column(G.INTEGER, k), G.Column(G.BIGINT, x, xv)])
+    exp = pd.DataFrame({"k": k, "x": np.where(xv == 1, x, np.nan)}).groupby("k")["x"].nunique()
+    assert dict(got) == {int(a): int(b) for a, b in exp.items()}
 
 
 def test_planted_mistake_is_caught():
