@@ -338,7 +338,11 @@ gbagg_status gbagg_batch_create(gbagg_batch **out);
 gbagg_status gbagg_batch_set_tune(gbagg_batch *b, const gbagg_batch_tune *tune);
 /* "cap_pct": hash-table slots per estimated distinct key, in percent
  * (default 150; a query whose table overflows re-runs on the single-query
- * operator). */
+ * operator).
+ * "mode_rule": which queries aggregate each row tile in an LDS table before
+ * their global table: 0 = those whose distinct estimate fits it, 1 = all,
+ * 2 (default) = those, and those whose keys repeat within a tile (first
+ * tile's estimate at most half its rows). Results never depend on it. */
 gbagg_status gbagg_batch_set_param(gbagg_batch *b, const char *name, int64_t value);
 /* Times each candidate launch shape on these items and keeps the fastest. */
 gbagg_status gbagg_batch_autotune(gbagg_batch *b, const gbagg_batch_item *items, uint32_t count);
@@ -380,6 +384,34 @@ gbagg_status gbagg_selector_train(const float *x, const float *y, uint32_t n, ui
 /* Forward pass on the GPU: out = n x 5 outputs for n x 32 features. */
 gbagg_status gbagg_selector_infer(const float *params, uint32_t num_params, const float *x, uint32_t n,
                                   float *out);
+
+/* ---- Parquet row-group decode on the GPU -----------------------------------
+ * The fused scan's column source: ParquetReader (include/slothdb/storage/
+ * parquet.hpp:213-284) opened on a file image, and ReadColumnInto
+ * (parquet.hpp:263-265), which fills ParquetColumnData (parquet.hpp:147-207)
+ * per row group and column. Here the decoded columns land in HBM as
+ * gbagg_column_views ready for gbagg_consume_device: INTEGER / BIGINT / DOUBLE
+ * values, VARCHAR as dictionary codes with the row group's dictionary (host
+ * memory), validity bytes for OPTIONAL columns.
+ * Supported: flat schemas; uncompressed column chunks; INT32, INT64, DOUBLE,
+ * BYTE_ARRAY; data pages V1 and V2; PLAIN, PLAIN_DICTIONARY and
+ * RLE_DICTIONARY (BYTE_ARRAY dictionary-encoded only). Anything else is
+ * GBAGG_INVALID with a message. The file image must outlive the handle; every
+ * decoded column stays in HBM until close, so one operator can consume many
+ * row groups before its finalize. */
+typedef struct gbagg_parquet gbagg_parquet;
+gbagg_status gbagg_parquet_open(const uint8_t *file, uint64_t size, gbagg_parquet **out);
+/* NumRowGroups (parquet.hpp:284), column count, NumRows (parquet.hpp:234). */
+gbagg_status gbagg_parquet_shape(gbagg_parquet *p, uint32_t *num_row_groups, uint32_t *num_columns,
+                                 uint64_t *num_rows);
+/* GetColumnTypes / GetColumnNames (parquet.hpp:220-224); nullable = OPTIONAL. */
+gbagg_status gbagg_parquet_column(gbagg_parquet *p, uint32_t column, gbagg_type *type, uint8_t *nullable,
+                                  const char **name);
+gbagg_status gbagg_parquet_row_group_rows(gbagg_parquet *p, uint32_t row_group, uint64_t *rows);
+/* Decodes `num_columns` columns of one row group; out[i] describes cols[i]. */
+gbagg_status gbagg_parquet_decode(gbagg_parquet *p, uint32_t row_group, const uint32_t *cols, uint32_t num_columns,
+                                  gbagg_column_view *out, uint64_t *rows);
+void gbagg_parquet_close(gbagg_parquet *p);
 
 /* Device memory helpers for callers without their own allocator. */
 gbagg_status gbagg_device_alloc(uint64_t bytes, void **out);

@@ -2104,6 +2104,7 @@ struct gbagg_batch {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     BatchTune tune{65536, 2048, 512, 2};
     uint32_t cap_pct = 150;  // table slots per estimated key (percent)
+    uint32_t mode_rule = 2;  // BatchCtl::mode_rule (2: LDS-first also for keys that repeat within a tile)
     // the batched queries of the last run
     std::vector<uint32_t> item_of;        // batched query j -> item index
     std::vector<BatchQuery> hq;           // uploaded description
@@ -2245,6 +2246,7 @@ void batch_prepare(gbagg_batch *b, const gbagg_batch_item *items, uint32_t count
     sig = mix_sig(sig, tn.block);
     sig = mix_sig(sig, tn.wgs_per_cu);
     sig = mix_sig(sig, b->cap_pct);
+    sig = mix_sig(sig, b->mode_rule);
     const uint32_t nq = (uint32_t)b->hq.size();
     if (nq == 0) {
         b->drop_graph();
@@ -2267,6 +2269,7 @@ void batch_prepare(gbagg_batch *b, const gbagg_batch_item *items, uint32_t count
     ctl.n_items = (uint32_t)b->items.size();
     ctl.lds_slots = tn.lds_slots;
     ctl.cap_pct = b->cap_pct;
+    ctl.mode_rule = b->mode_rule;
     HIPCHK(hipMemcpyAsync(b->dq.ptr, b->hq.data(), nq * sizeof(BatchQuery), hipMemcpyHostToDevice, b->stream));
     if (!b->items.empty())
         HIPCHK(hipMemcpyAsync(b->ditems.ptr, b->items.data(), b->items.size() * sizeof(uint2), hipMemcpyHostToDevice,
@@ -2425,6 +2428,9 @@ gbagg_status gbagg_batch_set_param(gbagg_batch *b, const char *name, int64_t val
         if (n == "cap_pct") {
             if (value < 1 || value > 1000) fail(GBAGG_INVALID, "cap_pct out of range [1, 1000]");
             b->cap_pct = (uint32_t)value;
+        } else if (n == "mode_rule") {
+            if (value < 0 || value > 2) fail(GBAGG_INVALID, "mode_rule out of range [0, 2]");
+            b->mode_rule = (uint32_t)value;
         } else {
             fail(GBAGG_INVALID, "unknown batch parameter: " + n);
         }
@@ -2495,5 +2501,82 @@ gbagg_status gbagg_batch_info_get(gbagg_batch *b, gbagg_batch_info *out) {
 }
 
 void gbagg_batch_destroy(gbagg_batch *b) { delete b; }
+
+}  // extern "C"
+
+// =============================================================================
+// Parquet row-group decode (gbagg_parquet_*): the fused path's column source
+// =============================================================================
+#include "gbagg_parquet.hpp"
+
+extern "C" {
+
+gbagg_status gbagg_parquet_open(const uint8_t *file, uint64_t size, gbagg_parquet **out) {
+    return guarded([&] {
+        if (!out) fail(GBAGG_INVALID, "null out");
+        *out = nullptr;
+        auto p = std::make_unique<gbagg_parquet>();
+        parquet_open(p.get(), file, size);
+        *out = p.release();
+    });
+}
+
+gbagg_status gbagg_parquet_shape(gbagg_parquet *p, uint32_t *num_row_groups, uint32_t *num_columns,
+                                 uint64_t *num_rows) {
+    return guarded([&] {
+        if (!p) fail(GBAGG_INVALID, "null handle");
+        if (num_row_groups) *num_row_groups = (uint32_t)p->rgs.size();
+        if (num_columns) *num_columns = (uint32_t)p->cols.size();
+        if (num_rows) *num_rows = (uint64_t)p->num_rows;
+    });
+}
+
+gbagg_status gbagg_parquet_column(gbagg_parquet *p, uint32_t col, gbagg_type *type, uint8_t *nullable,
+                                  const char **name) {
+    return guarded([&] {
+        if (!p || col >= p->cols.size()) fail(GBAGG_INVALID, "column out of range");
+        const PqSchemaCol &c = p->cols[col];
+        if (type) *type = parquet_type(c.type);
+        if (nullable) *nullable = c.repetition == 1;
+        if (name) *name = c.name.c_str();
+    });
+}
+
+gbagg_status gbagg_parquet_row_group_rows(gbagg_parquet *p, uint32_t rg, uint64_t *rows) {
+    return guarded([&] {
+        if (!p || rg >= p->rgs.size() || !rows) fail(GBAGG_INVALID, "row group out of range");
+        *rows = (uint64_t)p->rgs[rg].num_rows;
+    });
+}
+
+gbagg_status gbagg_parquet_decode(gbagg_parquet *p, uint32_t rg, const uint32_t *cols, uint32_t num_cols,
+                                  gbagg_column_view *out, uint64_t *rows) {
+    return guarded([&] {
+        if (!p || rg >= p->rgs.size()) fail(GBAGG_INVALID, "row group out of range");
+        if (num_cols && (!cols || !out)) fail(GBAGG_INVALID, "null columns");
+        if (!p->stream) {
+            (void)device_info();
+            HIPCHK(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
+        }
+        p->bad.ensure(4);
+        HIPCHK(hipMemsetAsync(p->bad.ptr, 0, 4, p->stream));
+        for (uint32_t i = 0; i < num_cols; i++) {
+            if (cols[i] >= p->cols.size()) fail(GBAGG_INVALID, "column out of range");
+            p->outs.push_back(std::make_unique<gbagg_parquet::Out>());
+            parquet_decode_column(p, rg, cols[i], *p->outs.back(), out[i]);
+        }
+        HIPCHK(hipStreamSynchronize(p->stream));
+        uint32_t bad = 0;
+        HIPCHK(hipMemcpy(&bad, p->bad.ptr, 4, hipMemcpyDeviceToHost));
+        if (bad) fail(GBAGG_INVALID, "parquet: dictionary index outside the dictionary");
+        if (rows) *rows = (uint64_t)p->rgs[rg].num_rows;
+    });
+}
+
+void gbagg_parquet_close(gbagg_parquet *p) {
+    if (!p) return;
+    if (p->stream) (void)hipStreamSynchronize(p->stream);
+    delete p;
+}
 
 }  // extern "C"

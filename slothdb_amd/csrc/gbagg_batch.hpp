@@ -9,7 +9,8 @@
 //   k_b_hll       per row tile: HyperLogLog registers (p = 10) of its keys
 //   k_b_estimate  one block per query: merged registers -> distinct estimate,
 //                 which sizes the query's hash table and picks its mode (LDS
-//                 first or global)
+//                 first when the query's keys fit the LDS table or repeat
+//                 within a tile, by the first tile's own estimate; else global)
 //   k_b_layout    one block: table offsets (prefix sum of capacities)
 //   k_b_init      fills every table with empty keys and identity states
 //   k_b_agg       persistent work queue over (query, tile) items: LDS-first
@@ -68,6 +69,8 @@ struct BatchCtl {
     uint32_t n_items;
     uint32_t lds_slots;    // copy of BatchTune::lds_slots for the sketch's mode choice
     uint32_t cap_pct;      // table slots per estimated key, in percent (150)
+    uint32_t mode_rule;    // 0: LDS-first iff est <= 4 lds_slots; 1: always; 2: also when keys repeat within a tile
+    uint32_t pad;
 };
 
 struct BatchPool {
@@ -103,11 +106,18 @@ __global__ __launch_bounds__(256) void k_b_hll(const BatchQuery *__restrict__ bq
         const uint2 it = items[item];
         const BatchQuery &q = bq[it.x];
         const uint64_t r0 = (uint64_t)it.y * tune.tile_rows, r1 = min(q.rows, r0 + tune.tile_rows);
-        for (uint64_t row = r0 + threadIdx.x; row < r1; row += blockDim.x) {
-            const uint64_t h = hash64(bq_key(q, row));
-            const uint64_t rest = h << BQ_HLL_P;
-            const uint32_t rank = rest ? (uint32_t)__clzll((long long)rest) + 1 : (64 - BQ_HLL_P + 1);
-            __hip_atomic_fetch_max(&r[h >> (64 - BQ_HLL_P)], rank, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        for (uint64_t base = r0; base < r1; base += (uint64_t)blockDim.x * 4) {
+            uint64_t kk[4];
+#pragma unroll
+            for (int j = 0; j < 4; j++) kk[j] = bq_key(q, min(base + (uint64_t)j * blockDim.x + threadIdx.x, r1 - 1));
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                if (base + (uint64_t)j * blockDim.x + threadIdx.x >= r1) continue;
+                const uint64_t h = hash64(kk[j]);
+                const uint64_t rest = h << BQ_HLL_P;
+                const uint32_t rank = rest ? (uint32_t)__clzll((long long)rest) + 1 : (64 - BQ_HLL_P + 1);
+                __hip_atomic_fetch_max(&r[h >> (64 - BQ_HLL_P)], rank, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            }
         }
         __syncthreads();
         for (uint32_t i = threadIdx.x; i < BQ_HLL_M; i += blockDim.x) regs[(uint64_t)item * BQ_HLL_M + i] = (uint8_t)r[i];
@@ -119,38 +129,55 @@ __global__ __launch_bounds__(256) void k_b_hll(const BatchQuery *__restrict__ bq
 // 2007, linear counting for small counts), size the table and pick the mode.
 __global__ __launch_bounds__(256) void k_b_estimate(BatchQuery *__restrict__ bq, const BatchCtl *__restrict__ ctl,
                                                     const uint8_t *__restrict__ regs) {
-    __shared__ double s_sum[256];
-    __shared__ uint32_t s_zero[256];
+    __shared__ double s_sum[256], s_sum1[256];
+    __shared__ uint32_t s_zero[256], s_zero1[256];
     BatchQuery &q = bq[blockIdx.x];
-    double sum = 0;
-    uint32_t zeros = 0;
+    double sum = 0, sum1 = 0;
+    uint32_t zeros = 0, zeros1 = 0;
     for (uint32_t i = threadIdx.x; i < BQ_HLL_M; i += blockDim.x) {
         uint32_t m = 0;
         for (uint32_t t = 0; t < q.nitems; t++) m = max(m, (uint32_t)regs[(uint64_t)(q.item0 + t) * BQ_HLL_M + i]);
+        const uint32_t m1 = q.nitems ? regs[(uint64_t)q.item0 * BQ_HLL_M + i] : 0;  // the first tile alone
         sum += ldexp(1.0, -(int)m);
         zeros += m == 0;
+        sum1 += ldexp(1.0, -(int)m1);
+        zeros1 += m1 == 0;
     }
     s_sum[threadIdx.x] = sum;
     s_zero[threadIdx.x] = zeros;
+    s_sum1[threadIdx.x] = sum1;
+    s_zero1[threadIdx.x] = zeros1;
     __syncthreads();
     for (int off = blockDim.x / 2; off > 0; off >>= 1) {
         if (threadIdx.x < (uint32_t)off) {
             s_sum[threadIdx.x] += s_sum[threadIdx.x + off];
             s_zero[threadIdx.x] += s_zero[threadIdx.x + off];
+            s_sum1[threadIdx.x] += s_sum1[threadIdx.x + off];
+            s_zero1[threadIdx.x] += s_zero1[threadIdx.x + off];
         }
         __syncthreads();
     }
     if (threadIdx.x == 0) {
         const double m = (double)BQ_HLL_M;
-        double est = 0.7213 / (1.0 + 1.079 / m) * m * m / s_sum[0];
-        if (est <= 2.5 * m && s_zero[0] > 0) est = m * log(m / (double)s_zero[0]);
+        auto hll = [&](double s, uint32_t z) {
+            double e = 0.7213 / (1.0 + 1.079 / m) * m * m / s;
+            if (e <= 2.5 * m && z > 0) e = m * log(m / (double)z);
+            return e;
+        };
+        double est = hll(s_sum[0], s_zero[0]);
+        const double tile_est = hll(s_sum1[0], s_zero1[0]);
         uint64_t e = (uint64_t)(est * 1.05) + 1;
         if (e > q.rows) e = q.rows;
         if (e < 1) e = 1;
         q.est = e;
         uint64_t cap = (e * ctl->cap_pct + 99) / 100 + 64;
         q.cap = (cap + BQ_CHUNK - 1) / BQ_CHUNK * BQ_CHUNK;
-        q.mode = e <= (uint64_t)ctl->lds_slots * 4 ? BM_LDS : BM_GLOBAL;
+        // LDS-first pays when a tile's keys fit the LDS table, or repeat
+        // within a tile (skewed keys: the frequent ones stay in LDS)
+        const double tile_rows = (double)q.rows / (double)max(q.nitems, 1u);
+        const bool fits = e <= (uint64_t)ctl->lds_slots * 4;
+        const bool repeats = tile_rows >= 2.0 * tile_est;
+        q.mode = (ctl->mode_rule == 1 || fits || (ctl->mode_rule == 2 && repeats)) ? BM_LDS : BM_GLOBAL;
         q.groups = 0;
         q.overflow = 0;
     }

@@ -118,6 +118,15 @@ _SIGS = {
     "gbagg_memcpy_d2h": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64]),
     "gbagg_device_synchronize": (C.c_int, []),
     "gbagg_hash_u64": (C.c_uint64, [C.c_uint64]),
+    "gbagg_parquet_open": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(C.c_void_p)]),
+    "gbagg_parquet_shape": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
+                                      C.POINTER(C.c_uint64)]),
+    "gbagg_parquet_column": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_int), C.POINTER(C.c_uint8),
+                                       C.POINTER(C.c_char_p)]),
+    "gbagg_parquet_row_group_rows": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_uint64)]),
+    "gbagg_parquet_decode": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p,
+                                       C.POINTER(C.c_uint64)]),
+    "gbagg_parquet_close": (None, [C.c_void_p]),
 }
 
 _lib = None
@@ -567,3 +576,78 @@ def hash_aggregate_i32(keys, values):
                                           os_.ctypes.data, oc.ctypes.data, cap, C.byref(g)))
     m = g.value
     return ok[:m], os_[:m], oc[:m]
+
+
+class DecodedColumn:
+    """A column decoded into HBM by ParquetFile.decode (valid until the next
+    decode on that file); usable wherever a DeviceColumn is."""
+
+    def __init__(self, view, n):
+        self.type = view.type
+        self.n = n
+        self.data = view.data
+        self.validity = view.validity
+        self.dictionary = None
+        if view.type == VARCHAR:
+            k = view.dict_size
+            off = np.ctypeslib.as_array(C.cast(view.dict_offsets, C.POINTER(C.c_uint32)), shape=(k + 1,)).copy()
+            nb = max(int(off[-1]), 1)
+            data = np.ctypeslib.as_array(C.cast(view.dict_bytes, C.POINTER(C.c_uint8)), shape=(nb,)).copy()
+            self.dictionary = Dictionary(offsets=off, data=data)
+
+    def to_host(self):
+        """(values, validity) copied back to host numpy arrays."""
+        vals = np.empty(self.n, _NP[self.type])
+        if self.n:
+            _check(lib().gbagg_memcpy_d2h(vals.ctypes.data, self.data, vals.nbytes))
+        valid = None
+        if self.validity:
+            valid = np.empty(self.n, np.uint8)
+            if self.n:
+                _check(lib().gbagg_memcpy_d2h(valid.ctypes.data, self.validity, self.n))
+        return vals, valid
+
+
+class ParquetFile:
+    """A Parquet file image decoded on the GPU one row group at a time
+    (gbagg_parquet_*; the reference's ParquetReader::ReadColumnInto,
+    include/slothdb/storage/parquet.hpp:263-265)."""
+
+    def __init__(self, image):
+        self._buf = np.frombuffer(bytes(image), np.uint8)
+        h = C.c_void_p()
+        _check(lib().gbagg_parquet_open(self._buf.ctypes.data, self._buf.nbytes, C.byref(h)))
+        self._h = h.value
+        nrg, ncol, nrows = C.c_uint32(), C.c_uint32(), C.c_uint64()
+        _check(lib().gbagg_parquet_shape(self._h, C.byref(nrg), C.byref(ncol), C.byref(nrows)))
+        self.num_row_groups, self.num_columns, self.num_rows = nrg.value, ncol.value, nrows.value
+        self.columns = []
+        for c in range(self.num_columns):
+            t, nl, name = C.c_int(), C.c_uint8(), C.c_char_p()
+            _check(lib().gbagg_parquet_column(self._h, c, C.byref(t), C.byref(nl), C.byref(name)))
+            self.columns.append((name.value.decode(), t.value, bool(nl.value)))
+
+    def row_group_rows(self, rg):
+        n = C.c_uint64()
+        _check(lib().gbagg_parquet_row_group_rows(self._h, rg, C.byref(n)))
+        return n.value
+
+    def decode(self, rg, cols=None):
+        """DecodedColumns of row group rg (all columns by default)."""
+        cols = list(range(self.num_columns)) if cols is None else list(cols)
+        idx = np.array(cols or [0], np.uint32)
+        views = (ColumnView * max(1, len(cols)))()
+        n = C.c_uint64()
+        _check(lib().gbagg_parquet_decode(self._h, rg, idx.ctypes.data, len(cols), C.addressof(views), C.byref(n)))
+        return [DecodedColumn(views[i], n.value) for i in range(len(cols))]
+
+    def close(self):
+        if self._h:
+            lib().gbagg_parquet_close(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

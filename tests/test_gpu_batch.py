@@ -135,3 +135,17 @@ def test_batch_autotune_keeps_results_exact():
         exp, types = _exact(q, cols)
         assert_same_groups(b.result(i).rows(), exp, 1, types)
     b.close()
+
+
+@pytest.mark.parametrize("rule", [0, 1, 2])
+def test_batch_mode_rules_keep_results_exact(rule):
+    # which queries aggregate tiles in LDS first is a speed choice only
+    host, dev = _entries(dict(rows_hi=300_000, distinct_hi=200_000), 16)
+    b = G.GpuBatch()
+    b.set_param("mode_rule", rule)
+    b.set_queries(dev)
+    b.run()
+    for i, (q, cols) in enumerate(host):
+        exp, types = _exact(q, cols)
+        assert_same_groups(b.result(i).rows(), exp, 1, types)
+    b.close()

@@ -1,0 +1,139 @@
+"""Parquet row-group decode on the GPU (gbagg_parquet_*) against pyarrow's own
+reading of the same files (pyarrow writes them: uncompressed, dictionary and
+PLAIN pages, data pages V1 and V2, NULLs, several pages and row groups), and
+the decoded row groups fed straight into the operator against the oracle."""
+import io
+
+import numpy as np
+import pytest
+
+pa = pytest.importorskip("pyarrow")
+pq = pytest.importorskip("pyarrow.parquet")
+
+from slothdb_amd import gbagg as G
+from oracle import oracle as O
+from tests._util import assert_same_groups, oracle_rows
+
+pytestmark = pytest.mark.gpu
+
+
+def _table(n, seed):
+    rng = np.random.default_rng(seed)
+    words = np.array([f"w{j:04d}" for j in range(500)])
+    i64 = rng.integers(-2**40, 2**40, 20_000)[rng.integers(0, 20_000, n)]
+    return pa.table({
+        "a": pa.array(rng.integers(0, 65536, n).astype(np.int32)),
+        "b": pa.array(words[rng.integers(0, 500, n)], mask=rng.random(n) < 0.05),
+        "c": pa.array(rng.integers(0, 100, n).astype(np.int32)),
+        "k": pa.array(i64, mask=rng.random(n) < 0.1),
+        "d": pa.array(rng.normal(100, 25, n), mask=rng.random(n) < 0.2),
+        "u": pa.array(rng.integers(-2**62, 2**62, n)),  # all distinct: PLAIN after the dictionary limit
+    }, schema=pa.schema([pa.field("a", pa.int32(), nullable=False), ("b", pa.string()),
+                         pa.field("c", pa.int32(), nullable=False), ("k", pa.int64()), ("d", pa.float64()),
+                         pa.field("u", pa.int64(), nullable=False)]))
+
+
+def _write(t, **kw):
+    buf = io.BytesIO()
+    pq.write_table(t, buf, compression="NONE", **kw)
+    return buf.getvalue()
+
+
+def _expected(col, tcol):
+    arr = tcol.combine_chunks() if hasattr(tcol, "combine_chunks") else tcol
+    valid = ~np.asarray(arr.is_null().to_numpy(zero_copy_only=False))
+    if pa.types.is_string(arr.type):
+        return [s if v else None for s, v in zip(arr.to_pylist(), valid)]
+    vals = np.asarray(arr.fill_null(0).to_numpy(zero_copy_only=False))
+    return vals, valid
+
+
+def _check_file(image, t, rg_rows):
+    f = G.ParquetFile(image)
+    assert f.num_columns == t.num_columns and f.num_rows == t.num_rows
+    assert [c[0] for c in f.columns] == t.column_names
+    row = 0
+    for rg in range(f.num_row_groups):
+        n = f.row_group_rows(rg)
+        print(f"[parquet] row group {rg} rows {n}", flush=True)
+        cols = f.decode(rg)
+        for c, dc in enumerate(cols):
+            exp = _expected(c, t.column(c).slice(row, n))
+            vals, valid = dc.to_host()
+            if dc.type == G.VARCHAR:
+                got = [dc.dictionary.get(int(v)) if (valid is None or valid[i]) else None for i, v in enumerate(vals)]
+                assert got == exp, f"column {c} row group {rg}"
+            else:
+                ev, evalid = exp
+                if valid is None:
+                    assert evalid.all()
+                    valid = np.ones(n, bool)
+                assert np.array_equal(valid.astype(bool), evalid), f"validity of column {c} row group {rg}"
+                assert np.array_equal(vals[evalid], ev[evalid].astype(vals.dtype)), f"column {c} row group {rg}"
+        row += n
+    assert row == t.num_rows
+    f.close()
+
+
+@pytest.mark.parametrize("version", ["1.0", "2.0"])
+def test_decode_matches_pyarrow(version):
+    t = _table(300_000, 1)
+    image = _write(t, row_group_size=122_880, data_page_size=32 * 1024, data_page_version=version,
+                   dictionary_pagesize_limit=64 * 1024, use_dictionary=["a", "b", "k", "d", "u"])
+    _check_file(image, t, 122_880)
+
+
+def test_plain_only_and_single_row_group():
+    t = _table(50_000, 2).drop_columns(["b"])
+    image = _write(t, row_group_size=1_000_000, use_dictionary=False)
+    _check_file(image, t, 1_000_000)
+
+
+def test_compressed_chunks_are_rejected():
+    t = _table(1000, 3)
+    buf = io.BytesIO()
+    pq.write_table(t, buf, compression="SNAPPY")
+    f = G.ParquetFile(buf.getvalue())
+    with pytest.raises(G.GbaggError, match="compressed"):
+        f.decode(0)
+    f.close()
+
+
+def _host_columns(t, types, nullable):
+    """pyarrow's reading of the table as the oracle's input columns."""
+    cols = []
+    for c in range(t.num_columns):
+        arr = t.column(c).combine_chunks()
+        valid = (~np.asarray(arr.is_null().to_numpy(zero_copy_only=False))).astype(np.uint8)
+        if pa.types.is_string(arr.type):
+            d = arr.dictionary_encode()
+            dic = G.Dictionary([str(x) for x in d.dictionary.to_pylist()])
+            codes = np.asarray(d.indices.fill_null(0).to_numpy(zero_copy_only=False)).astype(np.uint32)
+            cols.append(G.Column(G.VARCHAR, codes, valid, dictionary=dic))
+        else:
+            vals = np.asarray(arr.fill_null(0).to_numpy(zero_copy_only=False))
+            cols.append(G.Column(types[c], vals, valid if nullable[c] else None))
+    return cols
+
+
+def test_parquet_row_groups_into_the_operator():
+    # WHERE c < 30 GROUP BY a, b: COUNT(*), SUM(c), AVG(d), MAX(k); every
+    # decoded row group is one consume_device batch of one operator
+    t = _table(400_000, 4)
+    image = _write(t, row_group_size=122_880, data_page_size=64 * 1024)
+    f = G.ParquetFile(image)
+    types = [c[1] for c in f.columns]
+    nullable = [c[2] for c in f.columns]
+    q = G.Query(types, [0, 1], [(G.COUNT_STAR, None), (G.SUM, 2), (G.AVG, 4), (G.MAX, 3)],
+                predicates=[(2, G.LT, 30)], nullable=nullable, drop_null_keys=True)
+    op = G.GpuHashAggregate(q)
+    op.Init()
+    for rg in range(f.num_row_groups):
+        op.consume_device(f.decode(rg))
+    op.finalize()
+    got = op.result().rows()
+    op.close()
+    f.close()
+    cols = _host_columns(t, types, nullable)
+    c_, v_, ty, ov = O.groupby_exact(q, cols)
+    assert_same_groups(got, oracle_rows(c_, v_, ty, {1: cols[1].dictionary}), 2, ty)
