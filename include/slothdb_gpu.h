@@ -411,6 +411,10 @@ gbagg_status gbagg_parquet_row_group_rows(gbagg_parquet *p, uint32_t row_group, 
 /* Decodes `num_columns` columns of one row group; out[i] describes cols[i]. */
 gbagg_status gbagg_parquet_decode(gbagg_parquet *p, uint32_t row_group, const uint32_t *cols, uint32_t num_columns,
                                   gbagg_column_view *out, uint64_t *rows);
+/* Copies the whole file image to HBM once; later decodes read their chunks
+ * there instead of uploading them (no counterpart in the reference, which
+ * decodes from host memory). */
+gbagg_status gbagg_parquet_upload(gbagg_parquet *p);
 void gbagg_parquet_close(gbagg_parquet *p);
 
 /* Device memory helpers for callers without their own allocator. */

@@ -2560,16 +2560,55 @@ gbagg_status gbagg_parquet_decode(gbagg_parquet *p, uint32_t rg, const uint32_t 
         }
         p->bad.ensure(4);
         HIPCHK(hipMemsetAsync(p->bad.ptr, 0, 4, p->stream));
+        // chunk bytes on the device: in place in the resident image, else one
+        // upload per chunk into this call's buffer (16 readable bytes past each)
+        std::vector<uint64_t> at(num_cols, 0);
+        uint64_t total = 0;
         for (uint32_t i = 0; i < num_cols; i++) {
             if (cols[i] >= p->cols.size()) fail(GBAGG_INVALID, "column out of range");
-            p->outs.push_back(std::make_unique<gbagg_parquet::Out>());
-            parquet_decode_column(p, rg, cols[i], *p->outs.back(), out[i]);
+            int64_t start, size;
+            parquet_chunk_range(p, rg, cols[i], start, size);
+            at[i] = p->resident ? (uint64_t)start : total;
+            total += ((uint64_t)size + 16 + 255) & ~255ull;
         }
-        HIPCHK(hipStreamSynchronize(p->stream));
+        if (!p->resident && num_cols) {
+            p->chunks.ensure(total);
+            for (uint32_t i = 0; i < num_cols; i++) {
+                int64_t start, size;
+                parquet_chunk_range(p, rg, cols[i], start, size);
+                HIPCHK(hipMemsetAsync(p->chunks.as<uint8_t>() + at[i] + size, 0, 16, p->stream));
+                if (size)
+                    HIPCHK(hipMemcpyAsync(p->chunks.as<uint8_t>() + at[i], p->file + start, (size_t)size,
+                                          hipMemcpyHostToDevice, p->stream));
+            }
+        }
+        const uint8_t *base = p->resident ? p->image.as<uint8_t>() : p->chunks.as<uint8_t>();
+        PqCall call;
+        for (uint32_t i = 0; i < num_cols; i++) {
+            p->outs.push_back(std::make_unique<gbagg_parquet::Out>());
+            parquet_plan_column(p, rg, cols[i], base + at[i], *p->outs.back(), out[i], call);
+        }
+        parquet_launch(p, call);
+        HIPCHK(hipStreamSynchronize(p->stream));  // the tables' host copies end with this call
         uint32_t bad = 0;
         HIPCHK(hipMemcpy(&bad, p->bad.ptr, 4, hipMemcpyDeviceToHost));
         if (bad) fail(GBAGG_INVALID, "parquet: dictionary index outside the dictionary");
         if (rows) *rows = (uint64_t)p->rgs[rg].num_rows;
+    });
+}
+
+gbagg_status gbagg_parquet_upload(gbagg_parquet *p) {
+    return guarded([&] {
+        if (!p) fail(GBAGG_INVALID, "null handle");
+        if (!p->stream) {
+            (void)device_info();
+            HIPCHK(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
+        }
+        p->image.ensure(p->size + 16);
+        HIPCHK(hipMemsetAsync(p->image.as<uint8_t>() + p->size, 0, 16, p->stream));
+        HIPCHK(hipMemcpyAsync(p->image.ptr, p->file, p->size, hipMemcpyHostToDevice, p->stream));
+        HIPCHK(hipStreamSynchronize(p->stream));
+        p->resident = true;
     });
 }
 

@@ -13,82 +13,103 @@
 // levels and indices).
 //
 // Host: the footer and page headers, and each RLE / bit-packed hybrid
-// stream's run headers (a few bytes per run; bit-packed runs are not read).
-// Device: every column chunk's bytes are uploaded once, then
-//   k_pq_hybrid   one thread per value: its run (binary search), then the
-//                 repeated value or the bw bits at its bit offset
-//   k_pq_gather   dictionary pages: dense value i = dictionary[index i];
-//                 PLAIN pages were copied into the dense values as they are
+// stream's run headers (a few bytes per run; bit-packed runs are not read),
+// collected over all columns of a decode call. Device: the column chunks'
+// bytes (in place when the file image is resident in HBM), then
+//   k_pq_hybrid   every run of the call, one thread per value: definition
+//                 levels to validity bytes, dictionary indices to u32
+//   k_pq_pages    every page of the call, one thread per non-NULL value:
+//                 PLAIN values copied, dictionary indices gathered
 //   k_pq_count / k_pq_scan_top / k_pq_scatter
-//                 OPTIONAL columns: exclusive count of valid rows per row,
-//                 then value[row] = dense[count] where valid
+//                 OPTIONAL columns with NULLs: exclusive count of valid rows
+//                 per row, then value[row] = dense[count] where valid
 #pragma once
 
 namespace gbagg {
 
+// One RLE / bit-packed hybrid run of some column of a decode call. Runs are
+// ordered by gstart, their first value in the call's concatenation of values.
 struct PqRun {
-    uint64_t byte_off;  // packed runs: first data byte in the chunk buffer
-    uint32_t out0;      // first output position
-    uint32_t count;     // values in the run
-    uint32_t value;     // RLE runs: the repeated value
-    uint32_t bw;        // bit-packed runs: bit width (1..32); 0: an RLE run
+    uint64_t src;     // bit-packed runs: device address of the first data byte
+    uint64_t dst;     // device address of the run's first output element
+    uint64_t gstart;
+    uint32_t count;
+    uint32_t value;   // RLE runs: the repeated value
+    uint32_t bw;      // bit-packed runs: bit width (1..32); 0: an RLE run
+    uint32_t dwidth;  // output element bytes: 1 (definition levels) or 4 (dictionary indices)
 };
 
-// Decodes one hybrid stream set: out[i] for i < n (runs sorted by out0; each
-// page may have its own bit width). T = uint8_t for levels, uint32_t for
-// dictionary indices.
-template <class T>
-__global__ __launch_bounds__(256) void k_pq_hybrid(const uint8_t *__restrict__ buf, const PqRun *__restrict__ runs,
-                                                   uint32_t nruns, T *__restrict__ out, uint32_t n) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    uint32_t lo = 0, hi = nruns - 1;
-    while (lo < hi) {
-        const uint32_t mid = (lo + hi + 1) / 2;
-        if (runs[mid].out0 <= i) lo = mid;
-        else hi = mid - 1;
-    }
-    const PqRun r = runs[lo];
-    uint32_t v = r.value;
-    if (r.bw) {
-        const uint32_t bw = r.bw;
-        // values are packed least-significant bit first; the buffer carries
-        // 16 bytes of padding past its end, so the 8-byte window stays inside
-        const uint64_t bit = (uint64_t)(i - r.out0) * bw;
-        const uint8_t *p = buf + r.byte_off + (bit >> 3);
-        uint64_t w = 0;
+// Every run of a decode call, one thread per value: its run (binary search),
+// then the repeated value or the bw bits at its bit offset (least-significant
+// bit first; every source range carries 16 readable bytes past its end).
+__global__ __launch_bounds__(256) void k_pq_hybrid(const PqRun *__restrict__ runs, uint32_t nruns, uint64_t total) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        uint32_t lo = 0, hi = nruns - 1;
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi + 1) / 2;
+            if (runs[mid].gstart <= i) lo = mid;
+            else hi = mid - 1;
+        }
+        const PqRun r = runs[lo];
+        const uint64_t j = i - r.gstart;
+        uint32_t v = r.value;
+        if (r.bw) {
+            const uint64_t bit = j * r.bw;
+            const uint8_t *p = (const uint8_t *)r.src + (bit >> 3);
+            uint64_t w = 0;
 #pragma unroll
-        for (int b = 0; b < 8; b++) w |= (uint64_t)p[b] << (8 * b);
-        const uint64_t mask = bw == 32 ? 0xFFFFFFFFull : ((1ull << bw) - 1);
-        v = (uint32_t)((w >> (bit & 7)) & mask);
+            for (int b = 0; b < 8; b++) w |= (uint64_t)p[b] << (8 * b);
+            const uint64_t mask = r.bw == 32 ? 0xFFFFFFFFull : ((1ull << r.bw) - 1);
+            v = (uint32_t)((w >> (bit & 7)) & mask);
+        }
+        if (r.dwidth == 1) ((uint8_t *)r.dst)[j] = (uint8_t)v;
+        else ((uint32_t *)r.dst)[j] = v;
     }
-    out[i] = (T)v;
 }
 
-// Dense values of dictionary pages: dense[i] = dict[idx[i]] for i in the
-// segments' ranges; an index outside the dictionary sets *bad.
+// Dense (non-NULL) values of the columns of a decode call, by page: PLAIN
+// pages are copied, dictionary pages gathered through their dictionary; an
+// index outside the dictionary sets *bad.
 struct PqSeg {
-    uint32_t dense0;    // first dense value
+    uint64_t src;     // PLAIN: the page's values; dictionary: the dictionary's values
+    uint64_t idx;     // dictionary: device address of the page's decoded indices
+    uint64_t dst;     // device address of the page's first dense value
+    uint64_t gstart;  // first value in the call's concatenation
     uint32_t count;
-    uint64_t dict_off;  // dictionary values in the chunk buffer
-    uint32_t dict_n;
+    uint32_t dict_n;  // dictionary entries (0: a PLAIN page)
+    uint32_t width;   // value bytes: 4 or 8
     uint32_t pad;
 };
 
-template <class V>
-__global__ __launch_bounds__(256) void k_pq_gather(const uint8_t *__restrict__ buf, const PqSeg *__restrict__ segs,
-                                                   uint32_t nseg, const uint32_t *__restrict__ idx, V *__restrict__ dense,
-                                                   uint32_t *bad) {
-    const PqSeg s = segs[blockIdx.y];
-    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < s.count; j += gridDim.x * blockDim.x) {
-        const uint32_t k = idx[s.dense0 + j];
-        V v = 0;
-        if (k < s.dict_n) {
-            __builtin_memcpy(&v, buf + s.dict_off + (uint64_t)k * sizeof(V), sizeof(V));
-        } else {
-            __hip_atomic_fetch_or(bad, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+__global__ __launch_bounds__(256) void k_pq_pages(const PqSeg *__restrict__ segs, uint32_t nseg, uint64_t total,
+                                                  uint32_t *bad) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        uint32_t lo = 0, hi = nseg - 1;
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi + 1) / 2;
+            if (segs[mid].gstart <= i) lo = mid;
+            else hi = mid - 1;
         }
-        dense[s.dense0 + j] = v;
+        const PqSeg sg = segs[lo];
+        const uint64_t j = i - sg.gstart;
+        uint64_t at = j;
+        bool ok = true;
+        if (sg.dict_n) {
+            at = ((const uint32_t *)sg.idx)[j];
+            ok = at < sg.dict_n;
+            if (!ok) __hip_atomic_fetch_or(bad, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        if (sg.width == 4) {
+            uint32_t v = 0;
+            if (ok) __builtin_memcpy(&v, (const uint8_t *)sg.src + at * 4, 4);
+            ((uint32_t *)sg.dst)[j] = v;
+        } else {
+            uint64_t v = 0;
+            if (ok) __builtin_memcpy(&v, (const uint8_t *)sg.src + at * 8, 8);
+            ((uint64_t *)sg.dst)[j] = v;
+        }
     }
 }
 
@@ -293,6 +314,8 @@ PqPage read_page_header(TReader &r) {
 // Run headers of an RLE / bit-packed hybrid stream of `n` values at
 // [p, end) of the file, positions relative to `chunk0`. Returns the number of
 // ones among the values when `count_ones` (definition levels, bw = 1).
+// Run fields: src = byte offset from chunk0 (rebased by the caller), dst =
+// the output element index out0 + k (the caller turns it into an address).
 uint64_t hybrid_runs(const uint8_t *p, const uint8_t *end, const uint8_t *chunk0, uint32_t bw, uint32_t n,
                      uint32_t out0, std::vector<PqRun> &runs, bool count_ones) {
     if (bw > 32) fail(GBAGG_INVALID, "parquet: bit width above 32");
@@ -307,8 +330,7 @@ uint64_t hybrid_runs(const uint8_t *p, const uint8_t *end, const uint8_t *chunk0
             const uint64_t bytes = groups * bw;
             if (bytes > (uint64_t)(r.end - r.p)) fail(GBAGG_INVALID, "parquet: truncated bit-packed run");
             const uint32_t cnt = (uint32_t)std::min<uint64_t>(groups * 8, n - done);
-            if (cnt) runs.push_back(bw ? PqRun{(uint64_t)(r.p - chunk0), out0 + done, cnt, 0, bw}
-                                       : PqRun{0, out0 + done, cnt, 0, 0});
+            if (cnt) runs.push_back(PqRun{bw ? (uint64_t)(r.p - chunk0) : 0, (uint64_t)(out0 + done), 0, cnt, 0, bw, 0});
             if (count_ones)
                 for (uint32_t k = 0; k < cnt; k++) ones += (r.p[k >> 3] >> (k & 7)) & 1;
             r.p += bytes;
@@ -318,7 +340,7 @@ uint64_t hybrid_runs(const uint8_t *p, const uint8_t *end, const uint8_t *chunk0
             uint32_t v = 0;
             for (uint32_t b = 0; b < vbytes; b++) v |= (uint32_t)r.byte() << (8 * b);
             const uint32_t cnt = (uint32_t)std::min<uint64_t>(rep, n - done);
-            if (cnt) runs.push_back(PqRun{0, out0 + done, cnt, v, 0});
+            if (cnt) runs.push_back(PqRun{0, (uint64_t)(out0 + done), 0, cnt, v, 0, 0});
             if (count_ones && v) ones += cnt;
             done += cnt;
             if (rep == 0) fail(GBAGG_INVALID, "parquet: empty RLE run");
@@ -339,15 +361,35 @@ struct gbagg_parquet {
     // one decoded column: device values (and validity), its host dictionary;
     // kept until close so that an operator can consume several row groups
     struct Out {
-        DevBuf data, dense, valid;
+        uint8_t *data = nullptr, *dense = nullptr, *valid = nullptr, *idx = nullptr, *bsum = nullptr;  // in the arena
         std::vector<uint32_t> dict_off;
         std::vector<char> dict_bytes;
     };
     std::vector<std::unique_ptr<Out>> outs;
-    // scratch reused by every decode
-    DevBuf chunk, idx, bsum, runs, segs, bad;
+    DevBuf image;           // gbagg_parquet_upload: the whole file in HBM (+16 bytes)
+    bool resident = false;
+    DevBuf chunks, bad;     // per decode call: chunk bytes when the file is not resident
+    DevBuf dtab;            // per decode call: the runs and pages of all its columns
+    uint8_t *htab = nullptr;  // pinned host staging of those tables (one async copy per call)
+    size_t htab_bytes = 0;
+    // decoded columns are bump-allocated from arenas of 256 MB or more (one
+    // hipMalloc per arena, not per column), all kept until close
+    std::vector<DevBuf> arenas;
+    size_t arena_used = 0;
+    uint8_t *alloc(size_t n) {
+        n = (std::max<size_t>(n, 1) + 255) & ~(size_t)255;
+        if (arenas.empty() || arena_used + n > arenas.back().bytes) {
+            arenas.emplace_back();
+            arenas.back().ensure(std::max<size_t>(n, (size_t)256 << 20));
+            arena_used = 0;
+        }
+        uint8_t *p = arenas.back().as<uint8_t>() + arena_used;
+        arena_used += n;
+        return p;
+    }
     ~gbagg_parquet() {
         if (stream) (void)hipStreamDestroy(stream);
+        if (htab) (void)hipHostFree(htab);
     }
 };
 
@@ -439,9 +481,32 @@ gbagg_type parquet_type(int32_t physical) {
     }
 }
 
-// Decodes column c of row group g into o; fills the view.
-void parquet_decode_column(gbagg_parquet *pq, uint32_t g, uint32_t c, gbagg_parquet::Out &o, gbagg_column_view &view) {
-    gbagg_parquet &w = *pq;  // scratch buffers
+// Chunk bytes [start, start + size) of column c of row group g.
+void parquet_chunk_range(const gbagg_parquet *pq, uint32_t g, uint32_t c, int64_t &start, int64_t &size) {
+    const PqColumnMeta &m = pq->rgs[g].cols[c];
+    start = m.dict_off > 0 ? std::min(m.dict_off, m.data_off) : m.data_off;
+    size = m.total_size;
+    if (start < 4 || size < 0 || (uint64_t)(start + size) > pq->size - 8)
+        fail(GBAGG_INVALID, "parquet: column chunk outside the file");
+}
+
+// The runs and pages of every column of one decode call, launched together.
+struct PqCall {
+    std::vector<PqRun> runs;
+    std::vector<PqSeg> segs;
+    uint64_t nvals = 0, ndense = 0;
+    struct Nulls {
+        gbagg_parquet::Out *o;
+        uint32_t nrows, dense, width;
+    };
+    std::vector<Nulls> nulls;  // OPTIONAL columns with a NULL in this chunk
+};
+
+// Host pass over column c of row group g, whose chunk bytes are on the device
+// at dbuf (16 readable bytes past the chunk): o's buffers, the column's runs
+// and pages appended to `call`, and the view.
+void parquet_plan_column(gbagg_parquet *pq, uint32_t g, uint32_t c, const uint8_t *dbuf, gbagg_parquet::Out &o,
+                         gbagg_column_view &view, PqCall &call) {
     const PqRowGroup &rg = pq->rgs[g];
     const PqColumnMeta &m = rg.cols[c];
     const PqSchemaCol &sc = pq->cols[c];
@@ -451,23 +516,14 @@ void parquet_decode_column(gbagg_parquet *pq, uint32_t g, uint32_t c, gbagg_parq
     const gbagg_type type = parquet_type(m.type);
     const uint32_t width = type == GBAGG_TYPE_INTEGER || type == GBAGG_TYPE_VARCHAR ? 4 : 8;
     const bool optional = sc.repetition == 1;
-    const int64_t start = m.dict_off > 0 ? std::min(m.dict_off, m.data_off) : m.data_off;
-    if (start < 4 || m.total_size < 0 || (uint64_t)(start + m.total_size) > pq->size - 8)
-        fail(GBAGG_INVALID, "parquet: column chunk outside the file");
+    int64_t start, csize;
+    parquet_chunk_range(pq, g, c, start, csize);
     if (rg.num_rows > 0x7FFFFFFF) fail(GBAGG_INVALID, "parquet: row group above 2^31 rows");
     const uint32_t nrows = (uint32_t)rg.num_rows;
-    const uint8_t *c0 = pq->file + start, *cend = c0 + m.total_size;
-    hipStream_t s = pq->stream;
-    // the chunk's bytes, once, with padding for the 8-byte bit-window reads
-    w.chunk.ensure((size_t)m.total_size + 16);
-    HIPCHK(hipMemsetAsync(w.chunk.as<uint8_t>() + m.total_size, 0, 16, s));
-    if (m.total_size) HIPCHK(hipMemcpyAsync(w.chunk.ptr, c0, (size_t)m.total_size, hipMemcpyHostToDevice, s));
-    const uint8_t *dbuf = w.chunk.as<uint8_t>();
+    const uint8_t *c0 = pq->file + start, *cend = c0 + csize;
 
-    std::vector<PqRun> def_runs, idx_runs;
-    std::vector<PqSeg> segs;
-    struct PlainCopy { uint64_t src; uint32_t dense0, count; };
-    std::vector<PlainCopy> plains;
+    std::vector<PqRun> def_runs, idx_runs;  // dst = element index until the buffers exist
+    std::vector<PqSeg> segs;                // dst = dense index, idx = dense index; src = chunk offset
     bool have_dict = false;
     uint64_t dict_off = 0;
     uint32_t dict_n = 0;
@@ -534,17 +590,19 @@ void parquet_decode_column(gbagg_parquet *pq, uint32_t g, uint32_t c, gbagg_parq
         if (pg.encoding == 0) {  // PLAIN
             if (type == GBAGG_TYPE_VARCHAR) fail(GBAGG_INVALID, "parquet: BYTE_ARRAY pages must be dictionary-encoded");
             if ((uint64_t)nvalid * width > (uint64_t)(bend - vals)) fail(GBAGG_INVALID, "parquet: truncated values");
-            plains.push_back(PlainCopy{(uint64_t)(vals - c0), dense, nvalid});
+            if (nvalid) segs.push_back(PqSeg{(uint64_t)(vals - c0), 0, dense, 0, nvalid, 0, width, 0});
         } else if (pg.encoding == 2 || pg.encoding == 8) {  // PLAIN_DICTIONARY / RLE_DICTIONARY
             if (!have_dict) fail(GBAGG_INVALID, "parquet: dictionary-encoded page before the dictionary");
             if (bend - vals < 1) fail(GBAGG_INVALID, "parquet: truncated indices");
             const uint32_t bw = vals[0];
             if (bw > 32) fail(GBAGG_INVALID, "parquet: index bit width above 32");
             if (nvalid) {
-                if (bw == 0) idx_runs.push_back(PqRun{0, dense, nvalid, 0, 0});
+                if (!dict_n) fail(GBAGG_INVALID, "parquet: dictionary-encoded values with an empty dictionary");
+                if (bw == 0) idx_runs.push_back(PqRun{0, dense, 0, nvalid, 0, 0, 0});
                 else hybrid_runs(vals + 1, bend, c0, bw, nvalid, dense, idx_runs, false);
+                if (type != GBAGG_TYPE_VARCHAR)
+                    segs.push_back(PqSeg{dict_off, dense, dense, 0, nvalid, dict_n, width, 0});
             }
-            if (type != GBAGG_TYPE_VARCHAR && nvalid) segs.push_back(PqSeg{dense, nvalid, dict_off, dict_n, 0});
         } else {
             fail(GBAGG_INVALID, "parquet: value encoding not supported (PLAIN, PLAIN_DICTIONARY, RLE_DICTIONARY)");
         }
@@ -552,93 +610,115 @@ void parquet_decode_column(gbagg_parquet *pq, uint32_t g, uint32_t c, gbagg_parq
         dense += nvalid;
     }
     if (rows != nrows) fail(GBAGG_INVALID, "parquet: column chunk holds fewer values than the row group");
+    if (type == GBAGG_TYPE_VARCHAR && !have_dict) fail(GBAGG_INVALID, "parquet: BYTE_ARRAY column chunk without a dictionary");
 
-    // dense values (non-NULL rows, in order), then the rows
-    const size_t wbytes = (size_t)std::max<uint32_t>(dense, 1) * width;
-    o.dense.ensure(wbytes);
-    for (const auto &pc : plains)
-        if (pc.count)
-            HIPCHK(hipMemcpyAsync(o.dense.as<uint8_t>() + (size_t)pc.dense0 * width, dbuf + pc.src,
-                                  (size_t)pc.count * width, hipMemcpyDeviceToDevice, s));
-    auto launch_hybrid = [&](const std::vector<PqRun> &runs, void *out, uint32_t n, bool bytes) {
-        if (runs.empty() || !n) return;
-        w.runs.ensure(runs.size() * sizeof(PqRun));
-        HIPCHK(hipMemcpyAsync(w.runs.ptr, runs.data(), runs.size() * sizeof(PqRun), hipMemcpyHostToDevice, s));
-        const uint32_t grid = (n + 255) / 256;
-        if (bytes)
-            hipLaunchKernelGGL(k_pq_hybrid<uint8_t>, dim3(grid), dim3(256), 0, s, dbuf, w.runs.as<PqRun>(),
-                               (uint32_t)runs.size(), (uint8_t *)out, n);
-        else
-            hipLaunchKernelGGL(k_pq_hybrid<uint32_t>, dim3(grid), dim3(256), 0, s, dbuf, w.runs.as<PqRun>(),
-                               (uint32_t)runs.size(), (uint32_t *)out, n);
-        HIPCHK(hipGetLastError());
-        HIPCHK(hipStreamSynchronize(s));  // the run table buffer is reused
-    };
-    // dictionary indices: straight into the dense array for BYTE_ARRAY codes
+    // buffers, then device addresses in the tables
+    o.dense = pq->alloc((size_t)dense * width);
+    uint8_t *dense_p = o.dense;
+    uint32_t *idx_p = nullptr;
     if (!idx_runs.empty()) {
-        uint32_t *idx_out;
         if (type == GBAGG_TYPE_VARCHAR) {
-            idx_out = o.dense.as<uint32_t>();
+            idx_p = (uint32_t *)o.dense;  // the codes are the values
         } else {
-            w.idx.ensure((size_t)std::max<uint32_t>(dense, 1) * 4);
-            idx_out = w.idx.as<uint32_t>();
+            o.idx = pq->alloc((size_t)dense * 4);
+            idx_p = (uint32_t *)o.idx;
         }
-        launch_hybrid(idx_runs, idx_out, dense, false);
-        if (type != GBAGG_TYPE_VARCHAR && !segs.empty()) {
-            w.segs.ensure(segs.size() * sizeof(PqSeg));
-            HIPCHK(hipMemcpyAsync(w.segs.ptr, segs.data(), segs.size() * sizeof(PqSeg), hipMemcpyHostToDevice, s));
-            const dim3 grid(64, (uint32_t)segs.size());
-            if (width == 4)
-                hipLaunchKernelGGL(k_pq_gather<uint32_t>, grid, dim3(256), 0, s, dbuf, w.segs.as<PqSeg>(),
-                                   (uint32_t)segs.size(), (const uint32_t *)idx_out, o.dense.as<uint32_t>(),
-                                   w.bad.as<uint32_t>());
-            else
-                hipLaunchKernelGGL(k_pq_gather<uint64_t>, grid, dim3(256), 0, s, dbuf, w.segs.as<PqSeg>(),
-                                   (uint32_t)segs.size(), (const uint32_t *)idx_out, o.dense.as<uint64_t>(),
-                                   w.bad.as<uint32_t>());
-            HIPCHK(hipGetLastError());
-            HIPCHK(hipStreamSynchronize(s));  // the segment table buffer is reused
-        }
+    }
+    for (PqRun r : idx_runs) {
+        r.src = r.bw ? (uint64_t)(dbuf + r.src) : 0;
+        r.dst = (uint64_t)(idx_p + r.dst);
+        r.dwidth = 4;
+        r.gstart = call.nvals;
+        call.nvals += r.count;
+        call.runs.push_back(r);
+    }
+    for (PqSeg sg : segs) {
+        const uint64_t d0 = sg.dst;
+        sg.src = (uint64_t)(dbuf + sg.src);
+        sg.idx = sg.dict_n ? (uint64_t)(idx_p + sg.idx) : 0;
+        sg.dst = (uint64_t)(dense_p + d0 * width);
+        sg.gstart = call.ndense;
+        call.ndense += sg.count;
+        call.segs.push_back(sg);
     }
     view = gbagg_column_view{};
     view.type = type;
-    if (optional) {
-        o.valid.ensure(std::max<uint32_t>(nrows, 1));
-        launch_hybrid(def_runs, o.valid.ptr, nrows, true);
-        const uint32_t nb = (nrows + PQ_SCAN_BLOCK - 1) / PQ_SCAN_BLOCK;
-        if (nb > 1024u * 64) fail(GBAGG_INVALID, "parquet: row group too large");
-        w.bsum.ensure((size_t)std::max<uint32_t>(nb, 1) * 4);
-        o.data.ensure((size_t)std::max<uint32_t>(nrows, 1) * width);
-        if (nrows) {
-            hipLaunchKernelGGL(k_pq_count, dim3(nb), dim3(PQ_SCAN_BLOCK), 0, s, (const uint8_t *)o.valid.ptr, nrows,
-                               w.bsum.as<uint32_t>());
-            hipLaunchKernelGGL(k_pq_scan_top, dim3(1), dim3(1024), 0, s, w.bsum.as<uint32_t>(), nb);
-            if (width == 4)
-                hipLaunchKernelGGL(k_pq_scatter<uint32_t>, dim3(nb), dim3(PQ_SCAN_BLOCK), 0, s,
-                                   (const uint8_t *)o.valid.ptr, nrows, (const uint32_t *)w.bsum.ptr,
-                                   (const uint32_t *)o.dense.ptr, dense, o.data.as<uint32_t>());
-            else
-                hipLaunchKernelGGL(k_pq_scatter<uint64_t>, dim3(nb), dim3(PQ_SCAN_BLOCK), 0, s,
-                                   (const uint8_t *)o.valid.ptr, nrows, (const uint32_t *)w.bsum.ptr,
-                                   (const uint64_t *)o.dense.ptr, dense, o.data.as<uint64_t>());
-            HIPCHK(hipGetLastError());
-            HIPCHK(hipStreamSynchronize(s));  // the block sums buffer is reused
+    if (optional && dense < nrows) {
+        o.valid = pq->alloc(nrows);
+        o.data = pq->alloc((size_t)nrows * width);
+        o.bsum = pq->alloc((size_t)((nrows + PQ_SCAN_BLOCK - 1) / PQ_SCAN_BLOCK) * 4);
+        for (PqRun r : def_runs) {
+            r.src = r.bw ? (uint64_t)(dbuf + r.src) : 0;
+            r.dst = (uint64_t)(o.valid + r.dst);
+            r.dwidth = 1;
+            r.gstart = call.nvals;
+            call.nvals += r.count;
+            call.runs.push_back(r);
         }
-        view.data = o.data.ptr;
-        view.validity = o.valid.as<uint8_t>();
+        call.nulls.push_back(PqCall::Nulls{&o, nrows, dense, width});
+        view.data = o.data;
+        view.validity = o.valid;
     } else {
-        view.data = o.dense.ptr;
+        // REQUIRED, or OPTIONAL without a NULL in this chunk: the dense values are the rows
+        view.data = o.dense;
         view.validity = nullptr;
     }
-    HIPCHK(hipStreamSynchronize(s));  // the chunk buffer is reused by the next column
     if (type == GBAGG_TYPE_VARCHAR) {
-        if (!have_dict) fail(GBAGG_INVALID, "parquet: BYTE_ARRAY column chunk without a dictionary");
         if (o.dict_bytes.empty()) o.dict_bytes.push_back(0);
         view.dict_offsets = o.dict_off.data();
         view.dict_bytes = o.dict_bytes.data();
         view.dict_size = (uint32_t)(o.dict_off.size() - 1);
         view.flags = 0;  // Parquet does not promise distinct dictionary entries
     }
+}
+
+// The call's kernels: runs, then pages, then the NULL scatters; no wait.
+void parquet_launch(gbagg_parquet *pq, PqCall &call) {
+    hipStream_t s = pq->stream;
+    const uint32_t cap = (uint32_t)device_info().cus * 8;
+    // both tables in one pinned staging buffer, one copy (the previous call
+    // ended with a stream synchronisation, so the staging buffer is free)
+    const size_t rb = call.runs.size() * sizeof(PqRun), sb = call.segs.size() * sizeof(PqSeg);
+    if (rb + sb) {
+        if (pq->htab_bytes < rb + sb) {
+            if (pq->htab) HIPCHK(hipHostFree(pq->htab));
+            pq->htab = nullptr;
+            pq->htab_bytes = 0;
+            HIPCHK(hipHostMalloc((void **)&pq->htab, rb + sb, hipHostMallocDefault));
+            pq->htab_bytes = rb + sb;
+        }
+        if (rb) std::memcpy(pq->htab, call.runs.data(), rb);
+        if (sb) std::memcpy(pq->htab + rb, call.segs.data(), sb);
+        pq->dtab.ensure(rb + sb);
+        HIPCHK(hipMemcpyAsync(pq->dtab.ptr, pq->htab, rb + sb, hipMemcpyHostToDevice, s));
+    }
+    if (!call.runs.empty()) {
+        const uint32_t grid = (uint32_t)std::min<uint64_t>((call.nvals + 255) / 256, cap);
+        hipLaunchKernelGGL(k_pq_hybrid, dim3(grid), dim3(256), 0, s, (const PqRun *)pq->dtab.ptr,
+                           (uint32_t)call.runs.size(), call.nvals);
+    }
+    if (!call.segs.empty()) {
+        const uint32_t grid = (uint32_t)std::min<uint64_t>((call.ndense + 255) / 256, cap);
+        hipLaunchKernelGGL(k_pq_pages, dim3(grid), dim3(256), 0, s, (const PqSeg *)(pq->dtab.as<uint8_t>() + rb),
+                           (uint32_t)call.segs.size(), call.ndense, pq->bad.as<uint32_t>());
+    }
+    for (const auto &nl : call.nulls) {
+        gbagg_parquet::Out &o = *nl.o;
+        const uint32_t nb = (nl.nrows + PQ_SCAN_BLOCK - 1) / PQ_SCAN_BLOCK;
+        if (nb > 1024u * 64) fail(GBAGG_INVALID, "parquet: row group too large");
+        hipLaunchKernelGGL(k_pq_count, dim3(nb), dim3(PQ_SCAN_BLOCK), 0, s, (const uint8_t *)o.valid, nl.nrows,
+                           (uint32_t *)o.bsum);
+        hipLaunchKernelGGL(k_pq_scan_top, dim3(1), dim3(1024), 0, s, (uint32_t *)o.bsum, nb);
+        if (nl.width == 4)
+            hipLaunchKernelGGL(k_pq_scatter<uint32_t>, dim3(nb), dim3(PQ_SCAN_BLOCK), 0, s, (const uint8_t *)o.valid,
+                               nl.nrows, (const uint32_t *)o.bsum, (const uint32_t *)o.dense, nl.dense,
+                               (uint32_t *)o.data);
+        else
+            hipLaunchKernelGGL(k_pq_scatter<uint64_t>, dim3(nb), dim3(PQ_SCAN_BLOCK), 0, s, (const uint8_t *)o.valid,
+                               nl.nrows, (const uint32_t *)o.bsum, (const uint64_t *)o.dense, nl.dense,
+                               (uint64_t *)o.data);
+    }
+    HIPCHK(hipGetLastError());
 }
 
 }  // namespace

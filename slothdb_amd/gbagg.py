@@ -126,6 +126,7 @@ _SIGS = {
     "gbagg_parquet_row_group_rows": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_uint64)]),
     "gbagg_parquet_decode": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p,
                                        C.POINTER(C.c_uint64)]),
+    "gbagg_parquet_upload": (C.c_int, [C.c_void_p]),
     "gbagg_parquet_close": (None, [C.c_void_p]),
 }
 
@@ -626,6 +627,10 @@ class ParquetFile:
             t, nl, name = C.c_int(), C.c_uint8(), C.c_char_p()
             _check(lib().gbagg_parquet_column(self._h, c, C.byref(t), C.byref(nl), C.byref(name)))
             self.columns.append((name.value.decode(), t.value, bool(nl.value)))
+
+    def upload(self):
+        """Keep the whole file in HBM: decodes then read their chunks there."""
+        _check(lib().gbagg_parquet_upload(self._h))
 
     def row_group_rows(self, rg):
         n = C.c_uint64()

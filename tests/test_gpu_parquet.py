@@ -48,8 +48,10 @@ def _expected(col, tcol):
     return vals, valid
 
 
-def _check_file(image, t, rg_rows):
+def _check_file(image, t, rg_rows, resident=False):
     f = G.ParquetFile(image)
+    if resident:
+        f.upload()
     assert f.num_columns == t.num_columns and f.num_rows == t.num_rows
     assert [c[0] for c in f.columns] == t.column_names
     row = 0
@@ -75,12 +77,12 @@ def _check_file(image, t, rg_rows):
     f.close()
 
 
-@pytest.mark.parametrize("version", ["1.0", "2.0"])
-def test_decode_matches_pyarrow(version):
+@pytest.mark.parametrize("version,resident", [("1.0", False), ("2.0", False), ("1.0", True)])
+def test_decode_matches_pyarrow(version, resident):
     t = _table(300_000, 1)
     image = _write(t, row_group_size=122_880, data_page_size=32 * 1024, data_page_version=version,
                    dictionary_pagesize_limit=64 * 1024, use_dictionary=["a", "b", "k", "d", "u"])
-    _check_file(image, t, 122_880)
+    _check_file(image, t, 122_880, resident)
 
 
 def test_plain_only_and_single_row_group():
