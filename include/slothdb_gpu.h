@@ -411,6 +411,15 @@ gbagg_status gbagg_parquet_row_group_rows(gbagg_parquet *p, uint32_t row_group, 
 /* Decodes `num_columns` columns of one row group; out[i] describes cols[i]. */
 gbagg_status gbagg_parquet_decode(gbagg_parquet *p, uint32_t row_group, const uint32_t *cols, uint32_t num_columns,
                                   gbagg_column_view *out, uint64_t *rows);
+/* The same over row groups [first, last), concatenated into one batch (VARCHAR
+ * dictionaries merged into one of distinct strings): one set of kernels for
+ * the whole range (no counterpart in the reference, which decodes row group
+ * by row group on its worker threads). */
+gbagg_status gbagg_parquet_decode_range(gbagg_parquet *p, uint32_t first, uint32_t last, const uint32_t *cols,
+                                        uint32_t num_columns, gbagg_column_view *out, uint64_t *rows);
+/* Forgets every decoded column (their views become invalid); later decodes
+ * reuse the device memory. */
+gbagg_status gbagg_parquet_reset(gbagg_parquet *p);
 /* Copies the whole file image to HBM once; later decodes read their chunks
  * there instead of uploading them (no counterpart in the reference, which
  * decodes from host memory). */

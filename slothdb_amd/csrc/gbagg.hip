@@ -2549,10 +2549,10 @@ gbagg_status gbagg_parquet_row_group_rows(gbagg_parquet *p, uint32_t rg, uint64_
     });
 }
 
-gbagg_status gbagg_parquet_decode(gbagg_parquet *p, uint32_t rg, const uint32_t *cols, uint32_t num_cols,
-                                  gbagg_column_view *out, uint64_t *rows) {
+gbagg_status gbagg_parquet_decode_range(gbagg_parquet *p, uint32_t rg0, uint32_t rg1, const uint32_t *cols,
+                                        uint32_t num_cols, gbagg_column_view *out, uint64_t *rows) {
     return guarded([&] {
-        if (!p || rg >= p->rgs.size()) fail(GBAGG_INVALID, "row group out of range");
+        if (!p || rg0 >= rg1 || rg1 > p->rgs.size()) fail(GBAGG_INVALID, "row group range out of range");
         if (num_cols && (!cols || !out)) fail(GBAGG_INVALID, "null columns");
         if (!p->stream) {
             (void)device_info();
@@ -2560,40 +2560,64 @@ gbagg_status gbagg_parquet_decode(gbagg_parquet *p, uint32_t rg, const uint32_t 
         }
         p->bad.ensure(4);
         HIPCHK(hipMemsetAsync(p->bad.ptr, 0, 4, p->stream));
+        const uint32_t ng = rg1 - rg0;
         // chunk bytes on the device: in place in the resident image, else one
         // upload per chunk into this call's buffer (16 readable bytes past each)
-        std::vector<uint64_t> at(num_cols, 0);
+        std::vector<const uint8_t *> dbufs((size_t)num_cols * ng);
+        std::vector<uint64_t> at((size_t)num_cols * ng, 0);
         uint64_t total = 0;
         for (uint32_t i = 0; i < num_cols; i++) {
             if (cols[i] >= p->cols.size()) fail(GBAGG_INVALID, "column out of range");
-            int64_t start, size;
-            parquet_chunk_range(p, rg, cols[i], start, size);
-            at[i] = p->resident ? (uint64_t)start : total;
-            total += ((uint64_t)size + 16 + 255) & ~255ull;
+            for (uint32_t g = 0; g < ng; g++) {
+                int64_t start, size;
+                parquet_chunk_range(p, rg0 + g, cols[i], start, size);
+                at[(size_t)i * ng + g] = p->resident ? (uint64_t)start : total;
+                total += ((uint64_t)size + 16 + 255) & ~255ull;
+            }
         }
         if (!p->resident && num_cols) {
             p->chunks.ensure(total);
-            for (uint32_t i = 0; i < num_cols; i++) {
-                int64_t start, size;
-                parquet_chunk_range(p, rg, cols[i], start, size);
-                HIPCHK(hipMemsetAsync(p->chunks.as<uint8_t>() + at[i] + size, 0, 16, p->stream));
-                if (size)
-                    HIPCHK(hipMemcpyAsync(p->chunks.as<uint8_t>() + at[i], p->file + start, (size_t)size,
-                                          hipMemcpyHostToDevice, p->stream));
-            }
+            for (uint32_t i = 0; i < num_cols; i++)
+                for (uint32_t g = 0; g < ng; g++) {
+                    int64_t start, size;
+                    parquet_chunk_range(p, rg0 + g, cols[i], start, size);
+                    uint8_t *d = p->chunks.as<uint8_t>() + at[(size_t)i * ng + g];
+                    HIPCHK(hipMemsetAsync(d + size, 0, 16, p->stream));
+                    if (size) HIPCHK(hipMemcpyAsync(d, p->file + start, (size_t)size, hipMemcpyHostToDevice, p->stream));
+                }
         }
         const uint8_t *base = p->resident ? p->image.as<uint8_t>() : p->chunks.as<uint8_t>();
+        for (size_t k = 0; k < dbufs.size(); k++) dbufs[k] = base + at[k];
         PqCall call;
         for (uint32_t i = 0; i < num_cols; i++) {
             p->outs.push_back(std::make_unique<gbagg_parquet::Out>());
-            parquet_plan_column(p, rg, cols[i], base + at[i], *p->outs.back(), out[i], call);
+            parquet_plan_column(p, rg0, rg1, cols[i], dbufs.data() + (size_t)i * ng, *p->outs.back(), out[i], call);
         }
         parquet_launch(p, call);
         HIPCHK(hipStreamSynchronize(p->stream));  // the tables' host copies end with this call
         uint32_t bad = 0;
         HIPCHK(hipMemcpy(&bad, p->bad.ptr, 4, hipMemcpyDeviceToHost));
         if (bad) fail(GBAGG_INVALID, "parquet: dictionary index outside the dictionary");
-        if (rows) *rows = (uint64_t)p->rgs[rg].num_rows;
+        if (rows) {
+            uint64_t n = 0;
+            for (uint32_t g = rg0; g < rg1; g++) n += (uint64_t)p->rgs[g].num_rows;
+            *rows = n;
+        }
+    });
+}
+
+gbagg_status gbagg_parquet_decode(gbagg_parquet *p, uint32_t rg, const uint32_t *cols, uint32_t num_cols,
+                                  gbagg_column_view *out, uint64_t *rows) {
+    return gbagg_parquet_decode_range(p, rg, rg + 1, cols, num_cols, out, rows);
+}
+
+gbagg_status gbagg_parquet_reset(gbagg_parquet *p) {
+    return guarded([&] {
+        if (!p) fail(GBAGG_INVALID, "null handle");
+        if (p->stream) HIPCHK(hipStreamSynchronize(p->stream));
+        p->outs.clear();
+        p->arena = 0;
+        p->arena_used = 0;
     });
 }
 

@@ -79,7 +79,7 @@ struct PqSeg {
     uint32_t count;
     uint32_t dict_n;  // dictionary entries (0: a PLAIN page)
     uint32_t width;   // value bytes: 4 or 8
-    uint32_t pad;
+    uint32_t pad;     // host planning: 1 while src is a remap entry index
 };
 
 __global__ __launch_bounds__(256) void k_pq_pages(const PqSeg *__restrict__ segs, uint32_t nseg, uint64_t total,
@@ -362,6 +362,7 @@ struct gbagg_parquet {
     // kept until close so that an operator can consume several row groups
     struct Out {
         uint8_t *data = nullptr, *dense = nullptr, *valid = nullptr, *idx = nullptr, *bsum = nullptr;  // in the arena
+        std::vector<uint32_t> remap;  // VARCHAR: host side of the remap tables' async copy
         std::vector<uint32_t> dict_off;
         std::vector<char> dict_bytes;
     };
@@ -374,16 +375,21 @@ struct gbagg_parquet {
     size_t htab_bytes = 0;
     // decoded columns are bump-allocated from arenas of 256 MB or more (one
     // hipMalloc per arena, not per column), all kept until close
+    // (gbagg_parquet_reset forgets them and reuses the arenas)
     std::vector<DevBuf> arenas;
-    size_t arena_used = 0;
+    size_t arena = 0, arena_used = 0;  // current arena, bytes used in it
     uint8_t *alloc(size_t n) {
         n = (std::max<size_t>(n, 1) + 255) & ~(size_t)255;
-        if (arenas.empty() || arena_used + n > arenas.back().bytes) {
+        while (arena < arenas.size() && arena_used + n > arenas[arena].bytes) {
+            arena++;
+            arena_used = 0;
+        }
+        if (arena == arenas.size()) {
             arenas.emplace_back();
             arenas.back().ensure(std::max<size_t>(n, (size_t)256 << 20));
             arena_used = 0;
         }
-        uint8_t *p = arenas.back().as<uint8_t>() + arena_used;
+        uint8_t *p = arenas[arena].as<uint8_t>() + arena_used;
         arena_used += n;
         return p;
     }
@@ -502,173 +508,202 @@ struct PqCall {
     std::vector<Nulls> nulls;  // OPTIONAL columns with a NULL in this chunk
 };
 
-// Host pass over column c of row group g, whose chunk bytes are on the device
-// at dbuf (16 readable bytes past the chunk): o's buffers, the column's runs
-// and pages appended to `call`, and the view.
-void parquet_plan_column(gbagg_parquet *pq, uint32_t g, uint32_t c, const uint8_t *dbuf, gbagg_parquet::Out &o,
-                         gbagg_column_view &view, PqCall &call) {
-    const PqRowGroup &rg = pq->rgs[g];
-    const PqColumnMeta &m = rg.cols[c];
+// Host pass over column c of row groups [g0, g1), whose chunk bytes are on
+// the device at dbufs[g - g0] (16 readable bytes past each): o's buffers (the
+// row groups concatenated), the column's runs and pages appended to `call`,
+// and the view. VARCHAR dictionaries merge into one dictionary of distinct
+// strings; each row group's indices are gathered through its remap table.
+void parquet_plan_column(gbagg_parquet *pq, uint32_t g0, uint32_t g1, uint32_t c, const uint8_t *const *dbufs,
+                         gbagg_parquet::Out &o, gbagg_column_view &view, PqCall &call) {
     const PqSchemaCol &sc = pq->cols[c];
     if (sc.repetition == 2) fail(GBAGG_INVALID, "parquet: repeated columns are not supported");
-    if (m.codec != 0) fail(GBAGG_INVALID, "parquet: compressed column chunks are not decoded on the GPU (codec " +
-                                              std::to_string(m.codec) + ")");
-    const gbagg_type type = parquet_type(m.type);
+    const gbagg_type type = parquet_type(sc.type);
     const uint32_t width = type == GBAGG_TYPE_INTEGER || type == GBAGG_TYPE_VARCHAR ? 4 : 8;
     const bool optional = sc.repetition == 1;
-    int64_t start, csize;
-    parquet_chunk_range(pq, g, c, start, csize);
-    if (rg.num_rows > 0x7FFFFFFF) fail(GBAGG_INVALID, "parquet: row group above 2^31 rows");
-    const uint32_t nrows = (uint32_t)rg.num_rows;
-    const uint8_t *c0 = pq->file + start, *cend = c0 + csize;
+    const bool varchar = type == GBAGG_TYPE_VARCHAR;
 
-    std::vector<PqRun> def_runs, idx_runs;  // dst = element index until the buffers exist
-    std::vector<PqSeg> segs;                // dst = dense index, idx = dense index; src = chunk offset
-    bool have_dict = false;
-    uint64_t dict_off = 0;
-    uint32_t dict_n = 0;
-    uint32_t rows = 0, dense = 0;
+    // this column's runs and pages go straight into the call's tables, with
+    // dst (and idx) as row / dense indices until the buffers exist
+    std::vector<PqRun> def_runs;            // levels: appended after the index runs once NULLs are known
+    std::vector<PqRun> &idx_runs = call.runs;
+    std::vector<PqSeg> &segs = call.segs;
+    const size_t irun0 = call.runs.size(), seg0 = call.segs.size();
+    std::vector<uint32_t> remap;            // VARCHAR: dictionary code -> merged id, row group after row group
+    std::unordered_map<std::string, uint32_t> ids;
     o.dict_off.assign(1, 0);
     o.dict_bytes.clear();
-    const uint8_t *p = c0;
-    while (p < cend && rows < nrows) {
-        TReader hr{p, cend};
-        const PqPage pg = read_page_header(hr);
-        const uint8_t *body = hr.p;
-        if (pg.comp_size < 0 || pg.comp_size > cend - body) fail(GBAGG_INVALID, "parquet: page outside its chunk");
-        const uint8_t *bend = body + pg.comp_size;
-        p = bend;
-        if (pg.type == 2) {  // dictionary page: PLAIN values
-            if (pg.encoding != 0 && pg.encoding != 2) fail(GBAGG_INVALID, "parquet: dictionary page not PLAIN");
-            have_dict = true;
-            dict_n = (uint32_t)pg.num_values;
-            dict_off = (uint64_t)(body - c0);
-            if (type == GBAGG_TYPE_VARCHAR) {
-                const uint8_t *q = body;
-                for (uint32_t k = 0; k < dict_n; k++) {
-                    if (bend - q < 4) fail(GBAGG_INVALID, "parquet: truncated dictionary");
-                    uint32_t len;
-                    std::memcpy(&len, q, 4);
-                    q += 4;
-                    if (len > (uint64_t)(bend - q)) fail(GBAGG_INVALID, "parquet: truncated dictionary string");
-                    o.dict_bytes.insert(o.dict_bytes.end(), q, q + len);
-                    o.dict_off.push_back((uint32_t)o.dict_bytes.size());
-                    q += len;
+    uint64_t rows = 0, dense = 0;
+    for (uint32_t g = g0; g < g1; g++) {
+        const PqRowGroup &rg = pq->rgs[g];
+        const PqColumnMeta &m = rg.cols[c];
+        if (m.codec != 0) fail(GBAGG_INVALID, "parquet: compressed column chunks are not decoded on the GPU (codec " +
+                                                  std::to_string(m.codec) + ")");
+        if (m.type != sc.type) fail(GBAGG_INVALID, "parquet: column chunk type differs from the schema");
+        int64_t start, csize;
+        parquet_chunk_range(pq, g, c, start, csize);
+        if (rg.num_rows < 0 || rows + (uint64_t)rg.num_rows > 0x7FFFFFFFull)
+            fail(GBAGG_INVALID, "parquet: more than 2^31 rows in one decode");
+        const uint32_t nrows = (uint32_t)rg.num_rows;
+        const uint8_t *c0 = pq->file + start, *cend = c0 + csize;
+        const uint8_t *dbuf = dbufs[g - g0];
+        bool have_dict = false;
+        uint64_t dict_src = 0;  // numeric: device address of the values; VARCHAR: first remap entry
+        uint32_t dict_n = 0;
+        uint32_t rgrows = 0;
+        const uint8_t *p = c0;
+        while (p < cend && rgrows < nrows) {
+            TReader hr{p, cend};
+            const PqPage pg = read_page_header(hr);
+            const uint8_t *body = hr.p;
+            if (pg.comp_size < 0 || pg.comp_size > cend - body) fail(GBAGG_INVALID, "parquet: page outside its chunk");
+            const uint8_t *bend = body + pg.comp_size;
+            p = bend;
+            if (pg.type == 2) {  // dictionary page: PLAIN values
+                if (pg.encoding != 0 && pg.encoding != 2) fail(GBAGG_INVALID, "parquet: dictionary page not PLAIN");
+                if (pg.num_values < 0) fail(GBAGG_INVALID, "parquet: bad dictionary size");
+                have_dict = true;
+                dict_n = (uint32_t)pg.num_values;
+                if (varchar) {
+                    dict_src = remap.size();
+                    const uint8_t *q = body;
+                    for (uint32_t k = 0; k < dict_n; k++) {
+                        if (bend - q < 4) fail(GBAGG_INVALID, "parquet: truncated dictionary");
+                        uint32_t len;
+                        std::memcpy(&len, q, 4);
+                        q += 4;
+                        if (len > (uint64_t)(bend - q)) fail(GBAGG_INVALID, "parquet: truncated dictionary string");
+                        std::string str((const char *)q, len);
+                        auto it = ids.find(str);
+                        if (it == ids.end()) {
+                            it = ids.emplace(std::move(str), (uint32_t)(o.dict_off.size() - 1)).first;
+                            o.dict_bytes.insert(o.dict_bytes.end(), q, q + len);
+                            o.dict_off.push_back((uint32_t)o.dict_bytes.size());
+                        }
+                        remap.push_back(it->second);
+                        q += len;
+                    }
+                } else {
+                    if ((uint64_t)dict_n * width > (uint64_t)pg.comp_size) fail(GBAGG_INVALID, "parquet: truncated dictionary");
+                    dict_src = (uint64_t)(dbuf + (body - c0));
                 }
-            } else if ((uint64_t)dict_n * width > (uint64_t)pg.comp_size) {
-                fail(GBAGG_INVALID, "parquet: truncated dictionary");
+                continue;
             }
-            continue;
-        }
-        if (pg.type != 0 && pg.type != 3) continue;  // index pages
-        if (pg.type == 3 && pg.v2_compressed && m.codec != 0) fail(GBAGG_INVALID, "parquet: compressed V2 page");
-        const uint32_t n = (uint32_t)pg.num_values;
-        if (n > nrows - rows) fail(GBAGG_INVALID, "parquet: more values than the row group's rows");
-        const uint8_t *vals = body;
-        uint32_t nvalid = n;
-        if (optional) {
-            const uint8_t *lv, *lend;
-            if (pg.type == 0) {  // V1: 4-byte length, then the hybrid levels
-                if (pg.def_encoding != 3) fail(GBAGG_INVALID, "parquet: definition levels not RLE");
-                if (bend - body < 4) fail(GBAGG_INVALID, "parquet: truncated levels");
-                uint32_t len;
-                std::memcpy(&len, body, 4);
-                lv = body + 4;
-                if (len > (uint64_t)(bend - lv)) fail(GBAGG_INVALID, "parquet: truncated levels");
-                lend = lv + len;
+            if (pg.type != 0 && pg.type != 3) continue;  // index pages
+            if (pg.type == 3 && pg.v2_compressed && m.codec != 0) fail(GBAGG_INVALID, "parquet: compressed V2 page");
+            if (pg.num_values < 0 || (uint32_t)pg.num_values > nrows - rgrows)
+                fail(GBAGG_INVALID, "parquet: more values than the row group's rows");
+            const uint32_t n = (uint32_t)pg.num_values;
+            const uint8_t *vals = body;
+            uint32_t nvalid = n;
+            if (optional) {
+                const uint8_t *lv, *lend;
+                if (pg.type == 0) {  // V1: 4-byte length, then the hybrid levels
+                    if (pg.def_encoding != 3) fail(GBAGG_INVALID, "parquet: definition levels not RLE");
+                    if (bend - body < 4) fail(GBAGG_INVALID, "parquet: truncated levels");
+                    uint32_t len;
+                    std::memcpy(&len, body, 4);
+                    lv = body + 4;
+                    if (len > (uint64_t)(bend - lv)) fail(GBAGG_INVALID, "parquet: truncated levels");
+                    lend = lv + len;
+                } else {
+                    if (pg.v2_rep_len) fail(GBAGG_INVALID, "parquet: repetition levels are not supported");
+                    lv = body;
+                    if (pg.v2_def_len < 0 || pg.v2_def_len > bend - body) fail(GBAGG_INVALID, "parquet: truncated levels");
+                    lend = body + pg.v2_def_len;
+                }
+                const size_t r0 = def_runs.size();
+                nvalid = (uint32_t)hybrid_runs(lv, lend, c0, 1, n, (uint32_t)(rows + rgrows), def_runs, true);
+                for (size_t k = r0; k < def_runs.size(); k++)
+                    if (def_runs[k].bw) def_runs[k].src = (uint64_t)(dbuf + def_runs[k].src);
+                vals = lend;
+            } else if (pg.type == 3) {
+                if (pg.v2_def_len < 0 || pg.v2_rep_len < 0 || (int64_t)pg.v2_def_len + pg.v2_rep_len > bend - body)
+                    fail(GBAGG_INVALID, "parquet: truncated levels");
+                vals = body + pg.v2_def_len + pg.v2_rep_len;
+            }
+            if (pg.encoding == 0) {  // PLAIN
+                if (varchar) fail(GBAGG_INVALID, "parquet: BYTE_ARRAY pages must be dictionary-encoded");
+                if ((uint64_t)nvalid * width > (uint64_t)(bend - vals)) fail(GBAGG_INVALID, "parquet: truncated values");
+                if (nvalid) segs.push_back(PqSeg{(uint64_t)(dbuf + (vals - c0)), 0, dense, 0, nvalid, 0, width, 0});
+            } else if (pg.encoding == 2 || pg.encoding == 8) {  // PLAIN_DICTIONARY / RLE_DICTIONARY
+                if (!have_dict) fail(GBAGG_INVALID, "parquet: dictionary-encoded page before the dictionary");
+                if (bend - vals < 1) fail(GBAGG_INVALID, "parquet: truncated indices");
+                const uint32_t bw = vals[0];
+                if (bw > 32) fail(GBAGG_INVALID, "parquet: index bit width above 32");
+                if (nvalid) {
+                    if (!dict_n) fail(GBAGG_INVALID, "parquet: dictionary-encoded values with an empty dictionary");
+                    const size_t r0 = idx_runs.size();
+                    if (bw == 0) idx_runs.push_back(PqRun{0, dense, 0, nvalid, 0, 0, 4});
+                    else hybrid_runs(vals + 1, bend, c0, bw, nvalid, (uint32_t)dense, idx_runs, false);
+                    for (size_t k = r0; k < idx_runs.size(); k++)
+                        if (idx_runs[k].bw) idx_runs[k].src = (uint64_t)(dbuf + idx_runs[k].src);
+                    // VARCHAR: src is the row group's first remap entry, made an address below
+                    segs.push_back(PqSeg{dict_src, dense, dense, 0, nvalid, dict_n, width, varchar ? 1u : 0u});
+                }
             } else {
-                if (pg.v2_rep_len) fail(GBAGG_INVALID, "parquet: repetition levels are not supported");
-                lv = body;
-                if (pg.v2_def_len > bend - body) fail(GBAGG_INVALID, "parquet: truncated levels");
-                lend = body + pg.v2_def_len;
+                fail(GBAGG_INVALID, "parquet: value encoding not supported (PLAIN, PLAIN_DICTIONARY, RLE_DICTIONARY)");
             }
-            nvalid = (uint32_t)hybrid_runs(lv, lend, c0, 1, n, rows, def_runs, true);
-            vals = lend;
-        } else if (pg.type == 3) {
-            vals = body + pg.v2_def_len + pg.v2_rep_len;
+            rgrows += n;
+            dense += nvalid;
         }
-        if (pg.encoding == 0) {  // PLAIN
-            if (type == GBAGG_TYPE_VARCHAR) fail(GBAGG_INVALID, "parquet: BYTE_ARRAY pages must be dictionary-encoded");
-            if ((uint64_t)nvalid * width > (uint64_t)(bend - vals)) fail(GBAGG_INVALID, "parquet: truncated values");
-            if (nvalid) segs.push_back(PqSeg{(uint64_t)(vals - c0), 0, dense, 0, nvalid, 0, width, 0});
-        } else if (pg.encoding == 2 || pg.encoding == 8) {  // PLAIN_DICTIONARY / RLE_DICTIONARY
-            if (!have_dict) fail(GBAGG_INVALID, "parquet: dictionary-encoded page before the dictionary");
-            if (bend - vals < 1) fail(GBAGG_INVALID, "parquet: truncated indices");
-            const uint32_t bw = vals[0];
-            if (bw > 32) fail(GBAGG_INVALID, "parquet: index bit width above 32");
-            if (nvalid) {
-                if (!dict_n) fail(GBAGG_INVALID, "parquet: dictionary-encoded values with an empty dictionary");
-                if (bw == 0) idx_runs.push_back(PqRun{0, dense, 0, nvalid, 0, 0, 0});
-                else hybrid_runs(vals + 1, bend, c0, bw, nvalid, dense, idx_runs, false);
-                if (type != GBAGG_TYPE_VARCHAR)
-                    segs.push_back(PqSeg{dict_off, dense, dense, 0, nvalid, dict_n, width, 0});
-            }
-        } else {
-            fail(GBAGG_INVALID, "parquet: value encoding not supported (PLAIN, PLAIN_DICTIONARY, RLE_DICTIONARY)");
-        }
-        rows += n;
-        dense += nvalid;
+        if (rgrows != nrows) fail(GBAGG_INVALID, "parquet: column chunk holds fewer values than the row group");
+        rows += nrows;
     }
-    if (rows != nrows) fail(GBAGG_INVALID, "parquet: column chunk holds fewer values than the row group");
-    if (type == GBAGG_TYPE_VARCHAR && !have_dict) fail(GBAGG_INVALID, "parquet: BYTE_ARRAY column chunk without a dictionary");
 
-    // buffers, then device addresses in the tables
+    // buffers (row groups concatenated), then device addresses in the tables
     o.dense = pq->alloc((size_t)dense * width);
-    uint8_t *dense_p = o.dense;
-    uint32_t *idx_p = nullptr;
-    if (!idx_runs.empty()) {
-        if (type == GBAGG_TYPE_VARCHAR) {
-            idx_p = (uint32_t *)o.dense;  // the codes are the values
-        } else {
-            o.idx = pq->alloc((size_t)dense * 4);
-            idx_p = (uint32_t *)o.idx;
-        }
+    uint32_t *idx_p = call.runs.size() == irun0 ? nullptr : (uint32_t *)(o.idx = pq->alloc((size_t)dense * 4));
+    const uint32_t *remap_p = nullptr;
+    if (!remap.empty()) {
+        uint8_t *rp = pq->alloc(remap.size() * 4);
+        o.remap = std::move(remap);
+        HIPCHK(hipMemcpyAsync(rp, o.remap.data(), o.remap.size() * 4, hipMemcpyHostToDevice, pq->stream));
+        remap_p = (const uint32_t *)rp;
     }
-    for (PqRun r : idx_runs) {
-        r.src = r.bw ? (uint64_t)(dbuf + r.src) : 0;
+    for (size_t k = irun0; k < call.runs.size(); k++) {
+        PqRun &r = call.runs[k];
         r.dst = (uint64_t)(idx_p + r.dst);
         r.dwidth = 4;
         r.gstart = call.nvals;
         call.nvals += r.count;
-        call.runs.push_back(r);
     }
-    for (PqSeg sg : segs) {
+    for (size_t k = seg0; k < call.segs.size(); k++) {
+        PqSeg &sg = call.segs[k];
         const uint64_t d0 = sg.dst;
-        sg.src = (uint64_t)(dbuf + sg.src);
+        if (sg.pad) sg.src = (uint64_t)(remap_p + sg.src);  // VARCHAR: through the remap table
+        sg.pad = 0;
         sg.idx = sg.dict_n ? (uint64_t)(idx_p + sg.idx) : 0;
-        sg.dst = (uint64_t)(dense_p + d0 * width);
+        sg.dst = (uint64_t)(o.dense + d0 * width);
         sg.gstart = call.ndense;
         call.ndense += sg.count;
-        call.segs.push_back(sg);
     }
     view = gbagg_column_view{};
     view.type = type;
-    if (optional && dense < nrows) {
-        o.valid = pq->alloc(nrows);
-        o.data = pq->alloc((size_t)nrows * width);
-        o.bsum = pq->alloc((size_t)((nrows + PQ_SCAN_BLOCK - 1) / PQ_SCAN_BLOCK) * 4);
+    if (optional && dense < rows) {
+        o.valid = pq->alloc(rows);
+        o.data = pq->alloc((size_t)rows * width);
+        o.bsum = pq->alloc((size_t)((rows + PQ_SCAN_BLOCK - 1) / PQ_SCAN_BLOCK) * 4);
         for (PqRun r : def_runs) {
-            r.src = r.bw ? (uint64_t)(dbuf + r.src) : 0;
             r.dst = (uint64_t)(o.valid + r.dst);
             r.dwidth = 1;
             r.gstart = call.nvals;
             call.nvals += r.count;
             call.runs.push_back(r);
         }
-        call.nulls.push_back(PqCall::Nulls{&o, nrows, dense, width});
+        call.nulls.push_back(PqCall::Nulls{&o, (uint32_t)rows, (uint32_t)dense, width});
         view.data = o.data;
         view.validity = o.valid;
     } else {
-        // REQUIRED, or OPTIONAL without a NULL in this chunk: the dense values are the rows
+        // REQUIRED, or OPTIONAL without a NULL in these chunks: the dense values are the rows
         view.data = o.dense;
         view.validity = nullptr;
     }
-    if (type == GBAGG_TYPE_VARCHAR) {
+    if (varchar) {
         if (o.dict_bytes.empty()) o.dict_bytes.push_back(0);
         view.dict_offsets = o.dict_off.data();
         view.dict_bytes = o.dict_bytes.data();
         view.dict_size = (uint32_t)(o.dict_off.size() - 1);
-        view.flags = 0;  // Parquet does not promise distinct dictionary entries
+        view.flags = 0;  // distinct strings, but one dictionary per decode call (not shared across batches)
     }
 }
 

@@ -126,7 +126,10 @@ _SIGS = {
     "gbagg_parquet_row_group_rows": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_uint64)]),
     "gbagg_parquet_decode": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p,
                                        C.POINTER(C.c_uint64)]),
+    "gbagg_parquet_decode_range": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p,
+                                             C.POINTER(C.c_uint64)]),
     "gbagg_parquet_upload": (C.c_int, [C.c_void_p]),
+    "gbagg_parquet_reset": (C.c_int, [C.c_void_p]),
     "gbagg_parquet_close": (None, [C.c_void_p]),
 }
 
@@ -632,18 +635,25 @@ class ParquetFile:
         """Keep the whole file in HBM: decodes then read their chunks there."""
         _check(lib().gbagg_parquet_upload(self._h))
 
+    def reset(self):
+        """Forget every decoded column; later decodes reuse its device memory."""
+        _check(lib().gbagg_parquet_reset(self._h))
+
     def row_group_rows(self, rg):
         n = C.c_uint64()
         _check(lib().gbagg_parquet_row_group_rows(self._h, rg, C.byref(n)))
         return n.value
 
-    def decode(self, rg, cols=None):
-        """DecodedColumns of row group rg (all columns by default)."""
+    def decode(self, rg, cols=None, last=None):
+        """DecodedColumns of row group rg, or of row groups [rg, last)
+        concatenated (all columns by default)."""
         cols = list(range(self.num_columns)) if cols is None else list(cols)
         idx = np.array(cols or [0], np.uint32)
         views = (ColumnView * max(1, len(cols)))()
         n = C.c_uint64()
-        _check(lib().gbagg_parquet_decode(self._h, rg, idx.ctypes.data, len(cols), C.addressof(views), C.byref(n)))
+        last = rg + 1 if last is None else last
+        _check(lib().gbagg_parquet_decode_range(self._h, rg, last, idx.ctypes.data, len(cols), C.addressof(views),
+                                                C.byref(n)))
         return [DecodedColumn(views[i], n.value) for i in range(len(cols))]
 
     def close(self):

@@ -74,6 +74,19 @@ def _check_file(image, t, rg_rows, resident=False):
                 assert np.array_equal(vals[evalid], ev[evalid].astype(vals.dtype)), f"column {c} row group {rg}"
         row += n
     assert row == t.num_rows
+    # every row group at once: one concatenated batch, dictionaries merged
+    cols = f.decode(0, last=f.num_row_groups)
+    for c, dc in enumerate(cols):
+        exp = _expected(c, t.column(c))
+        vals, valid = dc.to_host()
+        if dc.type == G.VARCHAR:
+            assert len(set(dc.dictionary.get(i) for i in range(len(dc.dictionary)))) == len(dc.dictionary)
+            got = [dc.dictionary.get(int(v)) if (valid is None or valid[i]) else None for i, v in enumerate(vals)]
+            assert got == exp, f"column {c}, all row groups"
+        else:
+            ev, evalid = exp
+            valid = np.ones(t.num_rows, bool) if valid is None else valid.astype(bool)
+            assert np.array_equal(valid, evalid) and np.array_equal(vals[evalid], ev[evalid].astype(vals.dtype))
     f.close()
 
 
@@ -128,14 +141,18 @@ def test_parquet_row_groups_into_the_operator():
     nullable = [c[2] for c in f.columns]
     q = G.Query(types, [0, 1], [(G.COUNT_STAR, None), (G.SUM, 2), (G.AVG, 4), (G.MAX, 3)],
                 predicates=[(2, G.LT, 30)], nullable=nullable, drop_null_keys=True)
-    op = G.GpuHashAggregate(q)
-    op.Init()
-    for rg in range(f.num_row_groups):
-        op.consume_device(f.decode(rg))
-    op.finalize()
-    got = op.result().rows()
-    op.close()
-    f.close()
     cols = _host_columns(t, types, nullable)
     c_, v_, ty, ov = O.groupby_exact(q, cols)
-    assert_same_groups(got, oracle_rows(c_, v_, ty, {1: cols[1].dictionary}), 2, ty)
+    exp = oracle_rows(c_, v_, ty, {1: cols[1].dictionary})
+    for whole in (False, True):  # one batch per row group, or one for the file
+        op = G.GpuHashAggregate(q)
+        op.Init()
+        if whole:
+            op.consume_device(f.decode(0, last=f.num_row_groups))
+        else:
+            for rg in range(f.num_row_groups):
+                op.consume_device(f.decode(rg))
+        op.finalize()
+        assert_same_groups(op.result().rows(), exp, 2, ty)
+        op.close()
+    f.close()
