@@ -208,14 +208,15 @@ __device__ __forceinline__ LdsTable lds_table_setup(uint64_t *smem, const DevQue
 // Any table size: the home slot is the low 32 hash bits scaled to [0, S)
 // (Lemire's multiply-shift), so S need not be a power of two and the LDS can
 // be filled exactly. Radix partitions use the top hash bits, which stay
-// independent of these.
+// independent of these. The probe bound (128) is rarely reached even at the
+// radix tables' 4/5 fill; a key that reaches it goes to the global table.
 __device__ __forceinline__ int lds_find(const LdsTable &t, uint64_t h, uint64_t k) {
     uint32_t pos = (uint32_t)(((uint64_t)(uint32_t)h * t.S) >> 32);
     uint32_t probes = 1;
     uint64_t cur = __hip_atomic_load(&t.key[pos], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     for (;;) {
         // scan to k or to an empty slot (single exit: cheap divergent loop)
-        while (cur != k && cur != SENTINEL && probes < 32) {
+        while (cur != k && cur != SENTINEL && probes < 128) {
             pos = pos + 1 == t.S ? 0 : pos + 1;
             cur = __hip_atomic_load(&t.key[pos], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             probes++;
