@@ -86,6 +86,21 @@ typedef struct {
  * skips the per-batch host pass over the dictionary. */
 #define GBAGG_DICT_UNIQUE 1u
 
+/* A VARCHAR column given as its strings, one per row, instead of dictionary
+ * codes: the string_t vectors of a DataChunk
+ * (include/slothdb/common/types/string_type.hpp:19-96) flattened to
+ *   data        uint32 offsets[n + 1], ascending, into dict_bytes
+ *   dict_bytes  the strings' bytes; row r is [offsets[r], offsets[r + 1])
+ *   dict_offsets = NULL, dict_size = 0
+ * data and dict_bytes follow the batch's pointer kind (host for
+ * gbagg_consume, device for gbagg_consume_device). The operator interns every
+ * distinct string of the column on the GPU (exact byte comparison) and groups,
+ * filters and counts by its code, as for a dictionary-encoded column; this
+ * replaces the reference's non-dictionary VARCHAR group-key path
+ * (RadixHashCountStr, src/execution/radix_count_agg.cpp:390-761). All batches
+ * of one column use the same form. */
+#define GBAGG_STR_RAW 2u
+
 /* Aggregate functions on the path (agg_emit_helpers.cpp:8-98; return types
  * per src/binder/binder.cpp:1345-1392):
  *   COUNT(*)  BIGINT, never NULL

@@ -30,6 +30,7 @@
 #include "../../include/slothdb_gpu.h"
 #include "gbagg_radix.hpp"
 #include "gbagg_selector.hpp"
+#include "gbagg_strings.hpp"
 
 using namespace gbagg;
 
@@ -239,6 +240,14 @@ struct KeyDict {
     }
 };
 
+// Interning state of one GBAGG_STR_RAW column (device buffers reused across
+// finalizes of the handle).
+struct StrCol {
+    DevBuf segs, hash, pos, codes, key, val, rep, arena, code_off, code_len, ctr, hllp, hllm, plan;
+    std::vector<DevBuf> lit, match;  // per predicate
+    uint64_t ncodes = 0;
+};
+
 struct OutCol {
     int kind;
     gbagg_type type;
@@ -250,6 +259,8 @@ struct OutCol {
 struct Batch {
     DevBatch db{};
     uint64_t nrows = 0;
+    const uint32_t *str_off[MAXU] = {};  // GBAGG_STR_RAW columns: device offsets and bytes
+    const uint8_t *str_bytes[MAXU] = {};
     std::vector<std::unique_ptr<DevBuf>> owned;  // device copies / remaps / match tables
 };
 
@@ -285,6 +296,10 @@ struct gbagg_handle {
     std::vector<uint32_t> p1_dict_off;  // phase 1's VARCHAR group key dictionary, as phase 2 input
     std::vector<char> p1_dict_bytes;
     uint64_t wide_seed0 = 0;       // KM_WIDE: first seed of wide_mix
+    // VARCHAR columns given as strings (GBAGG_STR_RAW), interned at finalize
+    std::vector<uint8_t> str_mode;  // per column: 0 no batch yet, 1 dictionary codes, 2 strings
+    std::vector<std::unique_ptr<StrCol>> strs;
+    uint64_t str_key_mask = ~0ull;  // stored string-hash bits ("str_hash_bits")
     // TopN on the device
     bool topn_device = false;
     DevTopn tspec{};
@@ -785,14 +800,49 @@ void add_batch(gbagg_handle *h, const gbagg_column_view *cols, uint64_t n, bool 
     Batch b;
     b.nrows = n;
     b.db.nrows = n;
+    if (h->str_mode.size() != h->col_types.size()) h->str_mode.assign(h->col_types.size(), 0);
+    std::vector<uint8_t> modes(h->str_mode);
     for (int u = 0; u < h->nu; u++) {
         const gbagg_column_view &c = cols[h->ucol[u]];
         if (c.type != h->col_types[h->ucol[u]]) fail(GBAGG_INVALID, "column type differs from the plan");
         if (n && !c.data) fail(GBAGG_INVALID, "null column data");
         if (c.validity && !h->nullable[h->ucol[u]])
             fail(GBAGG_INVALID, "validity given for a column declared non-nullable");
+        const bool strs = (c.flags & GBAGG_STR_RAW) != 0;
+        if (strs && c.type != GBAGG_TYPE_VARCHAR) fail(GBAGG_INVALID, "GBAGG_STR_RAW is for VARCHAR columns");
+        if (strs && (c.dict_offsets || c.dict_size)) fail(GBAGG_INVALID, "GBAGG_STR_RAW columns carry no dictionary");
+        if (c.type == GBAGG_TYPE_VARCHAR) {
+            uint8_t &m = modes[h->ucol[u]];
+            if (m && m != (strs ? 2 : 1))
+                fail(GBAGG_INVALID, "batches of one VARCHAR column must all be strings or all dictionary codes");
+            m = strs ? 2 : 1;
+        }
         size_t w = type_width(c.type);
-        if (device || n == 0) {
+        if (strs) {
+            // offsets and bytes; the codes are written when the column is interned
+            if (device || n == 0) {
+                b.str_off[u] = (const uint32_t *)c.data;
+                b.str_bytes[u] = (const uint8_t *)c.dict_bytes;
+            } else {
+                const uint32_t *off = (const uint32_t *)c.data;
+                for (uint64_t r = 0; r < n; r++)
+                    if (off[r + 1] < off[r]) fail(GBAGG_INVALID, "VARCHAR string offsets must be ascending");
+                if (off[n] && !c.dict_bytes) fail(GBAGG_INVALID, "null string bytes");
+                auto ob = std::make_unique<DevBuf>();
+                ob->ensure((n + 1) * 4);
+                HIPCHK(hipMemcpyAsync(ob->ptr, off, (n + 1) * 4, hipMemcpyHostToDevice, h->stream));
+                auto bb = std::make_unique<DevBuf>();
+                bb->ensure((size_t)off[n] + 4);
+                if (off[n])
+                    HIPCHK(hipMemcpyAsync(bb->ptr, c.dict_bytes, off[n], hipMemcpyHostToDevice, h->stream));
+                b.str_off[u] = ob->as<uint32_t>();
+                b.str_bytes[u] = bb->as<uint8_t>();
+                b.owned.push_back(std::move(ob));
+                b.owned.push_back(std::move(bb));
+            }
+            b.db.data[u] = nullptr;
+            if (device || n == 0) b.db.valid[u] = c.validity;
+        } else if (device || n == 0) {
             b.db.data[u] = c.data;
             b.db.valid[u] = c.validity;
         } else {
@@ -801,19 +851,20 @@ void add_batch(gbagg_handle *h, const gbagg_column_view *cols, uint64_t n, bool 
             HIPCHK(hipMemcpyAsync(buf->ptr, c.data, n * w, hipMemcpyHostToDevice, h->stream));
             b.db.data[u] = buf->ptr;
             b.owned.push_back(std::move(buf));
-            if (c.validity) {
-                auto vb = std::make_unique<DevBuf>();
-                vb->ensure(n);
-                HIPCHK(hipMemcpyAsync(vb->ptr, c.validity, n, hipMemcpyHostToDevice, h->stream));
-                b.db.valid[u] = vb->as<uint8_t>();
-                b.owned.push_back(std::move(vb));
-            }
+        }
+        if (!device && n && c.validity) {
+            auto vb = std::make_unique<DevBuf>();
+            vb->ensure(n);
+            HIPCHK(hipMemcpyAsync(vb->ptr, c.validity, n, hipMemcpyHostToDevice, h->stream));
+            b.db.valid[u] = vb->as<uint8_t>();
+            b.owned.push_back(std::move(vb));
         }
     }
     // VARCHAR group keys: canonical ids across batches
     for (size_t g = 0; g < h->group_cols.size(); g++) {
         if (!h->key_varchar[g]) continue;
         const gbagg_column_view &c = cols[h->group_cols[g]];
+        if (c.flags & GBAGG_STR_RAW) continue;  // codes of the interned column
         KeyDict &kd = h->kdict[g];
         if (c.dict_size && (!c.dict_offsets || !c.dict_bytes)) fail(GBAGG_INVALID, "missing dictionary");
         b.db.key_dict_size[g] = c.dict_size;
@@ -867,6 +918,7 @@ void add_batch(gbagg_handle *h, const gbagg_column_view *cols, uint64_t n, bool 
             b.db.pred_count[i] = (uint32_t)h->in_sets[i].size();
         } else if (h->dq.pred[i].kind == PK_DICT) {
             const gbagg_column_view &c = cols[p.column];
+            if (c.flags & GBAGG_STR_RAW) continue;  // matched per distinct string when interned
             if (c.dict_size && (!c.dict_offsets || !c.dict_bytes)) fail(GBAGG_INVALID, "missing dictionary");
             std::vector<uint8_t> m(std::max<uint32_t>(1, c.dict_size));
             const std::string &nd = h->pred_str[i];
@@ -893,6 +945,7 @@ void add_batch(gbagg_handle *h, const gbagg_column_view *cols, uint64_t n, bool 
         }
     }
     if (!device) HIPCHK(hipStreamSynchronize(h->stream));  // host buffers valid only during the call
+    h->str_mode = std::move(modes);
     h->rows_total += n;
     h->batches.push_back(std::move(b));
 }
@@ -1502,10 +1555,187 @@ void finalize_once(gbagg_handle *h);
 
 void run_phase1(gbagg_handle *h);
 
+// Interns one GBAGG_STR_RAW column over all its batches: codes per row (the
+// batches' column data), the per-code match tables of its VARCHAR predicates
+// and, for a group column, its dictionary on the host.
+void intern_column(gbagg_handle *h, int u) {
+    const DeviceInfo &di = device_info();
+    hipStream_t s = h->stream;
+    const uint32_t col = (uint32_t)h->ucol[u];
+    if (h->strs.size() != h->col_types.size()) h->strs.resize(h->col_types.size());
+    if (!h->strs[col]) h->strs[col] = std::make_unique<StrCol>();
+    StrCol &sc = *h->strs[col];
+    std::vector<StrSeg> segs;
+    uint64_t total = 0;
+    for (const Batch &b : h->batches) {
+        if (!b.nrows) continue;
+        segs.push_back(StrSeg{b.str_off[u], b.str_bytes[u], b.db.valid[u], total, b.nrows});
+        total += b.nrows;
+    }
+    if (total > (1ull << 30)) fail(GBAGG_INVALID, "more than 2^30 rows in one aggregate");
+    sc.ncodes = 0;
+    uint64_t ctr[SC_N] = {};
+    StrTab t{};
+    if (total) {
+        sc.segs.ensure(segs.size() * sizeof(StrSeg));
+        HIPCHK(hipMemcpyAsync(sc.segs.ptr, segs.data(), segs.size() * sizeof(StrSeg), hipMemcpyHostToDevice, s));
+        sc.hash.ensure(total * 8);
+        sc.pos.ensure(total * 4);
+        sc.codes.ensure(total * 4);
+        sc.ctr.ensure(SC_N * 8);
+        sc.plan.ensure(sizeof(DevPlan));
+        t.ctr = sc.ctr.as<uint64_t>();
+        t.key_mask = h->str_key_mask;
+        t.seed = 0x5157C0DE5EEDull;
+        const StrSeg *dsegs = sc.segs.as<StrSeg>();
+        const uint32_t nseg = (uint32_t)segs.size();
+        const uint32_t W = (uint32_t)std::min<uint64_t>((total + 1023) / 1024, (uint64_t)di.cus * 4);
+        sc.hllp.ensure((size_t)W * HLL_M);
+        const uint32_t hslices = std::min<uint32_t>(32, W);
+        sc.hllm.ensure((size_t)HLL_M * hslices);
+        HIPCHK(hipMemsetAsync(t.ctr, 0, SC_N * 8, s));
+        HIPCHK(hipMemsetAsync(sc.plan.ptr, 0, sizeof(DevPlan), s));
+        hipLaunchKernelGGL(k_str_hash, dim3(W), dim3(256), 0, s, dsegs, nseg, total, t, sc.hash.as<uint64_t>(),
+                           sc.pos.as<uint32_t>(), sc.codes.as<uint32_t>(), sc.hllp.as<uint8_t>());
+        hipLaunchKernelGGL(k_hll_merge, dim3(HLL_M / 256, hslices), dim3(256), 0, s, (const uint8_t *)sc.hllp.ptr, W,
+                           sc.hllm.as<uint8_t>());
+        hipLaunchKernelGGL(k_hll_finish, dim3(1), dim3(1024), 0, s, (const uint8_t *)sc.hllm.ptr, hslices,
+                           sc.plan.as<DevPlan>());
+        DevPlan pl{};
+        HIPCHK(hipMemcpyAsync(ctr, t.ctr, SC_N * 8, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(&pl, sc.plan.ptr, sizeof(DevPlan), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        if (ctr[SC_FLAGS] & STR_BAD_OFFSETS) fail(GBAGG_INVALID, "VARCHAR string offsets must be ascending");
+        const uint64_t rows = ctr[SC_ROWS], bytes = ctr[SC_BYTES];
+        const uint64_t est = std::max<uint64_t>(1, pl.est_groups);
+        // codes: the estimate with a margin (at most one per valid row); slots
+        // at least twice the codes; arena words from the mean string length
+        uint64_t code_cap = std::max<uint64_t>(1, std::min<uint64_t>(rows, est + est / 4 + 1024));
+        uint64_t slots = next_pow2(std::max<uint64_t>(1024, 2 * code_cap));
+        const uint64_t words_max = bytes / 4 + rows + 1;
+        const uint64_t mean_words = rows ? bytes / rows / 4 + 2 : 1;
+        uint64_t arena_words = std::min<uint64_t>(words_max, code_cap * mean_words * 3 / 2 + 1024);
+        for (uint32_t attempt = 0; rows; attempt++) {
+            if (slots > (1ull << 31)) fail(GBAGG_ERROR, "string table larger than 2^31 slots");
+            sc.key.ensure(slots * 8);
+            sc.val.ensure(slots * 4);
+            sc.rep.ensure(slots * 4);
+            sc.arena.ensure(arena_words * 4);
+            sc.code_off.ensure(code_cap * 8);
+            sc.code_len.ensure(code_cap * 4);
+            t.key = sc.key.as<uint64_t>();
+            t.val = sc.val.as<uint32_t>();
+            t.rep = sc.rep.as<uint32_t>();
+            t.arena = sc.arena.as<uint32_t>();
+            t.code_off = sc.code_off.as<uint64_t>();
+            t.code_len = sc.code_len.as<uint32_t>();
+            t.mask = slots - 1;
+            t.arena_words = arena_words;
+            t.code_cap = code_cap;
+            const uint32_t ig = (uint32_t)std::min<uint64_t>((slots + 255) / 256, (uint64_t)di.cus * 8);
+            hipLaunchKernelGGL(k_str_init, dim3(ig), dim3(256), 0, s, t);
+            const uint32_t rg = (uint32_t)std::min<uint64_t>((total + 255) / 256, (uint64_t)di.cus * 8);
+            hipLaunchKernelGGL(k_str_rehome, dim3(rg), dim3(256), 0, s, total, (const uint64_t *)sc.hash.ptr,
+                               sc.pos.as<uint32_t>());
+            for (uint32_t round = 0;; round++) {
+                hipLaunchKernelGGL(k_str_probe, dim3(rg), dim3(256), 0, s, dsegs, nseg, total, t,
+                                   (const uint64_t *)sc.hash.ptr, sc.pos.as<uint32_t>(), sc.codes.as<uint32_t>());
+                hipLaunchKernelGGL(k_str_assign, dim3(rg), dim3(256), 0, s, dsegs, nseg, total, t,
+                                   (const uint32_t *)sc.pos.ptr);
+                hipLaunchKernelGGL(k_str_check, dim3(rg), dim3(256), 0, s, dsegs, nseg, total, t,
+                                   sc.pos.as<uint32_t>(), sc.codes.as<uint32_t>());
+                HIPCHK(hipMemcpyAsync(ctr, t.ctr, SC_N * 8, hipMemcpyDeviceToHost, s));
+                HIPCHK(hipStreamSynchronize(s));
+                if ((ctr[SC_FLAGS] & (STR_FULL_TABLE | STR_FULL_DICT)) || !ctr[SC_UNRESOLVED]) break;
+                // rows whose string differs from the code at their slot: probe on
+                if (round > (1u << 20)) fail(GBAGG_ERROR, "string interning did not converge");
+                HIPCHK(hipMemsetAsync(t.ctr + SC_UNRESOLVED, 0, 8, s));
+            }
+            if (!(ctr[SC_FLAGS] & (STR_FULL_TABLE | STR_FULL_DICT))) break;
+            if (attempt == 16) fail(GBAGG_ERROR, "string interning: tables still full after 16 resizes");
+            if (ctr[SC_FLAGS] & STR_FULL_TABLE) slots *= 2;
+            if (ctr[SC_FLAGS] & STR_FULL_DICT) {
+                code_cap = std::min<uint64_t>(rows, code_cap * 2);
+                arena_words = std::min<uint64_t>(words_max, arena_words * 2);
+                slots = std::max<uint64_t>(slots, next_pow2(2 * code_cap));
+            }
+        }
+        sc.ncodes = rows ? ctr[SC_CODES] : 0;
+    }
+    const uint32_t nc = (uint32_t)sc.ncodes;
+    // predicates on this column: one match per distinct string
+    sc.lit.resize(h->preds.size());
+    sc.match.resize(h->preds.size());
+    for (size_t i = 0; i < h->preds.size(); i++) {
+        if (h->preds[i].column != col || h->dq.pred[i].kind != PK_DICT) continue;
+        const std::string &lit = h->pred_str[i];
+        const gbagg_cmp_op op = h->preds[i].op;
+        const int mop = op == GBAGG_CMP_EQ ? 0 : op == GBAGG_CMP_NE ? 1 : op == GBAGG_CMP_LIKE ? 2 : 3;
+        sc.lit[i].ensure(lit.size() + 1);
+        if (!lit.empty()) HIPCHK(hipMemcpyAsync(sc.lit[i].ptr, lit.data(), lit.size(), hipMemcpyHostToDevice, s));
+        sc.match[i].ensure(std::max<uint32_t>(1, nc));
+        if (nc)
+            hipLaunchKernelGGL(k_str_match, dim3((uint32_t)std::min<uint64_t>((nc + 255) / 256, (uint64_t)di.cus * 4)),
+                               dim3(256), 0, s, t, (uint64_t)nc, (const uint8_t *)sc.lit[i].ptr,
+                               (uint32_t)lit.size(), mop, sc.match[i].as<uint8_t>());
+        for (Batch &b : h->batches) {
+            b.db.pred_ptr[i] = sc.match[i].ptr;
+            b.db.pred_count[i] = nc;
+        }
+    }
+    uint64_t row0 = 0;
+    for (Batch &b : h->batches) {
+        if (!b.nrows) continue;
+        b.db.data[u] = sc.codes.as<uint32_t>() + row0;
+        row0 += b.nrows;
+    }
+    // group column: the dictionary in code order, on the host
+    for (size_t g = 0; g < h->group_cols.size(); g++) {
+        if (h->group_cols[g] != col) continue;
+        KeyDict &kd = h->kdict[g];
+        kd = KeyDict();
+        kd.mode = 1;
+        kd.size = nc;
+        std::vector<uint64_t> off(nc);
+        std::vector<uint32_t> len(nc);
+        std::vector<uint32_t> words(std::max<uint64_t>(1, nc ? ctr[SC_WORDS] : 0));
+        if (nc) {
+            HIPCHK(hipMemcpyAsync(off.data(), sc.code_off.ptr, (size_t)nc * 8, hipMemcpyDeviceToHost, s));
+            HIPCHK(hipMemcpyAsync(len.data(), sc.code_len.ptr, (size_t)nc * 4, hipMemcpyDeviceToHost, s));
+            HIPCHK(hipMemcpyAsync(words.data(), sc.arena.ptr, (size_t)ctr[SC_WORDS] * 4, hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+        }
+        kd.own_offsets.resize((size_t)nc + 1);
+        kd.own_offsets[0] = 0;
+        uint64_t nb = 0;
+        for (uint32_t c = 0; c < nc; c++) {
+            nb += len[c];
+            if (nb > 0xFFFFFFFFull) fail(GBAGG_ERROR, "distinct strings exceed 4 GiB");
+            kd.own_offsets[c + 1] = (uint32_t)nb;
+        }
+        kd.own_bytes.resize(std::max<uint64_t>(1, nb));
+        const char *src = (const char *)words.data();
+        for (uint32_t c = 0; c < nc; c++)
+            std::memcpy(kd.own_bytes.data() + kd.own_offsets[c], src + off[c] * 4, len[c]);
+        kd.offsets = kd.own_offsets.data();
+        kd.bytes = kd.own_bytes.data();
+        for (Batch &b : h->batches) {
+            b.db.key_dict_size[g] = nc;
+            b.db.remap[g] = nullptr;
+        }
+    }
+}
+
+void intern_strings(gbagg_handle *h) {
+    for (int u = 0; u < h->nu; u++)
+        if ((size_t)h->ucol[u] < h->str_mode.size() && h->str_mode[h->ucol[u]] == 2) intern_column(h, u);
+}
+
 void finalize(gbagg_handle *h) {
     if (h->finalized) fail(GBAGG_INVALID, "already finalized: call gbagg_reset first");
     if (h->phase1) run_phase1(h);
     ensure_runtime(h);
+    intern_strings(h);
     uint64_t rows = std::max<uint64_t>(h->rows_total, h->expected_rows);
     if (rows > (1ULL << 30)) fail(GBAGG_INVALID, "more than 2^30 rows in one aggregate");
     h->cap_max = std::max<uint64_t>(h->cap_max, next_pow2(std::max<uint64_t>(1024, rows + rows / 4 + 2)));
@@ -1876,6 +2106,7 @@ void reset_handle(gbagg_handle *h) {
     h->result_ready = false;
     h->result_rows = 0;
     for (auto &kd : h->kdict) kd = KeyDict();
+    std::fill(h->str_mode.begin(), h->str_mode.end(), 0);
     if (h->phase1) reset_handle(h->phase1.get());
 }
 
@@ -1933,6 +2164,7 @@ gbagg_status gbagg_set_param(gbagg_handle *h, const char *name, int64_t value) {
         else if (n == "lds_kb") { in(8, 160); h->k4_lds_kb = (uint32_t)value; }
         else if (n == "features") { in(0, 1); h->collect_features = value != 0; }
         else if (n == "wide_seed") { in(0, 1LL << 40); h->wide_seed0 = (uint64_t)value; }
+        else if (n == "str_hash_bits") { in(1, 64); h->str_key_mask = value == 64 ? ~0ull : (1ull << value) - 1; }
         else if (n == "strategy") { in(0, 4); h->strategy = (gbagg_strategy)value; }
         else if (n == "static_shapes") { in(0, 1); h->allow_static = value != 0; pick_kernels(h); }
         else fail(GBAGG_INVALID, "unknown parameter " + n);

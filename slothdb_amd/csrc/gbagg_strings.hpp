@@ -1,0 +1,325 @@
+// gbagg_strings.hpp -- VARCHAR columns that arrive as strings (offsets +
+// bytes, one string per row) rather than as dictionary codes: the reference's
+// string_t vectors (include/slothdb/common/types/string_type.hpp:19-96) and
+// its non-dictionary VARCHAR group keys (RadixHashCountStr,
+// src/execution/radix_count_agg.cpp:390-761, which groups on a 64-bit string
+// hash). Here every distinct string of a column gets one dense code on the
+// GPU, and the codes feed the operator's dictionary-code paths unchanged.
+// Groups are exact: rows whose strings hash alike but differ in their bytes
+// get different codes.
+//
+// A column's batches are interned together, as one concatenation of rows:
+//   k_str_hash    per row: 64-bit hash of the bytes, offsets checked,
+//                 HyperLogLog sketch per block (sizes the table)
+//   then rounds of
+//   k_str_probe   per unresolved row: linear probing from its hash's home
+//                 slot; a slot with the row's hash and a code is compared
+//                 byte for byte (equal: the row has its code; different:
+//                 probe on); an empty slot is claimed for the hash; a slot
+//                 with the hash and no code yet records the smallest row
+//                 that reached it (atomicMin)
+//   k_str_assign  that row takes the next code, copies its string into the
+//                 arena and publishes the code on the slot
+//   k_str_check   the other rows that stopped there compare their bytes with
+//                 the code's string; rows with a different string probe on
+//                 from the next slot in the next round
+// until no row is unresolved (one round unless two distinct strings share a
+// 64-bit hash). Codes are dense from 0 in the order the representatives
+// took them.
+//   k_str_match   VARCHAR predicates (=, <>, LIKE '%x%', NOT LIKE) once per
+//                 distinct string, as the per-code match table the row
+//                 kernels read
+#pragma once
+
+namespace gbagg {
+
+constexpr uint32_t STR_NONE = 0xFFFFFFFFu;  // slot: no code yet; row: resolved or NULL
+constexpr uint32_t STR_HOME = 0xFFFFFFFEu;  // row: not probed yet, start at the home slot
+constexpr uint64_t STR_FULL_TABLE = 1, STR_FULL_DICT = 2, STR_BAD_OFFSETS = 4;
+enum { SC_CODES = 0, SC_WORDS, SC_UNRESOLVED, SC_FLAGS, SC_ROWS, SC_BYTES, SC_N };
+
+// Rows [row0, row0 + n) of the column's concatenation: one batch.
+struct StrSeg {
+    const uint32_t *off;   // n + 1 ascending offsets into bytes
+    const uint8_t *bytes;
+    const uint8_t *valid;  // one byte per row, 0 = NULL; null: all valid
+    uint64_t row0;
+    uint64_t n;
+};
+
+struct StrTab {
+    uint64_t *key;       // slot: stored hash, 0 = empty
+    uint32_t *val;       // slot: code, STR_NONE = none yet
+    uint32_t *rep;       // slot: smallest row that stopped there while it had no code
+    uint32_t *arena;     // distinct strings, each from a word boundary, zero padded
+    uint64_t *code_off;  // per code: first arena word
+    uint32_t *code_len;  // per code: bytes
+    uint64_t *ctr;       // SC_* counters
+    uint64_t mask;       // slots - 1
+    uint64_t arena_words;
+    uint64_t code_cap;
+    uint64_t key_mask;   // stored hash bits (all, unless a test narrows them)
+    uint64_t seed;
+};
+
+__device__ __forceinline__ uint32_t str_seg(const StrSeg *segs, uint32_t nseg, uint64_t g) {
+    uint32_t lo = 0, hi = nseg - 1;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) / 2;
+        if (segs[mid].row0 <= g) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// r (1..4) bytes from p, little endian, zero above them. Only the aligned
+// words holding those bytes are read, so nothing outside the words of the
+// string's own bytes is touched.
+__device__ __forceinline__ uint32_t str_word(const uint8_t *p, uint32_t r) {
+    const uintptr_t a = (uintptr_t)p & ~(uintptr_t)3;
+    const uint32_t sh = (uint32_t)((uintptr_t)p & 3);
+    const uint64_t lo = gld((const uint32_t *)a);
+    const uint64_t hi = sh + r > 4 ? gld((const uint32_t *)(a + 4)) : 0;
+    const uint32_t v = (uint32_t)(((hi << 32) | lo) >> (8 * sh));
+    return r >= 4 ? v : (v & ((1u << (8 * r)) - 1));
+}
+
+__device__ __forceinline__ uint64_t str_fold(uint64_t a, uint64_t b) {
+    return a * b ^ __umul64hi(a, b);
+}
+
+// 64-bit string hash: the length seeds the state, each 8-byte word (the last
+// one zero padded) is folded in with a 64x64->128-bit multiply (low ^ high
+// half), and SplitMix64's finalizer mixes the result.
+__device__ __forceinline__ uint64_t str_hash(const uint8_t *p, uint32_t len, uint64_t seed) {
+    uint64_t h = seed ^ ((uint64_t)len * 0x9E3779B97F4A7C15ull);
+    for (uint32_t i = 0; i < len; i += 8) {
+        const uint32_t left = len - i;
+        uint64_t w = str_word(p + i, left < 4 ? left : 4);
+        if (left > 4) w |= (uint64_t)str_word(p + i + 4, left - 4 < 4 ? left - 4 : 4) << 32;
+        h = str_fold(h ^ w, 0xBF58476D1CE4E5B9ull) + 0x94D049BB133111EBull;
+    }
+    h ^= h >> 30;
+    h *= 0xBF58476D1CE4E5B9ull;
+    h ^= h >> 27;
+    h *= 0x94D049BB133111EBull;
+    return h ^ (h >> 31);
+}
+
+__device__ __forceinline__ uint64_t str_home(uint64_t key, uint64_t mask) {
+    return ((key * 0x9E3779B97F4A7C15ull) >> 32) & mask;
+}
+
+// The row's string equals code c's arena string.
+__device__ __forceinline__ bool str_equal(const StrTab &t, uint32_t c, const uint8_t *p, uint32_t len) {
+    if (gld(&t.code_len[c]) != len) return false;
+    const uint32_t *a = t.arena + gld(&t.code_off[c]);
+    for (uint32_t i = 0; i < len; i += 4) {
+        const uint32_t left = len - i;
+        if (str_word(p + i, left < 4 ? left : 4) != gld(&a[i / 4])) return false;
+    }
+    return true;
+}
+
+struct StrRow {
+    const uint8_t *p;
+    uint32_t len;
+    bool valid;
+};
+
+__device__ __forceinline__ StrRow str_row(const StrSeg *segs, uint32_t nseg, uint64_t g) {
+    const StrSeg s = segs[str_seg(segs, nseg, g)];
+    const uint64_t r = g - s.row0;
+    const uint32_t o0 = gld(&s.off[r]), o1 = gld(&s.off[r + 1]);
+    StrRow out;
+    out.valid = !s.valid || gld(&s.valid[r]) != 0;
+    out.p = s.bytes + o0;
+    out.len = o1 >= o0 ? o1 - o0 : 0;
+    return out;
+}
+
+__global__ __launch_bounds__(256) void k_str_init(StrTab t) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i <= t.mask;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        t.key[i] = 0;
+        t.val[i] = STR_NONE;
+        t.rep[i] = STR_NONE;
+    }
+    if (blockIdx.x == 0 && threadIdx.x < SC_N) t.ctr[threadIdx.x] = 0;
+}
+
+// Every valid row back to its home slot (a restart with larger tables).
+__global__ __launch_bounds__(256) void k_str_rehome(uint64_t total, const uint64_t *hash, uint32_t *pos) {
+    for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total;
+         g += (uint64_t)gridDim.x * blockDim.x)
+        pos[g] = hash[g] ? STR_HOME : STR_NONE;
+}
+
+// Hashes, offsets check, valid row and byte counts, HyperLogLog per block.
+__global__ __launch_bounds__(256) void k_str_hash(const StrSeg *segs, uint32_t nseg, uint64_t total, StrTab t,
+                                                  uint64_t *hash, uint32_t *pos, uint32_t *codes, uint8_t *hllp) {
+    __shared__ uint32_t reg[HLL_M];
+    __shared__ uint64_t bsum[2];
+    for (uint32_t i = threadIdx.x; i < HLL_M; i += blockDim.x) reg[i] = 0;
+    if (threadIdx.x < 2) bsum[threadIdx.x] = 0;
+    __syncthreads();
+    uint64_t rows = 0, bytes = 0;
+    bool bad = false;
+    for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total;
+         g += (uint64_t)gridDim.x * blockDim.x) {
+        const StrSeg s = segs[str_seg(segs, nseg, g)];
+        const uint64_t r = g - s.row0;
+        const uint32_t o0 = gld(&s.off[r]), o1 = gld(&s.off[r + 1]);
+        const bool valid = !s.valid || gld(&s.valid[r]) != 0;
+        if (o1 < o0) bad = true;
+        if (!valid || o1 < o0) {
+            hash[g] = 0;
+            pos[g] = STR_NONE;
+            codes[g] = 0;
+            continue;
+        }
+        const uint32_t len = o1 - o0;
+        const uint64_t full = str_hash(s.bytes + o0, len, t.seed);
+        const uint64_t k = full & t.key_mask;
+        hash[g] = k ? k : 1;
+        pos[g] = STR_HOME;
+        hll_add(reg, full);
+        rows++;
+        bytes += len;
+    }
+    if (bad) __hip_atomic_fetch_or(&t.ctr[SC_FLAGS], STR_BAD_OFFSETS, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    atomicAdd((unsigned long long *)&bsum[0], (unsigned long long)rows);
+    atomicAdd((unsigned long long *)&bsum[1], (unsigned long long)bytes);
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < HLL_M; i += blockDim.x) hllp[(uint64_t)blockIdx.x * HLL_M + i] = (uint8_t)reg[i];
+    if (threadIdx.x == 0) {
+        atomicAdd((unsigned long long *)&t.ctr[SC_ROWS], (unsigned long long)bsum[0]);
+        atomicAdd((unsigned long long *)&t.ctr[SC_BYTES], (unsigned long long)bsum[1]);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_str_probe(const StrSeg *segs, uint32_t nseg, uint64_t total, StrTab t,
+                                                   const uint64_t *hash, uint32_t *pos, uint32_t *codes) {
+    for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total;
+         g += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t p0 = pos[g];
+        if (p0 == STR_NONE) continue;
+        const uint64_t h = hash[g];
+        const StrRow row = str_row(segs, nseg, g);
+        uint64_t s = p0 == STR_HOME ? str_home(h, t.mask) : p0;
+        uint32_t out = STR_NONE;
+        for (uint64_t probes = 0;; probes++) {
+            if (probes > t.mask) {
+                __hip_atomic_fetch_or(&t.ctr[SC_FLAGS], STR_FULL_TABLE, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                break;
+            }
+            uint64_t k = __hip_atomic_load(&t.key[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (k == 0) {
+                uint64_t expect = 0;
+                __hip_atomic_compare_exchange_strong(&t.key[s], &expect, h, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_AGENT);
+                k = expect == 0 ? h : expect;
+            }
+            if (k == h) {
+                const uint32_t c = t.val[s];
+                if (c == STR_NONE) {
+                    // no code yet: the smallest row here becomes the representative
+                    __hip_atomic_fetch_min(&t.rep[s], (uint32_t)g, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    out = (uint32_t)s;
+                    break;
+                }
+                if (str_equal(t, c, row.p, row.len)) {
+                    codes[g] = c;
+                    break;
+                }
+            }
+            s = (s + 1) & t.mask;
+        }
+        pos[g] = out;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_str_assign(const StrSeg *segs, uint32_t nseg, uint64_t total, StrTab t,
+                                                    const uint32_t *pos) {
+    for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total;
+         g += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t s = pos[g];
+        if (s == STR_NONE || t.rep[s] != (uint32_t)g) continue;
+        const StrRow row = str_row(segs, nseg, g);
+        const uint64_t words = (row.len + 3) / 4;
+        const uint64_t c = atomicAdd((unsigned long long *)&t.ctr[SC_CODES], 1ull);
+        const uint64_t w0 = atomicAdd((unsigned long long *)&t.ctr[SC_WORDS], (unsigned long long)words);
+        if (c >= t.code_cap || w0 + words > t.arena_words) {
+            __hip_atomic_fetch_or(&t.ctr[SC_FLAGS], STR_FULL_DICT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            continue;
+        }
+        for (uint32_t i = 0; i < row.len; i += 4) {
+            const uint32_t left = row.len - i;
+            gst(&t.arena[w0 + i / 4], str_word(row.p + i, left < 4 ? left : 4));
+        }
+        t.code_off[c] = w0;
+        t.code_len[c] = row.len;
+        t.val[s] = (uint32_t)c;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_str_check(const StrSeg *segs, uint32_t nseg, uint64_t total, StrTab t,
+                                                   uint32_t *pos, uint32_t *codes) {
+    uint64_t unresolved = 0;
+    for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total;
+         g += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t s = pos[g];
+        if (s == STR_NONE) continue;
+        const uint32_t c = t.val[s];
+        if (c == STR_NONE) {  // the dictionary was full: the host restarts larger
+            pos[g] = STR_NONE;
+            continue;
+        }
+        bool eq = t.rep[s] == (uint32_t)g;
+        if (!eq) {
+            const StrRow row = str_row(segs, nseg, g);
+            eq = str_equal(t, c, row.p, row.len);
+        }
+        if (eq) {
+            codes[g] = c;
+            pos[g] = STR_NONE;
+        } else {
+            pos[g] = (uint32_t)((s + 1) & t.mask);
+            unresolved++;
+        }
+    }
+    if (unresolved) atomicAdd((unsigned long long *)&t.ctr[SC_UNRESOLVED], (unsigned long long)unresolved);
+}
+
+// Byte i of code c's string.
+__device__ __forceinline__ uint32_t str_byte(const uint32_t *a, uint32_t i) {
+    return (a[i / 4] >> (8 * (i % 4))) & 0xFFu;
+}
+
+// Match table of one VARCHAR predicate over the column's codes: op 0 equal,
+// 1 not equal, 2 contains (LIKE '%lit%'), 3 does not contain.
+__global__ __launch_bounds__(256) void k_str_match(StrTab t, uint64_t ncodes, const uint8_t *lit, uint32_t litlen,
+                                                   int op, uint8_t *match) {
+    for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < ncodes;
+         c += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t len = t.code_len[c];
+        const uint32_t *a = t.arena + t.code_off[c];
+        bool r;
+        if (op <= 1) {
+            r = len == litlen;
+            for (uint32_t i = 0; r && i < len; i++) r = str_byte(a, i) == lit[i];
+            if (op == 1) r = !r;
+        } else {
+            r = litlen == 0;
+            for (uint32_t s = 0; !r && s + litlen <= len; s++) {
+                bool m = true;
+                for (uint32_t i = 0; m && i < litlen; i++) m = str_byte(a, s + i) == lit[i];
+                r = m;
+            }
+            if (op == 3) r = !r;
+        }
+        match[c] = r;
+    }
+}
+
+}  // namespace gbagg

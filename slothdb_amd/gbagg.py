@@ -27,6 +27,7 @@ LT, LE, GT, GE, EQ, NE, IN, LIKE, NOT_LIKE = range(9)
 # gbagg_strategy
 AUTO, DIRECT, LDS_HASH, GLOBAL_HASH, RADIX = range(5)
 DICT_UNIQUE = 1
+STR_RAW = 2
 
 OK, ERROR, INVALID = 0, 1, 2
 VECTOR_SIZE = 2048  # rows per DataChunk (include/slothdb/common/constants.hpp:16)
@@ -188,16 +189,34 @@ class Dictionary:
 
 class Column:
     """One input column of a batch: typed values, optional validity bytes
-    (0 = NULL) and, for VARCHAR, uint32 codes into a Dictionary."""
+    (0 = NULL) and, for VARCHAR, uint32 codes into a Dictionary, or the
+    strings themselves (Column.strings)."""
 
-    def __init__(self, type_, values, validity=None, dictionary=None):
+    def __init__(self, type_, values, validity=None, dictionary=None, raw=None):
         self.type = type_
         self.values = np.ascontiguousarray(values, _NP[type_])
         self.validity = None if validity is None else np.ascontiguousarray(validity, np.uint8)
         self.dictionary = dictionary
+        self.raw = None if raw is None else np.ascontiguousarray(raw, np.uint8)
+
+    @classmethod
+    def strings(cls, strings, offsets=None, data=None, validity=None):
+        """VARCHAR column given as its strings (GBAGG_STR_RAW): `values` holds
+        the n + 1 uint32 offsets into the bytes `raw`; None entries are NULL."""
+        if strings is not None:
+            enc = [b"" if x is None else (x.encode() if isinstance(x, str) else bytes(x)) for x in strings]
+            offsets = np.zeros(len(enc) + 1, np.uint32)
+            offsets[1:] = np.cumsum([len(x) for x in enc])
+            data = np.frombuffer(b"".join(enc) or b"\0", np.uint8).copy()
+            if validity is None and any(x is None for x in strings):
+                validity = np.array([x is not None for x in strings], np.uint8)
+        return cls(VARCHAR, offsets, validity, raw=data)
 
     def __len__(self):
-        return len(self.values)
+        return len(self.values) - 1 if self.raw is not None else len(self.values)
+
+    def string(self, i):
+        return bytes(self.raw[self.values[i]:self.values[i + 1]]).decode()
 
 
 class DeviceColumn:
@@ -210,6 +229,7 @@ class DeviceColumn:
         self._ptrs = []
         self.data = self._upload(col.values)
         self.validity = None if col.validity is None else self._upload(col.validity)
+        self.raw = None if col.raw is None else self._upload(col.raw)
 
     def _upload(self, arr):
         p = C.c_void_p()
@@ -243,6 +263,9 @@ def _views(columns, device):
         else:
             v.data = _ptr(c.values)
             v.validity = _ptr(c.validity)
+        if getattr(c, "raw", None) is not None:
+            v.dict_bytes = c.raw if device else c.raw.ctypes.data
+            v.flags = STR_RAW
         d = c.dictionary
         if d is not None:
             v.dict_offsets = d.offsets.ctypes.data

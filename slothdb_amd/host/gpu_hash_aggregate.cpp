@@ -4,7 +4,6 @@
 #include <algorithm>
 #include <cctype>
 #include <cstring>
-#include <unordered_map>
 
 namespace slothdb_gpu {
 
@@ -212,9 +211,9 @@ void PhysicalGpuHashAggregate::ConsumeChunk(const DataChunk &chunk) {
     const idx_t n = chunk.size();
     if (n == 0) return;
     // flat typed arrays pass as they are; validity bits become bytes; VARCHAR
-    // values become a per-chunk dictionary of distinct strings
+    // strings are flattened to offsets + bytes, which the GPU interns
     std::vector<std::vector<uint8_t>> valid(chunk.ColumnCount());
-    std::vector<std::vector<uint32_t>> codes(chunk.ColumnCount()), offsets(chunk.ColumnCount());
+    std::vector<std::vector<uint32_t>> offsets(chunk.ColumnCount());
     std::vector<std::string> bytes(chunk.ColumnCount());
     std::vector<gbagg_column_view> views(chunk.ColumnCount());
     for (size_t c = 0; c < chunk.ColumnCount(); c++) {
@@ -230,25 +229,15 @@ void PhysicalGpuHashAggregate::ConsumeChunk(const DataChunk &chunk) {
             g.validity = valid[c].data();
         }
         if (v.GetType() == LogicalTypeId::VARCHAR) {
-            std::unordered_map<std::string_view, uint32_t> ids;
-            codes[c].resize(n);
-            offsets[c].push_back(0);
-            std::vector<std::string_view> order;
+            offsets[c].resize(n + 1);
+            offsets[c][0] = 0;
             for (idx_t r = 0; r < n; r++) {
-                const std::string &s = v.Strings()[r];
-                auto [it, fresh] = ids.emplace(std::string_view(s), (uint32_t)order.size());
-                if (fresh) order.push_back(std::string_view(s));
-                codes[c][r] = it->second;
+                if (v.Validity().RowIsValid(r)) bytes[c].append(v.Strings()[r]);
+                offsets[c][r + 1] = (uint32_t)bytes[c].size();
             }
-            for (auto sv : order) {
-                bytes[c].append(sv);
-                offsets[c].push_back((uint32_t)bytes[c].size());
-            }
-            g.data = codes[c].data();
-            g.dict_offsets = offsets[c].data();
+            g.data = offsets[c].data();
             g.dict_bytes = bytes[c].data();
-            g.dict_size = (uint32_t)order.size();
-            g.flags = GBAGG_DICT_UNIQUE;
+            g.flags = GBAGG_STR_RAW;
         } else {
             g.data = v.GetData<uint8_t>();
         }

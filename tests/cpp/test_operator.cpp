@@ -141,6 +141,20 @@ static void test_emp_group_by() {
     CHECK((got["Marketing"] == std::vector<std::string>{"1", "95", "95.000000", "95", "95"}));
 }
 
+static void test_emp_two_chunks() {
+    // the same chunk twice: VARCHAR keys of both chunks meet in one group each
+    PhysicalGpuHashAggregate op({LT::VARCHAR, LT::INTEGER}, {0}, emp_aggs(), {}, {false, false});
+    op.Init();
+    op.ConsumeChunk(emp_chunk());
+    op.ConsumeChunk(emp_chunk());
+    std::map<std::string, std::vector<std::string>> got;
+    for (auto &r : drain(op)) got[r[0]] = std::vector<std::string>(r.begin() + 1, r.end());
+    CHECK(got.size() == 3);
+    CHECK((got["Engineering"] == std::vector<std::string>{"4", "440", "110.000000", "100", "120"}));
+    CHECK((got["Sales"] == std::vector<std::string>{"4", "340", "85.000000", "80", "90"}));
+    CHECK((got["Marketing"] == std::vector<std::string>{"2", "190", "95.000000", "95", "95"}));
+}
+
 static void test_emp_topn() {
     PhysicalGpuHashAggregate op({LT::VARCHAR, LT::INTEGER}, {0}, {{"count_star", -1}}, {}, {false, false});
     op.SetTopNHint(1, false, 3);  // ORDER BY cnt DESC
@@ -285,6 +299,7 @@ int main(int argc, char **argv) {
     if (!cpu_only) {
         test_emp_scalar();
         test_emp_group_by();
+        test_emp_two_chunks();
         test_emp_topn();
         test_sum_1_to_10000_rowgroup();
         test_50k_five_depts();
