@@ -260,6 +260,8 @@ typedef struct {
     double selector_distinct;   /* learned selector's distinct-group prediction (0 = not run) */
     int32_t selector_used;      /* 1: the plan followed the selector; 0: default plan (or no selector) */
     int32_t selector_path;      /* the selector's path: 0 LDS_HASH, 1 GLOBAL_HASH, 2 RADIX; -1 = not run */
+    double ms_strings;          /* interning of GBAGG_STR_RAW columns, before the aggregate (0 = none) */
+    uint64_t strings_distinct;  /* distinct strings over those columns */
 } gbagg_plan_info;
 
 typedef struct gbagg_handle gbagg_handle;

@@ -290,4 +290,25 @@ int sdg_gen_query_i32(uint64_t seed, uint64_t n, uint64_t distinct, double s, in
     return 0;
 }
 
+// A dictionary-coded string column as its strings (offsets + bytes): row i is
+// dictionary entry codes[i]. out_offsets has n + 1 entries; out_bytes holds
+// the returned byte count (call with out_bytes = nullptr to size it).
+int64_t sdg_strings_from_codes(const uint32_t *dict_off, const char *dict_bytes, const uint32_t *codes,
+                               uint64_t n, uint32_t *out_offsets, char *out_bytes, int threads) {
+    uint64_t pos = 0;
+    for (uint64_t i = 0; i < n; i++) {
+        if (out_offsets) out_offsets[i] = (uint32_t)pos;
+        pos += dict_off[codes[i] + 1] - dict_off[codes[i]];
+        if (pos > 0xFFFFFFFFull) return -1;
+    }
+    if (out_offsets) out_offsets[n] = (uint32_t)pos;
+    if (out_bytes && out_offsets)
+        parallel_rows(n, threads, [&](uint64_t b, uint64_t e) {
+            for (uint64_t i = b; i < e; i++)
+                std::memcpy(out_bytes + out_offsets[i], dict_bytes + dict_off[codes[i]],
+                            dict_off[codes[i] + 1] - dict_off[codes[i]]);
+        });
+    return (int64_t)pos;
+}
+
 }  // extern "C"

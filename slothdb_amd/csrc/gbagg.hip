@@ -218,10 +218,14 @@ bool valid_type(int t) {
            t == GBAGG_TYPE_VARCHAR;
 }
 
+struct StrCol;
+
 // Host string dictionary for one VARCHAR group column: codes of every batch
 // map to one id space so groups match across batches.
 struct KeyDict {
-    int mode = 0;  // 0 unset, 1 identity (unique dictionary shared by all batches), 2 map
+    int mode = 0;  // 0 unset, 1 identity (unique dictionary shared by all batches), 2 map,
+                   // 4 interned strings, dictionary still on the device (dev)
+    const StrCol *dev = nullptr;
     const uint32_t *offsets = nullptr;
     const char *bytes = nullptr;
     uint32_t size = 0;
@@ -243,9 +247,10 @@ struct KeyDict {
 // Interning state of one GBAGG_STR_RAW column (device buffers reused across
 // finalizes of the handle).
 struct StrCol {
-    DevBuf segs, hash, pos, codes, key, val, rep, arena, code_off, code_len, ctr, hllp, hllm, plan;
+    DevBuf segs, hash, pos, codes, key, val, rep, arena, code_off, code_len, ctr, hllp, hllm, plan, blk;
     std::vector<DevBuf> lit, match;  // per predicate
-    uint64_t ncodes = 0;
+    StrTab tab{};                    // the dictionary: arena, code_off, code_len
+    uint64_t ncodes = 0, words = 0;  // codes, arena words in use
 };
 
 struct OutCol {
@@ -342,6 +347,8 @@ struct gbagg_handle {
     gbagg_strategy used_strategy = GBAGG_STRATEGY_AUTO;
     uint32_t lds_slots = 0, agg_grid = 0, retries = 0, radix_bits = 0;
     double ms_agg = 0, ms_total = 0, ms_sample = 0, ms_count = 0, ms_partition = 0, ms_emit = 0;
+    double ms_strings = 0;
+    uint64_t strings_distinct = 0;
 
     // host result
     std::vector<std::vector<uint8_t>> res_data, res_valid;
@@ -1616,7 +1623,7 @@ void intern_column(gbagg_handle *h, int u) {
         const uint64_t mean_words = rows ? bytes / rows / 4 + 2 : 1;
         uint64_t arena_words = std::min<uint64_t>(words_max, code_cap * mean_words * 3 / 2 + 1024);
         for (uint32_t attempt = 0; rows; attempt++) {
-            if (slots > (1ull << 31)) fail(GBAGG_ERROR, "string table larger than 2^31 slots");
+            if (slots > (1ull << 30)) fail(GBAGG_ERROR, "string table larger than 2^30 slots");
             sc.key.ensure(slots * 8);
             sc.val.ensure(slots * 4);
             sc.rep.ensure(slots * 4);
@@ -1635,13 +1642,20 @@ void intern_column(gbagg_handle *h, int u) {
             const uint32_t ig = (uint32_t)std::min<uint64_t>((slots + 255) / 256, (uint64_t)di.cus * 8);
             hipLaunchKernelGGL(k_str_init, dim3(ig), dim3(256), 0, s, t);
             const uint32_t rg = (uint32_t)std::min<uint64_t>((total + 255) / 256, (uint64_t)di.cus * 8);
+            // code assignment: blocks of contiguous rows, at most 4096 of them
+            const uint64_t chunk = std::max<uint64_t>(256, ((total + 4095) / 4096 + 255) / 256 * 256);
+            const uint32_t nblk = (uint32_t)((total + chunk - 1) / chunk);
+            sc.blk.ensure((size_t)nblk * 16);
             hipLaunchKernelGGL(k_str_rehome, dim3(rg), dim3(256), 0, s, total, (const uint64_t *)sc.hash.ptr,
                                sc.pos.as<uint32_t>());
             for (uint32_t round = 0;; round++) {
                 hipLaunchKernelGGL(k_str_probe, dim3(rg), dim3(256), 0, s, dsegs, nseg, total, t,
                                    (const uint64_t *)sc.hash.ptr, sc.pos.as<uint32_t>(), sc.codes.as<uint32_t>());
-                hipLaunchKernelGGL(k_str_assign, dim3(rg), dim3(256), 0, s, dsegs, nseg, total, t,
-                                   (const uint32_t *)sc.pos.ptr);
+                hipLaunchKernelGGL(k_str_reps, dim3(nblk), dim3(256), 0, s, dsegs, nseg, total, t,
+                                   (const uint32_t *)sc.pos.ptr, chunk, sc.blk.as<uint64_t>());
+                hipLaunchKernelGGL(k_str_scan, dim3(1), dim3(1024), 0, s, sc.blk.as<uint64_t>(), nblk, t.ctr);
+                hipLaunchKernelGGL(k_str_assign, dim3(nblk), dim3(256), 0, s, dsegs, nseg, total, t,
+                                   (const uint32_t *)sc.pos.ptr, chunk, (const uint64_t *)sc.blk.ptr);
                 hipLaunchKernelGGL(k_str_check, dim3(rg), dim3(256), 0, s, dsegs, nseg, total, t,
                                    sc.pos.as<uint32_t>(), sc.codes.as<uint32_t>());
                 HIPCHK(hipMemcpyAsync(ctr, t.ctr, SC_N * 8, hipMemcpyDeviceToHost, s));
@@ -1661,7 +1675,9 @@ void intern_column(gbagg_handle *h, int u) {
             }
         }
         sc.ncodes = rows ? ctr[SC_CODES] : 0;
+        sc.words = rows ? ctr[SC_WORDS] : 0;
     }
+    sc.tab = t;
     const uint32_t nc = (uint32_t)sc.ncodes;
     // predicates on this column: one match per distinct string
     sc.lit.resize(h->preds.size());
@@ -1689,36 +1705,15 @@ void intern_column(gbagg_handle *h, int u) {
         b.db.data[u] = sc.codes.as<uint32_t>() + row0;
         row0 += b.nrows;
     }
-    // group column: the dictionary in code order, on the host
+    // group column: codes are the ids; the strings stay on the device until
+    // result rows (or a second aggregation phase) need them
     for (size_t g = 0; g < h->group_cols.size(); g++) {
         if (h->group_cols[g] != col) continue;
         KeyDict &kd = h->kdict[g];
         kd = KeyDict();
-        kd.mode = 1;
+        kd.mode = 4;
         kd.size = nc;
-        std::vector<uint64_t> off(nc);
-        std::vector<uint32_t> len(nc);
-        std::vector<uint32_t> words(std::max<uint64_t>(1, nc ? ctr[SC_WORDS] : 0));
-        if (nc) {
-            HIPCHK(hipMemcpyAsync(off.data(), sc.code_off.ptr, (size_t)nc * 8, hipMemcpyDeviceToHost, s));
-            HIPCHK(hipMemcpyAsync(len.data(), sc.code_len.ptr, (size_t)nc * 4, hipMemcpyDeviceToHost, s));
-            HIPCHK(hipMemcpyAsync(words.data(), sc.arena.ptr, (size_t)ctr[SC_WORDS] * 4, hipMemcpyDeviceToHost, s));
-            HIPCHK(hipStreamSynchronize(s));
-        }
-        kd.own_offsets.resize((size_t)nc + 1);
-        kd.own_offsets[0] = 0;
-        uint64_t nb = 0;
-        for (uint32_t c = 0; c < nc; c++) {
-            nb += len[c];
-            if (nb > 0xFFFFFFFFull) fail(GBAGG_ERROR, "distinct strings exceed 4 GiB");
-            kd.own_offsets[c + 1] = (uint32_t)nb;
-        }
-        kd.own_bytes.resize(std::max<uint64_t>(1, nb));
-        const char *src = (const char *)words.data();
-        for (uint32_t c = 0; c < nc; c++)
-            std::memcpy(kd.own_bytes.data() + kd.own_offsets[c], src + off[c] * 4, len[c]);
-        kd.offsets = kd.own_offsets.data();
-        kd.bytes = kd.own_bytes.data();
+        kd.dev = &sc;
         for (Batch &b : h->batches) {
             b.db.key_dict_size[g] = nc;
             b.db.remap[g] = nullptr;
@@ -1726,9 +1721,94 @@ void intern_column(gbagg_handle *h, int u) {
     }
 }
 
+// The whole dictionary of an interned group column, in code order, on the host.
+void host_dict(gbagg_handle *h, KeyDict &kd) {
+    if (kd.mode != 4) return;
+    const StrCol &sc = *kd.dev;
+    const uint64_t nc = sc.ncodes;
+    hipStream_t s = h->stream;
+    std::vector<uint64_t> off(nc);
+    std::vector<uint32_t> len(nc);
+    std::vector<uint32_t> words(std::max<uint64_t>(1, sc.words));
+    if (nc) {
+        HIPCHK(hipMemcpyAsync(off.data(), sc.code_off.ptr, (size_t)nc * 8, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(len.data(), sc.code_len.ptr, (size_t)nc * 4, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(words.data(), sc.arena.ptr, (size_t)sc.words * 4, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+    }
+    kd.own_offsets.resize((size_t)nc + 1);
+    kd.own_offsets[0] = 0;
+    uint64_t nb = 0;
+    for (uint64_t c = 0; c < nc; c++) {
+        nb += len[c];
+        if (nb > 0xFFFFFFFFull) fail(GBAGG_ERROR, "distinct strings exceed 4 GiB");
+        kd.own_offsets[c + 1] = (uint32_t)nb;
+    }
+    kd.own_bytes.resize(std::max<uint64_t>(1, nb));
+    const char *src = (const char *)words.data();
+    for (uint64_t c = 0; c < nc; c++) std::memcpy(kd.own_bytes.data() + kd.own_offsets[c], src + off[c] * 4, len[c]);
+    kd.offsets = kd.own_offsets.data();
+    kd.bytes = kd.own_bytes.data();
+    kd.mode = 1;
+    kd.dev = nullptr;
+}
+
+// Strings of the result rows of an interned group column, gathered on the
+// device: only the rows' own strings cross to the host.
+void gather_result_strings(gbagg_handle *h, size_t g, uint64_t n) {
+    const StrCol &sc = *h->kdict[g].dev;
+    const DeviceInfo &di = device_info();
+    hipStream_t s = h->stream;
+    auto &off = h->res_str_off[g];
+    auto &bytes = h->res_str_bytes[g];
+    off.assign(n + 1, 0);
+    bytes.clear();
+    if (!n || !sc.ncodes) return;
+    const uint32_t *ids = h->out_data[g].as<uint32_t>();
+    const uint8_t *valid = h->out_valid[g].as<uint8_t>();
+    const uint32_t grid = (uint32_t)std::min<uint64_t>((n + 255) / 256, (uint64_t)di.cus * 8);
+    DevBuf dlen, doff, dbytes;
+    dlen.ensure(n * 4);
+    hipLaunchKernelGGL(k_str_lengths, dim3(grid), dim3(256), 0, s, sc.tab, ids, valid, n, dlen.as<uint32_t>());
+    std::vector<uint32_t> len(n);
+    HIPCHK(hipMemcpyAsync(len.data(), dlen.ptr, n * 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    std::vector<uint64_t> o64(n);
+    uint64_t tot = 0;
+    for (uint64_t r = 0; r < n; r++) {
+        o64[r] = tot;
+        off[r] = (uint32_t)tot;
+        tot += len[r];
+        if (tot > 0xFFFFFFFFull) fail(GBAGG_ERROR, "result strings exceed 4 GiB");
+    }
+    off[n] = (uint32_t)tot;
+    bytes.resize(tot);
+    if (!tot) return;
+    doff.ensure(n * 8);
+    dbytes.ensure(tot);
+    HIPCHK(hipMemcpyAsync(doff.ptr, o64.data(), n * 8, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_str_gather, dim3(grid), dim3(256), 0, s, sc.tab, ids, valid, n,
+                       (const uint64_t *)doff.ptr, dbytes.as<uint8_t>());
+    HIPCHK(hipMemcpyAsync(bytes.data(), dbytes.ptr, tot, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+}
+
 void intern_strings(gbagg_handle *h) {
+    h->ms_strings = 0;
+    h->strings_distinct = 0;
+    bool any = false;
+    for (int u = 0; u < h->nu; u++) any |= (size_t)h->ucol[u] < h->str_mode.size() && h->str_mode[h->ucol[u]] == 2;
+    if (!any) return;
+    Timer tm{h, {}};
+    tm.mark();
     for (int u = 0; u < h->nu; u++)
-        if ((size_t)h->ucol[u] < h->str_mode.size() && h->str_mode[h->ucol[u]] == 2) intern_column(h, u);
+        if ((size_t)h->ucol[u] < h->str_mode.size() && h->str_mode[h->ucol[u]] == 2) {
+            intern_column(h, u);
+            h->strings_distinct += h->strs[h->ucol[u]]->ncodes;
+        }
+    tm.mark();
+    HIPCHK(hipStreamSynchronize(h->stream));
+    h->ms_strings = tm.between(0, 1);
 }
 
 void finalize(gbagg_handle *h) {
@@ -1783,7 +1863,8 @@ void run_phase1(gbagg_handle *h) {
         v[c].validity = h->nullable[c] ? p1->out_valid[c].as<uint8_t>() : nullptr;
         if (p1->outs[c].type == GBAGG_TYPE_VARCHAR && c < h->group_cols.size()) {
             // phase 1's canonical ids index its key dictionary: pass it on as a unique one
-            const KeyDict &kd = p1->kdict[c];
+            KeyDict &kd = p1->kdict[c];
+            host_dict(p1, kd);
             const uint32_t n = kd.mode == 1 ? kd.size : (uint32_t)kd.strings.size();
             h->p1_dict_off.assign(1, 0);
             h->p1_dict_bytes.clear();
@@ -1936,6 +2017,11 @@ void download_result(gbagg_handle *h) {
     // VARCHAR keys: ids -> strings
     for (size_t g = 0; g < h->group_cols.size(); g++) {
         if (!h->key_varchar[g]) continue;
+        if (h->kdict[g].mode == 4) {
+            gather_result_strings(h, g, n);
+            if (scalar_empty) h->res_str_off[g].assign(rows + 1, 0);
+            continue;
+        }
         const uint32_t *ids = (const uint32_t *)h->res_data[g].data();
         auto &off = h->res_str_off[g];
         auto &bytes = h->res_str_bytes[g];
@@ -2147,6 +2233,8 @@ gbagg_status gbagg_plan_info_get(gbagg_handle *h, gbagg_plan_info *out) {
         out->selector_distinct = h->sel_distinct;
         out->selector_used = h->sel_used;
         out->selector_path = h->sel_path;
+        out->ms_strings = h->ms_strings;
+        out->strings_distinct = h->strings_distinct;
     });
 }
 

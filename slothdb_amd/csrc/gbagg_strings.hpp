@@ -18,14 +18,15 @@
 //                 probe on); an empty slot is claimed for the hash; a slot
 //                 with the hash and no code yet records the smallest row
 //                 that reached it (atomicMin)
-//   k_str_assign  that row takes the next code, copies its string into the
-//                 arena and publishes the code on the slot
+//   k_str_reps / k_str_scan / k_str_assign
+//                 that row takes the next code (in row order), copies its
+//                 string into the arena and publishes the code on the slot
 //   k_str_check   the other rows that stopped there compare their bytes with
 //                 the code's string; rows with a different string probe on
 //                 from the next slot in the next round
 // until no row is unresolved (one round unless two distinct strings share a
-// 64-bit hash). Codes are dense from 0 in the order the representatives
-// took them.
+// 64-bit hash). Codes are dense from 0, in order of the strings' first rows
+// within each round.
 //   k_str_match   VARCHAR predicates (=, <>, LIKE '%x%', NOT LIKE) once per
 //                 distinct string, as the per-code match table the row
 //                 kernels read
@@ -35,6 +36,7 @@ namespace gbagg {
 
 constexpr uint32_t STR_NONE = 0xFFFFFFFFu;  // slot: no code yet; row: resolved or NULL
 constexpr uint32_t STR_HOME = 0xFFFFFFFEu;  // row: not probed yet, start at the home slot
+constexpr uint32_t STR_CAND = 0x80000000u;  // row's slot, flagged: the row lowered the slot's rep
 constexpr uint64_t STR_FULL_TABLE = 1, STR_FULL_DICT = 2, STR_BAD_OFFSETS = 4;
 enum { SC_CODES = 0, SC_WORDS, SC_UNRESOLVED, SC_FLAGS, SC_ROWS, SC_BYTES, SC_N };
 
@@ -223,9 +225,15 @@ __global__ __launch_bounds__(256) void k_str_probe(const StrSeg *segs, uint32_t 
             if (k == h) {
                 const uint32_t c = t.val[s];
                 if (c == STR_NONE) {
-                    // no code yet: the smallest row here becomes the representative
-                    __hip_atomic_fetch_min(&t.rep[s], (uint32_t)g, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    out = (uint32_t)s;
+                    // no code yet: the smallest row here becomes the representative;
+                    // only rows below the current one try (a hot string's rows
+                    // would otherwise all hit one word with atomics)
+                    const uint32_t cur = __hip_atomic_load(&t.rep[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    bool cand = false;
+                    if ((uint32_t)g < cur)
+                        cand = (uint32_t)g < __hip_atomic_fetch_min(&t.rep[s], (uint32_t)g, __ATOMIC_RELAXED,
+                                                                    __HIP_MEMORY_SCOPE_AGENT);
+                    out = (uint32_t)s | (cand ? STR_CAND : 0u);
                     break;
                 }
                 if (str_equal(t, c, row.p, row.len)) {
@@ -239,27 +247,133 @@ __global__ __launch_bounds__(256) void k_str_probe(const StrSeg *segs, uint32_t 
     }
 }
 
+// The row is its slot's representative: it lowered the slot's rep word in
+// k_str_probe and still holds it.
+__device__ __forceinline__ bool str_is_rep(const StrTab &t, const uint32_t *pos, uint64_t g, uint32_t &s) {
+    const uint32_t p = pos[g];
+    if (p == STR_NONE || !(p & STR_CAND)) return false;
+    s = p & ~STR_CAND;
+    return t.rep[s] == (uint32_t)g;
+}
+
+// Exclusive block-wide prefix sums of two values per thread (256 threads),
+// and their block totals.
+__device__ __forceinline__ void str_block_scan(uint64_t a, uint64_t b, uint64_t &ea, uint64_t &eb, uint64_t &ta,
+                                               uint64_t &tb) {
+    __shared__ uint64_t sa[256], sb[256];
+    const uint32_t i = threadIdx.x;
+    sa[i] = a;
+    sb[i] = b;
+    __syncthreads();
+    for (uint32_t d = 1; d < 256; d <<= 1) {
+        const uint64_t xa = i >= d ? sa[i - d] : 0, xb = i >= d ? sb[i - d] : 0;
+        __syncthreads();
+        sa[i] += xa;
+        sb[i] += xb;
+        __syncthreads();
+    }
+    ea = sa[i] - a;
+    eb = sb[i] - b;
+    ta = sa[255];
+    tb = sb[255];
+    __syncthreads();
+}
+
+// Codes are numbered in row order of their representatives (the order of
+// first appearance), by a scan instead of atomics: block b of the three
+// kernels owns rows [b * chunk, (b + 1) * chunk).
+//   k_str_reps    representatives and their arena words per block
+//   k_str_scan    exclusive scan of the block counts from the codes and words
+//                 already taken; the totals advance
+//   k_str_assign  each representative takes its code, copies its string into
+//                 the arena and publishes the code on its slot
+__global__ __launch_bounds__(256) void k_str_reps(const StrSeg *segs, uint32_t nseg, uint64_t total, StrTab t,
+                                                  const uint32_t *pos, uint64_t chunk, uint64_t *blk) {
+    const uint64_t r0 = (uint64_t)blockIdx.x * chunk, r1 = min(total, r0 + chunk);
+    uint64_t cnt = 0, words = 0;
+    for (uint64_t g = r0 + threadIdx.x; g < r1; g += blockDim.x) {
+        uint32_t s;
+        if (!str_is_rep(t, pos, g, s)) continue;
+        cnt++;
+        words += (str_row(segs, nseg, g).len + 3) / 4;
+    }
+    uint64_t ea, eb, ta, tb;
+    str_block_scan(cnt, words, ea, eb, ta, tb);
+    if (threadIdx.x == 0) {
+        blk[2 * blockIdx.x] = ta;
+        blk[2 * blockIdx.x + 1] = tb;
+    }
+}
+
+// One block of 1024 threads, nblk <= 4096: blk pairs become their exclusive
+// prefix sums plus the codes and words taken before this round.
+__global__ __launch_bounds__(1024) void k_str_scan(uint64_t *blk, uint32_t nblk, uint64_t *ctr) {
+    __shared__ uint64_t sa[1024], sb[1024];
+    const uint32_t i = threadIdx.x;
+    uint64_t a[4], b[4], ca = 0, cb = 0;
+    for (int j = 0; j < 4; j++) {
+        const uint32_t k = i * 4 + j;
+        a[j] = k < nblk ? blk[2 * k] : 0;
+        b[j] = k < nblk ? blk[2 * k + 1] : 0;
+        ca += a[j];
+        cb += b[j];
+    }
+    sa[i] = ca;
+    sb[i] = cb;
+    __syncthreads();
+    for (uint32_t d = 1; d < 1024; d <<= 1) {
+        const uint64_t xa = i >= d ? sa[i - d] : 0, xb = i >= d ? sb[i - d] : 0;
+        __syncthreads();
+        sa[i] += xa;
+        sb[i] += xb;
+        __syncthreads();
+    }
+    uint64_t pa = ctr[SC_CODES] + sa[i] - ca, pb = ctr[SC_WORDS] + sb[i] - cb;
+    for (int j = 0; j < 4; j++) {
+        const uint32_t k = i * 4 + j;
+        if (k < nblk) {
+            blk[2 * k] = pa;
+            blk[2 * k + 1] = pb;
+        }
+        pa += a[j];
+        pb += b[j];
+    }
+    __syncthreads();
+    if (i == 1023) {
+        ctr[SC_CODES] += sa[1023];
+        ctr[SC_WORDS] += sb[1023];
+    }
+}
+
 __global__ __launch_bounds__(256) void k_str_assign(const StrSeg *segs, uint32_t nseg, uint64_t total, StrTab t,
-                                                    const uint32_t *pos) {
-    for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total;
-         g += (uint64_t)gridDim.x * blockDim.x) {
-        const uint32_t s = pos[g];
-        if (s == STR_NONE || t.rep[s] != (uint32_t)g) continue;
-        const StrRow row = str_row(segs, nseg, g);
-        const uint64_t words = (row.len + 3) / 4;
-        const uint64_t c = atomicAdd((unsigned long long *)&t.ctr[SC_CODES], 1ull);
-        const uint64_t w0 = atomicAdd((unsigned long long *)&t.ctr[SC_WORDS], (unsigned long long)words);
-        if (c >= t.code_cap || w0 + words > t.arena_words) {
-            __hip_atomic_fetch_or(&t.ctr[SC_FLAGS], STR_FULL_DICT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            continue;
+                                                    const uint32_t *pos, uint64_t chunk, const uint64_t *blk) {
+    const uint64_t r0 = (uint64_t)blockIdx.x * chunk, r1 = min(total, r0 + chunk);
+    uint64_t bc = blk[2 * blockIdx.x], bw = blk[2 * blockIdx.x + 1];
+    for (uint64_t tile = r0; tile < r1; tile += blockDim.x) {
+        const uint64_t g = tile + threadIdx.x;
+        uint32_t s = 0;
+        const bool rep = g < r1 && str_is_rep(t, pos, g, s);
+        StrRow row{};
+        if (rep) row = str_row(segs, nseg, g);
+        const uint64_t words = rep ? (row.len + 3) / 4 : 0;
+        uint64_t ec, ew, tc, tw;
+        str_block_scan(rep ? 1 : 0, words, ec, ew, tc, tw);
+        if (rep) {
+            const uint64_t c = bc + ec, w0 = bw + ew;
+            if (c >= t.code_cap || w0 + words > t.arena_words) {
+                __hip_atomic_fetch_or(&t.ctr[SC_FLAGS], STR_FULL_DICT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            } else {
+                for (uint32_t i = 0; i < row.len; i += 4) {
+                    const uint32_t left = row.len - i;
+                    gst(&t.arena[w0 + i / 4], str_word(row.p + i, left < 4 ? left : 4));
+                }
+                t.code_off[c] = w0;
+                t.code_len[c] = row.len;
+                t.val[s] = (uint32_t)c;
+            }
         }
-        for (uint32_t i = 0; i < row.len; i += 4) {
-            const uint32_t left = row.len - i;
-            gst(&t.arena[w0 + i / 4], str_word(row.p + i, left < 4 ? left : 4));
-        }
-        t.code_off[c] = w0;
-        t.code_len[c] = row.len;
-        t.val[s] = (uint32_t)c;
+        bc += tc;
+        bw += tw;
     }
 }
 
@@ -268,14 +382,15 @@ __global__ __launch_bounds__(256) void k_str_check(const StrSeg *segs, uint32_t 
     uint64_t unresolved = 0;
     for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total;
          g += (uint64_t)gridDim.x * blockDim.x) {
-        const uint32_t s = pos[g];
-        if (s == STR_NONE) continue;
+        const uint32_t p = pos[g];
+        if (p == STR_NONE) continue;
+        const uint32_t s = p & ~STR_CAND;
         const uint32_t c = t.val[s];
         if (c == STR_NONE) {  // the dictionary was full: the host restarts larger
             pos[g] = STR_NONE;
             continue;
         }
-        bool eq = t.rep[s] == (uint32_t)g;
+        bool eq = (p & STR_CAND) && t.rep[s] == (uint32_t)g;
         if (!eq) {
             const StrRow row = str_row(segs, nseg, g);
             eq = str_equal(t, c, row.p, row.len);
@@ -289,6 +404,27 @@ __global__ __launch_bounds__(256) void k_str_check(const StrSeg *segs, uint32_t 
         }
     }
     if (unresolved) atomicAdd((unsigned long long *)&t.ctr[SC_UNRESOLVED], (unsigned long long)unresolved);
+}
+
+// Result rows of a group column of interned strings: bytes of each row's
+// string (0 for NULL rows), then the strings at the given offsets.
+__global__ __launch_bounds__(256) void k_str_lengths(StrTab t, const uint32_t *ids, const uint8_t *valid,
+                                                     uint64_t rows, uint32_t *len) {
+    for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < rows;
+         r += (uint64_t)gridDim.x * blockDim.x)
+        len[r] = valid[r] ? t.code_len[ids[r]] : 0;
+}
+
+__global__ __launch_bounds__(256) void k_str_gather(StrTab t, const uint32_t *ids, const uint8_t *valid,
+                                                    uint64_t rows, const uint64_t *off, uint8_t *out) {
+    for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < rows;
+         r += (uint64_t)gridDim.x * blockDim.x) {
+        if (!valid[r]) continue;
+        const uint32_t c = ids[r], len = t.code_len[c];
+        const uint32_t *a = t.arena + t.code_off[c];
+        uint8_t *o = out + off[r];
+        for (uint32_t i = 0; i < len; i++) o[i] = (uint8_t)(a[i / 4] >> (8 * (i % 4)));
+    }
 }
 
 // Byte i of code c's string.

@@ -46,6 +46,8 @@ def lib():
         L.sdg_gen_dict.argtypes = [u64, u32, vp, vp]
         L.sdg_gen_composite.argtypes = [u64, u64, u64, i64, u32, dbl, i64, vp, vp, vp, i32]
         L.sdg_gen_query_i32.argtypes = [u64, u64, u64, dbl, i64, i64, vp, vp, i32]
+        L.sdg_strings_from_codes.restype = i64
+        L.sdg_strings_from_codes.argtypes = [vp, vp, vp, u64, vp, vp, i32]
         _lib = L
     return _lib
 
@@ -164,6 +166,29 @@ def composite_workload(n=50_000_000, dict_size=1_000_000, seed=SEED, topn=10):
                 predicates=[(2, G.LT, 30)], nullable=[False] * 3,
                 topn=(2, False, topn) if topn else None)
     cols = [G.Column(G.INTEGER, a), G.Column(G.VARCHAR, b, dictionary=dic), G.Column(G.INTEGER, c)]
+    return q, cols
+
+
+def strings_from_codes(col):
+    """A dictionary-coded VARCHAR Column as the same strings given as such
+    (Column.strings form: offsets + bytes)."""
+    d = col.dictionary
+    codes = np.ascontiguousarray(col.values, np.uint32)
+    off = np.empty(len(codes) + 1, np.uint32)
+    nb = lib().sdg_strings_from_codes(d.offsets.ctypes.data, d.data.ctypes.data, codes.ctypes.data, len(codes),
+                                      off.ctypes.data, None, _threads())
+    if nb < 0:
+        raise ValueError("more than 4 GiB of strings")
+    data = np.empty(max(1, nb), np.uint8)
+    lib().sdg_strings_from_codes(d.offsets.ctypes.data, d.data.ctypes.data, codes.ctypes.data, len(codes),
+                                 off.ctypes.data, data.ctypes.data, _threads())
+    return G.Column(G.VARCHAR, off, col.validity, raw=data)
+
+
+def composite_strings_workload(n=50_000_000, dict_size=1_000_000, seed=SEED, topn=10):
+    """The composite workload with b given as its strings rather than codes."""
+    q, cols = composite_workload(n, dict_size, seed, topn)
+    cols[1] = strings_from_codes(cols[1])
     return q, cols
 
 

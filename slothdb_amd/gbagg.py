@@ -80,7 +80,8 @@ class PlanInfo(C.Structure):
                 ("ms_aggregate", C.c_double), ("ms_total", C.c_double), ("ms_sample", C.c_double),
                 ("ms_count", C.c_double), ("ms_partition", C.c_double), ("ms_emit", C.c_double),
                 ("kernel_shape", C.c_char * 32), ("selector_distinct", C.c_double),
-                ("selector_used", C.c_int32), ("selector_path", C.c_int32)]
+                ("selector_used", C.c_int32), ("selector_path", C.c_int32),
+                ("ms_strings", C.c_double), ("strings_distinct", C.c_uint64)]
 
 
 # Every symbol include/slothdb_gpu.h declares, with its ctypes signature.

@@ -63,3 +63,13 @@ def test_query_i32():
     assert len(np.unique(k)) <= 5000
     k2, _ = datagen.query_i32(100_000, 5000, 0.7, seed=3)
     assert (k == k2).all()
+
+
+def test_strings_from_codes():
+    # every row's string is its code's dictionary entry; offsets ascend from 0
+    q, cols = datagen.composite_workload(n=50_000, dict_size=2_000)
+    s = datagen.strings_from_codes(cols[1])
+    d = cols[1].dictionary
+    assert len(s) == 50_000 and s.values[0] == 0 and (np.diff(s.values.astype(np.int64)) >= 4).all()
+    for i in range(0, 50_000, 113):
+        assert s.string(i) == d.get(int(cols[1].values[i]))
