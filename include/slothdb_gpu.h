@@ -300,7 +300,10 @@ void gbagg_destroy(gbagg_handle *h);
  *   "wide_seed"     first seed of the 16-byte key mixing (reruns take the next
  *                   seeds while two groups share a key word; results never change)
  *   "static_shapes" 1 = use compile-time specialised row kernels when the query
- *                   shape matches one (default), 0 = always the generic kernels */
+ *                   shape matches one (default), 0 = always the generic kernels
+ *   "str_hash_bits" bits of the string hash kept when interning GBAGG_STR_RAW
+ *                   columns (1-64, default 64); fewer bits only force strings to
+ *                   share hashes, which the byte comparison resolves */
 gbagg_status gbagg_set_param(gbagg_handle *h, const char *name, int64_t value);
 
 /* GPUEngine::HashAggregate (include/slothdb/execution/gpu/gpu_engine.hpp:79-85):

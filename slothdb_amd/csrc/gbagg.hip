@@ -247,9 +247,9 @@ struct KeyDict {
 // Interning state of one GBAGG_STR_RAW column (device buffers reused across
 // finalizes of the handle).
 struct StrCol {
-    DevBuf segs, hash, pos, codes, key, val, rep, arena, code_off, code_len, ctr, hllp, hllm, plan, blk;
+    DevBuf segs, hash, pos, codes, slot, code, arena, ctr, hllp, hllm, plan, blk;
     std::vector<DevBuf> lit, match;  // per predicate
-    StrTab tab{};                    // the dictionary: arena, code_off, code_len
+    StrTab tab{};                    // the dictionary: arena and code records
     uint64_t ncodes = 0, words = 0;  // codes, arena words in use
 };
 
@@ -1624,18 +1624,13 @@ void intern_column(gbagg_handle *h, int u) {
         uint64_t arena_words = std::min<uint64_t>(words_max, code_cap * mean_words * 3 / 2 + 1024);
         for (uint32_t attempt = 0; rows; attempt++) {
             if (slots > (1ull << 30)) fail(GBAGG_ERROR, "string table larger than 2^30 slots");
-            sc.key.ensure(slots * 8);
-            sc.val.ensure(slots * 4);
-            sc.rep.ensure(slots * 4);
+            if (arena_words > 0xFFFFFFFFull) fail(GBAGG_ERROR, "distinct strings exceed 16 GiB");
+            sc.slot.ensure(slots * sizeof(StrSlot));
             sc.arena.ensure(arena_words * 4);
-            sc.code_off.ensure(code_cap * 8);
-            sc.code_len.ensure(code_cap * 4);
-            t.key = sc.key.as<uint64_t>();
-            t.val = sc.val.as<uint32_t>();
-            t.rep = sc.rep.as<uint32_t>();
+            sc.code.ensure(code_cap * sizeof(StrCode));
+            t.slot = sc.slot.as<StrSlot>();
+            t.code = sc.code.as<StrCode>();
             t.arena = sc.arena.as<uint32_t>();
-            t.code_off = sc.code_off.as<uint64_t>();
-            t.code_len = sc.code_len.as<uint32_t>();
             t.mask = slots - 1;
             t.arena_words = arena_words;
             t.code_cap = code_cap;
@@ -1727,12 +1722,10 @@ void host_dict(gbagg_handle *h, KeyDict &kd) {
     const StrCol &sc = *kd.dev;
     const uint64_t nc = sc.ncodes;
     hipStream_t s = h->stream;
-    std::vector<uint64_t> off(nc);
-    std::vector<uint32_t> len(nc);
+    std::vector<StrCode> cd(nc);
     std::vector<uint32_t> words(std::max<uint64_t>(1, sc.words));
     if (nc) {
-        HIPCHK(hipMemcpyAsync(off.data(), sc.code_off.ptr, (size_t)nc * 8, hipMemcpyDeviceToHost, s));
-        HIPCHK(hipMemcpyAsync(len.data(), sc.code_len.ptr, (size_t)nc * 4, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(cd.data(), sc.code.ptr, (size_t)nc * sizeof(StrCode), hipMemcpyDeviceToHost, s));
         HIPCHK(hipMemcpyAsync(words.data(), sc.arena.ptr, (size_t)sc.words * 4, hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
     }
@@ -1740,13 +1733,14 @@ void host_dict(gbagg_handle *h, KeyDict &kd) {
     kd.own_offsets[0] = 0;
     uint64_t nb = 0;
     for (uint64_t c = 0; c < nc; c++) {
-        nb += len[c];
+        nb += cd[c].len;
         if (nb > 0xFFFFFFFFull) fail(GBAGG_ERROR, "distinct strings exceed 4 GiB");
         kd.own_offsets[c + 1] = (uint32_t)nb;
     }
     kd.own_bytes.resize(std::max<uint64_t>(1, nb));
     const char *src = (const char *)words.data();
-    for (uint64_t c = 0; c < nc; c++) std::memcpy(kd.own_bytes.data() + kd.own_offsets[c], src + off[c] * 4, len[c]);
+    for (uint64_t c = 0; c < nc; c++)
+        std::memcpy(kd.own_bytes.data() + kd.own_offsets[c], src + (uint64_t)cd[c].word * 4, cd[c].len);
     kd.offsets = kd.own_offsets.data();
     kd.bytes = kd.own_bytes.data();
     kd.mode = 1;

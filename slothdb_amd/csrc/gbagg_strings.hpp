@@ -49,13 +49,23 @@ struct StrSeg {
     uint64_t n;
 };
 
+// One slot of the table: a probe reads one 16-byte record.
+struct StrSlot {
+    uint64_t key;  // stored hash, 0 = empty
+    uint32_t val;  // code, STR_NONE = none yet
+    uint32_t rep;  // smallest row that stopped here while the slot had no code
+};
+
+// One code: its string's first arena word and its length.
+struct StrCode {
+    uint32_t word;
+    uint32_t len;
+};
+
 struct StrTab {
-    uint64_t *key;       // slot: stored hash, 0 = empty
-    uint32_t *val;       // slot: code, STR_NONE = none yet
-    uint32_t *rep;       // slot: smallest row that stopped there while it had no code
+    StrSlot *slot;
+    StrCode *code;
     uint32_t *arena;     // distinct strings, each from a word boundary, zero padded
-    uint64_t *code_off;  // per code: first arena word
-    uint32_t *code_len;  // per code: bytes
     uint64_t *ctr;       // SC_* counters
     uint64_t mask;       // slots - 1
     uint64_t arena_words;
@@ -114,8 +124,10 @@ __device__ __forceinline__ uint64_t str_home(uint64_t key, uint64_t mask) {
 
 // The row's string equals code c's arena string.
 __device__ __forceinline__ bool str_equal(const StrTab &t, uint32_t c, const uint8_t *p, uint32_t len) {
-    if (gld(&t.code_len[c]) != len) return false;
-    const uint32_t *a = t.arena + gld(&t.code_off[c]);
+    const uint64_t v = gld((const uint64_t *)&t.code[c]);  // word, then length
+    const StrCode cd{(uint32_t)v, (uint32_t)(v >> 32)};
+    if (cd.len != len) return false;
+    const uint32_t *a = t.arena + cd.word;
     for (uint32_t i = 0; i < len; i += 4) {
         const uint32_t left = len - i;
         if (str_word(p + i, left < 4 ? left : 4) != gld(&a[i / 4])) return false;
@@ -143,9 +155,7 @@ __device__ __forceinline__ StrRow str_row(const StrSeg *segs, uint32_t nseg, uin
 __global__ __launch_bounds__(256) void k_str_init(StrTab t) {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i <= t.mask;
          i += (uint64_t)gridDim.x * blockDim.x) {
-        t.key[i] = 0;
-        t.val[i] = STR_NONE;
-        t.rep[i] = STR_NONE;
+        t.slot[i] = StrSlot{0, STR_NONE, STR_NONE};
     }
     if (blockIdx.x == 0 && threadIdx.x < SC_N) t.ctr[threadIdx.x] = 0;
 }
@@ -215,23 +225,24 @@ __global__ __launch_bounds__(256) void k_str_probe(const StrSeg *segs, uint32_t 
                 __hip_atomic_fetch_or(&t.ctr[SC_FLAGS], STR_FULL_TABLE, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 break;
             }
-            uint64_t k = __hip_atomic_load(&t.key[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            StrSlot *sl = &t.slot[s];
+            uint64_t k = __hip_atomic_load(&sl->key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if (k == 0) {
                 uint64_t expect = 0;
-                __hip_atomic_compare_exchange_strong(&t.key[s], &expect, h, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                __hip_atomic_compare_exchange_strong(&sl->key, &expect, h, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
                                                      __HIP_MEMORY_SCOPE_AGENT);
                 k = expect == 0 ? h : expect;
             }
             if (k == h) {
-                const uint32_t c = t.val[s];
+                const uint32_t c = sl->val;
                 if (c == STR_NONE) {
                     // no code yet: the smallest row here becomes the representative;
                     // only rows below the current one try (a hot string's rows
                     // would otherwise all hit one word with atomics)
-                    const uint32_t cur = __hip_atomic_load(&t.rep[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    const uint32_t cur = __hip_atomic_load(&sl->rep, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     bool cand = false;
                     if ((uint32_t)g < cur)
-                        cand = (uint32_t)g < __hip_atomic_fetch_min(&t.rep[s], (uint32_t)g, __ATOMIC_RELAXED,
+                        cand = (uint32_t)g < __hip_atomic_fetch_min(&sl->rep, (uint32_t)g, __ATOMIC_RELAXED,
                                                                     __HIP_MEMORY_SCOPE_AGENT);
                     out = (uint32_t)s | (cand ? STR_CAND : 0u);
                     break;
@@ -253,7 +264,7 @@ __device__ __forceinline__ bool str_is_rep(const StrTab &t, const uint32_t *pos,
     const uint32_t p = pos[g];
     if (p == STR_NONE || !(p & STR_CAND)) return false;
     s = p & ~STR_CAND;
-    return t.rep[s] == (uint32_t)g;
+    return t.slot[s].rep == (uint32_t)g;
 }
 
 // Exclusive block-wide prefix sums of two values per thread (256 threads),
@@ -367,9 +378,8 @@ __global__ __launch_bounds__(256) void k_str_assign(const StrSeg *segs, uint32_t
                     const uint32_t left = row.len - i;
                     gst(&t.arena[w0 + i / 4], str_word(row.p + i, left < 4 ? left : 4));
                 }
-                t.code_off[c] = w0;
-                t.code_len[c] = row.len;
-                t.val[s] = (uint32_t)c;
+                t.code[c] = StrCode{(uint32_t)w0, row.len};
+                t.slot[s].val = (uint32_t)c;
             }
         }
         bc += tc;
@@ -385,12 +395,13 @@ __global__ __launch_bounds__(256) void k_str_check(const StrSeg *segs, uint32_t 
         const uint32_t p = pos[g];
         if (p == STR_NONE) continue;
         const uint32_t s = p & ~STR_CAND;
-        const uint32_t c = t.val[s];
+        const uint64_t vr = gld((const uint64_t *)&t.slot[s].val);  // code and representative
+        const uint32_t c = (uint32_t)vr;
         if (c == STR_NONE) {  // the dictionary was full: the host restarts larger
             pos[g] = STR_NONE;
             continue;
         }
-        bool eq = (p & STR_CAND) && t.rep[s] == (uint32_t)g;
+        bool eq = (p & STR_CAND) && (uint32_t)(vr >> 32) == (uint32_t)g;
         if (!eq) {
             const StrRow row = str_row(segs, nseg, g);
             eq = str_equal(t, c, row.p, row.len);
@@ -412,7 +423,7 @@ __global__ __launch_bounds__(256) void k_str_lengths(StrTab t, const uint32_t *i
                                                      uint64_t rows, uint32_t *len) {
     for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < rows;
          r += (uint64_t)gridDim.x * blockDim.x)
-        len[r] = valid[r] ? t.code_len[ids[r]] : 0;
+        len[r] = valid[r] ? t.code[ids[r]].len : 0;
 }
 
 __global__ __launch_bounds__(256) void k_str_gather(StrTab t, const uint32_t *ids, const uint8_t *valid,
@@ -420,8 +431,9 @@ __global__ __launch_bounds__(256) void k_str_gather(StrTab t, const uint32_t *id
     for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < rows;
          r += (uint64_t)gridDim.x * blockDim.x) {
         if (!valid[r]) continue;
-        const uint32_t c = ids[r], len = t.code_len[c];
-        const uint32_t *a = t.arena + t.code_off[c];
+        const StrCode cd = t.code[ids[r]];
+        const uint32_t len = cd.len;
+        const uint32_t *a = t.arena + cd.word;
         uint8_t *o = out + off[r];
         for (uint32_t i = 0; i < len; i++) o[i] = (uint8_t)(a[i / 4] >> (8 * (i % 4)));
     }
@@ -438,8 +450,9 @@ __global__ __launch_bounds__(256) void k_str_match(StrTab t, uint64_t ncodes, co
                                                    int op, uint8_t *match) {
     for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < ncodes;
          c += (uint64_t)gridDim.x * blockDim.x) {
-        const uint32_t len = t.code_len[c];
-        const uint32_t *a = t.arena + t.code_off[c];
+        const StrCode cd = t.code[c];
+        const uint32_t len = cd.len;
+        const uint32_t *a = t.arena + cd.word;
         bool r;
         if (op <= 1) {
             r = len == litlen;
