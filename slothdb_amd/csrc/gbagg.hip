@@ -248,6 +248,9 @@ struct KeyDict {
 // finalizes of the handle).
 struct StrCol {
     DevBuf segs, hash, pos, codes, slot, code, arena, ctr, hllp, hllm, plan, blk;
+    DevBuf coff, cbytes;                    // group column: its dictionary batches' strings (canonical ids)
+    std::vector<uint32_t> coff_h;
+    std::vector<char> cbytes_h;
     std::vector<DevBuf> lit, match;  // per predicate
     StrTab tab{};                    // the dictionary: arena and code records
     uint64_t ncodes = 0, words = 0;  // codes, arena words in use
@@ -819,10 +822,9 @@ void add_batch(gbagg_handle *h, const gbagg_column_view *cols, uint64_t n, bool 
         if (strs && c.type != GBAGG_TYPE_VARCHAR) fail(GBAGG_INVALID, "GBAGG_STR_RAW is for VARCHAR columns");
         if (strs && (c.dict_offsets || c.dict_size)) fail(GBAGG_INVALID, "GBAGG_STR_RAW columns carry no dictionary");
         if (c.type == GBAGG_TYPE_VARCHAR) {
+            // a column with a batch of strings is interned at finalize
             uint8_t &m = modes[h->ucol[u]];
-            if (m && m != (strs ? 2 : 1))
-                fail(GBAGG_INVALID, "batches of one VARCHAR column must all be strings or all dictionary codes");
-            m = strs ? 2 : 1;
+            m = strs ? 2 : std::max<uint8_t>(m, 1);
         }
         size_t w = type_width(c.type);
         if (strs) {
@@ -1572,11 +1574,52 @@ void intern_column(gbagg_handle *h, int u) {
     if (h->strs.size() != h->col_types.size()) h->strs.resize(h->col_types.size());
     if (!h->strs[col]) h->strs[col] = std::make_unique<StrCol>();
     StrCol &sc = *h->strs[col];
+    // batches of strings, and for a group column also its batches of
+    // dictionary codes (read through the canonical dictionary), so that every
+    // batch ends with codes of one dictionary
+    int key_g = -1;
+    for (size_t g = 0; g < h->group_cols.size() && key_g < 0; g++)
+        if (h->group_cols[g] == col) key_g = (int)g;
+    bool dict_batches = false;
+    for (const Batch &b : h->batches) dict_batches |= b.nrows && !b.str_off[u];
+    if (key_g >= 0 && dict_batches) {
+        const KeyDict &kd = h->kdict[key_g];
+        sc.coff_h.assign(1, 0);
+        sc.cbytes_h.clear();
+        if (kd.mode == 1) {
+            const uint32_t *o = kd.own_offsets.empty() ? kd.offsets : kd.own_offsets.data();
+            const char *by = kd.own_bytes.empty() ? kd.bytes : kd.own_bytes.data();
+            sc.coff_h.assign(o, o + kd.size + 1);
+            sc.cbytes_h.assign(by, by + o[kd.size]);
+        } else if (kd.mode == 2) {
+            for (const std::string &str : kd.strings) {
+                sc.cbytes_h.insert(sc.cbytes_h.end(), str.begin(), str.end());
+                if (sc.cbytes_h.size() > 0xFFFFFFFFull) fail(GBAGG_ERROR, "dictionary larger than 4 GiB");
+                sc.coff_h.push_back((uint32_t)sc.cbytes_h.size());
+            }
+        }
+        sc.coff.ensure(sc.coff_h.size() * 4);
+        sc.cbytes.ensure(sc.cbytes_h.size() + 16);
+        HIPCHK(hipMemcpyAsync(sc.coff.ptr, sc.coff_h.data(), sc.coff_h.size() * 4, hipMemcpyHostToDevice, s));
+        if (!sc.cbytes_h.empty())
+            HIPCHK(hipMemcpyAsync(sc.cbytes.ptr, sc.cbytes_h.data(), sc.cbytes_h.size(), hipMemcpyHostToDevice, s));
+    }
     std::vector<StrSeg> segs;
+    std::vector<int64_t> row0s(h->batches.size(), -1);
     uint64_t total = 0;
-    for (const Batch &b : h->batches) {
+    for (size_t i = 0; i < h->batches.size(); i++) {
+        const Batch &b = h->batches[i];
         if (!b.nrows) continue;
-        segs.push_back(StrSeg{b.str_off[u], b.str_bytes[u], b.db.valid[u], total, b.nrows});
+        if (b.str_off[u]) {
+            segs.push_back(StrSeg{b.str_off[u], b.str_bytes[u], b.db.valid[u], nullptr, nullptr, total, b.nrows, 0, 0});
+        } else if (key_g >= 0) {
+            segs.push_back(StrSeg{sc.coff.as<uint32_t>(), sc.cbytes.as<uint8_t>(), b.db.valid[u],
+                                  (const uint32_t *)b.db.data[u], b.db.remap[key_g], total, b.nrows,
+                                  b.db.key_dict_size[key_g], 0});
+        } else {
+            continue;  // dictionary codes of a column that is not grouped by keep their own codes
+        }
+        row0s[i] = (int64_t)total;
         total += b.nrows;
     }
     if (total > (1ull << 30)) fail(GBAGG_INVALID, "more than 2^30 rows in one aggregate");
@@ -1689,17 +1732,14 @@ void intern_column(gbagg_handle *h, int u) {
             hipLaunchKernelGGL(k_str_match, dim3((uint32_t)std::min<uint64_t>((nc + 255) / 256, (uint64_t)di.cus * 4)),
                                dim3(256), 0, s, t, (uint64_t)nc, (const uint8_t *)sc.lit[i].ptr,
                                (uint32_t)lit.size(), mop, sc.match[i].as<uint8_t>());
-        for (Batch &b : h->batches) {
-            b.db.pred_ptr[i] = sc.match[i].ptr;
-            b.db.pred_count[i] = nc;
-        }
+        for (size_t k = 0; k < h->batches.size(); k++)
+            if (row0s[k] >= 0) {
+                h->batches[k].db.pred_ptr[i] = sc.match[i].ptr;
+                h->batches[k].db.pred_count[i] = nc;
+            }
     }
-    uint64_t row0 = 0;
-    for (Batch &b : h->batches) {
-        if (!b.nrows) continue;
-        b.db.data[u] = sc.codes.as<uint32_t>() + row0;
-        row0 += b.nrows;
-    }
+    for (size_t i = 0; i < h->batches.size(); i++)
+        if (row0s[i] >= 0) h->batches[i].db.data[u] = sc.codes.as<uint32_t>() + row0s[i];
     // group column: codes are the ids; the strings stay on the device until
     // result rows (or a second aggregation phase) need them
     for (size_t g = 0; g < h->group_cols.size(); g++) {
@@ -1710,7 +1750,7 @@ void intern_column(gbagg_handle *h, int u) {
         kd.size = nc;
         kd.dev = &sc;
         for (Batch &b : h->batches) {
-            b.db.key_dict_size[g] = nc;
+            b.db.key_dict_size[g] = std::max<uint32_t>(nc, 1);  // dropped rows carry STR_DROP
             b.db.remap[g] = nullptr;
         }
     }
@@ -2911,7 +2951,8 @@ gbagg_status gbagg_parquet_decode_range(gbagg_parquet *p, uint32_t rg0, uint32_t
         HIPCHK(hipStreamSynchronize(p->stream));  // the tables' host copies end with this call
         uint32_t bad = 0;
         HIPCHK(hipMemcpy(&bad, p->bad.ptr, 4, hipMemcpyDeviceToHost));
-        if (bad) fail(GBAGG_INVALID, "parquet: dictionary index outside the dictionary");
+        if (bad & 1) fail(GBAGG_INVALID, "parquet: dictionary index outside the dictionary");
+        if (bad & 2) fail(GBAGG_INVALID, "parquet: truncated BYTE_ARRAY value");
         if (rows) {
             uint64_t n = 0;
             for (uint32_t g = rg0; g < rg1; g++) n += (uint64_t)p->rgs[g].num_rows;

@@ -6,8 +6,11 @@
 //
 // Scope: flat schemas (REQUIRED or OPTIONAL top-level columns), uncompressed
 // column chunks, INT32 / INT64 / DOUBLE / BYTE_ARRAY, data pages V1 and V2,
-// PLAIN and PLAIN_DICTIONARY / RLE_DICTIONARY encodings (BYTE_ARRAY only
-// dictionary-encoded). The reference reader's page decoding is
+// PLAIN and PLAIN_DICTIONARY / RLE_DICTIONARY encodings. A BYTE_ARRAY column
+// whose pages are all dictionary-encoded is delivered as dictionary codes;
+// one with a PLAIN page (a writer's fallback once its dictionary grows too
+// large) is delivered as its strings (offsets + bytes, GBAGG_STR_RAW), which
+// the operator interns. The reference reader's page decoding is
 // src/storage/parquet.cpp:1480-1660; the formats are the Parquet format
 // specification's (Thrift compact protocol metadata, RLE / bit-packed hybrid
 // levels and indices).
@@ -170,6 +173,150 @@ __global__ __launch_bounds__(PQ_SCAN_BLOCK) void k_pq_scatter(const uint8_t *__r
     for (uint32_t w = 0; w < wv; w++) before += s_w[w];
     before += (uint32_t)__popcll(m & ((1ull << lane) - 1));
     if (i < n) out[i] = (v && before < ndense) ? dense[before] : (V)0;
+}
+
+// ---- BYTE_ARRAY columns with PLAIN pages: delivered as strings ------------
+// (offsets + bytes, GBAGG_STR_RAW). Per value: its length and source address,
+// from a PLAIN page (length-prefixed strings) or from the merged dictionary;
+// then offsets by a scan of the lengths, and the bytes packed one after another.
+
+// One PLAIN BYTE_ARRAY page of a decode call.
+struct PqStrPage {
+    uint64_t src, end;    // the page's values: 4-byte little-endian length, then the bytes
+    uint64_t lens, srcs;  // the column's per-value lengths (u32) and source addresses (u64)
+    uint32_t dense0, count;
+};
+
+// One thread per page follows its length prefixes (a sequential chain);
+// a prefix or string past the page end sets bit 2 of *bad.
+__global__ __launch_bounds__(64) void k_pq_str_plain(const PqStrPage *__restrict__ pages, uint32_t n, uint32_t *bad) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const PqStrPage pg = pages[i];
+    const uint8_t *p = (const uint8_t *)pg.src, *end = (const uint8_t *)pg.end;
+    uint32_t *lens = (uint32_t *)pg.lens;
+    uint64_t *srcs = (uint64_t *)pg.srcs;
+    for (uint32_t j = 0; j < pg.count; j++) {
+        bool ok = end - p >= 4;
+        uint32_t len = 0;
+        if (ok) {
+            len = (uint32_t)p[0] | (uint32_t)p[1] << 8 | (uint32_t)p[2] << 16 | (uint32_t)p[3] << 24;
+            p += 4;
+            ok = len <= (uint64_t)(end - p);
+        }
+        if (!ok) {
+            __hip_atomic_fetch_or(bad, 2u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            for (; j < pg.count; j++) {
+                lens[pg.dense0 + j] = 0;
+                srcs[pg.dense0 + j] = pg.src;
+            }
+            return;
+        }
+        lens[pg.dense0 + j] = len;
+        srcs[pg.dense0 + j] = (uint64_t)p;
+        p += len;
+    }
+}
+
+// Values of dictionary-encoded pages of such a column: their strings in the
+// merged dictionary (its device copy).
+struct PqStrDict {
+    uint64_t ids;            // the column's dense merged dictionary ids (u32)
+    uint64_t doff, dbytes;   // the merged dictionary: offsets (u32) and bytes
+    uint64_t lens, srcs;
+    uint64_t gstart;         // first value in the call's concatenation of such values
+    uint32_t dense0, count;
+};
+
+__global__ __launch_bounds__(256) void k_pq_str_dict(const PqStrDict *__restrict__ segs, uint32_t nseg,
+                                                     uint64_t total) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        uint32_t lo = 0, hi = nseg - 1;
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi + 1) / 2;
+            if (segs[mid].gstart <= i) lo = mid;
+            else hi = mid - 1;
+        }
+        const PqStrDict sg = segs[lo];
+        const uint64_t d = sg.dense0 + (i - sg.gstart);
+        const uint32_t id = ((const uint32_t *)sg.ids)[d];
+        const uint32_t *off = (const uint32_t *)sg.doff;
+        ((uint32_t *)sg.lens)[d] = off[id + 1] - off[id];
+        ((uint64_t *)sg.srcs)[d] = sg.dbytes + off[id];
+    }
+}
+
+// Sum of the lengths per 1024-value block.
+__global__ __launch_bounds__(PQ_SCAN_BLOCK) void k_pq_lsum(const uint32_t *__restrict__ len, uint32_t n,
+                                                          uint32_t *__restrict__ bsum) {
+    __shared__ uint32_t s_w[PQ_SCAN_BLOCK / 64];
+    const uint32_t i = blockIdx.x * PQ_SCAN_BLOCK + threadIdx.x;
+    uint32_t x = i < n ? len[i] : 0;
+    for (int d = 32; d >= 1; d >>= 1) x += __shfl_down(x, d, 64);
+    if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (uint32_t w = 0; w < PQ_SCAN_BLOCK / 64; w++) t += s_w[w];
+        bsum[blockIdx.x] = t;
+    }
+}
+
+// off[i] = lengths before value i (bsum: exclusive block prefixes); off[n] = total.
+__global__ __launch_bounds__(PQ_SCAN_BLOCK) void k_pq_lscan(const uint32_t *__restrict__ len, uint32_t n,
+                                                           const uint32_t *__restrict__ bsum,
+                                                           uint32_t *__restrict__ off) {
+    __shared__ uint32_t s_w[PQ_SCAN_BLOCK / 64];
+    const uint32_t i = blockIdx.x * PQ_SCAN_BLOCK + threadIdx.x;
+    const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const uint32_t v = i < n ? len[i] : 0;
+    uint32_t x = v;
+    for (uint32_t d = 1; d < 64; d <<= 1) {
+        const uint32_t y = __shfl_up(x, d, 64);
+        if (lane >= d) x += y;
+    }
+    if (lane == 63) s_w[wv] = x;
+    __syncthreads();
+    uint32_t before = bsum[blockIdx.x];
+    for (uint32_t w = 0; w < wv; w++) before += s_w[w];
+    if (i < n) {
+        off[i] = before + x - v;
+        if (i == n - 1) off[n] = before + x;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_pq_str_copy(const uint64_t *__restrict__ srcs, const uint32_t *__restrict__ len,
+                                                     const uint32_t *__restrict__ off, uint32_t n, uint8_t *out) {
+    for (uint64_t d = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; d < n; d += (uint64_t)gridDim.x * blockDim.x) {
+        const uint8_t *s = (const uint8_t *)srcs[d];
+        uint8_t *o = out + off[d];
+        const uint32_t l = len[d];
+        for (uint32_t k = 0; k < l; k++) o[k] = s[k];
+    }
+}
+
+// OPTIONAL string columns with NULLs: row offsets from the dense offsets
+// (a NULL row is an empty string); bsum holds the exclusive valid-row counts
+// of the 1024-row blocks.
+__global__ __launch_bounds__(PQ_SCAN_BLOCK) void k_pq_rowoff(const uint8_t *__restrict__ valid, uint32_t n,
+                                                            const uint32_t *__restrict__ bsum,
+                                                            const uint32_t *__restrict__ doff, uint32_t ndense,
+                                                            uint32_t *__restrict__ roff) {
+    __shared__ uint32_t s_w[PQ_SCAN_BLOCK / 64];
+    const uint32_t i = blockIdx.x * PQ_SCAN_BLOCK + threadIdx.x;
+    const bool v = i < n && valid[i] != 0;
+    const uint64_t m = __ballot(v);
+    const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) s_w[wv] = (uint32_t)__popcll(m);
+    __syncthreads();
+    uint32_t before = bsum[blockIdx.x];
+    for (uint32_t w = 0; w < wv; w++) before += s_w[w];
+    before += (uint32_t)__popcll(m & ((1ull << lane) - 1));
+    if (i < n) {
+        roff[i] = doff[min(before, ndense)];
+        if (i == n - 1) roff[n] = doff[ndense];
+    }
 }
 
 }  // namespace gbagg
@@ -362,6 +509,10 @@ struct gbagg_parquet {
     // kept until close so that an operator can consume several row groups
     struct Out {
         uint8_t *data = nullptr, *dense = nullptr, *valid = nullptr, *idx = nullptr, *bsum = nullptr;  // in the arena
+        // BYTE_ARRAY delivered as strings: per-value lengths and sources,
+        // offsets (dense, then per row), packed bytes, the merged dictionary
+        uint8_t *slen = nullptr, *ssrc = nullptr, *soff = nullptr, *sbsum = nullptr, *sbytes = nullptr;
+        uint8_t *roff = nullptr, *ddoff = nullptr, *ddbytes = nullptr;
         std::vector<uint32_t> remap;  // VARCHAR: host side of the remap tables' async copy
         std::vector<uint32_t> dict_off;
         std::vector<char> dict_bytes;
@@ -506,6 +657,16 @@ struct PqCall {
         uint32_t nrows, dense, width;
     };
     std::vector<Nulls> nulls;  // OPTIONAL columns with a NULL in this chunk
+    // BYTE_ARRAY columns delivered as strings
+    std::vector<PqStrPage> spages;
+    std::vector<PqStrDict> sdict;
+    uint64_t nsdict = 0;
+    struct Strs {
+        gbagg_parquet::Out *o;
+        uint32_t rows, dense;
+        bool nulls;
+    };
+    std::vector<Strs> strs;
 };
 
 // Host pass over column c of row groups [g0, g1), whose chunk bytes are on
@@ -529,6 +690,9 @@ void parquet_plan_column(gbagg_parquet *pq, uint32_t g0, uint32_t g1, uint32_t c
     std::vector<PqSeg> &segs = call.segs;
     const size_t irun0 = call.runs.size(), seg0 = call.segs.size();
     std::vector<uint32_t> remap;            // VARCHAR: dictionary code -> merged id, row group after row group
+    std::vector<PqStrPage> plain_pages;     // VARCHAR: PLAIN pages (the column is then delivered as strings)
+    std::vector<PqStrDict> dict_pages;      // VARCHAR: dictionary-encoded pages, for that case
+    uint64_t plain_bytes = 0, dict_vals = 0;
     std::unordered_map<std::string, uint32_t> ids;
     o.dict_off.assign(1, 0);
     o.dict_bytes.clear();
@@ -621,8 +785,12 @@ void parquet_plan_column(gbagg_parquet *pq, uint32_t g0, uint32_t g1, uint32_t c
                     fail(GBAGG_INVALID, "parquet: truncated levels");
                 vals = body + pg.v2_def_len + pg.v2_rep_len;
             }
-            if (pg.encoding == 0) {  // PLAIN
-                if (varchar) fail(GBAGG_INVALID, "parquet: BYTE_ARRAY pages must be dictionary-encoded");
+            if (pg.encoding == 0 && varchar) {  // PLAIN strings: walked on the device
+                if (nvalid)
+                    plain_pages.push_back(PqStrPage{(uint64_t)(dbuf + (vals - c0)), (uint64_t)(dbuf + (bend - c0)), 0, 0,
+                                                    (uint32_t)dense, nvalid});
+                plain_bytes += (uint64_t)(bend - vals);
+            } else if (pg.encoding == 0) {  // PLAIN
                 if ((uint64_t)nvalid * width > (uint64_t)(bend - vals)) fail(GBAGG_INVALID, "parquet: truncated values");
                 if (nvalid) segs.push_back(PqSeg{(uint64_t)(dbuf + (vals - c0)), 0, dense, 0, nvalid, 0, width, 0});
             } else if (pg.encoding == 2 || pg.encoding == 8) {  // PLAIN_DICTIONARY / RLE_DICTIONARY
@@ -639,6 +807,10 @@ void parquet_plan_column(gbagg_parquet *pq, uint32_t g0, uint32_t g1, uint32_t c
                         if (idx_runs[k].bw) idx_runs[k].src = (uint64_t)(dbuf + idx_runs[k].src);
                     // VARCHAR: src is the row group's first remap entry, made an address below
                     segs.push_back(PqSeg{dict_src, dense, dense, 0, nvalid, dict_n, width, varchar ? 1u : 0u});
+                    if (varchar) {
+                        dict_pages.push_back(PqStrDict{0, 0, 0, 0, 0, 0, (uint32_t)dense, nvalid});
+                        dict_vals += nvalid;
+                    }
                 }
             } else {
                 fail(GBAGG_INVALID, "parquet: value encoding not supported (PLAIN, PLAIN_DICTIONARY, RLE_DICTIONARY)");
@@ -679,6 +851,61 @@ void parquet_plan_column(gbagg_parquet *pq, uint32_t g0, uint32_t g1, uint32_t c
     }
     view = gbagg_column_view{};
     view.type = type;
+    if (varchar && !plain_pages.empty()) {
+        // strings: lengths and sources per value, offsets, packed bytes
+        uint32_t maxlen = 0;
+        for (size_t k = 0; k + 1 < o.dict_off.size(); k++) maxlen = std::max(maxlen, o.dict_off[k + 1] - o.dict_off[k]);
+        const uint64_t bound = plain_bytes + dict_vals * maxlen;
+        if (bound > 0xFFFFFFFFull) fail(GBAGG_INVALID, "parquet: more than 4 GiB of strings in one decode");
+        if (dense > (uint64_t)PQ_SCAN_BLOCK * 1024 * 64) fail(GBAGG_INVALID, "parquet: row groups too large");
+        o.slen = pq->alloc((size_t)dense * 4);
+        o.ssrc = pq->alloc((size_t)dense * 8);
+        o.soff = pq->alloc((size_t)(dense + 1) * 4);
+        o.sbsum = pq->alloc((size_t)((dense + PQ_SCAN_BLOCK - 1) / PQ_SCAN_BLOCK) * 4);
+        o.sbytes = pq->alloc((size_t)bound + 16);
+        if (!dict_pages.empty()) {
+            o.ddoff = pq->alloc(o.dict_off.size() * 4);
+            o.ddbytes = pq->alloc(o.dict_bytes.size() + 16);
+            HIPCHK(hipMemcpyAsync(o.ddoff, o.dict_off.data(), o.dict_off.size() * 4, hipMemcpyHostToDevice, pq->stream));
+            if (!o.dict_bytes.empty())
+                HIPCHK(hipMemcpyAsync(o.ddbytes, o.dict_bytes.data(), o.dict_bytes.size(), hipMemcpyHostToDevice,
+                                      pq->stream));
+        }
+        for (PqStrPage pp : plain_pages) {
+            pp.lens = (uint64_t)o.slen;
+            pp.srcs = (uint64_t)o.ssrc;
+            call.spages.push_back(pp);
+        }
+        for (PqStrDict ds : dict_pages) {
+            ds.ids = (uint64_t)o.dense;
+            ds.doff = (uint64_t)o.ddoff;
+            ds.dbytes = (uint64_t)o.ddbytes;
+            ds.lens = (uint64_t)o.slen;
+            ds.srcs = (uint64_t)o.ssrc;
+            ds.gstart = call.nsdict;
+            call.nsdict += ds.count;
+            call.sdict.push_back(ds);
+        }
+        const bool nulls = optional && dense < rows;
+        if (nulls) {
+            o.valid = pq->alloc(rows);
+            o.bsum = pq->alloc((size_t)((rows + PQ_SCAN_BLOCK - 1) / PQ_SCAN_BLOCK) * 4);
+            o.roff = pq->alloc((size_t)(rows + 1) * 4);
+            for (PqRun r : def_runs) {
+                r.dst = (uint64_t)(o.valid + r.dst);
+                r.dwidth = 1;
+                r.gstart = call.nvals;
+                call.nvals += r.count;
+                call.runs.push_back(r);
+            }
+        }
+        call.strs.push_back(PqCall::Strs{&o, (uint32_t)rows, (uint32_t)dense, nulls});
+        view.data = nulls ? o.roff : o.soff;
+        view.validity = nulls ? o.valid : nullptr;
+        view.dict_bytes = (const char *)o.sbytes;
+        view.flags = GBAGG_STR_RAW;
+        return;
+    }
     if (optional && dense < rows) {
         o.valid = pq->alloc(rows);
         o.data = pq->alloc((size_t)rows * width);
@@ -714,18 +941,22 @@ void parquet_launch(gbagg_parquet *pq, PqCall &call) {
     // both tables in one pinned staging buffer, one copy (the previous call
     // ended with a stream synchronisation, so the staging buffer is free)
     const size_t rb = call.runs.size() * sizeof(PqRun), sb = call.segs.size() * sizeof(PqSeg);
-    if (rb + sb) {
-        if (pq->htab_bytes < rb + sb) {
+    const size_t pb = call.spages.size() * sizeof(PqStrPage), db = call.sdict.size() * sizeof(PqStrDict);
+    const size_t tb = rb + sb + pb + db;
+    if (tb) {
+        if (pq->htab_bytes < tb) {
             if (pq->htab) HIPCHK(hipHostFree(pq->htab));
             pq->htab = nullptr;
             pq->htab_bytes = 0;
-            HIPCHK(hipHostMalloc((void **)&pq->htab, rb + sb, hipHostMallocDefault));
-            pq->htab_bytes = rb + sb;
+            HIPCHK(hipHostMalloc((void **)&pq->htab, tb, hipHostMallocDefault));
+            pq->htab_bytes = tb;
         }
         if (rb) std::memcpy(pq->htab, call.runs.data(), rb);
         if (sb) std::memcpy(pq->htab + rb, call.segs.data(), sb);
-        pq->dtab.ensure(rb + sb);
-        HIPCHK(hipMemcpyAsync(pq->dtab.ptr, pq->htab, rb + sb, hipMemcpyHostToDevice, s));
+        if (pb) std::memcpy(pq->htab + rb + sb, call.spages.data(), pb);
+        if (db) std::memcpy(pq->htab + rb + sb + pb, call.sdict.data(), db);
+        pq->dtab.ensure(tb);
+        HIPCHK(hipMemcpyAsync(pq->dtab.ptr, pq->htab, tb, hipMemcpyHostToDevice, s));
     }
     if (!call.runs.empty()) {
         const uint32_t grid = (uint32_t)std::min<uint64_t>((call.nvals + 255) / 256, cap);
@@ -736,6 +967,41 @@ void parquet_launch(gbagg_parquet *pq, PqCall &call) {
         const uint32_t grid = (uint32_t)std::min<uint64_t>((call.ndense + 255) / 256, cap);
         hipLaunchKernelGGL(k_pq_pages, dim3(grid), dim3(256), 0, s, (const PqSeg *)(pq->dtab.as<uint8_t>() + rb),
                            (uint32_t)call.segs.size(), call.ndense, pq->bad.as<uint32_t>());
+    }
+    if (!call.spages.empty())
+        hipLaunchKernelGGL(k_pq_str_plain, dim3((uint32_t)((call.spages.size() + 63) / 64)), dim3(64), 0, s,
+                           (const PqStrPage *)(pq->dtab.as<uint8_t>() + rb + sb), (uint32_t)call.spages.size(),
+                           pq->bad.as<uint32_t>());
+    if (!call.sdict.empty()) {
+        const uint32_t grid = (uint32_t)std::min<uint64_t>((call.nsdict + 255) / 256, cap);
+        hipLaunchKernelGGL(k_pq_str_dict, dim3(grid), dim3(256), 0, s,
+                           (const PqStrDict *)(pq->dtab.as<uint8_t>() + rb + sb + pb), (uint32_t)call.sdict.size(),
+                           call.nsdict);
+    }
+    for (const auto &st : call.strs) {
+        gbagg_parquet::Out &o = *st.o;
+        if (st.dense) {
+            const uint32_t nb = (st.dense + PQ_SCAN_BLOCK - 1) / PQ_SCAN_BLOCK;
+            hipLaunchKernelGGL(k_pq_lsum, dim3(nb), dim3(PQ_SCAN_BLOCK), 0, s, (const uint32_t *)o.slen, st.dense,
+                               (uint32_t *)o.sbsum);
+            hipLaunchKernelGGL(k_pq_scan_top, dim3(1), dim3(1024), 0, s, (uint32_t *)o.sbsum, nb);
+            hipLaunchKernelGGL(k_pq_lscan, dim3(nb), dim3(PQ_SCAN_BLOCK), 0, s, (const uint32_t *)o.slen, st.dense,
+                               (const uint32_t *)o.sbsum, (uint32_t *)o.soff);
+            const uint32_t grid = (uint32_t)std::min<uint64_t>((st.dense + 255) / 256, cap);
+            hipLaunchKernelGGL(k_pq_str_copy, dim3(grid), dim3(256), 0, s, (const uint64_t *)o.ssrc,
+                               (const uint32_t *)o.slen, (const uint32_t *)o.soff, st.dense, o.sbytes);
+        } else {
+            HIPCHK(hipMemsetAsync(o.soff, 0, 4, s));
+        }
+        if (st.nulls) {
+            const uint32_t nb = (st.rows + PQ_SCAN_BLOCK - 1) / PQ_SCAN_BLOCK;
+            if (nb > 1024u * 64) fail(GBAGG_INVALID, "parquet: row group too large");
+            hipLaunchKernelGGL(k_pq_count, dim3(nb), dim3(PQ_SCAN_BLOCK), 0, s, (const uint8_t *)o.valid, st.rows,
+                               (uint32_t *)o.bsum);
+            hipLaunchKernelGGL(k_pq_scan_top, dim3(1), dim3(1024), 0, s, (uint32_t *)o.bsum, nb);
+            hipLaunchKernelGGL(k_pq_rowoff, dim3(nb), dim3(PQ_SCAN_BLOCK), 0, s, (const uint8_t *)o.valid, st.rows,
+                               (const uint32_t *)o.bsum, (const uint32_t *)o.soff, st.dense, (uint32_t *)o.roff);
+        }
     }
     for (const auto &nl : call.nulls) {
         gbagg_parquet::Out &o = *nl.o;

@@ -40,14 +40,35 @@ constexpr uint32_t STR_CAND = 0x80000000u;  // row's slot, flagged: the row lowe
 constexpr uint64_t STR_FULL_TABLE = 1, STR_FULL_DICT = 2, STR_BAD_OFFSETS = 4;
 enum { SC_CODES = 0, SC_WORDS, SC_UNRESOLVED, SC_FLAGS, SC_ROWS, SC_BYTES, SC_N };
 
-// Rows [row0, row0 + n) of the column's concatenation: one batch.
+// Rows [row0, row0 + n) of the column's concatenation: one batch, given as
+// strings (off: n + 1 ascending offsets into bytes), or as dictionary codes
+// (dcodes) whose strings are dictionary entries off/bytes (through remap).
 struct StrSeg {
-    const uint32_t *off;   // n + 1 ascending offsets into bytes
+    const uint32_t *off;
     const uint8_t *bytes;
-    const uint8_t *valid;  // one byte per row, 0 = NULL; null: all valid
+    const uint8_t *valid;    // one byte per row, 0 = NULL; null: all valid
+    const uint32_t *dcodes;  // null: a batch of strings
+    const uint32_t *remap;   // batch code -> dictionary entry; null: identity
     uint64_t row0;
     uint64_t n;
+    uint32_t dict_n;         // batch codes >= dict_n drop their row
+    uint32_t pad;
 };
+
+constexpr uint32_t STR_DROP = 0xFFFFFFFFu;  // output code of a dropped row (>= any dictionary size)
+
+// The string of row r of a segment (a valid row with an in-range code).
+__device__ __forceinline__ void str_range(const StrSeg &s, uint64_t r, uint32_t &o0, uint32_t &o1) {
+    if (s.dcodes) {
+        const uint32_t c = gld(&s.dcodes[r]);
+        const uint32_t e = s.remap ? gld(&s.remap[c]) : c;
+        o0 = gld(&s.off[e]);
+        o1 = gld(&s.off[e + 1]);
+    } else {
+        o0 = gld(&s.off[r]);
+        o1 = gld(&s.off[r + 1]);
+    }
+}
 
 // One slot of the table: a probe reads one 16-byte record.
 struct StrSlot {
@@ -144,7 +165,8 @@ struct StrRow {
 __device__ __forceinline__ StrRow str_row(const StrSeg *segs, uint32_t nseg, uint64_t g) {
     const StrSeg s = segs[str_seg(segs, nseg, g)];
     const uint64_t r = g - s.row0;
-    const uint32_t o0 = gld(&s.off[r]), o1 = gld(&s.off[r + 1]);
+    uint32_t o0, o1;
+    str_range(s, r, o0, o1);
     StrRow out;
     out.valid = !s.valid || gld(&s.valid[r]) != 0;
     out.p = s.bytes + o0;
@@ -181,13 +203,26 @@ __global__ __launch_bounds__(256) void k_str_hash(const StrSeg *segs, uint32_t n
          g += (uint64_t)gridDim.x * blockDim.x) {
         const StrSeg s = segs[str_seg(segs, nseg, g)];
         const uint64_t r = g - s.row0;
-        const uint32_t o0 = gld(&s.off[r]), o1 = gld(&s.off[r + 1]);
         const bool valid = !s.valid || gld(&s.valid[r]) != 0;
-        if (o1 < o0) bad = true;
-        if (!valid || o1 < o0) {
+        // a dictionary code outside its dictionary drops the row, as it does
+        // for dictionary-coded keys
+        const bool drop = valid && s.dcodes && gld(&s.dcodes[r]) >= s.dict_n;
+        uint32_t o0 = 0, o1 = 0;
+        bool ok = true;
+        if (!s.dcodes) {
+            // every row's offsets ascend and stay within the batch's bytes
+            // (offsets[n]); a row outside them is never read
+            o0 = gld(&s.off[r]);
+            o1 = gld(&s.off[r + 1]);
+            ok = o0 <= o1 && o1 <= gld(&s.off[s.n]);
+            if (!ok) bad = true;
+        } else if (valid && !drop) {
+            str_range(s, r, o0, o1);
+        }
+        if (!valid || drop || !ok) {
             hash[g] = 0;
             pos[g] = STR_NONE;
-            codes[g] = 0;
+            codes[g] = drop ? STR_DROP : 0;
             continue;
         }
         const uint32_t len = o1 - o0;

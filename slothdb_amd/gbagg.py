@@ -616,12 +616,33 @@ class DecodedColumn:
         self.data = view.data
         self.validity = view.validity
         self.dictionary = None
-        if view.type == VARCHAR:
+        self.raw = None  # BYTE_ARRAY with PLAIN pages: data = offsets, raw = bytes (device)
+        if view.type == VARCHAR and view.flags & STR_RAW:
+            self.raw = view.dict_bytes
+        elif view.type == VARCHAR:
             k = view.dict_size
             off = np.ctypeslib.as_array(C.cast(view.dict_offsets, C.POINTER(C.c_uint32)), shape=(k + 1,)).copy()
             nb = max(int(off[-1]), 1)
             data = np.ctypeslib.as_array(C.cast(view.dict_bytes, C.POINTER(C.c_uint8)), shape=(nb,)).copy()
             self.dictionary = Dictionary(offsets=off, data=data)
+
+    def strings(self):
+        """A string column's rows as str (None for NULL), copied back to the host."""
+        valid = np.ones(self.n, bool)
+        if self.validity:
+            v = np.empty(self.n, np.uint8)
+            if self.n:
+                _check(lib().gbagg_memcpy_d2h(v.ctypes.data, self.validity, self.n))
+            valid = v.astype(bool)
+        if self.raw is None:
+            vals, _ = self.to_host()
+            return [self.dictionary.get(int(x)) if ok else None for x, ok in zip(vals, valid)]
+        off = np.empty(self.n + 1, np.uint32)
+        _check(lib().gbagg_memcpy_d2h(off.ctypes.data, self.data, off.nbytes))
+        data = np.empty(max(1, int(off[-1])), np.uint8)
+        if off[-1]:
+            _check(lib().gbagg_memcpy_d2h(data.ctypes.data, self.raw, int(off[-1])))
+        return [bytes(data[off[i]:off[i + 1]]).decode() if valid[i] else None for i in range(self.n)]
 
     def to_host(self):
         """(values, validity) copied back to host numpy arrays."""

@@ -1,7 +1,8 @@
 """Parquet row-group decode on the GPU (gbagg_parquet_*) against pyarrow's own
 reading of the same files (pyarrow writes them: uncompressed, dictionary and
-PLAIN pages, data pages V1 and V2, NULLs, several pages and row groups), and
-the decoded row groups fed straight into the operator against the oracle."""
+PLAIN pages, data pages V1 and V2, NULLs, several pages and row groups, string
+columns whose dictionary overflows into PLAIN pages), and the decoded row
+groups fed straight into the operator against the oracle."""
 import io
 
 import numpy as np
@@ -28,9 +29,12 @@ def _table(n, seed):
         "k": pa.array(i64, mask=rng.random(n) < 0.1),
         "d": pa.array(rng.normal(100, 25, n), mask=rng.random(n) < 0.2),
         "u": pa.array(rng.integers(-2**62, 2**62, n)),  # all distinct: PLAIN after the dictionary limit
+        # many distinct strings: dictionary pages, then PLAIN pages once the dictionary is full
+        "s": pa.array([f"s{v:07d}" + "x" * int(v % 23) for v in rng.integers(0, 10**6, n)],
+                      mask=rng.random(n) < 0.03),
     }, schema=pa.schema([pa.field("a", pa.int32(), nullable=False), ("b", pa.string()),
                          pa.field("c", pa.int32(), nullable=False), ("k", pa.int64()), ("d", pa.float64()),
-                         pa.field("u", pa.int64(), nullable=False)]))
+                         pa.field("u", pa.int64(), nullable=False), ("s", pa.string())]))
 
 
 def _write(t, **kw):
@@ -61,11 +65,10 @@ def _check_file(image, t, rg_rows, resident=False):
         cols = f.decode(rg)
         for c, dc in enumerate(cols):
             exp = _expected(c, t.column(c).slice(row, n))
-            vals, valid = dc.to_host()
             if dc.type == G.VARCHAR:
-                got = [dc.dictionary.get(int(v)) if (valid is None or valid[i]) else None for i, v in enumerate(vals)]
-                assert got == exp, f"column {c} row group {rg}"
+                assert dc.strings() == exp, f"column {c} row group {rg}"
             else:
+                vals, valid = dc.to_host()
                 ev, evalid = exp
                 if valid is None:
                     assert evalid.all()
@@ -78,12 +81,12 @@ def _check_file(image, t, rg_rows, resident=False):
     cols = f.decode(0, last=f.num_row_groups)
     for c, dc in enumerate(cols):
         exp = _expected(c, t.column(c))
-        vals, valid = dc.to_host()
         if dc.type == G.VARCHAR:
-            assert len(set(dc.dictionary.get(i) for i in range(len(dc.dictionary)))) == len(dc.dictionary)
-            got = [dc.dictionary.get(int(v)) if (valid is None or valid[i]) else None for i, v in enumerate(vals)]
-            assert got == exp, f"column {c}, all row groups"
+            if dc.dictionary is not None:
+                assert len(set(dc.dictionary.get(i) for i in range(len(dc.dictionary)))) == len(dc.dictionary)
+            assert dc.strings() == exp, f"column {c}, all row groups"
         else:
+            vals, valid = dc.to_host()
             ev, evalid = exp
             valid = np.ones(t.num_rows, bool) if valid is None else valid.astype(bool)
             assert np.array_equal(valid, evalid) and np.array_equal(vals[evalid], ev[evalid].astype(vals.dtype))
@@ -94,14 +97,23 @@ def _check_file(image, t, rg_rows, resident=False):
 def test_decode_matches_pyarrow(version, resident):
     t = _table(300_000, 1)
     image = _write(t, row_group_size=122_880, data_page_size=32 * 1024, data_page_version=version,
-                   dictionary_pagesize_limit=64 * 1024, use_dictionary=["a", "b", "k", "d", "u"])
+                   dictionary_pagesize_limit=64 * 1024, use_dictionary=["a", "b", "k", "d", "u", "s"])
     _check_file(image, t, 122_880, resident)
+    # s overflows its dictionary: delivered as strings; b stays dictionary codes
+    f = G.ParquetFile(image)
+    cols = f.decode(0)
+    assert cols[6].raw is not None and cols[1].raw is None
+    f.close()
 
 
 def test_plain_only_and_single_row_group():
-    t = _table(50_000, 2).drop_columns(["b"])
+    t = _table(50_000, 2)
     image = _write(t, row_group_size=1_000_000, use_dictionary=False)
     _check_file(image, t, 1_000_000)
+    f = G.ParquetFile(image)
+    cols = f.decode(0)
+    assert cols[1].raw is not None and cols[6].raw is not None
+    f.close()
 
 
 def test_compressed_chunks_are_rejected():
@@ -154,5 +166,20 @@ def test_parquet_row_groups_into_the_operator():
                 op.consume_device(f.decode(rg))
         op.finalize()
         assert_same_groups(op.result().rows(), exp, 2, ty)
+        op.close()
+    # GROUP BY s, whose row groups arrive as strings (PLAIN pages after the
+    # dictionary overflows): interned by the operator
+    q = G.Query(types, [6], [(G.COUNT_STAR, None), (G.SUM, 2), (G.MAX, 3)], nullable=nullable)
+    c_, v_, ty, ov = O.groupby_exact(q, cols)
+    exp = oracle_rows(c_, v_, ty, {0: cols[6].dictionary})
+    for whole in (False, True):
+        op = G.GpuHashAggregate(q)
+        op.Init()
+        batches = [f.decode(0, last=f.num_row_groups)] if whole else [f.decode(rg) for rg in range(f.num_row_groups)]
+        assert any(b[6].raw is not None for b in batches)
+        for b in batches:
+            op.consume_device(b)
+        op.finalize()
+        assert_same_groups(op.result().rows(), exp, 1, ty)
         op.close()
     f.close()

@@ -55,6 +55,20 @@ def _as_dict(col):
     return G.Column(G.VARCHAR, codes, col.validity, dictionary=d)
 
 
+def _as_shuffled_dict(col, seed):
+    """The strings of a batch as codes into a dictionary of its own (shuffled,
+    one entry repeated, so not unique): a batch from another row group."""
+    n = len(col)
+    valid = np.ones(n, bool) if col.validity is None else col.validity.astype(bool)
+    strs = [col.string(i) if valid[i] else None for i in range(n)]
+    uniq = sorted({x for x in strs if x is not None})
+    np.random.default_rng(seed).shuffle(uniq)
+    entries = uniq + uniq[:1]
+    pos = {x: i for i, x in enumerate(uniq)}
+    codes = np.array([pos[x] if x is not None else 7 for x in strs], np.uint32)
+    return G.Column(G.VARCHAR, codes, col.validity, dictionary=G.Dictionary(entries or [""]))
+
+
 def _expected(q, cols):
     dcols = [_as_dict(c) if c.raw is not None else c for c in cols]
     dicts = {g: dcols[c].dictionary for g, c in enumerate(q.group_cols) if dcols[c].dictionary is not None}
@@ -135,6 +149,14 @@ def test_int_and_string_key_with_nulls_and_batches(strategy):
         print(f"[strings] composite strategy={strategy} device={device}", flush=True)
         got, _ = _run(q, _split(cols, 4), device)
         assert_same_groups(got, exp, 2, t)
+    # batches 1 and 3 as dictionary codes (each its own dictionary): the
+    # column is interned across both forms
+    parts = _split(cols, 4)
+    for i in (1, 3):
+        parts[i][1] = _as_shuffled_dict(parts[i][1], i)
+    print(f"[strings] composite strategy={strategy} mixed forms", flush=True)
+    got, _ = _run(q, parts, True)
+    assert_same_groups(got, exp, 2, t)
 
 
 def test_forced_hash_collisions():
@@ -206,14 +228,15 @@ def test_empty_null_and_invalid_batches():
     cols = [G.Column.strings(["", "x", "", None]), G.Column(G.INTEGER, np.array([1, 2, 3, 4], np.int32))]
     got, _ = _run(q, [e, cols, e], False)
     assert sorted(got, key=lambda r: (r[0] is not None, r[0] or "")) == [(None, 1, 4), ("", 2, 4), ("x", 1, 2)]
-    # strings and dictionary codes for one column: refused
-    d = G.Dictionary(["x"], unique=True)
-    op = G.GpuHashAggregate(q)
-    op.Init()
-    op.consume(cols)
-    with pytest.raises(G.GbaggError, match="all be strings or all dictionary codes"):
-        op.consume([G.Column(G.VARCHAR, np.zeros(4, np.uint32), np.ones(4, np.uint8), dictionary=d), cols[1]])
-    op.close()
+    # strings in one batch, dictionary codes in the next: one set of groups;
+    # a code outside its dictionary drops its row, as for dictionary keys
+    d = G.Dictionary(["x", "y"], unique=True)
+    coded = [G.Column(G.VARCHAR, np.array([0, 1, 0, 5], np.uint32), np.array([1, 1, 0, 1], np.uint8), dictionary=d),
+             G.Column(G.INTEGER, np.array([10, 20, 30, 40], np.int32))]
+    for device in (False, True):
+        got, _ = _run(q, [cols, coded], device)
+        assert sorted(got, key=lambda r: (r[0] is not None, r[0] or "")) == \
+            [(None, 2, 34), ("", 2, 4), ("x", 2, 12), ("y", 1, 20)]
     # descending offsets: refused on the host (host batch) and by the hash kernel (device batch)
     bad = G.Column(G.VARCHAR, np.array([0, 3, 1, 4], np.uint32), raw=np.frombuffer(b"abcd", np.uint8))
     vals = G.Column(G.INTEGER, np.zeros(3, np.int32))
