@@ -97,8 +97,10 @@ typedef struct {
  * distinct string of the column on the GPU (exact byte comparison) and groups,
  * filters and counts by its code, as for a dictionary-encoded column; this
  * replaces the reference's non-dictionary VARCHAR group-key path
- * (RadixHashCountStr, src/execution/radix_count_agg.cpp:390-761). All batches
- * of one column use the same form. */
+ * (RadixHashCountStr, src/execution/radix_count_agg.cpp:390-761). Batches of
+ * one column may mix the two forms (row groups whose dictionary fitted next to
+ * row groups that fell back to PLAIN strings): a group column is then interned
+ * over all its batches, the coded ones read through their dictionaries. */
 #define GBAGG_STR_RAW 2u
 
 /* Aggregate functions on the path (agg_emit_helpers.cpp:8-98; return types
@@ -412,10 +414,13 @@ gbagg_status gbagg_selector_infer(const float *params, uint32_t num_params, cons
  * per row group and column. Here the decoded columns land in HBM as
  * gbagg_column_views ready for gbagg_consume_device: INTEGER / BIGINT / DOUBLE
  * values, VARCHAR as dictionary codes with the row group's dictionary (host
- * memory), validity bytes for OPTIONAL columns.
+ * memory) when all its pages are dictionary-encoded, else (a PLAIN page, as
+ * writers emit once a dictionary grows too large) as its strings
+ * (GBAGG_STR_RAW: device offsets and bytes), validity bytes for OPTIONAL
+ * columns.
  * Supported: flat schemas; uncompressed column chunks; INT32, INT64, DOUBLE,
  * BYTE_ARRAY; data pages V1 and V2; PLAIN, PLAIN_DICTIONARY and
- * RLE_DICTIONARY (BYTE_ARRAY dictionary-encoded only). Anything else is
+ * RLE_DICTIONARY. Anything else is
  * GBAGG_INVALID with a message. The file image must outlive the handle; every
  * decoded column stays in HBM until close, so one operator can consume many
  * row groups before its finalize. */
