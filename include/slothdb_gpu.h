@@ -36,7 +36,7 @@ extern "C" {
 
 /* Limits of one plan. */
 #define GBAGG_MAX_COLUMNS 16
-#define GBAGG_MAX_GROUP_COLS 2
+#define GBAGG_MAX_GROUP_COLS 8
 #define GBAGG_MAX_AGGS 16
 #define GBAGG_MAX_PREDICATES 8
 #define GBAGG_MAX_IN_SET 64

@@ -240,14 +240,29 @@ typedef struct gtable {
     struct gtable *dset; /* COUNT(DISTINCT): one entry per (group * 16 + agg, value) seen */
 } gtable;
 
+/* Keys hold every group column (unused ones zero). */
 static inline uint64_t gkey_hash(const gkey *k) {
-    uint64_t h = orc_hash_u64((uint64_t)k->k[0]);
-    h = (h * 0x9E3779B97F4A7C15ULL) ^ orc_hash_u64((uint64_t)k->k[1] + k->nullmask);
+    uint64_t h = orc_hash_u64((uint64_t)k->nullmask);
+    for (int g = 0; g < GBAGG_MAX_GROUP_COLS; g++) h = (h * 0x9E3779B97F4A7C15ULL) ^ orc_hash_u64((uint64_t)k->k[g]);
     return h;
 }
 
 static inline int gkey_eq(const gkey *a, const gkey *b) {
-    return a->k[0] == b->k[0] && a->k[1] == b->k[1] && a->nullmask == b->nullmask;
+    if (a->nullmask != b->nullmask) return 0;
+    for (int g = 0; g < GBAGG_MAX_GROUP_COLS; g++)
+        if (a->k[g] != b->k[g]) return 0;
+    return 1;
+}
+
+/* A DOUBLE group key as the bits of its canonical value: SQL groups -0.0
+ * with 0.0 and every NaN together. */
+static inline int64_t double_key(double d) {
+    uint64_t b;
+    memcpy(&b, &d, 8);
+    if ((b << 1) == 0) b = 0;  /* -0.0 */
+    if ((b & 0x7FF0000000000000ULL) == 0x7FF0000000000000ULL && (b & 0x000FFFFFFFFFFFFFULL))
+        b = 0x7FF8000000000000ULL; /* NaN */
+    return (int64_t)b;
 }
 
 static void gtable_init(gtable *t, uint32_t na) {
@@ -333,8 +348,7 @@ typedef struct {
 /* Builds the group key of row r; returns 0 if the row is dropped. */
 static inline int row_key(const exact_ctx *x, uint64_t r, gkey *k) {
     const gbagg_query_desc *q = x->q;
-    k->k[0] = k->k[1] = 0;
-    k->nullmask = 0;
+    memset(k, 0, sizeof(*k));
     for (uint32_t g = 0; g < q->num_group_cols; g++) {
         const gbagg_column_view *c = &x->cols[q->group_cols[g]];
         if (!col_valid(c, r)) {
@@ -342,7 +356,7 @@ static inline int row_key(const exact_ctx *x, uint64_t r, gkey *k) {
             k->nullmask |= (uint8_t)(1u << g);
             continue;
         }
-        int64_t v = col_i64(c, r);
+        int64_t v = c->type == GBAGG_TYPE_DOUBLE ? double_key(((const double *)c->data)[r]) : col_i64(c, r);
         if (x->canon[g]) v = x->canon[g][(uint32_t)v];
         k->k[g] = v;
     }
@@ -360,6 +374,7 @@ static inline void row_update(const exact_ctx *x, gtable *t, uint64_t g, uint64_
         if (ag->kind == GBAGG_AGG_COUNT_DISTINCT) {
             /* distinct non-NULL values per group (physical_planner.cpp:12325-12330) */
             gkey dk;
+            memset(&dk, 0, sizeof(dk));
             int64_t v = col_i64(c, r);
             if (x->acanon[a]) v = x->acanon[a][(uint32_t)v];
             dk.k[0] = (int64_t)(g * 16 + a);

@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 REF_PATH = os.path.join(HERE, "_ref", "libslothref.so")
 
-MAX_OUT = 18
+MAX_OUT = 24  # ORC_MAX_OUT: 8 group columns + 16 aggregates
 INTEGER, BIGINT, DOUBLE, VARCHAR = 5, 6, 12, 15
 
 

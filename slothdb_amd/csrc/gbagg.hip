@@ -269,6 +269,17 @@ struct Batch {
     uint64_t nrows = 0;
     const uint32_t *str_off[MAXU] = {};  // GBAGG_STR_RAW columns: device offsets and bytes
     const uint8_t *str_bytes[MAXU] = {};
+    // multi-column keys: the batch's group columns (device), and for VARCHAR
+    // columns given as codes a host copy of the batch's dictionary
+    struct TupleIn {
+        const void *data = nullptr;
+        const uint8_t *valid = nullptr;
+        const uint32_t *str_off = nullptr;
+        const uint8_t *str_bytes = nullptr;
+        std::vector<uint32_t> doff;
+        std::vector<char> dbytes;
+    };
+    std::vector<TupleIn> tin;
     std::vector<std::unique_ptr<DevBuf>> owned;  // device copies / remaps / match tables
 };
 
@@ -308,6 +319,19 @@ struct gbagg_handle {
     std::vector<uint8_t> str_mode;  // per column: 0 no batch yet, 1 dictionary codes, 2 strings
     std::vector<std::unique_ptr<StrCol>> strs;
     uint64_t str_key_mask = ~0ull;  // stored string-hash bits ("str_hash_bits")
+    // multi-column keys (more than two columns, a DOUBLE column, or a nullable
+    // pair without drop_null_keys): the plan groups by one hidden column, the
+    // code of each row's interned key tuple; results unpack the tuples
+    bool tuple = false;
+    uint32_t ext_cols = 0;              // the caller's columns; the hidden one is column ext_cols
+    std::vector<uint32_t> tcols;        // the caller's group columns
+    std::vector<gbagg_type> ttypes;
+    std::vector<uint32_t> twordof;      // first word of each column's value in a packed row
+    uint32_t twords = 0;                // words per packed row
+    int32_t tdrop = 0;                  // the caller's drop_null_keys
+    std::vector<std::unique_ptr<StrCol>> tstr;  // VARCHAR group columns' interning
+    std::unique_ptr<StrCol> tsc;        // the tuples' interning
+    DevBuf tpack, tkeep, tcomp;
     // TopN on the device
     bool topn_device = false;
     DevTopn tspec{};
@@ -393,11 +417,30 @@ int add_acc(gbagg_handle *h, int op, int u) {
 void build_distinct(gbagg_handle *h, const gbagg_query_desc *q);
 void pick_kernels(gbagg_handle *h);
 
+// Group keys that the one-word and 16-byte keys cannot hold: more than two
+// columns, a DOUBLE column, or two columns with NULLs that form groups.
+bool needs_tuple(const gbagg_query_desc *q) {
+    if (!q->column_types || !q->group_cols || q->num_group_cols == 0 || q->num_columns > GBAGG_MAX_COLUMNS)
+        return false;
+    if (q->num_group_cols > 2) return true;
+    bool nullable = false;
+    for (uint32_t g = 0; g < q->num_group_cols; g++) {
+        const uint32_t c = q->group_cols[g];
+        if (c >= q->num_columns) return false;
+        if (q->column_types[c] == GBAGG_TYPE_DOUBLE) return true;
+        nullable |= q->column_nullable ? q->column_nullable[c] != 0 : true;
+    }
+    return q->num_group_cols == 2 && nullable && !q->drop_null_keys;
+}
+
+void build_tuple(gbagg_handle *h, const gbagg_query_desc *q);
+
 void build_plan(gbagg_handle *h, const gbagg_query_desc *q) {
     if (!q) fail(GBAGG_INVALID, "null query");
     if (q->aggs && q->num_aggs <= GBAGG_MAX_AGGS)
         for (uint32_t a = 0; a < q->num_aggs; a++)
             if (q->aggs[a].kind == GBAGG_AGG_COUNT_DISTINCT) return build_distinct(h, q);
+    if (needs_tuple(q)) return build_tuple(h, q);
     if (q->num_columns == 0 || q->num_columns > GBAGG_MAX_COLUMNS || !q->column_types)
         fail(GBAGG_INVALID, "num_columns must be 1..16 with column_types");
     for (uint32_t c = 0; c < q->num_columns; c++) {
@@ -405,7 +448,7 @@ void build_plan(gbagg_handle *h, const gbagg_query_desc *q) {
         h->col_types.push_back(q->column_types[c]);
         h->nullable.push_back(q->column_nullable ? q->column_nullable[c] : 1);
     }
-    if (q->num_group_cols > GBAGG_MAX_GROUP_COLS) fail(GBAGG_INVALID, "at most 2 group columns");
+    if (q->num_group_cols > GBAGG_MAX_GROUP_COLS) fail(GBAGG_INVALID, "at most 8 group columns");
     if (q->num_aggs > GBAGG_MAX_AGGS) fail(GBAGG_INVALID, "at most 16 aggregates");
     if (q->num_predicates > GBAGG_MAX_PREDICATES) fail(GBAGG_INVALID, "at most 8 predicates");
     if (q->num_aggs == 0 && q->num_group_cols == 0) fail(GBAGG_INVALID, "nothing to aggregate");
@@ -609,12 +652,54 @@ void build_plan(gbagg_handle *h, const gbagg_query_desc *q) {
 //            MIN of MIN, MAX of MAX, AVG = SUM of sums / SUM of counts)
 // Rows with a NULL x must still reach the other aggregates, so a nullable x
 // allows only DISTINCT aggregates; a nullable k needs drop_null_keys.
+// Multi-column keys: the same plan over one more column, the tuple code,
+// grouped by that column alone (rows dropped through its validity).
+void build_tuple(gbagg_handle *h, const gbagg_query_desc *q) {
+    const uint32_t G = q->num_group_cols, nc = q->num_columns;
+    if (G > GBAGG_MAX_GROUP_COLS) fail(GBAGG_INVALID, "at most 8 group columns");
+    if (nc + 1 > GBAGG_MAX_COLUMNS) fail(GBAGG_INVALID, "multi-column keys take at most 15 columns");
+    if (q->topn_limit && q->topn_col >= 0 && (uint32_t)q->topn_col < G)
+        fail(GBAGG_INVALID, "TopN on a group column of a multi-column key is not supported");
+    std::vector<gbagg_type> types(q->column_types, q->column_types + nc);
+    for (gbagg_type t : types)
+        if (!valid_type(t)) fail(GBAGG_INVALID, "unsupported column type");
+    types.push_back(GBAGG_TYPE_INTEGER);
+    std::vector<uint8_t> nullable(nc + 1, 1);
+    for (uint32_t c = 0; c < nc; c++) nullable[c] = q->column_nullable ? q->column_nullable[c] : 1;
+    uint32_t code_col = nc;
+    gbagg_query_desc q2 = *q;
+    q2.num_columns = nc + 1;
+    q2.column_types = types.data();
+    q2.column_nullable = nullable.data();
+    q2.num_group_cols = 1;
+    q2.group_cols = &code_col;
+    q2.drop_null_keys = 1;
+    if (q->topn_limit && q->topn_col >= 0) q2.topn_col = q->topn_col - (int32_t)G + 1;
+    build_plan(h, &q2);
+    h->tuple = true;
+    h->ext_cols = nc;
+    h->tdrop = q->drop_null_keys;
+    uint32_t word = (G + 3) / 4;
+    for (uint32_t g = 0; g < G; g++) {
+        const uint32_t c = q->group_cols[g];
+        if (c >= nc) fail(GBAGG_INVALID, "group column out of range");
+        h->tcols.push_back(c);
+        h->ttypes.push_back(types[c]);
+        h->twordof.push_back(word);
+        word += types[c] == GBAGG_TYPE_BIGINT || types[c] == GBAGG_TYPE_DOUBLE ? 2 : 1;
+    }
+    h->twords = word;
+    h->tstr.resize(G);
+}
+
 void build_distinct(gbagg_handle *h, const gbagg_query_desc *q) {
     if (q->num_columns == 0 || q->num_columns > GBAGG_MAX_COLUMNS || !q->column_types)
         fail(GBAGG_INVALID, "num_columns must be 1..16 with column_types");
     if (q->num_group_cols > 1) fail(GBAGG_INVALID, "COUNT(DISTINCT) supports at most one group column");
     if (q->num_group_cols && (!q->group_cols || q->group_cols[0] >= q->num_columns))
         fail(GBAGG_INVALID, "group column out of range");
+    if (q->num_group_cols && q->column_types[q->group_cols[0]] == GBAGG_TYPE_DOUBLE)
+        fail(GBAGG_INVALID, "COUNT(DISTINCT) with a DOUBLE group column is not supported");
     auto nullable = [&](uint32_t c) { return q->column_nullable ? q->column_nullable[c] != 0 : true; };
     int32_t x = -1;
     bool only_distinct = true;
@@ -813,6 +898,11 @@ void add_batch(gbagg_handle *h, const gbagg_column_view *cols, uint64_t n, bool 
     if (h->str_mode.size() != h->col_types.size()) h->str_mode.assign(h->col_types.size(), 0);
     std::vector<uint8_t> modes(h->str_mode);
     for (int u = 0; u < h->nu; u++) {
+        if (h->tuple && (uint32_t)h->ucol[u] == h->ext_cols) {  // the tuple code: written at finalize
+            b.db.data[u] = nullptr;
+            b.db.valid[u] = nullptr;
+            continue;
+        }
         const gbagg_column_view &c = cols[h->ucol[u]];
         if (c.type != h->col_types[h->ucol[u]]) fail(GBAGG_INVALID, "column type differs from the plan");
         if (n && !c.data) fail(GBAGG_INVALID, "null column data");
@@ -867,6 +957,49 @@ void add_batch(gbagg_handle *h, const gbagg_column_view *cols, uint64_t n, bool 
             HIPCHK(hipMemcpyAsync(vb->ptr, c.validity, n, hipMemcpyHostToDevice, h->stream));
             b.db.valid[u] = vb->as<uint8_t>();
             b.owned.push_back(std::move(vb));
+        }
+    }
+    // multi-column keys: every group column of the batch on the device
+    if (h->tuple) {
+        auto to_dev = [&](const void *p, size_t bytes) -> const void * {
+            if (device || !n || !p) return p;
+            auto buf = std::make_unique<DevBuf>();
+            buf->ensure(bytes + 4);
+            HIPCHK(hipMemcpyAsync(buf->ptr, p, bytes, hipMemcpyHostToDevice, h->stream));
+            const void *d = buf->ptr;
+            b.owned.push_back(std::move(buf));
+            return d;
+        };
+        b.tin.resize(h->tcols.size());
+        for (size_t j = 0; j < h->tcols.size(); j++) {
+            const uint32_t col = h->tcols[j];
+            const gbagg_column_view &c = cols[col];
+            if (c.type != h->col_types[col]) fail(GBAGG_INVALID, "column type differs from the plan");
+            if (n && !c.data) fail(GBAGG_INVALID, "null column data");
+            if (c.validity && !h->nullable[col]) fail(GBAGG_INVALID, "validity given for a column declared non-nullable");
+            Batch::TupleIn &ti = b.tin[j];
+            ti.valid = (const uint8_t *)to_dev(c.validity, n);
+            if (c.type == GBAGG_TYPE_VARCHAR && (c.flags & GBAGG_STR_RAW)) {
+                if (c.dict_offsets || c.dict_size) fail(GBAGG_INVALID, "GBAGG_STR_RAW columns carry no dictionary");
+                const uint32_t *off = (const uint32_t *)c.data;
+                if (!device && n) {
+                    for (uint64_t r = 0; r < n; r++)
+                        if (off[r + 1] < off[r]) fail(GBAGG_INVALID, "VARCHAR string offsets must be ascending");
+                    if (off[n] && !c.dict_bytes) fail(GBAGG_INVALID, "null string bytes");
+                }
+                ti.str_off = (const uint32_t *)to_dev(c.data, (n + 1) * 4);
+                ti.str_bytes = (const uint8_t *)to_dev(c.dict_bytes, !device && n ? off[n] : 0);
+            } else {
+                ti.data = to_dev(c.data, n * type_width(c.type));
+                if (c.type == GBAGG_TYPE_VARCHAR) {
+                    if (c.dict_size && (!c.dict_offsets || !c.dict_bytes)) fail(GBAGG_INVALID, "missing dictionary");
+                    ti.doff.assign(1, 0);
+                    if (c.dict_size) {
+                        ti.doff.assign(c.dict_offsets, c.dict_offsets + c.dict_size + 1);
+                        ti.dbytes.assign(c.dict_bytes, c.dict_bytes + c.dict_offsets[c.dict_size]);
+                    }
+                }
+            }
         }
     }
     // VARCHAR group keys: canonical ids across batches
@@ -1567,6 +1700,109 @@ void run_phase1(gbagg_handle *h);
 // Interns one GBAGG_STR_RAW column over all its batches: codes per row (the
 // batches' column data), the per-code match tables of its VARCHAR predicates
 // and, for a group column, its dictionary on the host.
+// Interns rows [0, total) given by segs: codes per row in sc.codes, the
+// dictionary in sc (tab, ncodes, words).
+void intern_run(gbagg_handle *h, StrCol &sc, const std::vector<StrSeg> &segs, uint64_t total) {
+    const DeviceInfo &di = device_info();
+    hipStream_t s = h->stream;
+    uint64_t ctr[SC_N] = {};
+    StrTab t{};
+    sc.ncodes = 0;
+    sc.words = 0;
+    if (!total) {
+        sc.tab = t;
+        return;
+    }
+    sc.segs.ensure(segs.size() * sizeof(StrSeg));
+    HIPCHK(hipMemcpyAsync(sc.segs.ptr, segs.data(), segs.size() * sizeof(StrSeg), hipMemcpyHostToDevice, s));
+    sc.hash.ensure(total * 8);
+    sc.pos.ensure(total * 4);
+    sc.codes.ensure(total * 4);
+    sc.ctr.ensure(SC_N * 8);
+    sc.plan.ensure(sizeof(DevPlan));
+    t.ctr = sc.ctr.as<uint64_t>();
+    t.key_mask = h->str_key_mask;
+    t.seed = 0x5157C0DE5EEDull;
+    const StrSeg *dsegs = sc.segs.as<StrSeg>();
+    const uint32_t nseg = (uint32_t)segs.size();
+    const uint32_t W = (uint32_t)std::min<uint64_t>((total + 1023) / 1024, (uint64_t)di.cus * 4);
+    sc.hllp.ensure((size_t)W * HLL_M);
+    const uint32_t hslices = std::min<uint32_t>(32, W);
+    sc.hllm.ensure((size_t)HLL_M * hslices);
+    HIPCHK(hipMemsetAsync(t.ctr, 0, SC_N * 8, s));
+    HIPCHK(hipMemsetAsync(sc.plan.ptr, 0, sizeof(DevPlan), s));
+    hipLaunchKernelGGL(k_str_hash, dim3(W), dim3(256), 0, s, dsegs, nseg, total, t, sc.hash.as<uint64_t>(),
+                       sc.pos.as<uint32_t>(), sc.codes.as<uint32_t>(), sc.hllp.as<uint8_t>());
+    hipLaunchKernelGGL(k_hll_merge, dim3(HLL_M / 256, hslices), dim3(256), 0, s, (const uint8_t *)sc.hllp.ptr, W,
+                       sc.hllm.as<uint8_t>());
+    hipLaunchKernelGGL(k_hll_finish, dim3(1), dim3(1024), 0, s, (const uint8_t *)sc.hllm.ptr, hslices,
+                       sc.plan.as<DevPlan>());
+    DevPlan pl{};
+    HIPCHK(hipMemcpyAsync(ctr, t.ctr, SC_N * 8, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(&pl, sc.plan.ptr, sizeof(DevPlan), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (ctr[SC_FLAGS] & STR_BAD_OFFSETS) fail(GBAGG_INVALID, "VARCHAR string offsets must be ascending");
+    const uint64_t rows = ctr[SC_ROWS], bytes = ctr[SC_BYTES];
+    const uint64_t est = std::max<uint64_t>(1, pl.est_groups);
+    // codes: the estimate with a margin (at most one per valid row); slots
+    // at least twice the codes; arena words from the mean string length
+    uint64_t code_cap = std::max<uint64_t>(1, std::min<uint64_t>(rows, est + est / 4 + 1024));
+    uint64_t slots = next_pow2(std::max<uint64_t>(1024, 2 * code_cap));
+    const uint64_t words_max = bytes / 4 + rows + 1;
+    const uint64_t mean_words = rows ? bytes / rows / 4 + 2 : 1;
+    uint64_t arena_words = std::min<uint64_t>(words_max, code_cap * mean_words * 3 / 2 + 1024);
+    for (uint32_t attempt = 0; rows; attempt++) {
+        if (slots > (1ull << 30)) fail(GBAGG_ERROR, "string table larger than 2^30 slots");
+        if (arena_words > 0xFFFFFFFFull) fail(GBAGG_ERROR, "distinct strings exceed 16 GiB");
+        sc.slot.ensure(slots * sizeof(StrSlot));
+        sc.arena.ensure(arena_words * 4);
+        sc.code.ensure(code_cap * sizeof(StrCode));
+        t.slot = sc.slot.as<StrSlot>();
+        t.code = sc.code.as<StrCode>();
+        t.arena = sc.arena.as<uint32_t>();
+        t.mask = slots - 1;
+        t.arena_words = arena_words;
+        t.code_cap = code_cap;
+        const uint32_t ig = (uint32_t)std::min<uint64_t>((slots + 255) / 256, (uint64_t)di.cus * 8);
+        hipLaunchKernelGGL(k_str_init, dim3(ig), dim3(256), 0, s, t);
+        const uint32_t rg = (uint32_t)std::min<uint64_t>((total + 255) / 256, (uint64_t)di.cus * 8);
+        // code assignment: blocks of contiguous rows, at most 4096 of them
+        const uint64_t chunk = std::max<uint64_t>(256, ((total + 4095) / 4096 + 255) / 256 * 256);
+        const uint32_t nblk = (uint32_t)((total + chunk - 1) / chunk);
+        sc.blk.ensure((size_t)nblk * 16);
+        hipLaunchKernelGGL(k_str_rehome, dim3(rg), dim3(256), 0, s, total, (const uint64_t *)sc.hash.ptr,
+                           sc.pos.as<uint32_t>());
+        for (uint32_t round = 0;; round++) {
+            hipLaunchKernelGGL(k_str_probe, dim3(rg), dim3(256), 0, s, dsegs, nseg, total, t,
+                               (const uint64_t *)sc.hash.ptr, sc.pos.as<uint32_t>(), sc.codes.as<uint32_t>());
+            hipLaunchKernelGGL(k_str_reps, dim3(nblk), dim3(256), 0, s, dsegs, nseg, total, t,
+                               (const uint32_t *)sc.pos.ptr, chunk, sc.blk.as<uint64_t>());
+            hipLaunchKernelGGL(k_str_scan, dim3(1), dim3(1024), 0, s, sc.blk.as<uint64_t>(), nblk, t.ctr);
+            hipLaunchKernelGGL(k_str_assign, dim3(nblk), dim3(256), 0, s, dsegs, nseg, total, t,
+                               (const uint32_t *)sc.pos.ptr, chunk, (const uint64_t *)sc.blk.ptr);
+            hipLaunchKernelGGL(k_str_check, dim3(rg), dim3(256), 0, s, dsegs, nseg, total, t,
+                               sc.pos.as<uint32_t>(), sc.codes.as<uint32_t>());
+            HIPCHK(hipMemcpyAsync(ctr, t.ctr, SC_N * 8, hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+            if ((ctr[SC_FLAGS] & (STR_FULL_TABLE | STR_FULL_DICT)) || !ctr[SC_UNRESOLVED]) break;
+            // rows whose string differs from the code at their slot: probe on
+            if (round > (1u << 20)) fail(GBAGG_ERROR, "string interning did not converge");
+            HIPCHK(hipMemsetAsync(t.ctr + SC_UNRESOLVED, 0, 8, s));
+        }
+        if (!(ctr[SC_FLAGS] & (STR_FULL_TABLE | STR_FULL_DICT))) break;
+        if (attempt == 16) fail(GBAGG_ERROR, "string interning: tables still full after 16 resizes");
+        if (ctr[SC_FLAGS] & STR_FULL_TABLE) slots *= 2;
+        if (ctr[SC_FLAGS] & STR_FULL_DICT) {
+            code_cap = std::min<uint64_t>(rows, code_cap * 2);
+            arena_words = std::min<uint64_t>(words_max, arena_words * 2);
+            slots = std::max<uint64_t>(slots, next_pow2(2 * code_cap));
+        }
+    }
+    sc.ncodes = rows ? ctr[SC_CODES] : 0;
+    sc.words = rows ? ctr[SC_WORDS] : 0;
+    sc.tab = t;
+}
+
 void intern_column(gbagg_handle *h, int u) {
     const DeviceInfo &di = device_info();
     hipStream_t s = h->stream;
@@ -1624,98 +1860,8 @@ void intern_column(gbagg_handle *h, int u) {
     }
     if (total > (1ull << 30)) fail(GBAGG_INVALID, "more than 2^30 rows in one aggregate");
     sc.ncodes = 0;
-    uint64_t ctr[SC_N] = {};
-    StrTab t{};
-    if (total) {
-        sc.segs.ensure(segs.size() * sizeof(StrSeg));
-        HIPCHK(hipMemcpyAsync(sc.segs.ptr, segs.data(), segs.size() * sizeof(StrSeg), hipMemcpyHostToDevice, s));
-        sc.hash.ensure(total * 8);
-        sc.pos.ensure(total * 4);
-        sc.codes.ensure(total * 4);
-        sc.ctr.ensure(SC_N * 8);
-        sc.plan.ensure(sizeof(DevPlan));
-        t.ctr = sc.ctr.as<uint64_t>();
-        t.key_mask = h->str_key_mask;
-        t.seed = 0x5157C0DE5EEDull;
-        const StrSeg *dsegs = sc.segs.as<StrSeg>();
-        const uint32_t nseg = (uint32_t)segs.size();
-        const uint32_t W = (uint32_t)std::min<uint64_t>((total + 1023) / 1024, (uint64_t)di.cus * 4);
-        sc.hllp.ensure((size_t)W * HLL_M);
-        const uint32_t hslices = std::min<uint32_t>(32, W);
-        sc.hllm.ensure((size_t)HLL_M * hslices);
-        HIPCHK(hipMemsetAsync(t.ctr, 0, SC_N * 8, s));
-        HIPCHK(hipMemsetAsync(sc.plan.ptr, 0, sizeof(DevPlan), s));
-        hipLaunchKernelGGL(k_str_hash, dim3(W), dim3(256), 0, s, dsegs, nseg, total, t, sc.hash.as<uint64_t>(),
-                           sc.pos.as<uint32_t>(), sc.codes.as<uint32_t>(), sc.hllp.as<uint8_t>());
-        hipLaunchKernelGGL(k_hll_merge, dim3(HLL_M / 256, hslices), dim3(256), 0, s, (const uint8_t *)sc.hllp.ptr, W,
-                           sc.hllm.as<uint8_t>());
-        hipLaunchKernelGGL(k_hll_finish, dim3(1), dim3(1024), 0, s, (const uint8_t *)sc.hllm.ptr, hslices,
-                           sc.plan.as<DevPlan>());
-        DevPlan pl{};
-        HIPCHK(hipMemcpyAsync(ctr, t.ctr, SC_N * 8, hipMemcpyDeviceToHost, s));
-        HIPCHK(hipMemcpyAsync(&pl, sc.plan.ptr, sizeof(DevPlan), hipMemcpyDeviceToHost, s));
-        HIPCHK(hipStreamSynchronize(s));
-        if (ctr[SC_FLAGS] & STR_BAD_OFFSETS) fail(GBAGG_INVALID, "VARCHAR string offsets must be ascending");
-        const uint64_t rows = ctr[SC_ROWS], bytes = ctr[SC_BYTES];
-        const uint64_t est = std::max<uint64_t>(1, pl.est_groups);
-        // codes: the estimate with a margin (at most one per valid row); slots
-        // at least twice the codes; arena words from the mean string length
-        uint64_t code_cap = std::max<uint64_t>(1, std::min<uint64_t>(rows, est + est / 4 + 1024));
-        uint64_t slots = next_pow2(std::max<uint64_t>(1024, 2 * code_cap));
-        const uint64_t words_max = bytes / 4 + rows + 1;
-        const uint64_t mean_words = rows ? bytes / rows / 4 + 2 : 1;
-        uint64_t arena_words = std::min<uint64_t>(words_max, code_cap * mean_words * 3 / 2 + 1024);
-        for (uint32_t attempt = 0; rows; attempt++) {
-            if (slots > (1ull << 30)) fail(GBAGG_ERROR, "string table larger than 2^30 slots");
-            if (arena_words > 0xFFFFFFFFull) fail(GBAGG_ERROR, "distinct strings exceed 16 GiB");
-            sc.slot.ensure(slots * sizeof(StrSlot));
-            sc.arena.ensure(arena_words * 4);
-            sc.code.ensure(code_cap * sizeof(StrCode));
-            t.slot = sc.slot.as<StrSlot>();
-            t.code = sc.code.as<StrCode>();
-            t.arena = sc.arena.as<uint32_t>();
-            t.mask = slots - 1;
-            t.arena_words = arena_words;
-            t.code_cap = code_cap;
-            const uint32_t ig = (uint32_t)std::min<uint64_t>((slots + 255) / 256, (uint64_t)di.cus * 8);
-            hipLaunchKernelGGL(k_str_init, dim3(ig), dim3(256), 0, s, t);
-            const uint32_t rg = (uint32_t)std::min<uint64_t>((total + 255) / 256, (uint64_t)di.cus * 8);
-            // code assignment: blocks of contiguous rows, at most 4096 of them
-            const uint64_t chunk = std::max<uint64_t>(256, ((total + 4095) / 4096 + 255) / 256 * 256);
-            const uint32_t nblk = (uint32_t)((total + chunk - 1) / chunk);
-            sc.blk.ensure((size_t)nblk * 16);
-            hipLaunchKernelGGL(k_str_rehome, dim3(rg), dim3(256), 0, s, total, (const uint64_t *)sc.hash.ptr,
-                               sc.pos.as<uint32_t>());
-            for (uint32_t round = 0;; round++) {
-                hipLaunchKernelGGL(k_str_probe, dim3(rg), dim3(256), 0, s, dsegs, nseg, total, t,
-                                   (const uint64_t *)sc.hash.ptr, sc.pos.as<uint32_t>(), sc.codes.as<uint32_t>());
-                hipLaunchKernelGGL(k_str_reps, dim3(nblk), dim3(256), 0, s, dsegs, nseg, total, t,
-                                   (const uint32_t *)sc.pos.ptr, chunk, sc.blk.as<uint64_t>());
-                hipLaunchKernelGGL(k_str_scan, dim3(1), dim3(1024), 0, s, sc.blk.as<uint64_t>(), nblk, t.ctr);
-                hipLaunchKernelGGL(k_str_assign, dim3(nblk), dim3(256), 0, s, dsegs, nseg, total, t,
-                                   (const uint32_t *)sc.pos.ptr, chunk, (const uint64_t *)sc.blk.ptr);
-                hipLaunchKernelGGL(k_str_check, dim3(rg), dim3(256), 0, s, dsegs, nseg, total, t,
-                                   sc.pos.as<uint32_t>(), sc.codes.as<uint32_t>());
-                HIPCHK(hipMemcpyAsync(ctr, t.ctr, SC_N * 8, hipMemcpyDeviceToHost, s));
-                HIPCHK(hipStreamSynchronize(s));
-                if ((ctr[SC_FLAGS] & (STR_FULL_TABLE | STR_FULL_DICT)) || !ctr[SC_UNRESOLVED]) break;
-                // rows whose string differs from the code at their slot: probe on
-                if (round > (1u << 20)) fail(GBAGG_ERROR, "string interning did not converge");
-                HIPCHK(hipMemsetAsync(t.ctr + SC_UNRESOLVED, 0, 8, s));
-            }
-            if (!(ctr[SC_FLAGS] & (STR_FULL_TABLE | STR_FULL_DICT))) break;
-            if (attempt == 16) fail(GBAGG_ERROR, "string interning: tables still full after 16 resizes");
-            if (ctr[SC_FLAGS] & STR_FULL_TABLE) slots *= 2;
-            if (ctr[SC_FLAGS] & STR_FULL_DICT) {
-                code_cap = std::min<uint64_t>(rows, code_cap * 2);
-                arena_words = std::min<uint64_t>(words_max, arena_words * 2);
-                slots = std::max<uint64_t>(slots, next_pow2(2 * code_cap));
-            }
-        }
-        sc.ncodes = rows ? ctr[SC_CODES] : 0;
-        sc.words = rows ? ctr[SC_WORDS] : 0;
-    }
-    sc.tab = t;
+    intern_run(h, sc, segs, total);
+    const StrTab t = sc.tab;
     const uint32_t nc = (uint32_t)sc.ncodes;
     // predicates on this column: one match per distinct string
     sc.lit.resize(h->preds.size());
@@ -1827,6 +1973,94 @@ void gather_result_strings(gbagg_handle *h, size_t g, uint64_t n) {
     HIPCHK(hipStreamSynchronize(s));
 }
 
+// Multi-column keys: VARCHAR group columns interned (each over all its
+// batches, coded batches through their own dictionaries), every row's group
+// columns packed, the packed rows interned; their codes fill the hidden
+// key column of every batch.
+void intern_tuples(gbagg_handle *h) {
+    const DeviceInfo &di = device_info();
+    hipStream_t s = h->stream;
+    const size_t G = h->tcols.size();
+    std::vector<uint64_t> row0(h->batches.size());
+    uint64_t total = 0;
+    for (size_t i = 0; i < h->batches.size(); i++) {
+        row0[i] = total;
+        total += h->batches[i].nrows;
+    }
+    if (total > (1ull << 30)) fail(GBAGG_INVALID, "more than 2^30 rows in one aggregate");
+    for (size_t j = 0; j < G; j++) {
+        if (h->ttypes[j] != GBAGG_TYPE_VARCHAR) continue;
+        if (!h->tstr[j]) h->tstr[j] = std::make_unique<StrCol>();
+        std::vector<StrSeg> segs;
+        for (size_t i = 0; i < h->batches.size(); i++) {
+            Batch &b = h->batches[i];
+            if (!b.nrows) continue;
+            Batch::TupleIn &ti = b.tin[j];
+            if (ti.str_off) {
+                segs.push_back(StrSeg{ti.str_off, ti.str_bytes, ti.valid, nullptr, nullptr, row0[i], b.nrows, 0, 0});
+            } else {
+                auto off = std::make_unique<DevBuf>(), by = std::make_unique<DevBuf>();
+                off->ensure(ti.doff.size() * 4);
+                by->ensure(ti.dbytes.size() + 16);
+                HIPCHK(hipMemcpyAsync(off->ptr, ti.doff.data(), ti.doff.size() * 4, hipMemcpyHostToDevice, s));
+                if (!ti.dbytes.empty())
+                    HIPCHK(hipMemcpyAsync(by->ptr, ti.dbytes.data(), ti.dbytes.size(), hipMemcpyHostToDevice, s));
+                segs.push_back(StrSeg{off->as<uint32_t>(), by->as<uint8_t>(), ti.valid, (const uint32_t *)ti.data,
+                                      nullptr, row0[i], b.nrows, (uint32_t)(ti.doff.size() - 1), 0});
+                b.owned.push_back(std::move(off));
+                b.owned.push_back(std::move(by));
+            }
+        }
+        intern_run(h, *h->tstr[j], segs, total);
+    }
+    // pack every row's group columns
+    const uint32_t W = h->twords;
+    h->tpack.ensure(std::max<uint64_t>(1, total) * W * 4);
+    h->tkeep.ensure(std::max<uint64_t>(1, total));
+    std::vector<TupleComp> comps(h->batches.size() * G);
+    for (size_t i = 0; i < h->batches.size(); i++)
+        for (size_t j = 0; j < G; j++) {
+            const Batch::TupleIn &ti = h->batches[i].tin[j];
+            TupleComp &c = comps[i * G + j];
+            const gbagg_type t = h->ttypes[j];
+            c.valid = ti.valid;
+            c.word = h->twordof[j];
+            if (t == GBAGG_TYPE_VARCHAR) {
+                c.kind = TK_CODE;
+                c.data = h->tstr[j]->codes.as<uint32_t>() + row0[i];
+            } else {
+                c.kind = t == GBAGG_TYPE_INTEGER ? TK_I32 : t == GBAGG_TYPE_BIGINT ? TK_I64 : TK_F64;
+                c.data = ti.data;
+            }
+        }
+    h->tcomp.ensure(std::max<size_t>(1, comps.size()) * sizeof(TupleComp));
+    if (!comps.empty())
+        HIPCHK(hipMemcpyAsync(h->tcomp.ptr, comps.data(), comps.size() * sizeof(TupleComp), hipMemcpyHostToDevice, s));
+    std::vector<StrSeg> segs;
+    for (size_t i = 0; i < h->batches.size(); i++) {
+        const uint64_t n = h->batches[i].nrows;
+        if (!n) continue;
+        uint32_t *out = h->tpack.as<uint32_t>() + row0[i] * W;
+        uint8_t *keep = h->tkeep.as<uint8_t>() + row0[i];
+        const uint32_t grid = (uint32_t)std::min<uint64_t>((n + 255) / 256, (uint64_t)di.cus * 8);
+        hipLaunchKernelGGL(k_tuple_pack, dim3(grid), dim3(256), 0, s, (const TupleComp *)h->tcomp.as<TupleComp>() + i * G,
+                           (uint32_t)G, n, W, h->tdrop, out, keep);
+        segs.push_back(StrSeg{nullptr, (const uint8_t *)out, keep, nullptr, nullptr, row0[i], n, 0, W * 4});
+    }
+    if (!h->tsc) h->tsc = std::make_unique<StrCol>();
+    intern_run(h, *h->tsc, segs, total);
+    // the hidden key column: tuple codes, rows dropped where keep is 0
+    int ut = -1;
+    for (int u = 0; u < h->nu; u++)
+        if ((uint32_t)h->ucol[u] == h->ext_cols) ut = u;
+    if (ut < 0) fail(GBAGG_ERROR, "multi-column key: hidden key column missing from the plan");
+    for (size_t i = 0; i < h->batches.size(); i++) {
+        if (!h->batches[i].nrows) continue;
+        h->batches[i].db.data[ut] = h->tsc->codes.as<uint32_t>() + row0[i];
+        h->batches[i].db.valid[ut] = h->tkeep.as<uint8_t>() + row0[i];
+    }
+}
+
 void intern_strings(gbagg_handle *h) {
     h->ms_strings = 0;
     h->strings_distinct = 0;
@@ -1850,6 +2084,14 @@ void finalize(gbagg_handle *h) {
     if (h->phase1) run_phase1(h);
     ensure_runtime(h);
     intern_strings(h);
+    if (h->tuple) {
+        Timer tm{h, {}};
+        tm.mark();
+        intern_tuples(h);
+        tm.mark();
+        HIPCHK(hipStreamSynchronize(h->stream));
+        h->ms_strings += tm.between(0, 1);
+    }
     uint64_t rows = std::max<uint64_t>(h->rows_total, h->expected_rows);
     if (rows > (1ULL << 30)) fail(GBAGG_INVALID, "more than 2^30 rows in one aggregate");
     h->cap_max = std::max<uint64_t>(h->cap_max, next_pow2(std::max<uint64_t>(1024, rows + rows / 4 + 2)));
@@ -2024,6 +2266,85 @@ void finalize_once(gbagg_handle *h) {
     }
 }
 
+// A StrCol's dictionary on the host: per code its first arena word and length,
+// and the arena.
+void strcol_host(gbagg_handle *h, const StrCol &sc, std::vector<StrCode> &cd, std::vector<uint32_t> &words) {
+    cd.resize(sc.ncodes);
+    words.assign(std::max<uint64_t>(1, sc.words), 0);
+    if (!sc.ncodes) return;
+    HIPCHK(hipMemcpyAsync(cd.data(), sc.code.ptr, sc.ncodes * sizeof(StrCode), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipMemcpyAsync(words.data(), sc.arena.ptr, sc.words * 4, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+}
+
+// Multi-column keys: the result's first column holds tuple codes; replace it
+// by the group columns unpacked from the tuples.
+void expand_tuples(gbagg_handle *h, uint64_t rows) {
+    const size_t G = h->tcols.size(), nin = h->outs.size(), nout = G + nin - 1;
+    std::vector<StrCode> tcd;
+    std::vector<uint32_t> tw;
+    strcol_host(h, *h->tsc, tcd, tw);
+    std::vector<std::vector<StrCode>> scd(G);
+    std::vector<std::vector<uint32_t>> sw(G);
+    for (size_t j = 0; j < G; j++)
+        if (h->ttypes[j] == GBAGG_TYPE_VARCHAR) strcol_host(h, *h->tstr[j], scd[j], sw[j]);
+    std::vector<std::vector<uint8_t>> data(nout), valid(nout);
+    std::vector<std::vector<uint32_t>> soff(nout);
+    std::vector<std::vector<char>> sbytes(nout);
+    const uint32_t *codes = (const uint32_t *)h->res_data[0].data();
+    for (size_t j = 0; j < G; j++) {
+        const gbagg_type t = h->ttypes[j];
+        const size_t w = type_width(t);
+        data[j].assign(std::max<uint64_t>(1, rows) * w, 0);
+        valid[j].assign(std::max<uint64_t>(1, rows), 0);
+        if (t == GBAGG_TYPE_VARCHAR) soff[j].assign(rows + 1, 0);
+        for (uint64_t r = 0; r < rows; r++) {
+            const uint32_t c = codes[r];
+            if (c >= tcd.size()) fail(GBAGG_ERROR, "multi-column key: tuple code out of range");
+            const uint32_t *tup = tw.data() + tcd[c].word;
+            const bool v = (tup[j / 4] >> (8 * (j % 4))) & 0xFF;
+            valid[j][r] = v;
+            const uint32_t *val = tup + h->twordof[j];
+            if (t == GBAGG_TYPE_VARCHAR) {
+                soff[j][r] = (uint32_t)sbytes[j].size();
+                if (v) {
+                    const uint32_t sc = val[0];
+                    if (sc >= scd[j].size()) fail(GBAGG_ERROR, "multi-column key: string code out of range");
+                    const char *p = (const char *)(sw[j].data() + scd[j][sc].word);
+                    sbytes[j].insert(sbytes[j].end(), p, p + scd[j][sc].len);
+                }
+                std::memcpy(&data[j][r * 4], val, 4);
+            } else {
+                std::memcpy(&data[j][r * w], val, w);
+            }
+        }
+        if (t == GBAGG_TYPE_VARCHAR) soff[j][rows] = (uint32_t)sbytes[j].size();
+    }
+    std::vector<gbagg_type> types(nout);
+    for (size_t j = 0; j < G; j++) types[j] = h->ttypes[j];
+    for (size_t c = 1; c < nin; c++) {
+        data[G + c - 1] = std::move(h->res_data[c]);
+        valid[G + c - 1] = std::move(h->res_valid[c]);
+        types[G + c - 1] = h->outs[c].type;
+    }
+    h->res_data = std::move(data);
+    h->res_valid = std::move(valid);
+    h->res_str_off = std::move(soff);
+    h->res_str_bytes = std::move(sbytes);
+    h->res_cols.assign(nout, {});
+    for (size_t c = 0; c < nout; c++) {
+        gbagg_result_column &rc = h->res_cols[c];
+        rc.type = types[c];
+        rc.data = h->res_data[c].data();
+        rc.validity = h->res_valid[c].data();
+        if (rc.type == GBAGG_TYPE_VARCHAR) {
+            if (h->res_str_bytes[c].empty()) h->res_str_bytes[c].push_back(0);
+            rc.str_offsets = h->res_str_off[c].data();
+            rc.str_bytes = h->res_str_bytes[c].data();
+        }
+    }
+}
+
 void download_result(gbagg_handle *h) {
     if (!h->finalized) fail(GBAGG_INVALID, "gbagg_result before gbagg_finalize");
     if (h->result_ready) return;
@@ -2143,6 +2464,12 @@ void download_result(gbagg_handle *h) {
         rows = k;
     } else if (h->row_limit > 0 && rows > h->row_limit && !(h->topn_col >= 0 && h->topn_limit > 0)) {
         rows = h->row_limit;  // bare LIMIT: any `row_limit` groups
+    }
+    if (h->tuple) {
+        expand_tuples(h, rows);
+        h->result_rows = rows;
+        h->result_ready = true;
+        return;
     }
     h->res_cols.assign(nc, {});
     for (size_t c = 0; c < nc; c++) {

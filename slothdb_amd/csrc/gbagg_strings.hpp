@@ -41,8 +41,9 @@ constexpr uint64_t STR_FULL_TABLE = 1, STR_FULL_DICT = 2, STR_BAD_OFFSETS = 4;
 enum { SC_CODES = 0, SC_WORDS, SC_UNRESOLVED, SC_FLAGS, SC_ROWS, SC_BYTES, SC_N };
 
 // Rows [row0, row0 + n) of the column's concatenation: one batch, given as
-// strings (off: n + 1 ascending offsets into bytes), or as dictionary codes
-// (dcodes) whose strings are dictionary entries off/bytes (through remap).
+// strings (off: n + 1 ascending offsets into bytes), as dictionary codes
+// (dcodes) whose strings are dictionary entries off/bytes (through remap), or
+// as fixed-width rows of `fixed` bytes each (packed multi-column keys).
 struct StrSeg {
     const uint32_t *off;
     const uint8_t *bytes;
@@ -52,13 +53,27 @@ struct StrSeg {
     uint64_t row0;
     uint64_t n;
     uint32_t dict_n;         // batch codes >= dict_n drop their row
-    uint32_t pad;
+    uint32_t fixed;          // bytes per row of fixed-width rows (0: strings or codes)
 };
 
 constexpr uint32_t STR_DROP = 0xFFFFFFFFu;  // output code of a dropped row (>= any dictionary size)
 
+struct StrRow {
+    const uint8_t *p;
+    uint32_t len;
+    bool valid;
+};
+
 // The string of row r of a segment (a valid row with an in-range code).
-__device__ __forceinline__ void str_range(const StrSeg &s, uint64_t r, uint32_t &o0, uint32_t &o1) {
+__device__ __forceinline__ StrRow str_at(const StrSeg &s, uint64_t r) {
+    StrRow out;
+    out.valid = true;
+    if (s.fixed) {
+        out.p = s.bytes + r * s.fixed;
+        out.len = s.fixed;
+        return out;
+    }
+    uint32_t o0, o1;
     if (s.dcodes) {
         const uint32_t c = gld(&s.dcodes[r]);
         const uint32_t e = s.remap ? gld(&s.remap[c]) : c;
@@ -68,6 +83,9 @@ __device__ __forceinline__ void str_range(const StrSeg &s, uint64_t r, uint32_t 
         o0 = gld(&s.off[r]);
         o1 = gld(&s.off[r + 1]);
     }
+    out.p = s.bytes + o0;
+    out.len = o1 >= o0 ? o1 - o0 : 0;
+    return out;
 }
 
 // One slot of the table: a probe reads one 16-byte record.
@@ -156,21 +174,11 @@ __device__ __forceinline__ bool str_equal(const StrTab &t, uint32_t c, const uin
     return true;
 }
 
-struct StrRow {
-    const uint8_t *p;
-    uint32_t len;
-    bool valid;
-};
-
 __device__ __forceinline__ StrRow str_row(const StrSeg *segs, uint32_t nseg, uint64_t g) {
     const StrSeg s = segs[str_seg(segs, nseg, g)];
     const uint64_t r = g - s.row0;
-    uint32_t o0, o1;
-    str_range(s, r, o0, o1);
-    StrRow out;
+    StrRow out = str_at(s, r);
     out.valid = !s.valid || gld(&s.valid[r]) != 0;
-    out.p = s.bytes + o0;
-    out.len = o1 >= o0 ? o1 - o0 : 0;
     return out;
 }
 
@@ -207,17 +215,13 @@ __global__ __launch_bounds__(256) void k_str_hash(const StrSeg *segs, uint32_t n
         // a dictionary code outside its dictionary drops the row, as it does
         // for dictionary-coded keys
         const bool drop = valid && s.dcodes && gld(&s.dcodes[r]) >= s.dict_n;
-        uint32_t o0 = 0, o1 = 0;
         bool ok = true;
-        if (!s.dcodes) {
+        if (!s.dcodes && !s.fixed) {
             // every row's offsets ascend and stay within the batch's bytes
             // (offsets[n]); a row outside them is never read
-            o0 = gld(&s.off[r]);
-            o1 = gld(&s.off[r + 1]);
+            const uint32_t o0 = gld(&s.off[r]), o1 = gld(&s.off[r + 1]);
             ok = o0 <= o1 && o1 <= gld(&s.off[s.n]);
             if (!ok) bad = true;
-        } else if (valid && !drop) {
-            str_range(s, r, o0, o1);
         }
         if (!valid || drop || !ok) {
             hash[g] = 0;
@@ -225,8 +229,9 @@ __global__ __launch_bounds__(256) void k_str_hash(const StrSeg *segs, uint32_t n
             codes[g] = drop ? STR_DROP : 0;
             continue;
         }
-        const uint32_t len = o1 - o0;
-        const uint64_t full = str_hash(s.bytes + o0, len, t.seed);
+        const StrRow sr = str_at(s, r);
+        const uint32_t len = sr.len;
+        const uint64_t full = str_hash(sr.p, len, t.seed);
         const uint64_t k = full & t.key_mask;
         hash[g] = k ? k : 1;
         pos[g] = STR_HOME;
@@ -503,6 +508,62 @@ __global__ __launch_bounds__(256) void k_str_match(StrTab t, uint64_t ncodes, co
             if (op == 3) r = !r;
         }
         match[c] = r;
+    }
+}
+
+// ---- Multi-column group keys -----------------------------------------
+// Each row's group columns packed into `words` 32-bit words: one validity
+// byte per column (0 = NULL) padded to a word, then each column's value at a
+// word-aligned offset (NULL: zeros). DOUBLE values are canonical (-0.0 as
+// 0.0, one NaN), so SQL-equal keys pack to equal bytes. The packed rows are
+// interned like strings; their codes are the aggregate's one-column key.
+enum { TK_I32 = 0, TK_I64 = 1, TK_F64 = 2, TK_CODE = 3 };
+
+struct TupleComp {
+    const void *data;      // values; TK_CODE: the interned codes of a VARCHAR column
+    const uint8_t *valid;  // null: all valid
+    uint32_t kind;
+    uint32_t word;         // first word of the value in the packed row
+};
+
+// keep[r] = 0 drops the row: a code outside its dictionary, or a NULL key
+// column when drop_null is set.
+__global__ __launch_bounds__(256) void k_tuple_pack(const TupleComp *__restrict__ comps, uint32_t ncomp, uint64_t n,
+                                                    uint32_t words, int drop_null, uint32_t *__restrict__ out,
+                                                    uint8_t *__restrict__ keep) {
+    for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (uint64_t)gridDim.x * blockDim.x) {
+        uint32_t *o = out + r * words;
+        uint32_t v0 = 0, v1 = 0;
+        bool any_null = false, dropped = false;
+        for (uint32_t j = 0; j < ncomp; j++) {
+            const TupleComp c = comps[j];
+            const bool v = !c.valid || c.valid[r] != 0;
+            uint64_t x = 0;
+            if (v) {
+                if (c.kind == TK_I32) {
+                    x = (uint32_t)((const int32_t *)c.data)[r];
+                } else if (c.kind == TK_CODE) {
+                    x = ((const uint32_t *)c.data)[r];
+                    dropped |= x == STR_DROP;
+                } else {
+                    x = ((const uint64_t *)c.data)[r];
+                    if (c.kind == TK_F64) {
+                        if ((x << 1) == 0) x = 0;  // -0.0
+                        if ((x & 0x7FF0000000000000ull) == 0x7FF0000000000000ull && (x & 0x000FFFFFFFFFFFFFull))
+                            x = 0x7FF8000000000000ull;  // NaN
+                    }
+                }
+            } else {
+                any_null = true;
+            }
+            if (j < 4) v0 |= (v ? 1u : 0u) << (8 * j);
+            else v1 |= (v ? 1u : 0u) << (8 * (j - 4));
+            o[c.word] = (uint32_t)x;
+            if (c.kind == TK_I64 || c.kind == TK_F64) o[c.word + 1] = (uint32_t)(x >> 32);
+        }
+        o[0] = v0;
+        if (ncomp > 4) o[1] = v1;
+        keep[r] = !(dropped || (drop_null && any_null));
     }
 }
 

@@ -55,6 +55,15 @@ def _plan(q):
     return st, msg
 
 
+def test_multi_column_key_plans_create_without_device():
+    # more than two columns, DOUBLE columns, nullable pairs that keep NULL groups
+    assert _plan(G.Query([G.INTEGER, G.BIGINT, G.DOUBLE, G.VARCHAR, G.BIGINT], [0, 1, 2, 3],
+                         [(G.COUNT_STAR, None), (G.SUM, 4)]))[0] == G.OK
+    assert _plan(G.Query([G.DOUBLE], [0], [(G.COUNT_STAR, None)]))[0] == G.OK
+    assert _plan(G.Query([G.BIGINT, G.BIGINT], [0, 1], [(G.COUNT_STAR, None)]))[0] == G.OK
+    assert _plan(G.Query([G.INTEGER] * 9, list(range(8)), [(G.MAX, 8)], topn=(8, False, 5)))[0] == G.OK
+
+
 def test_valid_plans_create_without_device():
     assert _plan(G.Query([G.INTEGER, G.BIGINT], [0], [(G.COUNT_STAR, None), (G.SUM, 1)]))[0] == G.OK
     assert _plan(G.Query([G.INTEGER, G.VARCHAR, G.INTEGER], [0, 1], [(G.SUM, 2)],
@@ -71,10 +80,10 @@ def test_valid_plans_create_without_device():
 
 
 @pytest.mark.parametrize("q,needle", [
-    (G.Query([G.DOUBLE], [0], [(G.COUNT_STAR, None)]), "DOUBLE group keys"),
-    (G.Query([G.INTEGER] * 3, [0, 1, 2], [(G.COUNT_STAR, None)], nullable=[0, 0, 0]), "at most 2 group"),
-    (G.Query([G.BIGINT, G.BIGINT], [0, 1], [(G.COUNT_STAR, None)]), "drop_null_keys"),
-    (G.Query([G.INTEGER, G.INTEGER], [0, 1], [(G.COUNT_STAR, None)]), "drop_null_keys"),
+    (G.Query([G.INTEGER] * 10, list(range(9)), [(G.COUNT_STAR, None)], nullable=[0] * 10), "at most 8 group"),
+    (G.Query([G.INTEGER] * 16, [0, 1, 2], [(G.COUNT_STAR, None)], nullable=[0] * 16), "at most 15 columns"),
+    (G.Query([G.INTEGER] * 3, [0, 1, 2], [(G.COUNT_STAR, None)], topn=(1, False, 3)), "TopN on a group column"),
+    (G.Query([G.DOUBLE, G.BIGINT], [0], [(G.COUNT_DISTINCT, 1)]), "DOUBLE group column"),
     (G.Query([G.INTEGER, G.VARCHAR], [0], [(G.SUM, 1)]), "numeric"),
     (G.Query([G.INTEGER, G.VARCHAR], [0], [(G.MIN, 1)]), "VARCHAR"),
     (G.Query([G.INTEGER], [0], [(G.COUNT_STAR, None)], predicates=[(0, G.LIKE, "x")]), "LIKE"),

@@ -333,3 +333,47 @@ def test_oracle_topn_tie_break():
     O.lib().orc_result_free(C.byref(res))
     assert [(int(c[0][i]), int(c[1][i])) for i in idx[:m]] == [(1, 2), (3, 2), (5, 2), (7, 1)]
 
+
+
+def _dkey(x):
+    """SQL grouping of a double: -0.0 with 0.0, every NaN together."""
+    if x != x:
+        return "nan"
+    return 0.0 if x == 0.0 else x
+
+
+@pytest.mark.parametrize("drop", [False, True])
+def test_multi_column_and_double_keys_vs_brute_force(drop):
+    # GROUP BY (INTEGER, BIGINT, DOUBLE, VARCHAR) with NULLs, -0.0 and NaN
+    # keys; brute force groups Python tuples
+    rng = np.random.default_rng(21)
+    n = 4000
+    a = rng.integers(-3, 3, n).astype(np.int32)
+    b = rng.integers(0, 4, n) * (1 << 40)
+    d = rng.choice(np.array([0.0, -0.0, 1.5, np.nan, -2.25]), n)
+    words = G.Dictionary(["x", "y", "z", "x"])  # duplicate entry: same string, one group
+    s = rng.integers(0, 4, n).astype(np.uint32)
+    v = rng.integers(-1000, 1000, n)
+    va, vd, vs = (rng.random(n) > 0.1).astype(np.uint8), (rng.random(n) > 0.1).astype(np.uint8), \
+        (rng.random(n) > 0.1).astype(np.uint8)
+    cols = [G.Column(G.INTEGER, a, va), G.Column(G.BIGINT, b), G.Column(G.DOUBLE, d, vd),
+            G.Column(G.VARCHAR, s, vs, dictionary=words), G.Column(G.BIGINT, v)]
+    q = G.Query([c.type for c in cols], [0, 1, 2, 3], [(G.COUNT_STAR, None), (G.SUM, 4), (G.MIN, 4)],
+                nullable=[True, False, True, True, False], drop_null_keys=drop)
+    c_, v_, t, ov = O.groupby_exact(q, cols)
+    got = {}
+    for row in oracle_rows(c_, v_, t, {3: words}):
+        key = (row[0], row[1], None if row[2] is None else _dkey(row[2]), row[3])
+        assert key not in got
+        got[key] = row[4:]
+    exp = defaultdict(lambda: [0, 0, None])
+    for r in range(n):
+        key = (int(a[r]) if va[r] else None, int(b[r]), _dkey(float(d[r])) if vd[r] else None,
+               words.get(int(s[r])) if vs[r] else None)
+        if drop and None in key:
+            continue
+        e = exp[key]
+        e[0] += 1
+        e[1] += int(v[r])
+        e[2] = int(v[r]) if e[2] is None else min(e[2], int(v[r]))
+    assert got == {k: tuple(e) for k, e in exp.items()}
