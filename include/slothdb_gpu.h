@@ -194,10 +194,16 @@ typedef struct {
      * (validity must then be a null pointer), 1 = may contain NULLs.
      * A null pointer here means every column may contain NULLs. */
     const uint8_t *column_nullable;
-    /* 0-2 group columns. Two columns with a BIGINT among them form a 16-byte
-     * key (InlineRowAggBigKey / RadixMultiAggBigKey, inline_row_agg.cpp:269-517,
-     * radix_multi_agg.cpp:218-225); two 32-bit columns pack into 8 bytes. Two
-     * columns over nullable data require drop_null_keys. */
+    /* 0-8 group columns. One INTEGER/BIGINT/VARCHAR column is the key word;
+     * two columns with a BIGINT among them form a 16-byte key
+     * (InlineRowAggBigKey / RadixMultiAggBigKey, inline_row_agg.cpp:269-517,
+     * radix_multi_agg.cpp:218-225); two 32-bit columns pack into 8 bytes.
+     * Keys those forms cannot hold (more than two columns, a DOUBLE column,
+     * or two columns whose NULLs form groups; the generic paths' row keys,
+     * physical_planner.cpp:12291-12516, count_group_agg.cpp:32-227) are
+     * packed per row and interned on the GPU into one code; such a plan takes
+     * at most 15 columns (the code is one more) and no TopN hint on a group
+     * column. DOUBLE keys group -0.0 with 0.0 and all NaNs together. */
     uint32_t num_group_cols;
     const uint32_t *group_cols;
     uint32_t num_aggs;
