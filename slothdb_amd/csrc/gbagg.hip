@@ -449,6 +449,8 @@ void build_plan(gbagg_handle *h, const gbagg_query_desc *q) {
         h->nullable.push_back(q->column_nullable ? q->column_nullable[c] : 1);
     }
     if (q->num_group_cols > GBAGG_MAX_GROUP_COLS) fail(GBAGG_INVALID, "at most 8 group columns");
+    if (q->num_group_cols && !q->group_cols) fail(GBAGG_INVALID, "null group_cols");
+    if (q->num_group_cols > 2) fail(GBAGG_INVALID, "group columns out of range");  // keys of 3+ columns are tuples
     if (q->num_aggs > GBAGG_MAX_AGGS) fail(GBAGG_INVALID, "at most 16 aggregates");
     if (q->num_predicates > GBAGG_MAX_PREDICATES) fail(GBAGG_INVALID, "at most 8 predicates");
     if (q->num_aggs == 0 && q->num_group_cols == 0) fail(GBAGG_INVALID, "nothing to aggregate");
