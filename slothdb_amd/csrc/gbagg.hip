@@ -1767,29 +1767,40 @@ void intern_run(gbagg_handle *h, StrCol &sc, const std::vector<StrSeg> &segs, ui
         t.code_cap = code_cap;
         const uint32_t ig = (uint32_t)std::min<uint64_t>((slots + 255) / 256, (uint64_t)di.cus * 8);
         hipLaunchKernelGGL(k_str_init, dim3(ig), dim3(256), 0, s, t);
-        const uint32_t rg = (uint32_t)std::min<uint64_t>((total + 255) / 256, (uint64_t)di.cus * 8);
-        // code assignment: blocks of contiguous rows, at most 4096 of them
-        const uint64_t chunk = std::max<uint64_t>(256, ((total + 4095) / 4096 + 255) / 256 * 256);
-        const uint32_t nblk = (uint32_t)((total + chunk - 1) / chunk);
-        sc.blk.ensure((size_t)nblk * 16);
-        hipLaunchKernelGGL(k_str_rehome, dim3(rg), dim3(256), 0, s, total, (const uint64_t *)sc.hash.ptr,
+        sc.blk.ensure(4096 * 16);
+        const uint32_t rg_all = (uint32_t)std::min<uint64_t>((total + 255) / 256, (uint64_t)di.cus * 8);
+        hipLaunchKernelGGL(k_str_rehome, dim3(rg_all), dim3(256), 0, s, total, (const uint64_t *)sc.hash.ptr,
                            sc.pos.as<uint32_t>());
-        for (uint32_t round = 0;; round++) {
-            hipLaunchKernelGGL(k_str_probe, dim3(rg), dim3(256), 0, s, dsegs, nseg, total, t,
-                               (const uint64_t *)sc.hash.ptr, sc.pos.as<uint32_t>(), sc.codes.as<uint32_t>());
-            hipLaunchKernelGGL(k_str_reps, dim3(nblk), dim3(256), 0, s, dsegs, nseg, total, t,
-                               (const uint32_t *)sc.pos.ptr, chunk, sc.blk.as<uint64_t>());
-            hipLaunchKernelGGL(k_str_scan, dim3(1), dim3(1024), 0, s, sc.blk.as<uint64_t>(), nblk, t.ctr);
-            hipLaunchKernelGGL(k_str_assign, dim3(nblk), dim3(256), 0, s, dsegs, nseg, total, t,
-                               (const uint32_t *)sc.pos.ptr, chunk, (const uint64_t *)sc.blk.ptr);
-            hipLaunchKernelGGL(k_str_check, dim3(rg), dim3(256), 0, s, dsegs, nseg, total, t,
-                               sc.pos.as<uint32_t>(), sc.codes.as<uint32_t>());
-            HIPCHK(hipMemcpyAsync(ctr, t.ctr, SC_N * 8, hipMemcpyDeviceToHost, s));
-            HIPCHK(hipStreamSynchronize(s));
-            if ((ctr[SC_FLAGS] & (STR_FULL_TABLE | STR_FULL_DICT)) || !ctr[SC_UNRESOLVED]) break;
-            // rows whose string differs from the code at their slot: probe on
-            if (round > (1u << 20)) fail(GBAGG_ERROR, "string interning did not converge");
-            HIPCHK(hipMemsetAsync(t.ctr + SC_UNRESOLVED, 0, 8, s));
+        // a prefix of the rows first: its strings (the frequent ones above
+        // all) then have codes when the other rows probe, which settle in
+        // k_str_probe without the representative and check kernels
+        const uint64_t prefix = std::max<uint64_t>(1u << 16, total / 16);
+        std::vector<uint64_t> lims;
+        if (prefix < total) lims.push_back(prefix);
+        lims.push_back(total);
+        for (uint64_t lim : lims) {
+            const uint32_t rg = (uint32_t)std::min<uint64_t>((lim + 255) / 256, (uint64_t)di.cus * 8);
+            // code assignment: blocks of contiguous rows, at most 4096 of them
+            const uint64_t chunk = std::max<uint64_t>(256, ((lim + 4095) / 4096 + 255) / 256 * 256);
+            const uint32_t nblk = (uint32_t)((lim + chunk - 1) / chunk);
+            for (uint32_t round = 0;; round++) {
+                hipLaunchKernelGGL(k_str_probe, dim3(rg), dim3(256), 0, s, dsegs, nseg, lim, t,
+                                   (const uint64_t *)sc.hash.ptr, sc.pos.as<uint32_t>(), sc.codes.as<uint32_t>());
+                hipLaunchKernelGGL(k_str_reps, dim3(nblk), dim3(256), 0, s, dsegs, nseg, lim, t,
+                                   (const uint32_t *)sc.pos.ptr, chunk, sc.blk.as<uint64_t>());
+                hipLaunchKernelGGL(k_str_scan, dim3(1), dim3(1024), 0, s, sc.blk.as<uint64_t>(), nblk, t.ctr);
+                hipLaunchKernelGGL(k_str_assign, dim3(nblk), dim3(256), 0, s, dsegs, nseg, lim, t,
+                                   (const uint32_t *)sc.pos.ptr, chunk, (const uint64_t *)sc.blk.ptr);
+                hipLaunchKernelGGL(k_str_check, dim3(rg), dim3(256), 0, s, dsegs, nseg, lim, t,
+                                   sc.pos.as<uint32_t>(), sc.codes.as<uint32_t>());
+                HIPCHK(hipMemcpyAsync(ctr, t.ctr, SC_N * 8, hipMemcpyDeviceToHost, s));
+                HIPCHK(hipStreamSynchronize(s));
+                if ((ctr[SC_FLAGS] & (STR_FULL_TABLE | STR_FULL_DICT)) || !ctr[SC_UNRESOLVED]) break;
+                // rows whose string differs from the code at their slot: probe on
+                if (round > (1u << 20)) fail(GBAGG_ERROR, "string interning did not converge");
+                HIPCHK(hipMemsetAsync(t.ctr + SC_UNRESOLVED, 0, 8, s));
+            }
+            if (ctr[SC_FLAGS] & (STR_FULL_TABLE | STR_FULL_DICT)) break;
         }
         if (!(ctr[SC_FLAGS] & (STR_FULL_TABLE | STR_FULL_DICT))) break;
         if (attempt == 16) fail(GBAGG_ERROR, "string interning: tables still full after 16 resizes");
