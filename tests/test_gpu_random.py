@@ -1,0 +1,111 @@
+"""Seeded random queries against the oracle: random schemas (INTEGER, BIGINT,
+DOUBLE, VARCHAR as dictionary codes or as strings), 0-4 group columns of any
+of those types, 1-4 aggregates, 0-2 predicates, NULLs, drop_null_keys,
+forced strategies, and several host or device batches. Each case prints its
+number and seed before it runs."""
+import numpy as np
+import pytest
+
+from slothdb_amd import gbagg as G
+from oracle import oracle as O
+from tests._util import assert_same_groups, oracle_rows
+from tests.test_gpu_strings import _as_dict, _split
+
+pytestmark = pytest.mark.gpu
+
+WORDS = [f"w{i:03d}" + "z" * (i % 7) for i in range(400)] + ["", "a", "ab"]
+
+
+def _column(rng, t, n, null_p, raw):
+    valid = (rng.random(n) >= null_p).astype(np.uint8) if null_p else None
+    if t == G.INTEGER:
+        hi = int(rng.choice([3, 50, 100_000]))
+        return G.Column(t, rng.integers(-hi, hi, n).astype(np.int32), valid)
+    if t == G.BIGINT:  # |value| <= 1e9: sums of 200K rows stay far inside int64
+        hi, mult = [(5, 1), (1000, 1_000_003), (10**9, 1)][int(rng.integers(0, 3))]
+        return G.Column(t, rng.integers(-hi, hi, n) * mult, valid)
+    if t == G.DOUBLE:
+        if rng.random() < 0.5:  # few values, -0.0 among them
+            return G.Column(t, rng.choice(np.array([0.0, -0.0, 1.5, 2.25, 1e10]), n), valid)
+        return G.Column(t, rng.uniform(1, 100, n), valid)
+    m = int(rng.choice([3, 40, len(WORDS)]))
+    codes = rng.integers(0, m, n).astype(np.uint32)
+    if raw:
+        ok = np.ones(n, bool) if valid is None else valid.astype(bool)
+        return G.Column.strings([WORDS[c] if ok[i] else None for i, c in enumerate(codes)])
+    return G.Column(t, codes, valid, dictionary=G.Dictionary(WORDS[:m], unique=True))
+
+
+def _case(seed):
+    rng = np.random.default_rng(seed)
+    n = int(np.exp(rng.uniform(np.log(1), np.log(200_000))))
+    ncol = int(rng.integers(3, 7))
+    types = [int(rng.choice([G.INTEGER, G.BIGINT, G.DOUBLE, G.VARCHAR])) for _ in range(ncol)]
+    types[-1] = G.BIGINT  # one numeric column for SUM/MIN/MAX
+    raw = rng.random() < 0.5
+    nulls = [float(rng.choice([0.0, 0.0, 0.05, 0.3])) for _ in range(ncol)]
+    cols = [_column(rng, t, n, p, raw) for t, p in zip(types, nulls)]
+    ng = int(rng.integers(0, min(4, ncol) + 1))
+    gcols = [int(c) for c in rng.permutation(ncol)[:ng]]
+    numeric = [c for c in range(ncol) if types[c] != G.VARCHAR]
+    aggs = [(G.COUNT_STAR, None)]
+    for _ in range(int(rng.integers(0, 4))):
+        kind = int(rng.choice([G.COUNT, G.SUM, G.AVG, G.MIN, G.MAX]))
+        col = int(rng.choice(range(ncol) if kind == G.COUNT else numeric))
+        aggs.append((kind, col))
+    preds = []
+    for _ in range(int(rng.integers(0, 3))):
+        c = int(rng.integers(0, ncol))
+        if types[c] == G.VARCHAR:
+            preds.append((c, int(rng.choice([G.EQ, G.NE, G.LIKE, G.NOT_LIKE])), str(rng.choice(["w01", "a", "z", ""]))))
+        elif types[c] == G.DOUBLE:
+            preds.append((c, int(rng.choice([G.LT, G.GE])), float(rng.choice([1.0, 50.0]))))
+        else:
+            preds.append((c, int(rng.choice([G.LT, G.GT, G.NE])), int(rng.integers(-3, 4))))
+    strategy = int(rng.choice([G.AUTO, G.AUTO, G.LDS_HASH, G.GLOBAL_HASH, G.RADIX]))
+    q = G.Query(types, gcols, aggs, predicates=preds, nullable=[p > 0 for p in nulls],
+                drop_null_keys=bool(rng.random() < 0.3), strategy=strategy)
+    return q, cols, int(rng.integers(1, 4)), bool(rng.random() < 0.5)
+
+
+def _norm(rows, nk):
+    def f(x):
+        return (0.0 if x == 0.0 else x) if isinstance(x, float) else x
+    return [tuple(f(x) for x in r[:nk]) + tuple(r[nk:]) for r in rows]
+
+
+def test_random_queries_against_oracle():
+    ran = 0
+    for case in range(150):
+        seed = 1000 + case
+        q, cols, nbatch, device = _case(seed)
+        print(f"[random] case {case} seed {seed}: rows {len(cols[0])} types {q.column_types} groups {q.group_cols} "
+              f"aggs {q.aggs} preds {q.predicates} drop {q.drop_null_keys} strategy {q.strategy} "
+              f"batches {nbatch} device {device}", flush=True)
+        op = G.GpuHashAggregate(q)
+        try:
+            op.Init()
+        except G.GbaggError as e:  # plans outside the operator's limits (e.g. more than 8 input columns used)
+            print(f"[random] case {case}: plan refused: {e}", flush=True)
+            op.close()
+            continue
+        keep = []
+        for b in _split(cols, nbatch):
+            if device:
+                dc = [G.DeviceColumn(c) for c in b]
+                keep.append(dc)
+                op.consume_device(dc)
+            else:
+                op.consume(b)
+        op.finalize()
+        got = op.result().rows()
+        op.close()
+        dcols = [_as_dict(c) if c.raw is not None else c for c in cols]
+        dicts = {g: dcols[c].dictionary for g, c in enumerate(q.group_cols) if dcols[c].dictionary is not None}
+        c_, v_, t, ov = O.groupby_exact(q, dcols)
+        assert not ov
+        exp = oracle_rows(c_, v_, t, dicts)
+        nk = len(q.group_cols)
+        assert_same_groups(_norm(got, nk), _norm(exp, nk), nk, t)
+        ran += 1
+    assert ran >= 120
