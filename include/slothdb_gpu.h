@@ -113,11 +113,12 @@ typedef struct {
  *   COUNT(DISTINCT c)  BIGINT, distinct non-NULL values of an INTEGER, BIGINT
  *             or VARCHAR c (the generic path's distinct_set,
  *             physical_planner.cpp:12325-12330). Runs in two phases: GROUP BY
- *             (group column, c) with partial states of the other aggregates,
- *             then GROUP BY the group column over those rows. All DISTINCT
- *             aggregates of a query share one c; at most one group column,
- *             which must be non-nullable or have drop_null_keys; a nullable c
- *             allows only DISTINCT aggregates next to it.
+ *             (group column, c) with partial states of the other aggregates
+ *             (rows with a NULL c form their own groups there), then GROUP BY
+ *             the group column over those rows, counting non-NULL c; a group
+ *             whose c is always NULL counts 0. All DISTINCT aggregates of a
+ *             query share one c; at most one group column, not DOUBLE, which
+ *             must be non-nullable or have drop_null_keys.
  * Integer SUM is exact two's-complement int64 (the reference accumulates in
  * double and casts; the two agree while partial sums stay below 2^53). */
 typedef enum {

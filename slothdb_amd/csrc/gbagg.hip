@@ -331,7 +331,8 @@ struct gbagg_handle {
     int32_t tdrop = 0;                  // the caller's drop_null_keys
     std::vector<std::unique_ptr<StrCol>> tstr;  // VARCHAR group columns' interning
     std::unique_ptr<StrCol> tsc;        // the tuples' interning
-    DevBuf tpack, tkeep, tcomp;
+    DevBuf tpack, tkeep, tcomp, touts;
+    std::vector<DevBuf> tout_data, tout_valid;  // group columns unpacked on the device
     // TopN on the device
     bool topn_device = false;
     DevTopn tspec{};
@@ -722,8 +723,12 @@ void build_distinct(gbagg_handle *h, const gbagg_query_desc *q) {
     const uint32_t ng = q->num_group_cols;
     if (ng && nullable(q->group_cols[0]) && !q->drop_null_keys)
         fail(GBAGG_INVALID, "COUNT(DISTINCT) over a nullable group column requires drop_null_keys");
-    if (nullable((uint32_t)x) && !only_distinct)
-        fail(GBAGG_INVALID, "a nullable COUNT(DISTINCT) argument allows only DISTINCT aggregates");
+    (void)only_distinct;
+    // NULL arguments stay in phase 1 as a group of their own (its partial
+    // states reach the other aggregates, and a group whose arguments are all
+    // NULL still appears, with COUNT(DISTINCT) 0); phase 2 counts non-NULL
+    // arguments
+    const bool keep_null_x = nullable((uint32_t)x);
 
     // phase 1: partial states, deduplicated
     std::vector<gbagg_agg> a1;
@@ -761,7 +766,7 @@ void build_distinct(gbagg_handle *h, const gbagg_query_desc *q) {
     d1.topn_col = -1;
     d1.topn_limit = 0;
     d1.row_limit = 0;
-    d1.drop_null_keys = 1;
+    d1.drop_null_keys = keep_null_x ? 0 : 1;
     h->phase1 = std::make_unique<gbagg_handle>();
     build_plan(h->phase1.get(), &d1);
     pick_kernels(h->phase1.get());
@@ -770,12 +775,13 @@ void build_distinct(gbagg_handle *h, const gbagg_query_desc *q) {
     const gbagg_handle *p1 = h->phase1.get();
     std::vector<gbagg_type> t2;
     std::vector<uint8_t> n2;
-    for (size_t c = 0; c < p1->outs.size(); c++) {
-        t2.push_back(p1->outs[c].type);
+    for (size_t c = 0; c < ng + 1 + a1.size(); c++) {
         const bool key = c <= ng;
-        const bool cnt = !key && p1->outs[c].kind == OUT_COUNT;
+        t2.push_back(key ? q->column_types[gc1[c]] : p1->outs[p1->tuple ? c - ng : c].type);
+        const bool cnt = !key && p1->outs[p1->tuple ? c - ng : c].kind == OUT_COUNT;
         const int32_t arg = key ? -1 : a1[c - ng - 1].column;
-        n2.push_back(key || cnt ? 0 : (arg >= 0 && nullable((uint32_t)arg)) ? 1 : 0);
+        if (key) n2.push_back(keep_null_x && nullable(gc1[c]) ? 1 : 0);
+        else n2.push_back(cnt ? 0 : (arg >= 0 && nullable((uint32_t)arg)) ? 1 : 0);
     }
     const uint32_t base = ng + 1;  // first partial-state column
     std::vector<gbagg_agg> a2;
@@ -784,7 +790,7 @@ void build_distinct(gbagg_handle *h, const gbagg_query_desc *q) {
         const gbagg_agg &g = q->aggs[a];
         const int32_t col = (int32_t)(base + src[a]);
         switch (g.kind) {
-        case GBAGG_AGG_COUNT_DISTINCT: a2.push_back({GBAGG_AGG_COUNT_STAR, -1}); break;
+        case GBAGG_AGG_COUNT_DISTINCT: a2.push_back({GBAGG_AGG_COUNT, (int32_t)ng}); break;
         case GBAGG_AGG_MIN:
         case GBAGG_AGG_MAX: a2.push_back({g.kind, col}); break;
         default: a2.push_back({GBAGG_AGG_SUM, col}); break;  // COUNT(*), COUNT, SUM; AVG's sum
@@ -2140,16 +2146,71 @@ void finalize(gbagg_handle *h) {
 
 // COUNT(DISTINCT): phase 1 aggregates the consumed rows; its device-resident
 // result columns become this handle's only input batch.
+void strcol_host(gbagg_handle *h, const StrCol &sc, std::vector<StrCode> &cd, std::vector<uint32_t> &words);
+
+// Multi-column keys: the result's group columns unpacked from the tuple
+// codes into device columns (tout_data / tout_valid).
+void unpack_tuples_device(gbagg_handle *h) {
+    const size_t G = h->tcols.size();
+    const uint64_t rows = h->result_rows;
+    h->tout_data.resize(G);
+    h->tout_valid.resize(G);
+    std::vector<TupleOut> outs(G);
+    for (size_t j = 0; j < G; j++) {
+        const uint32_t words = h->ttypes[j] == GBAGG_TYPE_BIGINT || h->ttypes[j] == GBAGG_TYPE_DOUBLE ? 2 : 1;
+        h->tout_data[j].ensure(std::max<uint64_t>(1, rows) * words * 4);
+        h->tout_valid[j].ensure(std::max<uint64_t>(1, rows));
+        outs[j] = TupleOut{h->tout_data[j].as<uint32_t>(), h->tout_valid[j].as<uint8_t>(), h->twordof[j], words};
+    }
+    if (!rows) return;
+    h->touts.ensure(G * sizeof(TupleOut));
+    HIPCHK(hipMemcpyAsync(h->touts.ptr, outs.data(), G * sizeof(TupleOut), hipMemcpyHostToDevice, h->stream));
+    const uint32_t grid = (uint32_t)std::min<uint64_t>((rows + 255) / 256, (uint64_t)device_info().cus * 8);
+    hipLaunchKernelGGL(k_tuple_unpack, dim3(grid), dim3(256), 0, h->stream, (const uint32_t *)h->out_data[0].ptr, rows,
+                       h->tsc->tab, (const TupleOut *)h->touts.ptr, (uint32_t)G);
+    HIPCHK(hipStreamSynchronize(h->stream));  // the table of outputs is on the host stack
+}
+
 void run_phase1(gbagg_handle *h) {
     gbagg_handle *p1 = h->phase1.get();
     finalize(p1);
-    const size_t nc = p1->outs.size();
+    // phase 1's columns: its group columns (unpacked on the device when they
+    // form a multi-column key), then its partial states
+    const size_t G = p1->tuple ? p1->tcols.size() : 0;
+    const size_t nc = p1->tuple ? G + p1->outs.size() - 1 : p1->outs.size();
+    if (p1->tuple) unpack_tuples_device(p1);
     std::vector<gbagg_column_view> v(nc);
     for (size_t c = 0; c < nc; c++) {
         v[c] = gbagg_column_view{};
-        v[c].type = p1->outs[c].type;
-        v[c].data = p1->out_data[c].ptr;
-        v[c].validity = h->nullable[c] ? p1->out_valid[c].as<uint8_t>() : nullptr;
+        if (c < G) {
+            v[c].type = p1->ttypes[c];
+            v[c].data = p1->tout_data[c].ptr;
+            v[c].validity = h->nullable[c] ? p1->tout_valid[c].as<uint8_t>() : nullptr;
+            if (v[c].type == GBAGG_TYPE_VARCHAR && c < h->group_cols.size()) {
+                // the column's interned codes: its interned dictionary, as a unique one
+                std::vector<StrCode> cd;
+                std::vector<uint32_t> words;
+                strcol_host(p1, *p1->tstr[c], cd, words);
+                h->p1_dict_off.assign(1, 0);
+                h->p1_dict_bytes.clear();
+                const char *src = (const char *)words.data();
+                for (const StrCode &e : cd) {
+                    h->p1_dict_bytes.insert(h->p1_dict_bytes.end(), src + (uint64_t)e.word * 4,
+                                            src + (uint64_t)e.word * 4 + e.len);
+                    h->p1_dict_off.push_back((uint32_t)h->p1_dict_bytes.size());
+                }
+                if (h->p1_dict_bytes.empty()) h->p1_dict_bytes.push_back(0);
+                v[c].dict_offsets = h->p1_dict_off.data();
+                v[c].dict_bytes = h->p1_dict_bytes.data();
+                v[c].dict_size = (uint32_t)cd.size();
+                v[c].flags = GBAGG_DICT_UNIQUE;
+            }
+            continue;
+        }
+        const size_t ci = p1->tuple ? c - G + 1 : c;
+        v[c].type = p1->outs[ci].type;
+        v[c].data = p1->out_data[ci].ptr;
+        v[c].validity = h->nullable[c] ? p1->out_valid[ci].as<uint8_t>() : nullptr;
         if (p1->outs[c].type == GBAGG_TYPE_VARCHAR && c < h->group_cols.size()) {
             // phase 1's canonical ids index its key dictionary: pass it on as a unique one
             KeyDict &kd = p1->kdict[c];

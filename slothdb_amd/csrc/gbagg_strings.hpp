@@ -567,4 +567,27 @@ __global__ __launch_bounds__(256) void k_tuple_pack(const TupleComp *__restrict_
     }
 }
 
+// The group columns of result rows, unpacked on the device from their tuple
+// codes (for a second aggregation phase over this result).
+struct TupleOut {
+    uint32_t *data;  // 1 or 2 words per row
+    uint8_t *valid;
+    uint32_t word;   // first word of the value in the packed row
+    uint32_t words;  // 1 or 2
+};
+
+__global__ __launch_bounds__(256) void k_tuple_unpack(const uint32_t *__restrict__ codes, uint64_t rows, StrTab t,
+                                                      const TupleOut *__restrict__ outs, uint32_t ncomp) {
+    for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < rows;
+         r += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t *tup = t.arena + t.code[codes[r]].word;
+        for (uint32_t j = 0; j < ncomp; j++) {
+            const TupleOut o = outs[j];
+            o.valid[r] = (uint8_t)((tup[j / 4] >> (8 * (j % 4))) & 0xFF);
+            o.data[r * o.words] = tup[o.word];
+            if (o.words == 2) o.data[r * 2 + 1] = tup[o.word + 1];
+        }
+    }
+}
+
 }  // namespace gbagg

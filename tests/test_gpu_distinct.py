@@ -99,3 +99,21 @@ def test_high_cardinality_first_phase():
     q2 = G.Query(q.column_types, [0], [(G.COUNT_DISTINCT, 2), (G.COUNT_STAR, None), (G.SUM, 1), (G.AVG, 3)],
                  nullable=[False] * 4)
     _check(q2, cols)
+
+
+def test_groups_whose_argument_is_always_null():
+    # every k divisible by 7 has a NULL x on all its rows: COUNT(DISTINCT x)
+    # is 0 there, and the other aggregates still count those rows
+    rng = np.random.default_rng(12)
+    n = 200_000
+    k = rng.integers(0, 1000, n).astype(np.int32)
+    x = rng.integers(0, 50, n)
+    valid = ((rng.random(n) > 0.3) & (k % 7 != 0)).astype(np.uint8)
+    cols = [G.Column(G.INTEGER, k), G.Column(G.BIGINT, x, valid), G.Column(G.BIGINT, rng.integers(-100, 100, n))]
+    for aggs in ([(G.COUNT_DISTINCT, 1)], [(G.COUNT_DISTINCT, 1), (G.COUNT_STAR, None), (G.SUM, 2), (G.COUNT, 1)]):
+        for gc in ([0], []):
+            q = G.Query([G.INTEGER, G.BIGINT, G.BIGINT], gc, aggs, nullable=[False, True, False])
+            res, _ = _check(q, cols, device=bool(gc))
+            if gc:
+                zero = {r[0] for r in res.rows() if r[1] == 0}
+                assert zero == {v for v in range(0, 1000, 7)}

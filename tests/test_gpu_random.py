@@ -109,3 +109,65 @@ def test_random_queries_against_oracle():
         assert_same_groups(_norm(got, nk), _norm(exp, nk), nk, t)
         ran += 1
     assert ran >= 120
+
+
+def test_random_topn_and_count_distinct():
+    from tests._util import assert_topn_tie_aware
+    ran = 0
+    for case in range(80):
+        seed = 5000 + case
+        q, cols, nbatch, device = _case(seed)
+        rng = np.random.default_rng(seed + 1)
+        types = q.column_types
+        nk = len(q.group_cols)
+        if case % 2 == 0:
+            # ORDER BY COUNT(*) or COUNT(c) (never NULL), either direction
+            counts = [nk + i for i, (kind, _) in enumerate(q.aggs) if kind in (G.COUNT_STAR, G.COUNT)]
+            q.topn = (int(rng.choice(counts)), bool(rng.random() < 0.3), int(rng.choice([1, 5, 50])))
+        else:
+            # COUNT(DISTINCT x) next to COUNT(*): at most one non-DOUBLE group column
+            cand = [c for c in range(len(types)) if types[c] != G.DOUBLE]
+            x = int(rng.choice(cand))
+            keys = [c for c in cand if c != x][:1] if rng.random() < 0.7 else []
+            q.group_cols = keys
+            nk = len(keys)
+            q.aggs = [(G.COUNT_DISTINCT, x)] if q.nullable[x] else [(G.COUNT_DISTINCT, x), (G.COUNT_STAR, None)]
+            q.drop_null_keys = bool(keys and q.nullable[keys[0]]) or q.drop_null_keys
+        print(f"[random] case {case} seed {seed}: rows {len(cols[0])} types {types} groups {q.group_cols} "
+              f"aggs {q.aggs} preds {q.predicates} drop {q.drop_null_keys} topn {q.topn} "
+              f"batches {nbatch} device {device}", flush=True)
+        op = G.GpuHashAggregate(q)
+        try:
+            op.Init()
+        except G.GbaggError as e:
+            print(f"[random] case {case}: plan refused: {e}", flush=True)
+            op.close()
+            continue
+        keep = []
+        for b in _split(cols, nbatch):
+            if device:
+                dc = [G.DeviceColumn(c) for c in b]
+                keep.append(dc)
+                op.consume_device(dc)
+            else:
+                op.consume(b)
+        op.finalize()
+        got = op.result().rows()
+        op.close()
+        topn = q.topn
+        q.topn = None
+        dcols = [_as_dict(c) if c.raw is not None else c for c in cols]
+        dicts = {g: dcols[c].dictionary for g, c in enumerate(q.group_cols) if dcols[c].dictionary is not None}
+        c_, v_, t, ov = O.groupby_exact(q, dcols)
+        assert not ov
+        exp = _norm(oracle_rows(c_, v_, t, dicts), nk)
+        got = _norm(got, nk)
+        if topn:
+            col, asc, k = topn
+            assert len(got) == min(k, len(exp))
+            if got:
+                assert_topn_tie_aware(got, exp, order_col=col, k=min(k, len(exp)), types=t, descending=not asc)
+        else:
+            assert_same_groups(got, exp, nk, t)
+        ran += 1
+    assert ran >= 60
