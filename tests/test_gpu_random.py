@@ -171,3 +171,62 @@ def test_random_topn_and_count_distinct():
             assert_same_groups(got, exp, nk, t)
         ran += 1
     assert ran >= 60
+
+
+def test_random_params_and_row_limit():
+    # the same random queries under other knobs: DIRECT forced, generic row
+    # kernels only, 4-bit string/tuple hashes (shared hashes everywhere), and
+    # a bare LIMIT (any `row_limit` groups: a subset of the oracle's groups)
+    ran = 0
+    for case in range(60):
+        seed = 9000 + case
+        q, cols, nbatch, device = _case(seed)
+        rng = np.random.default_rng(seed + 7)
+        params = {}
+        knob = case % 4
+        if knob == 0:
+            q.strategy = G.DIRECT
+        elif knob == 1:
+            params["static_shapes"] = 0
+        elif knob == 2:
+            params["str_hash_bits"] = 4
+        else:
+            q.row_limit = int(rng.choice([1, 7, 100]))
+        print(f"[random] case {case} seed {seed}: rows {len(cols[0])} types {q.column_types} groups {q.group_cols} "
+              f"aggs {q.aggs} preds {q.predicates} drop {q.drop_null_keys} strategy {q.strategy} params {params} "
+              f"limit {q.row_limit} batches {nbatch} device {device}", flush=True)
+        op = G.GpuHashAggregate(q, params)
+        try:
+            op.Init()
+        except G.GbaggError as e:
+            print(f"[random] case {case}: plan refused: {e}", flush=True)
+            op.close()
+            continue
+        keep = []
+        for b in _split(cols, nbatch):
+            if device:
+                dc = [G.DeviceColumn(c) for c in b]
+                keep.append(dc)
+                op.consume_device(dc)
+            else:
+                op.consume(b)
+        op.finalize()
+        got = op.result().rows()
+        op.close()
+        limit = q.row_limit
+        q.row_limit = 0
+        dcols = [_as_dict(c) if c.raw is not None else c for c in cols]
+        dicts = {g: dcols[c].dictionary for g, c in enumerate(q.group_cols) if dcols[c].dictionary is not None}
+        c_, v_, t, ov = O.groupby_exact(q, dcols)
+        assert not ov
+        nk = len(q.group_cols)
+        exp = _norm(oracle_rows(c_, v_, t, dicts), nk)
+        got = _norm(got, nk)
+        if limit:
+            assert len(got) == min(limit, len(exp))
+            by_key = {r[:nk]: r for r in exp}
+            assert_same_groups(got, [by_key[r[:nk]] for r in got], nk, t)
+        else:
+            assert_same_groups(got, exp, nk, t)
+        ran += 1
+    assert ran >= 45
