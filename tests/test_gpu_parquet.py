@@ -183,3 +183,42 @@ def test_parquet_row_groups_into_the_operator():
         assert_same_groups(op.result().rows(), exp, 1, ty)
         op.close()
     f.close()
+
+
+def test_random_files_match_pyarrow():
+    # random flat tables: column types, NULL rates, cardinalities (bit widths
+    # of dictionary indices from 0 to 20), data page sizes, row group sizes,
+    # page versions and dictionary on/off per column
+    for case in range(24):
+        rng = np.random.default_rng(300 + case)
+        n = int(rng.integers(1, 60_000))
+        fields, arrays, use_dict = [], {}, []
+        for j in range(int(rng.integers(1, 6))):
+            kind = int(rng.integers(0, 4))
+            card = int(rng.choice([1, 2, 3, 200, 70_000]))
+            null_p = float(rng.choice([0.0, 0.0, 0.01, 0.5, 1.0]))
+            mask = rng.random(n) < null_p if null_p else None
+            name = f"c{j}"
+            if kind == 0:
+                a = pa.array(rng.integers(-card, card + 1, n).astype(np.int32), mask=mask, type=pa.int32())
+            elif kind == 1:
+                a = pa.array(rng.integers(0, card, n) * 7_777_777_777, mask=mask, type=pa.int64())
+            elif kind == 2:
+                a = pa.array(rng.integers(0, card, n) / 8.0, mask=mask, type=pa.float64())
+            else:
+                a = pa.array([f"v{v}" + "q" * int(v % 13) for v in rng.integers(0, card, n)], mask=mask,
+                             type=pa.string())
+            nullable = mask is not None or rng.random() < 0.5
+            fields.append(pa.field(name, a.type, nullable=nullable))
+            arrays[name] = a
+            if rng.random() < 0.7:
+                use_dict.append(name)
+        t = pa.table(arrays, schema=pa.schema(fields))
+        kw = dict(row_group_size=int(rng.choice([1_000, 30_000, 1_000_000])),
+                  data_page_size=int(rng.choice([512, 8 * 1024, 1 << 20])),
+                  data_page_version=str(rng.choice(["1.0", "2.0"])),
+                  dictionary_pagesize_limit=int(rng.choice([256, 64 * 1024, 1 << 20])),
+                  use_dictionary=use_dict)
+        print(f"[parquet] random case {case}: rows {n} schema {t.schema.types} {kw}", flush=True)
+        image = _write(t, **kw)
+        _check_file(image, t, kw["row_group_size"], resident=bool(case % 2))
