@@ -230,3 +230,32 @@ def test_random_params_and_row_limit():
             assert_same_groups(got, exp, nk, t)
         ran += 1
     assert ran >= 45
+
+
+def test_random_large_queries():
+    # fewer, larger cases: 1-1.5M rows, so RADIX partitions, heavy hitters and
+    # the two interning stages run at size
+    for case in range(6):
+        seed = 12000 + case
+        q, cols, nbatch, device = _case(seed)
+        rng = np.random.default_rng(seed)
+        n = int(rng.integers(1_000_000, 1_500_000))
+        cols = [_column(rng, t, n, 0.05 if q.nullable[i] else 0.0, bool(case % 2)) for i, t in enumerate(q.column_types)]
+        print(f"[random] large case {case} seed {seed}: rows {n} types {q.column_types} groups {q.group_cols} "
+              f"aggs {q.aggs} preds {q.predicates} drop {q.drop_null_keys} strategy {q.strategy}", flush=True)
+        op = G.GpuHashAggregate(q)
+        op.Init()
+        keep = []
+        for b in _split(cols, nbatch):
+            dc = [G.DeviceColumn(c) for c in b]
+            keep.append(dc)
+            op.consume_device(dc)
+        op.finalize()
+        got = op.result().rows()
+        op.close()
+        dcols = [_as_dict(c) if c.raw is not None else c for c in cols]
+        dicts = {g: dcols[c].dictionary for g, c in enumerate(q.group_cols) if dcols[c].dictionary is not None}
+        c_, v_, t, ov = O.groupby_exact(q, dcols)
+        assert not ov
+        nk = len(q.group_cols)
+        assert_same_groups(_norm(got, nk), _norm(oracle_rows(c_, v_, t, dicts), nk), nk, t)
