@@ -55,9 +55,14 @@ def main():
     ap.add_argument("--batch-queries", type=int, default=512)
     ap.add_argument("--no-strings", action="store_true")
     ap.add_argument("--no-parquet", action="store_true")
+    ap.add_argument("--batch-split", action="store_true",
+                    help="time the batched mix split by the queries' distinct keys, then exit")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.batch_split:
+        batch_split()
         return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (use --gpus 1)", file=sys.stderr)
@@ -283,6 +288,37 @@ def bench_strings(runs=5):
             "rows": n, "string_bytes": nbytes, "distinct": info_s["strings_distinct"],
             "intern_ms": ms, "intern_rows_per_s": n / (ms * 1e-3), "intern_GB/s": io_bytes / (ms * 1e-3) / 1e9,
             "query_ms_strings": t_str, "query_ms_codes": t_codes, "top_counts_equal": same}
+
+
+def batch_split(nq=512):
+    """The batched mix, and its queries split by distinct keys, each as a
+    batch of its own (autotuned, best of 5): where the batched time goes."""
+    from slothdb_amd import datagen, gbagg as G
+    specs = datagen.batch_specs(nq)
+    host = datagen.batch_workload(nq=nq)
+    sync = G.lib().gbagg_device_synchronize
+    out = []
+    for name, sel in [("all", lambda d: True), ("distinct<=10000", lambda d: d <= 10_000),
+                      ("distinct>10000", lambda d: d > 10_000), ("distinct>100000", lambda d: d > 100_000)]:
+        dev = [(q, [G.DeviceColumn(c) for c in cols]) for (q, cols), sp in zip(host, specs) if sel(sp[1])]
+        rows = sum(c[0].n for _, c in dev)
+        b = G.GpuBatch()
+        b.set_queries(dev)
+        b.autotune()
+        best = 1e9
+        for _ in range(5):
+            sync()
+            t = time.perf_counter()
+            b.run()
+            sync()
+            best = min(best, time.perf_counter() - t)
+        out.append({"queries": name, "count": len(dev), "rows": rows, "batched_ms": best * 1e3,
+                    "rows_per_s": rows / best, "tune": b.info()["tune"]})
+        b.close()
+        for _, cs in dev:
+            for c in cs:
+                c.free()
+    print(json.dumps({"batch_split": out}), flush=True)
 
 
 def bench_batched(nq):
