@@ -155,6 +155,38 @@ static void test_emp_two_chunks() {
     CHECK((got["Marketing"] == std::vector<std::string>{"2", "190", "95.000000", "95", "95"}));
 }
 
+static void test_three_column_key_chunks() {
+    // GROUP BY (dept VARCHAR, grade INTEGER, bonus DOUBLE): a key of three
+    // columns (interned row tuples), strings from two chunks, a NULL grade
+    PhysicalGpuHashAggregate op({LT::VARCHAR, LT::INTEGER, LT::DOUBLE, LT::BIGINT}, {0, 1, 2},
+                                {{"count_star", -1}, {"sum", 3}}, {}, {false, true, false, false});
+    op.Init();
+    const char *dept[] = {"Eng", "Eng", "Ops", "Eng", "Ops", "Eng"};
+    const int32_t grade[] = {1, 1, 2, 1, 2, 3};
+    const double bonus[] = {0.5, 0.5, 0.0, -0.0, 0.0, 0.5};
+    for (int rep = 0; rep < 2; rep++) {
+        DataChunk c;
+        c.Initialize({LT::VARCHAR, LT::INTEGER, LT::DOUBLE, LT::BIGINT});
+        for (int i = 0; i < 6; i++) {
+            c.data[0].Strings()[i] = dept[i];
+            c.data[1].GetData<int32_t>()[i] = grade[i];
+            c.data[2].GetData<double>()[i] = bonus[i];
+            c.data[3].GetData<int64_t>()[i] = 10 * i + rep;
+        }
+        c.data[1].Validity().SetInvalid(5);  // (Eng, NULL, 0.5)
+        c.SetCardinality(6);
+        op.ConsumeChunk(c);
+    }
+    std::map<std::string, std::vector<std::string>> got;
+    for (auto &r : drain(op)) got[r[0] + "|" + r[1] + "|" + r[2]] = {r[3], r[4]};
+    // (Eng,1,0.5): rows 0,1 twice; (Ops,2,0): rows 2,4 twice; (Eng,1,-0.0 = 0): row 3 twice
+    CHECK(got.size() == 4);
+    CHECK((got["Eng|1|0.500000"] == std::vector<std::string>{"4", "22"}));
+    CHECK((got["Ops|2|0.000000"] == std::vector<std::string>{"4", "122"}));
+    CHECK((got["Eng|1|0.000000"] == std::vector<std::string>{"2", "61"}));
+    CHECK((got["Eng|NULL|0.500000"] == std::vector<std::string>{"2", "101"}));
+}
+
 static void test_emp_topn() {
     PhysicalGpuHashAggregate op({LT::VARCHAR, LT::INTEGER}, {0}, {{"count_star", -1}}, {}, {false, false});
     op.SetTopNHint(1, false, 3);  // ORDER BY cnt DESC
@@ -300,6 +332,7 @@ int main(int argc, char **argv) {
         test_emp_scalar();
         test_emp_group_by();
         test_emp_two_chunks();
+        test_three_column_key_chunks();
         test_emp_topn();
         test_sum_1_to_10000_rowgroup();
         test_50k_five_depts();
