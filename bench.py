@@ -55,11 +55,17 @@ def main():
     ap.add_argument("--batch-queries", type=int, default=512)
     ap.add_argument("--no-strings", action="store_true")
     ap.add_argument("--no-parquet", action="store_true")
+    ap.add_argument("--cpu-calibration", action="store_true",
+                    help="time the CPU restatement against the reference's own RadixCountAgg "
+                         "(oracle/_ref, built from the reference sources), then exit")
     ap.add_argument("--batch-split", action="store_true",
                     help="time the batched mix split by the queries' distinct keys, then exit")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.cpu_calibration:
+        cpu_calibration(args.rows, args.keys, args.cpu_threads)
         return
     if args.batch_split:
         batch_split()
@@ -288,6 +294,38 @@ def bench_strings(runs=5):
             "rows": n, "string_bytes": nbytes, "distinct": info_s["strings_distinct"],
             "intern_ms": ms, "intern_rows_per_s": n / (ms * 1e-3), "intern_GB/s": io_bytes / (ms * 1e-3) / 1e9,
             "query_ms_strings": t_str, "query_ms_codes": t_codes, "top_counts_equal": same}
+
+
+def cpu_calibration(n, nkeys, threads):
+    """COUNT(*) GROUP BY k over the high-cardinality keys: the oracle's
+    restatement of RadixCountAgg (radix_count_agg.cpp:17-106) against the
+    reference's own class compiled into oracle/_ref, same threads, same keys;
+    best of 3 each. Their outputs must be the same (key, count) set."""
+    import ctypes as C
+    import numpy as np
+    from oracle import oracle as O
+    from slothdb_amd import datagen
+    L = O.ref()
+    if L is None:
+        print(json.dumps({"cpu_calibration": None, "reason": "oracle/_ref not built"}), flush=True)
+        return
+    keys = np.ascontiguousarray(datagen.highcard(n=n, nkeys=nkeys, columns=("k",))["k"], np.int64)
+    cap = min(n, nkeys) + 16
+    ok, oc, m = np.empty(cap, np.int64), np.empty(cap, np.int64), C.c_uint64()
+    t_ref, t_port = [], []
+    for _ in range(3):
+        t = time.perf_counter()
+        L.ref_radix_count(keys.ctypes.data, len(keys), threads, 0, ok.ctypes.data, oc.ctypes.data, cap, C.byref(m))
+        t_ref.append(time.perf_counter() - t)
+        t = time.perf_counter()
+        pk, pc = O.port_radix_count(keys, threads)
+        t_port.append(time.perf_counter() - t)
+    ref = dict(zip(ok[:m.value].tolist(), oc[:m.value].tolist()))
+    same = ref == dict(zip(pk.tolist(), pc.tolist()))
+    print(json.dumps({"cpu_calibration": {
+        "workload": f"COUNT(*) GROUP BY k, {n} rows, high-cardinality keys", "threads": threads,
+        "reference_radix_count_s": min(t_ref), "port_radix_count_s": min(t_port),
+        "port_over_reference": min(t_port) / min(t_ref), "same_result": same}}), flush=True)
 
 
 def batch_split(nq=512):
