@@ -345,6 +345,14 @@ typedef struct {
     pred_rt *preds;
 } exact_ctx;
 
+/* Byte order of two entries of a VARCHAR column's dictionary (memcmp, then
+ * the shorter string first). */
+static int dict_cmp(const gbagg_column_view *c, uint32_t a, uint32_t b) {
+    const uint32_t la = c->dict_offsets[a + 1] - c->dict_offsets[a], lb = c->dict_offsets[b + 1] - c->dict_offsets[b];
+    const int m = memcmp(c->dict_bytes + c->dict_offsets[a], c->dict_bytes + c->dict_offsets[b], la < lb ? la : lb);
+    return m ? m : (la < lb ? -1 : la > lb ? 1 : 0);
+}
+
 /* Builds the group key of row r; returns 0 if the row is dropped. */
 static inline int row_key(const exact_ctx *x, uint64_t r, gkey *k) {
     const gbagg_query_desc *q = x->q;
@@ -402,6 +410,13 @@ static inline void row_update(const exact_ctx *x, gtable *t, uint64_t g, uint64_
                 if (!t->a.fhas[i] || v > t->a.fmax[i]) t->a.fmax[i] = v;
                 t->a.fhas[i] = 1;
             }
+        } else if (c->type == GBAGG_TYPE_VARCHAR) {
+            /* MIN/MAX of strings: byte order of the dictionary entries
+             * (the generic path's Value comparison) */
+            const int64_t v = col_i64(c, r);
+            int64_t *cur = ag->kind == GBAGG_AGG_MIN ? &t->a.imin[i] : &t->a.imax[i];
+            const int d = t->a.cnt[i] == 1 ? 0 : dict_cmp(c, (uint32_t)v, (uint32_t)*cur);
+            if (t->a.cnt[i] == 1 || (ag->kind == GBAGG_AGG_MIN ? d < 0 : d > 0)) *cur = v;
         } else {
             int64_t v = col_i64(c, r);
             if (ag->kind == GBAGG_AGG_SUM || ag->kind == GBAGG_AGG_AVG) t->a.isum[i] += v;

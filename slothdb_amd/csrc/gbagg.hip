@@ -269,6 +269,8 @@ struct Batch {
     uint64_t nrows = 0;
     const uint32_t *str_off[MAXU] = {};  // GBAGG_STR_RAW columns: device offsets and bytes
     const uint8_t *str_bytes[MAXU] = {};
+    std::vector<uint32_t> odoff[MAXU];   // ordered VARCHAR columns given as codes: the batch's dictionary
+    std::vector<char> odbytes[MAXU];
     // multi-column keys: the batch's group columns (device), and for VARCHAR
     // columns given as codes a host copy of the batch's dictionary
     struct TupleIn {
@@ -317,6 +319,7 @@ struct gbagg_handle {
     uint64_t wide_seed0 = 0;       // KM_WIDE: first seed of wide_mix
     // VARCHAR columns given as strings (GBAGG_STR_RAW), interned at finalize
     std::vector<uint8_t> str_mode;  // per column: 0 no batch yet, 1 dictionary codes, 2 strings
+    std::vector<uint8_t> str_ordered;  // per column: MIN/MAX over it, so its codes follow byte order
     std::vector<std::unique_ptr<StrCol>> strs;
     uint64_t str_key_mask = ~0ull;  // stored string-hash bits ("str_hash_bits")
     // multi-column keys (more than two columns, a DOUBLE column, or a nullable
@@ -575,11 +578,16 @@ void build_plan(gbagg_handle *h, const gbagg_query_desc *q) {
                 break;
             case GBAGG_AGG_MIN:
             case GBAGG_AGG_MAX: {
-                if (t == GBAGG_TYPE_VARCHAR) fail(GBAGG_INVALID, "MIN/MAX over VARCHAR is not supported");
                 bool mn = ag.kind == GBAGG_AGG_MIN;
                 oc.cnt_acc = cnt;
                 oc.type = t;
-                if (t == GBAGG_TYPE_DOUBLE) {
+                if (t == GBAGG_TYPE_VARCHAR) {
+                    // the column is interned with codes in byte order: MIN/MAX of codes
+                    if (h->str_ordered.size() != h->col_types.size()) h->str_ordered.assign(h->col_types.size(), 0);
+                    h->str_ordered[ag.column] = 1;
+                    oc.val_acc = add_acc(h, mn ? ACC_MIN_I : ACC_MAX_I, u);
+                    oc.kind = OUT_MM_I32;
+                } else if (t == GBAGG_TYPE_DOUBLE) {
                     oc.val_acc = add_acc(h, mn ? ACC_MIN_F : ACC_MAX_F, u);
                     oc.kind = OUT_MM_F;
                 } else {
@@ -717,6 +725,10 @@ void build_distinct(gbagg_handle *h, const gbagg_query_desc *q) {
         if (x >= 0 && g.column != x) fail(GBAGG_INVALID, "all COUNT(DISTINCT) aggregates must share one column");
         x = g.column;
     }
+    for (uint32_t a = 0; a < q->num_aggs; a++)
+        if ((q->aggs[a].kind == GBAGG_AGG_MIN || q->aggs[a].kind == GBAGG_AGG_MAX) &&
+            q->column_types[q->aggs[a].column] == GBAGG_TYPE_VARCHAR)
+            fail(GBAGG_INVALID, "MIN/MAX over VARCHAR next to COUNT(DISTINCT) is not supported");
     const gbagg_type xt = q->column_types[x];
     if (xt != GBAGG_TYPE_INTEGER && xt != GBAGG_TYPE_BIGINT && xt != GBAGG_TYPE_VARCHAR)
         fail(GBAGG_INVALID, "COUNT(DISTINCT) supports INTEGER, BIGINT and VARCHAR arguments");
@@ -919,10 +931,19 @@ void add_batch(gbagg_handle *h, const gbagg_column_view *cols, uint64_t n, bool 
         const bool strs = (c.flags & GBAGG_STR_RAW) != 0;
         if (strs && c.type != GBAGG_TYPE_VARCHAR) fail(GBAGG_INVALID, "GBAGG_STR_RAW is for VARCHAR columns");
         if (strs && (c.dict_offsets || c.dict_size)) fail(GBAGG_INVALID, "GBAGG_STR_RAW columns carry no dictionary");
+        const bool ordered = (size_t)h->ucol[u] < h->str_ordered.size() && h->str_ordered[h->ucol[u]];
         if (c.type == GBAGG_TYPE_VARCHAR) {
-            // a column with a batch of strings is interned at finalize
+            // a column with a batch of strings, or under MIN/MAX, is interned at finalize
             uint8_t &m = modes[h->ucol[u]];
-            m = strs ? 2 : std::max<uint8_t>(m, 1);
+            m = strs || ordered ? 2 : std::max<uint8_t>(m, 1);
+        }
+        if (ordered && !strs && n) {
+            if (c.dict_size && (!c.dict_offsets || !c.dict_bytes)) fail(GBAGG_INVALID, "missing dictionary");
+            b.odoff[u].assign(1, 0);
+            if (c.dict_size) {
+                b.odoff[u].assign(c.dict_offsets, c.dict_offsets + c.dict_size + 1);
+                b.odbytes[u].assign(c.dict_bytes, c.dict_bytes + c.dict_offsets[c.dict_size]);
+            }
         }
         size_t w = type_width(c.type);
         if (strs) {
@@ -1822,6 +1843,8 @@ void intern_run(gbagg_handle *h, StrCol &sc, const std::vector<StrSeg> &segs, ui
     sc.tab = t;
 }
 
+void strcol_host(gbagg_handle *h, const StrCol &sc, std::vector<StrCode> &cd, std::vector<uint32_t> &words);
+
 void intern_column(gbagg_handle *h, int u) {
     const DeviceInfo &di = device_info();
     hipStream_t s = h->stream;
@@ -1865,8 +1888,22 @@ void intern_column(gbagg_handle *h, int u) {
     for (size_t i = 0; i < h->batches.size(); i++) {
         const Batch &b = h->batches[i];
         if (!b.nrows) continue;
+        const bool ordered = col < h->str_ordered.size() && h->str_ordered[col];
         if (b.str_off[u]) {
             segs.push_back(StrSeg{b.str_off[u], b.str_bytes[u], b.db.valid[u], nullptr, nullptr, total, b.nrows, 0, 0});
+        } else if (key_g < 0 && ordered) {
+            // codes under MIN/MAX: through the batch's own dictionary
+            Batch &bm = h->batches[i];
+            auto off = std::make_unique<DevBuf>(), by = std::make_unique<DevBuf>();
+            off->ensure(bm.odoff[u].size() * 4);
+            by->ensure(bm.odbytes[u].size() + 16);
+            HIPCHK(hipMemcpyAsync(off->ptr, bm.odoff[u].data(), bm.odoff[u].size() * 4, hipMemcpyHostToDevice, s));
+            if (!bm.odbytes[u].empty())
+                HIPCHK(hipMemcpyAsync(by->ptr, bm.odbytes[u].data(), bm.odbytes[u].size(), hipMemcpyHostToDevice, s));
+            segs.push_back(StrSeg{off->as<uint32_t>(), by->as<uint8_t>(), b.db.valid[u], (const uint32_t *)b.db.data[u],
+                                  nullptr, total, b.nrows, (uint32_t)(bm.odoff[u].size() - 1), 0});
+            bm.owned.push_back(std::move(off));
+            bm.owned.push_back(std::move(by));
         } else if (key_g >= 0) {
             segs.push_back(StrSeg{sc.coff.as<uint32_t>(), sc.cbytes.as<uint8_t>(), b.db.valid[u],
                                   (const uint32_t *)b.db.data[u], b.db.remap[key_g], total, b.nrows,
@@ -1880,6 +1917,33 @@ void intern_column(gbagg_handle *h, int u) {
     if (total > (1ull << 30)) fail(GBAGG_INVALID, "more than 2^30 rows in one aggregate");
     sc.ncodes = 0;
     intern_run(h, sc, segs, total);
+    if (col < h->str_ordered.size() && h->str_ordered[col] && sc.ncodes) {
+        // codes in byte order of their strings: rank on the host, relabel rows
+        // and code records on the device
+        std::vector<StrCode> cd;
+        std::vector<uint32_t> words;
+        strcol_host(h, sc, cd, words);
+        const char *a0 = (const char *)words.data();
+        std::vector<uint32_t> order(cd.size()), rank(cd.size());
+        for (uint32_t c = 0; c < order.size(); c++) order[c] = c;
+        std::sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
+            return std::string_view(a0 + (uint64_t)cd[x].word * 4, cd[x].len) <
+                   std::string_view(a0 + (uint64_t)cd[y].word * 4, cd[y].len);
+        });
+        std::vector<StrCode> sorted(cd.size());
+        for (uint32_t r = 0; r < order.size(); r++) {
+            rank[order[r]] = r;
+            sorted[r] = cd[order[r]];
+        }
+        auto drank = std::make_unique<DevBuf>();
+        drank->ensure(rank.size() * 4);
+        HIPCHK(hipMemcpyAsync(drank->ptr, rank.data(), rank.size() * 4, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(sc.code.ptr, sorted.data(), sorted.size() * sizeof(StrCode), hipMemcpyHostToDevice, s));
+        const uint32_t grid = (uint32_t)std::min<uint64_t>((total + 255) / 256, (uint64_t)di.cus * 8);
+        hipLaunchKernelGGL(k_str_relabel, dim3(grid), dim3(256), 0, s, sc.codes.as<uint32_t>(), total,
+                           (const uint32_t *)drank->ptr, sc.ncodes);
+        HIPCHK(hipStreamSynchronize(s));  // rank and sorted are host vectors
+    }
     const StrTab t = sc.tab;
     const uint32_t nc = (uint32_t)sc.ncodes;
     // predicates on this column: one match per distinct string
@@ -1952,19 +2016,15 @@ void host_dict(gbagg_handle *h, KeyDict &kd) {
     kd.dev = nullptr;
 }
 
-// Strings of the result rows of an interned group column, gathered on the
-// device: only the rows' own strings cross to the host.
-void gather_result_strings(gbagg_handle *h, size_t g, uint64_t n) {
-    const StrCol &sc = *h->kdict[g].dev;
+// Strings of result rows whose cells are codes of an interned column,
+// gathered on the device: only the rows' own strings cross to the host.
+void gather_strings(gbagg_handle *h, const StrCol &sc, const uint32_t *ids, const uint8_t *valid, uint64_t n,
+                    std::vector<uint32_t> &off, std::vector<char> &bytes) {
     const DeviceInfo &di = device_info();
     hipStream_t s = h->stream;
-    auto &off = h->res_str_off[g];
-    auto &bytes = h->res_str_bytes[g];
     off.assign(n + 1, 0);
     bytes.clear();
     if (!n || !sc.ncodes) return;
-    const uint32_t *ids = h->out_data[g].as<uint32_t>();
-    const uint8_t *valid = h->out_valid[g].as<uint8_t>();
     const uint32_t grid = (uint32_t)std::min<uint64_t>((n + 255) / 256, (uint64_t)di.cus * 8);
     DevBuf dlen, doff, dbytes;
     dlen.ensure(n * 4);
@@ -1990,6 +2050,11 @@ void gather_result_strings(gbagg_handle *h, size_t g, uint64_t n) {
                        (const uint64_t *)doff.ptr, dbytes.as<uint8_t>());
     HIPCHK(hipMemcpyAsync(bytes.data(), dbytes.ptr, tot, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
+}
+
+void gather_result_strings(gbagg_handle *h, size_t g, uint64_t n) {
+    gather_strings(h, *h->kdict[g].dev, h->out_data[g].as<uint32_t>(), h->out_valid[g].as<uint8_t>(), n,
+                   h->res_str_off[g], h->res_str_bytes[g]);
 }
 
 // Multi-column keys: VARCHAR group columns interned (each over all its
@@ -2399,6 +2464,8 @@ void expand_tuples(gbagg_handle *h, uint64_t rows) {
     for (size_t c = 1; c < nin; c++) {
         data[G + c - 1] = std::move(h->res_data[c]);
         valid[G + c - 1] = std::move(h->res_valid[c]);
+        soff[G + c - 1] = std::move(h->res_str_off[c]);
+        sbytes[G + c - 1] = std::move(h->res_str_bytes[c]);
         types[G + c - 1] = h->outs[c].type;
     }
     h->res_data = std::move(data);
@@ -2463,6 +2530,15 @@ void download_result(gbagg_handle *h) {
             }
         }
         off[rows] = (uint32_t)bytes.size();
+    }
+    // MIN/MAX over VARCHAR: codes of the argument's interned column
+    for (size_t c = h->group_cols.size(); c < nc; c++) {
+        if (h->outs[c].type != GBAGG_TYPE_VARCHAR) continue;
+        const uint32_t col = (uint32_t)h->aggs[c - h->group_cols.size()].column;
+        if (col >= h->strs.size() || !h->strs[col]) fail(GBAGG_ERROR, "VARCHAR MIN/MAX: column not interned");
+        gather_strings(h, *h->strs[col], h->out_data[c].as<uint32_t>(), h->out_valid[c].as<uint8_t>(), n,
+                       h->res_str_off[c], h->res_str_bytes[c]);
+        if (scalar_empty) h->res_str_off[c].assign(rows + 1, 0);
     }
     // TopN that could not run on the device: sort the full result here.
     if (h->topn_col >= 0 && h->topn_limit > 0 && !h->topn_device && rows > 0) {

@@ -479,6 +479,18 @@ __global__ __launch_bounds__(256) void k_str_gather(StrTab t, const uint32_t *id
     }
 }
 
+// Codes relabelled in the byte order of their strings (rank[old] = new), so
+// that MIN/MAX over codes is MIN/MAX over the strings; dropped rows keep
+// STR_DROP.
+__global__ __launch_bounds__(256) void k_str_relabel(uint32_t *codes, uint64_t total, const uint32_t *rank,
+                                                     uint64_t ncodes) {
+    for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total;
+         g += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t c = codes[g];
+        if (c < ncodes) codes[g] = rank[c];
+    }
+}
+
 // Byte i of code c's string.
 __device__ __forceinline__ uint32_t str_byte(const uint32_t *a, uint32_t i) {
     return (a[i / 4] >> (8 * (i % 4))) & 0xFFu;

@@ -73,6 +73,8 @@ def test_valid_plans_create_without_device():
     assert _plan(G.Query([G.INTEGER, G.VARCHAR, G.DOUBLE], [0],
                          [(G.COUNT_DISTINCT, 1), (G.AVG, 2), (G.COUNT_STAR, None)], nullable=[0, 0, 1]))[0] == G.OK
     assert _plan(G.Query([G.BIGINT], [], [(G.COUNT_DISTINCT, 0)]))[0] == G.OK
+    # MIN/MAX over VARCHAR (codes relabelled in byte order)
+    assert _plan(G.Query([G.INTEGER, G.VARCHAR], [0], [(G.MIN, 1), (G.MAX, 1)]))[0] == G.OK
     # a nullable argument next to other aggregates (its NULL rows form a phase-1 group)
     assert _plan(G.Query([G.INTEGER] * 3, [0], [(G.COUNT_DISTINCT, 1), (G.SUM, 2)], nullable=[0, 1, 0]))[0] == G.OK
     # 16-byte keys: two BIGINT columns, BIGINT with a VARCHAR or INTEGER column
@@ -87,7 +89,7 @@ def test_valid_plans_create_without_device():
     (G.Query([G.INTEGER] * 3, [0, 1, 2], [(G.COUNT_STAR, None)], topn=(1, False, 3)), "TopN on a group column"),
     (G.Query([G.DOUBLE, G.BIGINT], [0], [(G.COUNT_DISTINCT, 1)]), "DOUBLE group column"),
     (G.Query([G.INTEGER, G.VARCHAR], [0], [(G.SUM, 1)]), "numeric"),
-    (G.Query([G.INTEGER, G.VARCHAR], [0], [(G.MIN, 1)]), "VARCHAR"),
+    (G.Query([G.INTEGER, G.VARCHAR, G.INTEGER], [0], [(G.MIN, 1), (G.COUNT_DISTINCT, 2)]), "MIN/MAX over VARCHAR"),
     (G.Query([G.INTEGER], [0], [(G.COUNT_STAR, None)], predicates=[(0, G.LIKE, "x")]), "LIKE"),
     (G.Query([G.INTEGER], [0], [(G.SUM, 3)]), "out of range"),
     (G.Query([G.INTEGER] * 10, [0], [(G.SUM, c) for c in range(1, 10)]), "8 distinct"),

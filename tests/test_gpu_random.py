@@ -51,7 +51,7 @@ def _case(seed):
     aggs = [(G.COUNT_STAR, None)]
     for _ in range(int(rng.integers(0, 4))):
         kind = int(rng.choice([G.COUNT, G.SUM, G.AVG, G.MIN, G.MAX]))
-        col = int(rng.choice(range(ncol) if kind == G.COUNT else numeric))
+        col = int(rng.choice(range(ncol) if kind in (G.COUNT, G.MIN, G.MAX) else numeric))
         aggs.append((kind, col))
     preds = []
     for _ in range(int(rng.integers(0, 3))):
@@ -66,6 +66,16 @@ def _case(seed):
     q = G.Query(types, gcols, aggs, predicates=preds, nullable=[p > 0 for p in nulls],
                 drop_null_keys=bool(rng.random() < 0.3), strategy=strategy)
     return q, cols, int(rng.integers(1, 4)), bool(rng.random() < 0.5)
+
+
+def _dicts(q, dcols):
+    """Dictionaries of the output columns holding VARCHAR codes in the oracle's
+    table: VARCHAR group columns and MIN/MAX over VARCHAR."""
+    d = {g: dcols[c].dictionary for g, c in enumerate(q.group_cols) if dcols[c].dictionary is not None}
+    for i, (kind, col) in enumerate(q.aggs):
+        if kind in (G.MIN, G.MAX) and dcols[col].dictionary is not None:
+            d[len(q.group_cols) + i] = dcols[col].dictionary
+    return d
 
 
 def _norm(rows, nk):
@@ -101,7 +111,7 @@ def test_random_queries_against_oracle():
         got = op.result().rows()
         op.close()
         dcols = [_as_dict(c) if c.raw is not None else c for c in cols]
-        dicts = {g: dcols[c].dictionary for g, c in enumerate(q.group_cols) if dcols[c].dictionary is not None}
+        dicts = _dicts(q, dcols)
         c_, v_, t, ov = O.groupby_exact(q, dcols)
         assert not ov
         exp = oracle_rows(c_, v_, t, dicts)
@@ -157,7 +167,7 @@ def test_random_topn_and_count_distinct():
         topn = q.topn
         q.topn = None
         dcols = [_as_dict(c) if c.raw is not None else c for c in cols]
-        dicts = {g: dcols[c].dictionary for g, c in enumerate(q.group_cols) if dcols[c].dictionary is not None}
+        dicts = _dicts(q, dcols)
         c_, v_, t, ov = O.groupby_exact(q, dcols)
         assert not ov
         exp = _norm(oracle_rows(c_, v_, t, dicts), nk)
@@ -216,7 +226,7 @@ def test_random_params_and_row_limit():
         limit = q.row_limit
         q.row_limit = 0
         dcols = [_as_dict(c) if c.raw is not None else c for c in cols]
-        dicts = {g: dcols[c].dictionary for g, c in enumerate(q.group_cols) if dcols[c].dictionary is not None}
+        dicts = _dicts(q, dcols)
         c_, v_, t, ov = O.groupby_exact(q, dcols)
         assert not ov
         nk = len(q.group_cols)
@@ -254,7 +264,7 @@ def test_random_large_queries():
         got = op.result().rows()
         op.close()
         dcols = [_as_dict(c) if c.raw is not None else c for c in cols]
-        dicts = {g: dcols[c].dictionary for g, c in enumerate(q.group_cols) if dcols[c].dictionary is not None}
+        dicts = _dicts(q, dcols)
         c_, v_, t, ov = O.groupby_exact(q, dcols)
         assert not ov
         nk = len(q.group_cols)

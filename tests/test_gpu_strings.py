@@ -251,3 +251,38 @@ def test_empty_null_and_invalid_batches():
         op.consume([G.Column.strings(["a", "b", "c"]), G.Column(G.INTEGER, np.zeros(4, np.uint32).view(np.int32)[:3],
                                                                 raw=np.zeros(1, np.uint8))])
     op.close()
+
+
+def test_varchar_min_max():
+    # MIN/MAX of strings (byte order: prefixes first, non-ASCII after ASCII),
+    # the column given as strings in some batches and as codes with their own
+    # dictionaries in others; NULL strings; a group whose strings are all NULL
+    words = _words(3_000, 41)
+    rng = np.random.default_rng(42)
+    n = 300_000
+    k = rng.integers(0, 500, n).astype(np.int32)
+    idx = rng.integers(0, len(words), n)
+    s = _strings(words, idx, null_p=0.1, seed=43)
+    valid = s.validity.copy()
+    valid[k == 13] = 0
+    s = G.Column.strings([s.string(i) if valid[i] else None for i in range(n)])
+    cols = [G.Column(G.INTEGER, k), s, G.Column(G.BIGINT, rng.integers(-50, 50, n))]
+    dcols = [cols[0], _as_dict(cols[1]), cols[2]]
+    dic = dcols[1].dictionary
+    for gcols, aggs in (([0], [(G.MIN, 1), (G.MAX, 1), (G.COUNT, 1), (G.SUM, 2)]),
+                        ([], [(G.MIN, 1), (G.MAX, 1)]),
+                        ([0, 2], [(G.MAX, 1), (G.COUNT_STAR, None)])):
+        q = G.Query([G.INTEGER, G.VARCHAR, G.BIGINT], gcols, aggs, nullable=[False, True, False])
+        c_, v_, t, ov = O.groupby_exact(q, dcols)
+        outs = {len(gcols) + i: dic for i, (kind, col) in enumerate(aggs) if kind in (G.MIN, G.MAX)}
+        exp = oracle_rows(c_, v_, t, outs)
+        parts = _split(cols, 4)
+        for i in (1, 3):
+            parts[i][1] = _as_shuffled_dict(parts[i][1], i)
+        for device in (True, False):
+            print(f"[strings] MIN/MAX groups {gcols} aggs {aggs} device {device}", flush=True)
+            got, _ = _run(q, parts, device)
+            assert_same_groups(got, exp, len(gcols), t)
+    got, _ = _run(G.Query([G.INTEGER, G.VARCHAR, G.BIGINT], [0], [(G.MIN, 1)], nullable=[False, True, False]),
+                  [cols], True)
+    assert dict(got)[13] is None

@@ -377,3 +377,24 @@ def test_multi_column_and_double_keys_vs_brute_force(drop):
         e[1] += int(v[r])
         e[2] = int(v[r]) if e[2] is None else min(e[2], int(v[r]))
     assert got == {k: tuple(e) for k, e in exp.items()}
+
+
+def test_varchar_min_max_vs_brute_force():
+    # MIN/MAX of strings in byte order (prefixes first, non-ASCII after ASCII),
+    # a dictionary with a repeated entry, NULL arguments, an all-NULL group
+    rng = np.random.default_rng(23)
+    words = ["b", "a", "ab", "", "zz", "é", "Z", "a"]
+    dic = G.Dictionary(words)
+    n = 3000
+    k = rng.integers(0, 40, n).astype(np.int32)
+    s = rng.integers(0, len(words), n).astype(np.uint32)
+    valid = ((rng.random(n) > 0.2) & (k != 7)).astype(np.uint8)
+    cols = [G.Column(G.INTEGER, k), G.Column(G.VARCHAR, s, valid, dictionary=dic)]
+    q = G.Query([G.INTEGER, G.VARCHAR], [0], [(G.MIN, 1), (G.MAX, 1), (G.COUNT, 1)], nullable=[False, True])
+    c_, v_, t, ov = O.groupby_exact(q, cols)
+    got = {r[0]: r[1:] for r in oracle_rows(c_, v_, t, {1: dic, 2: dic})}
+    exp = {}
+    for key in range(40):
+        vals = [words[s[r]].encode() for r in range(n) if k[r] == key and valid[r]]
+        exp[key] = (min(vals).decode(), max(vals).decode(), len(vals)) if vals else (None, None, 0)
+    assert got == exp
