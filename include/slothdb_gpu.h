@@ -109,7 +109,10 @@ typedef struct {
  *   COUNT(c)  BIGINT, non-NULL inputs
  *   SUM(c)    BIGINT for INTEGER/BIGINT c, DOUBLE for DOUBLE c; NULL if no input
  *   AVG(c)    DOUBLE; NULL if no input
- *   MIN/MAX   type of c; NULL if no input
+ *   MIN/MAX   type of c; NULL if no input. Over VARCHAR, byte order of the
+ *             strings (the generic path's Value comparison); the column is
+ *             interned at finalize with codes relabelled in that order.
+ *             Not next to COUNT(DISTINCT).
  *   COUNT(DISTINCT c)  BIGINT, distinct non-NULL values of an INTEGER, BIGINT
  *             or VARCHAR c (the generic path's distinct_set,
  *             physical_planner.cpp:12325-12330). Runs in two phases: GROUP BY
