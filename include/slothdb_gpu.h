@@ -113,15 +113,17 @@ typedef struct {
  *             strings (the generic path's Value comparison); the column is
  *             interned at finalize with codes relabelled in that order.
  *             Not next to COUNT(DISTINCT).
- *   COUNT(DISTINCT c)  BIGINT, distinct non-NULL values of an INTEGER, BIGINT
- *             or VARCHAR c (the generic path's distinct_set,
+ *   COUNT(DISTINCT c)  BIGINT, distinct non-NULL values of c, of any type
+ *             (the generic path's distinct_set,
  *             physical_planner.cpp:12325-12330). Runs in two phases: GROUP BY
- *             (group column, c) with partial states of the other aggregates
+ *             (group columns, c) with partial states of the other aggregates
  *             (rows with a NULL c form their own groups there), then GROUP BY
- *             the group column over those rows, counting non-NULL c; a group
+ *             the group columns over those rows, counting non-NULL c; a group
  *             whose c is always NULL counts 0. All DISTINCT aggregates of a
- *             query share one c; at most one group column, not DOUBLE, which
- *             must be non-nullable or have drop_null_keys.
+ *             query share one c; at most 7 group columns, of any type, with
+ *             NULL keys grouped or dropped as for other queries. DOUBLE values
+ *             are distinct by value (-0.0 equals 0.0; all NaNs are one value),
+ *             where the reference compares their shortest decimal text.
  * Integer SUM is exact two's-complement int64 (the reference accumulates in
  * double and casts; the two agree while partial sums stay below 2^53). */
 typedef enum {

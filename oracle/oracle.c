@@ -383,7 +383,8 @@ static inline void row_update(const exact_ctx *x, gtable *t, uint64_t g, uint64_
             /* distinct non-NULL values per group (physical_planner.cpp:12325-12330) */
             gkey dk;
             memset(&dk, 0, sizeof(dk));
-            int64_t v = col_i64(c, r);
+            /* DOUBLE arguments by value: -0.0 as 0.0, one NaN */
+            int64_t v = c->type == GBAGG_TYPE_DOUBLE ? double_key(((const double *)c->data)[r]) : col_i64(c, r);
             if (x->acanon[a]) v = x->acanon[a][(uint32_t)v];
             dk.k[0] = (int64_t)(g * 16 + a);
             dk.k[1] = v;

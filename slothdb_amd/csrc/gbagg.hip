@@ -314,8 +314,8 @@ struct gbagg_handle {
     // COUNT(DISTINCT): phase 1 groups by (group column, argument) with partial
     // states of the other aggregates; this handle is phase 2 over its result
     std::unique_ptr<gbagg_handle> phase1;
-    std::vector<uint32_t> p1_dict_off;  // phase 1's VARCHAR group key dictionary, as phase 2 input
-    std::vector<char> p1_dict_bytes;
+    std::vector<std::vector<uint32_t>> p1_dict_off;  // per column: phase 1's VARCHAR group key
+    std::vector<std::vector<char>> p1_dict_bytes;    // dictionaries, as phase 2 input
     uint64_t wide_seed0 = 0;       // KM_WIDE: first seed of wide_mix
     // VARCHAR columns given as strings (GBAGG_STR_RAW), interned at finalize
     std::vector<uint8_t> str_mode;  // per column: 0 no batch yet, 1 dictionary codes, 2 strings
@@ -654,15 +654,6 @@ void build_plan(gbagg_handle *h, const gbagg_query_desc *q) {
     }
 }
 
-// COUNT(DISTINCT x) in two phases (the generic path's per-group distinct_set,
-// physical_planner.cpp:12325-12330, as a decomposed aggregate):
-//   phase 1  GROUP BY (k, x) with partial states of the other aggregates
-//            (COUNT, SUM, MIN, MAX; AVG as SUM and COUNT), NULL keys dropped
-//   phase 2  GROUP BY k over phase 1's device-resident result: COUNT(*) is the
-//            distinct count, the partial states combine (SUM of COUNT and SUM,
-//            MIN of MIN, MAX of MAX, AVG = SUM of sums / SUM of counts)
-// Rows with a NULL x must still reach the other aggregates, so a nullable x
-// allows only DISTINCT aggregates; a nullable k needs drop_null_keys.
 // Multi-column keys: the same plan over one more column, the tuple code,
 // grouped by that column alone (rows dropped through its validity).
 void build_tuple(gbagg_handle *h, const gbagg_query_desc *q) {
@@ -703,14 +694,24 @@ void build_tuple(gbagg_handle *h, const gbagg_query_desc *q) {
     h->tstr.resize(G);
 }
 
+// COUNT(DISTINCT x) in two phases (the generic path's per-group distinct_set,
+// physical_planner.cpp:12325-12330, as a decomposed aggregate):
+//   phase 1  GROUP BY (k..., x) with partial states of the other aggregates
+//            (COUNT, SUM, MIN, MAX; AVG as SUM and COUNT); a NULL x is a key
+//            value like any other
+//   phase 2  GROUP BY k... over phase 1's device-resident result: COUNT(x) is
+//            the distinct count, the partial states combine (SUM of COUNT and
+//            SUM, MIN of MIN, MAX of MAX, AVG = SUM of sums / SUM of counts)
+// k is zero to seven columns of any type; x is one column of any type (DOUBLE
+// arguments are distinct by value: -0.0 equals 0.0, and all NaNs are one).
 void build_distinct(gbagg_handle *h, const gbagg_query_desc *q) {
     if (q->num_columns == 0 || q->num_columns > GBAGG_MAX_COLUMNS || !q->column_types)
         fail(GBAGG_INVALID, "num_columns must be 1..16 with column_types");
-    if (q->num_group_cols > 1) fail(GBAGG_INVALID, "COUNT(DISTINCT) supports at most one group column");
-    if (q->num_group_cols && (!q->group_cols || q->group_cols[0] >= q->num_columns))
-        fail(GBAGG_INVALID, "group column out of range");
-    if (q->num_group_cols && q->column_types[q->group_cols[0]] == GBAGG_TYPE_DOUBLE)
-        fail(GBAGG_INVALID, "COUNT(DISTINCT) with a DOUBLE group column is not supported");
+    if (q->num_group_cols >= GBAGG_MAX_GROUP_COLS)
+        fail(GBAGG_INVALID, "COUNT(DISTINCT) supports at most 7 group columns");
+    if (q->num_group_cols && !q->group_cols) fail(GBAGG_INVALID, "null group_cols");
+    for (uint32_t g = 0; g < q->num_group_cols; g++)
+        if (q->group_cols[g] >= q->num_columns) fail(GBAGG_INVALID, "group column out of range");
     auto nullable = [&](uint32_t c) { return q->column_nullable ? q->column_nullable[c] != 0 : true; };
     int32_t x = -1;
     bool only_distinct = true;
@@ -730,17 +731,18 @@ void build_distinct(gbagg_handle *h, const gbagg_query_desc *q) {
             q->column_types[q->aggs[a].column] == GBAGG_TYPE_VARCHAR)
             fail(GBAGG_INVALID, "MIN/MAX over VARCHAR next to COUNT(DISTINCT) is not supported");
     const gbagg_type xt = q->column_types[x];
-    if (xt != GBAGG_TYPE_INTEGER && xt != GBAGG_TYPE_BIGINT && xt != GBAGG_TYPE_VARCHAR)
-        fail(GBAGG_INVALID, "COUNT(DISTINCT) supports INTEGER, BIGINT and VARCHAR arguments");
+    if (!valid_type(xt)) fail(GBAGG_INVALID, "unsupported column type");
     const uint32_t ng = q->num_group_cols;
-    if (ng && nullable(q->group_cols[0]) && !q->drop_null_keys)
-        fail(GBAGG_INVALID, "COUNT(DISTINCT) over a nullable group column requires drop_null_keys");
     (void)only_distinct;
     // NULL arguments stay in phase 1 as a group of their own (its partial
     // states reach the other aggregates, and a group whose arguments are all
     // NULL still appears, with COUNT(DISTINCT) 0); phase 2 counts non-NULL
-    // arguments
+    // arguments. NULL keys are kept by phase 1 unless the query drops them
+    // and the argument has no NULLs (then phase 1 drops them itself).
     const bool keep_null_x = nullable((uint32_t)x);
+    bool null_k = false;
+    for (uint32_t g = 0; g < ng; g++) null_k |= nullable(q->group_cols[g]);
+    const bool p1_drop = !keep_null_x && (q->drop_null_keys || !null_k);
 
     // phase 1: partial states, deduplicated
     std::vector<gbagg_agg> a1;
@@ -767,8 +769,8 @@ void build_distinct(gbagg_handle *h, const gbagg_query_desc *q) {
         default: fail(GBAGG_INVALID, "unknown aggregate kind");
         }
     }
-    uint32_t gc1[2];
-    if (ng) gc1[0] = q->group_cols[0];
+    uint32_t gc1[GBAGG_MAX_GROUP_COLS];
+    for (uint32_t g = 0; g < ng; g++) gc1[g] = q->group_cols[g];
     gc1[ng] = (uint32_t)x;
     gbagg_query_desc d1 = *q;
     d1.num_group_cols = ng + 1;
@@ -778,7 +780,7 @@ void build_distinct(gbagg_handle *h, const gbagg_query_desc *q) {
     d1.topn_col = -1;
     d1.topn_limit = 0;
     d1.row_limit = 0;
-    d1.drop_null_keys = keep_null_x ? 0 : 1;
+    d1.drop_null_keys = p1_drop ? 1 : 0;
     h->phase1 = std::make_unique<gbagg_handle>();
     build_plan(h->phase1.get(), &d1);
     pick_kernels(h->phase1.get());
@@ -792,7 +794,7 @@ void build_distinct(gbagg_handle *h, const gbagg_query_desc *q) {
         t2.push_back(key ? q->column_types[gc1[c]] : p1->outs[p1->tuple ? c - ng : c].type);
         const bool cnt = !key && p1->outs[p1->tuple ? c - ng : c].kind == OUT_COUNT;
         const int32_t arg = key ? -1 : a1[c - ng - 1].column;
-        if (key) n2.push_back(keep_null_x && nullable(gc1[c]) ? 1 : 0);
+        if (key) n2.push_back(!p1_drop && nullable(gc1[c]) ? 1 : 0);
         else n2.push_back(cnt ? 0 : (arg >= 0 && nullable((uint32_t)arg)) ? 1 : 0);
     }
     const uint32_t base = ng + 1;  // first partial-state column
@@ -813,13 +815,14 @@ void build_distinct(gbagg_handle *h, const gbagg_query_desc *q) {
             avg_hidden[a] = (int)a2.size();
             a2.push_back({GBAGG_AGG_SUM, (int32_t)(base + src_cnt[a])});
         }
-    uint32_t gc2 = 0;
+    uint32_t gc2[GBAGG_MAX_GROUP_COLS];
+    for (uint32_t g = 0; g < ng; g++) gc2[g] = g;
     gbagg_query_desc d2{};
     d2.num_columns = (uint32_t)t2.size();
     d2.column_types = t2.data();
     d2.column_nullable = n2.data();
     d2.num_group_cols = ng;
-    d2.group_cols = &gc2;
+    d2.group_cols = gc2;
     d2.num_aggs = (uint32_t)a2.size();
     d2.aggs = a2.data();
     d2.topn_col = q->topn_col;
@@ -829,23 +832,25 @@ void build_distinct(gbagg_handle *h, const gbagg_query_desc *q) {
     d2.drop_null_keys = q->drop_null_keys;
     d2.strategy = GBAGG_STRATEGY_AUTO;
     build_plan(h, &d2);
-    // user-visible semantics of the phase-2 outputs
+    // user-visible semantics of the phase-2 outputs (after the key columns,
+    // or after the hidden tuple code when phase 2 groups by several columns)
+    const uint32_t ko = h->tuple ? 1 : ng;
     bool topn_avg = false;
     for (uint32_t a = 0; a < q->num_aggs; a++) {
-        OutCol &oc = h->outs[ng + a];
+        OutCol &oc = h->outs[ko + a];
         const gbagg_agg_kind k = q->aggs[a].kind;
         if (k == GBAGG_AGG_COUNT_STAR || k == GBAGG_AGG_COUNT) {
             oc.kind = OUT_COUNT;  // sums of counts: never NULL
             oc.cnt_acc = -1;
         } else if (k == GBAGG_AGG_AVG) {
-            const OutCol &hid = h->outs[ng + avg_hidden[a]];
+            const OutCol &hid = h->outs[ko + avg_hidden[a]];
             oc.kind = oc.kind == OUT_SUM_F ? OUT_AVG_F : OUT_AVG_I;
             oc.type = GBAGG_TYPE_DOUBLE;
             oc.cnt_acc = hid.val_acc;
             topn_avg = topn_avg || (int32_t)(ng + a) == q->topn_col;
         }
     }
-    h->outs.resize(ng + q->num_aggs);
+    h->outs.resize(ko + q->num_aggs);
     if (topn_avg) h->topn_device = false;  // AVG order: the full result is sorted on the host
     h->strategy = q->strategy == GBAGG_STRATEGY_DIRECT ? GBAGG_STRATEGY_AUTO : q->strategy;
 }
@@ -2244,29 +2249,34 @@ void run_phase1(gbagg_handle *h) {
     const size_t G = p1->tuple ? p1->tcols.size() : 0;
     const size_t nc = p1->tuple ? G + p1->outs.size() - 1 : p1->outs.size();
     if (p1->tuple) unpack_tuples_device(p1);
+    // phase 2's group columns are its first columns; a VARCHAR one gets
+    // phase 1's dictionary for it (one buffer pair per column)
+    const size_t ng2 = h->tuple ? h->tcols.size() : h->group_cols.size();
+    h->p1_dict_off.assign(nc, {});
+    h->p1_dict_bytes.assign(nc, {});
     std::vector<gbagg_column_view> v(nc);
     for (size_t c = 0; c < nc; c++) {
         v[c] = gbagg_column_view{};
+        std::vector<uint32_t> &doff = h->p1_dict_off[c];
+        std::vector<char> &dbytes = h->p1_dict_bytes[c];
         if (c < G) {
             v[c].type = p1->ttypes[c];
             v[c].data = p1->tout_data[c].ptr;
             v[c].validity = h->nullable[c] ? p1->tout_valid[c].as<uint8_t>() : nullptr;
-            if (v[c].type == GBAGG_TYPE_VARCHAR && c < h->group_cols.size()) {
+            if (v[c].type == GBAGG_TYPE_VARCHAR && c < ng2) {
                 // the column's interned codes: its interned dictionary, as a unique one
                 std::vector<StrCode> cd;
                 std::vector<uint32_t> words;
                 strcol_host(p1, *p1->tstr[c], cd, words);
-                h->p1_dict_off.assign(1, 0);
-                h->p1_dict_bytes.clear();
+                doff.assign(1, 0);
                 const char *src = (const char *)words.data();
                 for (const StrCode &e : cd) {
-                    h->p1_dict_bytes.insert(h->p1_dict_bytes.end(), src + (uint64_t)e.word * 4,
-                                            src + (uint64_t)e.word * 4 + e.len);
-                    h->p1_dict_off.push_back((uint32_t)h->p1_dict_bytes.size());
+                    dbytes.insert(dbytes.end(), src + (uint64_t)e.word * 4, src + (uint64_t)e.word * 4 + e.len);
+                    doff.push_back((uint32_t)dbytes.size());
                 }
-                if (h->p1_dict_bytes.empty()) h->p1_dict_bytes.push_back(0);
-                v[c].dict_offsets = h->p1_dict_off.data();
-                v[c].dict_bytes = h->p1_dict_bytes.data();
+                if (dbytes.empty()) dbytes.push_back(0);
+                v[c].dict_offsets = doff.data();
+                v[c].dict_bytes = dbytes.data();
                 v[c].dict_size = (uint32_t)cd.size();
                 v[c].flags = GBAGG_DICT_UNIQUE;
             }
@@ -2276,21 +2286,20 @@ void run_phase1(gbagg_handle *h) {
         v[c].type = p1->outs[ci].type;
         v[c].data = p1->out_data[ci].ptr;
         v[c].validity = h->nullable[c] ? p1->out_valid[ci].as<uint8_t>() : nullptr;
-        if (p1->outs[c].type == GBAGG_TYPE_VARCHAR && c < h->group_cols.size()) {
+        if (p1->outs[c].type == GBAGG_TYPE_VARCHAR && c < ng2) {
             // phase 1's canonical ids index its key dictionary: pass it on as a unique one
             KeyDict &kd = p1->kdict[c];
             host_dict(p1, kd);
             const uint32_t n = kd.mode == 1 ? kd.size : (uint32_t)kd.strings.size();
-            h->p1_dict_off.assign(1, 0);
-            h->p1_dict_bytes.clear();
+            doff.assign(1, 0);
             for (uint32_t i = 0; i < n; i++) {
                 const std::string str = kd.lookup(i);
-                h->p1_dict_bytes.insert(h->p1_dict_bytes.end(), str.begin(), str.end());
-                h->p1_dict_off.push_back((uint32_t)h->p1_dict_bytes.size());
+                dbytes.insert(dbytes.end(), str.begin(), str.end());
+                doff.push_back((uint32_t)dbytes.size());
             }
-            if (h->p1_dict_bytes.empty()) h->p1_dict_bytes.push_back(0);
-            v[c].dict_offsets = h->p1_dict_off.data();
-            v[c].dict_bytes = h->p1_dict_bytes.data();
+            if (dbytes.empty()) dbytes.push_back(0);
+            v[c].dict_offsets = doff.data();
+            v[c].dict_bytes = dbytes.data();
             v[c].dict_size = n;
             v[c].flags = GBAGG_DICT_UNIQUE;
         }

@@ -117,3 +117,41 @@ def test_groups_whose_argument_is_always_null():
             if gc:
                 zero = {r[0] for r in res.rows() if r[1] == 0}
                 assert zero == {v for v in range(0, 1000, 7)}
+
+
+def _norm(rows, nk):
+    """Keys as SQL groups them, and sortable: -0.0 as 0.0, NaN as +inf (the
+    tables hold no infinity)."""
+    def f(x):
+        if isinstance(x, float):
+            return float("inf") if x != x else (0.0 if x == 0.0 else x)
+        return x
+    return [tuple(f(x) for x in r[:nk]) + tuple(r[nk:]) for r in rows]
+
+
+@pytest.mark.parametrize("keys", [[], [0], [5], [0, 2], [2, 6, 0], [6, 5, 0, 2]])
+def test_double_argument_and_multi_column_keys(keys):
+    # DOUBLE arguments and keys holding -0.0, 0.0 and NaNs of two bit patterns
+    # (distinct by value), two VARCHAR key columns with their own dictionaries,
+    # nullable keys whose NULLs form groups, and the other aggregates beside
+    n = 300_000
+    cols, words = _table(n, 40)
+    rng = np.random.default_rng(41)
+    pool = np.concatenate([np.array([0.0, -0.0, np.nan, -np.nan]), rng.normal(0, 100, 3000)])
+    cols.append(G.Column(G.DOUBLE, rng.choice(pool, n), (rng.random(n) > 0.1).astype(np.uint8)))
+    other = G.Dictionary([f"t{j}" for j in range(40)])
+    cols.append(G.Column(G.VARCHAR, rng.integers(0, 40, n).astype(np.uint32), (rng.random(n) > 0.05).astype(np.uint8),
+                         dictionary=other))
+    types = [c.type for c in cols]
+    dmap = {2: words, 6: other}
+    for x in (5, 4, 2):
+        if x in keys:
+            continue
+        for drop in (False, True):
+            q = G.Query(types, keys, [(G.COUNT_DISTINCT, x), (G.COUNT_STAR, None), (G.SUM, 3), (G.AVG, 4),
+                                      (G.MAX, 1)], drop_null_keys=drop)
+            dicts = {i: dmap[c] for i, c in enumerate(keys) if c in dmap}
+            print(f"[distinct] keys={keys} x={x} drop={drop}", flush=True)
+            res, _ = G.run(q, cols, device=True)
+            exp, t = _exact(q, cols, dicts)
+            assert_same_groups(_norm(res.rows(), len(keys)), _norm(exp, len(keys)), len(keys), t)

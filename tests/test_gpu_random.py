@@ -135,14 +135,15 @@ def test_random_topn_and_count_distinct():
             counts = [nk + i for i, (kind, _) in enumerate(q.aggs) if kind in (G.COUNT_STAR, G.COUNT)]
             q.topn = (int(rng.choice(counts)), bool(rng.random() < 0.3), int(rng.choice([1, 5, 50])))
         else:
-            # COUNT(DISTINCT x) next to COUNT(*): at most one non-DOUBLE group column
-            cand = [c for c in range(len(types)) if types[c] != G.DOUBLE]
-            x = int(rng.choice(cand))
-            keys = [c for c in cand if c != x][:1] if rng.random() < 0.7 else []
+            # COUNT(DISTINCT x) of any type next to COUNT(*) and COUNT(x), by
+            # zero to three group columns of any type
+            x = int(rng.integers(0, len(types)))
+            rest = [c for c in range(len(types)) if c != x]
+            keys = [int(c) for c in rng.permutation(rest)[:int(rng.integers(0, min(3, len(rest)) + 1))]]
             q.group_cols = keys
             nk = len(keys)
-            q.aggs = [(G.COUNT_DISTINCT, x)] if q.nullable[x] else [(G.COUNT_DISTINCT, x), (G.COUNT_STAR, None)]
-            q.drop_null_keys = bool(keys and q.nullable[keys[0]]) or q.drop_null_keys
+            q.aggs = [(G.COUNT_DISTINCT, x), (G.COUNT_STAR, None), (G.COUNT, x)][:int(rng.integers(1, 4))]
+            q.drop_null_keys = bool(rng.random() < 0.5)
         print(f"[random] case {case} seed {seed}: rows {len(cols[0])} types {types} groups {q.group_cols} "
               f"aggs {q.aggs} preds {q.predicates} drop {q.drop_null_keys} topn {q.topn} "
               f"batches {nbatch} device {device}", flush=True)

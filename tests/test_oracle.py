@@ -102,6 +102,15 @@ def test_closed_form_arithmetic_series_per_group():
 
 # ---- brute force on tiny inputs --------------------------------------------
 
+_NAN = float("nan")
+
+
+def _canon(v):
+    """One object for every NaN key, so that NaN keys form one group (Python
+    compares tuple and dict keys by identity before equality)."""
+    return _NAN if isinstance(v, float) and v != v else v
+
+
 def brute(q, cols):
     """Direct restatement of SQL GROUP BY semantics in pure Python."""
     n = len(cols[0])
@@ -136,7 +145,7 @@ def brute(q, cols):
                 break
         if not ok:
             continue
-        key = tuple(val(c, r) for c in q.group_cols)
+        key = tuple(_canon(val(c, r)) for c in q.group_cols)
         if q.drop_null_keys and any(k is None for k in key):
             continue
         groups[key].append(r)
@@ -151,7 +160,8 @@ def brute(q, cols):
             if kind == G.COUNT:
                 row.append(len(xs))
             elif kind == G.COUNT_DISTINCT:
-                row.append(len(set(xs)))
+                # SQL value equality: -0.0 == 0.0 (so in one set), every NaN one value
+                row.append(len({"nan" if isinstance(v, float) and v != v else v for v in xs}))
             elif not xs:
                 row.append(None)
             elif kind == G.SUM:
@@ -217,6 +227,34 @@ def test_count_distinct_vs_brute_force(seed):
             dicts = {i: cols[c].dictionary for i, c in enumerate(q.group_cols) if cols[c].type == G.VARCHAR}
             got, t, _ = exact_rows(q, cols, dicts)
             assert_same_groups(got, brute(q, cols), len(q.group_cols), t)
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_count_distinct_double_and_multi_column_keys_vs_brute_force(seed):
+    # DOUBLE arguments and keys with -0.0, 0.0 and NaNs of two bit patterns;
+    # zero to three group columns, nullable keys kept or dropped
+    cols = tiny_table(seed)
+    rng = np.random.default_rng(100 + seed)
+    n = len(cols[0])
+    pool = np.array([0.0, -0.0, 1.5, -1.5, np.nan, -np.nan, 3.0, 1e-300])
+    cols.append(G.Column(G.DOUBLE, rng.choice(pool, n), (rng.random(n) > 0.15).astype(np.uint8)))
+    types = [c.type for c in cols]
+    for x in (5, 2, 0, 3):
+        for keys, drop in (([], False), ([4, 0], False), ([0, 3, 5], True), ([5], False), ([4, 3, 0], False)):
+            if x in keys:
+                continue
+            q = G.Query(types, keys, [(G.COUNT_DISTINCT, x), (G.COUNT_STAR, None), (G.MIN, 1)],
+                        drop_null_keys=drop)
+            dicts = {i: cols[c].dictionary for i, c in enumerate(q.group_cols) if cols[c].type == G.VARCHAR}
+            got, t, _ = exact_rows(q, cols, dicts)
+            exp = brute(q, cols)
+            if not keys and not exp:
+                exp = [(0, 0, None)]
+            # keys as SQL groups them, and sortable: -0.0 as 0.0, NaN as +inf (absent here)
+            norm = lambda rows: [tuple(float("inf") if isinstance(v, float) and v != v else (0.0 if v == 0.0 and
+                                       isinstance(v, float) else v) for v in r[:len(keys)]) + tuple(r[len(keys):])
+                                 for r in rows]
+            assert_same_groups(norm(got), norm(exp), len(keys), t)
 
 
 def test_count_distinct_vs_pandas_nunique():
