@@ -667,14 +667,19 @@ class ParquetFile:
         h = C.c_void_p()
         _check(lib().gbagg_parquet_open(self._buf.ctypes.data, self._buf.nbytes, C.byref(h)))
         self._h = h.value
-        nrg, ncol, nrows = C.c_uint32(), C.c_uint32(), C.c_uint64()
-        _check(lib().gbagg_parquet_shape(self._h, C.byref(nrg), C.byref(ncol), C.byref(nrows)))
-        self.num_row_groups, self.num_columns, self.num_rows = nrg.value, ncol.value, nrows.value
-        self.columns = []
-        for c in range(self.num_columns):
-            t, nl, name = C.c_int(), C.c_uint8(), C.c_char_p()
-            _check(lib().gbagg_parquet_column(self._h, c, C.byref(t), C.byref(nl), C.byref(name)))
-            self.columns.append((name.value.decode(), t.value, bool(nl.value)))
+        try:
+            nrg, ncol, nrows = C.c_uint32(), C.c_uint32(), C.c_uint64()
+            _check(lib().gbagg_parquet_shape(self._h, C.byref(nrg), C.byref(ncol), C.byref(nrows)))
+            self.num_row_groups, self.num_columns, self.num_rows = nrg.value, ncol.value, nrows.value
+            self.columns = []
+            for c in range(self.num_columns):
+                t, nl, name = C.c_int(), C.c_uint8(), C.c_char_p()
+                _check(lib().gbagg_parquet_column(self._h, c, C.byref(t), C.byref(nl), C.byref(name)))
+                # column names are the file's bytes: not necessarily UTF-8
+                self.columns.append((name.value.decode("utf-8", "replace"), t.value, bool(nl.value)))
+        except BaseException:
+            self.close()
+            raise
 
     def upload(self):
         """Keep the whole file in HBM: decodes then read their chunks there."""

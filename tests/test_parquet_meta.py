@@ -47,3 +47,40 @@ def test_truncated_footer_is_rejected():
     img[-8:-4] = (flen // 2).to_bytes(4, "little")  # footer cut in half
     with pytest.raises(G.GbaggError):
         G.ParquetFile(bytes(img))
+
+
+def test_corrupted_footers_fail_cleanly():
+    # random byte changes and cuts inside the footer (Thrift compact protocol:
+    # field headers, varints, list sizes, string lengths): every image either
+    # opens with a consistent shape or raises, and never reads out of bounds
+    n = 5_000
+    t = pa.table({"i": pa.array(np.arange(n, dtype=np.int32)),
+                  "s": pa.array([f"x{j % 7}" for j in range(n)]),
+                  "l": pa.array(np.arange(n, dtype=np.int64), mask=np.arange(n) % 3 == 0)})
+    buf = io.BytesIO()
+    pq.write_table(t, buf, compression="NONE", row_group_size=1000)
+    img = buf.getvalue()
+    flen = int.from_bytes(img[-8:-4], "little")
+    f0 = len(img) - 8 - flen
+    rng = np.random.default_rng(77)
+    opened = failed = 0
+    for trial in range(3000):
+        b = bytearray(img)
+        for _ in range(int(rng.integers(1, 4))):
+            pos = f0 + int(rng.integers(0, flen))
+            b[pos] = int(rng.integers(0, 256)) if rng.random() < 0.7 else b[pos] ^ (1 << int(rng.integers(0, 8)))
+        if trial % 10 == 0:  # also cut the footer short, keeping its length field
+            cut = int(rng.integers(1, flen))
+            b = b[:f0 + flen - cut] + b[-8:]
+            b[-8:-4] = (flen - cut).to_bytes(4, "little")
+        try:
+            f = G.ParquetFile(bytes(b))
+        except G.GbaggError:
+            failed += 1
+            continue
+        assert len(f.columns) == f.num_columns
+        for rg in range(f.num_row_groups):
+            f.row_group_rows(rg)
+        f.close()
+        opened += 1
+    assert failed > 0 and opened + failed == 3000
