@@ -3429,6 +3429,7 @@ gbagg_status gbagg_parquet_decode_range(gbagg_parquet *p, uint32_t rg0, uint32_t
         const uint8_t *base = p->resident ? p->image.as<uint8_t>() : p->chunks.as<uint8_t>();
         for (size_t k = 0; k < dbufs.size(); k++) dbufs[k] = base + at[k];
         PqCall call;
+        call.dev_streams = (uint64_t)ng * num_cols >= 32;
         for (uint32_t i = 0; i < num_cols; i++) {
             p->outs.push_back(std::make_unique<gbagg_parquet::Out>());
             parquet_plan_column(p, rg0, rg1, cols[i], dbufs.data() + (size_t)i * ng, *p->outs.back(), out[i], call);
@@ -3439,6 +3440,7 @@ gbagg_status gbagg_parquet_decode_range(gbagg_parquet *p, uint32_t rg0, uint32_t
         HIPCHK(hipMemcpy(&bad, p->bad.ptr, 4, hipMemcpyDeviceToHost));
         if (bad & 1) fail(GBAGG_INVALID, "parquet: dictionary index outside the dictionary");
         if (bad & 2) fail(GBAGG_INVALID, "parquet: truncated BYTE_ARRAY value");
+        if (bad & 4) fail(GBAGG_INVALID, "parquet: malformed dictionary index stream");
         if (rows) {
             uint64_t n = 0;
             for (uint32_t g = rg0; g < rg1; g++) n += (uint64_t)p->rgs[g].num_rows;

@@ -15,14 +15,18 @@
 // specification's (Thrift compact protocol metadata, RLE / bit-packed hybrid
 // levels and indices).
 //
-// Host: the footer and page headers, and each RLE / bit-packed hybrid
-// stream's run headers (a few bytes per run; bit-packed runs are not read),
-// collected over all columns of a decode call. Device: the column chunks'
-// bytes (in place when the file image is resident in HBM), then
-//   k_pq_hybrid   every run of the call, one thread per value: definition
+// Host: the footer and page headers, and the run headers of the definition
+// levels (a few bytes per run; bit-packed runs are not read), collected over
+// all columns of a decode call. Device: the column chunks' bytes (in place
+// when the file image is resident in HBM), then
+//   k_pq_streams  one workgroup per dictionary-encoded page: its index
+//                 stream's run headers, then its indices to u32 (calls over
+//                 many column chunks; the host reads them in small calls)
+//   k_pq_hybrid   every run the host read, one thread per value: definition
 //                 levels to validity bytes, dictionary indices to u32
 //   k_pq_pages    every page of the call, one thread per non-NULL value:
 //                 PLAIN values copied, dictionary indices gathered
+//   (both in tiles of consecutive values whose runs or pages sit in LDS)
 //   k_pq_count / k_pq_scan_top / k_pq_scatter
 //                 OPTIONAL columns with NULLs: exclusive count of valid rows
 //                 per row, then value[row] = dense[count] where valid
@@ -42,32 +46,209 @@ struct PqRun {
     uint32_t dwidth;  // output element bytes: 1 (definition levels) or 4 (dictionary indices)
 };
 
-// Every run of a decode call, one thread per value: its run (binary search),
-// then the repeated value or the bw bits at its bit offset (least-significant
-// bit first; every source range carries 16 readable bytes past its end).
-__global__ __launch_bounds__(256) void k_pq_hybrid(const PqRun *__restrict__ runs, uint32_t nruns, uint64_t total) {
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
-         i += (uint64_t)gridDim.x * blockDim.x) {
-        uint32_t lo = 0, hi = nruns - 1;
-        while (lo < hi) {
-            const uint32_t mid = (lo + hi + 1) / 2;
-            if (runs[mid].gstart <= i) lo = mid;
-            else hi = mid - 1;
+// Values are processed in tiles of 256 * items consecutive values per
+// workgroup (items = 1..PQ_ITEMS, fewer for small calls so that they still
+// spread over the whole chip). Two threads find the tile's first and last run
+// (or page) by binary search in the call's table; when the tile spans at most
+// PQ_LDS_RUNS of them they are staged in LDS and each value searches there,
+// else in the table itself.
+constexpr uint32_t PQ_ITEMS = 16;
+constexpr uint32_t PQ_LDS_RUNS = 256;
+
+// The last entry of t[lo..hi] whose gstart is <= i (t[lo].gstart <= i).
+template <class R>
+__device__ inline uint32_t pq_find(const R *t, uint32_t lo, uint32_t hi, uint64_t i) {
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) / 2;
+        if (t[mid].gstart <= i) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Little-endian loads at any byte address through the aligned words that
+// cover it: the word below the address lies in the same 256-byte-aligned
+// chunk buffer or file image, and the word above within the 16 readable
+// bytes that follow every source range.
+__device__ inline uint64_t pq_ld8(const uint8_t *p) {
+    const uintptr_t a = (uintptr_t)p;
+    const uint64_t *w = (const uint64_t *)(a & ~(uintptr_t)7);
+    const uint32_t sh = (uint32_t)(a & 7) * 8;
+    const uint64_t lo = w[0];
+    return sh ? (lo >> sh) | (w[1] << (64 - sh)) : lo;
+}
+
+__device__ inline uint32_t pq_ld4(const uint8_t *p) {
+    const uintptr_t a = (uintptr_t)p;
+    const uint32_t *w = (const uint32_t *)(a & ~(uintptr_t)3);
+    const uint32_t sh = (uint32_t)(a & 3) * 8;
+    const uint32_t lo = w[0];
+    return sh ? (lo >> sh) | (w[1] << (32 - sh)) : lo;
+}
+
+// Runs the values of one tile [t0, t1) through f(i, entry): the tile's span
+// of the table in LDS when it fits.
+template <class R, class F>
+__device__ inline void pq_tile(const R *__restrict__ t, uint32_t n, uint64_t t0, uint64_t t1, uint32_t items, R *s_t,
+                               uint32_t *s_span, F &&f) {
+    if (threadIdx.x == 0) s_span[0] = pq_find(t, 0, n - 1, t0);
+    if (threadIdx.x == 64) s_span[1] = pq_find(t, 0, n - 1, t1 - 1);
+    __syncthreads();
+    const uint32_t lo = s_span[0], hi = s_span[1];
+    const bool staged = hi - lo < PQ_LDS_RUNS;
+    if (staged)
+        for (uint32_t r = threadIdx.x; r <= hi - lo; r += blockDim.x) s_t[r] = t[lo + r];
+    __syncthreads();
+    uint32_t at = 0;  // the run of this thread's previous value: runs only move forward
+#pragma unroll 4
+    for (uint32_t k = 0; k < items; k++) {
+        const uint64_t i = t0 + (uint64_t)k * 256 + threadIdx.x;
+        if (i >= t1) break;
+        if (staged) {
+            at = pq_find(s_t, at, hi - lo, i);
+            f(i, s_t[at]);
+        } else {
+            f(i, t[pq_find(t, lo, hi, i)]);
         }
-        const PqRun r = runs[lo];
-        const uint64_t j = i - r.gstart;
-        uint32_t v = r.value;
-        if (r.bw) {
-            const uint64_t bit = j * r.bw;
-            const uint8_t *p = (const uint8_t *)r.src + (bit >> 3);
-            uint64_t w = 0;
-#pragma unroll
-            for (int b = 0; b < 8; b++) w |= (uint64_t)p[b] << (8 * b);
-            const uint64_t mask = r.bw == 32 ? 0xFFFFFFFFull : ((1ull << r.bw) - 1);
-            v = (uint32_t)((w >> (bit & 7)) & mask);
+    }
+    __syncthreads();  // s_t and s_span are rewritten by the next tile
+}
+
+// Every run of a decode call, one thread per value: the repeated value or the
+// bw bits at its bit offset (least-significant bit first; every source range
+// carries 16 readable bytes past its end).
+__global__ __launch_bounds__(256) void k_pq_hybrid(const PqRun *__restrict__ runs, uint32_t nruns, uint64_t total,
+                                                   uint32_t items) {
+    __shared__ PqRun s_t[PQ_LDS_RUNS];
+    __shared__ uint32_t s_span[2];
+    const uint64_t tile = 256ull * items;
+    for (uint64_t t0 = (uint64_t)blockIdx.x * tile; t0 < total; t0 += (uint64_t)gridDim.x * tile) {
+        const uint64_t t1 = t0 + tile < total ? t0 + tile : total;
+        pq_tile(runs, nruns, t0, t1, items, s_t, s_span, [&](uint64_t i, const PqRun &r) {
+            const uint64_t j = i - r.gstart;
+            uint32_t v = r.value;
+            if (r.bw) {
+                const uint64_t bit = j * r.bw;
+                const uint64_t w = pq_ld8((const uint8_t *)r.src + (bit >> 3));
+                const uint64_t mask = r.bw == 32 ? 0xFFFFFFFFull : ((1ull << r.bw) - 1);
+                v = (uint32_t)((w >> (bit & 7)) & mask);
+            }
+            if (r.dwidth == 1) ((uint8_t *)r.dst)[j] = (uint8_t)v;
+            else ((uint32_t *)r.dst)[j] = v;
+        });
+    }
+}
+
+// The dictionary indices of one data page: an RLE / bit-packed hybrid stream
+// at [src, end) (device addresses) holding `count` values of bw bits
+// (1..32), written as u32 to dst. The run headers are read on the device.
+struct PqStream {
+    uint64_t src;
+    uint64_t end;
+    uint64_t dst;
+    uint32_t count;
+    uint32_t bw;
+};
+
+constexpr uint32_t PQ_SRUNS = 256;  // runs parsed per batch
+
+// One workgroup per stream: one thread reads up to PQ_SRUNS run headers into
+// LDS, then all threads write those runs' values; repeated until the stream
+// is done. A header or run that leaves the stream sets bit 4 of *bad and ends
+// the stream (the caller reports the error). Bit-packed data are read only
+// inside the stream; each source range carries 16 readable bytes past its end.
+__global__ __launch_bounds__(256) void k_pq_streams(const PqStream *__restrict__ st, uint32_t ns, uint32_t *bad) {
+    __shared__ uint64_t s_src[PQ_SRUNS];      // bit-packed: address of the run's data; RLE: 0
+    __shared__ uint32_t s_out[PQ_SRUNS + 1];  // first value of each run; s_out[nr]: the batch's end
+    __shared__ uint32_t s_val[PQ_SRUNS];      // RLE: the repeated value
+    __shared__ uint64_t s_p;
+    __shared__ uint32_t s_nr, s_fail;
+    for (uint32_t k = blockIdx.x; k < ns; k += gridDim.x) {
+        const PqStream S = st[k];
+        const uint64_t mask = S.bw >= 32 ? 0xFFFFFFFFull : ((1ull << S.bw) - 1);
+        if (threadIdx.x == 0) s_p = S.src;
+        uint32_t done = 0;
+        while (done < S.count) {
+            if (threadIdx.x == 0) {
+                const uint8_t *p = (const uint8_t *)s_p, *end = (const uint8_t *)S.end;
+                const uint32_t vbytes = (S.bw + 7) / 8;
+                uint32_t nr = 0, d = done, fail = 0;
+                while (nr < PQ_SRUNS && d < S.count) {
+                    uint64_t h = 0;
+                    uint32_t sh = 0;
+                    bool whole = false;
+                    while (p < end && sh < 35) {
+                        const uint8_t b = *p++;
+                        h |= (uint64_t)(b & 0x7F) << sh;
+                        sh += 7;
+                        if (!(b & 0x80)) {
+                            whole = true;
+                            break;
+                        }
+                    }
+                    if (!whole) {
+                        fail = 1;
+                        break;
+                    }
+                    if (h & 1) {  // bit-packed: (h >> 1) groups of 8 values
+                        const uint64_t bytes = (h >> 1) * S.bw;
+                        if (bytes > (uint64_t)(end - p)) {
+                            fail = 1;
+                            break;
+                        }
+                        const uint64_t left = S.count - d;
+                        const uint32_t cnt = (uint32_t)((h >> 1) * 8 < left ? (h >> 1) * 8 : left);
+                        if (cnt) {
+                            s_src[nr] = (uint64_t)p;
+                            s_val[nr] = 0;
+                            s_out[nr++] = d;
+                            d += cnt;
+                        }
+                        p += bytes;
+                    } else {  // RLE: (h >> 1) repeats of one value
+                        if ((h >> 1) == 0 || (uint64_t)(end - p) < vbytes) {
+                            fail = 1;
+                            break;
+                        }
+                        uint32_t v = 0;
+                        for (uint32_t b = 0; b < vbytes; b++) v |= (uint32_t)p[b] << (8 * b);
+                        p += vbytes;
+                        const uint64_t left = S.count - d;
+                        s_src[nr] = 0;
+                        s_val[nr] = v;
+                        s_out[nr++] = d;
+                        d += (uint32_t)((h >> 1) < left ? (h >> 1) : left);
+                    }
+                }
+                s_out[nr] = d;
+                s_nr = nr;
+                s_p = (uint64_t)p;
+                s_fail = fail;
+            }
+            __syncthreads();
+            const uint32_t nr = s_nr, d1 = s_out[nr], fail = s_fail;
+            uint32_t at = 0;  // the run of this thread's previous value: runs only move forward
+            for (uint32_t i = done + threadIdx.x; i < d1; i += blockDim.x) {
+                uint32_t hi = nr - 1;
+                while (at < hi) {
+                    const uint32_t mid = (at + hi + 1) / 2;
+                    if (s_out[mid] <= i) at = mid;
+                    else hi = mid - 1;
+                }
+                uint32_t v = s_val[at];
+                if (s_src[at]) {
+                    const uint64_t bit = (uint64_t)(i - s_out[at]) * S.bw;
+                    v = (uint32_t)((pq_ld8((const uint8_t *)s_src[at] + (bit >> 3)) >> (bit & 7)) & mask);
+                }
+                ((uint32_t *)S.dst)[i] = v;
+            }
+            __syncthreads();  // the next batch rewrites the run table
+            if (fail) {
+                if (threadIdx.x == 0) __hip_atomic_fetch_or(bad, 4u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                break;
+            }
+            done = d1;
         }
-        if (r.dwidth == 1) ((uint8_t *)r.dst)[j] = (uint8_t)v;
-        else ((uint32_t *)r.dst)[j] = v;
     }
 }
 
@@ -86,33 +267,24 @@ struct PqSeg {
 };
 
 __global__ __launch_bounds__(256) void k_pq_pages(const PqSeg *__restrict__ segs, uint32_t nseg, uint64_t total,
-                                                  uint32_t *bad) {
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
-         i += (uint64_t)gridDim.x * blockDim.x) {
-        uint32_t lo = 0, hi = nseg - 1;
-        while (lo < hi) {
-            const uint32_t mid = (lo + hi + 1) / 2;
-            if (segs[mid].gstart <= i) lo = mid;
-            else hi = mid - 1;
-        }
-        const PqSeg sg = segs[lo];
-        const uint64_t j = i - sg.gstart;
-        uint64_t at = j;
-        bool ok = true;
-        if (sg.dict_n) {
-            at = ((const uint32_t *)sg.idx)[j];
-            ok = at < sg.dict_n;
-            if (!ok) __hip_atomic_fetch_or(bad, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-        if (sg.width == 4) {
-            uint32_t v = 0;
-            if (ok) __builtin_memcpy(&v, (const uint8_t *)sg.src + at * 4, 4);
-            ((uint32_t *)sg.dst)[j] = v;
-        } else {
-            uint64_t v = 0;
-            if (ok) __builtin_memcpy(&v, (const uint8_t *)sg.src + at * 8, 8);
-            ((uint64_t *)sg.dst)[j] = v;
-        }
+                                                  uint32_t items, uint32_t *bad) {
+    __shared__ PqSeg s_t[PQ_LDS_RUNS];
+    __shared__ uint32_t s_span[2];
+    const uint64_t tile = 256ull * items;
+    for (uint64_t t0 = (uint64_t)blockIdx.x * tile; t0 < total; t0 += (uint64_t)gridDim.x * tile) {
+        const uint64_t t1 = t0 + tile < total ? t0 + tile : total;
+        pq_tile(segs, nseg, t0, t1, items, s_t, s_span, [&](uint64_t i, const PqSeg &sg) {
+            const uint64_t j = i - sg.gstart;
+            uint64_t at = j;
+            bool ok = true;
+            if (sg.dict_n) {
+                at = ((const uint32_t *)sg.idx)[j];
+                ok = at < sg.dict_n;
+                if (!ok) __hip_atomic_fetch_or(bad, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            if (sg.width == 4) ((uint32_t *)sg.dst)[j] = ok ? pq_ld4((const uint8_t *)sg.src + at * 4) : 0u;
+            else ((uint64_t *)sg.dst)[j] = ok ? pq_ld8((const uint8_t *)sg.src + at * 8) : 0ull;
+        });
     }
 }
 
@@ -650,6 +822,11 @@ void parquet_chunk_range(const gbagg_parquet *pq, uint32_t g, uint32_t c, int64_
 // The runs and pages of every column of one decode call, launched together.
 struct PqCall {
     std::vector<PqRun> runs;
+    std::vector<PqStream> streams;  // dictionary-index streams whose run headers the device reads
+    // the device reads index run headers only in calls over many column chunks:
+    // one stream is a chain of dependent reads (tens of microseconds), cheap
+    // only when many run side by side; small calls read them on the host
+    bool dev_streams = false;
     std::vector<PqSeg> segs;
     uint64_t nvals = 0, ndense = 0;
     struct Nulls {
@@ -688,7 +865,7 @@ void parquet_plan_column(gbagg_parquet *pq, uint32_t g0, uint32_t g1, uint32_t c
     std::vector<PqRun> def_runs;            // levels: appended after the index runs once NULLs are known
     std::vector<PqRun> &idx_runs = call.runs;
     std::vector<PqSeg> &segs = call.segs;
-    const size_t irun0 = call.runs.size(), seg0 = call.segs.size();
+    const size_t irun0 = call.runs.size(), seg0 = call.segs.size(), str0 = call.streams.size();
     std::vector<uint32_t> remap;            // VARCHAR: dictionary code -> merged id, row group after row group
     std::vector<PqStrPage> plain_pages;     // VARCHAR: PLAIN pages (the column is then delivered as strings)
     std::vector<PqStrDict> dict_pages;      // VARCHAR: dictionary-encoded pages, for that case
@@ -800,11 +977,20 @@ void parquet_plan_column(gbagg_parquet *pq, uint32_t g0, uint32_t g1, uint32_t c
                 if (bw > 32) fail(GBAGG_INVALID, "parquet: index bit width above 32");
                 if (nvalid) {
                     if (!dict_n) fail(GBAGG_INVALID, "parquet: dictionary-encoded values with an empty dictionary");
-                    const size_t r0 = idx_runs.size();
-                    if (bw == 0) idx_runs.push_back(PqRun{0, dense, 0, nvalid, 0, 0, 4});
-                    else hybrid_runs(vals + 1, bend, c0, bw, nvalid, (uint32_t)dense, idx_runs, false);
-                    for (size_t k = r0; k < idx_runs.size(); k++)
-                        if (idx_runs[k].bw) idx_runs[k].src = (uint64_t)(dbuf + idx_runs[k].src);
+                    // bit width 0: every index is 0 (one RLE run); otherwise the
+                    // device reads the stream's run headers (dst: a dense index for
+                    // now), or the host does in small calls
+                    if (bw == 0) {
+                        idx_runs.push_back(PqRun{0, dense, 0, nvalid, 0, 0, 4});
+                    } else if (call.dev_streams) {
+                        call.streams.push_back(PqStream{(uint64_t)(dbuf + (vals + 1 - c0)),
+                                                        (uint64_t)(dbuf + (bend - c0)), dense, nvalid, bw});
+                    } else {
+                        const size_t r0 = idx_runs.size();
+                        hybrid_runs(vals + 1, bend, c0, bw, nvalid, (uint32_t)dense, idx_runs, false);
+                        for (size_t k = r0; k < idx_runs.size(); k++)
+                            if (idx_runs[k].bw) idx_runs[k].src = (uint64_t)(dbuf + idx_runs[k].src);
+                    }
                     // VARCHAR: src is the row group's first remap entry, made an address below
                     segs.push_back(PqSeg{dict_src, dense, dense, 0, nvalid, dict_n, width, varchar ? 1u : 0u});
                     if (varchar) {
@@ -824,7 +1010,10 @@ void parquet_plan_column(gbagg_parquet *pq, uint32_t g0, uint32_t g1, uint32_t c
 
     // buffers (row groups concatenated), then device addresses in the tables
     o.dense = pq->alloc((size_t)dense * width);
-    uint32_t *idx_p = call.runs.size() == irun0 ? nullptr : (uint32_t *)(o.idx = pq->alloc((size_t)dense * 4));
+    uint32_t *idx_p = call.runs.size() == irun0 && call.streams.size() == str0
+                          ? nullptr
+                          : (uint32_t *)(o.idx = pq->alloc((size_t)dense * 4));
+    for (size_t k = str0; k < call.streams.size(); k++) call.streams[k].dst = (uint64_t)(idx_p + call.streams[k].dst);
     const uint32_t *remap_p = nullptr;
     if (!remap.empty()) {
         uint8_t *rp = pq->alloc(remap.size() * 4);
@@ -942,7 +1131,8 @@ void parquet_launch(gbagg_parquet *pq, PqCall &call) {
     // ended with a stream synchronisation, so the staging buffer is free)
     const size_t rb = call.runs.size() * sizeof(PqRun), sb = call.segs.size() * sizeof(PqSeg);
     const size_t pb = call.spages.size() * sizeof(PqStrPage), db = call.sdict.size() * sizeof(PqStrDict);
-    const size_t tb = rb + sb + pb + db;
+    const size_t qb = call.streams.size() * sizeof(PqStream);
+    const size_t tb = rb + sb + pb + db + qb;
     if (tb) {
         if (pq->htab_bytes < tb) {
             if (pq->htab) HIPCHK(hipHostFree(pq->htab));
@@ -955,18 +1145,29 @@ void parquet_launch(gbagg_parquet *pq, PqCall &call) {
         if (sb) std::memcpy(pq->htab + rb, call.segs.data(), sb);
         if (pb) std::memcpy(pq->htab + rb + sb, call.spages.data(), pb);
         if (db) std::memcpy(pq->htab + rb + sb + pb, call.sdict.data(), db);
+        if (qb) std::memcpy(pq->htab + rb + sb + pb + db, call.streams.data(), qb);
         pq->dtab.ensure(tb);
         HIPCHK(hipMemcpyAsync(pq->dtab.ptr, pq->htab, tb, hipMemcpyHostToDevice, s));
     }
+    // values per thread per tile: enough tiles for `cap` workgroups, at most PQ_ITEMS
+    auto items_for = [&](uint64_t n) {
+        return (uint32_t)std::min<uint64_t>(PQ_ITEMS, std::max<uint64_t>(1, n / ((uint64_t)cap * 256)));
+    };
+    if (!call.streams.empty())
+        hipLaunchKernelGGL(k_pq_streams, dim3((uint32_t)std::min<size_t>(call.streams.size(), cap)), dim3(256), 0, s,
+                           (const PqStream *)(pq->dtab.as<uint8_t>() + rb + sb + pb + db),
+                           (uint32_t)call.streams.size(), pq->bad.as<uint32_t>());
     if (!call.runs.empty()) {
-        const uint32_t grid = (uint32_t)std::min<uint64_t>((call.nvals + 255) / 256, cap);
+        const uint32_t items = items_for(call.nvals);
+        const uint32_t grid = (uint32_t)std::min<uint64_t>((call.nvals + 256ull * items - 1) / (256ull * items), cap);
         hipLaunchKernelGGL(k_pq_hybrid, dim3(grid), dim3(256), 0, s, (const PqRun *)pq->dtab.ptr,
-                           (uint32_t)call.runs.size(), call.nvals);
+                           (uint32_t)call.runs.size(), call.nvals, items);
     }
     if (!call.segs.empty()) {
-        const uint32_t grid = (uint32_t)std::min<uint64_t>((call.ndense + 255) / 256, cap);
+        const uint32_t items = items_for(call.ndense);
+        const uint32_t grid = (uint32_t)std::min<uint64_t>((call.ndense + 256ull * items - 1) / (256ull * items), cap);
         hipLaunchKernelGGL(k_pq_pages, dim3(grid), dim3(256), 0, s, (const PqSeg *)(pq->dtab.as<uint8_t>() + rb),
-                           (uint32_t)call.segs.size(), call.ndense, pq->bad.as<uint32_t>());
+                           (uint32_t)call.segs.size(), call.ndense, items, pq->bad.as<uint32_t>());
     }
     if (!call.spages.empty())
         hipLaunchKernelGGL(k_pq_str_plain, dim3((uint32_t)((call.spages.size() + 63) / 64)), dim3(64), 0, s,

@@ -93,12 +93,17 @@ def _check_file(image, t, rg_rows, resident=False):
     f.close()
 
 
-@pytest.mark.parametrize("version,resident", [("1.0", False), ("2.0", False), ("1.0", True)])
-def test_decode_matches_pyarrow(version, resident):
+# 122,880-row row groups: every call reads few column chunks, and the host
+# reads the index run headers; 10,000-row ones: the whole-file call reads 210
+# chunks, and the device reads them (k_pq_streams)
+@pytest.mark.parametrize("version,resident,rg_rows", [("1.0", False, 122_880), ("2.0", False, 122_880),
+                                                      ("1.0", True, 122_880), ("1.0", True, 10_000),
+                                                      ("2.0", False, 10_000)])
+def test_decode_matches_pyarrow(version, resident, rg_rows):
     t = _table(300_000, 1)
-    image = _write(t, row_group_size=122_880, data_page_size=32 * 1024, data_page_version=version,
+    image = _write(t, row_group_size=rg_rows, data_page_size=32 * 1024, data_page_version=version,
                    dictionary_pagesize_limit=64 * 1024, use_dictionary=["a", "b", "k", "d", "u", "s"])
-    _check_file(image, t, 122_880, resident)
+    _check_file(image, t, rg_rows, resident)
     # s overflows its dictionary: delivered as strings; b stays dictionary codes
     f = G.ParquetFile(image)
     cols = f.decode(0)
