@@ -650,8 +650,16 @@ uint64_t hybrid_runs(const uint8_t *p, const uint8_t *end, const uint8_t *chunk0
             if (bytes > (uint64_t)(r.end - r.p)) fail(GBAGG_INVALID, "parquet: truncated bit-packed run");
             const uint32_t cnt = (uint32_t)std::min<uint64_t>(groups * 8, n - done);
             if (cnt) runs.push_back(PqRun{bw ? (uint64_t)(r.p - chunk0) : 0, (uint64_t)(out0 + done), 0, cnt, 0, bw, 0});
-            if (count_ones)
-                for (uint32_t k = 0; k < cnt; k++) ones += (r.p[k >> 3] >> (k & 7)) & 1;
+            if (count_ones) {  // bw = 1: the first cnt bits, least-significant first, 64 at a time
+                uint32_t k = 0;
+                for (; k + 64 <= cnt; k += 64) {
+                    uint64_t w;
+                    std::memcpy(&w, r.p + k / 8, 8);
+                    ones += (uint64_t)__builtin_popcountll(w);
+                }
+                for (; k + 8 <= cnt; k += 8) ones += (uint64_t)__builtin_popcount(r.p[k / 8]);
+                if (k < cnt) ones += (uint64_t)__builtin_popcount(r.p[k / 8] & ((1u << (cnt - k)) - 1));
+            }
             r.p += bytes;
             done += cnt;
         } else {  // RLE: (h >> 1) repeats of one value
