@@ -3435,9 +3435,10 @@ gbagg_status gbagg_parquet_decode_range(gbagg_parquet *p, uint32_t rg0, uint32_t
             parquet_plan_column(p, rg0, rg1, cols[i], dbufs.data() + (size_t)i * ng, *p->outs.back(), out[i], call);
         }
         parquet_launch(p, call);
+        if (!p->hbad) HIPCHK(hipHostMalloc((void **)&p->hbad, 4, hipHostMallocDefault));
+        HIPCHK(hipMemcpyAsync(p->hbad, p->bad.ptr, 4, hipMemcpyDeviceToHost, p->stream));
         HIPCHK(hipStreamSynchronize(p->stream));  // the tables' host copies end with this call
-        uint32_t bad = 0;
-        HIPCHK(hipMemcpy(&bad, p->bad.ptr, 4, hipMemcpyDeviceToHost));
+        const uint32_t bad = *p->hbad;
         if (bad & 1) fail(GBAGG_INVALID, "parquet: dictionary index outside the dictionary");
         if (bad & 2) fail(GBAGG_INVALID, "parquet: truncated BYTE_ARRAY value");
         if (bad & 4) fail(GBAGG_INVALID, "parquet: malformed dictionary index stream");

@@ -704,6 +704,7 @@ struct gbagg_parquet {
     DevBuf dtab;            // per decode call: the runs and pages of all its columns
     uint8_t *htab = nullptr;  // pinned host staging of those tables (one async copy per call)
     size_t htab_bytes = 0;
+    uint32_t *hbad = nullptr;  // pinned host copy of the call's error bits, read after its wait
     // decoded columns are bump-allocated from arenas of 256 MB or more (one
     // hipMalloc per arena, not per column), all kept until close
     // (gbagg_parquet_reset forgets them and reuses the arenas)
@@ -727,6 +728,7 @@ struct gbagg_parquet {
     ~gbagg_parquet() {
         if (stream) (void)hipStreamDestroy(stream);
         if (htab) (void)hipHostFree(htab);
+        if (hbad) (void)hipHostFree(hbad);
     }
 };
 
