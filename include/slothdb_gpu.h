@@ -433,7 +433,10 @@ gbagg_status gbagg_selector_infer(const float *params, uint32_t num_params, cons
  * Supported: flat schemas; uncompressed column chunks; INT32, INT64, DOUBLE,
  * BYTE_ARRAY; data pages V1 and V2; PLAIN, PLAIN_DICTIONARY and
  * RLE_DICTIONARY. Anything else is
- * GBAGG_INVALID with a message. The file image must outlive the handle; every
+ * GBAGG_INVALID with a message, and so is malformed data: a page outside its
+ * chunk, truncated values, levels or index streams, a dictionary index
+ * outside its dictionary, a string length past its page (the kernels check
+ * what they read and report through error bits; the call then fails). The file image must outlive the handle; every
  * decoded column stays in HBM until close, so one operator can consume many
  * row groups before its finalize. */
 typedef struct gbagg_parquet gbagg_parquet;
