@@ -1,10 +1,11 @@
-"""The high-cardinality GROUP BY at its full size: 100M rows, about 4.85M
+"""The headline workloads at full size. The high-cardinality GROUP BY: 100M rows, about 4.85M
 groups (SELECT k, COUNT(*), SUM(v1), MIN(v2), MAX(v2), AVG(v3) GROUP BY k; the
 reference runs it on its per-thread-map path, physical_planner.cpp:7613-7931).
 The result is checked against an exact sort-based aggregation in numpy
 (COUNT, SUM, MIN, MAX per group, bit for bit) and against properties that hold
 at any size: every key once, the counts summing to the rows, the sum of
-AVG x COUNT equal to the column's sum."""
+AVG x COUNT equal to the column's sum. The composite-key TopN: 50M rows,
+against exact numpy counts and sums."""
 import numpy as np
 import pytest
 
@@ -41,3 +42,33 @@ def test_highcard_100m_exact():
     v3 = cols[3].values
     tot = float(np.sum(avg * cnt.astype(np.float64)))
     assert abs(tot - float(np.sum(v3))) <= 1e-9 * float(np.sum(np.abs(v3)))
+
+
+def test_composite_50m_topn_exact():
+    # WHERE c < 30 GROUP BY a, b ORDER BY COUNT(*) DESC LIMIT 10 over 50M rows
+    # (a INTEGER, b dictionary VARCHAR of 1M strings): the returned counts are
+    # the exact top 10 (ties allowed at the boundary), and every returned
+    # group carries its exact COUNT and SUM(c)
+    q, cols = datagen.composite_workload()
+    a, b, c = cols[0].values, cols[1].values, cols[2].values
+    dic = cols[1].dictionary
+    print(f"[fullsize] {len(a)} rows: composite TopN on the GPU", flush=True)
+    res, info = G.run(q, cols, device=True)
+    rows = res.rows()
+    assert len(rows) == 10
+    print("[fullsize] numpy exact counts", flush=True)
+    # SQL groups b by its string: each code maps to the first code of its string
+    first, canon = {}, np.empty(len(dic), np.int64)
+    for code in range(len(dic)):
+        canon[code] = first.setdefault(dic.get(code), code)
+    keep = c < 30
+    key = a[keep].astype(np.int64) * (1 << 32) + canon[b[keep]]
+    uk, inv, cnt = np.unique(key, return_inverse=True, return_counts=True)
+    sums = np.bincount(inv, weights=c[keep].astype(np.float64))
+    top = np.sort(cnt)[::-1][:10]
+    assert sorted((r[2] for r in rows), reverse=True) == [int(x) for x in top]
+    for ra, rb, rcnt, rsum in rows:
+        kk = int(ra) * (1 << 32) + first[rb]
+        i = int(np.searchsorted(uk, kk))
+        assert i < len(uk) and uk[i] == kk, (ra, rb)
+        assert rcnt == int(cnt[i]) and rsum == int(round(sums[i])), (ra, rb)
