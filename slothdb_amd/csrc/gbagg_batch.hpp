@@ -23,7 +23,9 @@
 //
 // Tables hold ceil(1.5 * estimate) + 64 slots rounded up to 256 (multiply-shift
 // home slot, linear probing, bounded); a query whose table overflows is flagged
-// and the host re-runs it on the single-query operator.
+// and the host re-runs it on the single-query operator. So is a query with the
+// key INT64_MIN, which equals the tables' empty-slot marker (the single-query
+// operator keeps that key in a side record).
 #pragma once
 
 #include "gbagg_kernels.hpp"
@@ -284,6 +286,10 @@ __global__ void k_b_agg(BatchQuery *__restrict__ bq, BatchCtl *ctl, const uint2 
             __syncthreads();
             for (uint64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
                 const uint64_t k = bq_key(q, r);
+                if (k == SENTINEL) {  // INT64_MIN is the empty-slot marker: the query re-runs unbatched
+                    q.overflow = 1;
+                    continue;
+                }
                 // LDS slot of k: find or claim, bounded; else the global table
                 uint32_t pos = (uint32_t)(((uint64_t)(uint32_t)hash64(k) * S) >> 32);
                 int slot = -1;
@@ -336,6 +342,10 @@ __global__ void k_b_agg(BatchQuery *__restrict__ bq, BatchCtl *ctl, const uint2 
         } else {
             for (uint64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
                 const uint64_t k = bq_key(q, r);
+                if (k == SENTINEL) {
+                    q.overflow = 1;
+                    continue;
+                }
                 const int64_t g = bq_find(q, pool, k);
                 if (g < 0) {
                     q.overflow = 1;

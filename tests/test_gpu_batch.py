@@ -149,3 +149,50 @@ def test_batch_mode_rules_keep_results_exact(rule):
         exp, types = _exact(q, cols)
         assert_same_groups(b.result(i).rows(), exp, 1, types)
     b.close()
+
+
+def test_batch_int64_min_key_runs_unbatched_and_exact():
+    # INT64_MIN equals the batched tables' empty-slot marker: such a query is
+    # flagged and re-runs on the single-query operator, which keeps that key in
+    # a side record; the other queries stay batched
+    rng = np.random.default_rng(17)
+    n = 200_000
+    k = rng.integers(-1000, 1000, n).astype(np.int64)
+    k[rng.random(n) < 0.1] = np.iinfo(np.int64).min
+    v = rng.integers(-1 << 40, 1 << 40, n)
+    q = G.Query([G.BIGINT, G.BIGINT], [0], [(G.COUNT_STAR, None), (G.SUM, 1), (G.MIN, 1), (G.MAX, 1)],
+                nullable=[False, False])
+    special = (q, [G.Column(G.BIGINT, k), G.Column(G.BIGINT, v)])
+    host, dev = _entries(dict(rows_hi=100_000, distinct_hi=50_000), 6, [special])
+    b = G.GpuBatch()
+    b.set_queries(dev)
+    b.run()
+    info = b.info()
+    assert info["single"] == 1 and info["batched"] == len(host) - 1
+    for i, (qq, cols) in enumerate(host):
+        exp, types = _exact(qq, cols)
+        assert_same_groups(b.result(i).rows(), exp, 1, types)
+    got = {r[0]: r for r in b.result(len(host) - 1).rows()}
+    assert np.iinfo(np.int64).min in got
+    b.close()
+
+
+@pytest.mark.slow
+def test_batch_full_512_queries_vs_oracle():
+    # the full mixed workload: 512 queries, 1K-4M rows each, every one checked
+    # against the oracle bit for bit
+    host = datagen.batch_workload(nq=512, seed=7)
+    dev = [(q, [G.DeviceColumn(c) for c in cols]) for q, cols in host]
+    b = G.GpuBatch()
+    b.set_queries(dev)
+    b.run()
+    info = b.info()
+    print(f"[batch512] {info}", flush=True)
+    assert info["queries"] == 512
+    rows = 0
+    for i, (q, cols) in enumerate(host):
+        exp, types = _exact(q, cols)
+        assert_same_groups(b.result(i).rows(), exp, 1, types)
+        rows += len(cols[0].values)
+    print(f"[batch512] {rows} rows checked", flush=True)
+    b.close()
