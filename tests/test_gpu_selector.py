@@ -133,3 +133,30 @@ def test_default_selector_keeps_results_exact():
     exp, types = _exact(q, cols)
     assert_same_groups(op.result().rows(), exp, 1, types)
     op.close()
+
+
+def test_selector_workload_queries_match_oracle():
+    # The learned-selector workload: synthetic queries with rows log-uniform up
+    # to 2M, distinct keys log-uniform in [1, rows], Zipf s in [0, 1.5], every
+    # key kind (INTEGER sparse and dense, BIGINT, 2 x INTEGER, 2 x BIGINT: 4, 8
+    # and 16 key bytes), run with the committed trained selector attached.
+    # Every query's groups equal the oracle's bit for bit.
+    from slothdb_amd import selector_train as T
+    params, _ = S.load()
+    specs = T.specs(30, 2_000_000, seed=20261204)
+    paths = set()
+    for i, (rows, distinct, s, _, qseed) in enumerate(specs):
+        key = T.KEYS[i % len(T.KEYS)]
+        q, cols = T.make_query(rows, distinct, s, key, qseed)
+        print(f"[c4] query {i}: {rows} rows, {distinct} distinct, s={s:.2f}, key {key}", flush=True)
+        op = G.GpuHashAggregate(q)
+        op.Init()
+        S.attach(op, params)
+        op.consume_device([G.DeviceColumn(c) for c in cols])
+        op.finalize()
+        info = op.plan_info()
+        paths.add(info["strategy"])
+        exp, types = _exact(q, cols)
+        assert_same_groups(op.result().rows(), exp, len(q.group_cols), types)
+        op.close()
+    print(f"[c4] strategies used: {sorted(paths)}", flush=True)
