@@ -276,6 +276,9 @@ typedef struct {
     int32_t selector_path;      /* the selector's path: 0 LDS_HASH, 1 GLOBAL_HASH, 2 RADIX; -1 = not run */
     double ms_strings;          /* interning of GBAGG_STR_RAW columns, before the aggregate (0 = none) */
     uint64_t strings_distinct;  /* distinct strings over those columns */
+    uint32_t sum_reruns;        /* re-runs for double SUM/AVG: exact bound, or IEEE atomics */
+    int32_t sum_ieee;           /* 1: double SUM/AVG ran with IEEE double atomics (order-dependent);
+                                   0: exact fixed-point sums, identical for any row order */
 } gbagg_plan_info;
 
 typedef struct gbagg_handle gbagg_handle;

@@ -18,6 +18,10 @@
  *                          cross-checks, the reference's InlineRowAgg<N>
  *                          and RadixCountAgg built into oracle/_ref, and
  *                          golden fixtures of the plain aggregate workload.
+ *                          Double SUM/AVG specials (IEEE result of the
+ *                          exact sum) and MIN/MAX over NaN (skipped unless
+ *                          a group holds only NaN) against math.fsum and
+ *                          brute-force rules on small inputs.
  *   orc_port_groupby       equal to orc_groupby_exact wherever the
  *                          reference's double arithmetic is exact
  *                          (|partial sums| < 2^53).
@@ -227,7 +231,7 @@ typedef struct {
     double *fsum, *fcomp;
     int64_t *imin, *imax;
     double *fmin, *fmax;
-    uint8_t *fhas;
+    uint8_t *fhas; /* MIN/MAX: a non-NaN value seen; SUM/AVG: 1 = +Inf seen, 2 = -Inf seen (NaN: both) */
 } accs;
 
 typedef struct gtable {
@@ -403,7 +407,15 @@ static inline void row_update(const exact_ctx *x, gtable *t, uint64_t g, uint64_
         if (c->type == GBAGG_TYPE_DOUBLE) {
             double v = ((const double *)c->data)[r];
             if (ag->kind == GBAGG_AGG_SUM || ag->kind == GBAGG_AGG_AVG) {
-                neumaier_add(&t->a.fsum[i], &t->a.fcomp[i], v);
+                /* IEEE semantics of the exact sum: a NaN, or +Inf together
+                 * with -Inf, gives NaN; otherwise an infinity gives itself;
+                 * finite values are summed compensated */
+                if (isnan(v)) t->a.fhas[i] |= 3;
+                else if (isinf(v)) t->a.fhas[i] |= v > 0 ? 1 : 2;
+                else neumaier_add(&t->a.fsum[i], &t->a.fcomp[i], v);
+            } else if (isnan(v)) {
+                /* MIN/MAX skip NaN unless every input of the group is NaN */
+                if (!t->a.fhas[i]) *(ag->kind == GBAGG_AGG_MIN ? &t->a.fmin[i] : &t->a.fmax[i]) = NAN;
             } else if (ag->kind == GBAGG_AGG_MIN) {
                 if (!t->a.fhas[i] || v < t->a.fmin[i]) t->a.fmin[i] = v;
                 t->a.fhas[i] = 1;
@@ -499,6 +511,15 @@ static int out_type_of(const gbagg_query_desc *q, const gbagg_column_view *cols,
     }
 }
 
+/* The double SUM of (group, aggregate) i: the specials' IEEE result, else
+ * the compensated sum. */
+static double fsum_value(const accs *a, uint64_t i) {
+    if (a->fhas[i] == 3) return NAN;
+    if (a->fhas[i] == 1) return INFINITY;
+    if (a->fhas[i] == 2) return -INFINITY;
+    return a->fsum[i] + a->fcomp[i];
+}
+
 static void result_alloc(orc_result *out, const gbagg_query_desc *q, const gbagg_column_view *cols,
                          uint64_t n) {
     memset(out, 0, sizeof(*out));
@@ -536,7 +557,7 @@ static void emit_exact(const gbagg_query_desc *q, const gbagg_column_view *cols,
         case GBAGG_AGG_SUM:
             out->valid[c][row] = cnt > 0;
             if (isdbl) {
-                d = t->a.fsum[i] + t->a.fcomp[i];
+                d = fsum_value(&t->a, i);
                 memcpy(&out->cells[c][row], &d, 8);
             } else {
                 __int128 s = t->a.isum[i];
@@ -547,7 +568,7 @@ static void emit_exact(const gbagg_query_desc *q, const gbagg_column_view *cols,
         case GBAGG_AGG_AVG:
             out->valid[c][row] = cnt > 0;
             if (cnt > 0) {
-                d = isdbl ? (t->a.fsum[i] + t->a.fcomp[i]) / (double)cnt
+                d = isdbl ? fsum_value(&t->a, i) / (double)cnt
                           : (double)t->a.isum[i] / (double)cnt;
                 memcpy(&out->cells[c][row], &d, 8);
             }

@@ -4,6 +4,7 @@
   slothdb_amd/lib/libslothdb_datagen.so  seeded synthetic inputs (host C++)
   slothdb_amd/lib/libslothdb_gpu_op.so   C++20 PhysicalGpuHashAggregate over the C ABI
   slothdb_amd/lib/test_operator          its C++ test program (tests/cpp/test_operator.cpp)
+  slothdb_amd/lib/test_fixed_point       host test of the exact double SUM arithmetic
 
 Run as `python -m slothdb_amd.build` or through __graft_entry__.build().
 """
@@ -71,6 +72,18 @@ def build_host():
     return so
 
 
+def build_fixed_point_test():
+    """Host-side test program of the exact double SUM arithmetic
+    (tests/cpp/test_fixed_point.cpp), which includes the HIP device header."""
+    os.makedirs(LIB, exist_ok=True)
+    exe = os.path.join(LIB, "test_fixed_point")
+    _run([HIPCC, "-x", "hip", f"--offload-arch={ARCH}", "-O2", "-std=c++17",
+          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wno-unused-command-line-argument",
+          "-o", exe + ".tmp", os.path.join(ROOT, "tests", "cpp", "test_fixed_point.cpp")])
+    os.replace(exe + ".tmp", exe)
+    return exe
+
+
 def build_oracle():
     """The CPU oracle (test infrastructure) and, with the reference tree
     present, oracle/_ref built from the reference's own aggregator sources."""
@@ -83,6 +96,7 @@ def build_all():
     build_oracle()
     build_gpu()
     build_host()
+    build_fixed_point_test()
 
 
 if __name__ == "__main__":

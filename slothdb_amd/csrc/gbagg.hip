@@ -86,7 +86,7 @@ using ShPlain = StaticShape<2, 1, 0x4, 0, KM_SINGLE, 0, 0, 4, 0x5420, 0x111F, 0,
 // GROUP BY int32 k: COUNT(*), SUM(int64 v)                              (mixed-query batches)
 using ShCountSum = StaticShape<2, 1, 0x4, 0, KM_SINGLE, 0, 0, 2, 0x20, 0x1F, 0, 0>;
 // GROUP BY int64 k: COUNT(*), SUM(int64), MIN/MAX(int32), SUM/AVG(double) (high cardinality)
-using ShHighcard = StaticShape<4, 1, 0x85, 0, KM_SINGLE, 0, 0, 5, 0x35420, 0x3221F, 0, 0>;
+using ShHighcard = StaticShape<4, 1, 0x85, 0, KM_SINGLE, 0, 0, 6, 0x835420, 0x33221F, 0, 0>;
 // WHERE c < lit GROUP BY int32 a, VARCHAR b: COUNT(*), SUM(int32 c)   (composite key + filter)
 using ShComposite = StaticShape<3, 3, 0xC, 0, KM_PACK32, 0, 1, 2, 0x20, 0x2F, 1, 2>;
 // GROUP BY int64 k: COUNT(*)                                          (RadixCountAgg shape)
@@ -262,6 +262,7 @@ struct OutCol {
     int val_acc;
     int cnt_acc;
     int key_type;
+    int lo_acc = -1;  // double SUM/AVG: the ACC_SUM_FL accumulator (val_acc: ACC_SUM_FH)
 };
 
 struct Batch {
@@ -336,6 +337,13 @@ struct gbagg_handle {
     std::unique_ptr<StrCol> tsc;        // the tuples' interning
     DevBuf tpack, tkeep, tcomp, touts;
     std::vector<DevBuf> tout_data, tout_valid;  // group columns unpacked on the device
+    // double sums (fixed-point words, see AccOp): per accumulator the scale s
+    std::vector<std::pair<int, int>> fx_pairs;  // (ACC_SUM_FH, ACC_SUM_FL) accumulators per DOUBLE column
+    bool fx_ieee = false;          // after OVF_FX_PREC: IEEE double atomics instead
+    bool fx_exact = false;         // scales from the columns' exact maxima (after OVF_FX)
+    bool fx_known = false;         // fx_exp holds those maxima
+    uint32_t fx_exp[MAXU] = {};    // exact largest biased exponents per loaded slot
+    uint32_t fx_reruns = 0;
     // TopN on the device
     bool topn_device = false;
     DevTopn tspec{};
@@ -567,7 +575,12 @@ void build_plan(gbagg_handle *h, const gbagg_query_desc *q) {
                 if (t == GBAGG_TYPE_VARCHAR) fail(GBAGG_INVALID, "SUM/AVG require a numeric argument");
                 oc.cnt_acc = cnt;
                 if (t == GBAGG_TYPE_DOUBLE) {
-                    oc.val_acc = add_acc(h, ACC_SUM_F, u);
+                    // exact fixed-point words; fx_mode() swaps in IEEE atomics when needed
+                    oc.val_acc = add_acc(h, ACC_SUM_FH, u);
+                    oc.lo_acc = add_acc(h, ACC_SUM_FL, u);
+                    const std::pair<int, int> pr(oc.val_acc, oc.lo_acc);
+                    if (std::find(h->fx_pairs.begin(), h->fx_pairs.end(), pr) == h->fx_pairs.end())
+                        h->fx_pairs.push_back(pr);
                     oc.kind = ag.kind == GBAGG_AGG_SUM ? OUT_SUM_F : OUT_AVG_F;
                     oc.type = GBAGG_TYPE_DOUBLE;
                 } else {
@@ -630,6 +643,7 @@ void build_plan(gbagg_handle *h, const gbagg_query_desc *q) {
         t.key_signed[0] = h->dq.key_signed32[0];
         t.key_signed[1] = h->dq.key_signed32[1];
         t.acc = oc.val_acc;
+        t.lo_acc = oc.lo_acc < 0 ? 0 : oc.lo_acc;
         t.cnt_acc = oc.cnt_acc < 0 ? 0 : oc.cnt_acc;
         bool ok = h->topn_limit <= 32;
         bool never_null = oc.cnt_acc <= 0;
@@ -1210,6 +1224,9 @@ void fill_out_topn(gbagg_handle *h) {
         o.val_acc[c] = h->outs[c].val_acc;
         o.cnt_acc[c] = h->outs[c].cnt_acc;
         o.key_type[c] = h->outs[c].key_type;
+        const int lo = h->outs[c].lo_acc;
+        o.lo_acc[c] = lo >= 0 && h->dq.acc_op[lo] == ACC_SUM_FL ? lo : -1;
+        o.fx_shift[c] = lo >= 0 ? h->dq.acc_fx[lo] : 0;
     }
     DevTopn &t = h->hctx->t;
     t = h->tspec;
@@ -1221,6 +1238,10 @@ void fill_out_topn(gbagg_handle *h) {
     t.wide_acc[0] = h->wide_acc[0];
     t.wide_acc[1] = h->wide_acc[1];
     t.flags = o.flags;
+    if (t.kind == TK_SUM_F || t.kind == TK_AVG_F) {
+        t.fx_shift = h->dq.acc_fx[t.lo_acc];
+        if (h->dq.acc_op[t.lo_acc] != ACC_SUM_FL) t.lo_acc = -1;
+    }
 }
 
 // The host context is pinned; callers modify it only after the stream has
@@ -1405,6 +1426,54 @@ void run_features(gbagg_handle *h, bool infer) {
     HIPCHK(hipMemcpyAsync(h->feat, h->dfeat.ptr, SEL_IN * 4, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     h->feat_valid = true;
+}
+
+// ---- double sums ------------------------------------------------------------
+// Fixed-point words (exact, order-independent) or, after a group flagged
+// possible precision loss, IEEE double atomics (see AccOp).
+void fx_mode(gbagg_handle *h, bool ieee) {
+    if (h->fx_pairs.empty() || h->fx_ieee == ieee) return;
+    for (const auto &pr : h->fx_pairs) {
+        h->dq.acc_op[pr.first] = ieee ? ACC_SUM_F : ACC_SUM_FH;
+        h->dq.acc_op[pr.second] = ieee ? ACC_CNT_STAR : ACC_SUM_FL;  // unused under atomics
+    }
+    h->fx_ieee = ieee;
+    pick_kernels(h);
+}
+
+constexpr uint32_t FX_MARGIN = 4;  // bound over the sampled maximum: 16x
+
+// The scale s of each fixed-point double SUM: |v| < 2^(63 - s) for every input,
+// from the sampled maximum with a margin (a row beyond it sets OVF_FX and the
+// query re-runs), or from the exact maximum over all rows (k_fx_range) when
+// the sample saw no value of the column or after such a re-run. Updates and
+// uploads the device context.
+void fx_prepare(gbagg_handle *h, bool sampled) {
+    if (h->fx_pairs.empty() || h->fx_ieee) return;
+    const DeviceInfo &di = device_info();
+    bool exact = h->fx_exact || h->fx_known || !sampled;
+    for (const auto &pr : h->fx_pairs)
+        if (sampled && h->hplan->fx_exp[h->dq.acc_u[pr.first]] == 0) exact = true;
+    if (exact && !h->fx_known) {
+        for (size_t i = 0; i < h->batches.size(); i++) {
+            const auto &b = h->batches[i];
+            if (!b.nrows) continue;
+            const uint32_t g = (uint32_t)std::min<uint64_t>((b.nrows + 255) / 256, (uint64_t)di.cus * 8);
+            hipLaunchKernelGGL(k_fx_range, dim3(g), dim3(256), 0, h->stream, d_q(h), d_b(h, i), h->dplan.as<DevPlan>());
+        }
+        read_plan(h);
+        std::memcpy(h->fx_exp, h->hplan->fx_exp, sizeof(h->fx_exp));
+        h->fx_known = true;
+    }
+    for (const auto &pr : h->fx_pairs) {
+        const int u = h->dq.acc_u[pr.first];
+        const uint32_t e = exact ? h->fx_exp[u] : h->hplan->fx_exp[u] + FX_MARGIN;
+        const int sc = e ? std::max(-1100, std::min(FX_SHIFT_MAX, 1085 - (int)e)) : 0;
+        h->dq.acc_fx[pr.first] = h->dq.acc_fx[pr.second] = sc;
+        h->hctx->q.acc_fx[pr.first] = h->hctx->q.acc_fx[pr.second] = sc;
+    }
+    fill_out_topn(h);
+    upload_ctx(h);
 }
 
 void run_lds_hash(gbagg_handle *h, Timer &tm, bool global_only, uint64_t est = 0) {
@@ -2198,14 +2267,30 @@ void finalize(gbagg_handle *h) {
     h->sel_distinct = 0;
     h->feat_valid = false;
     h->radix_bits = 0;
-    // 16-byte keys rerun with the next seed while two groups share a key word
-    for (uint32_t attempt = 0;; attempt++) {
-        h->dq.wide_seed = h->wide_seed0 + attempt;
+    fx_mode(h, false);
+    h->fx_exact = h->fx_known = false;
+    h->fx_reruns = 0;
+    // 16-byte keys rerun with the next seed while two groups share a key word;
+    // double sums rerun with the exact bound (a value beyond the sampled one)
+    // or with IEEE atomics (a group that may have lost precision)
+    for (uint32_t attempt = 0, seed = 0;; attempt++) {
+        h->dq.wide_seed = h->wide_seed0 + seed;
         finalize_once(h);
-        if (h->dq.key_mode != KM_WIDE) break;
         read_plan(h);
-        if (!(h->hplan->overflow & OVF_WIDE)) break;
-        if (attempt == 7) fail(GBAGG_ERROR, "16-byte group keys: key words still shared after 8 seeds");
+        const uint32_t ov = h->hplan->overflow;
+        if ((ov & OVF_FX) && !h->fx_ieee && !h->fx_exact) {
+            h->fx_exact = true;
+            h->fx_reruns++;
+            continue;
+        }
+        if (ov & OVF_FX) fail(GBAGG_ERROR, "double SUM: an input beyond the exact fixed-point bound");
+        if ((ov & OVF_FX_PREC) && !h->fx_ieee) {
+            fx_mode(h, true);
+            h->fx_reruns++;
+            continue;
+        }
+        if (h->dq.key_mode != KM_WIDE || !(ov & OVF_WIDE)) break;
+        if (++seed == 8) fail(GBAGG_ERROR, "16-byte group keys: key words still shared after 8 seeds");
         h->retries++;
     }
     h->result_rows = h->hplan->groups;
@@ -2325,6 +2410,7 @@ void finalize_once(gbagg_handle *h) {
     }
     std::memset(h->hplan, 0, sizeof(DevPlan));
     std::memset(h->hctx, 0, sizeof(DevCtx));
+    h->dq.fx_flags = &h->dplan.as<DevPlan>()->overflow;
     h->hctx->q = h->dq;
     fill_out_topn(h);
     upload_batches(h);
@@ -2332,6 +2418,8 @@ void finalize_once(gbagg_handle *h) {
     upload_plan(h, 1024);
     uint64_t span = 0;
     int64_t kmin = 0;
+    if (st == GBAGG_STRATEGY_DIRECT || (st == GBAGG_STRATEGY_GLOBAL_HASH && h->dq.key_mode == KM_NONE))
+        fx_prepare(h, false);  // no sample on these paths
     if (st == GBAGG_STRATEGY_DIRECT) {
         tm.mark();  // no sample
         if (!direct_range(h, span, kmin)) {
@@ -2350,6 +2438,7 @@ void finalize_once(gbagg_handle *h) {
         if (sel || h->collect_features) run_features(h, sel);
         tm.mark();  // sample end
         read_plan(h);
+        fx_prepare(h, true);
         if (st == GBAGG_STRATEGY_AUTO) {
             const DevPlan &p = *h->hplan;
             bool low_card = p.sample_distinct <= 4096 && p.sample_f1 * 8 <= p.sample_distinct + 8;
@@ -2755,6 +2844,8 @@ gbagg_status gbagg_plan_info_get(gbagg_handle *h, gbagg_plan_info *out) {
         out->selector_path = h->sel_path;
         out->ms_strings = h->ms_strings;
         out->strings_distinct = h->strings_distinct;
+        out->sum_reruns = h->fx_reruns;
+        out->sum_ieee = h->fx_ieee ? 1 : 0;
     });
 }
 

@@ -18,16 +18,45 @@ constexpr int HLL_P = 12;
 constexpr int HLL_M = 1 << HLL_P;
 
 // Accumulator kinds. Every accumulator is one 64-bit word per group.
+//
+// A double SUM is exact and independent of the order of the additions: each
+// finite input v becomes the integer X = round(v * 2^s) (|X| < 2^63; s per
+// accumulator, chosen by the host from the largest magnitude of the column),
+// split as X = hi * 2^31 + lo with 0 <= lo < 2^31. ACC_SUM_FH sums the hi
+// parts as plain int64 words; ACC_SUM_FL sums the lo parts in bits 0..60 (at
+// most 2^30 rows per aggregate keep that sum below 2^61) and ORs flags into
+// bits 61 (an input was rounded to the 2^-s grid), 62 (+Inf seen) and 63
+// (-Inf seen; NaN sets both). The group's sum is the 128-bit integer
+// hi * 2^31 + lo, scaled by 2^-s and rounded once to double at emit; with
+// specials it is the IEEE sum of them. Inputs of magnitude >= 2^(52 - s) are
+// exact; smaller ones are rounded, so a group whose rounded sum is not large
+// against its row count (fx_hazard) sets OVF_FX_PREC, and the host re-runs
+// the query with IEEE double atomics (ACC_SUM_F). A value beyond the host's
+// bound (|v| >= 2^(63 - s)) sets OVF_FX, and the host re-runs with the bound
+// of the column's exact maximum.
 enum AccOp : int32_t {
     ACC_CNT_STAR = 0,   // += 1 per row
     ACC_CNT_VALID = 1,  // += 1 per non-NULL input
     ACC_SUM_I = 2,      // int64 two's-complement sum
-    ACC_SUM_F = 3,      // IEEE double sum
+    ACC_SUM_FH = 3,     // double sum, fixed-point high words (int64 sum)
     ACC_MIN_I = 4,      // int64 min
     ACC_MAX_I = 5,      // int64 max
     ACC_MIN_F = 6,      // min of the order-preserving uint64 image of a double
     ACC_MAX_F = 7,      // max of the same
+    ACC_SUM_FL = 8,     // double sum, fixed-point low words + flags
+    ACC_SUM_F = 9,      // IEEE double sum with atomics (order-dependent; only after OVF_FX_PREC)
 };
+
+constexpr int FX_LO_BITS = 31;
+constexpr uint64_t FX_INEXACT = 1ULL << 61;
+constexpr uint64_t FX_POS_INF = 1ULL << 62;
+constexpr uint64_t FX_NEG_INF = 1ULL << 63;
+constexpr uint64_t FX_SPECIAL = FX_POS_INF | FX_NEG_INF;  // both: NaN
+constexpr uint64_t FX_FLAGS = FX_INEXACT | FX_SPECIAL;
+constexpr uint64_t FX_LOW = ~FX_FLAGS;
+constexpr uint32_t OVF_FX = 32u;       // DevPlan::overflow bit: a double SUM input beyond the fixed-point bound
+constexpr uint32_t OVF_FX_PREC = 64u;  // DevPlan::overflow bit: a group's fixed-point sum may be imprecise
+constexpr int FX_SHIFT_MAX = 1084;     // largest scale: every subnormal stays within the bound
 
 enum ColKind : int32_t { CK_I32 = 0, CK_I64 = 1, CK_F64 = 2, CK_U32 = 3 };
 enum PredKind : int32_t { PK_INT = 0, PK_F64 = 1, PK_IN = 2, PK_DICT = 3 };
@@ -64,6 +93,8 @@ struct DevQuery {
     int32_t wide_comp;       // KM_WIDE: the component kept in hidden accumulators (never VARCHAR)
     int32_t pad;
     uint64_t wide_seed;      // KM_WIDE: seed of wide_mix
+    int32_t acc_fx[MAX_ACC]; // ACC_SUM_FH / ACC_SUM_FL: the scale s of X = round(v * 2^s)
+    uint32_t *fx_flags;      // DevPlan::overflow (OVF_FX)
 };
 
 // Per-batch device pointers.
@@ -96,6 +127,8 @@ struct DevPlan {
     uint64_t topn_lb;          // TopN: no group with a smaller order value can be in the top K
     int64_t key_min;           // direct-mapped strategy: exact key word range
     int64_t key_max;
+    uint32_t fx_exp[MAXU];     // per loaded slot: largest biased exponent of its finite double
+                               // values (sampled rows, or all rows for the exact bound)
 };
 
 // The global table: (cap + 2) records of (1 + nacc) words: key, accumulators.
@@ -152,6 +185,112 @@ __host__ __device__ __forceinline__ uint64_t wide_mix(uint64_t c1, uint64_t seed
     return z ^ (z >> 31);
 }
 
+// X = round-half-even(v * 2^s) for a finite double v given by its bits; ok is
+// false when |v| >= 2^(63 - s), where X would not fit an int64; inexact tells
+// whether X differs from v * 2^s.
+__host__ __device__ __forceinline__ int64_t fx_scale(uint64_t bits, int s, bool &ok, bool &inexact) {
+    const int e = (int)((bits >> 52) & 0x7FF);
+    uint64_t m = bits & 0xFFFFFFFFFFFFFULL;
+    int k;  // |v| * 2^s = m * 2^k
+    if (e) {
+        m |= 1ULL << 52;
+        k = e - 1075 + s;
+    } else {
+        k = -1074 + s;
+    }
+    ok = (e ? e : 1) - 1022 + s <= 63;  // |v| < 2^(e - 1022)
+    inexact = false;
+    if (!ok) return 0;
+    uint64_t x;
+    if (k >= 0) {
+        x = m << k;  // k <= 10 here
+    } else if (k <= -55) {
+        x = 0;  // below half of the unit
+        inexact = m != 0;
+    } else {
+        const int r = -k;
+        x = m >> r;
+        const uint64_t rem = m & ((1ULL << r) - 1), half = 1ULL << (r - 1);
+        inexact = rem != 0;
+        if (rem > half || (rem == half && (x & 1))) x++;
+    }
+    return (bits >> 63) ? -(int64_t)x : (int64_t)x;
+}
+
+// The word a double input adds to ACC_SUM_FH (hi) or ACC_SUM_FL (lo).
+__device__ __forceinline__ uint64_t fx_part(uint64_t bits, int s, bool hi, uint32_t *flags) {
+    if (((bits >> 52) & 0x7FF) == 0x7FF) {  // Inf or NaN: flags in the low word only
+        if (hi) return 0;
+        if (bits & 0xFFFFFFFFFFFFFULL) return FX_SPECIAL;
+        return (bits >> 63) ? FX_NEG_INF : FX_POS_INF;
+    }
+    bool ok, inexact;
+    const int64_t x = fx_scale(bits, s, ok, inexact);
+    if (!ok) {
+        if (hi) __hip_atomic_fetch_or(flags, OVF_FX, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return 0;
+    }
+    if (hi) return (uint64_t)(x >> FX_LO_BITS);
+    return ((uint64_t)x & ((1ULL << FX_LO_BITS) - 1)) | (inexact ? FX_INEXACT : 0);
+}
+
+// Two ACC_SUM_FL words combined (partial states of one group).
+__host__ __device__ __forceinline__ uint64_t fx_lo_combine(uint64_t x, uint64_t y) {
+    return ((x & FX_LOW) + (y & FX_LOW)) | ((x | y) & FX_FLAGS);
+}
+
+// |hi * 2^31 + lo| as a 128-bit magnitude (th:tl); returns the sign.
+__host__ __device__ __forceinline__ bool fx_magnitude(uint64_t hi, uint64_t lo, uint64_t &th, uint64_t &tl) {
+    lo &= FX_LOW;
+    tl = (hi << FX_LO_BITS) + lo;
+    th = (uint64_t)((int64_t)hi >> (64 - FX_LO_BITS)) + (tl < lo ? 1 : 0);
+    const bool neg = (int64_t)th < 0;
+    if (neg) {
+        tl = ~tl + 1;
+        th = ~th + (tl == 0 ? 1 : 0);
+    }
+    return neg;
+}
+
+// The group's double sum: (hi * 2^31 + lo) * 2^-s rounded once to double, or
+// the IEEE sum of the specials it saw.
+__host__ __device__ __forceinline__ double fx_value(uint64_t hi, uint64_t lo, int s) {
+    const uint64_t f = lo & FX_SPECIAL;
+    if (f) {
+        if (f == FX_SPECIAL) return __builtin_bit_cast(double, 0x7FF8000000000000ULL);
+        return __builtin_bit_cast(double, f == FX_POS_INF ? 0x7FF0000000000000ULL : 0xFFF0000000000000ULL);
+    }
+    uint64_t th, tl;
+    const bool neg = fx_magnitude(hi, lo, th, tl);
+    double d;
+    if (th == 0) {
+        d = (double)tl;  // one rounding
+    } else {
+        // the top 64 bits, with every lower nonzero bit folded into bit 0 so
+        // that the conversion rounds as the full value would
+        const int lz = __builtin_clzll(th);
+        const int sh = 64 - lz;
+        const uint64_t top = lz ? (th << lz) | (tl >> sh) : th;
+        const uint64_t sticky = (lz ? (tl << lz) : tl) != 0;
+        d = ldexp((double)(top | sticky), sh);
+    }
+    d = ldexp(d, -s);
+    return neg ? -d : d;
+}
+
+// A group whose inputs were rounded to the 2^-s grid (at most half a unit
+// each, cnt of them) may be off by more than 2^-42 of its sum when the sum is
+// below cnt * 2^41 units.
+__host__ __device__ __forceinline__ bool fx_hazard(uint64_t hi, uint64_t lo, uint64_t cnt) {
+    if (!(lo & FX_INEXACT) || (lo & FX_SPECIAL)) return false;
+    uint64_t th, tl;
+    fx_magnitude(hi, lo, th, tl);
+    if (th >> 23) return false;  // |T| >= 2^87 > 2^30 * 2^41
+    // compare T with cnt * 2^41 in 128 bits
+    const uint64_t bh = cnt >> 23, bl = cnt << 41;
+    return th < bh || (th == bh && tl < bl);
+}
+
 __host__ __device__ __forceinline__ uint64_t acc_identity(int op) {
     switch (op) {
     case ACC_MIN_I: return 0x7FFFFFFFFFFFFFFFULL;
@@ -159,6 +298,23 @@ __host__ __device__ __forceinline__ uint64_t acc_identity(int op) {
     case ACC_MIN_F: return ~0ULL;
     default: return 0ULL;
     }
+}
+
+// Loaded slots feeding a fixed-point double SUM.
+__host__ __device__ __forceinline__ uint32_t fx_slots(const DevQuery &q) {
+    uint32_t m = 0;
+    for (int a = 0; a < q.nacc; a++)
+        if (q.acc_op[a] == ACC_SUM_FH) m |= 1u << q.acc_u[a];
+    return m;
+}
+
+// Biased exponent of a finite nonzero double (subnormals count as 1), 0 for
+// zero and the specials: |v| < 2^(e - 1022).
+__host__ __device__ __forceinline__ uint32_t fx_exp_of(uint64_t bits) {
+    const uint32_t e = (uint32_t)(bits >> 52) & 0x7FF;
+    if (e == 0x7FF) return 0;
+    if (e == 0) return (bits << 1) ? 1u : 0u;
+    return e;
 }
 
 // Order-preserving image of a double: x < y  <=>  ord(x) < ord(y) for
@@ -169,6 +325,24 @@ __host__ __device__ __forceinline__ uint64_t f64_ord(uint64_t bits) {
 }
 __host__ __device__ __forceinline__ uint64_t f64_unord(uint64_t o) {
     return (o >> 63) ? (o & 0x7FFFFFFFFFFFFFFFULL) : ~o;
+}
+__host__ __device__ __forceinline__ bool f64_isnan(uint64_t bits) {
+    return ((bits >> 52) & 0x7FF) == 0x7FF && (bits & 0xFFFFFFFFFFFFFULL);
+}
+// MIN/MAX over DOUBLE skip NaN unless every input of the group is NaN (then
+// the result is NaN), whatever the NaN's sign or payload: a NaN's image sits
+// next to the accumulator's identity, beyond every number (MIN: just below
+// ~0, MAX: 1), so any number replaces it.
+constexpr uint64_t ORD_NAN_MIN = ~0ULL - 1;
+constexpr uint64_t ORD_NAN_MAX = 1;
+__host__ __device__ __forceinline__ uint64_t f64_ord_mm(uint64_t bits, bool is_min) {
+    if (f64_isnan(bits)) return is_min ? ORD_NAN_MIN : ORD_NAN_MAX;
+    return f64_ord(bits);
+}
+// Inverse of f64_ord_mm (NaN comes back as the canonical quiet NaN).
+__host__ __device__ __forceinline__ uint64_t f64_unord_mm(uint64_t o) {
+    if (o == ORD_NAN_MIN || o == ORD_NAN_MAX) return 0x7FF8000000000000ULL;
+    return f64_unord(o);
 }
 
 struct Row {
@@ -305,7 +479,12 @@ __device__ __forceinline__ void acc_apply(uint64_t *p, int op, uint64_t x) {
     case ACC_CNT_STAR:
     case ACC_CNT_VALID:
     case ACC_SUM_I:
+    case ACC_SUM_FH:
         __hip_atomic_fetch_add(p, x, __ATOMIC_RELAXED, SCOPE);
+        break;
+    case ACC_SUM_FL:
+        if (x & FX_LOW) __hip_atomic_fetch_add(p, x & FX_LOW, __ATOMIC_RELAXED, SCOPE);
+        if (x & FX_FLAGS) __hip_atomic_fetch_or(p, x & FX_FLAGS, __ATOMIC_RELAXED, SCOPE);
         break;
     case ACC_SUM_F:
         if (SCOPE == __HIP_MEMORY_SCOPE_WORKGROUP)
@@ -337,15 +516,27 @@ __device__ __forceinline__ bool acc_input(const DevQuery &q, int a, const Row &r
     uint64_t v = sel(row, u);
     switch (op) {
     case ACC_CNT_VALID: x = 1; break;
-    case ACC_MIN_F:
-    case ACC_MAX_F: x = f64_ord(v); break;
-    case ACC_SUM_F:
-        // double inputs carry their bits; integer inputs are not planned here
-        x = v;
-        break;
+    case ACC_MIN_F: x = f64_ord_mm(v, true); break;
+    case ACC_MAX_F: x = f64_ord_mm(v, false); break;
+    case ACC_SUM_FH: x = fx_part(v, q.acc_fx[a], true, q.fx_flags); break;
+    case ACC_SUM_FL: x = fx_part(v, q.acc_fx[a], false, q.fx_flags); break;
     default: x = v; break;
     }
     return true;
+}
+
+// Two partial values of accumulator kind op combined (no atomics).
+__host__ __device__ __forceinline__ uint64_t acc_combine(int op, uint64_t x, uint64_t y) {
+    switch (op) {
+    case ACC_MIN_I: return (int64_t)y < (int64_t)x ? y : x;
+    case ACC_MAX_I: return (int64_t)y > (int64_t)x ? y : x;
+    case ACC_MIN_F: return y < x ? y : x;
+    case ACC_MAX_F: return y > x ? y : x;
+    case ACC_SUM_FL: return fx_lo_combine(x, y);
+    case ACC_SUM_F:
+        return __builtin_bit_cast(uint64_t, __builtin_bit_cast(double, x) + __builtin_bit_cast(double, y));
+    default: return x + y;  // counts, integer sums, fixed-point high words
+    }
 }
 
 // Whether a partial accumulator value changes the merged result.
@@ -549,8 +740,10 @@ __device__ __forceinline__ bool acc_input_s(const S &sh, int a, const Row &row, 
     const uint64_t v = sel_s<S>(row, u);
     switch (op) {
     case ACC_CNT_VALID: x = 1; break;
-    case ACC_MIN_F:
-    case ACC_MAX_F: x = f64_ord(v); break;
+    case ACC_MIN_F: x = f64_ord_mm(v, true); break;
+    case ACC_MAX_F: x = f64_ord_mm(v, false); break;
+    case ACC_SUM_FH: x = fx_part(v, sh.q.acc_fx[a], true, sh.q.fx_flags); break;
+    case ACC_SUM_FL: x = fx_part(v, sh.q.acc_fx[a], false, sh.q.fx_flags); break;
     default: x = v; break;
     }
     return true;

@@ -254,20 +254,7 @@ __device__ __forceinline__ void wave_reduce_apply(const DevQuery &q, const Row &
         uint64_t x;
         if (!acc_input(q, a, row, x)) x = acc_identity(op);
 #pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
-            uint64_t y = __shfl_xor(x, off);
-            switch (op) {
-            case ACC_SUM_F:
-                x = (uint64_t)__double_as_longlong(__longlong_as_double((long long)x) +
-                                                   __longlong_as_double((long long)y));
-                break;
-            case ACC_MIN_I: x = (int64_t)x < (int64_t)y ? x : y; break;
-            case ACC_MAX_I: x = (int64_t)x > (int64_t)y ? x : y; break;
-            case ACC_MIN_F: x = x < y ? x : y; break;
-            case ACC_MAX_F: x = x > y ? x : y; break;
-            default: x = x + y; break;
-            }
-        }
+        for (int off = 32; off > 0; off >>= 1) x = acc_combine(op, x, __shfl_xor(x, off));
         if (lane == 0 && acc_nontrivial(op, x)) {
             if (lds) acc_apply<__HIP_MEMORY_SCOPE_WORKGROUP>(dst + (uint64_t)a * stride, op, x);
             else acc_apply<__HIP_MEMORY_SCOPE_AGENT>(dst + (uint64_t)a * stride, op, x);
@@ -377,6 +364,35 @@ __global__ __launch_bounds__(AGG_BLOCK) void k_key_range(const DevQuery *__restr
 __global__ void k_key_range_init(DevPlan *plan) {
     plan->key_min = INT64_MAX;
     plan->key_max = INT64_MIN;
+}
+
+// Exact largest exponents of every double SUM input column over all rows of
+// a batch (valid rows; the filter is not applied, a superset is a bound too):
+// one atomic max per wave and column.
+__global__ __launch_bounds__(AGG_BLOCK) void k_fx_range(const DevQuery *__restrict__ qp,
+                                                        const DevBatch *__restrict__ bp, DevPlan *plan) {
+    const DevQuery &q = *qp;
+    const DevBatch b = *bp;
+    const uint32_t fxm = fx_slots(q);
+    uint32_t emax[MAXU] = {};
+    for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < b.nrows;
+         r += (uint64_t)gridDim.x * blockDim.x) {
+#pragma unroll
+        for (int u = 0; u < MAXU; u++) {
+            if (!((fxm >> u) & 1)) continue;
+            if (b.valid[u] && !gld_stream(b.valid[u] + r)) continue;
+            emax[u] = max(emax[u], fx_exp_of(gld_stream((const uint64_t *)b.data[u] + r)));
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < MAXU; u++) {
+        if (!((fxm >> u) & 1)) continue;
+        uint32_t e = emax[u];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) e = max(e, (uint32_t)__shfl_xor(e, off));
+        if ((threadIdx.x & 63) == 0 && e)
+            __hip_atomic_fetch_max(&plan->fx_exp[u], e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
 }
 
 // LDS: one workgroup-private dense table [nacc][span], folded into the global
@@ -537,6 +553,8 @@ struct DevOut {
     int32_t pad;
     uint64_t wide_seed;
     uint32_t *flags;           // DevPlan::overflow
+    int32_t lo_acc[MAX_OUT];   // OUT_SUM_F / OUT_AVG_F: the ACC_SUM_FL accumulator (val_acc: ACC_SUM_FH)
+    int32_t fx_shift[MAX_OUT]; // and their scale s
 };
 
 constexpr uint32_t OVF_WIDE = 8u;  // DevPlan::overflow bit: two KM_WIDE keys share a key word
@@ -557,6 +575,17 @@ __device__ __forceinline__ uint64_t key_component(const DevOut &o, uint64_t key,
         return comp == o.wide_comp ? c : key ^ wide_mix(c, o.wide_seed);
     }
     return key;
+}
+
+// Double SUM of output column c from its accumulators (fixed-point words, or
+// an IEEE double after the host switched a query to atomics); flags groups whose
+// fixed-point sum may have lost precision.
+__device__ __forceinline__ double out_fsum(const DevOut &o, int c, uint64_t v, const uint64_t *acc, uint32_t stride,
+                                           uint64_t cnt) {
+    if (o.lo_acc[c] < 0) return __longlong_as_double((long long)v);
+    const uint64_t lo = acc[(uint64_t)o.lo_acc[c] * stride];
+    if (fx_hazard(v, lo, cnt)) __hip_atomic_fetch_or(o.flags, OVF_FX_PREC, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return fx_value(v, lo, o.fx_shift[c]);
 }
 
 // One result row from a group state: acc a at acc[a * stride].
@@ -588,16 +617,16 @@ __device__ __forceinline__ void write_group(const DevOut &o, uint64_t key, const
             gst((int32_t *)o.data[c] + row, ok ? (int32_t)(int64_t)v : (int32_t)0);
             break;
         case OUT_SUM_F:
-            gst((double *)o.data[c] + row, ok ? __longlong_as_double((long long)v) : 0.0);
+            gst((double *)o.data[c] + row, ok ? out_fsum(o, c, v, acc, stride, cnt) : 0.0);
             break;
         case OUT_AVG_I:
             gst((double *)o.data[c] + row, ok ? (double)(int64_t)v / (double)cnt : 0.0);
             break;
         case OUT_AVG_F:
-            gst((double *)o.data[c] + row, ok ? __longlong_as_double((long long)v) / (double)cnt : 0.0);
+            gst((double *)o.data[c] + row, ok ? out_fsum(o, c, v, acc, stride, cnt) / (double)cnt : 0.0);
             break;
         default:  // OUT_MM_F
-            gst((double *)o.data[c] + row, ok ? __longlong_as_double((long long)f64_unord(v)) : 0.0);
+            gst((double *)o.data[c] + row, ok ? __longlong_as_double((long long)f64_unord_mm(v)) : 0.0);
             break;
         }
         gst(o.valid[c] + row, ok);
@@ -643,7 +672,7 @@ enum TopnKind : int32_t {
     TK_KEY = 0,    // the group key word (single-column keys)
     TK_ACC_I = 1,  // int64 accumulator (COUNT, integer SUM/MIN/MAX)
     TK_ACC_O = 2,  // ordered-double accumulator (MIN/MAX of DOUBLE)
-    TK_SUM_F = 3,  // double SUM
+    TK_SUM_F = 3,  // double SUM (fixed-point words acc, lo_acc)
     TK_AVG_I = 4,
     TK_AVG_F = 5,
 };
@@ -663,6 +692,8 @@ struct DevTopn {
     uint32_t cand_cap;
     int32_t wide_acc[2];  // KM_WIDE: the MIN and MAX accumulators of the kept component
     uint32_t *flags;      // DevPlan::overflow
+    int32_t lo_acc;       // TK_SUM_F / TK_AVG_F: the ACC_SUM_FL accumulator
+    int32_t fx_shift;     // and the scale s
 };
 
 __device__ __forceinline__ uint64_t key_order(const DevTopn &t, uint64_t key) {
@@ -672,6 +703,15 @@ __device__ __forceinline__ uint64_t key_order(const DevTopn &t, uint64_t key) {
         return key;
     }
     return t.key_signed[0] ? key ^ 0x8000000000000000ULL : key;
+}
+
+// The TopN order column's double sum (see out_fsum).
+__device__ __forceinline__ double topn_fsum(const DevTopn &t, uint64_t v, const uint64_t *acc, uint32_t stride) {
+    if (t.lo_acc < 0) return __longlong_as_double((long long)v);
+    const uint64_t lo = acc[(uint64_t)t.lo_acc * stride];
+    if (fx_hazard(v, lo, acc[(uint64_t)t.cnt_acc * stride]))
+        __hip_atomic_fetch_or(t.flags, OVF_FX_PREC, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return fx_value(v, lo, t.fx_shift);
 }
 
 __device__ __forceinline__ void group_order(const DevTopn &t, uint64_t key, const uint64_t *acc,
@@ -685,14 +725,16 @@ __device__ __forceinline__ void group_order(const DevTopn &t, uint64_t key, cons
     case TK_KEY: o = null_key ? 0 : key_order(t, key); break;
     case TK_ACC_I: o = v ^ 0x8000000000000000ULL; break;
     case TK_ACC_O: o = v; break;
-    case TK_SUM_F: o = f64_ord(v); break;
+    case TK_SUM_F:
+        o = f64_ord((uint64_t)__double_as_longlong(topn_fsum(t, v, acc, stride)));
+        break;
     case TK_AVG_I: {
         double d = (double)(int64_t)v / (double)acc[(uint64_t)t.cnt_acc * stride];
         o = f64_ord((uint64_t)__double_as_longlong(d));
         break;
     }
     default: {
-        double d = __longlong_as_double((long long)v) / (double)acc[(uint64_t)t.cnt_acc * stride];
+        double d = topn_fsum(t, v, acc, stride) / (double)acc[(uint64_t)t.cnt_acc * stride];
         o = f64_ord((uint64_t)__double_as_longlong(d));
         break;
     }

@@ -120,18 +120,26 @@ __global__ __launch_bounds__(256) void k_sample(const DevCtx *__restrict__ cx, c
     const DevBatch b = *bp;  // column pointers stay in registers
     __shared__ uint64_t lk[SAMPLE_LDS];
     __shared__ uint32_t lc[SAMPLE_LDS];
+    __shared__ uint32_t s_fx[MAXU];
     for (uint32_t i = threadIdx.x; i < SAMPLE_LDS; i += blockDim.x) { lk[i] = SENTINEL; lc[i] = 0; }
+    if (threadIdx.x < MAXU) s_fx[threadIdx.x] = 0;
+    const uint32_t fxm = fx_slots(q);
     __syncthreads();
     uint32_t kept = 0;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < sample_n; i += gridDim.x * blockDim.x) {
         uint64_t r = (uint64_t)i * sample_stride;
         if (r >= b.nrows) continue;
         Row row;
-        load_row(q, b, r, (1u << q.nkp) - 1, row);
+        load_row(q, b, r, ((1u << q.nkp) - 1) | fxm, row);
         if (q.npred && !eval_preds(q, b, row)) continue;
         uint64_t w;
         if (make_key(q, b, row, w) != KEY_OK || w == SENTINEL) continue;
         kept++;
+        for (int u = 0; u < MAXU; u++)  // largest exponents of the double SUM inputs
+            if (((fxm & row.vm) >> u) & 1) {
+                const uint32_t e = fx_exp_of(sel(row, u));
+                if (e) __hip_atomic_fetch_max(&s_fx[u], e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            }
         uint32_t pos = (uint32_t)hash64(w) & (SAMPLE_LDS - 1);
         bool done = false;
         for (int probe = 0; probe < 16 && !done; probe++) {
@@ -161,6 +169,8 @@ __global__ __launch_bounds__(256) void k_sample(const DevCtx *__restrict__ cx, c
         if (lc[i]) sample_global_add(rp, lk[i], lc[i]);
     __syncthreads();
     if (threadIdx.x == 0 && s_kept) atomicAdd((unsigned long long *)&plan->sample_rows, (unsigned long long)s_kept);
+    if (threadIdx.x < MAXU && s_fx[threadIdx.x])
+        __hip_atomic_fetch_max(&plan->fx_exp[threadIdx.x], s_fx[threadIdx.x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Heavy-hitter selection over the sample table, in three small kernels:
@@ -748,20 +758,11 @@ __global__ __launch_bounds__(RA_BLOCK) void k_rp_agg_topn(const DevCtx *__restri
 }
 
 // Folds the heavy partial states over workgroups: block = 64 heavy keys x 4
-// workgroup phases, one accumulator per blockIdx.y. The fold order is fixed,
-// so double sums are reproducible.
-__device__ __forceinline__ uint64_t fold_acc(int op, uint64_t x, uint64_t y) {
-    switch (op) {
-    case ACC_SUM_F:
-        return (uint64_t)__double_as_longlong(__longlong_as_double((long long)x) +
-                                              __longlong_as_double((long long)y));
-    case ACC_MIN_I: return (int64_t)y < (int64_t)x ? y : x;
-    case ACC_MAX_I: return (int64_t)y > (int64_t)x ? y : x;
-    case ACC_MIN_F: return y < x ? y : x;
-    case ACC_MAX_F: return y > x ? y : x;
-    default: return x + y;
-    }
-}
+// workgroup phases, one accumulator per blockIdx.y. Every accumulator combines
+// with an associative, commutative integer operation (double sums are
+// fixed-point words), so the folded states do not depend on the order in
+// which rows reached the per-workgroup partials.
+__device__ __forceinline__ uint64_t fold_acc(int op, uint64_t x, uint64_t y) { return acc_combine(op, x, y); }
 
 __global__ __launch_bounds__(256) void k_heavy_fold(const DevCtx *__restrict__ cx, DevTable T) {
     const DevQuery &q = cx->q;

@@ -81,7 +81,8 @@ class PlanInfo(C.Structure):
                 ("ms_count", C.c_double), ("ms_partition", C.c_double), ("ms_emit", C.c_double),
                 ("kernel_shape", C.c_char * 32), ("selector_distinct", C.c_double),
                 ("selector_used", C.c_int32), ("selector_path", C.c_int32),
-                ("ms_strings", C.c_double), ("strings_distinct", C.c_uint64)]
+                ("ms_strings", C.c_double), ("strings_distinct", C.c_uint64),
+                ("sum_reruns", C.c_uint32), ("sum_ieee", C.c_int32)]
 
 
 # Every symbol include/slothdb_gpu.h declares, with its ctypes signature.

@@ -41,6 +41,8 @@ def assert_same_groups(got_rows, exp_rows, nkeys, types, rel=REL_TOL_F64):
                 assert x is None and y is None, f"NULL mismatch in column {c}: {a} vs {b}"
                 continue
             if types[c] == 12:  # DOUBLE
+                if x == y or (math.isnan(x) and math.isnan(y)):
+                    continue
                 err = abs(x - y) / max(abs(y), 1e-300) if y != 0 else abs(x)
                 worst = max(worst, err)
                 assert err <= rel or (math.isnan(x) and math.isnan(y)), \

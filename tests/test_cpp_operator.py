@@ -26,3 +26,15 @@ def test_cpp_operator_planning_cpu():
 @pytest.mark.gpu
 def test_cpp_operator_gpu():
     _run()
+
+
+def test_fixed_point_double_sums_cpu():
+    # the exact double SUM arithmetic of the device header, on the host:
+    # round-half-even scaling, order-independent words, one rounding at the
+    # end, IEEE specials, precision-hazard detection, NaN images for MIN/MAX
+    exe = os.path.join(ROOT, "slothdb_amd", "lib", "test_fixed_point")
+    if not os.path.exists(exe):
+        pytest.fail(f"{exe} is missing: build with `python -m slothdb_amd.build`")
+    r = subprocess.run([exe], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=110)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout

@@ -290,6 +290,90 @@ def test_neumaier_sum():
     assert rows == [(0, 2000.0, 0.5)]  # naive left-to-right summation gives 0
 
 
+def _special_values_table(seed, n=3000, groups=40):
+    """Doubles with NaN (both signs), +-Inf, zeros of both signs, subnormals
+    and wide magnitudes, grouped so that some groups hold only NaN, only
+    infinities, both infinities, or no special at all."""
+    rng = np.random.default_rng(seed)
+    k = rng.integers(0, groups, n).astype(np.int32)
+    x = rng.normal(0, 1, n) * 10.0 ** rng.integers(-30, 30, n)
+    kind = rng.integers(0, 20, n)
+    neg_nan = np.frombuffer(np.uint64(0xFFF8000000000001).tobytes(), np.float64)[0]
+    x[kind == 0] = np.nan
+    x[kind == 1] = neg_nan
+    x[kind == 2] = np.inf
+    x[kind == 3] = -np.inf
+    x[kind == 4] = -0.0
+    x[kind == 5] = 5e-324
+    x[k == 0] = np.nan                       # only NaN
+    x[k == 1] = np.where(np.arange((k == 1).sum()) % 2, neg_nan, np.nan)
+    x[k == 2] = np.inf                       # only +Inf
+    x[(k == 3) & (kind < 10)] = np.inf       # +Inf and finite
+    x[(k == 4) & (kind < 10)] = -np.inf      # -Inf and finite
+    xs = x[k == 5]
+    xs[:2] = [np.inf, -np.inf]               # both infinities
+    x[k == 5] = xs
+    x[(k >= 6) & (k < 12)] = rng.normal(0, 1, ((k >= 6) & (k < 12)).sum())  # no specials
+    return k, x
+
+
+def _brute_sum(vals):
+    """IEEE result of the exact sum: NaN for a NaN or +Inf with -Inf, the
+    infinity otherwise, else the correctly rounded exact sum (math.fsum)."""
+    import math
+    if any(math.isnan(v) for v in vals) or (math.inf in vals and -math.inf in vals):
+        return math.nan
+    if math.inf in vals:
+        return math.inf
+    if -math.inf in vals:
+        return -math.inf
+    return math.fsum(vals)
+
+
+def _brute_minmax(vals, mn):
+    """MIN/MAX skipping NaN, NaN when every value is NaN."""
+    import math
+    num = [v for v in vals if not math.isnan(v)]
+    if not num:
+        return math.nan
+    return min(num) if mn else max(num)
+
+
+def test_double_specials_and_nan_min_max_vs_brute_force():
+    import math
+    k, x = _special_values_table(5)
+    q = G.Query([G.INTEGER, G.DOUBLE], [0], [(G.SUM, 1), (G.AVG, 1), (G.MIN, 1), (G.MAX, 1), (G.COUNT, 1)],
+                nullable=[0, 0])
+    rows, _, _ = exact_rows(q, [G.Column(G.INTEGER, k), G.Column(G.DOUBLE, x)])
+    by = defaultdict(list)
+    for kk, xx in zip(k.tolist(), x.tolist()):
+        by[kk].append(xx)
+
+    def same(a, b):
+        return (math.isnan(a) and math.isnan(b)) or a == b or abs(a - b) <= 1e-12 * abs(b)
+
+    assert len(rows) == len(by)
+    for key, sm, avg, mn, mx, cnt in rows:
+        vals = by[key]
+        assert cnt == len(vals)
+        exp = _brute_sum(vals)
+        assert same(sm, exp), (key, sm, exp)
+        assert same(avg, exp / len(vals)), (key, avg)
+        assert same(mn, _brute_minmax(vals, True)) and same(mx, _brute_minmax(vals, False)), (key, mn, mx)
+    got = {r[0]: r for r in rows}
+    assert math.isnan(got[0][1]) and math.isnan(got[0][3]) and math.isnan(got[1][4])
+    assert got[2][1] == math.inf and math.isnan(got[5][1])
+
+
+def test_planted_nan_policy_mistake_is_caught():
+    """A MIN that lets a NaN win (the reference's first-value behaviour) is
+    not what the brute-force rule accepts."""
+    import math
+    vals = [math.nan, 3.0, -1.0]
+    assert _brute_minmax(vals, True) == -1.0
+    assert math.isnan(vals[0]) and _brute_minmax(vals, False) == 3.0
+
+
 def test_int64_sum_beyond_2_53_is_exact():
     v = np.array([2**53 + 1, 2**53 + 1, -3], np.int64)
     q = G.Query([G.INTEGER, G.BIGINT], [0], [(G.SUM, 1)], nullable=[0, 0])
