@@ -331,6 +331,23 @@ gbagg_status gbagg_hash_aggregate_i32(const int32_t *keys, const int32_t *values
                                       int32_t *out_keys, int64_t *out_sums, int64_t *out_counts,
                                       uint64_t capacity, uint64_t *out_groups);
 
+/* gbagg_hash_aggregate_i32 without any device allocation. It has no
+ * counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc). Every pointer is device memory; the calls are
+ * enqueued on `stream` (a hipStream_t; NULL = the null stream) and the call
+ * returns without waiting. On completion out_keys/out_sums/out_counts hold
+ * min(groups, capacity) groups in unspecified order and *out_groups the group
+ * count (larger than capacity means rows were left out).
+ * Limit: count <= GBAGG_NOALLOC_MAX_ROWS (524288), else GBAGG_INVALID. The
+ * hash table is a fixed 24 MiB scratch array that each GPU the library runs
+ * on holds until the process ends; every call on one GPU shares it and
+ * clears it first, so calls that overlap on one GPU race. */
+#define GBAGG_NOALLOC_MAX_ROWS 524288u
+gbagg_status gbagg_hash_aggregate_i32_noalloc(const int32_t *keys, const int32_t *values, uint64_t count,
+                                              int32_t *out_keys, int64_t *out_sums, int64_t *out_counts,
+                                              uint64_t capacity, uint64_t *out_groups, void *stream);
+
 /* ---- Batches: many GROUP BY queries in one submission --------------------
  * For callers with many small queries, which would otherwise run the
  * aggregate operator (physical_planner.cpp:6009-12627) once per query.

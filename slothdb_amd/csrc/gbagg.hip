@@ -109,6 +109,7 @@ const ShapeKernels &dynamic_kernels() {
 }
 
 thread_local std::string g_last_error;
+thread_local const char *g_na_error = nullptr;  // gbagg_hash_aggregate_i32_noalloc's errors (static text)
 
 struct ApiError {
     gbagg_status status;
@@ -131,9 +132,11 @@ gbagg_status guarded(Fn &&fn) {
         return GBAGG_OK;
     } catch (const ApiError &e) {
         g_last_error = e.msg;
+        g_na_error = nullptr;
         return e.status;
     } catch (const std::exception &e) {
         g_last_error = e.what();
+        g_na_error = nullptr;
         return GBAGG_ERROR;
     }
 }
@@ -2740,7 +2743,7 @@ extern "C" {
 
 int32_t gbagg_abi_version(void) { return GBAGG_ABI_VERSION; }
 
-const char *gbagg_last_error(void) { return g_last_error.c_str(); }
+const char *gbagg_last_error(void) { return g_na_error ? g_na_error : g_last_error.c_str(); }
 
 gbagg_status gbagg_device_name(char *buf, uint32_t buflen) {
     return guarded([&] {
@@ -3578,3 +3581,5 @@ void gbagg_parquet_close(gbagg_parquet *p) {
 }
 
 }  // extern "C"
+
+#include "gbagg_noalloc.hpp"

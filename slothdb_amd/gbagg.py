@@ -102,6 +102,8 @@ _SIGS = {
     "gbagg_hash_aggregate_i32": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p,
                                            C.c_void_p, C.c_void_p, C.c_uint64,
                                            C.POINTER(C.c_uint64)]),
+    "gbagg_hash_aggregate_i32_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p,
+                                                   C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p]),
     "gbagg_batch_create": (C.c_int, [C.POINTER(C.c_void_p)]),
     "gbagg_batch_set_tune": (C.c_int, [C.c_void_p, C.c_void_p]),
     "gbagg_batch_set_param": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int64]),
