@@ -55,11 +55,14 @@ def main():
     ap.add_argument("--batch-queries", type=int, default=512)
     ap.add_argument("--no-strings", action="store_true")
     ap.add_argument("--no-parquet", action="store_true")
+    ap.add_argument("--no-configs", action="store_true", help="skip the plain and composite configurations")
     ap.add_argument("--cpu-calibration", action="store_true",
                     help="time the CPU restatement against the reference's own RadixCountAgg "
                          "(oracle/_ref, built from the reference sources), then exit")
     ap.add_argument("--batch-split", action="store_true",
                     help="time the batched mix split by the queries' distinct keys, then exit")
+    ap.add_argument("--param", action="append", default=[], metavar="NAME=VALUE",
+                    help="operator tuning knob (gbagg_set_param), e.g. part_bits=8; repeatable")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
@@ -81,7 +84,8 @@ def main():
     q.strategy = args.strategy
     print(f"[bench] generated {args.rows} rows in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
     dcols = [G.DeviceColumn(c) for c in cols]
-    op = G.GpuHashAggregate(q)
+    knobs = {k: int(v) for k, v in (p.split("=", 1) for p in args.param)}
+    op = G.GpuHashAggregate(q, knobs or None)
     op.Init()
 
     def step():
@@ -129,11 +133,14 @@ def main():
         t1 = time.perf_counter()
         O.port_groupby(q, sample, args.cpu_threads)
         cpu_s = time.perf_counter() - t1
-        cpu = {"value": m / cpu_s, "unit": "rows/s", "cores": args.cpu_threads, "kind": "port",
+        cpu = {"value": m / cpu_s, "unit": "rows/s", "cores": args.cpu_threads,
+               "host_cores": len(os.sched_getaffinity(0)),
+               "kind": "port",
                "sample": f"first {m} rows of the same workload, restated per-thread-map path "
                          f"(physical_planner.cpp:7613-7931), {cpu_s:.2f} s wall"}
         print(f"[bench] cpu port {m} rows in {cpu_s:.2f}s", file=sys.stderr, flush=True)
 
+    configs = None if args.no_configs else bench_configs()
     batched = None if args.no_batch else bench_batched(args.batch_queries)
     strings = None if args.no_strings else bench_strings()
     parquet = None if args.no_parquet else bench_parquet()
@@ -146,7 +153,8 @@ def main():
         "config": {"workload": "highcard_groupby_100M", "rows": args.rows, "distinct_ids": args.keys,
                    "key_dist": "zipf(1.1) over sparse int64 ids", "groups": groups,
                    "aggs": "COUNT(*), SUM(v1 int64), MIN/MAX(v2 int32), AVG(v3 double)",
-                   "strategy": info["strategy"], "parallelism": "1 GPU"},
+                   "strategy": info["strategy"], "selector_used": info["selector_used"],
+                   "selector_path": info["selector_path"], "parallelism": "1 GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "k_rp_scatter" if info["strategy"] == 4 else "k_aggregate",
@@ -154,6 +162,7 @@ def main():
         "pipeline": {"algorithmic_bytes": pipe_bytes, "device_ms": tot_avg, "GB/s": pipe_gbs,
                      "frac": pipe_gbs / HBM_PEAK_GBS},
         "cpu_baseline": cpu,
+        "configs": configs,
         "batched": batched,
         "strings": strings,
         "parquet": parquet,
@@ -296,6 +305,48 @@ def bench_strings(runs=5):
             "query_ms_strings": t_str, "query_ms_codes": t_codes, "top_counts_equal": same}
 
 
+def bench_configs(steps=10):
+    """The other two aggregate configurations, HBM-resident, AUTO plan, the
+    same step as the headline (Init + consume_device + finalize, results left
+    in HBM): the plain aggregate (1M rows, 1000 int32 keys; launch-bound, its
+    HBM fraction is reported but not meaningful) and the composite key with
+    filter and TopN (50M rows, int32 a + dictionary-coded VARCHAR b, WHERE
+    c < 30, ORDER BY COUNT(*) DESC LIMIT 10). Algorithmic bytes: the input
+    columns read once plus the result written once."""
+    from slothdb_amd import datagen, gbagg as G
+    sync = G.lib().gbagg_device_synchronize
+    out = {}
+    for name, (q, cols), in_bytes, out_bytes in (
+            ("plain_1M", datagen.plain_workload(n=1_000_000), 1_000_000 * 12, 1000 * 36),
+            ("composite_filter_topn_50M", datagen.composite_workload(), 50_000_000 * 12, 10 * 20)):
+        dcols = [G.DeviceColumn(c) for c in cols]
+        op = G.GpuHashAggregate(q)
+        for _ in range(3):
+            op.Init(); op.consume_device(dcols); op.finalize()
+        sync()
+        dev = []
+        t = time.perf_counter()
+        for _ in range(steps):
+            op.Init(); op.consume_device(dcols); op.finalize()
+            dev.append(op.plan_info()["ms_total"])
+        sync()
+        ms = (time.perf_counter() - t) * 1e3 / steps
+        info = op.plan_info()
+        dms = sum(dev) / len(dev)
+        n = len(cols[0].values)
+        gbs = (in_bytes + out_bytes) / (dms * 1e-3) / 1e9
+        out[name] = {"rows": n, "ms_per_step": ms, "rows_per_s": n / (ms * 1e-3), "device_ms": dms,
+                     "algorithmic_bytes": in_bytes + out_bytes, "GB/s": gbs, "hbm_frac": gbs / HBM_PEAK_GBS,
+                     "strategy": info["strategy"], "selector_used": info["selector_used"],
+                     "groups": info["groups"]}
+        print(f"[bench] {name}: {ms:.3f} ms per step (device {dms:.3f} ms), {n / (ms * 1e-3):.3g} rows/s, "
+              f"strategy {info['strategy']}", file=sys.stderr, flush=True)
+        op.close()
+        for c in dcols:
+            c.free()
+    return out
+
+
 def cpu_calibration(n, nkeys, threads):
     """COUNT(*) GROUP BY k over the high-cardinality keys: the oracle's
     restatement of RadixCountAgg (radix_count_agg.cpp:17-106) against the
@@ -398,6 +449,16 @@ def bench_batched(nq):
     mism = 0
     for i in range(0, nq, max(1, nq // 16)):
         mism += sorted(b.result(i).rows()) != sorted(ops[i].result().rows())
+    # the same tuned batch with the selector off (modes from the sketch rule alone)
+    b.set_param("selector", 0)
+    b.set_tune(**info["tune"])
+    walls0 = []
+    for rep in range(5):
+        sync()
+        t = time.perf_counter()
+        b.run()
+        sync()
+        walls0.append(time.perf_counter() - t)
     for op in ops:
         op.close()
     b.close()
@@ -405,10 +466,12 @@ def bench_batched(nq):
     batch_ms = min(walls) * 1e3
     in_bytes = rows * 12
     print(f"[bench] batched: {nq} queries, {rows} rows: plain {plain_ms:.1f} ms, batched {batch_ms:.2f} ms "
-          f"(device {min(devs):.2f} ms), tune {tune_s:.1f}s, mismatches {mism}", file=sys.stderr, flush=True)
+          f"(device {min(devs):.2f} ms; selector planned {info['selector_queries']} queries; without it "
+          f"{min(walls0) * 1e3:.2f} ms), tune {tune_s:.1f}s, mismatches {mism}", file=sys.stderr, flush=True)
     return {"workload": f"{nq} mixed COUNT/SUM queries", "rows": rows, "plain_ms": plain_ms,
             "batched_ms": batch_ms, "batched_device_ms": min(devs), "speedup": plain_ms / batch_ms,
             "batched_rows_per_s": rows / (batch_ms * 1e-3), "batched_hbm_frac": in_bytes / (batch_ms * 1e-3) / 8e12,
+            "selector_queries": info["selector_queries"], "batched_ms_without_selector": min(walls0) * 1e3,
             "tune": info["tune"], "autotune_s": tune_s, "spot_check_mismatches": mism}
 
 

@@ -272,7 +272,7 @@ typedef struct {
     double ms_emit;             /* result compaction / TopN */
     char kernel_shape[32];      /* compiled query shape the row kernels ran, or "dynamic" */
     double selector_distinct;   /* learned selector's distinct-group prediction (0 = not run) */
-    int32_t selector_used;      /* 1: the plan followed the selector; 0: default plan (or no selector) */
+    int32_t selector_used;      /* 1: the plan followed the selector's path; 0: default plan (or no selector) */
     int32_t selector_path;      /* the selector's path: 0 LDS_HASH, 1 GLOBAL_HASH, 2 RADIX; -1 = not run */
     double ms_strings;          /* interning of GBAGG_STR_RAW columns, before the aggregate (0 = none) */
     uint64_t strings_distinct;  /* distinct strings over those columns */
@@ -320,7 +320,9 @@ void gbagg_destroy(gbagg_handle *h);
  *                   shape matches one (default), 0 = always the generic kernels
  *   "str_hash_bits" bits of the string hash kept when interning GBAGG_STR_RAW
  *                   columns (1-64, default 64); fewer bits only force strings to
- *                   share hashes, which the byte comparison resolves */
+ *                   share hashes, which the byte comparison resolves
+ *   "selector"      1 = AUTO plans consult the built-in trained selector
+ *                   (default), 0 = the default plan only */
 gbagg_status gbagg_set_param(gbagg_handle *h, const char *name, int64_t value);
 
 /* GPUEngine::HashAggregate (include/slothdb/execution/gpu/gpu_engine.hpp:79-85):
@@ -386,6 +388,7 @@ typedef struct {
     double ms_device;        /* last run: batched graph, launch to completion (HIP events) */
     double ms_single;        /* last run: single-query operator runs (host clock) */
     gbagg_batch_tune tune;
+    uint32_t selector_queries; /* batched queries whose mode the learned selector chose */
 } gbagg_batch_info;
 
 gbagg_status gbagg_batch_create(gbagg_batch **out);
@@ -396,7 +399,15 @@ gbagg_status gbagg_batch_set_tune(gbagg_batch *b, const gbagg_batch_tune *tune);
  * "mode_rule": which queries aggregate each row tile in an LDS table before
  * their global table: 0 = those whose distinct estimate fits it, 1 = all,
  * 2 (default) = those, and those whose keys repeat within a tile (first
- * tile's estimate at most half its rows). Results never depend on it. */
+ * tile's estimate at most half its rows). Results never depend on it.
+ * "selector": 1 (default) = each query's mode follows the learned selector's
+ * path for its input (DIRECT or LDS_HASH: LDS-first; GLOBAL_HASH or RADIX:
+ * global table; planned on the host once per input, from the same row sample
+ * and features as the single-query operator's AUTO plan) and its predicted
+ * distinct count may enlarge the table, which never gets smaller than the
+ * HyperLogLog estimate sizes it; mode_rule applies to queries whose
+ * prediction is more than 4x from their sample's own estimate. 0 = mode_rule
+ * only. Results never depend on it. */
 gbagg_status gbagg_batch_set_param(gbagg_batch *b, const char *name, int64_t value);
 /* Times each candidate launch shape on these items and keeps the fastest. */
 gbagg_status gbagg_batch_autotune(gbagg_batch *b, const gbagg_batch_item *items, uint32_t count);
@@ -411,11 +422,21 @@ void gbagg_batch_destroy(gbagg_batch *b);
  * plan's row sample predicts the distinct-group count, as the correction
  * log2((distinct + 1) / (default + 1)) / 4 to the sample's own estimate
  * default = seen + f1 / sampled * rows, and the fastest path (DIRECT,
- * LDS_HASH, GLOBAL_HASH, RADIX). With a selector set, an AUTO plan takes
- * the predicted path and sizes its hash table from the predicted count, unless
- * the prediction and the sample's own estimate differ by more than 4x; then
- * the default plan runs. DIRECT falls back to LDS_HASH when the exact key
- * range is too wide. Results never depend on it.
+ * LDS_HASH, GLOBAL_HASH, RADIX). An AUTO plan takes the predicted path; the
+ * predicted count sizes an LDS_HASH / GLOBAL_HASH table when it is within 4x
+ * of the sample's own estimate, and an LDS_HASH / GLOBAL_HASH prediction
+ * farther off than that leaves the plan to the default rules (RADIX sizes
+ * itself from its HyperLogLog count pass, DIRECT from the exact key range,
+ * so those paths are taken whatever the count). DIRECT falls back to LDS_HASH
+ * when the exact key range is too wide. Results never depend on it.
+ * Every handle starts with the built-in selector: the trained parameters
+ * shipped with the library (slothdb_amd/selector_weights.json, compiled in).
+ * It is consulted for the query shapes it was trained on (one or two
+ * INTEGER/BIGINT group columns, no predicates, no TopN hint or row limit);
+ * other AUTO plans follow the default rules. gbagg_selector_set replaces it
+ * (an attached selector is consulted for every AUTO plan);
+ * gbagg_selector_set(h, NULL, 0) or gbagg_set_param(h, "selector", 0) leaves
+ * AUTO plans to the default rules.
  * Parameter vector: GBAGG_SELECTOR_PARAMS floats = weights and biases of the
  * four layers in order (W row-major [out][in], then b), then the input
  * standardisation mean[32] and scale[32]. */
@@ -423,7 +444,7 @@ void gbagg_batch_destroy(gbagg_batch *b);
 #define GBAGG_SELECTOR_OUTPUTS 5
 #define GBAGG_SELECTOR_PARAMS 10821
 
-/* Per handle; params = NULL removes the selector. */
+/* Per handle; params = NULL removes the selector (the built-in one too). */
 gbagg_status gbagg_selector_set(gbagg_handle *h, const float *params, uint32_t num_params);
 /* Features (32 floats) and, with a selector, its raw outputs (5 floats) of
  * the last finalize; features are kept when a selector is set or after

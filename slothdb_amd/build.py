@@ -36,11 +36,32 @@ def _run(cmd, cwd=None):
     return out
 
 
+GEN = os.path.join(LIB, "gen")
+
+
+def gen_selector_builtin():
+    """The committed trained selector (slothdb_amd/selector_weights.json) as a
+    C array the library compiles in: AUTO plans consult it by default."""
+    import json
+    with open(os.path.join(HERE, "selector_weights.json")) as f:
+        params = json.load(f)["params"]
+    os.makedirs(GEN, exist_ok=True)
+    body = ",\n".join(", ".join(repr(float(v)) + "f" for v in params[i:i + 8]) for i in range(0, len(params), 8))
+    text = ("// generated by slothdb_amd/build.py from slothdb_amd/selector_weights.json\n"
+            f"static const float kSelectorBuiltin[{len(params)}] = {{\n{body}\n}};\n")
+    path = os.path.join(GEN, "selector_builtin.inc")
+    with open(path + ".tmp", "w") as f:
+        f.write(text)
+    os.replace(path + ".tmp", path)
+    return path
+
+
 def build_gpu():
     os.makedirs(LIB, exist_ok=True)
+    gen_selector_builtin()
     out = os.path.join(LIB, "libslothdb_gpu.so")
     tmp = out + ".tmp"
-    _run([HIPCC, *HIP_FLAGS, "-o", tmp, os.path.join(CSRC, "gbagg.hip")])
+    _run([HIPCC, *HIP_FLAGS, "-I", GEN, "-o", tmp, os.path.join(CSRC, "gbagg.hip")])
     os.replace(tmp, out)
     return out
 

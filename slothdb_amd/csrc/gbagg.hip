@@ -35,6 +35,9 @@
 using namespace gbagg;
 
 static_assert(SEL_NPARAM == GBAGG_SELECTOR_PARAMS, "selector parameter layout");
+// The trained selector (slothdb_amd/selector_weights.json, generated at build time).
+#include "selector_builtin.inc"
+static_assert(sizeof(kSelectorBuiltin) / sizeof(float) == SEL_NPARAM, "built-in selector parameter count");
 static_assert(SEL_IN == GBAGG_SELECTOR_FEATURES && SEL_OUT == GBAGG_SELECTOR_OUTPUTS, "selector shape");
 
 namespace {
@@ -378,13 +381,15 @@ struct gbagg_handle {
     int32_t force_part_bits = -1;   // radix fanout override (-1 = from the estimate)
     uint32_t k4_lds_kb = 78;        // per-partition LDS table budget (two workgroups per CU)
     // learned selector
-    std::vector<float> sel_params;  // empty: no selector
+    std::vector<float> sel_params;  // empty: no selector attached
+    bool sel_builtin = true;        // AUTO plans without an attached selector use the built-in one
     bool collect_features = false;
     DevBuf dsel, dfeat, dpred;
     float feat[SEL_IN] = {}, pred[SEL_OUT] = {};
     bool feat_valid = false;
     double sel_distinct = 0;
     int32_t sel_used = 0, sel_path = -1;
+    bool sel_count = false;  // the selector's count is within 4x of the sample's estimate
     // last run
     gbagg_strategy used_strategy = GBAGG_STRATEGY_AUTO;
     uint32_t lds_slots = 0, agg_grid = 0, retries = 0, radix_bits = 0;
@@ -1412,23 +1417,30 @@ void run_sample(gbagg_handle *h) {
 // Selector features of the sample just taken, and with a selector its
 // outputs; one small copy back.
 void run_features(gbagg_handle *h, bool infer) {
-    hipStream_t s = h->stream;
     uint32_t key_kind = 0;
     if (h->dq.key_mode == KM_WIDE) key_kind = 3;
     else if (h->dq.key_mode == KM_PACK32) key_kind = 2;
     else if (h->dq.key_mode == KM_SINGLE && h->dq.ukind[h->dq.key_u[0]] == CK_I64) key_kind = 1;
-    h->dfeat.ensure(SEL_IN * 4);
-    hipLaunchKernelGGL(k_sel_features, dim3(1), dim3(1024), 0, s, d_rp(h), (const DevPlan *)h->dplan.as<DevPlan>(),
-                       h->rows_total, key_kind, h->dfeat.as<float>());
-    if (infer) {
-        h->dpred.ensure(SEL_OUT * 4);
-        hipLaunchKernelGGL(k_sel_infer, dim3(1), dim3(SEL_H), 0, s, (const float *)h->dsel.as<float>(),
-                           (const float *)h->dfeat.as<float>(), h->dpred.as<float>());
-        HIPCHK(hipMemcpyAsync(h->pred, h->dpred.ptr, SEL_OUT * 4, hipMemcpyDeviceToHost, s));
-    }
-    HIPCHK(hipMemcpyAsync(h->feat, h->dfeat.ptr, SEL_IN * 4, hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
+    hipLaunchKernelGGL(k_sel_plan, dim3(1), dim3(SEL_H), 0, h->stream, h->dplan.as<DevPlan>(), h->rows_total, key_kind,
+                       infer ? (const float *)h->dsel.as<float>() : (const float *)nullptr);
+}
+
+// After read_plan: the features and outputs run_features left in the plan block.
+void take_features(gbagg_handle *h) {
+    std::memcpy(h->feat, h->hplan->feat, sizeof(h->feat));
+    std::memcpy(h->pred, h->hplan->pred, sizeof(h->pred));
     h->feat_valid = true;
+}
+
+// Query shapes the built-in selector was trained on (one or two INTEGER /
+// BIGINT key columns, no predicates, no TopN or row limit); for others an AUTO
+// plan follows the default rules unless a selector was attached.
+bool selector_in_domain(const gbagg_handle *h) {
+    if (h->dq.npred || h->topn_device || h->topn_col >= 0 || h->row_limit || h->tuple || h->phase1) return false;
+    if (h->dq.key_mode == KM_NONE) return false;
+    for (uint32_t g = 0; g < h->group_cols.size(); g++)
+        if (h->key_varchar[g]) return false;
+    return true;
 }
 
 // ---- double sums ------------------------------------------------------------
@@ -1479,6 +1491,25 @@ void fx_prepare(gbagg_handle *h, bool sampled) {
     upload_ctx(h);
 }
 
+// The selector's outputs of the last sample (h->pred) as a distinct count and
+// a path (sel_distinct, sel_path); true when the count is within 4x of the
+// sample's own estimate, so the plan may follow it. Output 0 is
+// log2((distinct + 1) / (default + 1)) / 4, where default is the sample's own
+// estimate (keys seen plus the unseen share of the singletons).
+bool selector_decision(gbagg_handle *h) {
+    const DevPlan &p = *h->hplan;
+    const double seen = (double)p.sample_distinct;
+    const double rest = p.sample_rows ? (double)p.sample_f1 / p.sample_rows * (double)h->rows_total : 0;
+    const double dflt = std::max(1.0, std::min((double)h->rows_total, seen + rest));
+    const double pred = std::max(1.0, (dflt + 1.0) * std::exp2(4.0 * (double)h->pred[0]) - 1.0);
+    int path = 0;
+    for (int k = 1; k < SEL_CLS; k++)
+        if (h->pred[1 + k] > h->pred[1 + path]) path = k;
+    h->sel_distinct = pred;
+    h->sel_path = path;
+    return pred <= 4.0 * dflt && pred * 4.0 >= dflt;
+}
+
 void run_lds_hash(gbagg_handle *h, Timer &tm, bool global_only, uint64_t est = 0) {
     const DeviceInfo &di = device_info();
     hipStream_t s = h->stream;
@@ -1488,7 +1519,7 @@ void run_lds_hash(gbagg_handle *h, Timer &tm, bool global_only, uint64_t est = 0
     double seen = (double)hp.sample_distinct;
     double rest = hp.sample_rows ? (double)hp.sample_f1 / hp.sample_rows * (double)h->rows_total : 0;
     uint64_t want = (uint64_t)(2.0 * (seen + rest)) + 1024;
-    if (h->sel_used) want = (uint64_t)(2.0 * h->sel_distinct) + 1024;  // the learned count sizes the table
+    if (h->sel_used && h->sel_count) want = (uint64_t)(2.0 * h->sel_distinct) + 1024;  // the learned count sizes the table
     if (est) want = 2 * est + 1024;  // a HyperLogLog count over all rows
     if (h->dq.key_mode == KM_NONE) want = 1024;
     uint64_t cap = std::min<uint64_t>(h->cap_max, next_pow2(want));
@@ -2266,6 +2297,7 @@ void finalize(gbagg_handle *h) {
     }
     h->retries = 0;
     h->sel_used = 0;
+    h->sel_count = false;
     h->sel_path = -1;
     h->sel_distinct = 0;
     h->feat_valid = false;
@@ -2437,10 +2469,19 @@ void finalize_once(gbagg_handle *h) {
         run_lds_hash(h, tm, true);
     } else {
         run_sample(h);
+        // an attached selector, or for AUTO plans of the shapes it was trained
+        // on the built-in one
+        if (h->sel_params.empty() && h->sel_builtin && st == GBAGG_STRATEGY_AUTO && selector_in_domain(h)) {
+            h->sel_params.assign(kSelectorBuiltin, kSelectorBuiltin + SEL_NPARAM);
+            h->dsel.ensure((size_t)SEL_NPARAM * 4);
+            HIPCHK(hipMemcpyAsync(h->dsel.ptr, h->sel_params.data(), (size_t)SEL_NPARAM * 4, hipMemcpyHostToDevice,
+                                  h->stream));
+        }
         const bool sel = !h->sel_params.empty();
         if (sel || h->collect_features) run_features(h, sel);
         tm.mark();  // sample end
         read_plan(h);
+        if (sel || h->collect_features) take_features(h);
         fx_prepare(h, true);
         if (st == GBAGG_STRATEGY_AUTO) {
             const DevPlan &p = *h->hplan;
@@ -2454,18 +2495,12 @@ void finalize_once(gbagg_handle *h) {
             if (sel) {
                 // the selector's path and count, unless they disagree with the
                 // sample's own estimate by more than 4x (then the default plan)
-                // output 0 is log2((distinct + 1) / (default + 1)) / 4, where
-                // default is the sample's own estimate (seen + unseen share)
-                const double seen = (double)p.sample_distinct;
-                const double rest = p.sample_rows ? (double)p.sample_f1 / p.sample_rows * (double)h->rows_total : 0;
-                const double dflt = std::max(1.0, std::min((double)h->rows_total, seen + rest));
-                const double pred = std::max(1.0, (dflt + 1.0) * std::exp2(4.0 * (double)h->pred[0]) - 1.0);
-                int path = 0;
-                for (int k = 1; k < SEL_CLS; k++)
-                    if (h->pred[1 + k] > h->pred[1 + path]) path = k;
-                h->sel_distinct = pred;
-                h->sel_path = path;
-                if (pred <= 4.0 * dflt && pred * 4.0 >= dflt) {
+                // RADIX sizes itself from the count pass and DIRECT from the
+                // exact key range: their path is taken whatever the count
+                const bool count_ok = selector_decision(h);
+                h->sel_count = count_ok;
+                if (count_ok || h->sel_path == 0 || h->sel_path == 3) {
+                    const int path = h->sel_path;
                     h->sel_used = 1;
                     static const gbagg_strategy paths[SEL_CLS] = {GBAGG_STRATEGY_DIRECT, GBAGG_STRATEGY_LDS_HASH,
                                                                   GBAGG_STRATEGY_GLOBAL_HASH, GBAGG_STRATEGY_RADIX};
@@ -2504,6 +2539,34 @@ void finalize_once(gbagg_handle *h) {
         h->ms_partition = 0;
         h->ms_emit = tm.between(ne - 2, ne - 1);
     }
+}
+
+// The learned selector's plan for the consumed rows without aggregating them:
+// the AUTO plan's sample, its features and the MLP outputs. Returns false when
+// the prediction is more than 4x from the sample's own estimate.
+bool selector_plan(gbagg_handle *h, double &distinct, int &path) {
+    ensure_runtime(h);
+    std::memset(h->hplan, 0, sizeof(DevPlan));
+    std::memset(h->hctx, 0, sizeof(DevCtx));
+    h->dq.fx_flags = &h->dplan.as<DevPlan>()->overflow;
+    h->hctx->q = h->dq;
+    upload_batches(h);
+    upload_ctx(h);
+    upload_plan(h, 1024);
+    run_sample(h);
+    if (h->sel_params.empty()) {
+        h->sel_params.assign(kSelectorBuiltin, kSelectorBuiltin + SEL_NPARAM);
+        h->dsel.ensure((size_t)SEL_NPARAM * 4);
+        HIPCHK(hipMemcpyAsync(h->dsel.ptr, h->sel_params.data(), (size_t)SEL_NPARAM * 4, hipMemcpyHostToDevice,
+                              h->stream));
+    }
+    run_features(h, true);
+    read_plan(h);
+    take_features(h);
+    const bool ok = selector_decision(h);
+    distinct = h->sel_distinct;
+    path = h->sel_path;
+    return ok;
 }
 
 // A StrCol's dictionary on the host: per code its first arena word and length,
@@ -2869,6 +2932,11 @@ gbagg_status gbagg_set_param(gbagg_handle *h, const char *name, int64_t value) {
         else if (n == "str_hash_bits") { in(1, 64); h->str_key_mask = value == 64 ? ~0ull : (1ull << value) - 1; }
         else if (n == "strategy") { in(0, 4); h->strategy = (gbagg_strategy)value; }
         else if (n == "static_shapes") { in(0, 1); h->allow_static = value != 0; pick_kernels(h); }
+        else if (n == "selector") {  // 0: no selector; 1: the built-in trained one (default)
+            in(0, 1);
+            h->sel_builtin = value != 0;
+            if (!value) h->sel_params.clear();
+        }
         else fail(GBAGG_INVALID, "unknown parameter " + n);
     });
 }
@@ -2925,8 +2993,9 @@ gbagg_status gbagg_hash_aggregate_i32(const int32_t *keys, const int32_t *values
 gbagg_status gbagg_selector_set(gbagg_handle *h, const float *params, uint32_t num_params) {
     return guarded([&] {
         if (!h) fail(GBAGG_INVALID, "null handle");
-        if (!params) {
+        if (!params) {  // no selector at all, not even the built-in one
             h->sel_params.clear();
+            h->sel_builtin = false;
             return;
         }
         if (num_params != (uint32_t)SEL_NPARAM) fail(GBAGG_INVALID, "selector parameter count differs from GBAGG_SELECTOR_PARAMS");
@@ -3039,6 +3108,10 @@ struct gbagg_batch {
     BatchTune tune{65536, 2048, 512, 2};
     uint32_t cap_pct = 150;  // table slots per estimated key (percent)
     uint32_t mode_rule = 2;  // BatchCtl::mode_rule (2: LDS-first also for keys that repeat within a tile)
+    uint32_t use_selector = 1;  // per query: the learned selector's path picks the mode
+    // selector plans per input (key column, rows, query shape): (distinct, mode) or mode ~0u when not usable
+    std::unordered_map<uint64_t, std::pair<uint64_t, uint32_t>> sel_cache;
+    uint32_t sel_queries = 0;   // batched queries of the last run whose mode the selector chose
     // the batched queries of the last run
     std::vector<uint32_t> item_of;        // batched query j -> item index
     std::vector<BatchQuery> hq;           // uploaded description
@@ -3167,6 +3240,32 @@ void batch_prepare(gbagg_batch *b, const gbagg_batch_item *items, uint32_t count
         bq.item0 = (uint32_t)b->items.size();
         for (uint64_t t = 0; t * b->tune.tile_rows < bq.rows; t++) b->items.push_back(make_uint2(j, (uint32_t)t));
         bq.nitems = (uint32_t)b->items.size() - bq.item0;
+        if (b->use_selector) {
+            // the learned selector's plan of this input: its path picks the
+            // mode (DIRECT / LDS_HASH: LDS-first tiles; GLOBAL_HASH / RADIX:
+            // the global table), its count may enlarge the table
+            uint64_t ck = mix_sig(mix_sig(mix_sig(0x5e1ec7, (uint64_t)(uintptr_t)bq.key), bq.rows), bq.ops);
+            for (int a = 0; a < BQ_MAXACC; a++) ck = mix_sig(ck, (uint64_t)(uintptr_t)bq.val[a]);
+            auto it = b->sel_cache.find(ck);
+            if (it == b->sel_cache.end()) {
+                gbagg_handle *sh = nullptr;
+                std::pair<uint64_t, uint32_t> plan(0, ~0u);
+                if (gbagg_plan_create(q, &sh) == GBAGG_OK &&
+                    gbagg_consume_device(sh, items[i].columns, items[i].num_rows) == GBAGG_OK) {
+                    double d = 0;
+                    int path = -1;
+                    if (selector_plan(sh, d, path) && path >= 0)
+                        plan = {(uint64_t)std::min<double>(d, (double)bq.rows), path <= 1 ? (uint32_t)BM_LDS : (uint32_t)BM_GLOBAL};
+                }
+                if (sh) gbagg_destroy(sh);
+                it = b->sel_cache.emplace(ck, plan).first;
+            }
+            if (it->second.second != ~0u) {
+                bq.sel = 1;
+                bq.sel_est = it->second.first;
+                bq.sel_mode = it->second.second;
+            }
+        }
         pool += batch_pool_bound(bq.rows, b->cap_pct);
         sig = mix_sig(sig, (uint64_t)(uintptr_t)bq.key);
         sig = mix_sig(sig, bq.rows);
@@ -3181,6 +3280,7 @@ void batch_prepare(gbagg_batch *b, const gbagg_batch_item *items, uint32_t count
     sig = mix_sig(sig, tn.wgs_per_cu);
     sig = mix_sig(sig, b->cap_pct);
     sig = mix_sig(sig, b->mode_rule);
+    sig = mix_sig(sig, b->use_selector);
     const uint32_t nq = (uint32_t)b->hq.size();
     if (nq == 0) {
         b->drop_graph();
@@ -3365,6 +3465,9 @@ gbagg_status gbagg_batch_set_param(gbagg_batch *b, const char *name, int64_t val
         } else if (n == "mode_rule") {
             if (value < 0 || value > 2) fail(GBAGG_INVALID, "mode_rule out of range [0, 2]");
             b->mode_rule = (uint32_t)value;
+        } else if (n == "selector") {
+            if (value < 0 || value > 1) fail(GBAGG_INVALID, "selector out of range [0, 1]");
+            b->use_selector = (uint32_t)value;
         } else {
             fail(GBAGG_INVALID, "unknown batch parameter: " + n);
         }
@@ -3431,6 +3534,7 @@ gbagg_status gbagg_batch_info_get(gbagg_batch *b, gbagg_batch_info *out) {
         out->ms_device = b->ms_device;
         out->ms_single = b->ms_single;
         out->tune = gbagg_batch_tune{b->tune.tile_rows, b->tune.lds_slots, b->tune.block, b->tune.wgs_per_cu};
+        for (const auto &q : b->hq) out->selector_queries += q.sel;
     });
 }
 

@@ -10,7 +10,9 @@
 //   k_b_estimate  one block per query: merged registers -> distinct estimate,
 //                 which sizes the query's hash table and picks its mode (LDS
 //                 first when the query's keys fit the LDS table or repeat
-//                 within a tile, by the first tile's own estimate; else global)
+//                 within a tile, by the first tile's own estimate; else global);
+//                 a query the learned selector planned on the host takes the
+//                 selector's mode, and its predicted count may enlarge the table
 //   k_b_layout    one block: table offsets (prefix sum of capacities)
 //   k_b_init      fills every table with empty keys and identity states
 //   k_b_agg       persistent work queue over (query, tile) items: LDS-first
@@ -55,6 +57,10 @@ struct BatchQuery {
     uint64_t groups;      // result rows
     uint32_t mode;        // BatchMode
     uint32_t overflow;    // table probe bound hit
+    // the learned selector's plan of the query (host-set; sel = 0: none)
+    uint64_t sel_est;     // predicted distinct keys
+    uint32_t sel_mode;    // BatchMode from the predicted path
+    uint32_t sel;
 };
 
 struct BatchTune {
@@ -180,6 +186,15 @@ __global__ __launch_bounds__(256) void k_b_estimate(BatchQuery *__restrict__ bq,
         const bool fits = e <= (uint64_t)ctl->lds_slots * 4;
         const bool repeats = tile_rows >= 2.0 * tile_est;
         q.mode = (ctl->mode_rule == 1 || fits || (ctl->mode_rule == 2 && repeats)) ? BM_LDS : BM_GLOBAL;
+        // a query the learned selector planned takes its mode; the table keeps
+        // at least the HyperLogLog size
+        if (q.sel && ctl->mode_rule != 1) {
+            q.mode = q.sel_mode;
+            if (q.sel_est > e) {
+                cap = (min(q.sel_est, q.rows) * ctl->cap_pct + 99) / 100 + 64;
+                q.cap = (cap + BQ_CHUNK - 1) / BQ_CHUNK * BQ_CHUNK;
+            }
+        }
         q.groups = 0;
         q.overflow = 0;
     }

@@ -129,6 +129,13 @@ struct DevPlan {
     int64_t key_max;
     uint32_t fx_exp[MAXU];     // per loaded slot: largest biased exponent of its finite double
                                // values (sampled rows, or all rows for the exact bound)
+    // sample frequency statistics for the learned selector's features
+    uint64_t sample_f2, sample_f3;  // keys seen exactly twice / three times
+    uint64_t sample_max;            // largest count of one key
+    uint64_t sample_ent;            // sum over keys of c * log2(c), fixed point with 2^-20 units
+    uint64_t sample_hist[16];       // keys with count in [2^j, 2^(j+1))
+    float feat[32];                 // the selector's features of the sample
+    float pred[8];                  // its outputs (5 used)
 };
 
 // The global table: (cap + 2) records of (1 + nacc) words: key, accumulators.

@@ -181,21 +181,35 @@ __global__ __launch_bounds__(256) void k_heavy_hist(const RadixParams *__restric
                                                     uint32_t *ghist) {
     const RadixParams &rp = *rpp;
     __shared__ uint32_t hist[65];
-    __shared__ unsigned long long s_distinct, s_f1;
+    __shared__ unsigned long long s_distinct, s_f1, s_f2, s_f3, s_max, s_ent;
+    __shared__ unsigned long long s_h16[16];
     __shared__ int64_t s_lo, s_hi;
     if (threadIdx.x < 65) hist[threadIdx.x] = 0;
-    if (threadIdx.x == 0) { s_distinct = 0; s_f1 = 0; s_lo = INT64_MAX; s_hi = INT64_MIN; }
+    if (threadIdx.x < 16) s_h16[threadIdx.x] = 0;
+    if (threadIdx.x == 0) { s_distinct = 0; s_f1 = 0; s_f2 = 0; s_f3 = 0; s_max = 0; s_ent = 0; s_lo = INT64_MAX; s_hi = INT64_MIN; }
     __syncthreads();
     int64_t lo = INT64_MAX, hi = INT64_MIN;  // sampled key word range (plan->key_min/max preset)
+    unsigned long long f2 = 0, f3 = 0, mx = 0, ent = 0;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < rp.scap; i += gridDim.x * blockDim.x) {
         uint32_t c = rp.scnt[i];
         if (!c) continue;
         atomicAdd(&hist[min(c, 64u)], 1u);
         atomicAdd(&s_distinct, 1ULL);
         if (c == 1) atomicAdd(&s_f1, 1ULL);
+        f2 += c == 2;
+        f3 += c == 3;
+        mx = mx > c ? mx : c;
+        // c * log2(c) in 2^-20 units: integer sums, so the features do not
+        // depend on the order of the additions
+        ent += (unsigned long long)((double)c * log2((double)c) * 1048576.0 + 0.5);
+        atomicAdd(&s_h16[min(15u, 31u - (uint32_t)__clz(c))], 1ULL);
         lo = min(lo, (int64_t)rp.skeys[i]);
         hi = max(hi, (int64_t)rp.skeys[i]);
     }
+    if (f2) atomicAdd(&s_f2, f2);
+    if (f3) atomicAdd(&s_f3, f3);
+    if (mx) atomicMax(&s_max, mx);
+    if (ent) atomicAdd(&s_ent, ent);
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
         lo = min(lo, (int64_t)__shfl_xor(lo, off));
@@ -207,9 +221,15 @@ __global__ __launch_bounds__(256) void k_heavy_hist(const RadixParams *__restric
     }
     __syncthreads();
     if (threadIdx.x < 65 && hist[threadIdx.x]) atomicAdd(&ghist[threadIdx.x], hist[threadIdx.x]);
+    if (threadIdx.x < 16 && s_h16[threadIdx.x])
+        atomicAdd((unsigned long long *)&plan->sample_hist[threadIdx.x], s_h16[threadIdx.x]);
     if (threadIdx.x == 0) {
         if (s_distinct) atomicAdd((unsigned long long *)&plan->sample_distinct, s_distinct);
         if (s_f1) atomicAdd((unsigned long long *)&plan->sample_f1, s_f1);
+        if (s_f2) atomicAdd((unsigned long long *)&plan->sample_f2, s_f2);
+        if (s_f3) atomicAdd((unsigned long long *)&plan->sample_f3, s_f3);
+        if (s_max) atomicMax((unsigned long long *)&plan->sample_max, s_max);
+        if (s_ent) atomicAdd((unsigned long long *)&plan->sample_ent, s_ent);
         if (s_lo <= s_hi) {  // one device-scope update per block
             __hip_atomic_fetch_min(&plan->key_min, s_lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             __hip_atomic_fetch_max(&plan->key_max, s_hi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);

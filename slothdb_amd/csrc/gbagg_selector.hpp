@@ -10,7 +10,8 @@
 // sample's own estimate by more than 4x is ignored in favour of the default
 // plan.
 //
-//   k_sel_features  one block: frequency statistics of the sample table
+//   k_sel_plan      one block: features from the sample's statistics (gathered by
+//                   k_heavy_hist), then the forward pass, into the plan block
 //   k_sel_infer     one block of 64 per input row: forward pass
 //   k_sel_train     one block: minibatch Adam over the whole training set,
 //                   forward and backward passes with activations in LDS
@@ -50,74 +51,62 @@ constexpr int SEL_NPARAM = SEL_OFF_SIG + SEL_IN;
 // feat[30] log2(span + 1)/64, span = largest - smallest sampled key word
 //          (single-column keys; 1 otherwise)
 // feat[31] log2(rows/n + 1)/32
-__global__ __launch_bounds__(1024) void k_sel_features(const RadixParams *__restrict__ rpp,
-                                                       const DevPlan *__restrict__ plan, uint64_t rows,
-                                                       uint32_t key_kind, float *feat) {
-    const RadixParams &rp = *rpp;
-    __shared__ unsigned long long s_d, s_f1, s_f2, s_f3, s_max;
-    __shared__ long long s_lo, s_hi;
-    __shared__ double s_ent;
-    __shared__ unsigned long long s_hist[16];
+__device__ __forceinline__ void sel_layer(const float *W, const float *b, const float *in, int nin, float *out,
+                                          bool relu);
+
+// The sample's features from the statistics k_heavy_hist gathered in the
+// plan block, and with parameters P (may be null) the MLP's outputs, both
+// into the plan block (one block of SEL_H threads; no scan of the sample
+// table, no copy of its own: the host reads them with the plan).
+__global__ __launch_bounds__(SEL_H) void k_sel_plan(DevPlan *plan, uint64_t rows, uint32_t key_kind,
+                                                    const float *__restrict__ P) {
+    __shared__ float x[SEL_IN], h1[SEL_H], h2[SEL_H], h3[SEL_H];
     if (threadIdx.x == 0) {
-        s_d = s_f1 = s_f2 = s_f3 = s_max = 0;
-        s_ent = 0;
-        s_lo = INT64_MAX;
-        s_hi = INT64_MIN;
-    }
-    if (threadIdx.x < 16) s_hist[threadIdx.x] = 0;
-    __syncthreads();
-    unsigned long long d = 0, f1 = 0, f2 = 0, f3 = 0, mx = 0;
-    long long lo = INT64_MAX, hi = INT64_MIN;
-    double ent = 0;
-    for (uint32_t i = threadIdx.x; i < rp.scap; i += blockDim.x) {
-        const uint32_t c = rp.scnt[i];
-        if (!c) continue;
-        const long long k = (long long)rp.skeys[i];
-        lo = min(lo, k);
-        hi = max(hi, k);
-        d++;
-        f1 += c == 1;
-        f2 += c == 2;
-        f3 += c == 3;
-        mx = mx > c ? mx : c;
-        ent += (double)c * log2((double)c);
-        const uint32_t bin = min(15u, 31u - (uint32_t)__clz(c));
-        atomicAdd(&s_hist[bin], 1ull);
-    }
-    atomicAdd(&s_d, d);
-    atomicAdd(&s_f1, f1);
-    atomicAdd(&s_f2, f2);
-    atomicAdd(&s_f3, f3);
-    atomicMax(&s_max, mx);
-    atomicMin(&s_lo, lo);
-    atomicMax(&s_hi, hi);
-    atomicAdd(&s_ent, ent);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const double n = (double)max<uint64_t>(plan->sample_rows, 1), D = (double)max<unsigned long long>(s_d, 1);
+        float *feat = plan->feat;
+        const uint64_t d = plan->sample_distinct, f1 = plan->sample_f1, f2 = plan->sample_f2, f3 = plan->sample_f3;
+        const uint64_t mx = plan->sample_max;
+        const double n = (double)max<uint64_t>(plan->sample_rows, 1), D = (double)max<uint64_t>(d, 1);
         const double R = (double)rows;
-        const double gee = (double)(s_d - s_f1) + (double)s_f1 * sqrt(R / n);
-        const double chao = (double)s_d + (double)s_f1 * (double)s_f1 / (2.0 * (double)max<unsigned long long>(s_f2, 1));
+        const double ent = (double)plan->sample_ent / 1048576.0;
+        const double gee = (double)(d - f1) + (double)f1 * sqrt(R / n);
+        const double chao = (double)d + (double)f1 * (double)f1 / (2.0 * (double)max<uint64_t>(f2, 1));
         feat[0] = (float)(log2(R + 1) / 32);
         feat[1] = (float)(log2(n + 1) / 32);
-        feat[2] = (float)(log2((double)s_d + 1) / 32);
-        feat[3] = (float)((double)s_d / n);
-        feat[4] = (float)(s_f1 / D);
-        feat[5] = (float)(s_f2 / D);
-        feat[6] = (float)(s_f3 / D);
-        feat[7] = (float)(log2((double)s_max + 1) / 32);
-        feat[8] = (float)((double)s_max / n);
-        feat[9] = (float)((log2(n) - s_ent / n) / max(1.0, log2(n)));
-        for (int j = 0; j < 16; j++) feat[10 + j] = (float)(s_hist[j] / D);
+        feat[2] = (float)(log2((double)d + 1) / 32);
+        feat[3] = (float)((double)d / n);
+        feat[4] = (float)(f1 / D);
+        feat[5] = (float)(f2 / D);
+        feat[6] = (float)(f3 / D);
+        feat[7] = (float)(log2((double)mx + 1) / 32);
+        feat[8] = (float)((double)mx / n);
+        feat[9] = (float)((log2(n) - ent / n) / max(1.0, log2(n)));
+        for (int j = 0; j < 16; j++) feat[10 + j] = (float)(plan->sample_hist[j] / D);
         feat[26] = (float)(log2(gee + 1) / 32);
         feat[27] = (float)(log2(chao + 1) / 32);
         // key_kind: 0 one 4-byte column, 1 one 8-byte column, 2 two 4-byte
         // columns, 3 a 16-byte key
         feat[28] = key_kind == 0 ? 0.25f : key_kind == 3 ? 1.f : 0.5f;
         feat[29] = key_kind >= 2;
-        const double span = s_lo <= s_hi ? (double)((unsigned long long)s_hi - (unsigned long long)s_lo) : 0.0;
+        const int64_t lo = plan->key_min, hi = plan->key_max;  // sampled key word range
+        const double span = lo <= hi ? (double)((unsigned long long)hi - (unsigned long long)lo) : 0.0;
         feat[30] = key_kind <= 1 ? (float)(log2(span + 1) / 64) : 1.f;
         feat[31] = (float)(log2(R / n + 1) / 32);
+    }
+    __syncthreads();
+    if (!P) return;
+    if (threadIdx.x < SEL_IN)
+        x[threadIdx.x] = (plan->feat[threadIdx.x] - P[SEL_OFF_MU + threadIdx.x]) / P[SEL_OFF_SIG + threadIdx.x];
+    __syncthreads();
+    sel_layer(P + SEL_OFF_W1, P + SEL_OFF_B1, x, SEL_IN, h1, true);
+    __syncthreads();
+    sel_layer(P + SEL_OFF_W2, P + SEL_OFF_B2, h1, SEL_H, h2, true);
+    __syncthreads();
+    sel_layer(P + SEL_OFF_W3, P + SEL_OFF_B3, h2, SEL_H, h3, true);
+    __syncthreads();
+    if (threadIdx.x < SEL_OUT) {
+        float z = P[SEL_OFF_B4 + threadIdx.x];
+        for (int i = 0; i < SEL_H; i++) z += P[SEL_OFF_W4 + threadIdx.x * SEL_H + i] * h3[i];
+        plan->pred[threadIdx.x] = z;
     }
 }
 

@@ -423,7 +423,8 @@ class BatchTune(C.Structure):
 class BatchInfo(C.Structure):
     _fields_ = [("queries", C.c_uint32), ("batched", C.c_uint32), ("single", C.c_uint32),
                 ("graph_captures", C.c_uint32), ("graph_replays", C.c_uint32),
-                ("ms_device", C.c_double), ("ms_single", C.c_double), ("tune", BatchTune)]
+                ("ms_device", C.c_double), ("ms_single", C.c_double), ("tune", BatchTune),
+                ("selector_queries", C.c_uint32)]
 
 
 class GpuBatch:

@@ -196,3 +196,24 @@ def test_batch_full_512_queries_vs_oracle():
         rows += len(cols[0].values)
     print(f"[batch512] {rows} rows checked", flush=True)
     b.close()
+
+
+def test_batch_selector_plans_modes_results_unchanged():
+    # with the learned selector (default) most queries take its mode; without
+    # it the sketch rule decides; the results are the same either way
+    host, dev = _entries(dict(rows_hi=300_000, distinct_hi=200_000), 24)
+    got = {}
+    for use in (1, 0):
+        b = G.GpuBatch()
+        b.set_param("selector", use)
+        b.set_queries(dev)
+        b.run()
+        info = b.info()
+        print(f"[batch] selector={use}: {info['selector_queries']} queries planned by it", flush=True)
+        assert (info["selector_queries"] > 0) if use else info["selector_queries"] == 0
+        got[use] = [sorted(b.result(i).rows()) for i in range(len(host))]
+        b.close()
+    assert got[0] == got[1]
+    for i, (q, cols) in enumerate(host):
+        exp, types = _exact(q, cols)
+        assert_same_groups(got[1][i], exp, 1, types)

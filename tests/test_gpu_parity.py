@@ -17,9 +17,9 @@ def _exact(q, cols, dicts=None):
     return oracle_rows(c, v, t, dicts), t
 
 
-def _check(q, cols, device=True, dicts=None):
+def _check(q, cols, device=True, dicts=None, params=None):
     print(f"[parity] groups={q.group_cols} aggs={q.aggs} rows={len(cols[0])}", flush=True)
-    res, info = G.run(q, cols, device=device)
+    res, info = G.run(q, cols, device=device, params=params)
     exp, types = _exact(q, cols, dicts)
     worst = assert_same_groups(res.rows(), exp, len(q.group_cols), types)
     return res, info, worst
@@ -203,7 +203,7 @@ def test_topn_count_desc_matches_exact():
 def test_radix_highcard(strategy):
     q, cols = datagen.highcard_workload(n=3_000_000, nkeys=2_000_000)
     q.strategy = strategy
-    res, info, worst = _check(q, cols)
+    res, info, worst = _check(q, cols, params={"selector": 0})  # the default plan's rule
     assert info["strategy"] == G.RADIX
     assert info["heavy_keys"] > 0 and info["cold_rows"] < 3_000_000
     print(f"radix info {info} worst_rel={worst:.2e}")
@@ -257,7 +257,7 @@ def test_radix_composite_topn_and_full():
     q, cols = datagen.composite_workload(n=3_000_000, dict_size=200_000, topn=10)
     dic = cols[1].dictionary
     exp, types = _exact(datagen.composite_query(topn=None), cols, {1: dic})
-    res, info = G.run(q, cols, device=True)
+    res, info = G.run(q, cols, device=True, params={"selector": 0})  # the default plan's rule
     assert info["strategy"] == G.RADIX and res.num_rows == 10
     assert_topn_tie_aware(res.rows(), exp, order_col=2, k=10, types=types)
     q2 = datagen.composite_query(topn=None)
