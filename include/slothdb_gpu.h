@@ -119,8 +119,12 @@ typedef struct {
  *             (group columns, c) with partial states of the other aggregates
  *             (rows with a NULL c form their own groups there), then GROUP BY
  *             the group columns over those rows, counting non-NULL c; a group
- *             whose c is always NULL counts 0. All DISTINCT aggregates of a
- *             query share one c; at most 7 group columns, of any type, with
+ *             whose c is always NULL counts 0. Each DISTINCT argument column
+ *             of a query runs its own phase 1 (the other aggregates' partial
+ *             states ride on the first) and phase 2 counts each over its own
+ *             phase 1's groups (phase 2 loads the group key, one column per
+ *             argument and one per partial state: at most 8 in all, as any
+ *             plan); at most 7 group columns, of any type, with
  *             NULL keys grouped or dropped as for other queries. DOUBLE values
  *             are distinct by value (-0.0 equals 0.0; all NaNs are one value),
  *             where the reference compares their shortest decimal text.

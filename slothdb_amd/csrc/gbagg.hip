@@ -321,8 +321,14 @@ struct gbagg_handle {
     // COUNT(DISTINCT): phase 1 groups by (group column, argument) with partial
     // states of the other aggregates; this handle is phase 2 over its result
     std::unique_ptr<gbagg_handle> phase1;
-    std::vector<std::vector<uint32_t>> p1_dict_off;  // per column: phase 1's VARCHAR group key
+    // COUNT(DISTINCT) over further argument columns: one more phase 1 each,
+    // grouping by (group columns, that argument), no aggregates
+    std::vector<std::unique_ptr<gbagg_handle>> phase1_more;
+    uint32_t distinct_args = 0;                      // argument columns (phase 1 handles)
+    uint32_t distinct_ng = 0;                        // the query's group columns
+    std::vector<std::vector<uint32_t>> p1_dict_off;  // per (phase 1, column): VARCHAR group key
     std::vector<std::vector<char>> p1_dict_bytes;    // dictionaries, as phase 2 input
+    DevBuf p2_null;                                  // zeros: data and validity of the NULL columns
     uint64_t wide_seed0 = 0;       // KM_WIDE: first seed of wide_mix
     // VARCHAR columns given as strings (GBAGG_STR_RAW), interned at finalize
     std::vector<uint8_t> str_mode;  // per column: 0 no batch yet, 1 dictionary codes, 2 strings
@@ -726,6 +732,11 @@ void build_tuple(gbagg_handle *h, const gbagg_query_desc *q) {
 //            SUM, MIN of MIN, MAX of MAX, AVG = SUM of sums / SUM of counts)
 // k is zero to seven columns of any type; x is one column of any type (DOUBLE
 // arguments are distinct by value: -0.0 equals 0.0, and all NaNs are one).
+// Several argument columns x_0 .. x_{m-1} (each aggregate keeps its own
+// distinct set, physical_planner.cpp:6076, 12325-12349): phase 1 runs once
+// per argument, GROUP BY (k..., x_j), the partial states only in the first;
+// phase 2 takes every phase 1 result as a batch of its own and counts each
+// x_j over its own batch's rows (NULL in the others).
 void build_distinct(gbagg_handle *h, const gbagg_query_desc *q) {
     if (q->num_columns == 0 || q->num_columns > GBAGG_MAX_COLUMNS || !q->column_types)
         fail(GBAGG_INVALID, "num_columns must be 1..16 with column_types");
@@ -735,27 +746,25 @@ void build_distinct(gbagg_handle *h, const gbagg_query_desc *q) {
     for (uint32_t g = 0; g < q->num_group_cols; g++)
         if (q->group_cols[g] >= q->num_columns) fail(GBAGG_INVALID, "group column out of range");
     auto nullable = [&](uint32_t c) { return q->column_nullable ? q->column_nullable[c] != 0 : true; };
-    int32_t x = -1;
-    bool only_distinct = true;
+    std::vector<int32_t> xs;  // the argument columns, in first-use order
     for (uint32_t a = 0; a < q->num_aggs; a++) {
         const gbagg_agg &g = q->aggs[a];
         if (g.kind != GBAGG_AGG_COUNT_STAR && (g.column < 0 || (uint32_t)g.column >= q->num_columns))
             fail(GBAGG_INVALID, "aggregate column out of range");
-        if (g.kind != GBAGG_AGG_COUNT_DISTINCT) {
-            only_distinct = false;
-            continue;
-        }
-        if (x >= 0 && g.column != x) fail(GBAGG_INVALID, "all COUNT(DISTINCT) aggregates must share one column");
-        x = g.column;
+        if (g.kind == GBAGG_AGG_COUNT_DISTINCT && std::find(xs.begin(), xs.end(), g.column) == xs.end())
+            xs.push_back(g.column);
     }
+    const int32_t x = xs[0];
+    const uint32_t m = (uint32_t)xs.size();
     for (uint32_t a = 0; a < q->num_aggs; a++)
         if ((q->aggs[a].kind == GBAGG_AGG_MIN || q->aggs[a].kind == GBAGG_AGG_MAX) &&
             q->column_types[q->aggs[a].column] == GBAGG_TYPE_VARCHAR)
             fail(GBAGG_INVALID, "MIN/MAX over VARCHAR next to COUNT(DISTINCT) is not supported");
-    const gbagg_type xt = q->column_types[x];
-    if (!valid_type(xt)) fail(GBAGG_INVALID, "unsupported column type");
+    for (int32_t xc : xs)
+        if (!valid_type(q->column_types[xc])) fail(GBAGG_INVALID, "unsupported column type");
     const uint32_t ng = q->num_group_cols;
-    (void)only_distinct;
+    h->distinct_args = m;
+    h->distinct_ng = ng;
     // NULL arguments stay in phase 1 as a group of their own (its partial
     // states reach the other aggregates, and a group whose arguments are all
     // NULL still appears, with COUNT(DISTINCT) 0); phase 2 counts non-NULL
@@ -806,27 +815,68 @@ void build_distinct(gbagg_handle *h, const gbagg_query_desc *q) {
     h->phase1 = std::make_unique<gbagg_handle>();
     build_plan(h->phase1.get(), &d1);
     pick_kernels(h->phase1.get());
+    // the further arguments: GROUP BY (group columns, x_j), nothing else
+    for (uint32_t j = 1; j < m; j++) {
+        uint32_t gcj[GBAGG_MAX_GROUP_COLS];
+        for (uint32_t g = 0; g < ng; g++) gcj[g] = q->group_cols[g];
+        gcj[ng] = (uint32_t)xs[j];
+        gbagg_query_desc dj = *q;
+        dj.num_group_cols = ng + 1;
+        dj.group_cols = gcj;
+        dj.num_aggs = 0;
+        dj.aggs = nullptr;
+        dj.topn_col = -1;
+        dj.topn_limit = 0;
+        dj.row_limit = 0;
+        // a row with a NULL x_j adds nothing to its distinct count, so these
+        // drop as the query drops NULL keys (all groups come from phase 1 of x_0)
+        dj.drop_null_keys = q->drop_null_keys || !null_k ? 1 : 0;
+        h->phase1_more.push_back(std::make_unique<gbagg_handle>());
+        build_plan(h->phase1_more.back().get(), &dj);
+        pick_kernels(h->phase1_more.back().get());
+    }
 
-    // phase 2 over phase 1's output columns: [k,] x, partial states
+    // phase 2 over the phase 1 outputs: [k,] x_0 .. x_{m-1}, partial states.
+    // Each phase 1 result is one batch; its x_j fills column ng + j and the
+    // other arguments and (for j > 0) the partial states are NULL. Phase 2
+    // counts only whether an argument is NULL, so with several arguments they
+    // are declared by width (INTEGER for 4-byte values and codes, BIGINT for
+    // 8-byte values and DOUBLE bits).
     const gbagg_handle *p1 = h->phase1.get();
     std::vector<gbagg_type> t2;
     std::vector<uint8_t> n2;
-    for (size_t c = 0; c < ng + 1 + a1.size(); c++) {
-        const bool key = c <= ng;
-        t2.push_back(key ? q->column_types[gc1[c]] : p1->outs[p1->tuple ? c - ng : c].type);
-        const bool cnt = !key && p1->outs[p1->tuple ? c - ng : c].kind == OUT_COUNT;
-        const int32_t arg = key ? -1 : a1[c - ng - 1].column;
-        if (key) n2.push_back(!p1_drop && nullable(gc1[c]) ? 1 : 0);
-        else n2.push_back(cnt ? 0 : (arg >= 0 && nullable((uint32_t)arg)) ? 1 : 0);
+    for (uint32_t c = 0; c < ng; c++) {
+        t2.push_back(q->column_types[gc1[c]]);
+        n2.push_back(!p1_drop && nullable(gc1[c]) ? 1 : 0);
     }
-    const uint32_t base = ng + 1;  // first partial-state column
+    for (uint32_t j = 0; j < m; j++) {
+        const gbagg_type xt = q->column_types[xs[j]];
+        if (m == 1) {
+            t2.push_back(xt);
+            n2.push_back(!p1_drop && nullable((uint32_t)x) ? 1 : 0);
+        } else {
+            t2.push_back(xt == GBAGG_TYPE_BIGINT || xt == GBAGG_TYPE_DOUBLE ? GBAGG_TYPE_BIGINT : GBAGG_TYPE_INTEGER);
+            n2.push_back(1);
+        }
+    }
+    for (size_t i = 0; i < a1.size(); i++) {
+        const size_t oc = p1->tuple ? i + 1 : ng + 1 + i;  // phase 1's output column of partial state i
+        const bool cnt = p1->outs[oc].kind == OUT_COUNT;
+        const int32_t arg = a1[i].column;
+        t2.push_back(p1->outs[oc].type);
+        n2.push_back(m > 1 ? 1 : cnt ? 0 : (arg >= 0 && nullable((uint32_t)arg)) ? 1 : 0);
+    }
+    const uint32_t base = ng + m;  // first partial-state column
     std::vector<gbagg_agg> a2;
     std::vector<int> avg_hidden(q->num_aggs, -1);
     for (uint32_t a = 0; a < q->num_aggs; a++) {
         const gbagg_agg &g = q->aggs[a];
         const int32_t col = (int32_t)(base + src[a]);
         switch (g.kind) {
-        case GBAGG_AGG_COUNT_DISTINCT: a2.push_back({GBAGG_AGG_COUNT, (int32_t)ng}); break;
+        case GBAGG_AGG_COUNT_DISTINCT:
+            a2.push_back({GBAGG_AGG_COUNT,
+                          (int32_t)(ng + (std::find(xs.begin(), xs.end(), g.column) - xs.begin()))});
+            break;
         case GBAGG_AGG_MIN:
         case GBAGG_AGG_MAX: a2.push_back({g.kind, col}); break;
         default: a2.push_back({GBAGG_AGG_SUM, col}); break;  // COUNT(*), COUNT, SUM; AVG's sum
@@ -2361,29 +2411,36 @@ void unpack_tuples_device(gbagg_handle *h) {
     HIPCHK(hipStreamSynchronize(h->stream));  // the table of outputs is on the host stack
 }
 
-void run_phase1(gbagg_handle *h) {
-    gbagg_handle *p1 = h->phase1.get();
-    finalize(p1);
-    // phase 1's columns: its group columns (unpacked on the device when they
-    // form a multi-column key), then its partial states
+// Phase 1 result j as one phase-2 batch (see build_distinct): its group
+// columns (unpacked on the device when they form a multi-column key) and, for
+// j = 0, its partial states; phase-2 columns it does not fill are NULL.
+void add_phase1_batch(gbagg_handle *h, gbagg_handle *p1, uint32_t j, const void *zeros) {
+    const uint32_t ng = h->distinct_ng, m = h->distinct_args;
     const size_t G = p1->tuple ? p1->tcols.size() : 0;
-    const size_t nc = p1->tuple ? G + p1->outs.size() - 1 : p1->outs.size();
+    const size_t nc = p1->tuple ? G + p1->outs.size() - 1 : p1->outs.size();  // phase 1's columns
     if (p1->tuple) unpack_tuples_device(p1);
-    // phase 2's group columns are its first columns; a VARCHAR one gets
-    // phase 1's dictionary for it (one buffer pair per column)
     const size_t ng2 = h->tuple ? h->tcols.size() : h->group_cols.size();
-    h->p1_dict_off.assign(nc, {});
-    h->p1_dict_bytes.assign(nc, {});
-    std::vector<gbagg_column_view> v(nc);
-    for (size_t c = 0; c < nc; c++) {
+    const size_t n2 = h->tuple ? h->ext_cols : h->col_types.size();  // phase 2's columns
+    auto type2 = [&](size_t c) { return h->col_types[c]; };
+    std::vector<gbagg_column_view> v(n2);
+    for (size_t c = 0; c < n2; c++) {  // NULL unless filled below
         v[c] = gbagg_column_view{};
-        std::vector<uint32_t> &doff = h->p1_dict_off[c];
-        std::vector<char> &dbytes = h->p1_dict_bytes[c];
+        v[c].type = type2(c);
+        v[c].data = zeros;
+        v[c].validity = (const uint8_t *)zeros;
+    }
+    for (size_t c = 0; c < nc; c++) {
+        // phase 1 column c: a group column, x_j (c == ng), or partial state c - ng - 1
+        const size_t d = c < ng ? c : c == ng ? ng + j : c + (m - 1);
+        if (d >= n2) continue;
+        std::vector<uint32_t> &doff = h->p1_dict_off[(size_t)j * n2 + d];
+        std::vector<char> &dbytes = h->p1_dict_bytes[(size_t)j * n2 + d];
+        gbagg_column_view &w = v[d];
+        w.type = type2(d);
         if (c < G) {
-            v[c].type = p1->ttypes[c];
-            v[c].data = p1->tout_data[c].ptr;
-            v[c].validity = h->nullable[c] ? p1->tout_valid[c].as<uint8_t>() : nullptr;
-            if (v[c].type == GBAGG_TYPE_VARCHAR && c < ng2) {
+            w.data = p1->tout_data[c].ptr;
+            w.validity = h->nullable[d] ? p1->tout_valid[c].as<uint8_t>() : nullptr;
+            if (p1->ttypes[c] == GBAGG_TYPE_VARCHAR && d < ng2) {
                 // the column's interned codes: its interned dictionary, as a unique one
                 std::vector<StrCode> cd;
                 std::vector<uint32_t> words;
@@ -2395,20 +2452,19 @@ void run_phase1(gbagg_handle *h) {
                     doff.push_back((uint32_t)dbytes.size());
                 }
                 if (dbytes.empty()) dbytes.push_back(0);
-                v[c].dict_offsets = doff.data();
-                v[c].dict_bytes = dbytes.data();
-                v[c].dict_size = (uint32_t)cd.size();
-                v[c].flags = GBAGG_DICT_UNIQUE;
+                w.dict_offsets = doff.data();
+                w.dict_bytes = dbytes.data();
+                w.dict_size = (uint32_t)cd.size();
+                w.flags = m == 1 ? GBAGG_DICT_UNIQUE : 0;  // several phase 1 results: several dictionaries
             }
             continue;
         }
         const size_t ci = p1->tuple ? c - G + 1 : c;
-        v[c].type = p1->outs[ci].type;
-        v[c].data = p1->out_data[ci].ptr;
-        v[c].validity = h->nullable[c] ? p1->out_valid[ci].as<uint8_t>() : nullptr;
-        if (p1->outs[c].type == GBAGG_TYPE_VARCHAR && c < ng2) {
+        w.data = p1->out_data[ci].ptr;
+        w.validity = h->nullable[d] ? p1->out_valid[ci].as<uint8_t>() : nullptr;
+        if (p1->outs[ci].type == GBAGG_TYPE_VARCHAR && d < ng2) {
             // phase 1's canonical ids index its key dictionary: pass it on as a unique one
-            KeyDict &kd = p1->kdict[c];
+            KeyDict &kd = p1->kdict[ci];
             host_dict(p1, kd);
             const uint32_t n = kd.mode == 1 ? kd.size : (uint32_t)kd.strings.size();
             doff.assign(1, 0);
@@ -2418,16 +2474,40 @@ void run_phase1(gbagg_handle *h) {
                 doff.push_back((uint32_t)dbytes.size());
             }
             if (dbytes.empty()) dbytes.push_back(0);
-            v[c].dict_offsets = doff.data();
-            v[c].dict_bytes = dbytes.data();
-            v[c].dict_size = n;
-            v[c].flags = GBAGG_DICT_UNIQUE;
+            w.dict_offsets = doff.data();
+            w.dict_bytes = dbytes.data();
+            w.dict_size = n;
+            w.flags = m == 1 ? GBAGG_DICT_UNIQUE : 0;
         }
+    }
+    if (p1->result_rows) add_batch(h, v.data(), p1->result_rows, true);
+}
+
+// COUNT(DISTINCT): the phase 1 handles aggregate the consumed rows; their
+// device-resident results become this handle's input batches.
+void run_phase1(gbagg_handle *h) {
+    std::vector<gbagg_handle *> p1s{h->phase1.get()};
+    for (auto &p : h->phase1_more) p1s.push_back(p.get());
+    uint64_t rows = 1;
+    for (gbagg_handle *p1 : p1s) {
+        finalize(p1);
+        rows = std::max<uint64_t>(rows, p1->result_rows);
+    }
+    const size_t n2 = h->tuple ? h->ext_cols : h->col_types.size();
+    h->p1_dict_off.assign(p1s.size() * n2, {});
+    h->p1_dict_bytes.assign(p1s.size() * n2, {});
+    ensure_runtime(h);
+    const void *zeros = nullptr;
+    if (p1s.size() > 1) {  // NULL columns: zero validity bytes, zero data
+        h->p2_null.ensure(rows * 8);
+        HIPCHK(hipMemsetAsync(h->p2_null.ptr, 0, rows * 8, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+        zeros = h->p2_null.ptr;
     }
     h->batches.clear();
     h->rows_total = 0;
     for (auto &kd : h->kdict) kd = KeyDict();
-    if (p1->result_rows) add_batch(h, v.data(), p1->result_rows, true);
+    for (uint32_t j = 0; j < p1s.size(); j++) add_phase1_batch(h, p1s[j], j, zeros);
 }
 
 void finalize_once(gbagg_handle *h) {
@@ -2830,7 +2910,11 @@ gbagg_status gbagg_plan_create(const gbagg_query_desc *query, gbagg_handle **out
 gbagg_status gbagg_consume(gbagg_handle *h, const gbagg_column_view *columns, uint64_t num_rows) {
     return guarded([&] {
         if (!h) fail(GBAGG_INVALID, "null handle");
-        add_batch(h->phase1 ? h->phase1.get() : h, columns, num_rows, false);
+        if (!h->phase1) add_batch(h, columns, num_rows, false);
+        else {
+            add_batch(h->phase1.get(), columns, num_rows, false);
+            for (auto &p : h->phase1_more) add_batch(p.get(), columns, num_rows, false);
+        }
     });
 }
 
@@ -2838,7 +2922,11 @@ gbagg_status gbagg_consume_device(gbagg_handle *h, const gbagg_column_view *colu
                                   uint64_t num_rows) {
     return guarded([&] {
         if (!h) fail(GBAGG_INVALID, "null handle");
-        add_batch(h->phase1 ? h->phase1.get() : h, columns, num_rows, true);
+        if (!h->phase1) add_batch(h, columns, num_rows, true);
+        else {
+            add_batch(h->phase1.get(), columns, num_rows, true);
+            for (auto &p : h->phase1_more) add_batch(p.get(), columns, num_rows, true);
+        }
     });
 }
 
@@ -2869,6 +2957,7 @@ void reset_handle(gbagg_handle *h) {
     for (auto &kd : h->kdict) kd = KeyDict();
     std::fill(h->str_mode.begin(), h->str_mode.end(), 0);
     if (h->phase1) reset_handle(h->phase1.get());
+    for (auto &p : h->phase1_more) reset_handle(p.get());
 }
 
 gbagg_status gbagg_reset(gbagg_handle *h) {

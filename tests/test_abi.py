@@ -77,6 +77,11 @@ def test_valid_plans_create_without_device():
     assert _plan(G.Query([G.INTEGER, G.VARCHAR], [0], [(G.MIN, 1), (G.MAX, 1)]))[0] == G.OK
     # a nullable argument next to other aggregates (its NULL rows form a phase-1 group)
     assert _plan(G.Query([G.INTEGER] * 3, [0], [(G.COUNT_DISTINCT, 1), (G.SUM, 2)], nullable=[0, 1, 0]))[0] == G.OK
+    # COUNT(DISTINCT) over several argument columns, grouped, by two columns and scalar
+    for g in ([0], [0, 1], []):
+        assert _plan(G.Query([G.INTEGER, G.VARCHAR, G.BIGINT, G.DOUBLE], g,
+                             [(G.COUNT_DISTINCT, 2), (G.COUNT_DISTINCT, 3), (G.SUM, 2)],
+                             nullable=[0, 0, 0, 1]))[0] == G.OK
     # 16-byte keys: two BIGINT columns, BIGINT with a VARCHAR or INTEGER column
     for types in ([G.BIGINT, G.BIGINT], [G.VARCHAR, G.BIGINT], [G.BIGINT, G.INTEGER]):
         assert _plan(G.Query(types + [G.BIGINT], [0, 1], [(G.COUNT_STAR, None), (G.SUM, 2)],
@@ -93,7 +98,6 @@ def test_valid_plans_create_without_device():
     (G.Query([G.INTEGER], [0], [(G.SUM, 3)]), "out of range"),
     (G.Query([G.INTEGER] * 10, [0], [(G.SUM, c) for c in range(1, 10)]), "8 distinct"),
     (G.Query([G.INTEGER], [], []), "nothing"),
-    (G.Query([G.INTEGER] * 3, [0], [(G.COUNT_DISTINCT, 1), (G.COUNT_DISTINCT, 2)], nullable=[0, 0, 0]), "share one"),
     (G.Query([G.INTEGER] * 9, list(range(8)), [(G.COUNT_DISTINCT, 8)], nullable=[0] * 9), "at most 7 group"),
     (G.Query([G.INTEGER], [0], [(G.COUNT_STAR, None)], topn=(5, False, 3)), "topn_col"),
 ])

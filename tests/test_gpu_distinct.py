@@ -155,3 +155,72 @@ def test_double_argument_and_multi_column_keys(keys):
             res, _ = G.run(q, cols, device=True)
             exp, t = _exact(q, cols, dicts)
             assert_same_groups(_norm(res.rows(), len(keys)), _norm(exp, len(keys)), len(keys), t)
+
+
+# ---- several DISTINCT argument columns in one query (a distinct set per
+# aggregate, physical_planner.cpp:6076, 12325-12349) ----------------------
+
+def test_two_distinct_arguments_next_to_other_aggregates():
+    cols, words = _table(400_000, 11)
+    rng = np.random.default_rng(12)
+    xn = G.Column(G.BIGINT, cols[1].values, (rng.random(len(cols[1])) > 0.25).astype(np.uint8))
+    cols = cols + [xn]
+    types = [c.type for c in cols]
+    nullable = [False, False, False, False, True, True]
+    # phase 2 loads the key, the three arguments and four partial states
+    # (SUM(3), AVG(4) as SUM and COUNT, COUNT(*)): the 8-column plan limit
+    aggs = [(G.COUNT_DISTINCT, 1), (G.SUM, 3), (G.COUNT_DISTINCT, 2), (G.AVG, 4),
+            (G.COUNT_DISTINCT, 5), (G.COUNT_DISTINCT, 1), (G.COUNT_STAR, None)]
+    for k in ([0], [2], []):  # INTEGER key, VARCHAR key, scalar
+        q = G.Query(types, k, aggs, nullable=nullable)
+        _check(q, cols, {g: words for g, c in enumerate(k) if c == 2})
+    # two key columns (packed: both loaded) and three (an interned tuple code)
+    aggs2 = [(G.COUNT_DISTINCT, 1), (G.COUNT_DISTINCT, 5), (G.SUM, 3), (G.COUNT_STAR, None)]
+    for k in ([0, 2], [0, 2, 3]):
+        q = G.Query(types, k, aggs2, nullable=nullable)
+        _check(q, cols, {g: words for g, c in enumerate(k) if c == 2})
+
+
+def test_distinct_int_string_and_double_arguments_with_filter_and_nulls():
+    rng = np.random.default_rng(13)
+    n = 300_000
+    cols, words = _table(n, 14)
+    kv = (rng.random(n) > 0.05).astype(np.uint8)
+    cols[0] = G.Column(G.INTEGER, cols[0].values % 50, kv)
+    types = [c.type for c in cols]
+    nullable = [True, False, False, False, True]
+    aggs = [(G.COUNT_DISTINCT, 2), (G.COUNT_DISTINCT, 4), (G.COUNT_DISTINCT, 3), (G.SUM, 1)]
+    for drop in (False, True):
+        q = G.Query(types, [0], aggs, nullable=nullable, predicates=[(3, G.GT, -500)], drop_null_keys=drop)
+        _check(q, cols)
+
+
+def test_distinct_arguments_beyond_the_column_limit_are_rejected():
+    cols, words = _table(1000, 16)
+    types = [c.type for c in cols]
+    aggs = [(G.COUNT_DISTINCT, 1), (G.COUNT_DISTINCT, 2), (G.COUNT_DISTINCT, 4), (G.SUM, 3), (G.AVG, 4),
+            (G.MIN, 4), (G.MAX, 3)]
+    q = G.Query(types, [0], aggs, nullable=[False, False, False, False, True])
+    with pytest.raises(G.GbaggError, match="at most 8"):
+        G.run(q, cols)
+
+
+def test_two_distinct_arguments_topn_and_host_batches():
+    cols, words = _table(500_000, 15)
+    types = [c.type for c in cols]
+    aggs = [(G.COUNT_DISTINCT, 3), (G.COUNT_DISTINCT, 1), (G.COUNT_STAR, None)]
+    q = G.Query(types, [0], aggs, nullable=[False, False, False, False, True])
+    exp, etypes = _exact(q, cols)
+    qt = G.Query(types, [0], aggs, nullable=[False, False, False, False, True], topn=(2, False, 10))
+    res, info = G.run(qt, cols, device=True)
+    assert_topn_tie_aware(res.rows(), exp, order_col=2, k=10, types=etypes)
+    op = G.GpuHashAggregate(q)
+    op.Init()
+    RG = 122_880
+    for s in range(0, len(cols[0]), RG):
+        op.consume([G.Column(c.type, c.values[s:s + RG],
+                             None if c.validity is None else c.validity[s:s + RG], dictionary=c.dictionary)
+                    for c in cols])
+    op.finalize()
+    assert_same_groups(op.result().rows(), exp, 1, etypes)
+    op.close()

@@ -136,7 +136,8 @@ def test_random_topn_and_count_distinct():
             q.topn = (int(rng.choice(counts)), bool(rng.random() < 0.3), int(rng.choice([1, 5, 50])))
         else:
             # COUNT(DISTINCT x) of any type next to COUNT(*) and COUNT(x), by
-            # zero to three group columns of any type
+            # zero to three group columns of any type, half of them with a
+            # second COUNT(DISTINCT) argument
             x = int(rng.integers(0, len(types)))
             rest = [c for c in range(len(types)) if c != x]
             keys = [int(c) for c in rng.permutation(rest)[:int(rng.integers(0, min(3, len(rest)) + 1))]]
@@ -144,6 +145,9 @@ def test_random_topn_and_count_distinct():
             nk = len(keys)
             q.aggs = [(G.COUNT_DISTINCT, x), (G.COUNT_STAR, None), (G.COUNT, x)][:int(rng.integers(1, 4))]
             q.drop_null_keys = bool(rng.random() < 0.5)
+            others = [c for c in rest if c not in keys]
+            if others and rng.random() < 0.5:  # a second argument column: a distinct set of its own
+                q.aggs.append((G.COUNT_DISTINCT, int(rng.choice(others))))
         print(f"[random] case {case} seed {seed}: rows {len(cols[0])} types {types} groups {q.group_cols} "
               f"aggs {q.aggs} preds {q.predicates} drop {q.drop_null_keys} topn {q.topn} "
               f"batches {nbatch} device {device}", flush=True)
