@@ -49,8 +49,11 @@ typedef enum {
     GBAGG_INVALID = 2,
 } gbagg_status;
 
-/* Column types: same values as LogicalTypeId / slothdb_type
- * (include/slothdb/api/slothdb.h:31-39). VARCHAR columns arrive dictionary
+/* Column types: same values as slothdb_type (include/slothdb/api/slothdb.h:31-39).
+ * slothdb_type says it matches LogicalTypeId, but the engine's LogicalTypeId
+ * numbers FLOAT 12 and DOUBLE 13 (include/slothdb/common/types/logical_type.hpp),
+ * so engine-side callers map LogicalTypeId to these by name, not by value
+ * (slothdb_amd/host/engine/physical_gpu_hash_aggregate.hpp). VARCHAR columns arrive dictionary
  * encoded, as ParquetColumnData::str_dict_indices + str_dict_values
  * (include/slothdb/storage/parquet.hpp:166-168). */
 typedef enum {

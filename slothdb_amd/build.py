@@ -112,12 +112,21 @@ def build_oracle():
     return _run([make, "-s", "-B", "-C", os.path.join(ROOT, "oracle")])
 
 
+def build_engine_test():
+    """With the reference tree present: oracle/_ref/test_engine_operator, the
+    engine-side operator over the reference's own DataChunk / Vector / Value
+    sources, linked against the library built by build_gpu()."""
+    make = shutil.which("make") or "make"
+    return _run([make, "-s", "-C", os.path.join(ROOT, "oracle"), "engine"])
+
+
 def build_all():
     build_datagen()
     build_oracle()
     build_gpu()
     build_host()
     build_fixed_point_test()
+    build_engine_test()
 
 
 if __name__ == "__main__":
