@@ -505,6 +505,30 @@ gbagg_status gbagg_parquet_decode(gbagg_parquet *p, uint32_t row_group, const ui
  * by row group on its worker threads). */
 gbagg_status gbagg_parquet_decode_range(gbagg_parquet *p, uint32_t first, uint32_t last, const uint32_t *cols,
                                         uint32_t num_columns, gbagg_column_view *out, uint64_t *rows);
+/* The row-group consumer's form (RGConsumerFn, src/execution/
+ * physical_planner.cpp:1060-1062, called once per row group from
+ * RunParallelRGs, :1267-1310): plans row group `row_group` as
+ * gbagg_parquet_decode does and returns its views and row count at once, but
+ * queues its kernels instead of running them and waits for nothing. Queued
+ * row groups are launched together once `flush_rows` rows are queued
+ * (gbagg_parquet_set_flush_rows; 4M by default) and by gbagg_parquet_sync.
+ * The views' device buffers hold the values only after gbagg_parquet_sync
+ * returns GBAGG_OK: they may be passed to gbagg_consume_device before that
+ * (it reads no device data), but gbagg_parquet_sync must return before the
+ * operator's gbagg_finalize. Errors the host sees (unsupported encodings, a
+ * page outside its chunk) fail this call and leave the queue as it was;
+ * errors the kernels find (an index outside its dictionary, a truncated
+ * string) are reported by gbagg_parquet_sync. Calls on one handle must not
+ * overlap. gbagg_parquet_decode / _decode_range first run and wait for what
+ * is queued; gbagg_parquet_reset drops what is queued. */
+gbagg_status gbagg_parquet_decode_async(gbagg_parquet *p, uint32_t row_group, const uint32_t *cols,
+                                        uint32_t num_columns, gbagg_column_view *out, uint64_t *rows);
+/* Launches the queued row groups, waits for every decode of this handle, and
+ * returns the first error their kernels found (GBAGG_INVALID). */
+gbagg_status gbagg_parquet_sync(gbagg_parquet *p);
+/* Queued rows that trigger a launch (>= 1): fewer overlap decode with the
+ * caller's planning of later row groups, more launch fewer, larger kernels. */
+gbagg_status gbagg_parquet_set_flush_rows(gbagg_parquet *p, uint64_t rows);
 /* Forgets every decoded column (their views become invalid); later decodes
  * reuse the device memory. */
 gbagg_status gbagg_parquet_reset(gbagg_parquet *p);

@@ -45,12 +45,12 @@ namespace {
 // ---- compiled query shapes -------------------------------------------------
 // Queries whose shape matches one of these run straight-line specialisations
 // of the row kernels; every other query runs the same kernels over DynShape.
-using KAgg = void (*)(const DevQuery *, const DevBatch *, DevTable, uint32_t, uint32_t);
-using KCount = void (*)(const DevCtx *, const DevBatch *, uint32_t, uint64_t);
-using KScatter = void (*)(const DevCtx *, const DevBatch *, DevTable, uint32_t, uint64_t, uint32_t);
+using KAgg = void (*)(const DevQuery *, BatchRows, DevTable, uint32_t, uint32_t);
+using KCount = void (*)(const DevCtx *, BatchRows);
+using KScatter = void (*)(const DevCtx *, BatchRows, DevTable);
 using KPart = void (*)(const DevCtx *, DevTable, uint32_t, uint32_t);
-using KRange = void (*)(const DevQuery *, const DevBatch *, DevPlan *);
-using KDirect = void (*)(const DevQuery *, const DevBatch *, DevTable, int64_t, uint32_t, uint32_t);
+using KRange = void (*)(const DevQuery *, BatchRows, DevPlan *);
+using KDirect = void (*)(const DevQuery *, BatchRows, DevTable, int64_t, uint32_t, uint32_t);
 
 struct ShapeKernels {
     const char *name;
@@ -410,10 +410,42 @@ struct gbagg_handle {
     std::vector<gbagg_result_column> res_cols;
     bool result_ready = false;
 
+    // host batches (gbagg_consume): their bytes are gathered into pinned
+    // staging, two buffers that alternate (each reused once the event
+    // recorded after its copies has passed), and copied asynchronously into
+    // device memory bump-allocated from arenas kept until reset: a consume
+    // returns without waiting for its copies
+    uint8_t *hstage[2] = {nullptr, nullptr};
+    size_t hstage_bytes[2] = {0, 0};
+    hipEvent_t hstage_done[2] = {nullptr, nullptr};
+    int hstage_next = 0;
+    std::vector<DevBuf> barena;
+    size_t barena_i = 0, barena_used = 0;
+    uint8_t *batch_alloc(size_t n) {
+        n = (std::max<size_t>(n, 1) + 255) & ~(size_t)255;
+        while (barena_i < barena.size() && barena_used + n > barena[barena_i].bytes) {
+            barena_i++;
+            barena_used = 0;
+        }
+        if (barena_i == barena.size()) {
+            barena.emplace_back();
+            barena.back().ensure(std::max<size_t>(n, (size_t)64 << 20));
+            barena_used = 0;
+        }
+        uint8_t *p = barena[barena_i].as<uint8_t>() + barena_used;
+        barena_used += n;
+        return p;
+    }
+
     ~gbagg_handle() {
+        if (stream) (void)hipStreamSynchronize(stream);  // copies out of the staging buffers
         if (hplan) (void)hipHostFree(hplan);
         if (hctx) (void)hipHostFree(hctx);
         if (hbatches) (void)hipHostFree(hbatches);
+        for (int k = 0; k < 2; k++) {
+            if (hstage_done[k]) (void)hipEventDestroy(hstage_done[k]);
+            if (hstage[k]) (void)hipHostFree(hstage[k]);
+        }
         for (auto &e : ev)
             if (e) (void)hipEventDestroy(e);
         if (stream) (void)hipStreamDestroy(stream);
@@ -984,6 +1016,72 @@ bool like_contains(const char *s, uint32_t l, const std::string &nd) {
     return std::search(s, s + l, nd.begin(), nd.end()) != s + l;
 }
 
+// One batch's host-to-device copies: each piece gets its device memory when
+// it is registered; commit gathers the bytes into a pinned staging buffer and
+// copies them on the handle's stream, one copy when the pieces' device
+// memory is contiguous. Host data that would not outlive the call is kept.
+struct Stager {
+    gbagg_handle *h;
+    struct Piece {
+        const void *src;
+        size_t bytes;
+        uint8_t *dst;
+    };
+    std::vector<Piece> pieces;
+    std::vector<std::vector<uint8_t>> keep;
+    void *to_dev(const void *src, size_t bytes, size_t pad = 0) {
+        uint8_t *d = h->batch_alloc(bytes + pad);
+        if (bytes) pieces.push_back(Piece{src, bytes, d});
+        return d;
+    }
+    void *to_dev_kept(const void *src, size_t bytes, size_t pad = 0) {
+        keep.emplace_back((const uint8_t *)src, (const uint8_t *)src + bytes);
+        return to_dev(keep.back().data(), bytes, pad);
+    }
+    void commit() {
+        if (pieces.empty()) return;
+        size_t sum = 0;
+        bool one = true;
+        for (size_t i = 0; i < pieces.size(); i++) {
+            sum += pieces[i].bytes;
+            if (i && pieces[i].dst < pieces[i - 1].dst + pieces[i - 1].bytes) one = false;
+        }
+        const size_t span = (size_t)(pieces.back().dst + pieces.back().bytes - pieces.front().dst);
+        if (one && span > sum + pieces.size() * 256) one = false;  // the arena moved on mid-batch
+        size_t need = 0;
+        if (one) {
+            need = span;
+        } else {
+            for (const Piece &pc : pieces) need += (pc.bytes + 15) & ~(size_t)15;
+        }
+        const int k = h->hstage_next;
+        h->hstage_next ^= 1;
+        if (h->hstage_done[k]) HIPCHK(hipEventSynchronize(h->hstage_done[k]));  // its copies have read it
+        else HIPCHK(hipEventCreateWithFlags(&h->hstage_done[k], hipEventDisableTiming));
+        if (h->hstage_bytes[k] < need) {
+            if (h->hstage[k]) HIPCHK(hipHostFree(h->hstage[k]));
+            h->hstage[k] = nullptr;
+            h->hstage_bytes[k] = 0;
+            const size_t cap = std::max<size_t>(need, (size_t)4 << 20);
+            HIPCHK(hipHostMalloc((void **)&h->hstage[k], cap, hipHostMallocDefault));
+            h->hstage_bytes[k] = cap;
+        }
+        uint8_t *st = h->hstage[k];
+        if (one) {
+            for (const Piece &pc : pieces) std::memcpy(st + (pc.dst - pieces.front().dst), pc.src, pc.bytes);
+            HIPCHK(hipMemcpyAsync(pieces.front().dst, st, span, hipMemcpyHostToDevice, h->stream));
+        } else {
+            size_t off = 0;
+            for (const Piece &pc : pieces) {
+                std::memcpy(st + off, pc.src, pc.bytes);
+                HIPCHK(hipMemcpyAsync(pc.dst, st + off, pc.bytes, hipMemcpyHostToDevice, h->stream));
+                off += (pc.bytes + 15) & ~(size_t)15;
+            }
+        }
+        HIPCHK(hipEventRecord(h->hstage_done[k], h->stream));
+    }
+};
+
 // Registers one batch; `device` tells whether data/validity are device pointers.
 void add_batch(gbagg_handle *h, const gbagg_column_view *cols, uint64_t n, bool device) {
     if (h->finalized) fail(GBAGG_INVALID, "consume after finalize: call gbagg_reset first");
@@ -992,6 +1090,7 @@ void add_batch(gbagg_handle *h, const gbagg_column_view *cols, uint64_t n, bool 
     Batch b;
     b.nrows = n;
     b.db.nrows = n;
+    Stager st{h};
     if (h->str_mode.size() != h->col_types.size()) h->str_mode.assign(h->col_types.size(), 0);
     std::vector<uint8_t> modes(h->str_mode);
     for (int u = 0; u < h->nu; u++) {
@@ -1033,17 +1132,8 @@ void add_batch(gbagg_handle *h, const gbagg_column_view *cols, uint64_t n, bool 
                 for (uint64_t r = 0; r < n; r++)
                     if (off[r + 1] < off[r]) fail(GBAGG_INVALID, "VARCHAR string offsets must be ascending");
                 if (off[n] && !c.dict_bytes) fail(GBAGG_INVALID, "null string bytes");
-                auto ob = std::make_unique<DevBuf>();
-                ob->ensure((n + 1) * 4);
-                HIPCHK(hipMemcpyAsync(ob->ptr, off, (n + 1) * 4, hipMemcpyHostToDevice, h->stream));
-                auto bb = std::make_unique<DevBuf>();
-                bb->ensure((size_t)off[n] + 4);
-                if (off[n])
-                    HIPCHK(hipMemcpyAsync(bb->ptr, c.dict_bytes, off[n], hipMemcpyHostToDevice, h->stream));
-                b.str_off[u] = ob->as<uint32_t>();
-                b.str_bytes[u] = bb->as<uint8_t>();
-                b.owned.push_back(std::move(ob));
-                b.owned.push_back(std::move(bb));
+                b.str_off[u] = (const uint32_t *)st.to_dev(off, (n + 1) * 4);
+                b.str_bytes[u] = (const uint8_t *)st.to_dev(c.dict_bytes, off[n], 4);
             }
             b.db.data[u] = nullptr;
             if (device || n == 0) b.db.valid[u] = c.validity;
@@ -1051,30 +1141,15 @@ void add_batch(gbagg_handle *h, const gbagg_column_view *cols, uint64_t n, bool 
             b.db.data[u] = c.data;
             b.db.valid[u] = c.validity;
         } else {
-            auto buf = std::make_unique<DevBuf>();
-            buf->ensure(n * w);
-            HIPCHK(hipMemcpyAsync(buf->ptr, c.data, n * w, hipMemcpyHostToDevice, h->stream));
-            b.db.data[u] = buf->ptr;
-            b.owned.push_back(std::move(buf));
+            b.db.data[u] = st.to_dev(c.data, n * w);
         }
-        if (!device && n && c.validity) {
-            auto vb = std::make_unique<DevBuf>();
-            vb->ensure(n);
-            HIPCHK(hipMemcpyAsync(vb->ptr, c.validity, n, hipMemcpyHostToDevice, h->stream));
-            b.db.valid[u] = vb->as<uint8_t>();
-            b.owned.push_back(std::move(vb));
-        }
+        if (!device && n && c.validity) b.db.valid[u] = (const uint8_t *)st.to_dev(c.validity, n);
     }
     // multi-column keys: every group column of the batch on the device
     if (h->tuple) {
         auto to_dev = [&](const void *p, size_t bytes) -> const void * {
             if (device || !n || !p) return p;
-            auto buf = std::make_unique<DevBuf>();
-            buf->ensure(bytes + 4);
-            HIPCHK(hipMemcpyAsync(buf->ptr, p, bytes, hipMemcpyHostToDevice, h->stream));
-            const void *d = buf->ptr;
-            b.owned.push_back(std::move(buf));
-            return d;
+            return st.to_dev(p, bytes, 4);
         };
         b.tin.resize(h->tcols.size());
         for (size_t j = 0; j < h->tcols.size(); j++) {
@@ -1149,13 +1224,7 @@ void add_batch(gbagg_handle *h, const gbagg_column_view *cols, uint64_t n, bool 
                     remap[d] = it->second;
                 }
             }
-            auto buf = std::make_unique<DevBuf>();
-            buf->ensure(std::max<size_t>(4, remap.size() * 4));
-            if (!remap.empty())
-                HIPCHK(hipMemcpyAsync(buf->ptr, remap.data(), remap.size() * 4, hipMemcpyHostToDevice, h->stream));
-            b.db.remap[g] = buf->as<uint32_t>();
-            b.owned.push_back(std::move(buf));
-            HIPCHK(hipStreamSynchronize(h->stream));  // remap vector is on the host stack
+            b.db.remap[g] = (const uint32_t *)st.to_dev_kept(remap.data(), remap.size() * 4, 4);
         }
     }
     // predicates: IN sets and per-dictionary-entry VARCHAR matches
@@ -1183,16 +1252,13 @@ void add_batch(gbagg_handle *h, const gbagg_column_view *cols, uint64_t n, bool 
                 }
                 m[d] = r;
             }
-            auto buf = std::make_unique<DevBuf>();
-            buf->ensure(m.size());
-            HIPCHK(hipMemcpyAsync(buf->ptr, m.data(), m.size(), hipMemcpyHostToDevice, h->stream));
-            HIPCHK(hipStreamSynchronize(h->stream));
-            b.db.pred_ptr[i] = buf->ptr;
+            b.db.pred_ptr[i] = st.to_dev_kept(m.data(), m.size());
             b.db.pred_count[i] = c.dict_size;
-            b.owned.push_back(std::move(buf));
         }
     }
-    if (!device) HIPCHK(hipStreamSynchronize(h->stream));  // host buffers valid only during the call
+    // the caller's buffers are valid only during the call: gathered into
+    // pinned staging here, copied to the device without a wait
+    st.commit();
     h->str_mode = std::move(modes);
     h->rows_total += n;
     h->batches.push_back(std::move(b));
@@ -1308,18 +1374,49 @@ void upload_ctx(gbagg_handle *h) {
     HIPCHK(hipMemcpyAsync(h->ctx.ptr, h->hctx, sizeof(DevCtx), hipMemcpyHostToDevice, h->stream));
 }
 
+// The batches' DevBatch blocks, then their first rows in the concatenation
+// (nb + 1 words, the last one the total), in one copy.
 void upload_batches(gbagg_handle *h) {
-    size_t nb = std::max<size_t>(1, h->batches.size());
-    if (h->hbatch_cap < nb) {
+    const size_t nb = h->batches.size();
+    const size_t slots = std::max<size_t>(1, nb);
+    const size_t bytes = slots * sizeof(DevBatch) + (nb + 1) * 8;
+    if (h->hbatch_cap < bytes) {
         if (h->hbatches) HIPCHK(hipHostFree(h->hbatches));
         h->hbatches = nullptr;
-        HIPCHK(hipHostMalloc((void **)&h->hbatches, nb * sizeof(DevBatch), hipHostMallocDefault));
-        h->hbatch_cap = nb;
+        HIPCHK(hipHostMalloc((void **)&h->hbatches, bytes, hipHostMallocDefault));
+        h->hbatch_cap = bytes;
     }
-    for (size_t i = 0; i < h->batches.size(); i++) h->hbatches[i] = h->batches[i].db;
-    h->dbatches.ensure(nb * sizeof(DevBatch));
-    HIPCHK(hipMemcpyAsync(h->dbatches.ptr, h->hbatches, h->batches.size() * sizeof(DevBatch),
-                          hipMemcpyHostToDevice, h->stream));
+    uint64_t *start = (uint64_t *)(h->hbatches + slots);
+    uint64_t r = 0;
+    for (size_t i = 0; i < nb; i++) {
+        h->hbatches[i] = h->batches[i].db;
+        start[i] = r;
+        r += h->batches[i].nrows;
+    }
+    start[nb] = r;
+    h->dbatches.ensure(bytes);
+    HIPCHK(hipMemcpyAsync(h->dbatches.ptr, h->hbatches, bytes, hipMemcpyHostToDevice, h->stream));
+}
+
+// Launch rows over every batch: workgroup w takes `per` rows of the batches'
+// concatenation (after upload_batches).
+BatchRows batch_rows(gbagg_handle *h, uint64_t per) {
+    const size_t nb = h->batches.size();
+    BatchRows br{};
+    br.b = h->dbatches.as<DevBatch>();
+    br.start = (const uint64_t *)(h->dbatches.as<DevBatch>() + std::max<size_t>(1, nb));
+    br.nb = (uint32_t)nb;
+    br.per = std::max<uint64_t>(per, 1);
+    return br;
+}
+
+// Workgroups of `unit`-row tiles over all rows: at most `max_wg` of them,
+// each a whole number of tiles; per = rows per workgroup.
+uint32_t tiled_grid(uint64_t rows, uint64_t unit, uint64_t max_wg, uint64_t &per) {
+    const uint64_t tiles = std::max<uint64_t>(1, (rows + unit - 1) / unit);
+    const uint64_t g = std::max<uint64_t>(1, std::min<uint64_t>(tiles, max_wg));
+    per = (tiles + g - 1) / g * unit;
+    return (uint32_t)std::max<uint64_t>(1, (rows + per - 1) / per);
 }
 
 // Resets the run counters of the device plan block; the sample / heavy-hitter
@@ -1444,14 +1541,11 @@ void run_sample(gbagg_handle *h) {
     rp.hfinal = h->hfinal.as<uint64_t>();
     upload_ctx(h);
     hipLaunchKernelGGL(k_sample_init, dim3(256), dim3(256), 0, s, d_rp(h));
-    for (size_t i = 0; i < h->batches.size(); i++) {
-        auto &b = h->batches[i];
-        if (!b.nrows) continue;
-        uint64_t want = std::max<uint64_t>(1, (uint64_t)((double)S_TOTAL * b.nrows / std::max<uint64_t>(1, h->rows_total)));
-        want = std::min<uint64_t>(want, b.nrows);
-        uint64_t stride = std::max<uint64_t>(1, b.nrows / want);
-        uint32_t grid = (uint32_t)std::min<uint64_t>((want + 255) / 256, (uint64_t)di.cus * 2);
-        hipLaunchKernelGGL(k_sample, dim3(grid), dim3(256), 0, s, (const DevCtx *)dctx(h), d_b(h, i),
+    if (h->rows_total) {  // evenly strided over the batches' concatenation
+        const uint64_t want = std::min<uint64_t>(S_TOTAL, h->rows_total);
+        const uint64_t stride = std::max<uint64_t>(1, h->rows_total / want);
+        const uint32_t grid = (uint32_t)std::min<uint64_t>((want + 255) / 256, (uint64_t)di.cus * 2);
+        hipLaunchKernelGGL(k_sample, dim3(grid), dim3(256), 0, s, (const DevCtx *)dctx(h), batch_rows(h, 1),
                            (uint32_t)want, stride, h->dplan.as<DevPlan>());
     }
     h->ghist.ensure(65 * 4);
@@ -1520,11 +1614,11 @@ void fx_prepare(gbagg_handle *h, bool sampled) {
     for (const auto &pr : h->fx_pairs)
         if (sampled && h->hplan->fx_exp[h->dq.acc_u[pr.first]] == 0) exact = true;
     if (exact && !h->fx_known) {
-        for (size_t i = 0; i < h->batches.size(); i++) {
-            const auto &b = h->batches[i];
-            if (!b.nrows) continue;
-            const uint32_t g = (uint32_t)std::min<uint64_t>((b.nrows + 255) / 256, (uint64_t)di.cus * 8);
-            hipLaunchKernelGGL(k_fx_range, dim3(g), dim3(256), 0, h->stream, d_q(h), d_b(h, i), h->dplan.as<DevPlan>());
+        if (h->rows_total) {
+            uint64_t per;
+            const uint32_t g = tiled_grid(h->rows_total, 256, (uint64_t)di.cus * 8, per);
+            hipLaunchKernelGGL(k_fx_range, dim3(g), dim3(256), 0, h->stream, d_q(h), batch_rows(h, per),
+                               h->dplan.as<DevPlan>());
         }
         read_plan(h);
         std::memcpy(h->fx_exp, h->hplan->fx_exp, sizeof(h->fx_exp));
@@ -1589,22 +1683,22 @@ void run_lds_hash(gbagg_handle *h, Timer &tm, bool global_only, uint64_t est = 0
             allow_lds(h->kern->agg, lds_bytes);
         }
         h->lds_slots = S;
-        for (size_t i = 0; i < h->batches.size(); i++) {
-            auto &b = h->batches[i];
-            if (!b.nrows) continue;
-            uint64_t tiles = (b.nrows + (uint64_t)AGG_BLOCK * AGG_ROWS - 1) / ((uint64_t)AGG_BLOCK * AGG_ROWS);
+        if (h->rows_total) {  // one launch over every batch
+            const uint64_t n = h->rows_total;
+            const uint64_t tile = (uint64_t)AGG_BLOCK * AGG_ROWS;
+            uint64_t per;
             if (!global_only) {
                 // few workgroups for small inputs: each flushes its LDS table
                 uint32_t per_cu = (uint32_t)std::max<size_t>(1, std::min<size_t>(4, di.lds_per_cu / lds_bytes));
-                uint64_t by_rows = std::max<uint64_t>(1, b.nrows / 16384);
-                uint32_t g = (uint32_t)std::min<uint64_t>({tiles, (uint64_t)di.cus * per_cu, by_rows});
+                uint64_t by_rows = std::max<uint64_t>(1, n / 16384);
+                uint32_t g = tiled_grid(n, tile, std::min<uint64_t>((uint64_t)di.cus * per_cu, by_rows), per);
                 h->agg_grid = g;
-                hipLaunchKernelGGL(h->kern->agg, dim3(g), dim3(AGG_BLOCK), lds_bytes, s, d_q(h), d_b(h, i), T, S,
-                                   S * 3 / 4);
+                hipLaunchKernelGGL(h->kern->agg, dim3(g), dim3(AGG_BLOCK), lds_bytes, s, d_q(h), batch_rows(h, per), T,
+                                   S, S * 3 / 4);
             } else {
-                uint32_t g = (uint32_t)std::min<uint64_t>(tiles, (uint64_t)di.cus * 8);
+                uint32_t g = tiled_grid(n, tile, (uint64_t)di.cus * 8, per);
                 h->agg_grid = g;
-                hipLaunchKernelGGL(k_aggregate_global, dim3(g), dim3(AGG_BLOCK), 0, s, d_q(h), d_b(h, i), T);
+                hipLaunchKernelGGL(k_aggregate_global, dim3(g), dim3(AGG_BLOCK), 0, s, d_q(h), batch_rows(h, per), T);
             }
         }
         HIPCHK(hipGetLastError());
@@ -1644,27 +1738,16 @@ bool run_radix(gbagg_handle *h, Timer &tm, uint64_t &est_out) {
     const uint32_t per_cu3 = (uint32_t)std::max<size_t>(1, std::min<size_t>(4, di.lds_per_cu / k3_lds_max));
     const uint32_t per_cu1 = (uint32_t)std::max<size_t>(1, std::min<size_t>(4, di.lds_per_cu / k1_lds));
     const uint32_t g1 = std::max<uint32_t>(1, per_cu1 / per_cu3);
-    std::vector<uint32_t> wg_off, wg_n, wg3_off, wg3_n;
-    std::vector<uint64_t> rows_per;
-    uint32_t W = 0, W3 = 0;
-    for (auto &b : h->batches) {
-        const uint64_t tile = (uint64_t)RP_BLOCK * AGG_ROWS * 2 * g1;
-        uint64_t tiles = std::max<uint64_t>(1, (b.nrows + tile - 1) / tile);
-        uint32_t g = (uint32_t)std::min<uint64_t>(tiles, (uint64_t)di.cus * per_cu3);
-        uint64_t per3 = ((b.nrows + g - 1) / g + tile - 1) / tile * tile;
-        const uint64_t per1 = per3 / g1;
-        const uint32_t n1 = b.nrows ? (uint32_t)((b.nrows + per1 - 1) / per1) : 0;
-        const uint32_t n3 = (n1 + g1 - 1) / g1;
-        wg_off.push_back(W);
-        wg_n.push_back(n1);
-        wg3_off.push_back(W3);
-        wg3_n.push_back(n3);
-        rows_per.push_back(per1);
-        W += n1;
-        W3 += n3;
-    }
-    W = std::max<uint32_t>(W, 1);
-    W3 = std::max<uint32_t>(W3, 1);
+    // one launch per pass over every batch: scatter workgroup k takes rows
+    // [k * per3, (k + 1) * per3) of the batches' concatenation, count
+    // workgroup w rows [w * per1, (w + 1) * per1), per3 = g1 * per1
+    const uint64_t tile = (uint64_t)RP_BLOCK * AGG_ROWS * 2 * g1;
+    uint64_t per3;
+    (void)tiled_grid(h->rows_total, tile, (uint64_t)di.cus * per_cu3, per3);
+    const uint64_t per1 = per3 / g1;
+    const uint32_t W1 = h->rows_total ? (uint32_t)((h->rows_total + per1 - 1) / per1) : 0;
+    const uint32_t W = std::max<uint32_t>(W1, 1);
+    const uint32_t W3 = std::max<uint32_t>((W1 + g1 - 1) / g1, 1);
     rp.W = W;
     rp.W3 = W3;
     rp.g1 = g1;
@@ -1687,11 +1770,9 @@ bool run_radix(gbagg_handle *h, Timer &tm, uint64_t &est_out) {
         hipLaunchKernelGGL(k_table_init, dim3(grid), dim3(256), 0, s, T, d_q(h));
         tm.mark();  // count start
         allow_lds(h->kern->count, k1_lds);
-        for (size_t i = 0; i < h->batches.size(); i++) {
-            if (!h->batches[i].nrows) continue;
-            hipLaunchKernelGGL(h->kern->count, dim3(wg_n[i]), dim3(RP_BLOCK), k1_lds, s, (const DevCtx *)dctx(h),
-                               d_b(h, i), wg_off[i], rows_per[i]);
-        }
+        if (W1)
+            hipLaunchKernelGGL(h->kern->count, dim3(W1), dim3(RP_BLOCK), k1_lds, s, (const DevCtx *)dctx(h),
+                               batch_rows(h, per1));
         const uint32_t hslices = std::min<uint32_t>(32, W);
         h->hllm.ensure((size_t)HLL_M * hslices);
         hipLaunchKernelGGL(k_hll_merge, dim3(HLL_M / 256, hslices), dim3(256), 0, s, (const uint8_t *)rp.hll, W,
@@ -1745,11 +1826,9 @@ bool run_radix(gbagg_handle *h, Timer &tm, uint64_t &est_out) {
         upload_ctx(h);
         allow_lds(h->kern->scatter, k3_lds);
         tm.mark();  // scatter start
-        for (size_t i = 0; i < h->batches.size(); i++) {
-            if (!h->batches[i].nrows) continue;
-            hipLaunchKernelGGL(h->kern->scatter, dim3(wg3_n[i]), dim3(RP_BLOCK), k3_lds, s, (const DevCtx *)dctx(h),
-                               d_b(h, i), T, wg3_off[i], rows_per[i] * g1, wg_off[i]);
-        }
+        if (W1)
+            hipLaunchKernelGGL(h->kern->scatter, dim3(W3), dim3(RP_BLOCK), k3_lds, s, (const DevCtx *)dctx(h),
+                               batch_rows(h, per3), T);
         h->agg_grid = W3;
         tm.mark();  // scatter end
         const size_t k4_lds = lds_table_bytes(S, nacc);
@@ -1807,12 +1886,11 @@ bool direct_range(gbagg_handle *h, uint64_t &span, int64_t &kmin) {
     const DeviceInfo &di = device_info();
     hipStream_t s = h->stream;
     hipLaunchKernelGGL(k_key_range_init, dim3(1), dim3(1), 0, s, h->dplan.as<DevPlan>());
-    for (size_t i = 0; i < h->batches.size(); i++) {
-        const auto &b = h->batches[i];
-        if (!b.nrows) continue;
-        const uint64_t tiles = (b.nrows + (uint64_t)AGG_BLOCK * AGG_ROWS - 1) / ((uint64_t)AGG_BLOCK * AGG_ROWS);
-        const uint32_t g = (uint32_t)std::min<uint64_t>(tiles, (uint64_t)di.cus * 4);
-        hipLaunchKernelGGL(h->kern->range, dim3(g), dim3(AGG_BLOCK), 0, s, d_q(h), d_b(h, i), h->dplan.as<DevPlan>());
+    if (h->rows_total) {
+        uint64_t per;
+        const uint32_t g = tiled_grid(h->rows_total, (uint64_t)AGG_BLOCK * AGG_ROWS, (uint64_t)di.cus * 4, per);
+        hipLaunchKernelGGL(h->kern->range, dim3(g), dim3(AGG_BLOCK), 0, s, d_q(h), batch_rows(h, per),
+                           h->dplan.as<DevPlan>());
     }
     HIPCHK(hipGetLastError());
     read_plan(h);
@@ -1848,25 +1926,25 @@ void run_direct(gbagg_handle *h, Timer &tm, uint64_t span, int64_t kmin) {
     if (lds) allow_lds(h->kern->direct_lds, lds_bytes);
     else allow_lds(h->kern->direct_glb, cache_bytes);
     h->lds_slots = lds ? (uint32_t)span : 0;
-    for (size_t i = 0; i < h->batches.size(); i++) {
-        const auto &b = h->batches[i];
-        if (!b.nrows) continue;
-        const uint64_t tiles = (b.nrows + (uint64_t)AGG_BLOCK * AGG_ROWS - 1) / ((uint64_t)AGG_BLOCK * AGG_ROWS);
+    if (h->rows_total) {  // one launch over every batch
+        const uint64_t n = h->rows_total;
+        const uint64_t tile = (uint64_t)AGG_BLOCK * AGG_ROWS;
+        uint64_t per;
         uint32_t g;
         if (lds) {
             // each workgroup folds its whole table at the end: few of them for small inputs
             const uint32_t per_cu = (uint32_t)std::max<size_t>(1, std::min<size_t>(4, di.lds_per_cu / std::max<size_t>(lds_bytes, 1)));
-            const uint64_t by_rows = std::max<uint64_t>(1, b.nrows / std::max<uint64_t>(16384, span * 4));
-            g = (uint32_t)std::min<uint64_t>({tiles, (uint64_t)di.cus * per_cu, by_rows});
-            hipLaunchKernelGGL(h->kern->direct_lds, dim3(g), dim3(AGG_BLOCK), lds_bytes, s, d_q(h), d_b(h, i), T, kmin,
-                               (uint32_t)span, 0u);
+            const uint64_t by_rows = std::max<uint64_t>(1, n / std::max<uint64_t>(16384, span * 4));
+            g = tiled_grid(n, tile, std::min<uint64_t>((uint64_t)di.cus * per_cu, by_rows), per);
+            hipLaunchKernelGGL(h->kern->direct_lds, dim3(g), dim3(AGG_BLOCK), lds_bytes, s, d_q(h), batch_rows(h, per),
+                               T, kmin, (uint32_t)span, 0u);
         } else {
             // an LDS hash cache for the frequent keys in front of the dense records
             const uint32_t per_cu = (uint32_t)std::max<size_t>(1, std::min<size_t>(4, di.lds_per_cu / cache_bytes));
-            const uint64_t by_rows = std::max<uint64_t>(1, b.nrows / 16384);
-            g = (uint32_t)std::min<uint64_t>({tiles, (uint64_t)di.cus * per_cu, by_rows});
-            hipLaunchKernelGGL(h->kern->direct_glb, dim3(g), dim3(AGG_BLOCK), cache_bytes, s, d_q(h), d_b(h, i), T,
-                               kmin, (uint32_t)span, cache);
+            const uint64_t by_rows = std::max<uint64_t>(1, n / 16384);
+            g = tiled_grid(n, tile, std::min<uint64_t>((uint64_t)di.cus * per_cu, by_rows), per);
+            hipLaunchKernelGGL(h->kern->direct_glb, dim3(g), dim3(AGG_BLOCK), cache_bytes, s, d_q(h),
+                               batch_rows(h, per), T, kmin, (uint32_t)span, cache);
         }
         h->agg_grid = g;
     }
@@ -2950,6 +3028,8 @@ gbagg_status gbagg_result(gbagg_handle *h, gbagg_result_view *out) {
 void reset_handle(gbagg_handle *h) {
     if (h->stream) HIPCHK(hipStreamSynchronize(h->stream));
     h->batches.clear();
+    h->barena_i = 0;  // batch copies: the stream has drained
+    h->barena_used = 0;
     h->rows_total = 0;
     h->finalized = false;
     h->result_ready = false;
@@ -3636,6 +3716,89 @@ void gbagg_batch_destroy(gbagg_batch *b) { delete b; }
 // =============================================================================
 #include "gbagg_parquet.hpp"
 
+namespace {
+
+void parquet_stream(gbagg_parquet *p) {
+    if (!p->stream) {
+        (void)device_info();
+        HIPCHK(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
+    }
+    if (!p->hbad) HIPCHK(hipHostMalloc((void **)&p->hbad, 4, hipHostMallocDefault));
+}
+
+// Clears the error bits before the first launch since the last wait.
+void parquet_arm(gbagg_parquet *p) {
+    if (p->armed) return;
+    p->bad.ensure(4);
+    HIPCHK(hipMemsetAsync(p->bad.ptr, 0, 4, p->stream));
+    p->armed = true;
+}
+
+// Device addresses of column c's chunks of row groups [g0, g1): in place in
+// the resident image, else copied (16 readable bytes past each) into `own`
+// (a per-call buffer) or, when own is null, into arena memory kept until reset.
+std::vector<const uint8_t *> parquet_chunks(gbagg_parquet *p, uint32_t g0, uint32_t g1, const uint32_t *cols,
+                                            uint32_t num_cols, DevBuf *own) {
+    const uint32_t ng = g1 - g0;
+    std::vector<const uint8_t *> dbufs((size_t)num_cols * ng);
+    std::vector<uint64_t> at((size_t)num_cols * ng, 0);
+    uint64_t total = 0;
+    for (uint32_t i = 0; i < num_cols; i++) {
+        if (cols[i] >= p->cols.size()) fail(GBAGG_INVALID, "column out of range");
+        for (uint32_t g = 0; g < ng; g++) {
+            int64_t start, size;
+            parquet_chunk_range(p, g0 + g, cols[i], start, size);
+            at[(size_t)i * ng + g] = p->resident ? (uint64_t)start : total;
+            total += ((uint64_t)size + 16 + 255) & ~255ull;
+        }
+    }
+    const uint8_t *base = p->image.as<uint8_t>();
+    if (!p->resident && num_cols) {
+        uint8_t *dst;
+        if (own) {
+            own->ensure(total);
+            dst = own->as<uint8_t>();
+        } else {
+            dst = p->alloc(total);
+        }
+        for (uint32_t i = 0; i < num_cols; i++)
+            for (uint32_t g = 0; g < ng; g++) {
+                int64_t start, size;
+                parquet_chunk_range(p, g0 + g, cols[i], start, size);
+                uint8_t *d = dst + at[(size_t)i * ng + g];
+                HIPCHK(hipMemsetAsync(d + size, 0, 16, p->stream));
+                if (size) HIPCHK(hipMemcpyAsync(d, p->file + start, (size_t)size, hipMemcpyHostToDevice, p->stream));
+            }
+        base = dst;
+    }
+    for (size_t k = 0; k < dbufs.size(); k++) dbufs[k] = base + at[k];
+    return dbufs;
+}
+
+// Launches the queued row groups of gbagg_parquet_decode_async.
+void parquet_flush(gbagg_parquet *p) {
+    if (!p->pending || !p->pending_rows) return;
+    std::unique_ptr<PqCall> call = std::move(p->pending);
+    p->pending_rows = 0;
+    parquet_launch(p, *call);
+}
+
+// Launches what is queued, waits for every launch, and reports their errors.
+void parquet_wait(gbagg_parquet *p) {
+    if (!p->stream) return;
+    parquet_flush(p);
+    if (p->armed) HIPCHK(hipMemcpyAsync(p->hbad, p->bad.ptr, 4, hipMemcpyDeviceToHost, p->stream));
+    HIPCHK(hipStreamSynchronize(p->stream));
+    if (!p->armed) return;
+    p->armed = false;
+    const uint32_t bad = *p->hbad;
+    if (bad & 1) fail(GBAGG_INVALID, "parquet: dictionary index outside the dictionary");
+    if (bad & 2) fail(GBAGG_INVALID, "parquet: truncated BYTE_ARRAY value");
+    if (bad & 4) fail(GBAGG_INVALID, "parquet: malformed dictionary index stream");
+}
+
+}  // namespace
+
 extern "C" {
 
 gbagg_status gbagg_parquet_open(const uint8_t *file, uint64_t size, gbagg_parquet **out) {
@@ -3681,40 +3844,11 @@ gbagg_status gbagg_parquet_decode_range(gbagg_parquet *p, uint32_t rg0, uint32_t
     return guarded([&] {
         if (!p || rg0 >= rg1 || rg1 > p->rgs.size()) fail(GBAGG_INVALID, "row group range out of range");
         if (num_cols && (!cols || !out)) fail(GBAGG_INVALID, "null columns");
-        if (!p->stream) {
-            (void)device_info();
-            HIPCHK(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
-        }
-        p->bad.ensure(4);
-        HIPCHK(hipMemsetAsync(p->bad.ptr, 0, 4, p->stream));
+        parquet_stream(p);
+        parquet_wait(p);  // queued row groups first, with their errors
+        parquet_arm(p);
         const uint32_t ng = rg1 - rg0;
-        // chunk bytes on the device: in place in the resident image, else one
-        // upload per chunk into this call's buffer (16 readable bytes past each)
-        std::vector<const uint8_t *> dbufs((size_t)num_cols * ng);
-        std::vector<uint64_t> at((size_t)num_cols * ng, 0);
-        uint64_t total = 0;
-        for (uint32_t i = 0; i < num_cols; i++) {
-            if (cols[i] >= p->cols.size()) fail(GBAGG_INVALID, "column out of range");
-            for (uint32_t g = 0; g < ng; g++) {
-                int64_t start, size;
-                parquet_chunk_range(p, rg0 + g, cols[i], start, size);
-                at[(size_t)i * ng + g] = p->resident ? (uint64_t)start : total;
-                total += ((uint64_t)size + 16 + 255) & ~255ull;
-            }
-        }
-        if (!p->resident && num_cols) {
-            p->chunks.ensure(total);
-            for (uint32_t i = 0; i < num_cols; i++)
-                for (uint32_t g = 0; g < ng; g++) {
-                    int64_t start, size;
-                    parquet_chunk_range(p, rg0 + g, cols[i], start, size);
-                    uint8_t *d = p->chunks.as<uint8_t>() + at[(size_t)i * ng + g];
-                    HIPCHK(hipMemsetAsync(d + size, 0, 16, p->stream));
-                    if (size) HIPCHK(hipMemcpyAsync(d, p->file + start, (size_t)size, hipMemcpyHostToDevice, p->stream));
-                }
-        }
-        const uint8_t *base = p->resident ? p->image.as<uint8_t>() : p->chunks.as<uint8_t>();
-        for (size_t k = 0; k < dbufs.size(); k++) dbufs[k] = base + at[k];
+        const std::vector<const uint8_t *> dbufs = parquet_chunks(p, rg0, rg1, cols, num_cols, &p->chunks);
         PqCall call;
         call.dev_streams = (uint64_t)ng * num_cols >= 32;
         for (uint32_t i = 0; i < num_cols; i++) {
@@ -3722,13 +3856,7 @@ gbagg_status gbagg_parquet_decode_range(gbagg_parquet *p, uint32_t rg0, uint32_t
             parquet_plan_column(p, rg0, rg1, cols[i], dbufs.data() + (size_t)i * ng, *p->outs.back(), out[i], call);
         }
         parquet_launch(p, call);
-        if (!p->hbad) HIPCHK(hipHostMalloc((void **)&p->hbad, 4, hipHostMallocDefault));
-        HIPCHK(hipMemcpyAsync(p->hbad, p->bad.ptr, 4, hipMemcpyDeviceToHost, p->stream));
-        HIPCHK(hipStreamSynchronize(p->stream));  // the tables' host copies end with this call
-        const uint32_t bad = *p->hbad;
-        if (bad & 1) fail(GBAGG_INVALID, "parquet: dictionary index outside the dictionary");
-        if (bad & 2) fail(GBAGG_INVALID, "parquet: truncated BYTE_ARRAY value");
-        if (bad & 4) fail(GBAGG_INVALID, "parquet: malformed dictionary index stream");
+        parquet_wait(p);  // the chunk buffer is reused by the next call
         if (rows) {
             uint64_t n = 0;
             for (uint32_t g = rg0; g < rg1; g++) n += (uint64_t)p->rgs[g].num_rows;
@@ -3742,10 +3870,56 @@ gbagg_status gbagg_parquet_decode(gbagg_parquet *p, uint32_t rg, const uint32_t 
     return gbagg_parquet_decode_range(p, rg, rg + 1, cols, num_cols, out, rows);
 }
 
+gbagg_status gbagg_parquet_decode_async(gbagg_parquet *p, uint32_t rg, const uint32_t *cols, uint32_t num_cols,
+                                        gbagg_column_view *out, uint64_t *rows) {
+    return guarded([&] {
+        if (!p || rg >= p->rgs.size()) fail(GBAGG_INVALID, "row group out of range");
+        if (num_cols && (!cols || !out)) fail(GBAGG_INVALID, "null columns");
+        parquet_stream(p);
+        parquet_arm(p);
+        const std::vector<const uint8_t *> dbufs = parquet_chunks(p, rg, rg + 1, cols, num_cols, nullptr);
+        // planned on its own, then queued: a row group that fails to plan
+        // leaves the queue as it was
+        PqCall call;
+        call.dev_streams = true;  // index run headers read on the device, many streams per launch
+        std::vector<gbagg_column_view> views(num_cols);
+        for (uint32_t i = 0; i < num_cols; i++) {
+            p->outs.push_back(std::make_unique<gbagg_parquet::Out>());
+            parquet_plan_column(p, rg, rg + 1, cols[i], dbufs.data() + i, *p->outs.back(), views[i], call);
+        }
+        if (!p->pending) p->pending = std::make_unique<PqCall>();
+        call_append(*p->pending, std::move(call));
+        const uint64_t n = (uint64_t)p->rgs[rg].num_rows;
+        p->pending_rows += std::max<uint64_t>(n, 1);
+        for (uint32_t i = 0; i < num_cols; i++) out[i] = views[i];
+        if (rows) *rows = n;
+        if (p->pending_rows >= p->flush_rows) parquet_flush(p);
+    });
+}
+
+gbagg_status gbagg_parquet_sync(gbagg_parquet *p) {
+    return guarded([&] {
+        if (!p) fail(GBAGG_INVALID, "null handle");
+        parquet_wait(p);
+    });
+}
+
+gbagg_status gbagg_parquet_set_flush_rows(gbagg_parquet *p, uint64_t rows) {
+    return guarded([&] {
+        if (!p) fail(GBAGG_INVALID, "null handle");
+        if (!rows) fail(GBAGG_INVALID, "flush_rows must be at least 1");
+        p->flush_rows = rows;
+    });
+}
+
 gbagg_status gbagg_parquet_reset(gbagg_parquet *p) {
     return guarded([&] {
         if (!p) fail(GBAGG_INVALID, "null handle");
+        // queued row groups are dropped; launched ones finish first
+        p->pending.reset();
+        p->pending_rows = 0;
         if (p->stream) HIPCHK(hipStreamSynchronize(p->stream));
+        p->armed = false;
         p->outs.clear();
         p->arena = 0;
         p->arena_used = 0;
@@ -3755,10 +3929,8 @@ gbagg_status gbagg_parquet_reset(gbagg_parquet *p) {
 gbagg_status gbagg_parquet_upload(gbagg_parquet *p) {
     return guarded([&] {
         if (!p) fail(GBAGG_INVALID, "null handle");
-        if (!p->stream) {
-            (void)device_info();
-            HIPCHK(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
-        }
+        parquet_stream(p);
+        parquet_wait(p);  // queued row groups read the chunks from where they are now
         p->image.ensure(p->size + 16);
         HIPCHK(hipMemsetAsync(p->image.as<uint8_t>() + p->size, 0, 16, p->stream));
         HIPCHK(hipMemcpyAsync(p->image.ptr, p->file, p->size, hipMemcpyHostToDevice, p->stream));
@@ -3769,8 +3941,7 @@ gbagg_status gbagg_parquet_upload(gbagg_parquet *p) {
 
 void gbagg_parquet_close(gbagg_parquet *p) {
     if (!p) return;
-    if (p->stream) (void)hipStreamSynchronize(p->stream);
-    delete p;
+    delete p;  // waits for launched work; queued row groups are dropped
 }
 
 }  // extern "C"

@@ -28,6 +28,45 @@ constexpr int COMPACT_CHUNK = 4096;  // table slots per compaction block
 constexpr uint32_t INVALID_SLOT = 0xFFFFFFFFu;
 
 // ------------------------------------------------------------------------
+// Every batch of a finalize in one launch: the batches' rows concatenated,
+// start[i] the first row of batch i and start[nb] the total. Workgroup w
+// takes rows [w * per, (w + 1) * per) of the concatenation; f(b, lo, hi)
+// runs for each batch b overlapping them, [lo, hi) local to b. Batches of a
+// few row groups each thus share workgroups instead of a launch apiece.
+// ------------------------------------------------------------------------
+struct BatchRows {
+    const DevBatch *b;
+    const uint64_t *start;
+    uint32_t nb;
+    uint32_t pad;
+    uint64_t per;
+};
+
+// The batch holding concatenated row r (the last one starting at or before r).
+__device__ __forceinline__ uint32_t batch_of(const BatchRows &br, uint64_t r) {
+    uint32_t lo = 0, hi = br.nb - 1;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) / 2;
+        if (br.start[mid] <= r) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+template <class F>
+__device__ __forceinline__ void for_batch_rows(const BatchRows &br, F &&f) {
+    const uint64_t total = br.start[br.nb];
+    const uint64_t R0 = (uint64_t)blockIdx.x * br.per;
+    const uint64_t R1 = R0 + br.per < total ? R0 + br.per : total;
+    if (R0 >= R1) return;
+    for (uint32_t i = batch_of(br, R0); i < br.nb && br.start[i] < R1; i++) {
+        const uint64_t a = R0 > br.start[i] ? R0 : br.start[i];
+        const uint64_t e = R1 < br.start[i + 1] ? R1 : br.start[i + 1];
+        if (a < e) f(br.b[i], a - br.start[i], e - br.start[i]);
+    }
+}
+
+// ------------------------------------------------------------------------
 // HyperLogLog (Flajolet et al. 2007), p = 12: register = top 12 hash bits,
 // rank = leading zeros of the remaining bits + 1.
 // ------------------------------------------------------------------------
@@ -263,20 +302,19 @@ __device__ __forceinline__ void wave_reduce_apply(const DevQuery &q, const Row &
 }
 
 template <class S>
-__global__ __launch_bounds__(AGG_BLOCK) void k_aggregate(const DevQuery *__restrict__ qp,
-                                                         const DevBatch *__restrict__ bp, DevTable T,
+__global__ __launch_bounds__(AGG_BLOCK) void k_aggregate(const DevQuery *__restrict__ qp, BatchRows br, DevTable T,
                                                          uint32_t lds_slots, uint32_t fill_limit) {
     const DevQuery &q = *qp;
-    const DevBatch b = *bp;  // column pointers stay in registers
     const S sh(q);
     extern __shared__ uint64_t smem[];
     LdsTable t = lds_table_setup(smem, q, lds_slots, fill_limit);
     __syncthreads();
     const uint64_t cap = T.plan->cap;
     const uint64_t mask = cap - 1;
-    const uint64_t n = b.nrows;
     const uint64_t tile = (uint64_t)AGG_BLOCK * AGG_ROWS;
-    for (uint64_t base = (uint64_t)blockIdx.x * tile; base < n; base += (uint64_t)gridDim.x * tile) {
+    for_batch_rows(br, [&](const DevBatch &bb, uint64_t lo, uint64_t n) {
+    const DevBatch b = bb;  // column pointers stay in registers
+    for (uint64_t base = lo; base < n; base += tile) {
         Row rows[AGG_ROWS];
 #pragma unroll
         for (int j = 0; j < AGG_ROWS; j++) {
@@ -303,6 +341,7 @@ __global__ __launch_bounds__(AGG_BLOCK) void k_aggregate(const DevQuery *__restr
             }
         }
     }
+    });
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < t.S; i += blockDim.x) {
         uint64_t k = t.key[i];
@@ -323,15 +362,15 @@ constexpr uint32_t OVF_DIRECT = 16u;  // DevPlan::overflow bit: a key outside th
 // Exact range of the key words of the rows that pass the filter (NULL, dropped
 // and SENTINEL keys excluded): one atomic min / max per wave.
 template <class S>
-__global__ __launch_bounds__(AGG_BLOCK) void k_key_range(const DevQuery *__restrict__ qp,
-                                                         const DevBatch *__restrict__ bp, DevPlan *plan) {
+__global__ __launch_bounds__(AGG_BLOCK) void k_key_range(const DevQuery *__restrict__ qp, BatchRows br,
+                                                         DevPlan *plan) {
     const DevQuery &q = *qp;
-    const DevBatch b = *bp;
     const S sh(q);
-    const uint64_t n = b.nrows;
     const uint64_t tile = (uint64_t)AGG_BLOCK * AGG_ROWS;
     int64_t lo = INT64_MAX, hi = INT64_MIN;
-    for (uint64_t base = (uint64_t)blockIdx.x * tile; base < n; base += (uint64_t)gridDim.x * tile) {
+    for_batch_rows(br, [&](const DevBatch &bb, uint64_t r0, uint64_t n) {
+    const DevBatch b = bb;
+    for (uint64_t base = r0; base < n; base += tile) {
         Row rows[AGG_ROWS];
 #pragma unroll
         for (int j = 0; j < AGG_ROWS; j++) {
@@ -350,6 +389,7 @@ __global__ __launch_bounds__(AGG_BLOCK) void k_key_range(const DevQuery *__restr
             }
         }
     }
+    });
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
         lo = min(lo, (int64_t)__shfl_xor(lo, off));
@@ -369,21 +409,22 @@ __global__ void k_key_range_init(DevPlan *plan) {
 // Exact largest exponents of every double SUM input column over all rows of
 // a batch (valid rows; the filter is not applied, a superset is a bound too):
 // one atomic max per wave and column.
-__global__ __launch_bounds__(AGG_BLOCK) void k_fx_range(const DevQuery *__restrict__ qp,
-                                                        const DevBatch *__restrict__ bp, DevPlan *plan) {
+__global__ __launch_bounds__(AGG_BLOCK) void k_fx_range(const DevQuery *__restrict__ qp, BatchRows br,
+                                                        DevPlan *plan) {
     const DevQuery &q = *qp;
-    const DevBatch b = *bp;
     const uint32_t fxm = fx_slots(q);
     uint32_t emax[MAXU] = {};
-    for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < b.nrows;
-         r += (uint64_t)gridDim.x * blockDim.x) {
+    for_batch_rows(br, [&](const DevBatch &bb, uint64_t lo, uint64_t n) {
+        const DevBatch b = bb;
+        for (uint64_t r = lo + threadIdx.x; r < n; r += blockDim.x) {
 #pragma unroll
-        for (int u = 0; u < MAXU; u++) {
-            if (!((fxm >> u) & 1)) continue;
-            if (b.valid[u] && !gld_stream(b.valid[u] + r)) continue;
-            emax[u] = max(emax[u], fx_exp_of(gld_stream((const uint64_t *)b.data[u] + r)));
+            for (int u = 0; u < MAXU; u++) {
+                if (!((fxm >> u) & 1)) continue;
+                if (b.valid[u] && !gld_stream(b.valid[u] + r)) continue;
+                emax[u] = max(emax[u], fx_exp_of(gld_stream((const uint64_t *)b.data[u] + r)));
+            }
         }
-    }
+    });
 #pragma unroll
     for (int u = 0; u < MAXU; u++) {
         if (!((fxm >> u) & 1)) continue;
@@ -401,11 +442,9 @@ __global__ __launch_bounds__(AGG_BLOCK) void k_fx_range(const DevQuery *__restri
 // (as in k_aggregate) and the other rows update their global record directly
 // with device-scope atomics.
 template <class S, bool LDS>
-__global__ __launch_bounds__(AGG_BLOCK) void k_direct(const DevQuery *__restrict__ qp,
-                                                      const DevBatch *__restrict__ bp, DevTable T, int64_t kmin,
-                                                      uint32_t span, uint32_t cache) {
+__global__ __launch_bounds__(AGG_BLOCK) void k_direct(const DevQuery *__restrict__ qp, BatchRows br, DevTable T,
+                                                      int64_t kmin, uint32_t span, uint32_t cache) {
     const DevQuery &q = *qp;
-    const DevBatch b = *bp;
     const S sh(q);
     extern __shared__ uint64_t smem[];
     LdsTable t{};
@@ -419,9 +458,10 @@ __global__ __launch_bounds__(AGG_BLOCK) void k_direct(const DevQuery *__restrict
     }
     __syncthreads();
     const uint64_t cap = T.plan->cap;
-    const uint64_t n = b.nrows;
     const uint64_t tile = (uint64_t)AGG_BLOCK * AGG_ROWS;
-    for (uint64_t base = (uint64_t)blockIdx.x * tile; base < n; base += (uint64_t)gridDim.x * tile) {
+    for_batch_rows(br, [&](const DevBatch &bb, uint64_t lo, uint64_t n) {
+    const DevBatch b = bb;
+    for (uint64_t base = lo; base < n; base += tile) {
         Row rows[AGG_ROWS];
 #pragma unroll
         for (int j = 0; j < AGG_ROWS; j++) {
@@ -458,6 +498,7 @@ __global__ __launch_bounds__(AGG_BLOCK) void k_direct(const DevQuery *__restrict
             }
         }
     }
+    });
     __syncthreads();
     if constexpr (LDS) {
         for (uint32_t i = threadIdx.x; i < span; i += blockDim.x) {
@@ -478,16 +519,15 @@ __global__ __launch_bounds__(AGG_BLOCK) void k_direct(const DevQuery *__restrict
 }
 
 // Global-table-only variant (no LDS table).
-__global__ __launch_bounds__(AGG_BLOCK) void k_aggregate_global(const DevQuery *__restrict__ qp,
-                                                                const DevBatch *__restrict__ bp,
+__global__ __launch_bounds__(AGG_BLOCK) void k_aggregate_global(const DevQuery *__restrict__ qp, BatchRows br,
                                                                 DevTable T) {
     const DevQuery &q = *qp;
-    const DevBatch b = *bp;  // column pointers stay in registers
     const uint64_t cap = T.plan->cap;
     const uint64_t mask = cap - 1;
-    const uint64_t n = b.nrows;
     const uint64_t tile = (uint64_t)AGG_BLOCK * AGG_ROWS;
-    for (uint64_t base = (uint64_t)blockIdx.x * tile; base < n; base += (uint64_t)gridDim.x * tile) {
+    for_batch_rows(br, [&](const DevBatch &bb, uint64_t lo, uint64_t n) {
+    const DevBatch b = bb;  // column pointers stay in registers
+    for (uint64_t base = lo; base < n; base += tile) {
         Row rows[AGG_ROWS];
 #pragma unroll
         for (int j = 0; j < AGG_ROWS; j++) {
@@ -519,6 +559,7 @@ __global__ __launch_bounds__(AGG_BLOCK) void k_aggregate_global(const DevQuery *
             if (g >= 0) g_update(q, T, (uint64_t)g, rows[j]);
         }
     }
+    });
 }
 
 // ------------------------------------------------------------------------

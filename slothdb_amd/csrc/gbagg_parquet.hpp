@@ -676,6 +676,8 @@ uint64_t hybrid_runs(const uint8_t *p, const uint8_t *end, const uint8_t *chunk0
     return ones;
 }
 
+struct PqCall;
+
 }  // namespace
 
 struct gbagg_parquet {
@@ -701,10 +703,22 @@ struct gbagg_parquet {
     DevBuf image;           // gbagg_parquet_upload: the whole file in HBM (+16 bytes)
     bool resident = false;
     DevBuf chunks, bad;     // per decode call: chunk bytes when the file is not resident
-    DevBuf dtab;            // per decode call: the runs and pages of all its columns
-    uint8_t *htab = nullptr;  // pinned host staging of those tables (one async copy per call)
-    size_t htab_bytes = 0;
-    uint32_t *hbad = nullptr;  // pinned host copy of the call's error bits, read after its wait
+    // a launch's runs and pages go to the device in one copy from pinned
+    // staging; two staging buffers alternate, each reused once the event
+    // recorded after its copy has passed (the device tables themselves are
+    // arena memory, never reused before reset)
+    uint8_t *htab[2] = {nullptr, nullptr};
+    size_t htab_bytes[2] = {0, 0};
+    hipEvent_t htab_done[2] = {nullptr, nullptr};
+    int htab_next = 0;
+    uint32_t *hbad = nullptr;  // pinned host copy of the error bits, read after a wait
+    // gbagg_parquet_decode_async: row groups planned into one pending call,
+    // launched once flush_rows rows are pending and at gbagg_parquet_sync;
+    // the error bits collect every launch since the last wait (armed)
+    std::unique_ptr<PqCall> pending;
+    uint64_t pending_rows = 0;
+    uint64_t flush_rows = 4u << 20;
+    bool armed = false;
     // decoded columns are bump-allocated from arenas of 256 MB or more (one
     // hipMalloc per arena, not per column), all kept until close
     // (gbagg_parquet_reset forgets them and reuses the arenas)
@@ -725,11 +739,7 @@ struct gbagg_parquet {
         arena_used += n;
         return p;
     }
-    ~gbagg_parquet() {
-        if (stream) (void)hipStreamDestroy(stream);
-        if (htab) (void)hipHostFree(htab);
-        if (hbad) (void)hipHostFree(hbad);
-    }
+    ~gbagg_parquet();
 };
 
 namespace {
@@ -855,6 +865,43 @@ struct PqCall {
     };
     std::vector<Strs> strs;
 };
+
+// Appends call b (planned on its own) to call a: b's value indexes follow a's.
+void call_append(PqCall &a, PqCall &&b) {
+    for (PqRun r : b.runs) {
+        r.gstart += a.nvals;
+        a.runs.push_back(r);
+    }
+    for (PqSeg g : b.segs) {
+        g.gstart += a.ndense;
+        a.segs.push_back(g);
+    }
+    for (PqStrDict d : b.sdict) {
+        d.gstart += a.nsdict;
+        a.sdict.push_back(d);
+    }
+    a.nvals += b.nvals;
+    a.ndense += b.ndense;
+    a.nsdict += b.nsdict;
+    a.streams.insert(a.streams.end(), b.streams.begin(), b.streams.end());
+    a.spages.insert(a.spages.end(), b.spages.begin(), b.spages.end());
+    a.nulls.insert(a.nulls.end(), b.nulls.begin(), b.nulls.end());
+    a.strs.insert(a.strs.end(), b.strs.begin(), b.strs.end());
+}
+
+}  // namespace
+
+gbagg_parquet::~gbagg_parquet() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (int k = 0; k < 2; k++) {
+        if (htab_done[k]) (void)hipEventDestroy(htab_done[k]);
+        if (htab[k]) (void)hipHostFree(htab[k]);
+    }
+    if (hbad) (void)hipHostFree(hbad);
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+namespace {
 
 // Host pass over column c of row groups [g0, g1), whose chunk bytes are on
 // the device at dbufs[g - g0] (16 readable bytes past each): o's buffers (the
@@ -1137,27 +1184,33 @@ void parquet_plan_column(gbagg_parquet *pq, uint32_t g0, uint32_t g1, uint32_t c
 void parquet_launch(gbagg_parquet *pq, PqCall &call) {
     hipStream_t s = pq->stream;
     const uint32_t cap = (uint32_t)device_info().cus * 8;
-    // both tables in one pinned staging buffer, one copy (the previous call
-    // ended with a stream synchronisation, so the staging buffer is free)
+    // every table in one pinned staging buffer, one copy
     const size_t rb = call.runs.size() * sizeof(PqRun), sb = call.segs.size() * sizeof(PqSeg);
     const size_t pb = call.spages.size() * sizeof(PqStrPage), db = call.sdict.size() * sizeof(PqStrDict);
     const size_t qb = call.streams.size() * sizeof(PqStream);
     const size_t tb = rb + sb + pb + db + qb;
+    uint8_t *dtab = nullptr;
     if (tb) {
-        if (pq->htab_bytes < tb) {
-            if (pq->htab) HIPCHK(hipHostFree(pq->htab));
-            pq->htab = nullptr;
-            pq->htab_bytes = 0;
-            HIPCHK(hipHostMalloc((void **)&pq->htab, tb, hipHostMallocDefault));
-            pq->htab_bytes = tb;
+        const int k = pq->htab_next;
+        pq->htab_next ^= 1;
+        if (pq->htab_done[k]) HIPCHK(hipEventSynchronize(pq->htab_done[k]));  // its last copy has been read
+        else HIPCHK(hipEventCreateWithFlags(&pq->htab_done[k], hipEventDisableTiming));
+        if (pq->htab_bytes[k] < tb) {
+            if (pq->htab[k]) HIPCHK(hipHostFree(pq->htab[k]));
+            pq->htab[k] = nullptr;
+            pq->htab_bytes[k] = 0;
+            HIPCHK(hipHostMalloc((void **)&pq->htab[k], tb, hipHostMallocDefault));
+            pq->htab_bytes[k] = tb;
         }
-        if (rb) std::memcpy(pq->htab, call.runs.data(), rb);
-        if (sb) std::memcpy(pq->htab + rb, call.segs.data(), sb);
-        if (pb) std::memcpy(pq->htab + rb + sb, call.spages.data(), pb);
-        if (db) std::memcpy(pq->htab + rb + sb + pb, call.sdict.data(), db);
-        if (qb) std::memcpy(pq->htab + rb + sb + pb + db, call.streams.data(), qb);
-        pq->dtab.ensure(tb);
-        HIPCHK(hipMemcpyAsync(pq->dtab.ptr, pq->htab, tb, hipMemcpyHostToDevice, s));
+        uint8_t *h = pq->htab[k];
+        if (rb) std::memcpy(h, call.runs.data(), rb);
+        if (sb) std::memcpy(h + rb, call.segs.data(), sb);
+        if (pb) std::memcpy(h + rb + sb, call.spages.data(), pb);
+        if (db) std::memcpy(h + rb + sb + pb, call.sdict.data(), db);
+        if (qb) std::memcpy(h + rb + sb + pb + db, call.streams.data(), qb);
+        dtab = pq->alloc(tb);
+        HIPCHK(hipMemcpyAsync(dtab, h, tb, hipMemcpyHostToDevice, s));
+        HIPCHK(hipEventRecord(pq->htab_done[k], s));
     }
     // values per thread per tile: enough tiles for `cap` workgroups, at most PQ_ITEMS
     auto items_for = [&](uint64_t n) {
@@ -1165,28 +1218,28 @@ void parquet_launch(gbagg_parquet *pq, PqCall &call) {
     };
     if (!call.streams.empty())
         hipLaunchKernelGGL(k_pq_streams, dim3((uint32_t)std::min<size_t>(call.streams.size(), cap)), dim3(256), 0, s,
-                           (const PqStream *)(pq->dtab.as<uint8_t>() + rb + sb + pb + db),
+                           (const PqStream *)(dtab + rb + sb + pb + db),
                            (uint32_t)call.streams.size(), pq->bad.as<uint32_t>());
     if (!call.runs.empty()) {
         const uint32_t items = items_for(call.nvals);
         const uint32_t grid = (uint32_t)std::min<uint64_t>((call.nvals + 256ull * items - 1) / (256ull * items), cap);
-        hipLaunchKernelGGL(k_pq_hybrid, dim3(grid), dim3(256), 0, s, (const PqRun *)pq->dtab.ptr,
+        hipLaunchKernelGGL(k_pq_hybrid, dim3(grid), dim3(256), 0, s, (const PqRun *)dtab,
                            (uint32_t)call.runs.size(), call.nvals, items);
     }
     if (!call.segs.empty()) {
         const uint32_t items = items_for(call.ndense);
         const uint32_t grid = (uint32_t)std::min<uint64_t>((call.ndense + 256ull * items - 1) / (256ull * items), cap);
-        hipLaunchKernelGGL(k_pq_pages, dim3(grid), dim3(256), 0, s, (const PqSeg *)(pq->dtab.as<uint8_t>() + rb),
+        hipLaunchKernelGGL(k_pq_pages, dim3(grid), dim3(256), 0, s, (const PqSeg *)(dtab + rb),
                            (uint32_t)call.segs.size(), call.ndense, items, pq->bad.as<uint32_t>());
     }
     if (!call.spages.empty())
         hipLaunchKernelGGL(k_pq_str_plain, dim3((uint32_t)((call.spages.size() + 63) / 64)), dim3(64), 0, s,
-                           (const PqStrPage *)(pq->dtab.as<uint8_t>() + rb + sb), (uint32_t)call.spages.size(),
+                           (const PqStrPage *)(dtab + rb + sb), (uint32_t)call.spages.size(),
                            pq->bad.as<uint32_t>());
     if (!call.sdict.empty()) {
         const uint32_t grid = (uint32_t)std::min<uint64_t>((call.nsdict + 255) / 256, cap);
         hipLaunchKernelGGL(k_pq_str_dict, dim3(grid), dim3(256), 0, s,
-                           (const PqStrDict *)(pq->dtab.as<uint8_t>() + rb + sb + pb), (uint32_t)call.sdict.size(),
+                           (const PqStrDict *)(dtab + rb + sb + pb), (uint32_t)call.sdict.size(),
                            call.nsdict);
     }
     for (const auto &st : call.strs) {

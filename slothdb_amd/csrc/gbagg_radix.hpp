@@ -113,11 +113,12 @@ __device__ __forceinline__ void sample_global_add(const RadixParams &rp, uint64_
 
 constexpr uint32_t SAMPLE_LDS = 1024;
 
-__global__ __launch_bounds__(256) void k_sample(const DevCtx *__restrict__ cx, const DevBatch *__restrict__ bp,
-                                                uint32_t sample_n, uint64_t sample_stride, DevPlan *plan) {
+// Sample rows i * sample_stride of the batches' concatenation.
+__global__ __launch_bounds__(256) void k_sample(const DevCtx *__restrict__ cx, BatchRows br, uint32_t sample_n,
+                                                uint64_t sample_stride, DevPlan *plan) {
     const DevQuery &q = cx->q;
     const RadixParams &rp = cx->rp;
-    const DevBatch b = *bp;  // column pointers stay in registers
+    const uint64_t total = br.start[br.nb];
     __shared__ uint64_t lk[SAMPLE_LDS];
     __shared__ uint32_t lc[SAMPLE_LDS];
     __shared__ uint32_t s_fx[MAXU];
@@ -128,7 +129,10 @@ __global__ __launch_bounds__(256) void k_sample(const DevCtx *__restrict__ cx, c
     uint32_t kept = 0;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < sample_n; i += gridDim.x * blockDim.x) {
         uint64_t r = (uint64_t)i * sample_stride;
-        if (r >= b.nrows) continue;
+        if (r >= total) continue;
+        const uint32_t bi = batch_of(br, r);
+        const DevBatch &b = br.b[bi];
+        r -= br.start[bi];
         Row row;
         load_row(q, b, r, ((1u << q.nkp) - 1) | fxm, row);
         if (q.npred && !eval_preds(q, b, row)) continue;
@@ -340,11 +344,9 @@ __device__ __forceinline__ int heavy_find(const uint64_t *hk, const uint32_t *hi
 
 template <class S>
 __global__ __launch_bounds__(RP_BLOCK) __attribute__((amdgpu_waves_per_eu(8))) void k_rp_count(const DevCtx *__restrict__ cx,
-                                                       const DevBatch *__restrict__ bp, uint32_t wg_offset,
-                                                       uint64_t rows_per_wg) {
+                                                                                                BatchRows br) {
     const DevQuery &q = cx->q;
     const RadixParams &rp = cx->rp;
-    const DevBatch b = *bp;  // column pointers stay in registers
     const S sh(q);
     extern __shared__ uint64_t smem[];
     uint64_t *hk = smem;                                  // [hslots]
@@ -356,21 +358,22 @@ __global__ __launch_bounds__(RP_BLOCK) __attribute__((amdgpu_waves_per_eu(8))) v
     for (uint32_t i = threadIdx.x; i < HLL_M; i += blockDim.x) reg[i] = 0;
     __syncthreads();
     const uint32_t hmask = rp.hslots - 1;
-    const uint64_t r0 = (uint64_t)blockIdx.x * rows_per_wg;
-    const uint64_t r1 = min(b.nrows, r0 + rows_per_wg);
     const uint32_t shift = 64 - rp.bucket_bits;
-    auto count_row = [&](const Row &row) {
-        if (sh.npred() && !eval_preds_s(sh, q, b, row)) return;
-        uint64_t w;
-        if (make_key_s(sh, q, b, row, w) != KEY_OK || w == SENTINEL) return;
-        const uint64_t h = hash64(w);
-        hll_add(reg, h);
-        if (heavy_has(hk, hmask, h, w)) return;
-        __hip_atomic_fetch_add(&cnt[h >> shift], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    };
-    for_rows_pipelined<S::kStatic ? AGG_ROWS : 1, RP_BLOCK>(sh, b, r0, r1, sh.nkp(), threadIdx.x, count_row);
+    for_batch_rows(br, [&](const DevBatch &bb, uint64_t r0, uint64_t r1) {
+        const DevBatch b = bb;  // column pointers stay in registers
+        auto count_row = [&](const Row &row) {
+            if (sh.npred() && !eval_preds_s(sh, q, b, row)) return;
+            uint64_t w;
+            if (make_key_s(sh, q, b, row, w) != KEY_OK || w == SENTINEL) return;
+            const uint64_t h = hash64(w);
+            hll_add(reg, h);
+            if (heavy_has(hk, hmask, h, w)) return;
+            __hip_atomic_fetch_add(&cnt[h >> shift], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        };
+        for_rows_pipelined<S::kStatic ? AGG_ROWS : 1, RP_BLOCK>(sh, b, r0, r1, sh.nkp(), threadIdx.x, count_row);
+    });
     __syncthreads();
-    const uint32_t wg = wg_offset + blockIdx.x;
+    const uint32_t wg = blockIdx.x;
     uint32_t *out = rp.hist + (uint64_t)wg * nb;
     for (uint32_t i = threadIdx.x; i < nb; i += blockDim.x) out[i] = cnt[i];
     uint8_t *hl = rp.hll + (uint64_t)wg * HLL_M;
@@ -460,14 +463,13 @@ __host__ __device__ inline size_t scatter_lds_bytes(uint32_t hslots, int nacc, u
     return (size_t)hslots * 12 + ((size_t)nacc * hmax + 1) / 2 * 16 + (size_t)ring * (rw * 8 + 12) + (size_t)P * 4 + 8;
 }
 
+// Workgroup w's rows are those of count workgroups g1 * w .. g1 * w + g1 - 1
+// (br.per = g1 * the count pass's rows per workgroup), which are contiguous
+// in every partition.
 template <class S>
-__global__ __launch_bounds__(RP_BLOCK) void k_rp_scatter(const DevCtx *__restrict__ cx,
-                                                         const DevBatch *__restrict__ bp, DevTable T,
-                                                         uint32_t wg_offset, uint64_t rows_per_wg,
-                                                         uint32_t count_wg_offset) {
+__global__ __launch_bounds__(RP_BLOCK) void k_rp_scatter(const DevCtx *__restrict__ cx, BatchRows br, DevTable T) {
     const DevQuery &q = cx->q;
     const RadixParams &rp = cx->rp;
-    const DevBatch b = *bp;  // column pointers stay in registers
     const S sh(q);
     extern __shared__ uint64_t smem[];
     const uint32_t P = 1u << rp.part_bits;
@@ -482,7 +484,7 @@ __global__ __launch_bounds__(RP_BLOCK) void k_rp_scatter(const DevCtx *__restric
     uint32_t *rpub = rpos + NR;                               // [NR] n + 1 once entry n is readable
     uint32_t *rfree = rpub + NR;                              // [NR] next entry number the slot may take
     uint32_t *rctl = rfree + NR;                              // [0] entries reserved, [1] producers done
-    const uint32_t wg = wg_offset + blockIdx.x;
+    const uint32_t wg = blockIdx.x;
     for (uint32_t i = threadIdx.x; i < rp.hslots; i += blockDim.x) {
         hk[i] = rp.hkeys[i];
         hi[i] = rp.hidx[i];
@@ -496,16 +498,12 @@ __global__ __launch_bounds__(RP_BLOCK) void k_rp_scatter(const DevCtx *__restric
         rfree[i] = i;
     }
     if (threadIdx.x < 2) rctl[threadIdx.x] = 0;
-    // this workgroup's rows are those of count workgroups g1 * b .. g1 * b + g1 - 1,
-    // which are contiguous in every partition
-    const uint64_t cw = (uint64_t)count_wg_offset + (uint64_t)blockIdx.x * rp.g1;
+    const uint64_t cw = (uint64_t)blockIdx.x * rp.g1;
     for (uint32_t p = threadIdx.x; p < P; p += blockDim.x)
         cur[p] = (uint32_t)rp.pstart[p] + rp.base[cw * P + p];
     __syncthreads();
     const uint32_t hmask = rp.hslots - 1;
     const uint64_t cap = T.plan->cap;
-    const uint64_t r0 = (uint64_t)blockIdx.x * rows_per_wg;
-    const uint64_t r1 = min(b.nrows, r0 + rows_per_wg);
     const uint32_t shift = 64 - rp.part_bits;
     const int vw = vmask_word_s<S>(rp);
     const uint32_t lane = threadIdx.x & 63;
@@ -513,6 +511,8 @@ __global__ __launch_bounds__(RP_BLOCK) void k_rp_scatter(const DevCtx *__restric
     if (threadIdx.x < SC_PROD) {
         // ---- producers: filter, key, hash; heavy rows into LDS state, cold rows into the ring
         bool failed = false;
+        for_batch_rows(br, [&](const DevBatch &bb, uint64_t r0, uint64_t r1) {
+        const DevBatch b = bb;  // column pointers stay in registers
         auto scatter_row = [&](const Row &row) {
             if (sh.npred() && !eval_preds_s(sh, q, b, row)) return;
             uint64_t w;
@@ -567,6 +567,7 @@ __global__ __launch_bounds__(RP_BLOCK) void k_rp_scatter(const DevCtx *__restric
             __hip_atomic_store(&rpub[i], n + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
         };
         for_rows_pipelined<S::kStatic ? AGG_ROWS : 1, SC_PROD>(sh, b, r0, r1, sh.nu(), threadIdx.x, scatter_row);
+        });
         if (failed) __hip_atomic_fetch_or(&T.plan->overflow, OVF_RING, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (lane == 0) __hip_atomic_fetch_add(&rctl[1], 64u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
     } else {

@@ -133,6 +133,10 @@ _SIGS = {
                                        C.POINTER(C.c_uint64)]),
     "gbagg_parquet_decode_range": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p,
                                              C.POINTER(C.c_uint64)]),
+    "gbagg_parquet_decode_async": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p,
+                                             C.POINTER(C.c_uint64)]),
+    "gbagg_parquet_sync": (C.c_int, [C.c_void_p]),
+    "gbagg_parquet_set_flush_rows": (C.c_int, [C.c_void_p, C.c_uint64]),
     "gbagg_parquet_upload": (C.c_int, [C.c_void_p]),
     "gbagg_parquet_reset": (C.c_int, [C.c_void_p]),
     "gbagg_parquet_close": (None, [C.c_void_p]),
@@ -709,6 +713,24 @@ class ParquetFile:
         _check(lib().gbagg_parquet_decode_range(self._h, rg, last, idx.ctypes.data, len(cols), C.addressof(views),
                                                 C.byref(n)))
         return [DecodedColumn(views[i], n.value) for i in range(len(cols))]
+
+    def decode_async(self, rg, cols=None):
+        """Queued decode of row group rg (gbagg_parquet_decode_async): its
+        DecodedColumns, whose device values are there after sync()."""
+        cols = list(range(self.num_columns)) if cols is None else list(cols)
+        idx = np.array(cols or [0], np.uint32)
+        views = (ColumnView * max(1, len(cols)))()
+        n = C.c_uint64()
+        _check(lib().gbagg_parquet_decode_async(self._h, rg, idx.ctypes.data, len(cols), C.addressof(views),
+                                                C.byref(n)))
+        return [DecodedColumn(views[i], n.value) for i in range(len(cols))]
+
+    def sync(self):
+        """Runs the queued decodes and waits for them; raises what their kernels found."""
+        _check(lib().gbagg_parquet_sync(self._h))
+
+    def set_flush_rows(self, rows):
+        _check(lib().gbagg_parquet_set_flush_rows(self._h, rows))
 
     def close(self):
         if self._h:

@@ -115,6 +115,42 @@ def test_plain_host_batches_rowgroups():
     op.close()
 
 
+@pytest.mark.parametrize("shape,strategy", [("highcard", G.AUTO), ("highcard", G.RADIX), ("highcard", G.LDS_HASH),
+                                            ("highcard", G.GLOBAL_HASH), ("plain", G.DIRECT), ("plain", G.AUTO)])
+def test_many_uneven_batches(shape, strategy):
+    # batches of 0 to ~90,000 rows, empty and one-row ones among them, every
+    # pass of every strategy one launch over all of them: workgroups whose
+    # rows span several batches, and batches spread over several workgroups;
+    # device batches, then the same rows as host batches
+    n = 1_500_000
+    if shape == "highcard":
+        q, cols = datagen.highcard_workload(n=n, nkeys=300_000)
+    else:
+        q, cols = datagen.plain_workload(n=n)
+    q.strategy = strategy
+    rng = np.random.default_rng(11 + strategy)
+    cuts = sorted(set(rng.integers(0, n, 40).tolist()) | {0, 1, 2, 2, 700_000, 700_001, n, n})
+    cuts += [n] * 3  # empty batches at the end too
+    bounds = list(zip(cuts[:-1], cuts[1:]))
+    print(f"[parity] {shape} strategy {strategy}: {len(bounds)} batches, sizes {[b - a for a, b in bounds][:12]}...",
+          flush=True)
+    exp, types = _exact(q, cols)
+    for device in (True, False):
+        op = G.GpuHashAggregate(q)
+        op.Init()
+        for a, b in bounds:
+            part = [G.Column(c.type, c.values[a:b]) for c in cols]
+            if device:
+                op.consume_device([G.DeviceColumn(c) for c in part])
+            else:
+                op.consume(part)
+        op.finalize()
+        info = op.plan_info()
+        assert info["rows_consumed"] == n
+        assert_same_groups(op.result().rows(), exp, 1, types)
+        op.close()
+
+
 @pytest.mark.parametrize("n,nkeys", [(2_000_000, 1_000_000), (4_000_000, 10_000_000)])
 def test_highcard(n, nkeys):
     q, cols = datagen.highcard_workload(n=n, nkeys=nkeys)

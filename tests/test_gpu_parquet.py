@@ -52,6 +52,21 @@ def _expected(col, tcol):
     return vals, valid
 
 
+def _check_row_group(cols, t, row, n, rg):
+    for c, dc in enumerate(cols):
+        exp = _expected(c, t.column(c).slice(row, n))
+        if dc.type == G.VARCHAR:
+            assert dc.strings() == exp, f"column {c} row group {rg}"
+        else:
+            vals, valid = dc.to_host()
+            ev, evalid = exp
+            if valid is None:
+                assert evalid.all()
+                valid = np.ones(n, bool)
+            assert np.array_equal(valid.astype(bool), evalid), f"validity of column {c} row group {rg}"
+            assert np.array_equal(vals[evalid], ev[evalid].astype(vals.dtype)), f"column {c} row group {rg}"
+
+
 def _check_file(image, t, rg_rows, resident=False):
     f = G.ParquetFile(image)
     if resident:
@@ -62,19 +77,7 @@ def _check_file(image, t, rg_rows, resident=False):
     for rg in range(f.num_row_groups):
         n = f.row_group_rows(rg)
         print(f"[parquet] row group {rg} rows {n}", flush=True)
-        cols = f.decode(rg)
-        for c, dc in enumerate(cols):
-            exp = _expected(c, t.column(c).slice(row, n))
-            if dc.type == G.VARCHAR:
-                assert dc.strings() == exp, f"column {c} row group {rg}"
-            else:
-                vals, valid = dc.to_host()
-                ev, evalid = exp
-                if valid is None:
-                    assert evalid.all()
-                    valid = np.ones(n, bool)
-                assert np.array_equal(valid.astype(bool), evalid), f"validity of column {c} row group {rg}"
-                assert np.array_equal(vals[evalid], ev[evalid].astype(vals.dtype)), f"column {c} row group {rg}"
+        _check_row_group(f.decode(rg), t, row, n, rg)
         row += n
     assert row == t.num_rows
     # every row group at once: one concatenated batch, dictionaries merged
@@ -227,3 +230,118 @@ def test_random_files_match_pyarrow():
         print(f"[parquet] random case {case}: rows {n} schema {t.schema.types} {kw}", flush=True)
         image = _write(t, **kw)
         _check_file(image, t, kw["row_group_size"], resident=bool(case % 2))
+
+
+@pytest.mark.parametrize("resident,flush_rows", [(True, 1), (False, 100_000), (True, 1 << 40)])
+def test_queued_row_groups_match_pyarrow(resident, flush_rows):
+    # gbagg_parquet_decode_async: every row group queued, launched every
+    # flush_rows rows (1: each on its own; 2^40: all at the sync), then read
+    t = _table(300_000, 5)
+    image = _write(t, row_group_size=40_000, data_page_size=32 * 1024, data_page_version="2.0",
+                   dictionary_pagesize_limit=64 * 1024, use_dictionary=["a", "b", "k", "d", "u", "s"])
+    f = G.ParquetFile(image)
+    if resident:
+        f.upload()
+    f.set_flush_rows(flush_rows)
+    queued = [f.decode_async(rg) for rg in range(f.num_row_groups)]
+    f.sync()
+    row = 0
+    for rg, cols in enumerate(queued):
+        n = f.row_group_rows(rg)
+        assert all(c.n == n for c in cols)
+        print(f"[parquet] queued row group {rg} rows {n}", flush=True)
+        _check_row_group(cols, t, row, n, rg)
+        row += n
+    assert row == t.num_rows
+    # a synchronous decode after queued ones, and queued ones dropped by reset
+    _check_row_group(f.decode(1), t, 40_000, 40_000, 1)
+    f.decode_async(0)
+    f.reset()
+    f.sync()
+    f.close()
+
+
+def test_queued_row_groups_into_the_operator():
+    t = _table(400_000, 6)
+    image = _write(t, row_group_size=122_880, data_page_size=64 * 1024)
+    f = G.ParquetFile(image)
+    f.upload()
+    types = [c[1] for c in f.columns]
+    nullable = [c[2] for c in f.columns]
+    q = G.Query(types, [0, 1], [(G.COUNT_STAR, None), (G.SUM, 2), (G.AVG, 4), (G.MAX, 3)],
+                predicates=[(2, G.LT, 30)], nullable=nullable, drop_null_keys=True)
+    cols = _host_columns(t, types, nullable)
+    c_, v_, ty, ov = O.groupby_exact(q, cols)
+    exp = oracle_rows(c_, v_, ty, {1: cols[1].dictionary})
+    op = G.GpuHashAggregate(q)
+    op.Init()
+    for rg in range(f.num_row_groups):
+        op.consume_device(f.decode_async(rg))  # recorded; read at finalize
+    f.sync()
+    op.finalize()
+    assert_same_groups(op.result().rows(), exp, 2, ty)
+    op.close()
+    f.close()
+
+
+def _dictionary_page_num_values(image, offset):
+    """Byte range of the num_values varint of the dictionary page header at
+    `offset` (Thrift compact PageHeader: field 7 is the DictionaryPageHeader,
+    whose field 1 is num_values)."""
+    p = offset
+
+    def varint():
+        nonlocal p
+        v, sh = 0, 0
+        while True:
+            b = image[p]
+            p += 1
+            v |= (b & 0x7F) << sh
+            sh += 7
+            if not b & 0x80:
+                return v
+
+    fid = 0
+    while True:
+        h = image[p]
+        p += 1
+        assert h, "no dictionary page header"
+        fid = fid + (h >> 4) if h >> 4 else None
+        assert fid is not None, "long-form field header"
+        if (h & 15) == 5:
+            varint()
+        elif (h & 15) == 12 and fid == 7:
+            h2 = image[p]
+            p += 1
+            assert h2 == 0x15, "num_values is the first field"
+            start = p
+            n = varint() >> 1
+            return start, p, n
+        else:
+            raise AssertionError(f"unexpected field type {h & 15}")
+
+
+def test_queued_kernel_errors_reported_by_sync():
+    # a dictionary page that claims fewer entries than the data pages index:
+    # the host accepts the page (its values fit), the kernels find indices
+    # outside the dictionary, and the queued decode reports it at the sync
+    rng = np.random.default_rng(7)
+    t = pa.table({"x": pa.array(rng.integers(0, 100, 50_000).astype(np.int32))},
+                 schema=pa.schema([pa.field("x", pa.int32(), nullable=False)]))
+    image = bytearray(_write(t, row_group_size=50_000, use_dictionary=True))
+    meta = pq.ParquetFile(io.BytesIO(bytes(image))).metadata
+    off = meta.row_group(0).column(0).dictionary_page_offset
+    start, end, n = _dictionary_page_num_values(image, off)
+    assert n == 100 and end - start == 2  # zigzag 200: two bytes
+    image[start:end] = bytes([0x8C, 0x01])  # 70 entries, same two bytes
+    for queued in (True, False):
+        f = G.ParquetFile(bytes(image))
+        if queued:
+            f.decode_async(0)
+            with pytest.raises(G.GbaggError, match="outside the dictionary"):
+                f.sync()
+            f.sync()  # reported once
+        else:
+            with pytest.raises(G.GbaggError, match="outside the dictionary"):
+                f.decode(0)
+        f.close()
