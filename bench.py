@@ -17,9 +17,19 @@ bytes = 28 B/row of input read once, over the kernel's average duration
 measured with HIP events on the operator's stream (plan_info ms_aggregate).
 `traffic` is the HBM bytes one launch moves, from rocprofv3 FETCH_SIZE and
 WRITE_SIZE, read from profiles/k_rp_scatter_highcard_100m_traffic.json.
-`cpu_baseline` times the restated reference CPU path (oracle/oracle.c
+`cpu_baseline` times the reference's own engine (kind "reference"): its
+sources compiled into oracle/_ref/ref_query (oracle/Makefile, target cli),
+which runs the same query through the engine's C API on the first 20M rows of
+the workload written as Parquet (the engine's fused Parquet scan + per-thread
+aggregation + merge + result materialization, its default 8 threads; median
+of 3 runs after a warm-up run). The restated CPU path (oracle/oracle.c
 orc_port_groupby, the per-thread-map path of physical_planner.cpp:7613-7931)
-with the reference's own 8-thread cap on the same 100M rows.
+on the same rows in memory is reported beside it. Without oracle/_ref/ref_query
+the restatement is the baseline (kind "port").
+
+`--ref-calibration` times the reference engine and the restatement on the
+three aggregate configurations (plain 1M, high-cardinality 20M of 100M,
+composite 5M of 50M), median of 5 runs each, then exits.
 
 `--copy-calibration` instead copies 1 GiB device to device three times with
 hipMemcpy: the reference point that scales the rocprofv3 byte counters.
@@ -49,6 +59,7 @@ def main():
     ap.add_argument("--strategy", type=int, default=0)
     ap.add_argument("--cpu-sample-rows", type=int, default=100_000_000)
     ap.add_argument("--cpu-threads", type=int, default=8)
+    ap.add_argument("--ref-sample-rows", type=int, default=20_000_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
@@ -59,6 +70,9 @@ def main():
     ap.add_argument("--cpu-calibration", action="store_true",
                     help="time the CPU restatement against the reference's own RadixCountAgg "
                          "(oracle/_ref, built from the reference sources), then exit")
+    ap.add_argument("--ref-calibration", action="store_true",
+                    help="time the reference engine (oracle/_ref/ref_query, built from the reference "
+                         "sources) and the CPU restatement on the three aggregate configurations, then exit")
     ap.add_argument("--batch-split", action="store_true",
                     help="time the batched mix split by the queries' distinct keys, then exit")
     ap.add_argument("--param", action="append", default=[], metavar="NAME=VALUE",
@@ -72,6 +86,9 @@ def main():
         return
     if args.batch_split:
         batch_split()
+        return
+    if args.ref_calibration:
+        ref_calibration(args.cpu_threads)
         return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (use --gpus 1)", file=sys.stderr)
@@ -133,12 +150,18 @@ def main():
         t1 = time.perf_counter()
         O.port_groupby(q, sample, args.cpu_threads)
         cpu_s = time.perf_counter() - t1
-        cpu = {"value": m / cpu_s, "unit": "rows/s", "cores": args.cpu_threads,
-               "host_cores": len(os.sched_getaffinity(0)),
-               "kind": "port",
-               "sample": f"first {m} rows of the same workload, restated per-thread-map path "
-                         f"(physical_planner.cpp:7613-7931), {cpu_s:.2f} s wall"}
+        port = {"value": m / cpu_s, "unit": "rows/s", "cores": args.cpu_threads, "kind": "port",
+                "sample": f"first {m} rows of the same workload, restated per-thread-map path "
+                          f"(physical_planner.cpp:7613-7931), in memory, {cpu_s:.2f} s wall"}
         print(f"[bench] cpu port {m} rows in {cpu_s:.2f}s", file=sys.stderr, flush=True)
+        ref = reference_time("highcard", q, cols, min(args.ref_sample_rows, args.rows), reps=3)
+        if ref:
+            cpu = {"value": ref["rows_per_s"], "unit": "rows/s", "cores": ref["threads"],
+                   "host_cores": len(os.sched_getaffinity(0)), "kind": "reference",
+                   "sample": ref["sample"], "scan_aggregate_s": ref["scan_aggregate_s"], "port": port}
+            print(f"[bench] cpu reference {ref['rows']} rows: {ref['median_s']:.2f}s", file=sys.stderr, flush=True)
+        else:
+            cpu = dict(port, host_cores=len(os.sched_getaffinity(0)))
 
     configs = None if args.no_configs else bench_configs()
     batched = None if args.no_batch else bench_batched(args.batch_queries)
@@ -433,6 +456,107 @@ def cpu_calibration(n, nkeys, threads):
         "workload": f"COUNT(*) GROUP BY k, {n} rows, high-cardinality keys", "threads": threads,
         "reference_radix_count_s": min(t_ref), "port_radix_count_s": min(t_port),
         "port_over_reference": min(t_port) / min(t_ref), "same_result": same}}), flush=True)
+
+
+REF_QUERY = os.path.join(ROOT, "oracle", "_ref", "ref_query")
+# Parquet column names and the query, per configuration (the columns of the
+# generated workloads, in order)
+REF_CONFIGS = {
+    "plain": (("k", "v"), "SELECT k, COUNT(*), SUM(v), MIN(v), MAX(v) FROM '{f}' GROUP BY k"),
+    "highcard": (("k", "v1", "v2", "v3"),
+                 "SELECT k, COUNT(*), SUM(v1), MIN(v2), MAX(v2), AVG(v3) FROM '{f}' GROUP BY k"),
+    "composite": (("a", "b", "c"),
+                  "SELECT a, b, COUNT(*), SUM(c) FROM '{f}' WHERE c < 30 GROUP BY a, b "
+                  "ORDER BY COUNT(*) DESC LIMIT 10"),
+}
+
+
+def _arrow(col, n):
+    """The first n rows of a workload column as a pyarrow array (VARCHAR codes
+    with their dictionary as a dictionary-encoded string column)."""
+    import numpy as np
+    import pyarrow as pa
+    v = col.values[:n]
+    if col.dictionary is not None:
+        d = col.dictionary
+        dic = pa.StringArray.from_buffers(len(d), pa.py_buffer(d.offsets.astype(np.int32)), pa.py_buffer(d.data))
+        return pa.DictionaryArray.from_arrays(pa.array(v.astype(np.int32)), dic)
+    return pa.array(v)
+
+
+def reference_time(config, q, cols, n, reps):
+    """The reference engine's own time for configuration `config` on the first
+    n rows: the rows written as Parquet (uncompressed, 122,880-row row groups)
+    to a temporary directory, then oracle/_ref/ref_query runs the query
+    reps + 1 times through the engine's C API (SLOTH_PROFILE=1; the first run
+    warms the page cache and is dropped). None when ref_query or pyarrow is
+    missing."""
+    import shutil
+    import statistics
+    import subprocess
+    import tempfile
+    if not os.path.exists(REF_QUERY):
+        return None
+    try:
+        import pyarrow as pa
+        import pyarrow.parquet as pq
+    except ImportError:
+        return None
+    names, sql = REF_CONFIGS[config]
+    tmp = tempfile.mkdtemp(prefix="slothdb_ref_")
+    try:
+        path = os.path.join(tmp, config + ".parquet")
+        pq.write_table(pa.table({nm: _arrow(c, n) for nm, c in zip(names, cols)}), path,
+                       compression="NONE", row_group_size=122_880)
+        r = subprocess.run([REF_QUERY, str(reps + 1), sql.format(f=path)], env=dict(os.environ, SLOTH_PROFILE="1"),
+                           stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, timeout=900)
+        if r.returncode != 0:
+            print(f"[bench] reference engine failed: {r.stderr.strip()[-300:]}", file=sys.stderr, flush=True)
+            return None
+        runs = [l.split() for l in r.stdout.splitlines() if l.startswith("run ")]
+        secs = [float(x[2]) for x in runs][1:]
+        walls = [float(l.split("wall=")[1].split("ms")[0]) * 1e-3 for l in r.stderr.splitlines()
+                 if "RunParallelRGs wall=" in l][1:]
+        threads = [int(l.split(" nt=")[1].split()[0]) for l in r.stderr.splitlines() if " nt=" in l]
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    med = statistics.median(secs)
+    return {"rows": n, "median_s": med, "runs_s": secs, "rows_per_s": n / med,
+            "scan_aggregate_s": statistics.median(walls) if walls else None,
+            "threads": threads[0] if threads else None, "result_rows": int(runs[-1][3]),
+            "sample": f"first {n} rows of the same workload as Parquet (uncompressed, 122880-row row groups); "
+                      f"the reference engine (oracle/_ref/ref_query, built from its sources) runs the query "
+                      f"through its C API: fused Parquet scan, per-thread aggregation, merge, result "
+                      f"materialized; median of {reps} runs after a warm-up run"}
+
+
+def ref_calibration(threads):
+    """The reference engine against the CPU restatement (orc_port_groupby) on
+    the three aggregate configurations, median of 5 runs each. The reference
+    reads the rows from Parquet and materializes its result; the restatement
+    reads them from memory. Prints one JSON line."""
+    import statistics
+    from oracle import oracle as O
+    from slothdb_amd import datagen, gbagg as G
+    out = {"host_cores": len(os.sched_getaffinity(0)), "port_threads": threads}
+    for config, (q, cols), n in (("plain", datagen.plain_workload(n=1_000_000), 1_000_000),
+                                 ("highcard", datagen.highcard_workload(n=20_000_000), 20_000_000),
+                                 ("composite", datagen.composite_workload(n=5_000_000), 5_000_000)):
+        ref = reference_time(config, q, cols, n, reps=5)
+        pm = None
+        if config != "composite":  # the restatement covers one integer group column
+            sample = [G.Column(c.type, c.values[:n]) for c in cols]
+            port = []
+            for _ in range(5):
+                t = time.perf_counter()
+                O.port_groupby(q, sample, threads)
+                port.append(time.perf_counter() - t)
+            pm = statistics.median(port)
+        out[config] = {"rows": n, "reference": ref, "port_median_s": pm, "port_rows_per_s": n / pm if pm else None,
+                       "port_over_reference": (pm / ref["median_s"]) if ref and pm else None}
+        print(f"[bench] {config}: reference {ref['median_s'] if ref else float('nan'):.3f}s, "
+              f"port {pm if pm else float('nan'):.3f}s", file=sys.stderr, flush=True)
+    print(json.dumps({"ref_calibration": out}), flush=True)
 
 
 def batch_split(nq=512):

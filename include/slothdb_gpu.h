@@ -406,10 +406,17 @@ gbagg_status gbagg_batch_set_tune(gbagg_batch *b, const gbagg_batch_tune *tune);
  * "mode_rule": which queries aggregate each row tile in an LDS table before
  * their global table: 0 = those whose distinct estimate fits it, 1 = all,
  * 2 (default) = those, and those whose keys repeat within a tile (first
- * tile's estimate at most half its rows). Results never depend on it.
+ * tile's estimate at most half its rows); 3 = none: every query whose keys
+ * 1024 partitions hold takes radix partitions. Results never depend on it.
+ * "radix": 1 (default) = a query that is not LDS-first writes its rows as
+ * records to radix partitions of its key hash (up to 1024, sized from its
+ * HyperLogLog estimate), each then aggregated in one LDS table, instead of
+ * updating a global table row by row; 0 = the global table.
+ * "part_slots": LDS table slots per radix partition (default 2048, 256-8192,
+ * a multiple of 64). Results never depend on either.
  * "selector": 1 (default) = each query's mode follows the learned selector's
  * path for its input (DIRECT or LDS_HASH: LDS-first; GLOBAL_HASH or RADIX:
- * global table; planned on the host once per input, from the same row sample
+ * not LDS-first; planned on the host once per input, from the same row sample
  * and features as the single-query operator's AUTO plan) and its predicted
  * distinct count may enlarge the table, which never gets smaller than the
  * HyperLogLog estimate sizes it; mode_rule applies to queries whose

@@ -3277,6 +3277,8 @@ struct gbagg_batch {
     BatchTune tune{65536, 2048, 512, 2};
     uint32_t cap_pct = 150;  // table slots per estimated key (percent)
     uint32_t mode_rule = 2;  // BatchCtl::mode_rule (2: LDS-first also for keys that repeat within a tile)
+    uint32_t radix = 1;      // BatchCtl::radix: queries not LDS-first take radix partitions
+    uint32_t part_slots = 2048;  // k_b_part LDS table slots
     uint32_t use_selector = 1;  // per query: the learned selector's path picks the mode
     // selector plans per input (key column, rows, query shape): (distinct, mode) or mode ~0u when not usable
     std::unordered_map<uint64_t, std::pair<uint64_t, uint32_t>> sel_cache;
@@ -3288,7 +3290,7 @@ struct gbagg_batch {
     std::vector<uint2> items;
     std::vector<std::vector<gbagg_type>> out_types;  // per batched query: key, aggs
     uint32_t nacc_max = 1;
-    DevBuf dq, ditems, dctl, keys, acc, out_key, out_acc, hll;
+    DevBuf dq, ditems, dctl, keys, acc, out_key, out_acc, hll, bhist, pstart, recs;
     uint64_t pool_slots = 0;
     // graph of the current shape
     hipGraph_t graph = nullptr;
@@ -3375,7 +3377,7 @@ void batch_prepare(gbagg_batch *b, const gbagg_batch_item *items, uint32_t count
     b->items.clear();
     b->batched_index.assign(count, -1);
     b->nacc_max = 1;
-    uint64_t pool = 0, sig = 0xcbf29ce484222325ull;
+    uint64_t pool = 0, rec_words = 0, sig = 0xcbf29ce484222325ull;
     for (uint32_t i = 0; i < count; i++) {
         if (!batchable(items[i])) continue;
         const gbagg_query_desc *q = items[i].query;
@@ -3386,6 +3388,7 @@ void batch_prepare(gbagg_batch *b, const gbagg_batch_item *items, uint32_t count
         bq.rows = items[i].num_rows;
         bq.nacc = q->num_aggs;
         std::vector<gbagg_type> types{q->column_types[kc]};
+        bq.rec_words = 1;
         for (uint32_t a = 0; a < q->num_aggs; a++) {
             const gbagg_agg &g = q->aggs[a];
             int op = ACC_CNT_STAR;
@@ -3397,6 +3400,8 @@ void batch_prepare(gbagg_batch *b, const gbagg_batch_item *items, uint32_t count
                 bq.val[a] = items[i].columns[g.column].data;
                 if (q->column_types[g.column] == GBAGG_TYPE_BIGINT) bq.val64 |= 1u << a;
                 if (g.kind != GBAGG_AGG_SUM) t = q->column_types[g.column];
+                bq.valmask |= 1u << a;
+                bq.rec_words++;
             }
             bq.ops |= (uint32_t)op << (4 * a);
             types.push_back(t);
@@ -3436,6 +3441,7 @@ void batch_prepare(gbagg_batch *b, const gbagg_batch_item *items, uint32_t count
             }
         }
         pool += batch_pool_bound(bq.rows, b->cap_pct);
+        rec_words += b->radix || b->mode_rule == 3 ? bq.rows * bq.rec_words : 0;
         sig = mix_sig(sig, (uint64_t)(uintptr_t)bq.key);
         sig = mix_sig(sig, bq.rows);
         sig = mix_sig(sig, bq.ops | (uint64_t)bq.val64 << 32);
@@ -3449,6 +3455,8 @@ void batch_prepare(gbagg_batch *b, const gbagg_batch_item *items, uint32_t count
     sig = mix_sig(sig, tn.wgs_per_cu);
     sig = mix_sig(sig, b->cap_pct);
     sig = mix_sig(sig, b->mode_rule);
+    sig = mix_sig(sig, b->radix);
+    sig = mix_sig(sig, b->part_slots);
     sig = mix_sig(sig, b->use_selector);
     const uint32_t nq = (uint32_t)b->hq.size();
     if (nq == 0) {
@@ -3467,12 +3475,20 @@ void batch_prepare(gbagg_batch *b, const gbagg_batch_item *items, uint32_t count
     b->out_key.ensure(pool * 8);
     b->out_acc.ensure(pool * 8 * BQ_MAXACC);
     b->hll.ensure(std::max<size_t>(1, b->items.size()) * BQ_HLL_M);
+    b->bhist.ensure(std::max<size_t>(1, b->items.size()) * BQ_RBUCKETS * 4);
+    b->pstart.ensure((size_t)nq * (BQ_RBUCKETS + 1) * 4);
+    // records of every query that may take radix partitions (the modes are
+    // picked on the device)
+    b->recs.ensure(std::max<uint64_t>(1, rec_words) * 8);
     BatchCtl ctl{};
     ctl.pool_slots = pool;
     ctl.n_items = (uint32_t)b->items.size();
     ctl.lds_slots = tn.lds_slots;
     ctl.cap_pct = b->cap_pct;
     ctl.mode_rule = b->mode_rule;
+    ctl.radix = b->radix;
+    ctl.part_slots = b->part_slots;
+    ctl.rec_cap = rec_words;
     HIPCHK(hipMemcpyAsync(b->dq.ptr, b->hq.data(), nq * sizeof(BatchQuery), hipMemcpyHostToDevice, b->stream));
     if (!b->items.empty())
         HIPCHK(hipMemcpyAsync(b->ditems.ptr, b->items.data(), b->items.size() * sizeof(uint2), hipMemcpyHostToDevice,
@@ -3480,10 +3496,15 @@ void batch_prepare(gbagg_batch *b, const gbagg_batch_item *items, uint32_t count
     HIPCHK(hipMemcpyAsync(b->dctl.ptr, &ctl, sizeof(ctl), hipMemcpyHostToDevice, b->stream));
     HIPCHK(hipStreamSynchronize(b->stream));
 
-    BatchPool bp{b->keys.as<uint64_t>(), b->acc.as<uint64_t>(), b->out_key.as<int64_t>(), b->out_acc.as<int64_t>()};
-    const size_t agg_lds = (size_t)tn.lds_slots * 8 * (1 + b->nacc_max);
+    BatchPool bp{b->keys.as<uint64_t>(), b->acc.as<uint64_t>(), b->out_key.as<int64_t>(), b->out_acc.as<int64_t>(),
+                 b->bhist.as<uint32_t>(), b->pstart.as<uint32_t>(), b->recs.as<uint64_t>()};
+    const size_t agg_lds = std::max<size_t>((size_t)tn.lds_slots * 8 * (1 + b->nacc_max), BQ_RBUCKETS * 4);
     if (agg_lds > di.lds_per_block) fail(GBAGG_INVALID, "batch tune: LDS table larger than the device allows");
     allow_lds(k_b_agg, agg_lds);
+    const size_t part_lds = (size_t)b->part_slots * 8 * (1 + b->nacc_max);
+    if (part_lds > di.lds_per_block) fail(GBAGG_INVALID, "batch: partition LDS table larger than the device allows");
+    allow_lds(k_b_part, part_lds);
+    const uint32_t part_per_cu = (uint32_t)std::max<size_t>(1, std::min<size_t>(4, di.lds_per_cu / part_lds));
     const uint32_t chunks = (uint32_t)std::min<uint64_t>((pool + BQ_CHUNK - 1) / BQ_CHUNK, (uint64_t)di.cus * 16);
     BatchQuery *dqp = b->dq.as<BatchQuery>();
     BatchCtl *dctl = b->dctl.as<BatchCtl>();
@@ -3491,14 +3512,17 @@ void batch_prepare(gbagg_batch *b, const gbagg_batch_item *items, uint32_t count
     HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
     const uint32_t hgrid = (uint32_t)std::min<size_t>(std::max<size_t>(1, b->items.size()), (size_t)di.cus * 8);
     hipLaunchKernelGGL(k_b_hll, dim3(hgrid), dim3(256), 0, b->stream, (const BatchQuery *)dqp, (const BatchCtl *)dctl,
-                       (const uint2 *)b->ditems.ptr, b->hll.as<uint8_t>(), tn);
+                       (const uint2 *)b->ditems.ptr, b->hll.as<uint8_t>(), b->bhist.as<uint32_t>(), tn);
     hipLaunchKernelGGL(k_b_estimate, dim3(nq), dim3(256), 0, b->stream, dqp, (const BatchCtl *)dctl,
                        (const uint8_t *)b->hll.as<uint8_t>());
     hipLaunchKernelGGL(k_b_layout, dim3(1), dim3(1024), 0, b->stream, dqp, nq, dctl);
+    hipLaunchKernelGGL(k_b_rscan, dim3(nq), dim3(1024), 0, b->stream, (const BatchQuery *)dqp, bp);
     hipLaunchKernelGGL(k_b_init, dim3(chunks), dim3(BQ_CHUNK), 0, b->stream, (const BatchQuery *)dqp, nq,
                        (const BatchCtl *)dctl, bp);
     hipLaunchKernelGGL(k_b_agg, dim3((uint32_t)di.cus * tn.wgs_per_cu), dim3(tn.block), agg_lds, b->stream, dqp, dctl,
                        (const uint2 *)b->ditems.ptr, bp, tn);
+    hipLaunchKernelGGL(k_b_part, dim3((uint32_t)di.cus * part_per_cu), dim3(BQ_PART_BLOCK), part_lds, b->stream, dqp,
+                       nq, dctl, bp);
     hipLaunchKernelGGL(k_b_compact, dim3(chunks), dim3(BQ_CHUNK), 0, b->stream, dqp, nq, (const BatchCtl *)dctl, bp);
     HIPCHK(hipStreamEndCapture(b->stream, &b->graph));
     HIPCHK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
@@ -3632,8 +3656,14 @@ gbagg_status gbagg_batch_set_param(gbagg_batch *b, const char *name, int64_t val
             if (value < 1 || value > 1000) fail(GBAGG_INVALID, "cap_pct out of range [1, 1000]");
             b->cap_pct = (uint32_t)value;
         } else if (n == "mode_rule") {
-            if (value < 0 || value > 2) fail(GBAGG_INVALID, "mode_rule out of range [0, 2]");
+            if (value < 0 || value > 3) fail(GBAGG_INVALID, "mode_rule out of range [0, 3]");
             b->mode_rule = (uint32_t)value;
+        } else if (n == "radix") {
+            if (value < 0 || value > 1) fail(GBAGG_INVALID, "radix out of range [0, 1]");
+            b->radix = (uint32_t)value;
+        } else if (n == "part_slots") {
+            if (value < 256 || value > 8192 || value % 64) fail(GBAGG_INVALID, "part_slots out of range [256, 8192] or not a multiple of 64");
+            b->part_slots = (uint32_t)value;
         } else if (n == "selector") {
             if (value < 0 || value > 1) fail(GBAGG_INVALID, "selector out of range [0, 1]");
             b->use_selector = (uint32_t)value;

@@ -4,30 +4,45 @@
 //   SELECT k, agg(v)... FROM t_i GROUP BY k
 // with one INTEGER/BIGINT key and up to four COUNT/SUM/MIN/MAX aggregates over
 // INTEGER/BIGINT columns, no NULLs, no predicates, no TopN. The whole batch is
-// six kernels captured once in a hipGraph and replayed:
+// eight kernels captured once in a hipGraph and replayed:
 //
-//   k_b_hll       per row tile: HyperLogLog registers (p = 10) of its keys
+//   k_b_hll       per row tile: HyperLogLog registers (p = 10) of its keys and
+//                 a histogram of the keys' top 10 hash bits
 //   k_b_estimate  one block per query: merged registers -> distinct estimate,
-//                 which sizes the query's hash table and picks its mode (LDS
+//                 which sizes the query's hash table and picks its mode: LDS
 //                 first when the query's keys fit the LDS table or repeat
-//                 within a tile, by the first tile's own estimate; else global);
-//                 a query the learned selector planned on the host takes the
-//                 selector's mode, and its predicted count may enlarge the table
-//   k_b_layout    one block: table offsets (prefix sum of capacities)
-//   k_b_init      fills every table with empty keys and identity states
+//                 within a tile (by the first tile's own estimate), else radix
+//                 partitions (else, beyond what 1024 partitions hold, the
+//                 global table); a query the learned selector planned on the
+//                 host takes the selector's mode, and its predicted count may
+//                 enlarge the table
+//   k_b_layout    one block: table offsets, record regions and partition work
+//                 items (prefix sums)
+//   k_b_rscan     one block per radix query: exact partition offsets per row
+//                 tile from the histograms
+//   k_b_init      fills every global table with empty keys and identity states
 //   k_b_agg       persistent work queue over (query, tile) items: LDS-first
 //                 queries aggregate a tile in an LDS table and merge it into
-//                 the query's global table; high-cardinality queries update the
-//                 global table directly. Every wave leaves through the same
-//                 item-counter check.
-//   k_b_compact   per 256-slot chunk of a table: occupied records to the
-//                 query's dense result rows (one reservation per block)
+//                 the query's global table; radix queries write each row as a
+//                 record (key, values) at its partition's exact offset;
+//                 global-table queries update the table directly. Every wave
+//                 leaves through the same item-counter check.
+//   k_b_part      persistent work queue over the radix queries' partitions:
+//                 all groups of one partition in an LDS table, then straight
+//                 to the query's dense result rows
+//   k_b_compact   per 256-slot chunk of a global table: occupied records to
+//                 the query's dense result rows (one reservation per block)
 //
 // Tables hold ceil(1.5 * estimate) + 64 slots rounded up to 256 (multiply-shift
-// home slot, linear probing, bounded); a query whose table overflows is flagged
-// and the host re-runs it on the single-query operator. So is a query with the
-// key INT64_MIN, which equals the tables' empty-slot marker (the single-query
-// operator keeps that key in a side record).
+// home slot, linear probing, bounded); a radix query's result rows use the
+// same capacity. A query whose table, partition table or result rows overflow
+// is flagged and the host re-runs it on the single-query operator. So is a
+// query with the key INT64_MIN, which equals the tables' empty-slot marker
+// (the single-query operator keeps that key in a side record).
+//
+// Per-row global atomics run at a fixed rate on this GPU, whatever the table
+// size or the atomic's scope; a radix query instead moves each row twice
+// through HBM as a record and does all of its hashing and atomics in LDS.
 #pragma once
 
 #include "gbagg_kernels.hpp"
@@ -37,8 +52,11 @@ namespace gbagg {
 constexpr int BQ_MAXACC = 4;
 constexpr uint32_t BQ_CHUNK = 256;           // table slots per compaction block (capacities are multiples)
 constexpr uint32_t BQ_MAX_PROBE = 128;
+constexpr uint32_t BQ_RBITS = 10;                 // radix: histogram of the top 10 hash bits per tile
+constexpr uint32_t BQ_RBUCKETS = 1u << BQ_RBITS;  // (partitions = the top pbits <= 10 of them)
+constexpr uint32_t BQ_PART_BLOCK = 512;           // k_b_part threads
 
-enum BatchMode : uint32_t { BM_LDS = 0, BM_GLOBAL = 1 };
+enum BatchMode : uint32_t { BM_LDS = 0, BM_GLOBAL = 1, BM_RADIX = 2 };
 
 struct BatchQuery {
     const void *key;
@@ -61,6 +79,12 @@ struct BatchQuery {
     uint64_t sel_est;     // predicted distinct keys
     uint32_t sel_mode;    // BatchMode from the predicted path
     uint32_t sel;
+    // radix mode (host-set record shape; the rest filled on the device)
+    uint32_t rec_words;   // key + one word per accumulator with a value column
+    uint32_t valmask;     // bit a: accumulator a reads a value column
+    uint64_t rec_off;     // first word of the query's record region
+    uint32_t pbits;       // partitions 1 << pbits
+    uint32_t part0;       // first partition work item
 };
 
 struct BatchTune {
@@ -77,8 +101,14 @@ struct BatchCtl {
     uint32_t n_items;
     uint32_t lds_slots;    // copy of BatchTune::lds_slots for the sketch's mode choice
     uint32_t cap_pct;      // table slots per estimated key, in percent (150)
-    uint32_t mode_rule;    // 0: LDS-first iff est <= 4 lds_slots; 1: always; 2: also when keys repeat within a tile
-    uint32_t pad;
+    uint32_t mode_rule;    // 0: LDS-first iff est <= 4 lds_slots; 1: always; 2: also when keys repeat within a
+                           // tile; 3: radix partitions for every query they can hold (tests)
+    uint32_t radix;        // 1: queries not LDS-first take radix partitions when they fit
+    uint32_t part_slots;   // k_b_part LDS table slots
+    uint32_t n_parts;      // radix partition work items
+    uint32_t next_part;    // their work-queue head
+    uint64_t rec_words;    // record words in use
+    uint64_t rec_cap;      // record words allocated
 };
 
 struct BatchPool {
@@ -86,6 +116,10 @@ struct BatchPool {
     uint64_t *acc;         // [BQ_MAXACC][pool_slots]
     int64_t *out_key;      // [pool_slots] result rows of query q at tbl_off ...
     int64_t *out_acc;      // [BQ_MAXACC][pool_slots]
+    uint32_t *bhist;       // [n_items][BQ_RBUCKETS] rows per top-hash bucket; then, for radix
+                           // queries, each partition's first record of the tile
+    uint32_t *pstart;      // [nq][BQ_RBUCKETS + 1] radix: partition starts in the query's records
+    uint64_t *recs;        // radix records, query regions at rec_off
 };
 
 __device__ __forceinline__ int bq_op(const BatchQuery &q, int a) { return (int)((q.ops >> (4 * a)) & 15u); }
@@ -106,10 +140,16 @@ constexpr int BQ_HLL_P = 10;
 constexpr uint32_t BQ_HLL_M = 1u << BQ_HLL_P;
 
 __global__ __launch_bounds__(256) void k_b_hll(const BatchQuery *__restrict__ bq, const BatchCtl *__restrict__ ctl,
-                                               const uint2 *__restrict__ items, uint8_t *regs, BatchTune tune) {
+                                               const uint2 *__restrict__ items, uint8_t *regs, uint32_t *bhist,
+                                               BatchTune tune) {
+    static_assert(BQ_RBITS == BQ_HLL_P, "register index and radix bucket are the same top hash bits");
     __shared__ uint32_t r[BQ_HLL_M];
+    __shared__ uint32_t bh[BQ_RBUCKETS];
     for (uint32_t item = blockIdx.x; item < ctl->n_items; item += gridDim.x) {
-        for (uint32_t i = threadIdx.x; i < BQ_HLL_M; i += blockDim.x) r[i] = 0;
+        for (uint32_t i = threadIdx.x; i < BQ_HLL_M; i += blockDim.x) {
+            r[i] = 0;
+            bh[i] = 0;
+        }
         __syncthreads();
         const uint2 it = items[item];
         const BatchQuery &q = bq[it.x];
@@ -125,10 +165,14 @@ __global__ __launch_bounds__(256) void k_b_hll(const BatchQuery *__restrict__ bq
                 const uint64_t rest = h << BQ_HLL_P;
                 const uint32_t rank = rest ? (uint32_t)__clzll((long long)rest) + 1 : (64 - BQ_HLL_P + 1);
                 __hip_atomic_fetch_max(&r[h >> (64 - BQ_HLL_P)], rank, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                __hip_atomic_fetch_add(&bh[h >> (64 - BQ_RBITS)], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             }
         }
         __syncthreads();
-        for (uint32_t i = threadIdx.x; i < BQ_HLL_M; i += blockDim.x) regs[(uint64_t)item * BQ_HLL_M + i] = (uint8_t)r[i];
+        for (uint32_t i = threadIdx.x; i < BQ_HLL_M; i += blockDim.x) {
+            regs[(uint64_t)item * BQ_HLL_M + i] = (uint8_t)r[i];
+            bhist[(uint64_t)item * BQ_RBUCKETS + i] = bh[i];
+        }
         __syncthreads();
     }
 }
@@ -195,21 +239,36 @@ __global__ __launch_bounds__(256) void k_b_estimate(BatchQuery *__restrict__ bq,
                 q.cap = (cap + BQ_CHUNK - 1) / BQ_CHUNK * BQ_CHUNK;
             }
         }
+        // radix partitions: each holds about 0.6 part_slots groups or fewer,
+        // so the largest of them stays below the 7/8 fill limit
+        q.pbits = 0;
+        if ((ctl->radix && q.mode == BM_GLOBAL) || ctl->mode_rule == 3) {
+            const uint64_t want = max<uint64_t>(e, q.sel ? min(q.sel_est, q.rows) : 0);
+            const double per = 0.6 * (double)ctl->part_slots;
+            uint32_t pb = 0;
+            while (pb < BQ_RBITS && (double)want > per * (double)(1u << pb)) pb++;
+            if ((double)want <= per * (double)(1u << pb)) {
+                q.mode = BM_RADIX;
+                q.pbits = pb;
+            }
+        }
         q.groups = 0;
         q.overflow = 0;
     }
 }
 
-// ---- layout: table offsets --------------------------------------------------
-__global__ __launch_bounds__(1024) void k_b_layout(BatchQuery *__restrict__ bq, uint32_t nq, BatchCtl *ctl) {
-    __shared__ uint64_t s_part[1024];
+// ---- layout: table offsets, record regions, partition work items ------------
+// Exclusive prefix over the queries of val(q), written by put(q, offset);
+// returns the total. Each thread takes `per` consecutive queries.
+template <class V, class P>
+__device__ uint64_t block_prefix(uint32_t nq, uint64_t *s_part, V val, P put) {
     const uint32_t per = (nq + blockDim.x - 1) / blockDim.x;
     const uint32_t q0 = threadIdx.x * per, q1 = min(nq, q0 + per);
     uint64_t local = 0;
-    for (uint32_t q = q0; q < q1; q++) local += bq[q].cap;
+    for (uint32_t q = q0; q < q1; q++) local += val(q);
     s_part[threadIdx.x] = local;
     __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
+    for (int off = 1; off < (int)blockDim.x; off <<= 1) {
         uint64_t v = threadIdx.x >= (uint32_t)off ? s_part[threadIdx.x - off] : 0;
         __syncthreads();
         s_part[threadIdx.x] += v;
@@ -217,12 +276,84 @@ __global__ __launch_bounds__(1024) void k_b_layout(BatchQuery *__restrict__ bq, 
     }
     uint64_t run = threadIdx.x ? s_part[threadIdx.x - 1] : 0;
     for (uint32_t q = q0; q < q1; q++) {
-        bq[q].tbl_off = run;
-        run += bq[q].cap;
+        put(q, run);
+        run += val(q);
     }
-    if (threadIdx.x == blockDim.x - 1) {
-        ctl->total_slots = s_part[blockDim.x - 1];
+    const uint64_t total = s_part[blockDim.x - 1];
+    __syncthreads();  // s_part is reused by the next call
+    return total;
+}
+
+__global__ __launch_bounds__(1024) void k_b_layout(BatchQuery *__restrict__ bq, uint32_t nq, BatchCtl *ctl) {
+    __shared__ uint64_t s_part[1024];
+    const uint64_t slots = block_prefix(nq, s_part, [&](uint32_t q) { return bq[q].cap; },
+                                        [&](uint32_t q, uint64_t o) { bq[q].tbl_off = o; });
+    const uint64_t words = block_prefix(
+        nq, s_part, [&](uint32_t q) { return bq[q].mode == BM_RADIX ? bq[q].rows * bq[q].rec_words : 0; },
+        [&](uint32_t q, uint64_t o) { bq[q].rec_off = o; });
+    const uint64_t parts = block_prefix(
+        nq, s_part, [&](uint32_t q) { return bq[q].mode == BM_RADIX ? (uint64_t)1 << bq[q].pbits : 0; },
+        [&](uint32_t q, uint64_t o) { bq[q].part0 = (uint32_t)o; });
+    if (threadIdx.x == 0) {
+        ctl->total_slots = slots;
+        ctl->rec_words = words;
+        ctl->n_parts = (uint32_t)parts;
         ctl->next_item = 0;
+        ctl->next_part = 0;
+    }
+}
+
+// ---- radix: exact partition offsets per tile ----------------------------------
+// One block of BQ_RBUCKETS threads per query. Partition p of a query with
+// pbits bits is buckets [p << (10 - pbits), (p + 1) << (10 - pbits)) of the
+// tiles' histograms. The partitions are laid out one after another in the
+// query's record region; within one, the tiles' rows follow in tile order.
+// Each tile's histogram row is overwritten with the first record of each of
+// its partitions (at the partition's first bucket). Thread b reads bucket b
+// of every tile (coalesced); partition counts are LDS sums.
+__global__ __launch_bounds__(BQ_RBUCKETS) void k_b_rscan(const BatchQuery *__restrict__ bq, BatchPool pool) {
+    __shared__ uint32_t s_scan[BQ_RBUCKETS];
+    __shared__ uint32_t s_tile[BQ_RBUCKETS];
+    const BatchQuery &q = bq[blockIdx.x];
+    if (q.mode != BM_RADIX) return;
+    const uint32_t P = 1u << q.pbits, sh = BQ_RBITS - q.pbits;
+    const uint32_t b = threadIdx.x, pb = b >> sh;  // bucket b lies in partition pb
+    uint32_t *const hist = pool.bhist + (uint64_t)q.item0 * BQ_RBUCKETS;
+    // rows per bucket over all tiles, then per partition
+    uint32_t tot = 0;
+    for (uint32_t t = 0; t < q.nitems; t++) tot += hist[(uint64_t)t * BQ_RBUCKETS + b];
+    s_scan[b] = 0;
+    __syncthreads();
+    __hip_atomic_fetch_add(&s_scan[pb], tot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    __syncthreads();
+    const uint32_t ptot = b < P ? s_scan[b] : 0;
+    __syncthreads();
+    // exclusive scan of the partition totals (Hillis-Steele over the block)
+    s_scan[b] = ptot;
+    __syncthreads();
+    for (uint32_t off = 1; off < BQ_RBUCKETS; off <<= 1) {
+        const uint32_t v = b >= off ? s_scan[b - off] : 0;
+        __syncthreads();
+        s_scan[b] += v;
+        __syncthreads();
+    }
+    uint32_t run = s_scan[b] - ptot;  // thread p < P: partition p's first record
+    uint32_t *ps = pool.pstart + (uint64_t)blockIdx.x * (BQ_RBUCKETS + 1);
+    if (b < P) ps[b] = run;
+    if (b == P - 1) ps[P] = run + ptot;
+    s_tile[b] = 0;
+    __syncthreads();
+    for (uint32_t t = 0; t < q.nitems; t++) {
+        uint32_t *h = hist + (uint64_t)t * BQ_RBUCKETS;
+        const uint32_t c = h[b];
+        if (c) __hip_atomic_fetch_add(&s_tile[pb], c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        __syncthreads();  // every bucket of tile t is read before its row is overwritten
+        if (b < P) {
+            h[b << sh] = run;
+            run += s_tile[b];
+            s_tile[b] = 0;
+        }
+        __syncthreads();
     }
 }
 
@@ -244,6 +375,7 @@ __global__ __launch_bounds__(256) void k_b_init(const BatchQuery *__restrict__ b
     for (uint64_t c = blockIdx.x; c * BQ_CHUNK < total; c += gridDim.x) {
         const uint64_t s = c * BQ_CHUNK + threadIdx.x;  // chunks never straddle two tables
         const BatchQuery &q = bq[bq_owner(bq, nq, c * BQ_CHUNK)];
+        if (q.mode == BM_RADIX) continue;  // no table: its slots are the result rows of k_b_part
         pool.keys[s] = SENTINEL;
         for (int a = 0; a < BQ_MAXACC; a++)
             if (a < (int)q.nacc) pool.acc[(uint64_t)a * ctl->pool_slots + s] = acc_identity(bq_op(q, a));
@@ -276,6 +408,70 @@ __device__ __forceinline__ void bq_apply(uint64_t *p, int op, uint64_t x) {
 }
 
 // ---- aggregation: persistent work queue over (query, tile) items -------------
+// Rows are taken BQ_U per thread at a time: every key and value load of the
+// group is issued before the first row is used, so the loads overlap.
+constexpr int BQ_U = 4;
+
+struct BqRows {
+    uint64_t k[BQ_U];
+    uint64_t v[BQ_U][BQ_MAXACC];
+};
+
+// Loads rows base + j * blockDim.x + threadIdx.x (clamped to the tile's last
+// row, so every load is in bounds) of query q.
+__device__ __forceinline__ void bq_load(const BatchQuery &q, uint64_t base, uint64_t r1, BqRows &x) {
+#pragma unroll
+    for (int j = 0; j < BQ_U; j++) {
+        const uint64_t r = min(base + (uint64_t)j * blockDim.x + threadIdx.x, r1 - 1);
+        x.k[j] = bq_key(q, r);
+#pragma unroll
+        for (int a = 0; a < BQ_MAXACC; a++) x.v[j][a] = a < (int)q.nacc ? bq_val(q, a, r) : 0;
+    }
+}
+
+// LDS slot of key k in the table lkey[S]: found or claimed while the fill is
+// below `limit`; -1 when neither happens within `probes` slots.
+__device__ __forceinline__ int bq_lds_slot(uint64_t *lkey, uint32_t S, uint32_t *fill, uint32_t limit,
+                                           uint64_t k, uint32_t probes) {
+    uint32_t pos = (uint32_t)(((uint64_t)(uint32_t)hash64(k) * S) >> 32);
+    for (uint32_t p = 0; p < probes; p++) {
+        uint64_t cur = __hip_atomic_load(&lkey[pos], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (cur == SENTINEL) {
+            if (__hip_atomic_load(fill, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) >= limit) return -1;
+            uint64_t e = SENTINEL;
+            __hip_atomic_compare_exchange_strong(&lkey[pos], &e, k, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (e == SENTINEL) __hip_atomic_fetch_add(fill, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            cur = e == SENTINEL ? k : e;
+        }
+        if (cur == k) return (int)pos;
+        pos = pos + 1 == S ? 0 : pos + 1;
+    }
+    return -1;
+}
+
+// Partition cursor of a record: lanes of a wave in the same partition as the
+// first active lane take consecutive offsets from one LDS atomic (ballot +
+// popcount rank) when at least 8 of them share it; the others take one each.
+__device__ __forceinline__ uint32_t bq_wave_offset(uint32_t *cur, bool active, uint32_t part) {
+    const uint64_t act = __ballot(active);
+    const int lane = threadIdx.x & 63;
+    if (act) {
+        const int first = __ffsll((unsigned long long)act) - 1;
+        const uint32_t lead = __shfl(part, first);
+        const uint64_t m = __ballot(active && part == lead);
+        if (__popcll(m) >= 8) {
+            uint32_t base = 0;
+            if (lane == first)
+                base = __hip_atomic_fetch_add(&cur[lead], (uint32_t)__popcll(m), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            base = __shfl(base, first);
+            if ((m >> lane) & 1) return base + (uint32_t)__popcll(m & ((1ULL << lane) - 1));
+        }
+    }
+    if (!active) return 0;
+    return __hip_atomic_fetch_add(&cur[part], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
 __global__ void k_b_agg(BatchQuery *__restrict__ bq, BatchCtl *ctl, const uint2 *__restrict__ items,
                         BatchPool pool, BatchTune tune) {
     extern __shared__ uint64_t smem[];
@@ -292,6 +488,20 @@ __global__ void k_b_agg(BatchQuery *__restrict__ bq, BatchCtl *ctl, const uint2 
         BatchQuery &q = bq[it.x];
         const uint64_t r0 = (uint64_t)it.y * tune.tile_rows, r1 = min(q.rows, r0 + tune.tile_rows);
         const int nacc = (int)q.nacc;
+        const uint64_t step = (uint64_t)blockDim.x * BQ_U;
+        // a row into the query's global table
+        // (value arrays are indexed with constants only, so they stay in registers)
+        auto to_global = [&](uint64_t k, const uint64_t (&v)[BQ_MAXACC]) {
+            const int64_t g = bq_find(q, pool, k);
+            if (g < 0) {
+                q.overflow = 1;
+                return;
+            }
+#pragma unroll
+            for (int a = 0; a < BQ_MAXACC; a++)
+                if (a < nacc)
+                    bq_apply<__HIP_MEMORY_SCOPE_AGENT>(pool.acc + (uint64_t)a * ctl->pool_slots + g, bq_op(q, a), v[a]);
+        };
         if (q.mode == BM_LDS) {
             for (uint32_t i = threadIdx.x; i < S; i += blockDim.x) {
                 lkey[i] = SENTINEL;
@@ -299,43 +509,28 @@ __global__ void k_b_agg(BatchQuery *__restrict__ bq, BatchCtl *ctl, const uint2 
             }
             if (threadIdx.x == 0) s_fill = 0;
             __syncthreads();
-            for (uint64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
-                const uint64_t k = bq_key(q, r);
-                if (k == SENTINEL) {  // INT64_MIN is the empty-slot marker: the query re-runs unbatched
-                    q.overflow = 1;
-                    continue;
-                }
-                // LDS slot of k: find or claim, bounded; else the global table
-                uint32_t pos = (uint32_t)(((uint64_t)(uint32_t)hash64(k) * S) >> 32);
-                int slot = -1;
-                for (int p = 0; p < 16; p++) {
-                    uint64_t cur = __hip_atomic_load(&lkey[pos], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                    if (cur == SENTINEL) {
-                        if (__hip_atomic_load(&s_fill, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) >= S / 2) break;
-                        uint64_t e = SENTINEL;
-                        __hip_atomic_compare_exchange_strong(&lkey[pos], &e, k, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
-                                                             __HIP_MEMORY_SCOPE_WORKGROUP);
-                        if (e == SENTINEL) atomicAdd(&s_fill, 1u);
-                        cur = e == SENTINEL ? k : e;
-                    }
-                    if (cur == k) {
-                        slot = (int)pos;
-                        break;
-                    }
-                    pos = pos + 1 == S ? 0 : pos + 1;
-                }
-                if (slot >= 0) {
-                    for (int a = 0; a < nacc; a++)
-                        bq_apply<__HIP_MEMORY_SCOPE_WORKGROUP>(lacc + (uint64_t)a * S + slot, bq_op(q, a), bq_val(q, a, r));
-                } else {
-                    const int64_t g = bq_find(q, pool, k);
-                    if (g < 0) {
+            for (uint64_t base = r0; base < r1; base += step) {
+                BqRows x;
+                bq_load(q, base, r1, x);
+#pragma unroll
+                for (int j = 0; j < BQ_U; j++) {
+                    const bool live = base + (uint64_t)j * blockDim.x + threadIdx.x < r1;
+                    const uint64_t k = x.k[j];
+                    if (!live) continue;
+                    if (k == SENTINEL) {  // INT64_MIN is the empty-slot marker: the query re-runs unbatched
                         q.overflow = 1;
                         continue;
                     }
-                    for (int a = 0; a < nacc; a++)
-                        bq_apply<__HIP_MEMORY_SCOPE_AGENT>(pool.acc + (uint64_t)a * ctl->pool_slots + g, bq_op(q, a),
-                                                           bq_val(q, a, r));
+                    // LDS slot of k: find or claim, bounded; else the global table
+                    const int slot = bq_lds_slot(lkey, S, &s_fill, S / 2, k, 16);
+                    if (slot >= 0) {
+#pragma unroll
+                        for (int a = 0; a < BQ_MAXACC; a++)
+                            if (a < nacc)
+                                bq_apply<__HIP_MEMORY_SCOPE_WORKGROUP>(lacc + (uint64_t)a * S + slot, bq_op(q, a), x.v[j][a]);
+                    } else {
+                        to_global(k, x.v[j]);
+                    }
                 }
             }
             __syncthreads();
@@ -354,22 +549,138 @@ __global__ void k_b_agg(BatchQuery *__restrict__ bq, BatchCtl *ctl, const uint2 
                         bq_apply<__HIP_MEMORY_SCOPE_AGENT>(pool.acc + (uint64_t)a * ctl->pool_slots + g, bq_op(q, a), x);
                 }
             }
-        } else {
-            for (uint64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
-                const uint64_t k = bq_key(q, r);
-                if (k == SENTINEL) {
-                    q.overflow = 1;
-                    continue;
+        } else if (q.mode == BM_RADIX) {
+            // each row as a record at its partition's next offset (the LDS
+            // cursors start at the tile's first record of each partition)
+            uint32_t *cur = (uint32_t *)smem;  // [P]
+            const uint32_t P = 1u << q.pbits, sh = BQ_RBITS - q.pbits;
+            for (uint32_t i = threadIdx.x; i < P; i += blockDim.x)
+                cur[i] = pool.bhist[(uint64_t)item * BQ_RBUCKETS + ((uint64_t)i << sh)];
+            __syncthreads();
+            const uint32_t rw = q.rec_words;
+            uint64_t *rq = pool.recs + q.rec_off;
+            for (uint64_t base = r0; base < r1; base += step) {
+                BqRows x;
+                bq_load(q, base, r1, x);
+#pragma unroll
+                for (int j = 0; j < BQ_U; j++) {
+                    const bool live = base + (uint64_t)j * blockDim.x + threadIdx.x < r1;
+                    const uint64_t k = x.k[j];
+                    if (live && k == SENTINEL) q.overflow = 1;  // the record is still written: offsets stay exact
+                    const uint32_t part = q.pbits ? (uint32_t)(hash64(k) >> (64 - q.pbits)) : 0;
+                    const uint32_t pos = bq_wave_offset(cur, live, part);
+                    if (!live) continue;
+                    uint64_t *rec = rq + (uint64_t)pos * rw;
+                    gst(rec, k);
+                    uint32_t w = 1;
+#pragma unroll
+                    for (int a = 0; a < BQ_MAXACC; a++)
+                        if (a < nacc && ((q.valmask >> a) & 1)) gst(rec + w++, x.v[j][a]);
                 }
-                const int64_t g = bq_find(q, pool, k);
-                if (g < 0) {
-                    q.overflow = 1;
-                    continue;
-                }
-                for (int a = 0; a < nacc; a++)
-                    bq_apply<__HIP_MEMORY_SCOPE_AGENT>(pool.acc + (uint64_t)a * ctl->pool_slots + g, bq_op(q, a),
-                                                       bq_val(q, a, r));
             }
+        } else {
+            for (uint64_t base = r0; base < r1; base += step) {
+                BqRows x;
+                bq_load(q, base, r1, x);
+#pragma unroll
+                for (int j = 0; j < BQ_U; j++) {
+                    if (base + (uint64_t)j * blockDim.x + threadIdx.x >= r1) continue;
+                    if (x.k[j] == SENTINEL) {
+                        q.overflow = 1;
+                        continue;
+                    }
+                    to_global(x.k[j], x.v[j]);
+                }
+            }
+        }
+        __syncthreads();  // LDS table and s_item are reused by the next item
+    }
+}
+
+// ---- radix partitions: one LDS table per partition ---------------------------
+// Persistent workgroups take partition work items in order; item i belongs to
+// the last query whose first item is <= i. A group that finds no LDS slot (the
+// partition holds more groups than the fill limit) or no result row flags the
+// query, which the host then re-runs unbatched.
+__device__ __forceinline__ uint32_t bq_part_owner(const BatchQuery *bq, uint32_t nq, uint32_t item) {
+    uint32_t lo = 0, hi = nq - 1;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) / 2;
+        if (bq[mid].part0 <= item) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(BQ_PART_BLOCK) void k_b_part(BatchQuery *__restrict__ bq, uint32_t nq, BatchCtl *ctl,
+                                                          BatchPool pool) {
+    extern __shared__ uint64_t smem[];
+    __shared__ uint32_t s_item, s_fill;
+    const uint32_t S = ctl->part_slots, limit = S / 8 * 7;
+    uint64_t *lkey = smem;       // [S]
+    uint64_t *lacc = smem + S;   // [BQ_MAXACC][S]
+    for (;;) {
+        if (threadIdx.x == 0) s_item = __hip_atomic_fetch_add(&ctl->next_part, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __syncthreads();
+        const uint32_t item = s_item;
+        if (item >= ctl->n_parts) break;  // every wave of the block sees the same item
+        const uint32_t qi = bq_part_owner(bq, nq, item);
+        BatchQuery &q = bq[qi];
+        const uint32_t p = item - q.part0;
+        const uint32_t *ps = pool.pstart + (uint64_t)qi * (BQ_RBUCKETS + 1);
+        const uint64_t s0 = ps[p], s1 = ps[p + 1];
+        const int nacc = (int)q.nacc;
+        const uint32_t rw = q.rec_words;
+        const uint64_t *rq = pool.recs + q.rec_off;
+        for (uint32_t i = threadIdx.x; i < S; i += blockDim.x) {
+            lkey[i] = SENTINEL;
+            for (int a = 0; a < nacc; a++) lacc[(uint64_t)a * S + i] = acc_identity(bq_op(q, a));
+        }
+        if (threadIdx.x == 0) s_fill = 0;
+        __syncthreads();
+        for (uint64_t base = s0; base < s1; base += (uint64_t)blockDim.x * BQ_U) {
+            uint64_t kk[BQ_U], vv[BQ_U][BQ_MAXACC];
+#pragma unroll
+            for (int j = 0; j < BQ_U; j++) {  // clamped to the partition's last record
+                const uint64_t *rec = rq + min(base + (uint64_t)j * blockDim.x + threadIdx.x, s1 - 1) * rw;
+                kk[j] = gld(rec);
+#pragma unroll
+                for (int a = 0; a < BQ_MAXACC; a++)  // accumulator a's value: word 1 + (value columns before a)
+                    vv[j][a] = a < nacc && ((q.valmask >> a) & 1)
+                                   ? gld(rec + 1 + __popc(q.valmask & ((1u << a) - 1)))
+                                   : 1;
+            }
+#pragma unroll
+            for (int j = 0; j < BQ_U; j++) {
+                const bool live = base + (uint64_t)j * blockDim.x + threadIdx.x < s1;
+                const uint64_t k = kk[j];
+                if (!live || k == SENTINEL) continue;  // SENTINEL: the query is flagged already
+                const int slot = bq_lds_slot(lkey, S, &s_fill, limit, k, BQ_MAX_PROBE);
+                if (slot < 0) {
+                    q.overflow = 1;
+                    continue;
+                }
+#pragma unroll
+                for (int a = 0; a < BQ_MAXACC; a++)
+                    if (a < nacc) bq_apply<__HIP_MEMORY_SCOPE_WORKGROUP>(lacc + (uint64_t)a * S + slot, bq_op(q, a), vv[j][a]);
+            }
+        }
+        __syncthreads();
+        // the partition's groups to the query's result rows
+        uint32_t mine = 0;
+        for (uint32_t i = threadIdx.x; i < S; i += blockDim.x) mine += lkey[i] != SENTINEL;
+        uint64_t row = block_reserve(mine, &q.groups);
+        for (uint32_t i = threadIdx.x; i < S; i += blockDim.x) {
+            const uint64_t k = lkey[i];
+            if (k == SENTINEL) continue;
+            if (row >= q.cap) {  // more groups than the estimate allowed for
+                q.overflow = 1;
+                break;
+            }
+            const uint64_t dst = q.tbl_off + row++;
+            pool.out_key[dst] = (int64_t)k;
+            for (int a = 0; a < nacc; a++)
+                pool.out_acc[(uint64_t)a * ctl->pool_slots + dst] = (int64_t)lacc[(uint64_t)a * S + i];
         }
         __syncthreads();  // LDS table and s_item are reused by the next item
     }
@@ -382,6 +693,7 @@ __global__ __launch_bounds__(BQ_CHUNK) void k_b_compact(BatchQuery *__restrict__
     for (uint64_t c = blockIdx.x; c * BQ_CHUNK < total; c += gridDim.x) {
         const uint64_t s = c * BQ_CHUNK + threadIdx.x;
         BatchQuery &q = bq[bq_owner(bq, nq, c * BQ_CHUNK)];
+        if (q.mode == BM_RADIX) continue;  // the same for every thread of the block: k_b_part wrote its rows
         const uint64_t k = pool.keys[s];
         const uint32_t mine = k != SENTINEL;
         const uint64_t row = block_reserve(mine, &q.groups);

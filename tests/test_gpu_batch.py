@@ -137,10 +137,11 @@ def test_batch_autotune_keeps_results_exact():
     b.close()
 
 
-@pytest.mark.parametrize("rule", [0, 1, 2])
+@pytest.mark.parametrize("rule", [0, 1, 2, 3])
 def test_batch_mode_rules_keep_results_exact(rule):
-    # which queries aggregate tiles in LDS first is a speed choice only
-    host, dev = _entries(dict(rows_hi=300_000, distinct_hi=200_000), 16)
+    # which queries aggregate tiles in LDS first, and which take radix
+    # partitions (rule 3: every query), is a speed choice only
+    host, dev = _entries(dict(rows_hi=300_000, distinct_hi=200_000), 16, _special_queries())
     b = G.GpuBatch()
     b.set_param("mode_rule", rule)
     b.set_queries(dev)
@@ -151,10 +152,11 @@ def test_batch_mode_rules_keep_results_exact(rule):
     b.close()
 
 
-def test_batch_int64_min_key_runs_unbatched_and_exact():
+@pytest.mark.parametrize("rule", [2, 3])
+def test_batch_int64_min_key_runs_unbatched_and_exact(rule):
     # INT64_MIN equals the batched tables' empty-slot marker: such a query is
     # flagged and re-runs on the single-query operator, which keeps that key in
-    # a side record; the other queries stay batched
+    # a side record; the other queries stay batched (rule 3: as radix records)
     rng = np.random.default_rng(17)
     n = 200_000
     k = rng.integers(-1000, 1000, n).astype(np.int64)
@@ -165,6 +167,7 @@ def test_batch_int64_min_key_runs_unbatched_and_exact():
     special = (q, [G.Column(G.BIGINT, k), G.Column(G.BIGINT, v)])
     host, dev = _entries(dict(rows_hi=100_000, distinct_hi=50_000), 6, [special])
     b = G.GpuBatch()
+    b.set_param("mode_rule", rule)
     b.set_queries(dev)
     b.run()
     info = b.info()
@@ -212,6 +215,39 @@ def test_batch_selector_plans_modes_results_unchanged():
         print(f"[batch] selector={use}: {info['selector_queries']} queries planned by it", flush=True)
         assert (info["selector_queries"] > 0) if use else info["selector_queries"] == 0
         got[use] = [sorted(b.result(i).rows()) for i in range(len(host))]
+        b.close()
+    assert got[0] == got[1]
+    for i, (q, cols) in enumerate(host):
+        exp, types = _exact(q, cols)
+        assert_same_groups(got[1][i], exp, 1, types)
+
+
+@pytest.mark.parametrize("part_slots", [256, 2048, 3072])
+def test_batch_radix_partitions_match_global_table_and_oracle(part_slots):
+    # high-cardinality queries as radix partitions (default) and as a global
+    # table updated row by row give the same rows, equal to the oracle; BIGINT
+    # and INTEGER keys and values, MIN/MAX/SUM/COUNT, one to 1024 partitions
+    rng = np.random.default_rng(23)
+    extra = []
+    for n, d, key64 in [(600_000, 400_000, True), (300_000, 250_000, False), (5_000, 4_999, True)]:
+        kk = rng.integers(0, d, n) * 7919 - 3_000_000
+        k = G.Column(G.BIGINT, kk.astype(np.int64)) if key64 else G.Column(G.INTEGER, kk.astype(np.int32))
+        v32 = G.Column(G.INTEGER, rng.integers(-2**31, 2**31, n).astype(np.int32))
+        v64 = G.Column(G.BIGINT, rng.integers(-1 << 60, 1 << 60, n))
+        q = G.Query([k.type, G.INTEGER, G.BIGINT], [0],
+                    [(G.MIN, 1), (G.COUNT_STAR, None), (G.MAX, 2), (G.SUM, 2)], nullable=[False] * 3)
+        extra.append((q, [k, v32, v64]))
+    host, dev = _entries(dict(rows_hi=400_000, distinct_hi=300_000), 20, extra)
+    got = {}
+    for radix in (1, 0):
+        b = G.GpuBatch()
+        b.set_param("radix", radix)
+        b.set_param("part_slots", part_slots)
+        b.set_queries(dev)
+        b.run()
+        info = b.info()
+        print(f"[batch] radix={radix} part_slots={part_slots}: {info}", flush=True)
+        got[radix] = [sorted(b.result(i).rows()) for i in range(len(host))]
         b.close()
     assert got[0] == got[1]
     for i, (q, cols) in enumerate(host):
