@@ -64,6 +64,10 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-queries", type=int, default=512)
+    ap.add_argument("--batch-only", action="store_true",
+                    help="run only the batched mix (no plain runs), print its JSON, then exit")
+    ap.add_argument("--batch-tune", default=None, metavar="TILE,SLOTS,BLOCK,WPC",
+                    help="with --batch-only: this launch shape instead of autotuning")
     ap.add_argument("--no-strings", action="store_true")
     ap.add_argument("--no-parquet", action="store_true")
     ap.add_argument("--no-configs", action="store_true", help="skip the plain and composite configurations")
@@ -89,6 +93,10 @@ def main():
         return
     if args.ref_calibration:
         ref_calibration(args.cpu_threads)
+        return
+    if args.batch_only:
+        tune = [int(x) for x in args.batch_tune.split(",")] if args.batch_tune else None
+        print(json.dumps({"batched": bench_batched(args.batch_queries, plain=False, tune=tune)}), flush=True)
         return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (use --gpus 1)", file=sys.stderr)
@@ -590,7 +598,7 @@ def batch_split(nq=512):
     print(json.dumps({"batch_split": out}), flush=True)
 
 
-def bench_batched(nq):
+def bench_batched(nq, plain=True, tune=None):
     """The mixed-query workload: nq COUNT/SUM GROUP BY queries (rows
     log-uniform 1e3..4e6, distinct log-uniform 1..1e6, Zipf s in [0, 1.2]),
     HBM-resident. Plain = the single-query operator once per query; batched =
@@ -600,9 +608,9 @@ def bench_batched(nq):
     rows = sum(len(c[0]) for _, c in host)
     dev = [(q, [G.DeviceColumn(c) for c in cols]) for q, cols in host]
     sync = G.lib().gbagg_device_synchronize
-    ops = [G.GpuHashAggregate(q) for q, _ in dev]
-    plain = []
-    for rep in range(3):
+    ops = [G.GpuHashAggregate(q) for q, _ in dev] if plain else []
+    plain_s = []
+    for rep in range(3 if plain else 0):
         sync()
         t = time.perf_counter()
         for op, (_, dcols) in zip(ops, dev):
@@ -610,11 +618,14 @@ def bench_batched(nq):
             op.consume_device(dcols)
             op.finalize()
         sync()
-        plain.append(time.perf_counter() - t)
+        plain_s.append(time.perf_counter() - t)
     b = G.GpuBatch()
     b.set_queries(dev)
     t = time.perf_counter()
-    b.autotune()
+    if tune:
+        b.set_tune(*tune)
+    else:
+        b.autotune()
     tune_s = time.perf_counter() - t
     walls, devs = [], []
     for rep in range(5):
@@ -627,7 +638,7 @@ def bench_batched(nq):
     info = b.info()
     # exactness spot check against the single-query operator
     mism = 0
-    for i in range(0, nq, max(1, nq // 16)):
+    for i in range(0, nq, max(1, nq // 16)) if plain else ():
         mism += sorted(b.result(i).rows()) != sorted(ops[i].result().rows())
     # the same tuned batch with the selector off (modes from the sketch rule alone)
     b.set_param("selector", 0)
@@ -642,14 +653,14 @@ def bench_batched(nq):
     for op in ops:
         op.close()
     b.close()
-    plain_ms = min(plain) * 1e3
+    plain_ms = min(plain_s) * 1e3 if plain_s else None
     batch_ms = min(walls) * 1e3
     in_bytes = rows * 12
-    print(f"[bench] batched: {nq} queries, {rows} rows: plain {plain_ms:.1f} ms, batched {batch_ms:.2f} ms "
+    print(f"[bench] batched: {nq} queries, {rows} rows: plain {plain_ms or float('nan'):.1f} ms, batched {batch_ms:.2f} ms "
           f"(device {min(devs):.2f} ms; selector planned {info['selector_queries']} queries; without it "
           f"{min(walls0) * 1e3:.2f} ms), tune {tune_s:.1f}s, mismatches {mism}", file=sys.stderr, flush=True)
     return {"workload": f"{nq} mixed COUNT/SUM queries", "rows": rows, "plain_ms": plain_ms,
-            "batched_ms": batch_ms, "batched_device_ms": min(devs), "speedup": plain_ms / batch_ms,
+            "batched_ms": batch_ms, "batched_device_ms": min(devs), "speedup": plain_ms / batch_ms if plain_ms else None,
             "batched_rows_per_s": rows / (batch_ms * 1e-3), "batched_hbm_frac": in_bytes / (batch_ms * 1e-3) / 8e12,
             "selector_queries": info["selector_queries"], "batched_ms_without_selector": min(walls0) * 1e3,
             "tune": info["tune"], "autotune_s": tune_s, "spot_check_mismatches": mism}

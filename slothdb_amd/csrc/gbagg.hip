@@ -1550,9 +1550,12 @@ void run_sample(gbagg_handle *h) {
     }
     h->ghist.ensure(65 * 4);
     HIPCHK(hipMemsetAsync(h->ghist.ptr, 0, 65 * 4, s));
+    // the histogram runs in few blocks (each ends with device-scope atomics on
+    // the same plan words); the insert takes one atomic per wave
+    const uint32_t hgrid = (uint32_t)std::min<uint64_t>((rp.scap + 255) / 256, 128);
     const uint32_t sgrid = (uint32_t)std::min<uint64_t>((rp.scap + 255) / 256, (uint64_t)di.cus * 2);
     hipLaunchKernelGGL(k_key_range_init, dim3(1), dim3(1), 0, s, h->dplan.as<DevPlan>());  // sampled key range
-    hipLaunchKernelGGL(k_heavy_hist, dim3(sgrid), dim3(256), 0, s, d_rp(h), h->dplan.as<DevPlan>(), h->ghist.as<uint32_t>());
+    hipLaunchKernelGGL(k_heavy_hist, dim3(hgrid), dim3(256), 0, s, d_rp(h), h->dplan.as<DevPlan>(), h->ghist.as<uint32_t>());
     hipLaunchKernelGGL(k_heavy_threshold, dim3(1), dim3(64), 0, s, d_rp(h), h->dplan.as<DevPlan>(),
                        (const uint32_t *)h->ghist.as<uint32_t>());
     hipLaunchKernelGGL(k_heavy_insert, dim3(sgrid), dim3(256), 0, s, d_rp(h), h->dplan.as<DevPlan>());

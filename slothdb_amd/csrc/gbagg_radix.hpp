@@ -255,18 +255,34 @@ __global__ void k_heavy_threshold(const RadixParams *__restrict__ rpp, DevPlan *
     plan->heavy_count = 0;
 }
 
+// Dense indexes come from one device-scope atomic per wave and round (ballot
+// rank within the wave): same-address device atomics serialise.
 __global__ __launch_bounds__(256) void k_heavy_insert(const RadixParams *__restrict__ rpp, DevPlan *plan) {
     const RadixParams &rp = *rpp;
     const uint32_t t = plan->heavy_threshold;
     const uint32_t hmask = rp.hslots - 1;
     if (t > 64) return;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < rp.scap; i += gridDim.x * blockDim.x) {
-        uint32_t c = rp.scnt[i];
-        if (min(c, 64u) < t || c == 0) continue;
-        uint32_t d = (uint32_t)atomicAdd((unsigned long long *)&plan->heavy_count, 1ULL);
-        if (d >= rp.hmax) continue;
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t i0 = blockIdx.x * blockDim.x; i0 < rp.scap; i0 += stride) {  // uniform per wave
+        const uint32_t i = i0 + threadIdx.x;
+        const uint32_t c = i < rp.scap ? rp.scnt[i] : 0;
+        const bool heavy = c != 0 && min(c, 64u) >= t;
+        const uint64_t m = __ballot(heavy);
+        if (!m) continue;
+        const int first = __ffsll((unsigned long long)m) - 1;
+        uint32_t base = 0;
+        if ((int)lane == first)
+            base = (uint32_t)atomicAdd((unsigned long long *)&plan->heavy_count, (unsigned long long)__popcll(m));
+        base = __shfl(base, first);
+        const uint32_t d = base + (uint32_t)__popcll(m & ((1ULL << lane) - 1));
+        const bool kept = heavy && d < rp.hmax;  // the table holds hmax keys
+        unsigned long long rows = kept ? c : 0;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) rows += __shfl_xor(rows, off);
+        if ((int)lane == first && rows) atomicAdd((unsigned long long *)&plan->heavy_sample_rows, rows);
+        if (!kept) continue;
         uint64_t k = rp.skeys[i];
-        atomicAdd((unsigned long long *)&plan->heavy_sample_rows, (unsigned long long)c);
         rp.hdense[d] = k;
         uint32_t pos = (uint32_t)hash64(k) & hmask;
         for (uint32_t probe = 0; probe < rp.hslots; probe++) {  // hslots >= 2 * hmax: finds a slot
