@@ -411,9 +411,12 @@ gbagg_status gbagg_batch_set_tune(gbagg_batch *b, const gbagg_batch_tune *tune);
  * "radix": 1 (default) = a query that is not LDS-first writes its rows as
  * records to radix partitions of its key hash (up to 1024, sized from its
  * HyperLogLog estimate), each then aggregated in one LDS table, instead of
- * updating a global table row by row; 0 = the global table.
- * "part_slots": LDS table slots per radix partition (default 2048, 256-8192,
- * a multiple of 64). Results never depend on either.
+ * updating a global table row by row; an LDS-first query writes the rows
+ * that find no room in a tile's LDS table, and each tile's partial states,
+ * to such partitions instead of a global table; 0 = the global table.
+ * "part_slots": LDS table slots per radix partition (default 4096, 256-8192,
+ * a multiple of 64; fewer when the LDS holds fewer for the batch's widest
+ * query). Results never depend on either.
  * "selector": 1 (default) = each query's mode follows the learned selector's
  * path for its input (DIRECT or LDS_HASH: LDS-first; GLOBAL_HASH or RADIX:
  * not LDS-first; planned on the host once per input, from the same row sample

@@ -3281,7 +3281,7 @@ struct gbagg_batch {
     uint32_t cap_pct = 150;  // table slots per estimated key (percent)
     uint32_t mode_rule = 2;  // BatchCtl::mode_rule (2: LDS-first also for keys that repeat within a tile)
     uint32_t radix = 1;      // BatchCtl::radix: queries not LDS-first take radix partitions
-    uint32_t part_slots = 2048;  // k_b_part LDS table slots
+    uint32_t part_slots = 4096;  // k_b_part LDS table slots (fewer when the LDS holds fewer)
     uint32_t use_selector = 1;  // per query: the learned selector's path picks the mode
     // selector plans per input (key column, rows, query shape): (distinct, mode) or mode ~0u when not usable
     std::unordered_map<uint64_t, std::pair<uint64_t, uint32_t>> sel_cache;
@@ -3293,7 +3293,7 @@ struct gbagg_batch {
     std::vector<uint2> items;
     std::vector<std::vector<gbagg_type>> out_types;  // per batched query: key, aggs
     uint32_t nacc_max = 1;
-    DevBuf dq, ditems, dctl, keys, acc, out_key, out_acc, hll, bhist, pstart, recs;
+    DevBuf dq, ditems, dctl, keys, acc, out_key, out_acc, hll, bhist, pstart, bfill, recs;
     uint64_t pool_slots = 0;
     // graph of the current shape
     hipGraph_t graph = nullptr;
@@ -3444,7 +3444,8 @@ void batch_prepare(gbagg_batch *b, const gbagg_batch_item *items, uint32_t count
             }
         }
         pool += batch_pool_bound(bq.rows, b->cap_pct);
-        rec_words += b->radix || b->mode_rule == 3 ? bq.rows * bq.rec_words : 0;
+        // the wider record layout (one word per accumulator) bounds both
+        rec_words += b->radix || b->mode_rule == 3 ? bq.rows * (1 + bq.nacc) : 0;
         sig = mix_sig(sig, (uint64_t)(uintptr_t)bq.key);
         sig = mix_sig(sig, bq.rows);
         sig = mix_sig(sig, bq.ops | (uint64_t)bq.val64 << 32);
@@ -3480,6 +3481,7 @@ void batch_prepare(gbagg_batch *b, const gbagg_batch_item *items, uint32_t count
     b->hll.ensure(std::max<size_t>(1, b->items.size()) * BQ_HLL_M);
     b->bhist.ensure(std::max<size_t>(1, b->items.size()) * BQ_RBUCKETS * 4);
     b->pstart.ensure((size_t)nq * (BQ_RBUCKETS + 1) * 4);
+    b->bfill.ensure(std::max<size_t>(1, b->items.size()) * BQ_RBUCKETS * 4);
     // records of every query that may take radix partitions (the modes are
     // picked on the device)
     b->recs.ensure(std::max<uint64_t>(1, rec_words) * 8);
@@ -3490,7 +3492,8 @@ void batch_prepare(gbagg_batch *b, const gbagg_batch_item *items, uint32_t count
     ctl.cap_pct = b->cap_pct;
     ctl.mode_rule = b->mode_rule;
     ctl.radix = b->radix;
-    ctl.part_slots = b->part_slots;
+    ctl.part_slots = (uint32_t)std::min<size_t>(
+        b->part_slots, (di.lds_per_block - 4096) / (8 * (1 + (size_t)b->nacc_max)) / 64 * 64);
     ctl.rec_cap = rec_words;
     HIPCHK(hipMemcpyAsync(b->dq.ptr, b->hq.data(), nq * sizeof(BatchQuery), hipMemcpyHostToDevice, b->stream));
     if (!b->items.empty())
@@ -3500,12 +3503,16 @@ void batch_prepare(gbagg_batch *b, const gbagg_batch_item *items, uint32_t count
     HIPCHK(hipStreamSynchronize(b->stream));
 
     BatchPool bp{b->keys.as<uint64_t>(), b->acc.as<uint64_t>(), b->out_key.as<int64_t>(), b->out_acc.as<int64_t>(),
-                 b->bhist.as<uint32_t>(), b->pstart.as<uint32_t>(), b->recs.as<uint64_t>()};
-    const size_t agg_lds = std::max<size_t>((size_t)tn.lds_slots * 8 * (1 + b->nacc_max), BQ_RBUCKETS * 4);
+                 b->bhist.as<uint32_t>(), b->pstart.as<uint32_t>(), b->bfill.as<uint32_t>(), b->recs.as<uint64_t>()};
+    // LDS table, then the partition cursors of a tile that writes records
+    const size_t agg_lds = (size_t)tn.lds_slots * 8 * (1 + b->nacc_max) + BQ_RBUCKETS * 4;
     if (agg_lds > di.lds_per_block) fail(GBAGG_INVALID, "batch tune: LDS table larger than the device allows");
     allow_lds(k_b_agg, agg_lds);
-    const size_t part_lds = (size_t)b->part_slots * 8 * (1 + b->nacc_max);
-    if (part_lds > di.lds_per_block) fail(GBAGG_INVALID, "batch: partition LDS table larger than the device allows");
+    // the partition table: part_slots, or as many as the LDS holds next to the
+    // kernel's 4 KiB of static LDS
+    const uint32_t part_slots = (uint32_t)std::min<size_t>(
+        b->part_slots, (di.lds_per_block - 4096) / (8 * (1 + (size_t)b->nacc_max)) / 64 * 64);
+    const size_t part_lds = (size_t)part_slots * 8 * (1 + b->nacc_max);
     allow_lds(k_b_part, part_lds);
     const uint32_t part_per_cu = (uint32_t)std::max<size_t>(1, std::min<size_t>(4, di.lds_per_cu / part_lds));
     const uint32_t chunks = (uint32_t)std::min<uint64_t>((pool + BQ_CHUNK - 1) / BQ_CHUNK, (uint64_t)di.cus * 16);
@@ -3684,11 +3691,14 @@ gbagg_status gbagg_batch_autotune(gbagg_batch *b, const gbagg_batch_item *items,
         const BatchTune keep = b->tune;
         BatchTune best = keep;
         double best_ms = 1e30;
+        uint32_t nacc_max = 1;
+        for (uint32_t i = 0; i < count; i++)
+            if (items && batchable(items[i])) nacc_max = std::max<uint32_t>(nacc_max, items[i].query->num_aggs);
         for (uint32_t tile : {32768u, 65536u, 131072u})
             for (uint32_t slots : {1024u, 2048u, 4096u})
                 for (uint32_t block : {256u, 512u, 1024u})
                     for (uint32_t wpc : {1u, 2u, 4u}) {
-                        if ((size_t)slots * 8 * (1 + BQ_MAXACC) > di.lds_per_block) continue;
+                        if ((size_t)slots * 8 * (1 + nacc_max) + BQ_RBUCKETS * 4 > di.lds_per_block) continue;
                         if ((size_t)block * wpc > 2048) continue;  // 32 waves per CU
                         b->tune = BatchTune{tile, slots, block, wpc};
                         b->drop_graph();

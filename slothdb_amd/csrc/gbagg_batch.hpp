@@ -22,14 +22,18 @@
 //                 tile from the histograms
 //   k_b_init      fills every global table with empty keys and identity states
 //   k_b_agg       persistent work queue over (query, tile) items: LDS-first
-//                 queries aggregate a tile in an LDS table and merge it into
-//                 the query's global table; radix queries write each row as a
-//                 record (key, values) at its partition's exact offset;
-//                 global-table queries update the table directly. Every wave
-//                 leaves through the same item-counter check.
-//   k_b_part      persistent work queue over the radix queries' partitions:
-//                 all groups of one partition in an LDS table, then straight
-//                 to the query's dense result rows
+//                 queries aggregate a tile in an LDS table; a row that finds
+//                 no room there, and at the tile's end every partial state of
+//                 the table, becomes a record in the query's radix partitions
+//                 (or, beyond what 1024 partitions hold, goes to the query's
+//                 global table); radix queries write each row as a record
+//                 (key, values) at its partition's exact offset; global-table
+//                 queries update the table directly. Every wave leaves through
+//                 the same item-counter check.
+//   k_b_part      persistent work queue over the partitions of the queries
+//                 that write records: all groups of one partition in an LDS
+//                 table (raw rows and partial states combine alike), then
+//                 straight to the query's dense result rows
 //   k_b_compact   per 256-slot chunk of a global table: occupied records to
 //                 the query's dense result rows (one reservation per block)
 //
@@ -55,8 +59,17 @@ constexpr uint32_t BQ_MAX_PROBE = 128;
 constexpr uint32_t BQ_RBITS = 10;                 // radix: histogram of the top 10 hash bits per tile
 constexpr uint32_t BQ_RBUCKETS = 1u << BQ_RBITS;  // (partitions = the top pbits <= 10 of them)
 constexpr uint32_t BQ_PART_BLOCK = 512;           // k_b_part threads
+constexpr uint32_t BQ_SEGS = 256;                 // k_b_part: tile segments mapped at a time (<= BQ_PART_BLOCK)
+constexpr uint64_t BQ_PART_ROWS = 65536;          // radix: rows per partition the partition count aims at
+constexpr uint32_t BQ_TILE_PROBES = 32;           // LDS-first tile table: probe bound (fill limit 3/4)
 
-enum BatchMode : uint32_t { BM_LDS = 0, BM_GLOBAL = 1, BM_RADIX = 2 };
+// BM_LDS: LDS-first tiles merged into a global table; BM_GLOBAL: global table
+// per row; BM_RADIX: every row a record (key, value words); BM_LDS_RADIX:
+// LDS-first tiles, rows without room and the tiles' partial states as records
+// (key, one word per accumulator)
+enum BatchMode : uint32_t { BM_LDS = 0, BM_GLOBAL = 1, BM_RADIX = 2, BM_LDS_RADIX = 3 };
+
+__host__ __device__ __forceinline__ bool bq_records(uint32_t mode) { return mode == BM_RADIX || mode == BM_LDS_RADIX; }
 
 struct BatchQuery {
     const void *key;
@@ -80,7 +93,8 @@ struct BatchQuery {
     uint32_t sel_mode;    // BatchMode from the predicted path
     uint32_t sel;
     // radix mode (host-set record shape; the rest filled on the device)
-    uint32_t rec_words;   // key + one word per accumulator with a value column
+    uint32_t rec_words;   // BM_RADIX: key + one word per accumulator with a value column;
+                          // BM_LDS_RADIX: key + one word per accumulator (set on the device)
     uint32_t valmask;     // bit a: accumulator a reads a value column
     uint64_t rec_off;     // first word of the query's record region
     uint32_t pbits;       // partitions 1 << pbits
@@ -119,6 +133,7 @@ struct BatchPool {
     uint32_t *bhist;       // [n_items][BQ_RBUCKETS] rows per top-hash bucket; then, for radix
                            // queries, each partition's first record of the tile
     uint32_t *pstart;      // [nq][BQ_RBUCKETS + 1] radix: partition starts in the query's records
+    uint32_t *bfill;       // [n_items][BQ_RBUCKETS] BM_LDS_RADIX: records a tile wrote per partition
     uint64_t *recs;        // radix records, query regions at rec_off
 };
 
@@ -240,15 +255,24 @@ __global__ __launch_bounds__(256) void k_b_estimate(BatchQuery *__restrict__ bq,
             }
         }
         // radix partitions: each holds about 0.6 part_slots groups or fewer,
-        // so the largest of them stays below the 7/8 fill limit
+        // so the largest of them stays below the 7/8 fill limit, and at most
+        // BQ_PART_ROWS rows (records), so one partition's workgroup does not
+        // hold up the rest
         q.pbits = 0;
-        if ((ctl->radix && q.mode == BM_GLOBAL) || ctl->mode_rule == 3) {
+        if (ctl->radix || ctl->mode_rule == 3) {
             const uint64_t want = max<uint64_t>(e, q.sel ? min(q.sel_est, q.rows) : 0);
             const double per = 0.6 * (double)ctl->part_slots;
             uint32_t pb = 0;
-            while (pb < BQ_RBITS && (double)want > per * (double)(1u << pb)) pb++;
+            while (pb < BQ_RBITS && ((double)want > per * (double)(1u << pb) ||
+                                     q.rows > (uint64_t)BQ_PART_ROWS << pb))
+                pb++;
             if ((double)want <= per * (double)(1u << pb)) {
-                q.mode = BM_RADIX;
+                if (ctl->mode_rule == 3 || q.mode == BM_GLOBAL) {
+                    q.mode = BM_RADIX;
+                } else {
+                    q.mode = BM_LDS_RADIX;
+                    q.rec_words = 1 + q.nacc;
+                }
                 q.pbits = pb;
             }
         }
@@ -289,10 +313,10 @@ __global__ __launch_bounds__(1024) void k_b_layout(BatchQuery *__restrict__ bq, 
     const uint64_t slots = block_prefix(nq, s_part, [&](uint32_t q) { return bq[q].cap; },
                                         [&](uint32_t q, uint64_t o) { bq[q].tbl_off = o; });
     const uint64_t words = block_prefix(
-        nq, s_part, [&](uint32_t q) { return bq[q].mode == BM_RADIX ? bq[q].rows * bq[q].rec_words : 0; },
+        nq, s_part, [&](uint32_t q) { return bq_records(bq[q].mode) ? bq[q].rows * bq[q].rec_words : 0; },
         [&](uint32_t q, uint64_t o) { bq[q].rec_off = o; });
     const uint64_t parts = block_prefix(
-        nq, s_part, [&](uint32_t q) { return bq[q].mode == BM_RADIX ? (uint64_t)1 << bq[q].pbits : 0; },
+        nq, s_part, [&](uint32_t q) { return bq_records(bq[q].mode) ? (uint64_t)1 << bq[q].pbits : 0; },
         [&](uint32_t q, uint64_t o) { bq[q].part0 = (uint32_t)o; });
     if (threadIdx.x == 0) {
         ctl->total_slots = slots;
@@ -315,7 +339,7 @@ __global__ __launch_bounds__(BQ_RBUCKETS) void k_b_rscan(const BatchQuery *__res
     __shared__ uint32_t s_scan[BQ_RBUCKETS];
     __shared__ uint32_t s_tile[BQ_RBUCKETS];
     const BatchQuery &q = bq[blockIdx.x];
-    if (q.mode != BM_RADIX) return;
+    if (!bq_records(q.mode)) return;
     const uint32_t P = 1u << q.pbits, sh = BQ_RBITS - q.pbits;
     const uint32_t b = threadIdx.x, pb = b >> sh;  // bucket b lies in partition pb
     uint32_t *const hist = pool.bhist + (uint64_t)q.item0 * BQ_RBUCKETS;
@@ -375,7 +399,7 @@ __global__ __launch_bounds__(256) void k_b_init(const BatchQuery *__restrict__ b
     for (uint64_t c = blockIdx.x; c * BQ_CHUNK < total; c += gridDim.x) {
         const uint64_t s = c * BQ_CHUNK + threadIdx.x;  // chunks never straddle two tables
         const BatchQuery &q = bq[bq_owner(bq, nq, c * BQ_CHUNK)];
-        if (q.mode == BM_RADIX) continue;  // no table: its slots are the result rows of k_b_part
+        if (bq_records(q.mode)) continue;  // no table: its slots are the result rows of k_b_part
         pool.keys[s] = SENTINEL;
         for (int a = 0; a < BQ_MAXACC; a++)
             if (a < (int)q.nacc) pool.acc[(uint64_t)a * ctl->pool_slots + s] = acc_identity(bq_op(q, a));
@@ -502,11 +526,31 @@ __global__ void k_b_agg(BatchQuery *__restrict__ bq, BatchCtl *ctl, const uint2 
                 if (a < nacc)
                     bq_apply<__HIP_MEMORY_SCOPE_AGENT>(pool.acc + (uint64_t)a * ctl->pool_slots + g, bq_op(q, a), v[a]);
         };
-        if (q.mode == BM_LDS) {
+        if (q.mode == BM_LDS || q.mode == BM_LDS_RADIX) {
+            // BM_LDS_RADIX: partition cursors after the table, from the tile's
+            // first record of each partition
+            const bool recs = q.mode == BM_LDS_RADIX;
+            uint32_t *cur = (uint32_t *)(lacc + (uint64_t)nacc * S);  // [P]
+            const uint32_t P = 1u << q.pbits, sh = BQ_RBITS - q.pbits;
+            const uint32_t rw = q.rec_words;
+            uint64_t *rq = pool.recs + q.rec_off;
+            // a record (key, one word per accumulator) at its partition's cursor
+            auto to_record = [&](uint64_t k, const uint64_t (&v)[BQ_MAXACC]) {
+                const uint32_t part = q.pbits ? (uint32_t)(hash64(k) >> (64 - q.pbits)) : 0;
+                const uint32_t pos = __hip_atomic_fetch_add(&cur[part], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                uint64_t *rec = rq + (uint64_t)pos * rw;
+                gst(rec, k);
+#pragma unroll
+                for (int a = 0; a < BQ_MAXACC; a++)
+                    if (a < nacc) gst(rec + 1 + a, v[a]);
+            };
             for (uint32_t i = threadIdx.x; i < S; i += blockDim.x) {
                 lkey[i] = SENTINEL;
                 for (int a = 0; a < nacc; a++) lacc[(uint64_t)a * S + i] = acc_identity(bq_op(q, a));
             }
+            if (recs)
+                for (uint32_t i = threadIdx.x; i < P; i += blockDim.x)
+                    cur[i] = pool.bhist[(uint64_t)item * BQ_RBUCKETS + ((uint64_t)i << sh)];
             if (threadIdx.x == 0) s_fill = 0;
             __syncthreads();
             for (uint64_t base = r0; base < r1; base += step) {
@@ -521,21 +565,40 @@ __global__ void k_b_agg(BatchQuery *__restrict__ bq, BatchCtl *ctl, const uint2 
                         q.overflow = 1;
                         continue;
                     }
-                    // LDS slot of k: find or claim, bounded; else the global table
-                    const int slot = bq_lds_slot(lkey, S, &s_fill, S / 2, k, 16);
+                    // LDS slot of k: find or claim, bounded; else a record or the global table
+                    const int slot = bq_lds_slot(lkey, S, &s_fill, S / 4 * 3, k, BQ_TILE_PROBES);
                     if (slot >= 0) {
 #pragma unroll
                         for (int a = 0; a < BQ_MAXACC; a++)
                             if (a < nacc)
                                 bq_apply<__HIP_MEMORY_SCOPE_WORKGROUP>(lacc + (uint64_t)a * S + slot, bq_op(q, a), x.v[j][a]);
+                    } else if (recs) {
+                        to_record(k, x.v[j]);
                     } else {
                         to_global(k, x.v[j]);
                     }
                 }
             }
             __syncthreads();
+            if (recs) {
+                // the tile's partial states as records; a partition gets at
+                // most as many records as the tile has rows in it (each state
+                // stands for at least one row that did not become a record)
+                for (uint32_t i = threadIdx.x; i < S; i += blockDim.x) {
+                    const uint64_t k = lkey[i];
+                    if (k == SENTINEL) continue;
+                    uint64_t v[BQ_MAXACC];
+#pragma unroll
+                    for (int a = 0; a < BQ_MAXACC; a++) v[a] = a < nacc ? lacc[(uint64_t)a * S + i] : 0;
+                    to_record(k, v);
+                }
+                __syncthreads();
+                for (uint32_t i = threadIdx.x; i < P; i += blockDim.x)
+                    pool.bfill[(uint64_t)item * BQ_RBUCKETS + i] =
+                        cur[i] - pool.bhist[(uint64_t)item * BQ_RBUCKETS + ((uint64_t)i << sh)];
+            }
             // the tile's LDS states into the query's global table
-            for (uint32_t i = threadIdx.x; i < S; i += blockDim.x) {
+            for (uint32_t i = threadIdx.x; i < S && !recs; i += blockDim.x) {
                 const uint64_t k = lkey[i];
                 if (k == SENTINEL) continue;
                 const int64_t g = bq_find(q, pool, k);
@@ -616,6 +679,7 @@ __global__ __launch_bounds__(BQ_PART_BLOCK) void k_b_part(BatchQuery *__restrict
                                                           BatchPool pool) {
     extern __shared__ uint64_t smem[];
     __shared__ uint32_t s_item, s_fill;
+    __shared__ uint32_t s_seg[BQ_SEGS], s_pre[BQ_SEGS + 1];
     const uint32_t S = ctl->part_slots, limit = S / 8 * 7;
     uint64_t *lkey = smem;       // [S]
     uint64_t *lacc = smem + S;   // [BQ_MAXACC][S]
@@ -628,9 +692,9 @@ __global__ __launch_bounds__(BQ_PART_BLOCK) void k_b_part(BatchQuery *__restrict
         BatchQuery &q = bq[qi];
         const uint32_t p = item - q.part0;
         const uint32_t *ps = pool.pstart + (uint64_t)qi * (BQ_RBUCKETS + 1);
-        const uint64_t s0 = ps[p], s1 = ps[p + 1];
         const int nacc = (int)q.nacc;
         const uint32_t rw = q.rec_words;
+        const bool lr = q.mode == BM_LDS_RADIX;  // records hold one word per accumulator
         const uint64_t *rq = pool.recs + q.rec_off;
         for (uint32_t i = threadIdx.x; i < S; i += blockDim.x) {
             lkey[i] = SENTINEL;
@@ -638,31 +702,69 @@ __global__ __launch_bounds__(BQ_PART_BLOCK) void k_b_part(BatchQuery *__restrict
         }
         if (threadIdx.x == 0) s_fill = 0;
         __syncthreads();
-        for (uint64_t base = s0; base < s1; base += (uint64_t)blockDim.x * BQ_U) {
-            uint64_t kk[BQ_U], vv[BQ_U][BQ_MAXACC];
+        // records rec_of(0 .. n - 1) of the query's region into the table
+        auto run = [&](uint64_t n, auto rec_of) {
+            for (uint64_t base = 0; base < n; base += (uint64_t)blockDim.x * BQ_U) {
+                uint64_t kk[BQ_U], vv[BQ_U][BQ_MAXACC];
 #pragma unroll
-            for (int j = 0; j < BQ_U; j++) {  // clamped to the partition's last record
-                const uint64_t *rec = rq + min(base + (uint64_t)j * blockDim.x + threadIdx.x, s1 - 1) * rw;
-                kk[j] = gld(rec);
+                for (int j = 0; j < BQ_U; j++) {  // clamped to the last record
+                    const uint64_t *rec = rq + rec_of(min(base + (uint64_t)j * blockDim.x + threadIdx.x, n - 1)) * rw;
+                    kk[j] = gld(rec);
 #pragma unroll
-                for (int a = 0; a < BQ_MAXACC; a++)  // accumulator a's value: word 1 + (value columns before a)
-                    vv[j][a] = a < nacc && ((q.valmask >> a) & 1)
-                                   ? gld(rec + 1 + __popc(q.valmask & ((1u << a) - 1)))
-                                   : 1;
-            }
-#pragma unroll
-            for (int j = 0; j < BQ_U; j++) {
-                const bool live = base + (uint64_t)j * blockDim.x + threadIdx.x < s1;
-                const uint64_t k = kk[j];
-                if (!live || k == SENTINEL) continue;  // SENTINEL: the query is flagged already
-                const int slot = bq_lds_slot(lkey, S, &s_fill, limit, k, BQ_MAX_PROBE);
-                if (slot < 0) {
-                    q.overflow = 1;
-                    continue;
+                    for (int a = 0; a < BQ_MAXACC; a++) {
+                        // accumulator a's word: 1 + a, or 1 + (value columns before a)
+                        const uint32_t w = lr ? 1 + a : 1 + __popc(q.valmask & ((1u << a) - 1));
+                        vv[j][a] = a < nacc && (lr || ((q.valmask >> a) & 1)) ? gld(rec + w) : 1;
+                    }
                 }
 #pragma unroll
-                for (int a = 0; a < BQ_MAXACC; a++)
-                    if (a < nacc) bq_apply<__HIP_MEMORY_SCOPE_WORKGROUP>(lacc + (uint64_t)a * S + slot, bq_op(q, a), vv[j][a]);
+                for (int j = 0; j < BQ_U; j++) {
+                    const bool live = base + (uint64_t)j * blockDim.x + threadIdx.x < n;
+                    const uint64_t k = kk[j];
+                    if (!live || k == SENTINEL) continue;  // SENTINEL: the query is flagged already
+                    const int slot = bq_lds_slot(lkey, S, &s_fill, limit, k, BQ_MAX_PROBE);
+                    if (slot < 0) {
+                        q.overflow = 1;
+                        continue;
+                    }
+#pragma unroll
+                    for (int a = 0; a < BQ_MAXACC; a++)
+                        if (a < nacc) bq_apply<__HIP_MEMORY_SCOPE_WORKGROUP>(lacc + (uint64_t)a * S + slot, bq_op(q, a), vv[j][a]);
+                }
+            }
+        };
+        if (!lr) {
+            const uint64_t s0 = ps[p];
+            run(ps[p + 1] - s0, [&](uint64_t i) { return s0 + i; });
+        } else {
+            // one segment per tile (its first record of the partition and its
+            // count), BQ_SEGS tiles at a time: a prefix of their counts maps
+            // each index of their concatenation to its segment, so the loads
+            // of all of them are in flight together
+            const uint32_t sh = BQ_RBITS - q.pbits;
+            for (uint32_t t0 = 0; t0 < q.nitems; t0 += BQ_SEGS) {
+                const uint32_t nt = min(BQ_SEGS, q.nitems - t0);
+                if (threadIdx.x < nt) {
+                    const uint64_t row = (uint64_t)(q.item0 + t0 + threadIdx.x) * BQ_RBUCKETS;
+                    s_seg[threadIdx.x] = pool.bhist[row + ((uint64_t)p << sh)];
+                    s_pre[threadIdx.x + 1] = pool.bfill[row + p];
+                }
+                __syncthreads();
+                if (threadIdx.x == 0) {
+                    s_pre[0] = 0;
+                    for (uint32_t i = 1; i <= nt; i++) s_pre[i] += s_pre[i - 1];
+                }
+                __syncthreads();
+                run(s_pre[nt], [&](uint64_t i) {
+                    uint32_t lo = 0, hi = nt - 1;  // the last segment starting at or before i
+                    while (lo < hi) {
+                        const uint32_t mid = (lo + hi + 1) / 2;
+                        if (s_pre[mid] <= i) lo = mid;
+                        else hi = mid - 1;
+                    }
+                    return (uint64_t)s_seg[lo] + (i - s_pre[lo]);
+                });
+                __syncthreads();  // s_seg / s_pre are refilled by the next group of tiles
             }
         }
         __syncthreads();
@@ -693,7 +795,7 @@ __global__ __launch_bounds__(BQ_CHUNK) void k_b_compact(BatchQuery *__restrict__
     for (uint64_t c = blockIdx.x; c * BQ_CHUNK < total; c += gridDim.x) {
         const uint64_t s = c * BQ_CHUNK + threadIdx.x;
         BatchQuery &q = bq[bq_owner(bq, nq, c * BQ_CHUNK)];
-        if (q.mode == BM_RADIX) continue;  // the same for every thread of the block: k_b_part wrote its rows
+        if (bq_records(q.mode)) continue;  // the same for every thread of the block: k_b_part wrote its rows
         const uint64_t k = pool.keys[s];
         const uint32_t mine = k != SENTINEL;
         const uint64_t row = block_reserve(mine, &q.groups);
