@@ -61,7 +61,7 @@ constexpr uint32_t BQ_RBUCKETS = 1u << BQ_RBITS;  // (partitions = the top pbits
 constexpr uint32_t BQ_PART_BLOCK = 512;           // k_b_part threads
 constexpr uint32_t BQ_SEGS = 256;                 // k_b_part: tile segments mapped at a time (<= BQ_PART_BLOCK)
 constexpr uint64_t BQ_PART_ROWS = 65536;          // radix: rows per partition the partition count aims at
-constexpr uint32_t BQ_TILE_PROBES = 32;           // LDS-first tile table: probe bound (fill limit 3/4)
+constexpr uint32_t BQ_TILE_PROBES = 16;           // LDS-first tile table: probe bound (fill limit 1/2)
 
 // BM_LDS: LDS-first tiles merged into a global table; BM_GLOBAL: global table
 // per row; BM_RADIX: every row a record (key, value words); BM_LDS_RADIX:
@@ -566,7 +566,7 @@ __global__ void k_b_agg(BatchQuery *__restrict__ bq, BatchCtl *ctl, const uint2 
                         continue;
                     }
                     // LDS slot of k: find or claim, bounded; else a record or the global table
-                    const int slot = bq_lds_slot(lkey, S, &s_fill, S / 4 * 3, k, BQ_TILE_PROBES);
+                    const int slot = bq_lds_slot(lkey, S, &s_fill, S / 2, k, BQ_TILE_PROBES);
                     if (slot >= 0) {
 #pragma unroll
                         for (int a = 0; a < BQ_MAXACC; a++)
