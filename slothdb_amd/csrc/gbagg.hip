@@ -1891,7 +1891,7 @@ bool direct_range(gbagg_handle *h, uint64_t &span, int64_t &kmin) {
     hipLaunchKernelGGL(k_key_range_init, dim3(1), dim3(1), 0, s, h->dplan.as<DevPlan>());
     if (h->rows_total) {
         uint64_t per;
-        const uint32_t g = tiled_grid(h->rows_total, (uint64_t)AGG_BLOCK * AGG_ROWS, (uint64_t)di.cus * 4, per);
+        const uint32_t g = tiled_grid(h->rows_total, (uint64_t)AGG_BLOCK * AGG_ROWS, (uint64_t)di.cus * 2, per);
         hipLaunchKernelGGL(h->kern->range, dim3(g), dim3(AGG_BLOCK), 0, s, d_q(h), batch_rows(h, per),
                            h->dplan.as<DevPlan>());
     }

@@ -395,9 +395,22 @@ __global__ __launch_bounds__(AGG_BLOCK) void k_key_range(const DevQuery *__restr
         lo = min(lo, (int64_t)__shfl_xor(lo, off));
         hi = max(hi, (int64_t)__shfl_xor(hi, off));
     }
-    if ((threadIdx.x & 63) == 0 && lo <= hi) {
-        __hip_atomic_fetch_min(&plan->key_min, lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_fetch_max(&plan->key_max, hi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    // one device-scope update per block: updates of one word serialise
+    __shared__ int64_t s_lo[AGG_BLOCK / 64], s_hi[AGG_BLOCK / 64];
+    if ((threadIdx.x & 63) == 0) {
+        s_lo[threadIdx.x >> 6] = lo;
+        s_hi[threadIdx.x >> 6] = hi;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < AGG_BLOCK / 64; w++) {
+            lo = min(lo, s_lo[w]);
+            hi = max(hi, s_hi[w]);
+        }
+        if (lo <= hi) {
+            __hip_atomic_fetch_min(&plan->key_min, lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_max(&plan->key_max, hi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
     }
 }
 
