@@ -176,6 +176,16 @@ DeviceInfo &device_info() {
 }
 
 // Device buffer that only grows.
+// Bytes of the parameter block ahead of the plan block in one allocation.
+constexpr size_t CTX_BYTES = (sizeof(DevCtx) + 255) & ~(size_t)255;
+
+// A device block owned elsewhere.
+struct PlanRef {
+    void *ptr = nullptr;
+    template <typename T>
+    T *as() const { return static_cast<T *>(ptr); }
+};
+
 struct DevBuf {
     void *ptr = nullptr;
     size_t bytes = 0;
@@ -366,11 +376,13 @@ struct gbagg_handle {
     hipEvent_t ev[4] = {};
     std::vector<Batch> batches;
     std::vector<DevBuf> in_set_bufs;
-    DevBuf dplan, table, hll, blkcnt, cand_rec, cand_null, topn_lvl;
+    DevBuf table, hll, blkcnt, cand_rec, cand_null, topn_lvl;
+    PlanRef dplan;  // the device plan block, inside the ctx allocation
     DevBuf ghist, hllm, skeys, scnt, hkeys, hidx, hdense, hfinal, hist, hpart, base, ptotal, pstart, recs;
     std::vector<DevBuf> out_data, out_valid;
     DevPlan *hplan = nullptr;  // pinned
-    DevCtx *hctx = nullptr;    // pinned parameter blocks
+    DevCtx *hctx = nullptr;    // pinned parameter blocks: hctx, then hplan at CTX_BYTES
+    bool ctx_stale = false;    // hctx changed since its last upload
     DevBatch *hbatches = nullptr;
     size_t hbatch_cap = 0;
     DevBuf ctx, dbatches;
@@ -439,8 +451,7 @@ struct gbagg_handle {
 
     ~gbagg_handle() {
         if (stream) (void)hipStreamSynchronize(stream);  // copies out of the staging buffers
-        if (hplan) (void)hipHostFree(hplan);
-        if (hctx) (void)hipHostFree(hctx);
+        if (hctx) (void)hipHostFree(hctx);  // hplan lives in the same block
         if (hbatches) (void)hipHostFree(hbatches);
         for (int k = 0; k < 2; k++) {
             if (hstage_done[k]) (void)hipEventDestroy(hstage_done[k]);
@@ -995,12 +1006,13 @@ void ensure_runtime(gbagg_handle *h) {
     device_info();
     HIPCHK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
     for (auto &e : h->ev) HIPCHK(hipEventCreate(&e));
-    HIPCHK(hipHostMalloc((void **)&h->hplan, sizeof(DevPlan), hipHostMallocDefault));
-    std::memset(h->hplan, 0, sizeof(DevPlan));
-    h->dplan.ensure(sizeof(DevPlan));
-    HIPCHK(hipHostMalloc((void **)&h->hctx, sizeof(DevCtx), hipHostMallocDefault));
-    std::memset(h->hctx, 0, sizeof(DevCtx));
-    h->ctx.ensure(sizeof(DevCtx));
+    // the parameter block and the plan block side by side, host and device,
+    // so one copy can upload both
+    HIPCHK(hipHostMalloc((void **)&h->hctx, CTX_BYTES + sizeof(DevPlan), hipHostMallocDefault));
+    std::memset((void *)h->hctx, 0, CTX_BYTES + sizeof(DevPlan));
+    h->hplan = (DevPlan *)((char *)h->hctx + CTX_BYTES);
+    h->ctx.ensure(CTX_BYTES + sizeof(DevPlan));
+    h->dplan.ptr = (char *)h->ctx.ptr + CTX_BYTES;
     h->in_set_bufs.resize(h->preds.size());
     for (size_t i = 0; i < h->preds.size(); i++) {
         if (h->in_sets[i].empty()) continue;
@@ -1372,7 +1384,9 @@ void fill_out_topn(gbagg_handle *h) {
 // consumed the previous upload (every caller syncs through read_plan first).
 void upload_ctx(gbagg_handle *h) {
     HIPCHK(hipMemcpyAsync(h->ctx.ptr, h->hctx, sizeof(DevCtx), hipMemcpyHostToDevice, h->stream));
+    h->ctx_stale = false;
 }
+
 
 // The batches' DevBatch blocks, then their first rows in the concatenation
 // (nb + 1 words, the last one the total), in one copy.
@@ -1421,7 +1435,7 @@ uint32_t tiled_grid(uint64_t rows, uint64_t unit, uint64_t max_wg, uint64_t &per
 
 // Resets the run counters of the device plan block; the sample / heavy-hitter
 // statistics of this run (already read back into hplan) are kept.
-void upload_plan(gbagg_handle *h, uint64_t cap) {
+void prepare_plan(gbagg_handle *h, uint64_t cap) {
     DevPlan keep = *h->hplan;
     std::memset(h->hplan, 0, sizeof(DevPlan));
     h->hplan->sample_rows = keep.sample_rows;
@@ -1432,7 +1446,18 @@ void upload_plan(gbagg_handle *h, uint64_t cap) {
     h->hplan->heavy_threshold = keep.heavy_threshold;
     h->hplan->cap = cap;
     h->hplan->cap_max = h->cap_max;
+}
+
+void upload_plan(gbagg_handle *h, uint64_t cap) {
+    prepare_plan(h, cap);
     HIPCHK(hipMemcpyAsync(h->dplan.ptr, h->hplan, sizeof(DevPlan), hipMemcpyHostToDevice, h->stream));
+}
+
+// Parameter block and plan block (as upload_plan) in one copy.
+void upload_ctx_plan(gbagg_handle *h, uint64_t cap) {
+    prepare_plan(h, cap);
+    HIPCHK(hipMemcpyAsync(h->ctx.ptr, h->hctx, CTX_BYTES + sizeof(DevPlan), hipMemcpyHostToDevice, h->stream));
+    h->ctx_stale = false;
 }
 
 void read_plan(gbagg_handle *h) {
@@ -1540,7 +1565,9 @@ void run_sample(gbagg_handle *h) {
     rp.hdense = h->hdense.as<uint64_t>();
     rp.hfinal = h->hfinal.as<uint64_t>();
     upload_ctx(h);
-    hipLaunchKernelGGL(k_sample_init, dim3(256), dim3(256), 0, s, d_rp(h));
+    h->ghist.ensure(65 * 4);
+    hipLaunchKernelGGL(k_sample_init, dim3(256), dim3(256), 0, s, d_rp(h), h->dplan.as<DevPlan>(),
+                       h->ghist.as<uint32_t>());
     if (h->rows_total) {  // evenly strided over the batches' concatenation
         const uint64_t want = std::min<uint64_t>(S_TOTAL, h->rows_total);
         const uint64_t stride = std::max<uint64_t>(1, h->rows_total / want);
@@ -1548,17 +1575,13 @@ void run_sample(gbagg_handle *h) {
         hipLaunchKernelGGL(k_sample, dim3(grid), dim3(256), 0, s, (const DevCtx *)dctx(h), batch_rows(h, 1),
                            (uint32_t)want, stride, h->dplan.as<DevPlan>());
     }
-    h->ghist.ensure(65 * 4);
-    HIPCHK(hipMemsetAsync(h->ghist.ptr, 0, 65 * 4, s));
     // the histogram runs in few blocks (each ends with device-scope atomics on
     // the same plan words); the insert takes one atomic per wave
     const uint32_t hgrid = (uint32_t)std::min<uint64_t>((rp.scap + 255) / 256, 128);
     const uint32_t sgrid = (uint32_t)std::min<uint64_t>((rp.scap + 255) / 256, (uint64_t)di.cus * 2);
-    hipLaunchKernelGGL(k_key_range_init, dim3(1), dim3(1), 0, s, h->dplan.as<DevPlan>());  // sampled key range
     hipLaunchKernelGGL(k_heavy_hist, dim3(hgrid), dim3(256), 0, s, d_rp(h), h->dplan.as<DevPlan>(), h->ghist.as<uint32_t>());
-    hipLaunchKernelGGL(k_heavy_threshold, dim3(1), dim3(64), 0, s, d_rp(h), h->dplan.as<DevPlan>(),
+    hipLaunchKernelGGL(k_heavy_insert, dim3(sgrid), dim3(256), 0, s, d_rp(h), h->dplan.as<DevPlan>(),
                        (const uint32_t *)h->ghist.as<uint32_t>());
-    hipLaunchKernelGGL(k_heavy_insert, dim3(sgrid), dim3(256), 0, s, d_rp(h), h->dplan.as<DevPlan>());
 }
 
 // Selector features of the sample just taken, and with a selector its
@@ -1635,7 +1658,7 @@ void fx_prepare(gbagg_handle *h, bool sampled) {
         h->hctx->q.acc_fx[pr.first] = h->hctx->q.acc_fx[pr.second] = sc;
     }
     fill_out_topn(h);
-    upload_ctx(h);
+    h->ctx_stale = true;  // uploaded by the strategy that runs next
 }
 
 // The selector's outputs of the last sample (h->pred) as a distinct count and
@@ -1673,7 +1696,8 @@ void run_lds_hash(gbagg_handle *h, Timer &tm, bool global_only, uint64_t est = 0
     for (;;) {
         ensure_table(h, cap);
         DevTable T = make_table(h);
-        upload_plan(h, cap);
+        if (h->ctx_stale) upload_ctx_plan(h, cap);
+        else upload_plan(h, cap);
         uint32_t grid = (uint32_t)std::min<uint64_t>((cap + 2 + 255) / 256, (uint64_t)di.cus * 8);
         hipLaunchKernelGGL(k_table_init, dim3(grid), dim3(256), 0, s, T, d_q(h));
         tm.mark();  // aggregate start
@@ -1760,15 +1784,16 @@ bool run_radix(gbagg_handle *h, Timer &tm, uint64_t &est_out) {
     rp.hist = h->hist.as<uint32_t>();
     rp.hll = h->hll.as<uint8_t>();
     rp.hpart = h->hpart.as<uint64_t>();
-    HIPCHK(hipMemsetAsync(rp.hll, 0, (size_t)W * HLL_M, s));
+    // every count workgroup writes its whole sketch row; with no rows the one
+    // row the merge reads is cleared here
+    if (h->rows_total == 0) HIPCHK(hipMemsetAsync(rp.hll, 0, (size_t)W * HLL_M, s));
 
     // fallback global table (small; NULL / SENTINEL keys and LDS overflow)
     uint64_t cap = h->radix_fallback_cap;
     for (;;) {
         ensure_table(h, cap);
         DevTable T = make_table(h);
-        upload_plan(h, cap);
-        upload_ctx(h);
+        upload_ctx_plan(h, cap);
         uint32_t grid = (uint32_t)std::min<uint64_t>((cap + 2 + 255) / 256, (uint64_t)di.cus * 8);
         hipLaunchKernelGGL(k_table_init, dim3(grid), dim3(256), 0, s, T, d_q(h));
         tm.mark();  // count start
@@ -1815,10 +1840,6 @@ bool run_radix(gbagg_handle *h, Timer &tm, uint64_t &est_out) {
         rp.ptotal = h->ptotal.as<uint32_t>();
         rp.pstart = h->pstart.as<uint64_t>();
         rp.recs = h->recs.as<uint64_t>();
-        upload_ctx(h);
-        hipLaunchKernelGGL(k_rp_scan_w, dim3((P + SCAN_PARTS - 1) / SCAN_PARTS), dim3(SCAN_PARTS * SCAN_CHUNKS), 0, s,
-                           d_rp(h));
-        hipLaunchKernelGGL(k_rp_scan_p, dim3(1), dim3(1024), 0, s, d_rp(h), T.plan);
         // record ring: as deep as the LDS left next to the cursors allows
         uint32_t ring = 2048;
         while (ring > 256 && scatter_lds_bytes(rp.hslots, nacc, rp.hmax, P, ring, rp.rec_words) > di.lds_per_block)
@@ -1827,6 +1848,9 @@ bool run_radix(gbagg_handle *h, Timer &tm, uint64_t &est_out) {
         const size_t k3_lds = scatter_lds_bytes(rp.hslots, nacc, rp.hmax, P, ring, rp.rec_words);
         if (k3_lds > di.lds_per_block) fail(GBAGG_ERROR, "scatter kernel LDS exceeds the device limit");
         upload_ctx(h);
+        hipLaunchKernelGGL(k_rp_scan_w, dim3((P + SCAN_PARTS - 1) / SCAN_PARTS), dim3(SCAN_PARTS * SCAN_CHUNKS), 0, s,
+                           d_rp(h));
+        hipLaunchKernelGGL(k_rp_scan_p, dim3(1), dim3(1024), 0, s, d_rp(h), T.plan);
         allow_lds(h->kern->scatter, k3_lds);
         tm.mark();  // scatter start
         if (W1)
@@ -1918,7 +1942,8 @@ void run_direct(gbagg_handle *h, Timer &tm, uint64_t span, int64_t kmin) {
     const uint64_t cap = std::max<uint64_t>(1024, next_pow2(span));
     ensure_table(h, cap);
     DevTable T = make_table(h);
-    upload_plan(h, cap);
+    if (h->ctx_stale) upload_ctx_plan(h, cap);
+    else upload_plan(h, cap);
     const uint32_t igrid = (uint32_t)std::min<uint64_t>((cap + 2 + 255) / 256, (uint64_t)di.cus * 8);
     hipLaunchKernelGGL(k_table_init, dim3(igrid), dim3(256), 0, s, T, d_q(h));
     tm.mark();  // aggregate start
@@ -2610,8 +2635,7 @@ void finalize_once(gbagg_handle *h) {
     h->hctx->q = h->dq;
     fill_out_topn(h);
     upload_batches(h);
-    upload_ctx(h);
-    upload_plan(h, 1024);
+    upload_ctx_plan(h, 1024);
     uint64_t span = 0;
     int64_t kmin = 0;
     if (st == GBAGG_STRATEGY_DIRECT || (st == GBAGG_STRATEGY_GLOBAL_HASH && h->dq.key_mode == KM_NONE))
@@ -2712,8 +2736,7 @@ bool selector_plan(gbagg_handle *h, double &distinct, int &path) {
     h->dq.fx_flags = &h->dplan.as<DevPlan>()->overflow;
     h->hctx->q = h->dq;
     upload_batches(h);
-    upload_ctx(h);
-    upload_plan(h, 1024);
+    upload_ctx_plan(h, 1024);
     run_sample(h);
     if (h->sel_params.empty()) {
         h->sel_params.assign(kSelectorBuiltin, kSelectorBuiltin + SEL_NPARAM);

@@ -24,7 +24,7 @@ namespace gbagg {
 
 constexpr int AGG_BLOCK = 256;
 constexpr int AGG_ROWS = 4;          // rows per thread per tile
-constexpr int COMPACT_CHUNK = 4096;  // table slots per compaction block
+constexpr int COMPACT_CHUNK = 1024;  // table slots per compaction block
 constexpr uint32_t INVALID_SLOT = 0xFFFFFFFFu;
 
 // ------------------------------------------------------------------------
@@ -74,7 +74,10 @@ __device__ __forceinline__ void hll_add(uint32_t *reg, uint64_t h) {
     uint32_t idx = (uint32_t)(h >> (64 - HLL_P));
     uint64_t rest = h << HLL_P;
     uint32_t rank = rest ? (uint32_t)__clzll((long long)rest) + 1 : (64 - HLL_P + 1);
-    __hip_atomic_fetch_max(&reg[idx], rank, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    // registers only grow: once warm, almost every row's rank is no larger
+    // than its register, and a plain LDS read replaces the atomic
+    if (__hip_atomic_load(&reg[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) < rank)
+        __hip_atomic_fetch_max(&reg[idx], rank, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
 // Register-wise max over `nparts` sketches: thread per register, blockIdx.y
@@ -104,10 +107,19 @@ __global__ __launch_bounds__(1024) void k_hll_finish(const uint8_t *partial, uin
     __shared__ uint32_t s_zero[1024];
     double sum = 0.0;
     uint32_t zeros = 0;
-    for (int i = threadIdx.x; i < HLL_M; i += blockDim.x) {
-        uint32_t m = 0;
+    // thread t owns registers 4t .. 4t + 3 (HLL_M == 4 * 1024), read as one
+    // word per sketch; the bytewise max of a word is four register maxima
+    static_assert(HLL_M == 4 * 1024, "one word of registers per thread");
+    uint32_t m4 = 0;
 #pragma unroll 8
-        for (uint32_t p = 0; p < nparts; p++) m = max(m, (uint32_t)partial[(uint64_t)p * HLL_M + i]);
+    for (uint32_t p = 0; p < nparts; p++) {
+        const uint32_t x = ((const uint32_t *)(partial + (uint64_t)p * HLL_M))[threadIdx.x];
+        m4 = max(m4 & 0xFFu, x & 0xFFu) | max(m4 & 0xFF00u, x & 0xFF00u) | max(m4 & 0xFF0000u, x & 0xFF0000u) |
+             max(m4 & 0xFF000000u, x & 0xFF000000u);
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const uint32_t m = (m4 >> (8 * k)) & 0xFFu;
         sum += ldexp(1.0, -(int)m);
         zeros += (m == 0);
     }

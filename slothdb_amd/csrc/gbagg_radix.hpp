@@ -80,7 +80,9 @@ struct DevCtx {
     RadixParams rp;
 };
 
-__global__ void k_sample_init(const RadixParams *__restrict__ rpp) {
+// Clears the sample table, the heavy table and the count histogram of the
+// heavy-key selection (ghist, 65 bins), and presets the sampled key range.
+__global__ void k_sample_init(const RadixParams *__restrict__ rpp, DevPlan *plan, uint32_t *ghist) {
     const RadixParams &rp = *rpp;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < rp.scap; i += gridDim.x * blockDim.x) {
         rp.skeys[i] = SENTINEL;
@@ -88,6 +90,14 @@ __global__ void k_sample_init(const RadixParams *__restrict__ rpp) {
     }
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < rp.hslots; i += gridDim.x * blockDim.x)
         rp.hkeys[i] = SENTINEL;
+    if (blockIdx.x == 0) {
+        if (threadIdx.x < 65) ghist[threadIdx.x] = 0;
+        if (threadIdx.x == 0) {
+            plan->key_min = INT64_MAX;
+            plan->key_max = INT64_MIN;
+            plan->heavy_count = 0;
+        }
+    }
 }
 
 // Sample keys are first counted in a small per-block LDS table, so a hot key
@@ -167,10 +177,27 @@ __global__ __launch_bounds__(256) void k_sample(const DevCtx *__restrict__ cx, B
     if (threadIdx.x == 0) s_kept = 0;
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) kept += __shfl_xor(kept, off);
+    __shared__ uint32_t s_nocc;
+    __shared__ uint16_t s_occ[SAMPLE_LDS];
+    if (threadIdx.x == 0) s_nocc = 0;
     __syncthreads();
     if ((threadIdx.x & 63) == 0 && kept) atomicAdd(&s_kept, kept);
-    for (uint32_t i = threadIdx.x; i < SAMPLE_LDS; i += blockDim.x)
-        if (lc[i]) sample_global_add(rp, lk[i], lc[i]);
+    // occupied slots first gathered into a dense list (ballot ranks, one LDS
+    // atomic per wave), so the device-scope adds run in as few rounds as the
+    // block has distinct keys per thread, not SAMPLE_LDS / blockDim.x rounds
+    const uint32_t lane = threadIdx.x & 63;
+    for (uint32_t i0 = 0; i0 < SAMPLE_LDS; i0 += blockDim.x) {  // uniform
+        const uint32_t i = i0 + threadIdx.x;
+        const bool occ = i < SAMPLE_LDS && lc[i] != 0;
+        const uint64_t m = __ballot(occ);
+        uint32_t base = 0;
+        if (lane == 0 && m)
+            base = __hip_atomic_fetch_add(&s_nocc, (uint32_t)__popcll(m), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        base = __shfl(base, 0);
+        if (occ) s_occ[base + __popcll(m & ((1ULL << lane) - 1))] = (uint16_t)i;
+    }
+    __syncthreads();
+    for (uint32_t j = threadIdx.x; j < s_nocc; j += blockDim.x) sample_global_add(rp, lk[s_occ[j]], lc[s_occ[j]]);
     __syncthreads();
     if (threadIdx.x == 0 && s_kept) atomicAdd((unsigned long long *)&plan->sample_rows, (unsigned long long)s_kept);
     if (threadIdx.x < MAXU && s_fx[threadIdx.x])
@@ -193,22 +220,39 @@ __global__ __launch_bounds__(256) void k_heavy_hist(const RadixParams *__restric
     if (threadIdx.x == 0) { s_distinct = 0; s_f1 = 0; s_f2 = 0; s_f3 = 0; s_max = 0; s_ent = 0; s_lo = INT64_MAX; s_hi = INT64_MIN; }
     __syncthreads();
     int64_t lo = INT64_MAX, hi = INT64_MIN;  // sampled key word range (plan->key_min/max preset)
-    unsigned long long f2 = 0, f3 = 0, mx = 0, ent = 0;
+    unsigned long long dist = 0, f1 = 0, f2 = 0, f3 = 0, mx = 0, ent = 0;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < rp.scap; i += gridDim.x * blockDim.x) {
         uint32_t c = rp.scnt[i];
         if (!c) continue;
-        atomicAdd(&hist[min(c, 64u)], 1u);
-        atomicAdd(&s_distinct, 1ULL);
-        if (c == 1) atomicAdd(&s_f1, 1ULL);
+        dist++;
+        // singletons (most keys) only count locally: bins 1 of both histograms
+        // take the block's singleton count at the end
+        if (c == 1) f1++;
+        else {
+            atomicAdd(&hist[min(c, 64u)], 1u);
+            atomicAdd(&s_h16[min(15u, 31u - (uint32_t)__clz(c))], 1ULL);
+        }
         f2 += c == 2;
         f3 += c == 3;
         mx = mx > c ? mx : c;
         // c * log2(c) in 2^-20 units: integer sums, so the features do not
         // depend on the order of the additions
         ent += (unsigned long long)((double)c * log2((double)c) * 1048576.0 + 0.5);
-        atomicAdd(&s_h16[min(15u, 31u - (uint32_t)__clz(c))], 1ULL);
         lo = min(lo, (int64_t)rp.skeys[i]);
         hi = max(hi, (int64_t)rp.skeys[i]);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        dist += __shfl_xor(dist, off);
+        f1 += __shfl_xor(f1, off);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        if (dist) atomicAdd(&s_distinct, dist);
+        if (f1) {
+            atomicAdd(&s_f1, f1);
+            atomicAdd(&hist[1], (uint32_t)f1);
+            atomicAdd(&s_h16[0], f1);
+        }
     }
     if (f2) atomicAdd(&s_f2, f2);
     if (f3) atomicAdd(&s_f3, f3);
@@ -241,9 +285,9 @@ __global__ __launch_bounds__(256) void k_heavy_hist(const RadixParams *__restric
     }
 }
 
-__global__ void k_heavy_threshold(const RadixParams *__restrict__ rpp, DevPlan *plan, const uint32_t *ghist) {
-    const RadixParams &rp = *rpp;
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// Smallest sample count >= heavy_min that admits at most hmax keys (65:
+// none). Every k_heavy_insert block computes it from the 65-bin histogram.
+__device__ __forceinline__ uint32_t heavy_threshold(const RadixParams &rp, const uint32_t *ghist) {
     uint32_t t = 65, above = 0;
     for (uint32_t c = 64; c >= max(rp.heavy_min, 1u); c--) {
         if (above + ghist[c] > rp.hmax) break;
@@ -251,15 +295,24 @@ __global__ void k_heavy_threshold(const RadixParams *__restrict__ rpp, DevPlan *
         t = c;
     }
     if (t == 65 && ghist[64] > 0) t = 64;  // more than hmax very frequent keys: take hmax of them
-    plan->heavy_threshold = t;
-    plan->heavy_count = 0;
+    return t;
 }
 
 // Dense indexes come from one device-scope atomic per wave and round (ballot
 // rank within the wave): same-address device atomics serialise.
-__global__ __launch_bounds__(256) void k_heavy_insert(const RadixParams *__restrict__ rpp, DevPlan *plan) {
+// heavy_count starts at zero (k_sample_init).
+__global__ __launch_bounds__(256) void k_heavy_insert(const RadixParams *__restrict__ rpp, DevPlan *plan,
+                                                      const uint32_t *ghist) {
     const RadixParams &rp = *rpp;
-    const uint32_t t = plan->heavy_threshold;
+    __shared__ uint32_t s_t, s_hist[65];
+    if (threadIdx.x < 65) s_hist[threadIdx.x] = ghist[threadIdx.x];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        s_t = heavy_threshold(rp, s_hist);
+        if (blockIdx.x == 0) plan->heavy_threshold = s_t;
+    }
+    __syncthreads();
+    const uint32_t t = s_t;
     const uint32_t hmask = rp.hslots - 1;
     if (t > 64) return;
     const uint32_t lane = threadIdx.x & 63;
@@ -413,13 +466,16 @@ __global__ __launch_bounds__(SCAN_PARTS * SCAN_CHUNKS) void k_rp_scan_w(const Ra
     const uint32_t w0 = min(rp.W, c * per), w1 = min(rp.W, w0 + per);
     auto part_count = [&](uint32_t w) {
         const uint32_t *h = rp.hist + (uint64_t)w * nb + ((uint64_t)p << shift);
+        if (shift == 0) return h[0];
         uint32_t x = 0;
         for (uint32_t j = 0; j < (1u << shift); j++) x += h[j];
         return x;
     };
     uint32_t sum = 0;
-    if (p < P)
+    if (p < P) {
+#pragma unroll 8
         for (uint32_t w = w0; w < w1; w++) sum += part_count(w);
+    }
     s_sum[c][pl] = sum;
     __syncthreads();
     if (p >= P) return;
@@ -432,31 +488,46 @@ __global__ __launch_bounds__(SCAN_PARTS * SCAN_CHUNKS) void k_rp_scan_w(const Ra
     if (c == SCAN_CHUNKS - 1) rp.ptotal[p] = run;
 }
 
-// One block of 1024: exclusive scan of the partition totals.
+// One block of 1024: exclusive scan of the partition totals (wave scans by
+// shuffles, then the 16 wave totals).
 __global__ __launch_bounds__(1024) void k_rp_scan_p(const RadixParams *__restrict__ rpp, DevPlan *plan) {
     const RadixParams &rp = *rpp;
-    __shared__ uint64_t s_part[1024];
+    __shared__ uint64_t s_wave[16];
     const uint32_t P = 1u << rp.part_bits;
     const uint32_t per = (P + blockDim.x - 1) / blockDim.x;
-    const uint32_t p0 = threadIdx.x * per, p1 = min(P, p0 + per);
+    const uint32_t p0 = min(P, threadIdx.x * per), p1 = min(P, p0 + per);
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    // P <= 8192: at most 8 partitions per thread, loaded together
+    uint32_t tot[8];
+#pragma unroll
+    for (uint32_t j = 0; j < 8; j++) tot[j] = p0 + j < p1 ? rp.ptotal[p0 + j] : 0;
     uint64_t local = 0;
-    for (uint32_t p = p0; p < p1; p++) local += rp.ptotal[p];
-    s_part[threadIdx.x] = local;
+#pragma unroll
+    for (uint32_t j = 0; j < 8; j++) local += tot[j];
+    uint64_t incl = local;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint64_t y = __shfl_up(incl, off);
+        if (lane >= (uint32_t)off) incl += y;
+    }
+    if (lane == 63) s_wave[wave] = incl;
     __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        uint64_t v = threadIdx.x >= (uint32_t)off ? s_part[threadIdx.x - off] : 0;
-        __syncthreads();
-        s_part[threadIdx.x] += v;
-        __syncthreads();
+    uint64_t before = 0, total = 0;
+    for (uint32_t w = 0; w < 16; w++) {
+        const uint64_t x = s_wave[w];
+        before += w < wave ? x : 0;
+        total += x;
     }
-    uint64_t run = threadIdx.x ? s_part[threadIdx.x - 1] : 0;
-    for (uint32_t p = p0; p < p1; p++) {
-        rp.pstart[p] = run;
-        run += rp.ptotal[p];
-    }
-    if (threadIdx.x == blockDim.x - 1) {
-        rp.pstart[P] = s_part[blockDim.x - 1];
-        plan->cold_rows = s_part[blockDim.x - 1];
+    uint64_t run = before + incl - local;
+#pragma unroll
+    for (uint32_t j = 0; j < 8; j++)
+        if (p0 + j < p1) {
+            rp.pstart[p0 + j] = run;
+            run += tot[j];
+        }
+    if (threadIdx.x == 0) {
+        rp.pstart[P] = total;
+        plan->cold_rows = total;
     }
 }
 
