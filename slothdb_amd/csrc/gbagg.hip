@@ -46,7 +46,7 @@ namespace {
 // Queries whose shape matches one of these run straight-line specialisations
 // of the row kernels; every other query runs the same kernels over DynShape.
 using KAgg = void (*)(const DevQuery *, BatchRows, DevTable, uint32_t, uint32_t);
-using KCount = void (*)(const DevCtx *, BatchRows);
+using KCount = void (*)(const DevCtx *, BatchRows, const uint32_t *);
 using KScatter = void (*)(const DevCtx *, BatchRows, DevTable);
 using KPart = void (*)(const DevCtx *, DevTable, uint32_t, uint32_t);
 using KRange = void (*)(const DevQuery *, BatchRows, DevPlan *);
@@ -178,6 +178,14 @@ DeviceInfo &device_info() {
 // Device buffer that only grows.
 // Bytes of the parameter block ahead of the plan block in one allocation.
 constexpr size_t CTX_BYTES = (sizeof(DevCtx) + 255) & ~(size_t)255;
+
+// Launch shape of the radix count and scatter passes.
+struct RadixLaunch {
+    uint32_t W1 = 0;      // count workgroups (0 without rows)
+    uint64_t per1 = 0;    // rows per count workgroup
+    uint64_t per3 = 0;    // rows per scatter workgroup
+    size_t k1_lds = 0;    // count pass LDS bytes
+};
 
 // A device block owned elsewhere.
 struct PlanRef {
@@ -383,6 +391,7 @@ struct gbagg_handle {
     DevPlan *hplan = nullptr;  // pinned
     DevCtx *hctx = nullptr;    // pinned parameter blocks: hctx, then hplan at CTX_BYTES
     bool ctx_stale = false;    // hctx changed since its last upload
+    RadixLaunch rl;            // radix passes of the current finalize
     DevBatch *hbatches = nullptr;
     size_t hbatch_cap = 0;
     DevBuf ctx, dbatches;
@@ -1444,6 +1453,7 @@ void prepare_plan(gbagg_handle *h, uint64_t cap) {
     h->hplan->heavy_count = keep.heavy_count;
     h->hplan->heavy_sample_rows = keep.heavy_sample_rows;
     h->hplan->heavy_threshold = keep.heavy_threshold;
+    h->hplan->est_groups = keep.est_groups;  // a count pass run ahead of the plan
     h->hplan->cap = cap;
     h->hplan->cap_max = h->cap_max;
 }
@@ -1746,35 +1756,39 @@ void run_lds_hash(gbagg_handle *h, Timer &tm, bool global_only, uint64_t est = 0
 // Returns false, with nothing aggregated, when the HyperLogLog count of cold
 // groups exceeds what the largest fanout's LDS tables hold (est_out: the
 // count); the caller then runs the global table.
-bool run_radix(gbagg_handle *h, Timer &tm, uint64_t &est_out) {
+// Inputs from this many rows run the radix count pass ahead of the plan.
+constexpr uint64_t RADIX_SPEC_ROWS = 4u << 20;
+
+// Launch shape of the radix count and scatter passes, and their buffers
+// (parameter block fields; uploaded with it).
+void radix_setup(gbagg_handle *h) {
     const DeviceInfo &di = device_info();
-    hipStream_t s = h->stream;
     const int nacc = h->dq.nacc;
     RadixParams &rp = h->hctx->rp;
+    RadixLaunch &rl = h->rl;
     rp.bucket_bits = 13;
     const uint32_t nb = 1u << rp.bucket_bits;
     record_layout(h);
     // workgroup slots for count + scatter
     const size_t k3_fixed = scatter_lds_bytes(rp.hslots, nacc, rp.hmax, 0, 256, rp.rec_words);
-    const size_t k1_lds = (size_t)rp.hslots * 8 + (size_t)nb * 4 + HLL_M * 4;
+    rl.k1_lds = (size_t)rp.hslots * 8 + (size_t)nb * 4 + HLL_M * 4;
     const size_t k3_lds_max = k3_fixed + (size_t)nb * 4;
     // The scatter pass runs as many workgroups per CU as its LDS allows; the
     // lighter count pass runs g1 times as many, each over 1/g1 of a scatter
     // workgroup's rows, so its histogram rows stay contiguous per scatter
     // workgroup.
     const uint32_t per_cu3 = (uint32_t)std::max<size_t>(1, std::min<size_t>(4, di.lds_per_cu / k3_lds_max));
-    const uint32_t per_cu1 = (uint32_t)std::max<size_t>(1, std::min<size_t>(4, di.lds_per_cu / k1_lds));
+    const uint32_t per_cu1 = (uint32_t)std::max<size_t>(1, std::min<size_t>(4, di.lds_per_cu / rl.k1_lds));
     const uint32_t g1 = std::max<uint32_t>(1, per_cu1 / per_cu3);
     // one launch per pass over every batch: scatter workgroup k takes rows
     // [k * per3, (k + 1) * per3) of the batches' concatenation, count
     // workgroup w rows [w * per1, (w + 1) * per1), per3 = g1 * per1
     const uint64_t tile = (uint64_t)RP_BLOCK * AGG_ROWS * 2 * g1;
-    uint64_t per3;
-    (void)tiled_grid(h->rows_total, tile, (uint64_t)di.cus * per_cu3, per3);
-    const uint64_t per1 = per3 / g1;
-    const uint32_t W1 = h->rows_total ? (uint32_t)((h->rows_total + per1 - 1) / per1) : 0;
-    const uint32_t W = std::max<uint32_t>(W1, 1);
-    const uint32_t W3 = std::max<uint32_t>((W1 + g1 - 1) / g1, 1);
+    (void)tiled_grid(h->rows_total, tile, (uint64_t)di.cus * per_cu3, rl.per3);
+    rl.per1 = rl.per3 / g1;
+    rl.W1 = h->rows_total ? (uint32_t)((h->rows_total + rl.per1 - 1) / rl.per1) : 0;
+    const uint32_t W = std::max<uint32_t>(rl.W1, 1);
+    const uint32_t W3 = std::max<uint32_t>((rl.W1 + g1 - 1) / g1, 1);
     rp.W = W;
     rp.W3 = W3;
     rp.g1 = g1;
@@ -1786,29 +1800,56 @@ bool run_radix(gbagg_handle *h, Timer &tm, uint64_t &est_out) {
     rp.hpart = h->hpart.as<uint64_t>();
     // every count workgroup writes its whole sketch row; with no rows the one
     // row the merge reads is cleared here
-    if (h->rows_total == 0) HIPCHK(hipMemsetAsync(rp.hll, 0, (size_t)W * HLL_M, s));
+    if (h->rows_total == 0) HIPCHK(hipMemsetAsync(rp.hll, 0, (size_t)W * HLL_M, h->stream));
+}
+
+// The count pass and the HyperLogLog estimate of its keys (into the plan
+// block). gate: see k_rp_count.
+void radix_count(gbagg_handle *h, const uint32_t *gate) {
+    hipStream_t s = h->stream;
+    const RadixParams &rp = h->hctx->rp;
+    const RadixLaunch &rl = h->rl;
+    allow_lds(h->kern->count, rl.k1_lds);
+    if (rl.W1)
+        hipLaunchKernelGGL(h->kern->count, dim3(rl.W1), dim3(RP_BLOCK), rl.k1_lds, s, (const DevCtx *)dctx(h),
+                           batch_rows(h, rl.per1), gate);
+    const uint32_t hslices = std::min<uint32_t>(32, rp.W);
+    h->hllm.ensure((size_t)HLL_M * hslices);
+    hipLaunchKernelGGL(k_hll_merge, dim3(HLL_M / 256, hslices), dim3(256), 0, s, (const uint8_t *)rp.hll, rp.W,
+                       h->hllm.as<uint8_t>(), gate);
+    hipLaunchKernelGGL(k_hll_finish, dim3(1), dim3(1024), 0, s, (const uint8_t *)h->hllm.as<uint8_t>(), hslices,
+                       h->dplan.as<DevPlan>(), gate);
+}
+
+// counted: radix_setup and radix_count already ran for this finalize and the
+// plan block read back holds their estimate.
+bool run_radix(gbagg_handle *h, Timer &tm, uint64_t &est_out, bool counted) {
+    const DeviceInfo &di = device_info();
+    hipStream_t s = h->stream;
+    const int nacc = h->dq.nacc;
+    RadixParams &rp = h->hctx->rp;
+    if (!counted) radix_setup(h);
+    const RadixLaunch &rl = h->rl;
+    const uint32_t W1 = rl.W1;
+    const uint64_t per3 = rl.per3;
+    const uint32_t W = rp.W, W3 = rp.W3;
 
     // fallback global table (small; NULL / SENTINEL keys and LDS overflow)
     uint64_t cap = h->radix_fallback_cap;
     for (;;) {
         ensure_table(h, cap);
         DevTable T = make_table(h);
-        upload_ctx_plan(h, cap);
-        uint32_t grid = (uint32_t)std::min<uint64_t>((cap + 2 + 255) / 256, (uint64_t)di.cus * 8);
-        hipLaunchKernelGGL(k_table_init, dim3(grid), dim3(256), 0, s, T, d_q(h));
-        tm.mark();  // count start
-        allow_lds(h->kern->count, k1_lds);
-        if (W1)
-            hipLaunchKernelGGL(h->kern->count, dim3(W1), dim3(RP_BLOCK), k1_lds, s, (const DevCtx *)dctx(h),
-                               batch_rows(h, per1));
-        const uint32_t hslices = std::min<uint32_t>(32, W);
-        h->hllm.ensure((size_t)HLL_M * hslices);
-        hipLaunchKernelGGL(k_hll_merge, dim3(HLL_M / 256, hslices), dim3(256), 0, s, (const uint8_t *)rp.hll, W,
-                           h->hllm.as<uint8_t>());
-        hipLaunchKernelGGL(k_hll_finish, dim3(1), dim3(1024), 0, s, (const uint8_t *)h->hllm.as<uint8_t>(), hslices,
-                           T.plan);
-        tm.mark();  // count end
-        read_plan(h);  // HLL estimate and heavy count pick the fanout
+        const uint32_t grid = (uint32_t)std::min<uint64_t>((cap + 2 + 255) / 256, (uint64_t)di.cus * 8);
+        const bool fresh = !counted;
+        counted = false;  // a retry counts again
+        if (fresh) {
+            upload_ctx_plan(h, cap);
+            hipLaunchKernelGGL(k_table_init, dim3(grid), dim3(256), 0, s, T, d_q(h));
+            tm.mark();  // count start
+            radix_count(h, nullptr);
+            tm.mark();  // count end
+            read_plan(h);  // HLL estimate and heavy count pick the fanout
+        }
         const uint64_t est = h->hplan->est_groups;
         const uint64_t heavy = h->hplan->heavy_count;
         const uint64_t cold = est > heavy ? est - heavy : 1;
@@ -1847,7 +1888,12 @@ bool run_radix(gbagg_handle *h, Timer &tm, uint64_t &est_out) {
         rp.ring = ring;
         const size_t k3_lds = scatter_lds_bytes(rp.hslots, nacc, rp.hmax, P, ring, rp.rec_words);
         if (k3_lds > di.lds_per_block) fail(GBAGG_ERROR, "scatter kernel LDS exceeds the device limit");
-        upload_ctx(h);
+        if (fresh) {
+            upload_ctx(h);
+        } else {  // the count ran ahead: parameters, plan and table set-up in one go
+            upload_ctx_plan(h, cap);
+            hipLaunchKernelGGL(k_table_init, dim3(grid), dim3(256), 0, s, T, d_q(h));
+        }
         hipLaunchKernelGGL(k_rp_scan_w, dim3((P + SCAN_PARTS - 1) / SCAN_PARTS), dim3(SCAN_PARTS * SCAN_CHUNKS), 0, s,
                            d_rp(h));
         hipLaunchKernelGGL(k_rp_scan_p, dim3(1), dim3(1024), 0, s, d_rp(h), T.plan);
@@ -2027,9 +2073,9 @@ void intern_run(gbagg_handle *h, StrCol &sc, const std::vector<StrSeg> &segs, ui
     hipLaunchKernelGGL(k_str_hash, dim3(W), dim3(256), 0, s, dsegs, nseg, total, t, sc.hash.as<uint64_t>(),
                        sc.pos.as<uint32_t>(), sc.codes.as<uint32_t>(), sc.hllp.as<uint8_t>());
     hipLaunchKernelGGL(k_hll_merge, dim3(HLL_M / 256, hslices), dim3(256), 0, s, (const uint8_t *)sc.hllp.ptr, W,
-                       sc.hllm.as<uint8_t>());
+                       sc.hllm.as<uint8_t>(), (const uint32_t *)nullptr);
     hipLaunchKernelGGL(k_hll_finish, dim3(1), dim3(1024), 0, s, (const uint8_t *)sc.hllm.ptr, hslices,
-                       sc.plan.as<DevPlan>());
+                       sc.plan.as<DevPlan>(), (const uint32_t *)nullptr);
     DevPlan pl{};
     HIPCHK(hipMemcpyAsync(ctr, t.ctr, SC_N * 8, hipMemcpyDeviceToHost, s));
     HIPCHK(hipMemcpyAsync(&pl, sc.plan.ptr, sizeof(DevPlan), hipMemcpyDeviceToHost, s));
@@ -2665,7 +2711,23 @@ void finalize_once(gbagg_handle *h) {
         const bool sel = !h->sel_params.empty();
         if (sel || h->collect_features) run_features(h, sel);
         tm.mark();  // sample end
+        // Inputs of many rows: the radix count pass runs before the plan is
+        // read back, so the host's round trip overlaps it and the count's
+        // estimate returns with the sample's statistics (no second round trip).
+        // Under AUTO it is gated on the device by the default plan's radix test
+        // (DevPlan::radix_gate); it is wasted only when the selector then picks
+        // another path.
+        const bool spec = (st == GBAGG_STRATEGY_AUTO || st == GBAGG_STRATEGY_RADIX) &&
+                          h->rows_total >= RADIX_SPEC_ROWS && h->dq.key_mode != KM_NONE;
+        if (spec) {
+            radix_setup(h);
+            upload_ctx(h);
+            tm.mark();  // count start
+            radix_count(h, st == GBAGG_STRATEGY_RADIX ? nullptr : &h->dplan.as<DevPlan>()->radix_gate);
+            tm.mark();  // count end
+        }
         read_plan(h);
+        const bool counted = spec && (st == GBAGG_STRATEGY_RADIX || h->hplan->radix_gate);
         if (sel || h->collect_features) take_features(h);
         fx_prepare(h, true);
         if (st == GBAGG_STRATEGY_AUTO) {
@@ -2697,9 +2759,11 @@ void finalize_once(gbagg_handle *h) {
             if (st == GBAGG_STRATEGY_DIRECT && !direct_range(h, span, kmin)) st = GBAGG_STRATEGY_LDS_HASH;
         }
         uint64_t est = 0;
+        // the count's marks stay only for a radix plan that uses its pass
+        if (spec && !(st == GBAGG_STRATEGY_RADIX && counted)) tm.truncate(2);
         if (st == GBAGG_STRATEGY_DIRECT) {
             run_direct(h, tm, span, kmin);
-        } else if (st == GBAGG_STRATEGY_RADIX && !run_radix(h, tm, est)) {
+        } else if (st == GBAGG_STRATEGY_RADIX && !run_radix(h, tm, est, counted)) {
             // more groups than the partitions' LDS tables hold: one global table
             st = GBAGG_STRATEGY_GLOBAL_HASH;
             run_lds_hash(h, tm, true, est);

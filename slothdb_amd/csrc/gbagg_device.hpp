@@ -123,7 +123,7 @@ struct DevPlan {
     uint64_t heavy_sample_rows;// sample rows whose key is heavy
     uint64_t cold_rows;        // rows scattered to partitions
     uint32_t heavy_threshold;  // minimum sample count of a heavy key
-    uint32_t pad;
+    uint32_t radix_gate;       // the sample is not of low cardinality (the default plan's radix test)
     uint64_t topn_lb;          // TopN: no group with a smaller order value can be in the top K
     int64_t key_min;           // direct-mapped strategy: exact key word range
     int64_t key_max;

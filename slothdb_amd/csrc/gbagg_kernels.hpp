@@ -83,7 +83,9 @@ __device__ __forceinline__ void hll_add(uint32_t *reg, uint64_t h) {
 // Register-wise max over `nparts` sketches: thread per register, blockIdx.y
 // picks one of gridDim.y slices of the sketches; slice y's merge goes to
 // merged[y * HLL_M ...] (k_hll_finish folds the slices).
-__global__ __launch_bounds__(256) void k_hll_merge(const uint8_t *partial, uint32_t nparts, uint8_t *merged) {
+__global__ __launch_bounds__(256) void k_hll_merge(const uint8_t *partial, uint32_t nparts, uint8_t *merged,
+                                                   const uint32_t *gate) {
+    if (gate && *gate == 0) return;  // gate: as k_rp_count's
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= HLL_M) return;
     const uint32_t per = (nparts + gridDim.y - 1) / gridDim.y;
@@ -102,7 +104,8 @@ __global__ __launch_bounds__(256) void k_hll_merge(const uint8_t *partial, uint3
 
 // One block of 1024 threads: max-merge `nparts` sketches, write the estimate.
 __global__ __launch_bounds__(1024) void k_hll_finish(const uint8_t *partial, uint32_t nparts,
-                                                     DevPlan *plan) {
+                                                     DevPlan *plan, const uint32_t *gate) {
+    if (gate && *gate == 0) return;
     __shared__ double s_sum[1024];
     __shared__ uint32_t s_zero[1024];
     double sum = 0.0;

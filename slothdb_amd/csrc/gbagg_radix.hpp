@@ -304,6 +304,12 @@ __device__ __forceinline__ uint32_t heavy_threshold(const RadixParams &rp, const
 __global__ __launch_bounds__(256) void k_heavy_insert(const RadixParams *__restrict__ rpp, DevPlan *plan,
                                                       const uint32_t *ghist) {
     const RadixParams &rp = *rpp;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        // the default plan's rule for few distinct keys (also on the host):
+        // a count pass launched before the plan is read runs only without it
+        const uint64_t d = plan->sample_distinct, f1 = plan->sample_f1;
+        plan->radix_gate = !(d <= 4096 && f1 * 8 <= d + 8);
+    }
     __shared__ uint32_t s_t, s_hist[65];
     if (threadIdx.x < 65) s_hist[threadIdx.x] = ghist[threadIdx.x];
     __syncthreads();
@@ -411,9 +417,13 @@ __device__ __forceinline__ int heavy_find(const uint64_t *hk, const uint32_t *hi
     return found ? (int)hi[pos] : -1;
 }
 
+// gate: null, or a word that must be non-zero for the pass to run (a count
+// launched ahead of the plan decision, see DevPlan::radix_gate).
 template <class S>
 __global__ __launch_bounds__(RP_BLOCK) __attribute__((amdgpu_waves_per_eu(8))) void k_rp_count(const DevCtx *__restrict__ cx,
-                                                                                                BatchRows br) {
+                                                                                                BatchRows br,
+                                                                                                const uint32_t *gate) {
+    if (gate && *gate == 0) return;
     const DevQuery &q = cx->q;
     const RadixParams &rp = cx->rp;
     const S sh(q);
