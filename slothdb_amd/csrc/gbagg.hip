@@ -1480,7 +1480,7 @@ void emit_table(gbagg_handle *h, const DevTable &T, uint64_t cap) {
     const DeviceInfo &di = device_info();
     hipStream_t s = h->stream;
     if (h->topn_device) {
-        uint32_t grid = (uint32_t)std::min<uint64_t>((cap + 2 + 4095) / 4096, (uint64_t)di.cus * 4);
+        uint32_t grid = (uint32_t)std::min<uint64_t>((cap + 2 + 1023) / 1024, (uint64_t)di.cus * 4);
         if (h->topn_k <= 16) hipLaunchKernelGGL(k_topn_table<16>, dim3(grid), dim3(256), 0, s, T, d_t(h));
         else hipLaunchKernelGGL(k_topn_table<32>, dim3(grid), dim3(256), 0, s, T, d_t(h));
     } else {
@@ -1854,7 +1854,7 @@ bool run_radix(gbagg_handle *h, Timer &tm, uint64_t &est_out, bool counted) {
         const uint64_t heavy = h->hplan->heavy_count;
         const uint64_t cold = est > heavy ? est - heavy : 1;
         const size_t k4_budget = std::min<size_t>(di.lds_per_block, (size_t)h->k4_lds_kb * 1024);
-        const uint32_t S = lds_slots_exact(k4_budget, nacc, RA_SMAX);
+        const uint32_t S = lds_slots_exact(k4_budget, nacc, h->topn_device ? RA_TOPN_SMAX : RA_SMAX);
         if (h->force_part_bits < 0 && (double)cold > 0.75 * (double)S * (double)(1u << rp.bucket_bits)) {
             est_out = est;
             return false;

@@ -855,10 +855,23 @@ template <int KMAX>
 __device__ void block_select(TopList<KMAX> &L, uint32_t K, uint32_t *s_win) {
     __shared__ uint64_t s_o[16], s_t[16];
     __shared__ uint32_t s_s[16], s_owner[16];
-    __shared__ uint32_t s_win_owner;
+    __shared__ uint32_t s_win_owner, s_valid;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = (blockDim.x + 63) >> 6;
+    // rounds: no more than the block's listed entries (often none: a producer
+    // whose groups all fell below the TopN bound)
+    if (threadIdx.x == 0) s_valid = 0;
+    __syncthreads();
+    uint32_t mine = 0;
+#pragma unroll
+    for (int i = 0; i < KMAX; i++) mine += (uint32_t)i < K && L.s[i] != INVALID_SLOT;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) mine += __shfl_xor(mine, off);
+    if (lane == 0 && mine) atomicAdd(&s_valid, mine);
+    __syncthreads();
+    const uint32_t rounds = min(K, s_valid);
+    for (uint32_t r = rounds + threadIdx.x; r < K; r += blockDim.x) s_win[r] = INVALID_SLOT;
     uint32_t used = 0;  // bit i: list entry i already emitted
-    for (uint32_t r = 0; r < K; r++) {
+    for (uint32_t r = 0; r < rounds; r++) {
         uint64_t bo = 0, bt = 0;
         uint32_t bs = INVALID_SLOT;
         int bi = -1;
@@ -891,6 +904,7 @@ __device__ void block_select(TopList<KMAX> &L, uint32_t K, uint32_t *s_win) {
         if (s_win_owner == threadIdx.x && bi >= 0) used |= 1u << bi;
         __syncthreads();
     }
+    __syncthreads();
 }
 
 // One round of wave-wide argmax under topn_better: every lane ends with the
@@ -916,8 +930,19 @@ __device__ void block_topk(const uint64_t (&o)[NI], const uint64_t (&t)[NI], con
     __shared__ uint64_t w_o[16 * KMAX], w_t[16 * KMAX];
     __shared__ uint32_t w_s[16 * KMAX];
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = (blockDim.x + 63) >> 6;
+    // rounds: no more than the entries present (wave-wide counts by ballot)
+    uint32_t nw = 0;
+#pragma unroll
+    for (int j = 0; j < NI; j++) nw += (uint32_t)__popcll(__ballot(s[j] != INVALID_SLOT));
+    const uint32_t rounds1 = min(K, nw);
+    if (lane == 0)
+        for (uint32_t r = rounds1; r < K; r++) {
+            w_o[wave * KMAX + r] = 0;
+            w_t[wave * KMAX + r] = 0;
+            w_s[wave * KMAX + r] = INVALID_SLOT;
+        }
     uint32_t taken = 0;
-    for (uint32_t r = 0; r < K; r++) {
+    for (uint32_t r = 0; r < rounds1; r++) {
         uint64_t bo = 0, bt = 0;
         uint32_t bs = INVALID_SLOT;
         int bj = -1;
@@ -948,8 +973,13 @@ __device__ void block_topk(const uint64_t (&o)[NI], const uint64_t (&t)[NI], con
             et[j] = ok ? w_t[e] : 0;
             es[j] = ok ? w_s[e] : INVALID_SLOT;
         }
+        uint32_t ne = 0;
+#pragma unroll
+        for (int j = 0; j < NE; j++) ne += (uint32_t)__popcll(__ballot(es[j] != INVALID_SLOT));
+        const uint32_t rounds2 = min(K, ne);
+        for (uint32_t r = rounds2 + lane; r < K; r += 64) s_win[r] = INVALID_SLOT;
         uint32_t tk = 0;
-        for (uint32_t r = 0; r < K; r++) {
+        for (uint32_t r = 0; r < rounds2; r++) {
             uint64_t bo = 0, bt = 0;
             uint32_t bs = INVALID_SLOT;
             int bj = -1;
@@ -1102,12 +1132,13 @@ __global__ __launch_bounds__(256) void k_topn_table(DevTable T, const DevTopn *_
     L.init();
     const uint64_t per = (nrec + gridDim.x - 1) / gridDim.x;
     const uint64_t s0 = (uint64_t)blockIdx.x * per, s1 = min(nrec, s0 + per);
+    const uint64_t lb = T.plan->topn_lb;  // groups below it cannot be in the top K
     for (uint64_t s = s0 + threadIdx.x; s < s1; s += blockDim.x) {
         const uint64_t *rec = T.rec + s * T.stride;
         if (!slot_occupied(rec, s, cap)) continue;
         uint64_t o, ti;
         group_order(t, rec[0], rec + 1, 1, s == cap + 1, o, ti);
-        L.offer(o, ti, (uint32_t)s, t.k);
+        if (o >= lb) L.offer(o, ti, (uint32_t)s, t.k);
     }
     block_select<KMAX>(L, t.k, s_win);
     topn_push(t, T.plan, s_win, [&](uint32_t s, uint64_t &key, const uint64_t *&acc, uint32_t &stride, bool &nk) {
@@ -1130,6 +1161,9 @@ __global__ __launch_bounds__(1024) void k_topn_level1(const DevTopn *__restrict_
     const DevTopn &t = *tp;
     __shared__ uint32_t s_win[KMAX];
     const uint32_t total = min(plan->cand_count, t.cand_cap);
+    // no group below topn_lb (a lower bound of the K-th best value) can be in
+    // the top K: such candidates count as empty
+    const uint64_t lb = plan->topn_lb;
     uint64_t o[TOPN_NI1], ti[TOPN_NI1];
     uint32_t sl[TOPN_NI1];
 #pragma unroll
@@ -1141,7 +1175,8 @@ __global__ __launch_bounds__(1024) void k_topn_level1(const DevTopn *__restrict_
         if (i < total && t.cand_null[i] != CAND_EMPTY) {
             const uint64_t *rec = t.cand_rec + (uint64_t)i * t.rec_words;
             group_order(t, rec[0], rec + 1, 1, t.cand_null[i] != 0, o[j], ti[j]);
-            sl[j] = i;
+            if (o[j] >= lb) sl[j] = i;
+            else o[j] = ti[j] = 0;
         }
     }
     block_topk<KMAX, TOPN_NI1>(o, ti, sl, t.k, s_win);

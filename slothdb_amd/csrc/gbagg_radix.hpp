@@ -28,7 +28,10 @@ constexpr int RP_BLOCK = 1024;   // count / scatter workgroup: 16 waves share on
 constexpr int RA_BLOCK = 1024;   // per-partition aggregation workgroup
 constexpr int RA_ROWS = 4;       // records per thread in flight
 constexpr uint32_t RA_SMAX = 8192;  // largest per-partition LDS table
-constexpr int RA_TOPN_NI = (RA_SMAX + RA_BLOCK - 1) / RA_BLOCK;
+// TopN partitions hold at most RA_TOPN_SMAX slots, so each thread ranks at most
+// RA_TOPN_NI of them in registers and two workgroups fit a CU's registers
+constexpr uint32_t RA_TOPN_SMAX = 4096;
+constexpr int RA_TOPN_NI = (RA_TOPN_SMAX + RA_BLOCK - 1) / RA_BLOCK;
 // Scatter workgroup roles: the first SC_PROD threads load and classify rows;
 // the other SC_CONS threads only write cold records to HBM, taking them from
 // an LDS ring. The vector-memory counter drains in issue order for loads and
@@ -805,7 +808,7 @@ __device__ __forceinline__ void rp_agg_body(const DevCtx *__restrict__ cx, DevTa
         }
         __syncthreads();
         const uint64_t lb = __hip_atomic_load(&T.plan->topn_lb, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        // at most RA_TOPN_NI slots per thread (S <= 8192, RA_BLOCK threads)
+        // at most RA_TOPN_NI slots per thread (S <= RA_TOPN_SMAX, RA_BLOCK threads)
         uint64_t o[RA_TOPN_NI], ti[RA_TOPN_NI];
         uint32_t sl[RA_TOPN_NI];
         uint32_t mine = 0;
@@ -868,9 +871,9 @@ __global__ __launch_bounds__(RA_BLOCK) __attribute__((amdgpu_waves_per_eu(8))) v
     rp_agg_body<SH, 1, false>(cx, T, S, limit);
 }
 
-// TopN: K candidates per partition.
+// TopN: K candidates per partition; two workgroups per CU, like k_rp_agg.
 template <class SH, int KMAX>
-__global__ __launch_bounds__(RA_BLOCK) void k_rp_agg_topn(const DevCtx *__restrict__ cx, DevTable T, uint32_t S,
+__global__ __launch_bounds__(RA_BLOCK) __attribute__((amdgpu_waves_per_eu(8))) void k_rp_agg_topn(const DevCtx *__restrict__ cx, DevTable T, uint32_t S,
                                                           uint32_t limit) {
     rp_agg_body<SH, KMAX, true>(cx, T, S, limit);
 }
@@ -939,12 +942,13 @@ __global__ __launch_bounds__(256) void k_heavy_emit(const DevCtx *__restrict__ c
     __shared__ uint32_t s_win[KMAX];
     TopList<KMAX> L;
     L.init();
+    const uint64_t lb = T.plan->topn_lb;  // groups below it cannot be in the top K
     for (uint32_t d = threadIdx.x; d < hc; d += blockDim.x) {
         const uint64_t *rec = rp.hfinal + (uint64_t)d * stride;
         if (rec[1] == 0) continue;
         uint64_t ord, tie;
         group_order(t, rec[0], rec + 1, 1, false, ord, tie);
-        L.offer(ord, tie, d, t.k);
+        if (ord >= lb) L.offer(ord, tie, d, t.k);
     }
     block_select<KMAX>(L, t.k, s_win);
     topn_push(t, T.plan, s_win, [&](uint32_t d, uint64_t &key, const uint64_t *&acc, uint32_t &st, bool &nk) {
