@@ -1588,7 +1588,7 @@ void run_sample(gbagg_handle *h) {
     // the histogram runs in few blocks (each ends with device-scope atomics on
     // the same plan words); the insert takes one atomic per wave
     const uint32_t hgrid = (uint32_t)std::min<uint64_t>((rp.scap + 255) / 256, 128);
-    const uint32_t sgrid = (uint32_t)std::min<uint64_t>((rp.scap + 255) / 256, (uint64_t)di.cus * 2);
+    const uint32_t sgrid = (uint32_t)std::min<uint64_t>((rp.scap + 255) / 256, 64);
     hipLaunchKernelGGL(k_heavy_hist, dim3(hgrid), dim3(256), 0, s, d_rp(h), h->dplan.as<DevPlan>(), h->ghist.as<uint32_t>());
     hipLaunchKernelGGL(k_heavy_insert, dim3(sgrid), dim3(256), 0, s, d_rp(h), h->dplan.as<DevPlan>(),
                        (const uint32_t *)h->ghist.as<uint32_t>());
@@ -1850,9 +1850,11 @@ bool run_radix(gbagg_handle *h, Timer &tm, uint64_t &est_out, bool counted) {
             tm.mark();  // count end
             read_plan(h);  // HLL estimate and heavy count pick the fanout
         }
-        const uint64_t est = h->hplan->est_groups;
-        const uint64_t heavy = h->hplan->heavy_count;
-        const uint64_t cold = est > heavy ? est - heavy : 1;
+        // the count pass sketches the cold keys; heavy keys are counted exactly
+        const uint64_t heavy = std::min<uint64_t>(h->hplan->heavy_count, h->hctx->rp.hmax);
+        const uint64_t cold = std::max<uint64_t>(1, h->hplan->est_groups);
+        const uint64_t est = cold + heavy;
+        h->hplan->est_groups = est;  // reported as the estimate of all groups
         const size_t k4_budget = std::min<size_t>(di.lds_per_block, (size_t)h->k4_lds_kb * 1024);
         const uint32_t S = lds_slots_exact(k4_budget, nacc, h->topn_device ? RA_TOPN_SMAX : RA_SMAX);
         if (h->force_part_bits < 0 && (double)cold > 0.75 * (double)S * (double)(1u << rp.bucket_bits)) {

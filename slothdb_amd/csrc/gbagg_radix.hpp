@@ -4,7 +4,7 @@
 //   k_heavy_hist/_threshold/_insert  keys frequent in the sample become the heavy-hitter
 //                       table (a static, read-only LDS hash in the passes below)
 //   k_rp_count          per workgroup and hash bucket: rows with a cold key;
-//                       HyperLogLog sketch of all keys on the way
+//                       HyperLogLog sketch of the cold keys on the way
 //   k_rp_scan_w/_p      exact partition offsets (partition p = top hash bits)
 //   k_rp_scatter        fused filter + key hash; heavy keys aggregate in LDS
 //                       atomics, cold rows are written to their partition at
@@ -325,25 +325,47 @@ __global__ __launch_bounds__(256) void k_heavy_insert(const RadixParams *__restr
     const uint32_t hmask = rp.hslots - 1;
     if (t > 64) return;
     const uint32_t lane = threadIdx.x & 63;
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t i0 = blockIdx.x * blockDim.x; i0 < rp.scap; i0 += stride) {  // uniform per wave
+    // The block owns a contiguous range of sample slots. Pass 1 counts its
+    // heavy keys, one device-scope atomic then reserves their dense indexes
+    // (same-address device atomics serialise: one per block, not per wave),
+    // pass 2 hands them out by wave ballot ranks.
+    const uint32_t per = (rp.scap + gridDim.x - 1) / gridDim.x;
+    const uint32_t r0 = min(rp.scap, blockIdx.x * per), r1 = min(rp.scap, r0 + per);
+    __shared__ uint32_t s_cnt, s_base, s_run;
+    __shared__ unsigned long long s_rows;
+    if (threadIdx.x == 0) {
+        s_cnt = 0;
+        s_run = 0;
+        s_rows = 0;
+    }
+    __syncthreads();
+    auto heavy_at = [&](uint32_t i, uint32_t &c) {
+        c = i < r1 ? rp.scnt[i] : 0;
+        return c != 0 && min(c, 64u) >= t;
+    };
+    for (uint32_t i0 = r0; i0 < r1; i0 += blockDim.x) {  // uniform
+        uint32_t c;
+        const uint64_t m = __ballot(heavy_at(i0 + threadIdx.x, c));
+        if (lane == 0 && m) atomicAdd(&s_cnt, (uint32_t)__popcll(m));
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+        s_base = s_cnt ? (uint32_t)atomicAdd((unsigned long long *)&plan->heavy_count, (unsigned long long)s_cnt) : 0;
+    __syncthreads();
+    unsigned long long rows = 0;
+    for (uint32_t i0 = r0; i0 < r1; i0 += blockDim.x) {  // uniform
         const uint32_t i = i0 + threadIdx.x;
-        const uint32_t c = i < rp.scap ? rp.scnt[i] : 0;
-        const bool heavy = c != 0 && min(c, 64u) >= t;
+        uint32_t c;
+        const bool heavy = heavy_at(i, c);
         const uint64_t m = __ballot(heavy);
         if (!m) continue;
-        const int first = __ffsll((unsigned long long)m) - 1;
         uint32_t base = 0;
-        if ((int)lane == first)
-            base = (uint32_t)atomicAdd((unsigned long long *)&plan->heavy_count, (unsigned long long)__popcll(m));
-        base = __shfl(base, first);
-        const uint32_t d = base + (uint32_t)__popcll(m & ((1ULL << lane) - 1));
+        if (lane == 0) base = atomicAdd(&s_run, (uint32_t)__popcll(m));
+        base = __shfl(base, 0);
+        const uint32_t d = s_base + base + (uint32_t)__popcll(m & ((1ULL << lane) - 1));
         const bool kept = heavy && d < rp.hmax;  // the table holds hmax keys
-        unsigned long long rows = kept ? c : 0;
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) rows += __shfl_xor(rows, off);
-        if ((int)lane == first && rows) atomicAdd((unsigned long long *)&plan->heavy_sample_rows, rows);
         if (!kept) continue;
+        rows += c;
         uint64_t k = rp.skeys[i];
         rp.hdense[d] = k;
         uint32_t pos = (uint32_t)hash64(k) & hmask;
@@ -355,6 +377,11 @@ __global__ __launch_bounds__(256) void k_heavy_insert(const RadixParams *__restr
             pos = (pos + 1) & hmask;
         }
     }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) rows += __shfl_xor(rows, off);
+    if (lane == 0 && rows) atomicAdd(&s_rows, rows);
+    __syncthreads();
+    if (threadIdx.x == 0 && s_rows) atomicAdd((unsigned long long *)&plan->heavy_sample_rows, s_rows);
 }
 
 // Static heavy table lookup in LDS: membership only.
@@ -448,8 +475,8 @@ __global__ __launch_bounds__(RP_BLOCK) __attribute__((amdgpu_waves_per_eu(8))) v
             uint64_t w;
             if (make_key_s(sh, q, b, row, w) != KEY_OK || w == SENTINEL) return;
             const uint64_t h = hash64(w);
-            hll_add(reg, h);
             if (heavy_has(hk, hmask, h, w)) return;
+            hll_add(reg, h);  // the sketch counts the cold keys (the heavy ones are known)
             __hip_atomic_fetch_add(&cnt[h >> shift], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         };
         for_rows_pipelined<S::kStatic ? AGG_ROWS : 1, RP_BLOCK>(sh, b, r0, r1, sh.nkp(), threadIdx.x, count_row);
