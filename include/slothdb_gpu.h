@@ -319,6 +319,9 @@ void gbagg_destroy(gbagg_handle *h);
  *   "heavy_max"     heavy-hitter keys aggregated in LDS by the radix strategy (0-4096)
  *   "heavy_min"     minimum sample count of a heavy key (1-64)
  *   "part_bits"     radix fanout override, -1 = from the HyperLogLog estimate
+ *   "msplit_fanout" largest radix fanout whose scatter takes write offsets by
+ *                   wave multisplit (ballot ranks per shared partition) rather
+ *                   than one LDS atomic per row (0-8192, default 0: off)
  *   "lds_kb"        LDS budget of a per-partition table (8-160)
  *   "fallback_cap"  initial slots of the radix strategy's global fallback table
  *   "wide_seed"     first seed of the 16-byte key mixing (reruns take the next

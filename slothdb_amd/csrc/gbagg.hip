@@ -406,6 +406,7 @@ struct gbagg_handle {
     uint32_t heavy_min = 4;
     uint64_t radix_fallback_cap = 65536;
     int32_t force_part_bits = -1;   // radix fanout override (-1 = from the estimate)
+    uint32_t msplit_fanout = MSPLIT_MAX_FANOUT;  // radix scatter offsets by wave multisplit up to this fanout
     uint32_t k4_lds_kb = 78;        // per-partition LDS table budget (two workgroups per CU)
     // learned selector
     std::vector<float> sel_params;  // empty: no selector attached
@@ -1888,6 +1889,7 @@ bool run_radix(gbagg_handle *h, Timer &tm, uint64_t &est_out, bool counted) {
         while (ring > 256 && scatter_lds_bytes(rp.hslots, nacc, rp.hmax, P, ring, rp.rec_words) > di.lds_per_block)
             ring >>= 1;
         rp.ring = ring;
+        rp.msplit_max = h->msplit_fanout;
         const size_t k3_lds = scatter_lds_bytes(rp.hslots, nacc, rp.hmax, P, ring, rp.rec_words);
         if (k3_lds > di.lds_per_block) fail(GBAGG_ERROR, "scatter kernel LDS exceeds the device limit");
         if (fresh) {
@@ -3187,6 +3189,7 @@ gbagg_status gbagg_set_param(gbagg_handle *h, const char *name, int64_t value) {
         else if (n == "heavy_min") { in(1, 64); h->heavy_min = (uint32_t)value; }
         else if (n == "fallback_cap") { in(1024, 1LL << 30); h->radix_fallback_cap = next_pow2((uint64_t)value); }
         else if (n == "part_bits") { in(-1, 13); h->force_part_bits = (int32_t)value; }
+        else if (n == "msplit_fanout") { in(0, 8192); h->msplit_fanout = (uint32_t)value; }
         else if (n == "lds_kb") { in(8, 160); h->k4_lds_kb = (uint32_t)value; }
         else if (n == "features") { in(0, 1); h->collect_features = value != 0; }
         else if (n == "wide_seed") { in(0, 1LL << 40); h->wide_seed0 = (uint64_t)value; }

@@ -260,6 +260,22 @@ def test_radix_params(params):
     print(info)
 
 
+@pytest.mark.parametrize("part_bits,msplit", [(4, 64), (6, 64), (6, 0), (9, 8192), (13, 8192), (0, 1)])
+def test_radix_scatter_offsets_by_multisplit(part_bits, msplit):
+    # forced fanouts with the scatter's wave multisplit on: 1, 16 and 64
+    # partitions (lanes of a wave share partitions: the multisplit's groups),
+    # 512 and 8192; and 64 with it off (the default); Zipf keys make some
+    # partitions far larger than others
+    q, cols = datagen.highcard_workload(n=600_000, nkeys=2_000)
+    q.strategy = G.RADIX
+    params = {"part_bits": part_bits, "msplit_fanout": msplit, "selector": 0}
+    print(f"[parity] radix multisplit {params}", flush=True)
+    res, info = G.run(q, cols, device=True, params=params)
+    assert info["radix_bits"] == part_bits
+    exp, types = _exact(q, cols)
+    assert_same_groups(res.rows(), exp, len(q.group_cols), types)
+
+
 def test_radix_nullable_keys_and_inputs():
     rng = np.random.default_rng(21)
     n = 400_000
