@@ -322,6 +322,8 @@ void gbagg_destroy(gbagg_handle *h);
  *   "msplit_fanout" largest radix fanout whose scatter takes write offsets by
  *                   wave multisplit (ballot ranks per shared partition) rather
  *                   than one LDS atomic per row (0-8192, default 0: off)
+ *   "radix_ahead_rows" inputs of at least this many rows run the radix count
+ *                   pass before the plan is read back (default 4M rows)
  *   "lds_kb"        LDS budget of a per-partition table (8-160)
  *   "fallback_cap"  initial slots of the radix strategy's global fallback table
  *   "wide_seed"     first seed of the 16-byte key mixing (reruns take the next

@@ -176,6 +176,10 @@ DeviceInfo &device_info() {
 }
 
 // Device buffer that only grows.
+// Inputs from this many rows run the radix count pass ahead of the plan
+// (default of the "radix_ahead_rows" knob).
+constexpr uint64_t RADIX_SPEC_ROWS = 4u << 20;
+
 // Bytes of the parameter block ahead of the plan block in one allocation.
 constexpr size_t CTX_BYTES = (sizeof(DevCtx) + 255) & ~(size_t)255;
 
@@ -407,6 +411,7 @@ struct gbagg_handle {
     uint64_t radix_fallback_cap = 65536;
     int32_t force_part_bits = -1;   // radix fanout override (-1 = from the estimate)
     uint32_t msplit_fanout = MSPLIT_MAX_FANOUT;  // radix scatter offsets by wave multisplit up to this fanout
+    uint64_t radix_ahead_rows = RADIX_SPEC_ROWS;  // inputs from this many rows count ahead of the plan
     uint32_t k4_lds_kb = 78;        // per-partition LDS table budget (two workgroups per CU)
     // learned selector
     std::vector<float> sel_params;  // empty: no selector attached
@@ -1757,9 +1762,6 @@ void run_lds_hash(gbagg_handle *h, Timer &tm, bool global_only, uint64_t est = 0
 // Returns false, with nothing aggregated, when the HyperLogLog count of cold
 // groups exceeds what the largest fanout's LDS tables hold (est_out: the
 // count); the caller then runs the global table.
-// Inputs from this many rows run the radix count pass ahead of the plan.
-constexpr uint64_t RADIX_SPEC_ROWS = 4u << 20;
-
 // Launch shape of the radix count and scatter passes, and their buffers
 // (parameter block fields; uploaded with it).
 void radix_setup(gbagg_handle *h) {
@@ -2722,7 +2724,7 @@ void finalize_once(gbagg_handle *h) {
         // (DevPlan::radix_gate); it is wasted only when the selector then picks
         // another path.
         const bool spec = (st == GBAGG_STRATEGY_AUTO || st == GBAGG_STRATEGY_RADIX) &&
-                          h->rows_total >= RADIX_SPEC_ROWS && h->dq.key_mode != KM_NONE;
+                          h->rows_total >= h->radix_ahead_rows && h->dq.key_mode != KM_NONE;
         if (spec) {
             radix_setup(h);
             upload_ctx(h);
@@ -3190,6 +3192,7 @@ gbagg_status gbagg_set_param(gbagg_handle *h, const char *name, int64_t value) {
         else if (n == "fallback_cap") { in(1024, 1LL << 30); h->radix_fallback_cap = next_pow2((uint64_t)value); }
         else if (n == "part_bits") { in(-1, 13); h->force_part_bits = (int32_t)value; }
         else if (n == "msplit_fanout") { in(0, 8192); h->msplit_fanout = (uint32_t)value; }
+        else if (n == "radix_ahead_rows") { in(0, 1LL << 40); h->radix_ahead_rows = (uint64_t)value; }
         else if (n == "lds_kb") { in(8, 160); h->k4_lds_kb = (uint32_t)value; }
         else if (n == "features") { in(0, 1); h->collect_features = value != 0; }
         else if (n == "wide_seed") { in(0, 1LL << 40); h->wide_seed0 = (uint64_t)value; }

@@ -317,6 +317,51 @@ def test_radix_composite_topn_and_full():
     assert_same_groups(res2.rows(), exp, 2, types)
 
 
+# The radix count pass launched before the plan is read back (default: inputs
+# of 4M rows or more), forced here at smaller sizes with radix_ahead_rows = 0.
+@pytest.mark.parametrize("case", ["highcard_auto", "highcard_radix", "highcard_selector", "lowcard_auto",
+                                  "composite_topn", "nullable_radix"])
+def test_radix_count_ahead_of_plan(case):
+    params = {"radix_ahead_rows": 0, "selector": 0}
+    print(f"[parity] count ahead: {case}", flush=True)
+    if case.startswith("highcard"):
+        q, cols = datagen.highcard_workload(n=1_500_000, nkeys=600_000)
+        if case == "highcard_radix":
+            q.strategy = G.RADIX
+        if case == "highcard_selector":
+            params.pop("selector")  # the built-in selector may then pick another path
+        res, info, _ = _check(q, cols, params=params)
+        if case != "highcard_selector":
+            assert info["strategy"] == G.RADIX
+            # the count's marks stay with the radix plan that used them
+            assert info["ms_count"] > 0
+    elif case == "lowcard_auto":
+        # the device gate says low cardinality: the count's workgroups return
+        # at once and the low-cardinality plan runs
+        q, cols = datagen.plain_workload(n=1_000_000)
+        res, info, _ = _check(q, cols, params=params)
+        assert info["strategy"] in (G.DIRECT, G.LDS_HASH)
+    elif case == "composite_topn":
+        q, cols = datagen.composite_workload(n=2_000_000, dict_size=100_000, topn=10)
+        dic = cols[1].dictionary
+        res, info = G.run(q, cols, device=True, params=params)
+        exp, types = _exact(datagen.composite_query(topn=None), cols, {1: dic})
+        assert info["strategy"] == G.RADIX and res.num_rows == 10
+        assert_topn_tie_aware(res.rows(), exp, order_col=2, k=10, types=types)
+    else:
+        rng = np.random.default_rng(23)
+        n = 800_000
+        k = rng.integers(0, 400_000, n).astype(np.int64)
+        kv = (rng.random(n) > 0.02).astype(np.uint8)
+        v = rng.integers(-10**9, 10**9, n).astype(np.int64)
+        vv = (rng.random(n) > 0.2).astype(np.uint8)
+        cols = [G.Column(G.BIGINT, k, kv), G.Column(G.BIGINT, v, vv)]
+        q = G.Query([G.BIGINT, G.BIGINT], [0], [(G.COUNT_STAR, None), (G.SUM, 1), (G.MIN, 1), (G.AVG, 1)])
+        q.strategy = G.RADIX
+        res, info, _ = _check(q, cols, params=params)
+        assert any(r[0] is None for r in res.rows())  # the NULL-key group
+
+
 @pytest.mark.parametrize("k", [10, 32])
 def test_radix_topn_all_partitions_produce(k):
     # 8192 partitions each push K candidates: the candidate buffer must hold
