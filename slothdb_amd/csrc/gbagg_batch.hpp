@@ -179,7 +179,10 @@ __global__ __launch_bounds__(256) void k_b_hll(const BatchQuery *__restrict__ bq
                 const uint64_t h = hash64(kk[j]);
                 const uint64_t rest = h << BQ_HLL_P;
                 const uint32_t rank = rest ? (uint32_t)__clzll((long long)rest) + 1 : (64 - BQ_HLL_P + 1);
-                __hip_atomic_fetch_max(&r[h >> (64 - BQ_HLL_P)], rank, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                // registers only grow: a read first, the atomic only when it grows
+                uint32_t *rp = &r[h >> (64 - BQ_HLL_P)];
+                if (__hip_atomic_load(rp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) < rank)
+                    __hip_atomic_fetch_max(rp, rank, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                 __hip_atomic_fetch_add(&bh[h >> (64 - BQ_RBITS)], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             }
         }

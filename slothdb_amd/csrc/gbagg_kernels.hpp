@@ -80,6 +80,49 @@ __device__ __forceinline__ void hll_add(uint32_t *reg, uint64_t h) {
         __hip_atomic_fetch_max(&reg[idx], rank, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
+// Wave multisplit over LDS counters: every active lane adds 1 to cnt[b] (its
+// partition) and gets its exclusive offset. Lanes sharing
+// the bucket of the first remaining lane take one atomic for the whole group
+// and their ranks in it from the group's ballot; this repeats for up to
+// MSPLIT_GROUPS groups while the groups hold two lanes or more, and the
+// remaining lanes (all distinct buckets, in a high fanout) take one atomic
+// each. Every active lane of the wave must call it (ballots and shuffles).
+constexpr int MSPLIT_GROUPS = 4;
+// Default of the radix scatter's RadixParams::msplit_max: 0, the multisplit
+// off. Measured on that scatter against one LDS atomic per row, it was slower
+// at every fanout tried (1 to 256 partitions: +2-13%; 8192: +1%), the scatter
+// being bound by its memory traffic rather than by same-address LDS atomics.
+constexpr uint32_t MSPLIT_MAX_FANOUT = 0;
+template <bool RET>
+__device__ __forceinline__ uint32_t wave_multisplit(uint32_t *cnt, uint32_t b) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint64_t below = (1ULL << lane) - 1;
+    uint64_t todo = __ballot(1);
+    uint32_t pos = 0;
+    bool done = false;
+    for (int g = 0; g < MSPLIT_GROUPS; g++) {
+        const int leader = __ffsll((unsigned long long)todo) - 1;
+        const uint32_t lb = __shfl(b, leader);
+        const uint64_t same = __ballot(!done && b == lb);
+        uint32_t base = 0;
+        if ((int)lane == leader)
+            base = __hip_atomic_fetch_add(&cnt[lb], (uint32_t)__popcll(same), __ATOMIC_RELAXED,
+                                          __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (RET) base = __shfl(base, leader);
+        if (!done && b == lb) {
+            pos = base + (uint32_t)__popcll(same & below);
+            done = true;
+        }
+        todo &= ~same;
+        if (!todo || __popcll(same) < 2) break;  // uniform: ballot results
+    }
+    if (!done) {
+        if (RET) pos = __hip_atomic_fetch_add(&cnt[b], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        else __hip_atomic_fetch_add(&cnt[b], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+    return pos;
+}
+
 // Register-wise max over `nparts` sketches: thread per register, blockIdx.y
 // picks one of gridDim.y slices of the sketches; slice y's merge goes to
 // merged[y * HLL_M ...] (k_hll_finish folds the slices).

@@ -384,49 +384,6 @@ __global__ __launch_bounds__(256) void k_heavy_insert(const RadixParams *__restr
     if (threadIdx.x == 0 && s_rows) atomicAdd((unsigned long long *)&plan->heavy_sample_rows, s_rows);
 }
 
-// Wave multisplit over LDS counters: every active lane adds 1 to cnt[b] (its
-// partition) and gets its exclusive offset. Lanes sharing
-// the bucket of the first remaining lane take one atomic for the whole group
-// and their ranks in it from the group's ballot; this repeats for up to
-// MSPLIT_GROUPS groups while the groups hold two lanes or more, and the
-// remaining lanes (all distinct buckets, in a high fanout) take one atomic
-// each. Every active lane of the wave must call it (ballots and shuffles).
-constexpr int MSPLIT_GROUPS = 4;
-// Default RadixParams::msplit_max: 0, the multisplit off. Measured on the
-// scatter against one LDS atomic per row, it was slower at every fanout tried
-// (1 to 256 partitions: +2-13%; 8192: +1%), the scatter being bound by its
-// memory traffic rather than by same-address LDS atomics.
-constexpr uint32_t MSPLIT_MAX_FANOUT = 0;
-template <bool RET>
-__device__ __forceinline__ uint32_t wave_multisplit(uint32_t *cnt, uint32_t b) {
-    const uint32_t lane = threadIdx.x & 63;
-    const uint64_t below = (1ULL << lane) - 1;
-    uint64_t todo = __ballot(1);
-    uint32_t pos = 0;
-    bool done = false;
-    for (int g = 0; g < MSPLIT_GROUPS; g++) {
-        const int leader = __ffsll((unsigned long long)todo) - 1;
-        const uint32_t lb = __shfl(b, leader);
-        const uint64_t same = __ballot(!done && b == lb);
-        uint32_t base = 0;
-        if ((int)lane == leader)
-            base = __hip_atomic_fetch_add(&cnt[lb], (uint32_t)__popcll(same), __ATOMIC_RELAXED,
-                                          __HIP_MEMORY_SCOPE_WORKGROUP);
-        if (RET) base = __shfl(base, leader);
-        if (!done && b == lb) {
-            pos = base + (uint32_t)__popcll(same & below);
-            done = true;
-        }
-        todo &= ~same;
-        if (!todo || __popcll(same) < 2) break;  // uniform: ballot results
-    }
-    if (!done) {
-        if (RET) pos = __hip_atomic_fetch_add(&cnt[b], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        else __hip_atomic_fetch_add(&cnt[b], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    }
-    return pos;
-}
-
 // Static heavy table lookup in LDS: membership only.
 // Runs f(row) over rows [r0, r1) of a batch, R rows per thread per tile of
 // BLOCK * R rows. Tile k+1's loads are issued before tile k's work, into the
