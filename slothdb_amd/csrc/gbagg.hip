@@ -49,7 +49,7 @@ using KAgg = void (*)(const DevQuery *, BatchRows, DevTable, uint32_t, uint32_t)
 using KCount = void (*)(const DevCtx *, BatchRows, const uint32_t *);
 using KScatter = void (*)(const DevCtx *, BatchRows, DevTable);
 using KPart = void (*)(const DevCtx *, DevTable, uint32_t, uint32_t);
-using KRange = void (*)(const DevQuery *, BatchRows, DevPlan *);
+using KRange = void (*)(const DevQuery *, BatchRows, const DevPlan *, int64_t *, uint64_t);
 using KDirect = void (*)(const DevQuery *, BatchRows, DevTable, int64_t, uint32_t, uint32_t);
 
 struct ShapeKernels {
@@ -396,6 +396,7 @@ struct gbagg_handle {
     DevCtx *hctx = nullptr;    // pinned parameter blocks: hctx, then hplan at CTX_BYTES
     bool ctx_stale = false;    // hctx changed since its last upload
     RadixLaunch rl;            // radix passes of the current finalize
+    bool range_ahead = false;  // the exact key range of this finalize came back with the sample's plan
     DevBatch *hbatches = nullptr;
     size_t hbatch_cap = 0;
     DevBuf ctx, dbatches;
@@ -1597,7 +1598,7 @@ void run_sample(gbagg_handle *h) {
     const uint32_t sgrid = (uint32_t)std::min<uint64_t>((rp.scap + 255) / 256, 64);
     hipLaunchKernelGGL(k_heavy_hist, dim3(hgrid), dim3(256), 0, s, d_rp(h), h->dplan.as<DevPlan>(), h->ghist.as<uint32_t>());
     hipLaunchKernelGGL(k_heavy_insert, dim3(sgrid), dim3(256), 0, s, d_rp(h), h->dplan.as<DevPlan>(),
-                       (const uint32_t *)h->ghist.as<uint32_t>());
+                       (const uint32_t *)h->ghist.as<uint32_t>(), (uint32_t)(h->strategy == GBAGG_STRATEGY_RADIX));
 }
 
 // Selector features of the sample just taken, and with a selector its
@@ -1960,19 +1961,28 @@ uint64_t direct_lds_span(const gbagg_handle *h) { return direct_lds_budget() / (
 
 // Exact key range of the filtered rows (one pass over the key and filter
 // columns): false for multi-column keys or a range wider than DIRECT_MAX_SPAN.
+// The exact key range pass over every batch into out[0..1] (see k_key_range).
+void launch_key_range(gbagg_handle *h, int64_t *out, uint64_t gate_span) {
+    if (!h->rows_total) return;
+    const DeviceInfo &di = device_info();
+    uint64_t per;
+    const uint32_t g = tiled_grid(h->rows_total, (uint64_t)AGG_BLOCK * AGG_ROWS, (uint64_t)di.cus * 2, per);
+    hipLaunchKernelGGL(h->kern->range, dim3(g), dim3(AGG_BLOCK), 0, h->stream, d_q(h), batch_rows(h, per),
+                       (const DevPlan *)h->dplan.as<DevPlan>(), out, gate_span);
+}
+
 bool direct_range(gbagg_handle *h, uint64_t &span, int64_t &kmin) {
     if (h->dq.key_mode != KM_SINGLE) return false;
-    const DeviceInfo &di = device_info();
-    hipStream_t s = h->stream;
-    hipLaunchKernelGGL(k_key_range_init, dim3(1), dim3(1), 0, s, h->dplan.as<DevPlan>());
-    if (h->rows_total) {
-        uint64_t per;
-        const uint32_t g = tiled_grid(h->rows_total, (uint64_t)AGG_BLOCK * AGG_ROWS, (uint64_t)di.cus * 2, per);
-        hipLaunchKernelGGL(h->kern->range, dim3(g), dim3(AGG_BLOCK), 0, s, d_q(h), batch_rows(h, per),
-                           h->dplan.as<DevPlan>());
+    if (h->range_ahead) {
+        // the pass ran with the sample (range_ahead_ready): its result came back with the plan
+        h->hplan->key_min = h->hplan->rng_min;
+        h->hplan->key_max = h->hplan->rng_max;
+    } else {
+        hipLaunchKernelGGL(k_key_range_init, dim3(1), dim3(1), 0, h->stream, h->dplan.as<DevPlan>());
+        launch_key_range(h, &h->dplan.as<DevPlan>()->key_min, 0);
+        HIPCHK(hipGetLastError());
+        read_plan(h);
     }
-    HIPCHK(hipGetLastError());
-    read_plan(h);
     const DevPlan &p = *h->hplan;
     if (p.key_min > p.key_max) {  // no key: every row NULL-keyed or filtered out
         kmin = 0;
@@ -2683,6 +2693,7 @@ void finalize_once(gbagg_handle *h) {
     }
     std::memset(h->hplan, 0, sizeof(DevPlan));
     std::memset(h->hctx, 0, sizeof(DevCtx));
+    h->range_ahead = false;
     h->dq.fx_flags = &h->dplan.as<DevPlan>()->overflow;
     h->hctx->q = h->dq;
     fill_out_topn(h);
@@ -2725,6 +2736,12 @@ void finalize_once(gbagg_handle *h) {
         // another path.
         const bool spec = (st == GBAGG_STRATEGY_AUTO || st == GBAGG_STRATEGY_RADIX) &&
                           h->rows_total >= h->radix_ahead_rows && h->dq.key_mode != KM_NONE;
+        // Likewise the exact key range of a one-column key, which a
+        // low-cardinality plan needs to go direct-mapped: gated on the device
+        // by the sample (few distinct keys, sampled range within a
+        // workgroup's dense table), so it runs when that plan is likely.
+        const bool spec_range = st == GBAGG_STRATEGY_AUTO && h->dq.key_mode == KM_SINGLE && h->rows_total > 0;
+        if (spec_range) launch_key_range(h, &h->dplan.as<DevPlan>()->rng_min, direct_lds_span(h));
         if (spec) {
             radix_setup(h);
             upload_ctx(h);
@@ -2734,6 +2751,11 @@ void finalize_once(gbagg_handle *h) {
         }
         read_plan(h);
         const bool counted = spec && (st == GBAGG_STRATEGY_RADIX || h->hplan->radix_gate);
+        {
+            const DevPlan &p = *h->hplan;  // the same test as the range pass's gate
+            h->range_ahead = spec_range && !p.radix_gate && p.key_min <= p.key_max &&
+                             (uint64_t)p.key_max - (uint64_t)p.key_min < direct_lds_span(h);
+        }
         if (sel || h->collect_features) take_features(h);
         fx_prepare(h, true);
         if (st == GBAGG_STRATEGY_AUTO) {

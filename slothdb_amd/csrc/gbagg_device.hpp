@@ -127,6 +127,8 @@ struct DevPlan {
     uint64_t topn_lb;          // TopN: no group with a smaller order value can be in the top K
     int64_t key_min;           // direct-mapped strategy: exact key word range
     int64_t key_max;
+    int64_t rng_min;           // the same, from a pass launched before the plan is read back
+    int64_t rng_max;
     uint32_t fx_exp[MAXU];     // per loaded slot: largest biased exponent of its finite double
                                // values (sampled rows, or all rows for the exact bound)
     // sample frequency statistics for the learned selector's features

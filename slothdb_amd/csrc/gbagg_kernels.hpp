@@ -418,10 +418,16 @@ __global__ __launch_bounds__(AGG_BLOCK) void k_aggregate(const DevQuery *__restr
 constexpr uint32_t OVF_DIRECT = 16u;  // DevPlan::overflow bit: a key outside the measured range
 
 // Exact range of the key words of the rows that pass the filter (NULL, dropped
-// and SENTINEL keys excluded): one atomic min / max per wave.
+// and SENTINEL keys excluded) into out[0] (min) and out[1] (max): one atomic
+// min / max per block. gate_span > 0: a pass launched before the plan is read
+// back, which runs only for a sample of few distinct keys (radix_gate clear)
+// whose sampled key range (plan->key_min / key_max) spans less than gate_span.
 template <class S>
 __global__ __launch_bounds__(AGG_BLOCK) void k_key_range(const DevQuery *__restrict__ qp, BatchRows br,
-                                                         DevPlan *plan) {
+                                                         const DevPlan *plan, int64_t *out, uint64_t gate_span) {
+    if (gate_span && (plan->radix_gate || plan->key_min > plan->key_max ||
+                      (uint64_t)plan->key_max - (uint64_t)plan->key_min >= gate_span))
+        return;
     const DevQuery &q = *qp;
     const S sh(q);
     const uint64_t tile = (uint64_t)AGG_BLOCK * AGG_ROWS;
@@ -466,8 +472,8 @@ __global__ __launch_bounds__(AGG_BLOCK) void k_key_range(const DevQuery *__restr
             hi = max(hi, s_hi[w]);
         }
         if (lo <= hi) {
-            __hip_atomic_fetch_min(&plan->key_min, lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_fetch_max(&plan->key_max, hi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_min(&out[0], lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_max(&out[1], hi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
     }
 }

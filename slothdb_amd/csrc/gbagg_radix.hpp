@@ -98,6 +98,8 @@ __global__ void k_sample_init(const RadixParams *__restrict__ rpp, DevPlan *plan
         if (threadIdx.x == 0) {
             plan->key_min = INT64_MAX;
             plan->key_max = INT64_MIN;
+            plan->rng_min = INT64_MAX;
+            plan->rng_max = INT64_MIN;
             plan->heavy_count = 0;
         }
     }
@@ -303,16 +305,18 @@ __device__ __forceinline__ uint32_t heavy_threshold(const RadixParams &rp, const
 
 // Dense indexes come from one device-scope atomic per wave and round (ballot
 // rank within the wave): same-address device atomics serialise.
-// heavy_count starts at zero (k_sample_init).
+// heavy_count starts at zero (k_sample_init). The heavy table serves only the
+// radix strategy: for a sample of few distinct keys, which the default plan
+// does not aggregate by radix, it is left empty unless radix is forced (a
+// radix plan then runs without heavy keys, exactly, if the selector picks it).
 __global__ __launch_bounds__(256) void k_heavy_insert(const RadixParams *__restrict__ rpp, DevPlan *plan,
-                                                      const uint32_t *ghist) {
+                                                      const uint32_t *ghist, uint32_t force) {
     const RadixParams &rp = *rpp;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        // the default plan's rule for few distinct keys (also on the host):
-        // a count pass launched before the plan is read runs only without it
-        const uint64_t d = plan->sample_distinct, f1 = plan->sample_f1;
-        plan->radix_gate = !(d <= 4096 && f1 * 8 <= d + 8);
-    }
+    // the default plan's rule for few distinct keys (also on the host): a count
+    // pass launched before the plan is read runs only without it
+    const uint64_t sd = plan->sample_distinct, sf1 = plan->sample_f1;
+    const bool gate = !(sd <= 4096 && sf1 * 8 <= sd + 8);
+    if (blockIdx.x == 0 && threadIdx.x == 0) plan->radix_gate = gate;
     __shared__ uint32_t s_t, s_hist[65];
     if (threadIdx.x < 65) s_hist[threadIdx.x] = ghist[threadIdx.x];
     __syncthreads();
@@ -323,7 +327,7 @@ __global__ __launch_bounds__(256) void k_heavy_insert(const RadixParams *__restr
     __syncthreads();
     const uint32_t t = s_t;
     const uint32_t hmask = rp.hslots - 1;
-    if (t > 64) return;
+    if (t > 64 || !(gate || force)) return;
     const uint32_t lane = threadIdx.x & 63;
     // The block owns a contiguous range of sample slots. Pass 1 counts its
     // heavy keys, one device-scope atomic then reserves their dense indexes
