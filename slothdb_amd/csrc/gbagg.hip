@@ -1490,6 +1490,8 @@ void emit_table(gbagg_handle *h, const DevTable &T, uint64_t cap) {
         uint32_t grid = (uint32_t)std::min<uint64_t>((cap + 2 + 1023) / 1024, (uint64_t)di.cus * 4);
         if (h->topn_k <= 16) hipLaunchKernelGGL(k_topn_table<16>, dim3(grid), dim3(256), 0, s, T, d_t(h));
         else hipLaunchKernelGGL(k_topn_table<32>, dim3(grid), dim3(256), 0, s, T, d_t(h));
+    } else if (cap <= COMPACT_SMALL) {
+        hipLaunchKernelGGL(k_compact_small, dim3(1), dim3(1024), 0, s, T, d_o(h));
     } else {
         uint64_t nblk = (cap + 2 + COMPACT_CHUNK - 1) / COMPACT_CHUNK;
         uint32_t grid = (uint32_t)std::min<uint64_t>(nblk, (uint64_t)di.cus * 4);

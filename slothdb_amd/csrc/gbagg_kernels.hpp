@@ -1172,6 +1172,33 @@ __global__ __launch_bounds__(256) void k_compact_write(DevTable T, const uint64_
     }
 }
 
+// Tables of at most COMPACT_SMALL slots (the direct-mapped and small global
+// tables) in one block of 1024 threads and one launch: thread t owns slots
+// t + 1024 i and keeps their occupancy in a mask, thread 0 also the two side
+// records; one reservation on plan->groups, then each thread writes its groups.
+constexpr uint64_t COMPACT_SMALL = 4096;
+__global__ __launch_bounds__(1024) void k_compact_small(DevTable T, const DevOut *__restrict__ op) {
+    const DevOut &o = *op;
+    const uint64_t cap = T.plan->cap;  // <= COMPACT_SMALL
+    uint32_t m = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < COMPACT_SMALL / 1024; i++) {
+        const uint64_t s = (uint64_t)i * 1024 + threadIdx.x;
+        if (s < cap && T.rec[s * T.stride] != SENTINEL) m |= 1u << i;
+    }
+    if (threadIdx.x == 0)
+        for (uint32_t j = 0; j < 2; j++)
+            if (slot_occupied(T.rec + (cap + j) * T.stride, cap + j, cap)) m |= 1u << (16 + j);
+    uint64_t row = block_reserve((uint32_t)__popc(m), &T.plan->groups);
+    while (m) {
+        const uint32_t i = (uint32_t)__ffs(m) - 1;
+        m &= m - 1;
+        const uint64_t s = i >= 16 ? cap + (i - 16) : (uint64_t)i * 1024 + threadIdx.x;
+        const uint64_t *rec = T.rec + s * T.stride;
+        write_group(o, rec[0], rec + 1, 1, s == cap + 1, row++);
+    }
+}
+
 template <int KMAX>
 __global__ __launch_bounds__(256) void k_topn_table(DevTable T, const DevTopn *__restrict__ tp) {
     const DevTopn &t = *tp;
