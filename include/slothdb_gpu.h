@@ -324,7 +324,7 @@ void gbagg_destroy(gbagg_handle *h);
  *                   than one LDS atomic per row (0-8192, default 0: off)
  *   "radix_ahead_rows" inputs of at least this many rows run the radix count
  *                   pass before the plan is read back (default 4M rows)
- *   "lds_kb"        LDS budget of a per-partition table (8-160)
+ *   "lds_kb"        LDS budget of a per-partition table (8-160; default 78, 156 for TopN)
  *   "fallback_cap"  initial slots of the radix strategy's global fallback table
  *   "wide_seed"     first seed of the 16-byte key mixing (reruns take the next
  *                   seeds while two groups share a key word; results never change)

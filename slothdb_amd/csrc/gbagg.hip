@@ -413,7 +413,8 @@ struct gbagg_handle {
     int32_t force_part_bits = -1;   // radix fanout override (-1 = from the estimate)
     uint32_t msplit_fanout = MSPLIT_MAX_FANOUT;  // radix scatter offsets by wave multisplit up to this fanout
     uint64_t radix_ahead_rows = RADIX_SPEC_ROWS;  // inputs from this many rows count ahead of the plan
-    uint32_t k4_lds_kb = 78;        // per-partition LDS table budget (two workgroups per CU)
+    uint32_t k4_lds_kb = 0;         // per-partition LDS table budget in KiB; 0: 78 (two workgroups per
+                                    // CU), 156 for TopN plans (one workgroup per CU, larger tables)
     // learned selector
     std::vector<float> sel_params;  // empty: no selector attached
     bool sel_builtin = true;        // AUTO plans without an attached selector use the built-in one
@@ -1861,7 +1862,17 @@ bool run_radix(gbagg_handle *h, Timer &tm, uint64_t &est_out, bool counted) {
         const uint64_t cold = std::max<uint64_t>(1, h->hplan->est_groups);
         const uint64_t est = cold + heavy;
         h->hplan->est_groups = est;  // reported as the estimate of all groups
-        const size_t k4_budget = std::min<size_t>(di.lds_per_block, (size_t)h->k4_lds_kb * 1024);
+        // TopN partitions: one workgroup per CU with a larger table measured
+        // faster than two with smaller ones (composite 50M-row TopN: partition
+        // pass 0.34 against 0.37 ms)
+        const uint32_t lds_kb = h->k4_lds_kb ? h->k4_lds_kb : h->topn_device ? 156u : 78u;
+        // the partition kernel's own static LDS comes out of the block's limit
+        const void *kpart = !h->topn_device ? (const void *)h->kern->part_full
+                            : h->topn_k <= 16 ? (const void *)h->kern->part16 : (const void *)h->kern->part32;
+        hipFuncAttributes fa{};
+        HIPCHK(hipFuncGetAttributes(&fa, kpart));
+        const size_t k4_budget = std::min<size_t>(di.lds_per_block - std::min<size_t>(di.lds_per_block, fa.sharedSizeBytes),
+                                                  (size_t)lds_kb * 1024);
         const uint32_t S = lds_slots_exact(k4_budget, nacc, h->topn_device ? RA_TOPN_SMAX : RA_SMAX);
         if (h->force_part_bits < 0 && (double)cold > 0.75 * (double)S * (double)(1u << rp.bucket_bits)) {
             est_out = est;
