@@ -9,8 +9,9 @@
 //   k_hll (per batch) -> k_hll_finish       table sized on the device
 //   k_table_init
 //   k_aggregate (per batch)                 filter + hash + LDS table + global merge
-//   k_occ_count -> k_scan_blocks -> k_compact_write    (full result), or
-//   k_topn_local -> k_topn_final -> k_gather_write     (TopN hint, K <= 32)
+//   k_occ_count -> k_scan_blocks -> k_compact_write    (full result; k_compact_small
+//                                                       for tables of <= 4096 slots), or
+//   k_topn_table -> k_topn_level1 -> k_topn_final      (TopN hint, K <= 32)
 //
 // One device->host copy of the small plan block ends the run; a probe
 // overflow (estimate far too low) reruns with a 4x table.
