@@ -135,6 +135,38 @@ __device__ __forceinline__ uint32_t str_word(const uint8_t *p, uint32_t r) {
     return r >= 4 ? v : (v & ((1u << (8 * r)) - 1));
 }
 
+// Strings of at most STR_SHORT bytes are read in one go, as 4 little-endian
+// 64-bit words, zero above len (len >= 1). The aligned 8-byte words holding
+// the bytes load together: indices past the last of them are clamped to it,
+// so every load is unconditional and none waits on another. An aligned
+// 8-byte word that holds a byte of the string lies in that byte's memory
+// page; bytes outside the string are masked off.
+constexpr uint32_t STR_SHORT = 32;
+
+__device__ __forceinline__ void str_load5(const uint8_t *p, uint32_t len, uint64_t (&d)[5]) {
+    const uint64_t *a = (const uint64_t *)((uintptr_t)p & ~(uintptr_t)7);
+    const uint32_t last = ((uint32_t)((uintptr_t)p & 7) + len - 1) / 8;  // 0..4
+#pragma unroll
+    for (uint32_t i = 0; i < 5; i++) d[i] = gld(a + (i < last ? i : last));
+}
+
+__device__ __forceinline__ void str_extract4(const uint64_t (&d)[5], const uint8_t *p, uint32_t len,
+                                             uint64_t (&w)[4]) {
+    const uint32_t sh = 8 * (uint32_t)((uintptr_t)p & 7);
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const uint64_t v = sh ? (d[j] >> sh) | (d[j + 1] << (64 - sh)) : d[j];
+        const int rem = (int)len - 8 * j;
+        w[j] = rem >= 8 ? v : rem <= 0 ? 0ull : v & ((1ull << (8 * rem)) - 1ull);
+    }
+}
+
+__device__ __forceinline__ void str_words4(const uint8_t *p, uint32_t len, uint64_t (&w)[4]) {
+    uint64_t d[5];
+    str_load5(p, len, d);
+    str_extract4(d, p, len, w);
+}
+
 __device__ __forceinline__ uint64_t str_fold(uint64_t a, uint64_t b) {
     return a * b ^ __umul64hi(a, b);
 }
@@ -144,11 +176,19 @@ __device__ __forceinline__ uint64_t str_fold(uint64_t a, uint64_t b) {
 // half), and SplitMix64's finalizer mixes the result.
 __device__ __forceinline__ uint64_t str_hash(const uint8_t *p, uint32_t len, uint64_t seed) {
     uint64_t h = seed ^ ((uint64_t)len * 0x9E3779B97F4A7C15ull);
-    for (uint32_t i = 0; i < len; i += 8) {
-        const uint32_t left = len - i;
-        uint64_t w = str_word(p + i, left < 4 ? left : 4);
-        if (left > 4) w |= (uint64_t)str_word(p + i + 4, left - 4 < 4 ? left - 4 : 4) << 32;
-        h = str_fold(h ^ w, 0xBF58476D1CE4E5B9ull) + 0x94D049BB133111EBull;
+    if (len && len <= STR_SHORT) {
+        uint64_t w[4];
+        str_words4(p, len, w);
+#pragma unroll
+        for (uint32_t i = 0; i < 4; i++)
+            if (8 * i < len) h = str_fold(h ^ w[i], 0xBF58476D1CE4E5B9ull) + 0x94D049BB133111EBull;
+    } else {
+        for (uint32_t i = 0; i < len; i += 8) {
+            const uint32_t left = len - i;
+            uint64_t w = str_word(p + i, left < 4 ? left : 4);
+            if (left > 4) w |= (uint64_t)str_word(p + i + 4, left - 4 < 4 ? left - 4 : 4) << 32;
+            h = str_fold(h ^ w, 0xBF58476D1CE4E5B9ull) + 0x94D049BB133111EBull;
+        }
     }
     h ^= h >> 30;
     h *= 0xBF58476D1CE4E5B9ull;
@@ -167,6 +207,16 @@ __device__ __forceinline__ bool str_equal(const StrTab &t, uint32_t c, const uin
     const StrCode cd{(uint32_t)v, (uint32_t)(v >> 32)};
     if (cd.len != len) return false;
     const uint32_t *a = t.arena + cd.word;
+    if (len == 0) return true;
+    if (len <= STR_SHORT) {
+        // both sides' words load together
+        uint64_t dp[5], da[5], w[4], x[4];
+        str_load5(p, len, dp);
+        str_load5((const uint8_t *)a, len, da);
+        str_extract4(dp, p, len, w);
+        str_extract4(da, (const uint8_t *)a, len, x);
+        return ((w[0] ^ x[0]) | (w[1] ^ x[1]) | (w[2] ^ x[2]) | (w[3] ^ x[3])) == 0;
+    }
     for (uint32_t i = 0; i < len; i += 4) {
         const uint32_t left = len - i;
         if (str_word(p + i, left < 4 ? left : 4) != gld(&a[i / 4])) return false;
@@ -414,9 +464,17 @@ __global__ __launch_bounds__(256) void k_str_assign(const StrSeg *segs, uint32_t
             if (c >= t.code_cap || w0 + words > t.arena_words) {
                 __hip_atomic_fetch_or(&t.ctr[SC_FLAGS], STR_FULL_DICT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             } else {
-                for (uint32_t i = 0; i < row.len; i += 4) {
-                    const uint32_t left = row.len - i;
-                    gst(&t.arena[w0 + i / 4], str_word(row.p + i, left < 4 ? left : 4));
+                if (row.len && row.len <= STR_SHORT) {
+                    uint64_t w[4];
+                    str_words4(row.p, row.len, w);
+#pragma unroll
+                    for (uint32_t i = 0; i < 8; i++)
+                        if (4 * i < row.len) gst(&t.arena[w0 + i], (uint32_t)(w[i / 2] >> (32 * (i & 1))));
+                } else {
+                    for (uint32_t i = 0; i < row.len; i += 4) {
+                        const uint32_t left = row.len - i;
+                        gst(&t.arena[w0 + i / 4], str_word(row.p + i, left < 4 ? left : 4));
+                    }
                 }
                 t.code[c] = StrCode{(uint32_t)w0, row.len};
                 t.slot[s].val = (uint32_t)c;
