@@ -331,8 +331,12 @@ void gbagg_destroy(gbagg_handle *h);
  *   "static_shapes" 1 = use compile-time specialised row kernels when the query
  *                   shape matches one (default), 0 = always the generic kernels
  *   "str_hash_bits" bits of the string hash kept when interning GBAGG_STR_RAW
- *                   columns (1-64, default 64); fewer bits only force strings to
- *                   share hashes, which the byte comparison resolves
+ *                   columns (1-64, default 64; a row's key holds at most 48 of
+ *                   them beside the string's length); fewer bits only force
+ *                   strings to share keys, which the byte comparison resolves
+ *   "str_codes"     first code capacity and slot count of that interning
+ *                   (0 = from the distinct-count estimate, default); smaller
+ *                   values only make the tables grow, results never change
  *   "selector"      1 = AUTO plans consult the built-in trained selector
  *                   (default), 0 = the default plan only */
 gbagg_status gbagg_set_param(gbagg_handle *h, const char *name, int64_t value);

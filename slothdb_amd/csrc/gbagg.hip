@@ -358,6 +358,7 @@ struct gbagg_handle {
     std::vector<uint8_t> str_ordered;  // per column: MIN/MAX over it, so its codes follow byte order
     std::vector<std::unique_ptr<StrCol>> strs;
     uint64_t str_key_mask = ~0ull;  // stored string-hash bits ("str_hash_bits")
+    uint64_t str_codes0 = 0;        // first code capacity and slot count, 0 = from the estimate ("str_codes")
     // multi-column keys (more than two columns, a DOUBLE column, or a nullable
     // pair without drop_null_keys): the plan groups by one hidden column, the
     // code of each row's interned key tuple; results unpack the tuples
@@ -2120,6 +2121,11 @@ void intern_run(gbagg_handle *h, StrCol &sc, const std::vector<StrSeg> &segs, ui
     const uint64_t words_max = bytes / 4 + rows + 1;
     const uint64_t mean_words = rows ? bytes / rows / 4 + 2 : 1;
     uint64_t arena_words = std::min<uint64_t>(words_max, code_cap * mean_words * 3 / 2 + 1024);
+    if (h->str_codes0) {  // small first tables: the resizes below run
+        code_cap = std::min<uint64_t>(std::max<uint64_t>(1, rows), h->str_codes0);
+        slots = next_pow2(h->str_codes0);
+        arena_words = std::min<uint64_t>(words_max, code_cap * mean_words);
+    }
     for (uint32_t attempt = 0; rows; attempt++) {
         if (slots > (1ull << 30)) fail(GBAGG_ERROR, "string table larger than 2^30 slots");
         if (arena_words > 0xFFFFFFFFull) fail(GBAGG_ERROR, "distinct strings exceed 16 GiB");
@@ -2135,9 +2141,12 @@ void intern_run(gbagg_handle *h, StrCol &sc, const std::vector<StrSeg> &segs, ui
         const uint32_t ig = (uint32_t)std::min<uint64_t>((slots + 255) / 256, (uint64_t)di.cus * 8);
         hipLaunchKernelGGL(k_str_init, dim3(ig), dim3(256), 0, s, t);
         sc.blk.ensure(4096 * 16);
-        const uint32_t rg_all = (uint32_t)std::min<uint64_t>((total + 255) / 256, (uint64_t)di.cus * 8);
-        hipLaunchKernelGGL(k_str_rehome, dim3(rg_all), dim3(256), 0, s, total, (const uint64_t *)sc.hash.ptr,
-                           sc.pos.as<uint32_t>());
+        // k_str_hash left every valid row at its home slot; a restart puts them back
+        if (attempt) {
+            const uint32_t rg_all = (uint32_t)std::min<uint64_t>((total + 255) / 256, (uint64_t)di.cus * 8);
+            hipLaunchKernelGGL(k_str_rehome, dim3(rg_all), dim3(256), 0, s, total, (const uint64_t *)sc.hash.ptr,
+                               sc.pos.as<uint32_t>());
+        }
         // a prefix of the rows first: its strings (the frequent ones above
         // all) then have codes when the other rows probe, which settle in
         // k_str_probe without the representative and check kernels
@@ -2145,8 +2154,16 @@ void intern_run(gbagg_handle *h, StrCol &sc, const std::vector<StrSeg> &segs, ui
         std::vector<uint64_t> lims;
         if (prefix < total) lims.push_back(prefix);
         lims.push_back(total);
+        // the probe's workgroups all resident at once (no partial last wave of
+        // workgroups): as many per CU as its registers allow
+        static const int probe_per_cu = [] {
+            int n = 0;
+            HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_str_probe, 256, 0));
+            return std::max(1, std::min(8, n));
+        }();
         for (uint64_t lim : lims) {
-            const uint32_t rg = (uint32_t)std::min<uint64_t>((lim + 255) / 256, (uint64_t)di.cus * 8);
+            const uint32_t rg = (uint32_t)std::min<uint64_t>((lim + 255) / 256, (uint64_t)di.cus * probe_per_cu);
+            const uint32_t rgc = (uint32_t)std::min<uint64_t>((lim + 255) / 256, (uint64_t)di.cus * 8);
             // code assignment: blocks of contiguous rows, at most 4096 of them
             const uint64_t chunk = std::max<uint64_t>(256, ((lim + 4095) / 4096 + 255) / 256 * 256);
             const uint32_t nblk = (uint32_t)((lim + chunk - 1) / chunk);
@@ -2154,11 +2171,12 @@ void intern_run(gbagg_handle *h, StrCol &sc, const std::vector<StrSeg> &segs, ui
                 hipLaunchKernelGGL(k_str_probe, dim3(rg), dim3(256), 0, s, dsegs, nseg, lim, t,
                                    (const uint64_t *)sc.hash.ptr, sc.pos.as<uint32_t>(), sc.codes.as<uint32_t>());
                 hipLaunchKernelGGL(k_str_reps, dim3(nblk), dim3(256), 0, s, dsegs, nseg, lim, t,
-                                   (const uint32_t *)sc.pos.ptr, chunk, sc.blk.as<uint64_t>());
+                                   sc.pos.as<uint32_t>(), chunk, sc.blk.as<uint64_t>());
                 hipLaunchKernelGGL(k_str_scan, dim3(1), dim3(1024), 0, s, sc.blk.as<uint64_t>(), nblk, t.ctr);
                 hipLaunchKernelGGL(k_str_assign, dim3(nblk), dim3(256), 0, s, dsegs, nseg, lim, t,
-                                   (const uint32_t *)sc.pos.ptr, chunk, (const uint64_t *)sc.blk.ptr);
-                hipLaunchKernelGGL(k_str_check, dim3(rg), dim3(256), 0, s, dsegs, nseg, lim, t,
+                                   sc.pos.as<uint32_t>(), sc.codes.as<uint32_t>(), chunk,
+                                   (const uint64_t *)sc.blk.ptr);
+                hipLaunchKernelGGL(k_str_check, dim3(rgc), dim3(256), 0, s, dsegs, nseg, lim, t,
                                    sc.pos.as<uint32_t>(), sc.codes.as<uint32_t>());
                 HIPCHK(hipMemcpyAsync(ctr, t.ctr, SC_N * 8, hipMemcpyDeviceToHost, s));
                 HIPCHK(hipStreamSynchronize(s));
@@ -3233,6 +3251,7 @@ gbagg_status gbagg_set_param(gbagg_handle *h, const char *name, int64_t value) {
         else if (n == "features") { in(0, 1); h->collect_features = value != 0; }
         else if (n == "wide_seed") { in(0, 1LL << 40); h->wide_seed0 = (uint64_t)value; }
         else if (n == "str_hash_bits") { in(1, 64); h->str_key_mask = value == 64 ? ~0ull : (1ull << value) - 1; }
+        else if (n == "str_codes") { in(0, 1LL << 30); h->str_codes0 = (uint64_t)value; }
         else if (n == "strategy") { in(0, 4); h->strategy = (gbagg_strategy)value; }
         else if (n == "static_shapes") { in(0, 1); h->allow_static = value != 0; pick_kernels(h); }
         else if (n == "selector") {  // 0: no selector; 1: the built-in trained one (default)

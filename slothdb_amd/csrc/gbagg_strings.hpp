@@ -20,7 +20,9 @@
 //                 that reached it (atomicMin)
 //   k_str_reps / k_str_scan / k_str_assign
 //                 that row takes the next code (in row order), copies its
-//                 string into the arena and publishes the code on the slot
+//                 string into the arena and publishes the code, with the
+//                 string's first arena word, on the slot (a probe then loads
+//                 the arena words and the code's length at the same time)
 //   k_str_check   the other rows that stopped there compare their bytes with
 //                 the code's string; rows with a different string probe on
 //                 from the next slot in the next round
@@ -37,7 +39,14 @@ namespace gbagg {
 constexpr uint32_t STR_NONE = 0xFFFFFFFFu;  // slot: no code yet; row: resolved or NULL
 constexpr uint32_t STR_HOME = 0xFFFFFFFEu;  // row: not probed yet, start at the home slot
 constexpr uint32_t STR_CAND = 0x80000000u;  // row's slot, flagged: the row lowered the slot's rep
+constexpr uint32_t STR_REP = 0x40000000u;   // row's slot, flagged: the row is the slot's representative
 constexpr uint64_t STR_FULL_TABLE = 1, STR_FULL_DICT = 2, STR_BAD_OFFSETS = 4;
+// A row's key: 48 bits of its string's hash, and its length in the top 16
+// bits (STR_LEN_LONG for 65535 bytes and more). Equal keys then mean equal
+// lengths below STR_LEN_LONG, and a comparison needs the code's own length
+// only for long strings.
+constexpr uint64_t STR_KEY_HASH = (1ull << 48) - 1;
+constexpr uint32_t STR_LEN_LONG = 0xFFFFu;
 enum { SC_CODES = 0, SC_WORDS, SC_UNRESOLVED, SC_FLAGS, SC_ROWS, SC_BYTES, SC_N };
 
 // Rows [row0, row0 + n) of the column's concatenation: one batch, given as
@@ -92,7 +101,7 @@ __device__ __forceinline__ StrRow str_at(const StrSeg &s, uint64_t r) {
 struct StrSlot {
     uint64_t key;  // stored hash, 0 = empty
     uint32_t val;  // code, STR_NONE = none yet
-    uint32_t rep;  // smallest row that stopped here while the slot had no code
+    uint32_t rep;  // no code yet: smallest row that stopped here; with a code: its first arena word
 };
 
 // One code: its string's first arena word and its length.
@@ -109,7 +118,7 @@ struct StrTab {
     uint64_t mask;       // slots - 1
     uint64_t arena_words;
     uint64_t code_cap;
-    uint64_t key_mask;   // stored hash bits (all, unless a test narrows them)
+    uint64_t key_mask;   // hash bits kept in a row's key (all 48, unless a test narrows them)
     uint64_t seed;
 };
 
@@ -201,25 +210,37 @@ __device__ __forceinline__ uint64_t str_home(uint64_t key, uint64_t mask) {
     return ((key * 0x9E3779B97F4A7C15ull) >> 32) & mask;
 }
 
-// The row's string equals code c's arena string.
-__device__ __forceinline__ bool str_equal(const StrTab &t, uint32_t c, const uint8_t *p, uint32_t len) {
-    const uint64_t v = gld((const uint64_t *)&t.code[c]);  // word, then length
-    const StrCode cd{(uint32_t)v, (uint32_t)(v >> 32)};
-    if (cd.len != len) return false;
-    const uint32_t *a = t.arena + cd.word;
-    if (len == 0) return true;
-    if (len <= STR_SHORT) {
-        // both sides' words load together
-        uint64_t dp[5], da[5], w[4], x[4];
-        str_load5(p, len, dp);
-        str_load5((const uint8_t *)a, len, da);
-        str_extract4(dp, p, len, w);
-        str_extract4(da, (const uint8_t *)a, len, x);
-        return ((w[0] ^ x[0]) | (w[1] ^ x[1]) | (w[2] ^ x[2]) | (w[3] ^ x[3])) == 0;
+// A row's string, as str_equal compares it: a short string's words are
+// loaded once, before its probe sequence.
+struct StrKey {
+    const uint8_t *p;
+    uint32_t len;
+    uint64_t w[4];  // len 1..STR_SHORT: the string's words
+};
+
+__device__ __forceinline__ StrKey str_key(const uint8_t *p, uint32_t len) {
+    StrKey k;
+    k.p = p;
+    k.len = len;
+    if (len && len <= STR_SHORT) str_words4(p, len, k.w);
+    return k;
+}
+
+// The row's string equals code c's string, whose arena copy starts at word w0
+// (the slot holds it next to the code). The slot's key equals the row's, so
+// the lengths are equal unless the row's is STR_LEN_LONG or more.
+__device__ __forceinline__ bool str_equal(const StrTab &t, uint32_t c, uint32_t w0, const StrKey &k) {
+    const uint32_t len = k.len;
+    const uint32_t *a = t.arena + w0;
+    if (len && len <= STR_SHORT) {
+        uint64_t x[4];
+        str_words4((const uint8_t *)a, len, x);
+        return ((k.w[0] ^ x[0]) | (k.w[1] ^ x[1]) | (k.w[2] ^ x[2]) | (k.w[3] ^ x[3])) == 0;
     }
+    if (len >= STR_LEN_LONG && gld(&t.code[c].len) != len) return false;
     for (uint32_t i = 0; i < len; i += 4) {
         const uint32_t left = len - i;
-        if (str_word(p + i, left < 4 ? left : 4) != gld(&a[i / 4])) return false;
+        if (str_word(k.p + i, left < 4 ? left : 4) != gld(&a[i / 4])) return false;
     }
     return true;
 }
@@ -282,7 +303,7 @@ __global__ __launch_bounds__(256) void k_str_hash(const StrSeg *segs, uint32_t n
         const StrRow sr = str_at(s, r);
         const uint32_t len = sr.len;
         const uint64_t full = str_hash(sr.p, len, t.seed);
-        const uint64_t k = full & t.key_mask;
+        const uint64_t k = (full & t.key_mask & STR_KEY_HASH) | (uint64_t)min(len, STR_LEN_LONG) << 48;
         hash[g] = k ? k : 1;
         pos[g] = STR_HOME;
         hll_add(reg, full);
@@ -308,6 +329,7 @@ __global__ __launch_bounds__(256) void k_str_probe(const StrSeg *segs, uint32_t 
         if (p0 == STR_NONE) continue;
         const uint64_t h = hash[g];
         const StrRow row = str_row(segs, nseg, g);
+        const StrKey key = str_key(row.p, row.len);
         uint64_t s = p0 == STR_HOME ? str_home(h, t.mask) : p0;
         uint32_t out = STR_NONE;
         for (uint64_t probes = 0;; probes++) {
@@ -317,6 +339,8 @@ __global__ __launch_bounds__(256) void k_str_probe(const StrSeg *segs, uint32_t 
             }
             StrSlot *sl = &t.slot[s];
             uint64_t k = __hip_atomic_load(&sl->key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            // code and arena word: set by an earlier kernel, or no code yet
+            const uint64_t vr = gld((const uint64_t *)&sl->val);
             if (k == 0) {
                 uint64_t expect = 0;
                 __hip_atomic_compare_exchange_strong(&sl->key, &expect, h, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
@@ -324,7 +348,7 @@ __global__ __launch_bounds__(256) void k_str_probe(const StrSeg *segs, uint32_t 
                 k = expect == 0 ? h : expect;
             }
             if (k == h) {
-                const uint32_t c = sl->val;
+                const uint32_t c = (uint32_t)vr;
                 if (c == STR_NONE) {
                     // no code yet: the smallest row here becomes the representative;
                     // only rows below the current one try (a hot string's rows
@@ -337,7 +361,7 @@ __global__ __launch_bounds__(256) void k_str_probe(const StrSeg *segs, uint32_t 
                     out = (uint32_t)s | (cand ? STR_CAND : 0u);
                     break;
                 }
-                if (str_equal(t, c, row.p, row.len)) {
+                if (str_equal(t, c, (uint32_t)(vr >> 32), key)) {
                     codes[g] = c;
                     break;
                 }
@@ -355,6 +379,12 @@ __device__ __forceinline__ bool str_is_rep(const StrTab &t, const uint32_t *pos,
     if (p == STR_NONE || !(p & STR_CAND)) return false;
     s = p & ~STR_CAND;
     return t.slot[s].rep == (uint32_t)g;
+}
+
+// k_str_reps flagged the row as its slot's representative.
+__device__ __forceinline__ bool str_flagged_rep(uint32_t p, uint32_t &s) {
+    s = p & ~STR_REP;
+    return (p & (STR_CAND | STR_REP)) == STR_REP;
 }
 
 // Exclusive block-wide prefix sums of two values per thread (256 threads),
@@ -389,12 +419,15 @@ __device__ __forceinline__ void str_block_scan(uint64_t a, uint64_t b, uint64_t 
 //   k_str_assign  each representative takes its code, copies its string into
 //                 the arena and publishes the code on its slot
 __global__ __launch_bounds__(256) void k_str_reps(const StrSeg *segs, uint32_t nseg, uint64_t total, StrTab t,
-                                                  const uint32_t *pos, uint64_t chunk, uint64_t *blk) {
+                                                  uint32_t *pos, uint64_t chunk, uint64_t *blk) {
     const uint64_t r0 = (uint64_t)blockIdx.x * chunk, r1 = min(total, r0 + chunk);
     uint64_t cnt = 0, words = 0;
     for (uint64_t g = r0 + threadIdx.x; g < r1; g += blockDim.x) {
         uint32_t s;
         if (!str_is_rep(t, pos, g, s)) continue;
+        // k_str_assign then knows its representatives without the slots'
+        // rep words, which it overwrites
+        pos[g] = s | STR_REP;
         cnt++;
         words += (str_row(segs, nseg, g).len + 3) / 4;
     }
@@ -447,13 +480,14 @@ __global__ __launch_bounds__(1024) void k_str_scan(uint64_t *blk, uint32_t nblk,
 }
 
 __global__ __launch_bounds__(256) void k_str_assign(const StrSeg *segs, uint32_t nseg, uint64_t total, StrTab t,
-                                                    const uint32_t *pos, uint64_t chunk, const uint64_t *blk) {
+                                                    uint32_t *pos, uint32_t *codes, uint64_t chunk,
+                                                    const uint64_t *blk) {
     const uint64_t r0 = (uint64_t)blockIdx.x * chunk, r1 = min(total, r0 + chunk);
     uint64_t bc = blk[2 * blockIdx.x], bw = blk[2 * blockIdx.x + 1];
     for (uint64_t tile = r0; tile < r1; tile += blockDim.x) {
         const uint64_t g = tile + threadIdx.x;
         uint32_t s = 0;
-        const bool rep = g < r1 && str_is_rep(t, pos, g, s);
+        const bool rep = g < r1 && str_flagged_rep(pos[g], s);
         StrRow row{};
         if (rep) row = str_row(segs, nseg, g);
         const uint64_t words = rep ? (row.len + 3) / 4 : 0;
@@ -477,7 +511,10 @@ __global__ __launch_bounds__(256) void k_str_assign(const StrSeg *segs, uint32_t
                     }
                 }
                 t.code[c] = StrCode{(uint32_t)w0, row.len};
-                t.slot[s].val = (uint32_t)c;
+                // the code and its arena word; the representative is resolved
+                gst((uint64_t *)&t.slot[s].val, (uint64_t)c | (uint64_t)w0 << 32);
+                codes[g] = (uint32_t)c;
+                pos[g] = STR_NONE;
             }
         }
         bc += tc;
@@ -492,19 +529,16 @@ __global__ __launch_bounds__(256) void k_str_check(const StrSeg *segs, uint32_t 
          g += (uint64_t)gridDim.x * blockDim.x) {
         const uint32_t p = pos[g];
         if (p == STR_NONE) continue;
-        const uint32_t s = p & ~STR_CAND;
-        const uint64_t vr = gld((const uint64_t *)&t.slot[s].val);  // code and representative
+        // representatives with a code were resolved by k_str_assign
+        const uint32_t s = p & ~(STR_CAND | STR_REP);
+        const uint64_t vr = gld((const uint64_t *)&t.slot[s].val);  // code and arena word
         const uint32_t c = (uint32_t)vr;
         if (c == STR_NONE) {  // the dictionary was full: the host restarts larger
             pos[g] = STR_NONE;
             continue;
         }
-        bool eq = (p & STR_CAND) && (uint32_t)(vr >> 32) == (uint32_t)g;
-        if (!eq) {
-            const StrRow row = str_row(segs, nseg, g);
-            eq = str_equal(t, c, row.p, row.len);
-        }
-        if (eq) {
+        const StrRow row = str_row(segs, nseg, g);
+        if (str_equal(t, c, (uint32_t)(vr >> 32), str_key(row.p, row.len))) {
             codes[g] = c;
             pos[g] = STR_NONE;
         } else {

@@ -175,6 +175,44 @@ def test_forced_hash_collisions():
         assert len(got) == len(set(idx.tolist()))
 
 
+@pytest.mark.parametrize("codes", [1, 16])
+def test_tables_grow_from_small(codes):
+    # first tables of 1 or 16 codes and slots: the dictionary and the slot table
+    # fill, interning restarts with larger ones (rows back at their home slots)
+    # until every string has its code; NULL strings form their own group
+    words = _words(3_000, 21)
+    rng = np.random.default_rng(22)
+    n = 150_000
+    idx = datagen.zipf_ranks(n, len(words), 0.8, seed=23).astype(np.int64) % len(words)
+    cols = [_strings(words, idx, null_p=0.05, seed=24), G.Column(G.BIGINT, rng.integers(-10**6, 10**6, n)),
+            G.Column(G.INTEGER, rng.integers(0, 7, n).astype(np.int32))]
+    q = G.Query([G.VARCHAR, G.BIGINT, G.INTEGER], [2, 0], [(G.COUNT_STAR, None), (G.SUM, 1)],
+                nullable=[True, False, False])
+    print(f"[strings] first tables of {codes} codes", flush=True)
+    got, _ = _run(q, _split(cols, 2), True, params={"str_codes": codes})
+    exp, t = _expected(q, cols)
+    assert_same_groups(got, exp, 2, t)
+
+
+def test_long_strings():
+    # strings of 65,535 bytes and more: the row key's length field saturates,
+    # so with one hash bit kept, strings that differ only in length share keys
+    # and the code's own length must separate them
+    base = "x" * 65_534
+    words = [base, base + "x", base + "xx", base + "xy", "x" * 70_000, "x" * 70_001, "", "x", "y" * 40]
+    rng = np.random.default_rng(31)
+    n = 400
+    idx = rng.integers(0, len(words), n)
+    cols = [_strings(words, idx), G.Column(G.BIGINT, rng.integers(-100, 100, n))]
+    q = G.Query([G.VARCHAR, G.BIGINT], [0], [(G.COUNT_STAR, None), (G.SUM, 1)], nullable=[False, False])
+    exp, t = _expected(q, cols)
+    for bits in (64, 1):
+        print(f"[strings] long strings, hash bits {bits}", flush=True)
+        got, _ = _run(q, _split(cols, 2), True, params={"str_hash_bits": bits})
+        assert_same_groups(got, exp, 1, t)
+        assert len(got) == len(set(idx.tolist()))
+
+
 def test_string_predicates():
     # WHERE s = / <> / LIKE '%x%' / NOT LIKE over a string column, GROUP BY k
     n, m = 200_000, 2_000
