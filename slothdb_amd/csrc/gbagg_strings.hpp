@@ -245,11 +245,30 @@ __device__ __forceinline__ bool str_equal(const StrTab &t, uint32_t c, uint32_t 
     return true;
 }
 
+// Calls f(segment, row within it) for row g. A wave's rows are contiguous, so
+// they are nearly always in one segment: then the segment is found and read
+// once for the wave, with scalar loads, and its fields are uniform; otherwise
+// each lane reads its own.
+template <class F>
+__device__ __forceinline__ void with_str_seg(const StrSeg *__restrict__ segs, uint32_t nseg, uint64_t g, F &&f) {
+    const uint64_t g0 = (uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(g >> 32)) << 32 |
+                        (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)g);
+    const uint32_t si = (uint32_t)__builtin_amdgcn_readfirstlane(str_seg(segs, nseg, g0));
+    const StrSeg sw = segs[si];
+    if (__all(g >= sw.row0 && g - sw.row0 < sw.n)) {
+        f(sw, g - sw.row0);
+    } else {
+        const StrSeg sl = segs[str_seg(segs, nseg, g)];
+        f(sl, g - sl.row0);
+    }
+}
+
 __device__ __forceinline__ StrRow str_row(const StrSeg *segs, uint32_t nseg, uint64_t g) {
-    const StrSeg s = segs[str_seg(segs, nseg, g)];
-    const uint64_t r = g - s.row0;
-    StrRow out = str_at(s, r);
-    out.valid = !s.valid || gld(&s.valid[r]) != 0;
+    StrRow out;
+    with_str_seg(segs, nseg, g, [&](const StrSeg &s, uint64_t r) {
+        out = str_at(s, r);
+        out.valid = !s.valid || gld(&s.valid[r]) != 0;
+    });
     return out;
 }
 
@@ -279,37 +298,54 @@ __global__ __launch_bounds__(256) void k_str_hash(const StrSeg *segs, uint32_t n
     uint64_t rows = 0, bytes = 0;
     bool bad = false;
     for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total;
-         g += (uint64_t)gridDim.x * blockDim.x) {
-        const StrSeg s = segs[str_seg(segs, nseg, g)];
-        const uint64_t r = g - s.row0;
-        const bool valid = !s.valid || gld(&s.valid[r]) != 0;
-        // a dictionary code outside its dictionary drops the row, as it does
-        // for dictionary-coded keys
-        const bool drop = valid && s.dcodes && gld(&s.dcodes[r]) >= s.dict_n;
-        bool ok = true;
-        if (!s.dcodes && !s.fixed) {
+         g += (uint64_t)gridDim.x * blockDim.x) with_str_seg(segs, nseg, g, [&](const StrSeg &s, uint64_t r) {
+        // the row's validity byte, code and offsets load before any of them
+        // is used (a row without a validity array reads a stand-in byte)
+        const uint8_t vb = gld(s.valid ? s.valid + r : (const uint8_t *)segs);
+        const uint8_t *p;
+        uint32_t len;
+        bool drop = false, ok = true;
+        if (s.fixed) {
+            p = s.bytes + r * s.fixed;
+            len = s.fixed;
+        } else if (s.dcodes) {
+            // a dictionary code outside its dictionary drops the row, as it
+            // does for dictionary-coded keys
+            const uint32_t c = gld(&s.dcodes[r]);
+            drop = c >= s.dict_n;
+            p = s.bytes;
+            len = 0;
+            if (!drop) {
+                const uint32_t e = s.remap ? gld(&s.remap[c]) : c;
+                const uint32_t o0 = gld(&s.off[e]), o1 = gld(&s.off[e + 1]);
+                p = s.bytes + o0;
+                len = o1 >= o0 ? o1 - o0 : 0;
+            }
+        } else {
             // every row's offsets ascend and stay within the batch's bytes
             // (offsets[n]); a row outside them is never read
-            const uint32_t o0 = gld(&s.off[r]), o1 = gld(&s.off[r + 1]);
-            ok = o0 <= o1 && o1 <= gld(&s.off[s.n]);
+            const uint32_t o0 = gld(&s.off[r]), o1 = gld(&s.off[r + 1]), end = gld(&s.off[s.n]);
+            ok = o0 <= o1 && o1 <= end;
             if (!ok) bad = true;
+            p = s.bytes + o0;
+            len = o1 - o0;
         }
+        const bool valid = !s.valid || vb != 0;
+        drop = drop && valid;
         if (!valid || drop || !ok) {
             hash[g] = 0;
             pos[g] = STR_NONE;
             codes[g] = drop ? STR_DROP : 0;
-            continue;
+            return;
         }
-        const StrRow sr = str_at(s, r);
-        const uint32_t len = sr.len;
-        const uint64_t full = str_hash(sr.p, len, t.seed);
+        const uint64_t full = str_hash(p, len, t.seed);
         const uint64_t k = (full & t.key_mask & STR_KEY_HASH) | (uint64_t)min(len, STR_LEN_LONG) << 48;
         hash[g] = k ? k : 1;
         pos[g] = STR_HOME;
         hll_add(reg, full);
         rows++;
         bytes += len;
-    }
+    });
     if (bad) __hip_atomic_fetch_or(&t.ctr[SC_FLAGS], STR_BAD_OFFSETS, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     atomicAdd((unsigned long long *)&bsum[0], (unsigned long long)rows);
     atomicAdd((unsigned long long *)&bsum[1], (unsigned long long)bytes);
