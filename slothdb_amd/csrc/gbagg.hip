@@ -276,7 +276,7 @@ struct KeyDict {
 // Interning state of one GBAGG_STR_RAW column (device buffers reused across
 // finalizes of the handle).
 struct StrCol {
-    DevBuf segs, hash, pos, codes, slot, code, arena, ctr, hllp, hllm, plan, blk;
+    DevBuf segs, hash, pos, codes, slot, code, arena, ctr, hllp, hllm, plan, blk, list, lcnt;
     DevBuf coff, cbytes;                    // group column: its dictionary batches' strings (canonical ids)
     std::vector<uint32_t> coff_h;
     std::vector<char> cbytes_h;
@@ -2083,6 +2083,8 @@ void intern_run(gbagg_handle *h, StrCol &sc, const std::vector<StrSeg> &segs, ui
         sc.tab = t;
         return;
     }
+    // rows are 32-bit ids in the slots and lists, below the flag values
+    if (total >= STR_HOME) fail(GBAGG_ERROR, "a GBAGG_STR_RAW column holds at most 2^32 - 3 rows");
     sc.segs.ensure(segs.size() * sizeof(StrSeg));
     HIPCHK(hipMemcpyAsync(sc.segs.ptr, segs.data(), segs.size() * sizeof(StrSeg), hipMemcpyHostToDevice, s));
     sc.hash.ensure(total * 8);
@@ -2126,6 +2128,18 @@ void intern_run(gbagg_handle *h, StrCol &sc, const std::vector<StrSeg> &segs, ui
         slots = next_pow2(h->str_codes0);
         arena_words = std::min<uint64_t>(words_max, code_cap * mean_words);
     }
+    // the probe's workgroups all resident at once (no partial last wave of
+    // workgroups): as many per CU as its registers allow
+    static const int probe_per_cu = [] {
+        int n = 0;
+        HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_str_probe, 256, 0));
+        return std::max(1, std::min(8, n));
+    }();
+    // the probe's row lists for k_str_check: per workgroup, room for the
+    // rows it visits (256 per grid stride)
+    const uint64_t rg_max = std::min<uint64_t>((total + 255) / 256, (uint64_t)di.cus * probe_per_cu);
+    sc.list.ensure((total + rg_max * 256) * 4);
+    sc.lcnt.ensure(rg_max * 4);
     for (uint32_t attempt = 0; rows; attempt++) {
         if (slots > (1ull << 30)) fail(GBAGG_ERROR, "string table larger than 2^30 slots");
         if (arena_words > 0xFFFFFFFFull) fail(GBAGG_ERROR, "distinct strings exceed 16 GiB");
@@ -2154,30 +2168,25 @@ void intern_run(gbagg_handle *h, StrCol &sc, const std::vector<StrSeg> &segs, ui
         std::vector<uint64_t> lims;
         if (prefix < total) lims.push_back(prefix);
         lims.push_back(total);
-        // the probe's workgroups all resident at once (no partial last wave of
-        // workgroups): as many per CU as its registers allow
-        static const int probe_per_cu = [] {
-            int n = 0;
-            HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_str_probe, 256, 0));
-            return std::max(1, std::min(8, n));
-        }();
         for (uint64_t lim : lims) {
             const uint32_t rg = (uint32_t)std::min<uint64_t>((lim + 255) / 256, (uint64_t)di.cus * probe_per_cu);
-            const uint32_t rgc = (uint32_t)std::min<uint64_t>((lim + 255) / 256, (uint64_t)di.cus * 8);
+            const uint64_t per_block = 256 * ((lim + (uint64_t)rg * 256 - 1) / ((uint64_t)rg * 256));
             // code assignment: blocks of contiguous rows, at most 4096 of them
             const uint64_t chunk = std::max<uint64_t>(256, ((lim + 4095) / 4096 + 255) / 256 * 256);
             const uint32_t nblk = (uint32_t)((lim + chunk - 1) / chunk);
             for (uint32_t round = 0;; round++) {
                 hipLaunchKernelGGL(k_str_probe, dim3(rg), dim3(256), 0, s, dsegs, nseg, lim, t,
-                                   (const uint64_t *)sc.hash.ptr, sc.pos.as<uint32_t>(), sc.codes.as<uint32_t>());
+                                   (const uint64_t *)sc.hash.ptr, sc.pos.as<uint32_t>(), sc.codes.as<uint32_t>(),
+                                   sc.list.as<uint32_t>(), sc.lcnt.as<uint32_t>(), per_block);
                 hipLaunchKernelGGL(k_str_reps, dim3(nblk), dim3(256), 0, s, dsegs, nseg, lim, t,
                                    sc.pos.as<uint32_t>(), chunk, sc.blk.as<uint64_t>());
                 hipLaunchKernelGGL(k_str_scan, dim3(1), dim3(1024), 0, s, sc.blk.as<uint64_t>(), nblk, t.ctr);
                 hipLaunchKernelGGL(k_str_assign, dim3(nblk), dim3(256), 0, s, dsegs, nseg, lim, t,
                                    sc.pos.as<uint32_t>(), sc.codes.as<uint32_t>(), chunk,
                                    (const uint64_t *)sc.blk.ptr);
-                hipLaunchKernelGGL(k_str_check, dim3(rgc), dim3(256), 0, s, dsegs, nseg, lim, t,
-                                   sc.pos.as<uint32_t>(), sc.codes.as<uint32_t>());
+                hipLaunchKernelGGL(k_str_check, dim3(rg), dim3(256), 0, s, dsegs, nseg, t, sc.pos.as<uint32_t>(),
+                                   sc.codes.as<uint32_t>(), (const uint32_t *)sc.list.ptr,
+                                   (const uint32_t *)sc.lcnt.ptr, per_block, rg);
                 HIPCHK(hipMemcpyAsync(ctr, t.ctr, SC_N * 8, hipMemcpyDeviceToHost, s));
                 HIPCHK(hipStreamSynchronize(s));
                 if ((ctr[SC_FLAGS] & (STR_FULL_TABLE | STR_FULL_DICT)) || !ctr[SC_UNRESOLVED]) break;

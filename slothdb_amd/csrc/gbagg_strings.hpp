@@ -357,8 +357,16 @@ __global__ __launch_bounds__(256) void k_str_hash(const StrSeg *segs, uint32_t n
     }
 }
 
+// Rows that stop at a slot without a code are listed for k_str_check, which
+// then reads only them: workgroup b lists its rows in list[b * per_block ...)
+// (per_block: at least the rows it visits) and their number in lcnt[b].
 __global__ __launch_bounds__(256) void k_str_probe(const StrSeg *segs, uint32_t nseg, uint64_t total, StrTab t,
-                                                   const uint64_t *hash, uint32_t *pos, uint32_t *codes) {
+                                                   const uint64_t *hash, uint32_t *pos, uint32_t *codes,
+                                                   uint32_t *list, uint32_t *lcnt, uint64_t per_block) {
+    __shared__ uint32_t s_n;
+    if (threadIdx.x == 0) s_n = 0;
+    __syncthreads();
+    uint32_t *mine = list + (uint64_t)blockIdx.x * per_block;
     for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total;
          g += (uint64_t)gridDim.x * blockDim.x) {
         const uint32_t p0 = pos[g];
@@ -405,7 +413,20 @@ __global__ __launch_bounds__(256) void k_str_probe(const StrSeg *segs, uint32_t 
             s = (s + 1) & t.mask;
         }
         pos[g] = out;
+        // one reservation per wave in the workgroup's list
+        const uint64_t m = __ballot(out != STR_NONE);
+        if (m) {
+            const uint32_t lane = __lane_id(), leader = (uint32_t)__ffsll((unsigned long long)m) - 1;
+            uint32_t base = 0;
+            if (lane == leader)
+                base = __hip_atomic_fetch_add(&s_n, (uint32_t)__popcll(m), __ATOMIC_RELAXED,
+                                              __HIP_MEMORY_SCOPE_WORKGROUP);
+            base = __shfl(base, (int)leader);
+            if (out != STR_NONE) mine[base + __popcll(m & ((1ull << lane) - 1))] = (uint32_t)g;
+        }
     }
+    __syncthreads();
+    if (threadIdx.x == 0) lcnt[blockIdx.x] = s_n;
 }
 
 // The row is its slot's representative: it lowered the slot's rep word in
@@ -558,11 +579,19 @@ __global__ __launch_bounds__(256) void k_str_assign(const StrSeg *segs, uint32_t
     }
 }
 
-__global__ __launch_bounds__(256) void k_str_check(const StrSeg *segs, uint32_t nseg, uint64_t total, StrTab t,
-                                                   uint32_t *pos, uint32_t *codes) {
+// The rows k_str_probe listed: nlist workgroup lists.
+__global__ __launch_bounds__(256) void k_str_check(const StrSeg *segs, uint32_t nseg, StrTab t, uint32_t *pos,
+                                                   uint32_t *codes, const uint32_t *list, const uint32_t *lcnt,
+                                                   uint64_t per_block, uint32_t nlist) {
     uint64_t unresolved = 0;
-    for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total;
-         g += (uint64_t)gridDim.x * blockDim.x) {
+    for (uint64_t i = 0, b = blockIdx.x;; i += blockDim.x) {
+        while (b < nlist && i >= lcnt[b]) {
+            b += gridDim.x;
+            i = 0;
+        }
+        if (b >= nlist) break;
+        if (i + threadIdx.x >= lcnt[b]) continue;
+        const uint64_t g = list[b * per_block + i + threadIdx.x];
         const uint32_t p = pos[g];
         if (p == STR_NONE) continue;
         // representatives with a code were resolved by k_str_assign
