@@ -444,27 +444,42 @@ __device__ __forceinline__ bool str_flagged_rep(uint32_t p, uint32_t &s) {
     return (p & (STR_CAND | STR_REP)) == STR_REP;
 }
 
-// Exclusive block-wide prefix sums of two values per thread (256 threads),
-// and their block totals.
+// Exclusive block-wide prefix sums of two values per thread (256 threads:
+// four waves), and their block totals: shuffle scans within each wave, then
+// the waves' totals through LDS.
 __device__ __forceinline__ void str_block_scan(uint64_t a, uint64_t b, uint64_t &ea, uint64_t &eb, uint64_t &ta,
                                                uint64_t &tb) {
-    __shared__ uint64_t sa[256], sb[256];
-    const uint32_t i = threadIdx.x;
-    sa[i] = a;
-    sb[i] = b;
-    __syncthreads();
-    for (uint32_t d = 1; d < 256; d <<= 1) {
-        const uint64_t xa = i >= d ? sa[i - d] : 0, xb = i >= d ? sb[i - d] : 0;
-        __syncthreads();
-        sa[i] += xa;
-        sb[i] += xb;
-        __syncthreads();
+    __shared__ uint64_t sa[4], sb[4];
+    const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint64_t xa = a, xb = b;
+#pragma unroll
+    for (uint32_t d = 1; d < 64; d <<= 1) {
+        const uint64_t ya = __shfl_up(xa, d), yb = __shfl_up(xb, d);
+        if (lane >= d) {
+            xa += ya;
+            xb += yb;
+        }
     }
-    ea = sa[i] - a;
-    eb = sb[i] - b;
-    ta = sa[255];
-    tb = sb[255];
+    if (lane == 63) {
+        sa[w] = xa;
+        sb[w] = xb;
+    }
     __syncthreads();
+    uint64_t pa = 0, pb = 0;
+    ta = 0;
+    tb = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < 4; i++) {
+        if (i < w) {
+            pa += sa[i];
+            pb += sb[i];
+        }
+        ta += sa[i];
+        tb += sb[i];
+    }
+    ea = pa + xa - a;
+    eb = pb + xb - b;
+    __syncthreads();  // sa, sb are written again by the next call
 }
 
 // Codes are numbered in row order of their representatives (the order of
