@@ -2168,7 +2168,8 @@ void intern_run(gbagg_handle *h, StrCol &sc, const std::vector<StrSeg> &segs, ui
         std::vector<uint64_t> lims;
         if (prefix < total) lims.push_back(prefix);
         lims.push_back(total);
-        for (uint64_t lim : lims) {
+        for (size_t li = 0; li < lims.size(); li++) {
+            const uint64_t lim = lims[li];
             const uint32_t rg = (uint32_t)std::min<uint64_t>((lim + 255) / 256, (uint64_t)di.cus * probe_per_cu);
             const uint64_t per_block = 256 * ((lim + (uint64_t)rg * 256 - 1) / ((uint64_t)rg * 256));
             // code assignment: blocks of contiguous rows, at most 4096 of them
@@ -2187,6 +2188,9 @@ void intern_run(gbagg_handle *h, StrCol &sc, const std::vector<StrSeg> &segs, ui
                 hipLaunchKernelGGL(k_str_check, dim3(rg), dim3(256), 0, s, dsegs, nseg, t, sc.pos.as<uint32_t>(),
                                    sc.codes.as<uint32_t>(), (const uint32_t *)sc.list.ptr,
                                    (const uint32_t *)sc.lcnt.ptr, per_block, rg);
+                // the prefix's rows left unresolved probe on in the next stage,
+                // and a full table shows there too: no host round trip here
+                if (li + 1 < lims.size()) break;
                 HIPCHK(hipMemcpyAsync(ctr, t.ctr, SC_N * 8, hipMemcpyDeviceToHost, s));
                 HIPCHK(hipStreamSynchronize(s));
                 if ((ctr[SC_FLAGS] & (STR_FULL_TABLE | STR_FULL_DICT)) || !ctr[SC_UNRESOLVED]) break;
@@ -2194,8 +2198,8 @@ void intern_run(gbagg_handle *h, StrCol &sc, const std::vector<StrSeg> &segs, ui
                 if (round > (1u << 20)) fail(GBAGG_ERROR, "string interning did not converge");
                 HIPCHK(hipMemsetAsync(t.ctr + SC_UNRESOLVED, 0, 8, s));
             }
-            if (ctr[SC_FLAGS] & (STR_FULL_TABLE | STR_FULL_DICT)) break;
         }
+        // ctr: as the last stage's last round left it
         if (!(ctr[SC_FLAGS] & (STR_FULL_TABLE | STR_FULL_DICT))) break;
         if (attempt == 16) fail(GBAGG_ERROR, "string interning: tables still full after 16 resizes");
         if (ctr[SC_FLAGS] & STR_FULL_TABLE) slots *= 2;
