@@ -382,9 +382,14 @@ __global__ __launch_bounds__(256) void k_str_probe(const StrSeg *segs, uint32_t 
                 break;
             }
             StrSlot *sl = &t.slot[s];
-            uint64_t k = __hip_atomic_load(&sl->key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            // a key only ever changes from 0 to a row's key, so a cached read
+            // is safe: a 0 is read again from the L2 before the
+            // compare-and-swap (which returns the key in place if another row
+            // was first). The hot strings' slots then stay in the L1.
+            uint64_t k = gld(&sl->key);
             // code and arena word: set by an earlier kernel, or no code yet
             const uint64_t vr = gld((const uint64_t *)&sl->val);
+            if (k == 0) k = __hip_atomic_load(&sl->key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if (k == 0) {
                 uint64_t expect = 0;
                 __hip_atomic_compare_exchange_strong(&sl->key, &expect, h, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
