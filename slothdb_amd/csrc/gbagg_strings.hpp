@@ -561,38 +561,59 @@ __global__ __launch_bounds__(256) void k_str_assign(const StrSeg *segs, uint32_t
                                                     const uint64_t *blk) {
     const uint64_t r0 = (uint64_t)blockIdx.x * chunk, r1 = min(total, r0 + chunk);
     uint64_t bc = blk[2 * blockIdx.x], bw = blk[2 * blockIdx.x + 1];
-    for (uint64_t tile = r0; tile < r1; tile += blockDim.x) {
-        const uint64_t g = tile + threadIdx.x;
-        uint32_t s = 0;
-        const bool rep = g < r1 && str_flagged_rep(pos[g], s);
-        StrRow row{};
-        if (rep) row = str_row(segs, nseg, g);
-        const uint64_t words = rep ? (row.len + 3) / 4 : 0;
+    // each thread takes STR_AR consecutive rows of a tile: their loads overlap,
+    // and one block scan serves STR_AR rows per thread
+    constexpr int STR_AR = 4;
+    for (uint64_t tile = r0; tile < r1; tile += (uint64_t)blockDim.x * STR_AR) {
+        const uint64_t g0 = tile + (uint64_t)threadIdx.x * STR_AR;
+        uint32_t sl[STR_AR];
+        bool isrep[STR_AR];
+#pragma unroll
+        for (int j = 0; j < STR_AR; j++) isrep[j] = g0 + j < r1 && str_flagged_rep(pos[g0 + j], sl[j]);
+        StrRow row[STR_AR];
+        uint32_t wd[STR_AR];
+        uint64_t cnt = 0, words = 0;
+#pragma unroll
+        for (int j = 0; j < STR_AR; j++) {
+            wd[j] = 0;
+            if (isrep[j]) {
+                row[j] = str_row(segs, nseg, g0 + j);
+                wd[j] = (row[j].len + 3) / 4;
+                cnt++;
+                words += wd[j];
+            }
+        }
         uint64_t ec, ew, tc, tw;
-        str_block_scan(rep ? 1 : 0, words, ec, ew, tc, tw);
-        if (rep) {
-            const uint64_t c = bc + ec, w0 = bw + ew;
-            if (c >= t.code_cap || w0 + words > t.arena_words) {
+        str_block_scan(cnt, words, ec, ew, tc, tw);
+        uint64_t c = bc + ec, w0 = bw + ew;
+#pragma unroll
+        for (int j = 0; j < STR_AR; j++) {
+            if (!isrep[j]) continue;
+            const uint64_t g = g0 + j;
+            const StrRow &rw = row[j];
+            if (c >= t.code_cap || w0 + wd[j] > t.arena_words) {
                 __hip_atomic_fetch_or(&t.ctr[SC_FLAGS], STR_FULL_DICT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             } else {
-                if (row.len && row.len <= STR_SHORT) {
+                if (rw.len && rw.len <= STR_SHORT) {
                     uint64_t w[4];
-                    str_words4(row.p, row.len, w);
+                    str_words4(rw.p, rw.len, w);
 #pragma unroll
                     for (uint32_t i = 0; i < 8; i++)
-                        if (4 * i < row.len) gst(&t.arena[w0 + i], (uint32_t)(w[i / 2] >> (32 * (i & 1))));
+                        if (4 * i < rw.len) gst(&t.arena[w0 + i], (uint32_t)(w[i / 2] >> (32 * (i & 1))));
                 } else {
-                    for (uint32_t i = 0; i < row.len; i += 4) {
-                        const uint32_t left = row.len - i;
-                        gst(&t.arena[w0 + i / 4], str_word(row.p + i, left < 4 ? left : 4));
+                    for (uint32_t i = 0; i < rw.len; i += 4) {
+                        const uint32_t left = rw.len - i;
+                        gst(&t.arena[w0 + i / 4], str_word(rw.p + i, left < 4 ? left : 4));
                     }
                 }
-                t.code[c] = StrCode{(uint32_t)w0, row.len};
+                t.code[c] = StrCode{(uint32_t)w0, rw.len};
                 // the code and its arena word; the representative is resolved
-                gst((uint64_t *)&t.slot[s].val, (uint64_t)c | (uint64_t)w0 << 32);
+                gst((uint64_t *)&t.slot[sl[j]].val, c | w0 << 32);
                 codes[g] = (uint32_t)c;
                 pos[g] = STR_NONE;
             }
+            c++;
+            w0 += wd[j];
         }
         bc += tc;
         bw += tw;
