@@ -434,15 +434,6 @@ __global__ __launch_bounds__(256) void k_str_probe(const StrSeg *segs, uint32_t 
     if (threadIdx.x == 0) lcnt[blockIdx.x] = s_n;
 }
 
-// The row is its slot's representative: it lowered the slot's rep word in
-// k_str_probe and still holds it.
-__device__ __forceinline__ bool str_is_rep(const StrTab &t, const uint32_t *pos, uint64_t g, uint32_t &s) {
-    const uint32_t p = pos[g];
-    if (p == STR_NONE || !(p & STR_CAND)) return false;
-    s = p & ~STR_CAND;
-    return t.slot[s].rep == (uint32_t)g;
-}
-
 // k_str_reps flagged the row as its slot's representative.
 __device__ __forceinline__ bool str_flagged_rep(uint32_t p, uint32_t &s) {
     s = p & ~STR_REP;
@@ -499,14 +490,31 @@ __global__ __launch_bounds__(256) void k_str_reps(const StrSeg *segs, uint32_t n
                                                   uint32_t *pos, uint64_t chunk, uint64_t *blk) {
     const uint64_t r0 = (uint64_t)blockIdx.x * chunk, r1 = min(total, r0 + chunk);
     uint64_t cnt = 0, words = 0;
-    for (uint64_t g = r0 + threadIdx.x; g < r1; g += blockDim.x) {
-        uint32_t s;
-        if (!str_is_rep(t, pos, g, s)) continue;
-        // k_str_assign then knows its representatives without the slots'
-        // rep words, which it overwrites
-        pos[g] = s | STR_REP;
-        cnt++;
-        words += (str_row(segs, nseg, g).len + 3) / 4;
+    // four rows per thread and step, their loads overlapping
+    for (uint64_t g0 = r0 + threadIdx.x; g0 < r1; g0 += (uint64_t)blockDim.x * 4) {
+        uint32_t p[4], rp[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const uint64_t g = g0 + (uint64_t)j * blockDim.x;
+            p[j] = g < r1 ? pos[g] : STR_NONE;
+        }
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            // candidates: rows that lowered their slot's rep word in k_str_probe
+            const bool cand = p[j] != STR_NONE && p[j] != STR_HOME && (p[j] & STR_CAND);
+            rp[j] = cand ? t.slot[p[j] & ~STR_CAND].rep : STR_NONE;
+        }
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const uint64_t g = g0 + (uint64_t)j * blockDim.x;
+            if (rp[j] != (uint32_t)g) continue;
+            // the row still holds its slot's rep word: the representative.
+            // k_str_assign then knows it without the slots' rep words, which
+            // it overwrites
+            pos[g] = (p[j] & ~STR_CAND) | STR_REP;
+            cnt++;
+            words += (str_row(segs, nseg, g).len + 3) / 4;
+        }
     }
     uint64_t ea, eb, ta, tb;
     str_block_scan(cnt, words, ea, eb, ta, tb);
