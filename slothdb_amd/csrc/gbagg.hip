@@ -2097,7 +2097,9 @@ void intern_run(gbagg_handle *h, StrCol &sc, const std::vector<StrSeg> &segs, ui
     t.seed = 0x5157C0DE5EEDull;
     const StrSeg *dsegs = sc.segs.as<StrSeg>();
     const uint32_t nseg = (uint32_t)segs.size();
-    const uint32_t W = (uint32_t)std::min<uint64_t>((total + 1023) / 1024, (uint64_t)di.cus * 4);
+    // 8 workgroups per CU (their sketches take 128 KB of a CU's LDS): 4 per
+    // CU left k_str_hash's loads too few waves to overlap
+    const uint32_t W = (uint32_t)std::min<uint64_t>((total + 255) / 256, (uint64_t)di.cus * 8);
     sc.hllp.ensure((size_t)W * HLL_M);
     const uint32_t hslices = std::min<uint32_t>(32, W);
     sc.hllm.ensure((size_t)HLL_M * hslices);
